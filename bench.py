@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark: Jacobi-PCG solves of the 2236 x 2236 LSCM system on one MI355X.
+
+Workload (the large single-system configuration): LSCM normal equations of a
+2236 x 2236 height-field grid (n = 9,999,388 unknowns, nnz = 139,919,888,
+FP64 CRS), right-hand side from two pinned corners, x0 = 0, stopping rule
+|Ax-b| <= 1e-10 |b| or 5000 iterations (the iteration cap of the 317 x 317
+configuration of the same construction).  One step = one complete solve with
+the inputs resident in HBM.  Synthetic data from a fixed seed.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n: int, nnz: int, sv: int = 8, sx: int = 8) -> float:
+    """Algorithmic bytes of y = A x: values + colind + rowptr + x once + y."""
+    return nnz * (sv + 4) + (n + 1) * 4 + n * sx + n * sx
+
+
+def cg_spmv_step_bytes(n: int, nnz: int, s: int = 8) -> float:
+    """Algorithmic bytes of the fused spmv step of one PCG iteration: values,
+    colind, rowptr; h and d_prev read once; d and q written."""
+    return nnz * (s + 4) + (n + 1) * 4 + 2 * n * s + 2 * n * s
+
+
+def cg_update_step_bytes(n: int, s: int = 8) -> float:
+    """x, d, r, q, dinv read; x, r, h written."""
+    return 8 * n * s
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--k", type=int, default=2236, help="grid side")
+    ap.add_argument("--max-iter", type=int, default=5000)
+    ap.add_argument("--eps", type=float, default=1e-10)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-iters", type=int, default=40,
+                    help="oracle PCG iterations timed for the CPU baseline (0: skip)")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the "
+              "solver has no multi-GPU path in this build", file=sys.stderr)
+        sys.exit(2)
+
+    from graphitethree_amd import capi
+    from graphitethree_amd import workloads as W
+
+    L = capi.lib()
+    if not L.nlInitExtension(b"HIP"):
+        raise SystemExit("nlInitExtension(\"HIP\") failed: " + capi.last_error())
+    dev_name, cu, hbm = capi.device_info()
+    log(f"device: {dev_name}, {cu} CUs, {hbm / 2**30:.0f} GiB")
+
+    t0 = time.time()
+    Sw = W.lscm_grid(args.k, seed=args.seed)
+    n, nnz = Sw.n, Sw.nnz
+    log(f"LSCM grid {args.k}^2: n={n} nnz={nnz} built in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    S = capi.System.from_crs(Sw, fp32=not args.no_mixed)
+    lpr, wg = S.launch()
+    log(f"uploaded in {time.time() - t0:.1f}s; launch lanes/row={lpr} workgroups={wg}")
+
+    # ---- standalone SpMV bandwidth (FP64, and FP32 values)
+    t_spmv = S.bench_spmv(fp32=False, reps=50)
+    spmv_gbs = spmv_bytes(n, nnz) / t_spmv / 1e9
+    log(f"SpMV fp64: {t_spmv * 1e3:.3f} ms  {spmv_gbs:.0f} GB/s")
+    t_spmv32 = spmv32_gbs = None
+    if not args.no_mixed:
+        t_spmv32 = S.bench_spmv(fp32=True, reps=50)
+        spmv32_gbs = spmv_bytes(n, nnz, 4, 4) / t_spmv32 / 1e9
+        log(f"SpMV fp32: {t_spmv32 * 1e3:.3f} ms  {spmv32_gbs:.0f} GB/s")
+
+    # ---- warmup + timed solves (FP64 Jacobi-PCG, the plain GPU CG)
+    b = Sw.b
+    params = dict(eps=args.eps, max_iter=args.max_iter)
+    for w in range(args.warmup):
+        x, st = S.solve(b, **params)
+        log(f"warmup {w}: {st.iterations} it, rel res {st.rel_residual:.3e}, "
+            f"{st.device_seconds:.3f}s")
+    lat = []
+    stats = None
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        x, st = S.solve(b, **params)
+        lat.append(st.device_seconds)
+        stats = st
+        log(f"step {k}: {st.iterations} it, rel res {st.rel_residual:.3e}, "
+            f"dev {st.device_seconds:.3f}s wall {st.wall_seconds:.3f}s")
+    t_total = time.perf_counter() - t_start
+    ms_per_step = 1e3 * float(np.mean(lat))
+    solves_per_s = 1e3 / ms_per_step
+    iters = int(stats.iterations)
+
+    # ---- dominant kernel timing (HIP events on the system's stream)
+    t_a, t_b = capi.profile_cg(S, iters=50)
+    bytes_a = cg_spmv_step_bytes(n, nnz)
+    bytes_b = cg_update_step_bytes(n)
+    achieved = bytes_a / t_a / 1e9
+    log(f"cg spmv step {t_a * 1e3:.3f} ms ({achieved:.0f} GB/s), update step "
+        f"{t_b * 1e3:.3f} ms ({bytes_b / t_b / 1e9:.0f} GB/s)")
+    iter_bytes = bytes_a + bytes_b
+    iter_gbs = iters * iter_bytes / (ms_per_step * 1e-3) / 1e9 if iters else 0.0
+
+    result_extra = {}
+    # ---- mixed precision (FP32 inner PCG inside FP64 refinement), same budget
+    if not args.no_mixed:
+        xm, sm = S.solve(b, precision="mixed", eps=args.eps, max_iter=args.max_iter,
+                         inner_eps=1e-4)
+        result_extra["mixed"] = {
+            "solves_per_s": 1.0 / sm.device_seconds, "iterations": int(sm.iterations),
+            "outer": int(sm.outer_iterations), "rel_residual": sm.rel_residual,
+            "rel_diff_vs_fp64": float(np.linalg.norm(xm - x) / np.linalg.norm(x))}
+        log(f"mixed: {sm.iterations} inner it / {sm.outer_iterations} outer, rel res "
+            f"{sm.rel_residual:.3e}, {sm.device_seconds:.3f}s")
+        del xm
+    del x
+
+    # ---- CPU baseline: the oracle's OpenNL-order PCG on the same matrix
+    cpu = None
+    if args.cpu_iters > 0:
+        from oracle import oracle as O
+        O.build() if not os.path.exists(O._LIB_PATH) else None
+        A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        secs = O.pcg_timed_iters(A, b, args.cpu_iters, variant=1)
+        per_iter = secs / args.cpu_iters
+        cpu_solve_s = per_iter * max(iters, 1)
+        cpu = {"value": 1.0 / cpu_solve_s, "unit": "solves/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{args.cpu_iters} Jacobi-PCG iterations of the oracle (OpenMP "
+                         f"SpMV, serial BLAS-1, OpenNL operation order) on this system, "
+                         f"{secs:.1f}s, scaled to the {iters}-iteration solve"}
+        log(f"cpu baseline: {per_iter * 1e3:.1f} ms/iter on {threads} threads -> "
+            f"{cpu['value']:.4f} solves/s")
+        del A
+
+    # ---- C1 (317 x 317, cache-resident) to full convergence, no cap
+    if not args.no_c1:
+        S1w = W.lscm_grid(317, seed=args.seed)
+        S1 = capi.System.from_crs(S1w)
+        x1, s1 = S1.solve(S1w.b, eps=1e-10, max_iter=100000)
+        l1 = []
+        for _ in range(3):
+            x1, s1 = S1.solve(S1w.b, eps=1e-10, max_iter=100000)
+            l1.append(s1.device_seconds)
+        x1c, s1c = S1.solve(S1w.b, eps=1e-10, max_iter=5000)
+        result_extra["lscm317"] = {
+            "n": S1w.n, "nnz": S1w.nnz, "iterations_to_1e-10": int(s1.iterations),
+            "rel_residual": s1.rel_residual, "solves_per_s_to_1e-10": 1.0 / float(np.median(l1)),
+            "capped_5000": {"solves_per_s": 1.0 / s1c.device_seconds,
+                            "rel_residual": s1c.rel_residual}}
+        log(f"317^2: {s1.iterations} it to {s1.rel_residual:.2e} in "
+            f"{np.median(l1):.3f}s; capped 5000: {s1c.device_seconds:.3f}s")
+        S1.close()
+
+    out = {
+        "metric": "CG solves/s + SpMV HBM GB/s at 1e-10 rel. residual; p50/p99 solve latency",
+        "value": solves_per_s,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded LSCM height-field grid)",
+        "config": {"workload": f"lscm_grid_{args.k}", "n": n, "nnz": nnz,
+                   "solver": "Jacobi-PCG fp64", "eps": args.eps, "max_iter": args.max_iter,
+                   "iterations": iters, "converged": bool(stats.converged),
+                   "rel_residual": stats.rel_residual, "lanes_per_row": lpr,
+                   "workgroups": wg},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "spmv_gbs": {"fp64": spmv_gbs, "fp32": spmv32_gbs,
+                     "bytes_fp64": spmv_bytes(n, nnz)},
+        "iteration_gbs": iter_gbs,
+        "roofline": {"bound": "hbm", "kernel": "k_cg_spmv", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": bytes_a,
+                     "launch_ms": t_a * 1e3},
+        "cpu_baseline": cpu,
+        "device": dev_name,
+        **result_extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

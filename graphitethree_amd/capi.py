@@ -1,0 +1,294 @@
+"""ctypes bindings of libnlhip.so (include/nl.h and include/nl_hip.h).
+
+Signatures are declared here once; ``lib()`` loads the in-tree library and
+raises when it is missing -- there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+from . import _native
+
+ROOT = Path(__file__).resolve().parent.parent
+INCLUDE = ROOT / "include"
+
+NL_FALSE, NL_TRUE = 0, 1
+NL_SOLVER, NL_NB_VARIABLES, NL_LEAST_SQUARES = 0x100, 0x101, 0x102
+NL_MAX_ITERATIONS, NL_THRESHOLD, NL_OMEGA, NL_SYMMETRIC = 0x103, 0x104, 0x105, 0x106
+NL_USED_ITERATIONS, NL_ERROR, NL_INNER_ITERATIONS = 0x107, 0x108, 0x109
+NL_ELAPSED_TIME, NL_PRECONDITIONER, NL_GFLOPS, NL_NNZ, NL_NB_SYSTEMS = (
+    0x10a, 0x10b, 0x10c, 0x10d, 0x10e)
+NL_SOLVER_DEFAULT, NL_CG, NL_BICGSTAB, NL_GMRES = 0x000, 0x200, 0x201, 0x202
+NL_SUPERLU_EXT, NL_PERM_SUPERLU_EXT, NL_SYMMETRIC_SUPERLU_EXT, NL_CHOLMOD_EXT = (
+    0x210, 0x211, 0x212, 0x213)
+NL_PRECOND_NONE, NL_PRECOND_JACOBI, NL_PRECOND_SSOR = 0x000, 0x300, 0x301
+NL_NORMALIZE_ROWS, NL_VERBOSE = 0x400, 0x401
+NL_SYSTEM, NL_MATRIX, NL_ROW = 0x0, 0x1, 0x2
+NL_MATRIX_SPARSE_DYNAMIC, NL_MATRIX_CRS, NL_MATRIX_OTHER = 0x1001, 0x1002, 0x1006
+NL_MATRIX_STORE_ROWS = 1
+
+NLHIP_OK = 0
+NLHIP_ERR_INVALID, NLHIP_ERR_NO_DEVICE, NLHIP_ERR_HIP, NLHIP_ERR_OOM = -1, -2, -3, -4
+NLHIP_ERR_NONFINITE, NLHIP_ERR_UNSUPPORTED, NLHIP_ERR_BUDGET, NLHIP_ERR_INJECTED = (
+    -5, -6, -7, -8)
+NLHIP_PREC_FP64, NLHIP_PREC_MIXED = 0, 1
+NLHIP_SYSTEM_FP32 = 1
+
+
+class NLMatrixStruct(C.Structure):
+    _fields_ = [("m", C.c_uint), ("n", C.c_uint), ("type", C.c_uint),
+                ("destroy_func", C.c_void_p), ("mult_func", C.c_void_p)]
+
+
+NLMatrix = C.POINTER(NLMatrixStruct)
+
+_DCOPY = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int)
+_DSCAL = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_int)
+_DDOT = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                    C.c_int)
+_DNRM2 = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_int)
+_DAXPY = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_int,
+                     C.c_void_p, C.c_int)
+_DMUL = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p)
+_DZERO = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_void_p)
+
+
+class NLBlasStruct(C.Structure):
+    _fields_ = [("Malloc", C.c_void_p), ("Free", C.c_void_p), ("Memcpy", C.c_void_p),
+                ("Dcopy", _DCOPY), ("Dscal", _DSCAL), ("Ddot", _DDOT), ("Dnrm2", _DNRM2),
+                ("Daxpy", _DAXPY), ("Dmul", _DMUL), ("Dzero", _DZERO),
+                ("has_unique_memory", C.c_ubyte), ("start_time", C.c_ulong),
+                ("flops", C.c_ulong), ("used_ram", C.c_ulong * 2),
+                ("max_used_ram", C.c_ulong * 2)]
+
+
+NLBlas_t = C.POINTER(NLBlasStruct)
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("precond", C.c_int32), ("eps", C.c_double),
+                ("max_iter", C.c_uint32), ("check_every", C.c_uint32),
+                ("inner_eps", C.c_double), ("max_outer", C.c_uint32),
+                ("inner_max_iter", C.c_uint32)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("outer_iterations", C.c_uint32),
+                ("converged", C.c_int32), ("status", C.c_int32),
+                ("rel_residual", C.c_double), ("cg_rel_residual", C.c_double),
+                ("device_seconds", C.c_double), ("wall_seconds", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_u32p = C.POINTER(C.c_uint32)
+_f64p = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+SIGNATURES = {
+    # nl.h
+    "nlSparseMatrixNew": (NLMatrix, [C.c_uint, C.c_uint, C.c_uint]),
+    "nlSparseMatrixAdd": (None, [NLMatrix, C.c_uint, C.c_uint, C.c_double]),
+    "nlSparseMatrixZero": (None, [NLMatrix]),
+    "nlSparseMatrixNNZ": (C.c_uint, [NLMatrix]),
+    "nlCRSMatrixNewFromArrays": (NLMatrix, [C.c_uint, C.c_uint, _vp, _vp, _vp]),
+    "nlMatrixCompress": (None, [C.POINTER(NLMatrix)]),
+    "nlMultMatrixVector": (None, [NLMatrix, _vp, _vp]),
+    "nlDeleteMatrix": (None, [NLMatrix]),
+    "nlMatrixFactorize": (NLMatrix, [NLMatrix, C.c_uint]),
+    "nlNewJacobiPreconditioner": (NLMatrix, [NLMatrix]),
+    "nlNewSSORPreconditioner": (NLMatrix, [NLMatrix, C.c_double]),
+    "nlHostBlas": (NLBlas_t, []),
+    "nlBlasResetStats": (None, [NLBlas_t]),
+    "nlBlasGFlops": (C.c_double, [NLBlas_t]),
+    "nlSolveSystemIterative": (C.c_uint, [NLBlas_t, NLMatrix, NLMatrix, _vp, _vp, C.c_uint,
+                                          C.c_double, C.c_uint, C.c_uint]),
+    "nlInitExtension": (C.c_ubyte, [C.c_char_p]),
+    "nlExtensionIsInitialized": (C.c_ubyte, [C.c_char_p]),
+    "nlNewContext": (_vp, []),
+    "nlDeleteContext": (None, [_vp]),
+    "nlMakeCurrent": (None, [_vp]),
+    "nlGetCurrent": (_vp, []),
+    "nlSolverParameterd": (None, [C.c_uint, C.c_double]),
+    "nlSolverParameteri": (None, [C.c_uint, C.c_int]),
+    "nlGetBooleanv": (None, [C.c_uint, C.POINTER(C.c_ubyte)]),
+    "nlGetDoublev": (None, [C.c_uint, _f64p]),
+    "nlGetIntegerv": (None, [C.c_uint, C.POINTER(C.c_int)]),
+    "nlEnable": (None, [C.c_uint]),
+    "nlDisable": (None, [C.c_uint]),
+    "nlIsEnabled": (C.c_ubyte, [C.c_uint]),
+    "nlSetVariable": (None, [C.c_uint, C.c_double]),
+    "nlMultiSetVariable": (None, [C.c_uint, C.c_uint, C.c_double]),
+    "nlGetVariable": (C.c_double, [C.c_uint]),
+    "nlMultiGetVariable": (C.c_double, [C.c_uint, C.c_uint]),
+    "nlLockVariable": (None, [C.c_uint]),
+    "nlUnlockVariable": (None, [C.c_uint]),
+    "nlVariableIsLocked": (C.c_ubyte, [C.c_uint]),
+    "nlBegin": (None, [C.c_uint]),
+    "nlEnd": (None, [C.c_uint]),
+    "nlCoefficient": (None, [C.c_uint, C.c_double]),
+    "nlAddIJCoefficient": (None, [C.c_uint, C.c_uint, C.c_double]),
+    "nlRightHandSide": (None, [C.c_double]),
+    "nlMultiRightHandSide": (None, [C.c_uint, C.c_double]),
+    "nlRowScaling": (None, [C.c_double]),
+    "nlSolve": (C.c_ubyte, []),
+    # nl_hip.h
+    "nlInitExtension_HIP": (C.c_ubyte, []),
+    "nlExtensionIsInitialized_HIP": (C.c_ubyte, []),
+    "nlHIPBlas": (NLBlas_t, []),
+    "nlHIPMatrixNewFromCRSMatrix": (NLMatrix, [NLMatrix]),
+    "nlHIPJacobiPreconditionerNewFromCRSMatrix": (NLMatrix, [NLMatrix]),
+    "nlhipSolveParamsDefault": (None, [C.POINTER(SolveParams)]),
+    "nlhipSystemCreate": (C.c_int, [C.c_uint32, _vp, _vp, _vp, C.c_uint32,
+                                    C.POINTER(_vp)]),
+    "nlhipSystemDestroy": (C.c_int, [_vp]),
+    "nlhipSystemInfo": (C.c_int, [_vp, _u32p, C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_uint64)]),
+    "nlhipSystemSetLaunch": (C.c_int, [_vp, C.c_int, C.c_uint32]),
+    "nlhipSystemGetLaunch": (C.c_int, [_vp, C.POINTER(C.c_int), _u32p]),
+    "nlhipSolve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
+                             C.POINTER(SolveStats)]),
+    "nlhipSpMV": (C.c_int, [_vp, _vp, _vp]),
+    "nlhipBenchSpMV": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
+    "nlhipProfileCG": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p, _f64p]),
+    "nlhipLastError": (C.c_char_p, []),
+    "nlhipInjectFailures": (None, [C.c_int]),
+    "nlhipDeviceInfo": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int),
+                                  C.POINTER(C.c_uint64)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        L = _native.load("libnlhip.so")
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols() -> list[str]:
+    """Every function declared in include/*.h."""
+    names = []
+    for h in sorted(INCLUDE.glob("*.h")):
+        names += re.findall(r"NLAPIENTRY\s+([A-Za-z_0-9]+)\s*\(", h.read_text())
+    return names
+
+
+def last_error() -> str:
+    return (lib().nlhipLastError() or b"").decode()
+
+
+class NLHIPError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        self.code = code
+        super().__init__(f"{where} failed ({code}): {last_error()}")
+
+
+def check(rc: int, where: str) -> None:
+    if rc != NLHIP_OK:
+        raise NLHIPError(rc, where)
+
+
+def ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def default_params(**kw) -> SolveParams:
+    p = SolveParams()
+    lib().nlhipSolveParamsDefault(C.byref(p))
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = {"fp64": NLHIP_PREC_FP64, "mixed": NLHIP_PREC_MIXED}[v]
+        if k == "precond" and isinstance(v, str):
+            v = {"jacobi": NL_PRECOND_JACOBI, "none": NL_PRECOND_NONE}[v.lower()]
+        setattr(p, k, v)
+    return p
+
+
+class System:
+    """A CRS system resident on the GPU (nlhipSystemCreate)."""
+
+    def __init__(self, rowptr, colind, val, fp32: bool = False):
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.uint32)
+        self.colind = np.ascontiguousarray(colind, dtype=np.uint32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.n = len(self.rowptr) - 1
+        h = C.c_void_p()
+        check(lib().nlhipSystemCreate(self.n, ptr(self.rowptr), ptr(self.colind),
+                                      ptr(self.val), NLHIP_SYSTEM_FP32 if fp32 else 0,
+                                      C.byref(h)), "nlhipSystemCreate")
+        self._h = h
+
+    @classmethod
+    def from_crs(cls, S, fp32: bool = False) -> "System":
+        return cls(S.rowptr, S.colind, S.val, fp32=fp32)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().nlhipSystemDestroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def launch(self):
+        lpr, wg = C.c_int(), C.c_uint32()
+        check(lib().nlhipSystemGetLaunch(self._h, C.byref(lpr), C.byref(wg)), "get launch")
+        return int(lpr.value), int(wg.value)
+
+    def set_launch(self, lanes_per_row: int, workgroups: int):
+        check(lib().nlhipSystemSetLaunch(self._h, lanes_per_row, workgroups), "set launch")
+
+    def info(self):
+        n, nnz, b = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        check(lib().nlhipSystemInfo(self._h, C.byref(n), C.byref(nnz), C.byref(b)), "info")
+        return int(n.value), int(nnz.value), int(b.value)
+
+    def solve(self, b, x0=None, **params):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        p = default_params(**params)
+        st = SolveStats()
+        check(lib().nlhipSolve(self._h, ptr(b), ptr(x), C.byref(p), C.byref(st)), "nlhipSolve")
+        return x, st
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n)
+        check(lib().nlhipSpMV(self._h, ptr(x), ptr(y)), "nlhipSpMV")
+        return y
+
+    def bench_spmv(self, fp32: bool = False, reps: int = 20) -> float:
+        s = C.c_double()
+        check(lib().nlhipBenchSpMV(self._h, 1 if fp32 else 0, reps, C.byref(s)),
+              "nlhipBenchSpMV")
+        return float(s.value)
+
+
+def profile_cg(system: "System", iters: int = 50, fp32: bool = False):
+    """Mean (spmv step, update step) kernel seconds over `iters` iterations."""
+    a, b = C.c_double(), C.c_double()
+    check(lib().nlhipProfileCG(system.handle, 1 if fp32 else 0, iters, C.byref(a),
+                               C.byref(b)), "nlhipProfileCG")
+    return float(a.value), float(b.value)
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    cu, hbm = C.c_int(), C.c_uint64()
+    check(lib().nlhipDeviceInfo(name, 256, C.byref(cu), C.byref(hbm)), "nlhipDeviceInfo")
+    return name.value.decode(), int(cu.value), int(hbm.value)

@@ -1,0 +1,116 @@
+// NLBlas level-1 kernels (scene_graph/NL/blas.cpp:86,96,115,125,144 call
+// Dcopy, Dscal, Ddot, Dnrm2, Daxpy with stride 1).
+//
+// Element-wise updates use separately rounded multiply and add, so y = y + a x
+// rounds like the scalar host loop.  Dot products are two-stage: each block
+// sums one contiguous chunk (threads strided, then a fixed wave butterfly and
+// wave order), and a single block sums the per-block partials in index order;
+// the result is bitwise reproducible for a given grid size.
+#include "dev_common.h"
+#include "nlhip_kernels.h"
+
+namespace nlhip {
+
+__global__ __launch_bounds__(kBlock) void k_dscal(uint64_t n, double a, double* __restrict__ x) {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
+        x[i] = mul_rn(a, x[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_daxpy(uint64_t n, double a,
+                                                  const double* __restrict__ x,
+                                                  double* __restrict__ y) {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
+        y[i] = add_rn(y[i], mul_rn(a, x[i]));
+}
+
+__global__ __launch_bounds__(kBlock) void k_dmul(uint64_t n, const double* __restrict__ a,
+                                                 const double* __restrict__ x,
+                                                 double* __restrict__ y) {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
+        y[i] = mul_rn(a[i], x[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(uint64_t n, T v, T* __restrict__ x) {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) x[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_partial(uint64_t n, uint64_t chunk,
+                                                        const double* __restrict__ x,
+                                                        const double* __restrict__ y,
+                                                        double* __restrict__ part) {
+    __shared__ double lds[kWavesPerBlock];
+    const uint64_t b = uint64_t(blockIdx.x) * chunk;
+    const uint64_t e = (b + chunk < n) ? b + chunk : n;
+    double s = 0.0;
+    for (uint64_t i = b + threadIdx.x; i < e; i += kBlock) s += x[i] * y[i];
+    s = block_sum_all(s, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[0] = sum(part[0..P)); out[1] = sqrt(out[0]) (for Dnrm2).
+__global__ __launch_bounds__(kBlock) void k_reduce_final(const double* __restrict__ part,
+                                                         uint32_t P, double* __restrict__ out) {
+    __shared__ double lds[kWavesPerBlock];
+    const double s = reduce_partials(part, P, lds);
+    if (threadIdx.x == 0) {
+        out[0] = s;
+        out[1] = sqrt(s);
+    }
+}
+
+static uint32_t elementwise_grid(uint64_t n) {
+    uint64_t g = (n + kBlock - 1) / kBlock;
+    if (g > 4096) g = 4096;
+    if (g == 0) g = 1;
+    return uint32_t(g);
+}
+
+hipError_t blas_dscal(uint64_t n, double a, double* x, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_dscal<<<elementwise_grid(n), kBlock, 0, s>>>(n, a, x);
+    return hipGetLastError();
+}
+
+hipError_t blas_daxpy(uint64_t n, double a, const double* x, double* y, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_daxpy<<<elementwise_grid(n), kBlock, 0, s>>>(n, a, x, y);
+    return hipGetLastError();
+}
+
+hipError_t blas_dmul(uint64_t n, const double* a, const double* x, double* y, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_dmul<<<elementwise_grid(n), kBlock, 0, s>>>(n, a, x, y);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t fill(uint64_t n, T v, T* x, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_fill<T><<<elementwise_grid(n), kBlock, 0, s>>>(n, v, x);
+    return hipGetLastError();
+}
+template hipError_t fill<double>(uint64_t, double, double*, hipStream_t);
+template hipError_t fill<float>(uint64_t, float, float*, hipStream_t);
+template hipError_t fill<int32_t>(uint64_t, int32_t, int32_t*, hipStream_t);
+
+hipError_t blas_ddot(uint64_t n, const double* x, const double* y, double* part,
+                     uint32_t grid, double* out, hipStream_t s) {
+    if (grid == 0) return hipErrorInvalidValue;
+    uint64_t chunk = (n + grid - 1) / grid;
+    if (chunk == 0) chunk = 1;
+    k_dot_partial<<<grid, kBlock, 0, s>>>(n, chunk, x, y, part);
+    k_reduce_final<<<1, kBlock, 0, s>>>(part, grid, out);
+    return hipGetLastError();
+}
+
+hipError_t reduce_final(const double* part, uint32_t P, double* out, hipStream_t s) {
+    k_reduce_final<<<1, kBlock, 0, s>>>(part, P, out);
+    return hipGetLastError();
+}
+
+}  // namespace nlhip

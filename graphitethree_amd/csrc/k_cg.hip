@@ -1,0 +1,260 @@
+// Fused Jacobi-preconditioned CG for gfx950 (FP64, and FP32 for the inner
+// solve of mixed-precision refinement).
+//
+// Follows the CG of nlSolveSystemIterative(..., NL_CG, ...)
+// (src/lib/OGF/scene_graph/NL/matrix.cpp:390-393) with the residual
+// convention r = A x - b and the operation order
+//   r = A x - b; h = P r; d = h; rho = r.h; err = r.r
+//   while err > eps^2 b.b and it < max_iter:
+//     Ad = A d; alpha = rho / d.Ad; x -= alpha d; r -= alpha Ad; h = P r
+//     rho' = r.h; beta = rho'/rho; rho = rho'; d = beta d + h; err = r.r
+// and the stopping rule |Ax-b|/|b| < eps of matrix.h:177-179.
+//
+// One iteration is two launches and no host round trip:
+//   spmv step   reduce r.h and r.r partials -> rho, err; stopping test;
+//               beta = rho/rho_prev; q = A d with d = beta d_prev + h formed
+//               on the fly in the gather (d is double-buffered); write d, q;
+//               partials of d.q.
+//   update step reduce d.q partials -> alpha; x -= alpha d; r -= alpha q;
+//               h = P r; partials of r.h and r.r.
+// Every block reduces the partials of the previous launch itself, in a fixed
+// order, so all blocks hold bitwise-identical scalars.  After the stopping
+// test holds, later launches exit at once, so a host can enqueue (or replay
+// a captured graph of) many iterations and read the status afterwards.
+#include "dev_common.h"
+#include "nlhip_kernels.h"
+
+namespace nlhip {
+
+template <typename T>
+__device__ __forceinline__ bool block_done(const CGStatus<T>* st, int* s_flag) {
+    if (threadIdx.x == 0) *s_flag = __hip_atomic_load(&st->done, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return *s_flag != 0;
+}
+
+template <int LPR, typename T, bool PREC>
+__global__ __launch_bounds__(kBlock) void k_cg_init(CGArgs<T> a, int x_zero) {
+    __shared__ T lds[kWavesPerBlock];
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    uint32_t r0, r1;
+    block_row_range(a.m, a.chunk, L, r0, r1);
+    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    T prh = T(0), prr = T(0), pbb = T(0);
+    for (uint32_t row = r0 + g; row < r1; row += kBlock / LPR) {
+        T s = T(0);
+        if (!x_zero) s = row_product<LPR>(row, a.rp, a.ci, a.val, a.x, lane);
+        if (lane == 0) {
+            const T bi = a.b[row];
+            const T ri = add_rn(s, mul_rn(T(-1), bi));
+            const T hi = PREC ? mul_rn(a.dinv[row], ri) : ri;
+            a.r[row] = ri;
+            a.h[row] = hi;
+            a.d0[row] = T(0);
+            prh += ri * hi;
+            prr += ri * ri;
+            pbb += bi * bi;
+        }
+    }
+    prh = block_sum_all(prh, lds);
+    prr = block_sum_all(prr, lds);
+    pbb = block_sum_all(pbb, lds);
+    if (threadIdx.x == 0) {
+        a.part_rh[L] = prh;
+        a.part_rr[L] = prr;
+        a.part_bb[L] = pbb;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_init_finish(CGArgs<T> a, T eps, int max_iter) {
+    __shared__ T lds[kWavesPerBlock];
+    const T bb = reduce_partials(a.part_bb, a.P, lds);
+    if (threadIdx.x == 0) {
+        CGStatus<T>* st = a.st;
+        st->bb = bb;
+        st->thresh = (eps * eps) * bb;
+        st->rho = T(0);
+        st->rho_prev = T(1);
+        st->err = T(0);
+        st->alpha = T(0);
+        st->beta = T(0);
+        st->iter_a = 0;
+        st->iter_b = 0;
+        st->iters = 0;
+        st->max_iter = max_iter;
+        __hip_atomic_store(&st->done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <int LPR, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(CGArgs<T> a) {
+    __shared__ T lds[kWavesPerBlock];
+    __shared__ int s_flag;
+    CGStatus<T>* st = a.st;
+    if (block_done(st, &s_flag)) return;
+    const int it = st->iter_a;
+    const T rho = reduce_partials(a.part_rh, a.P, lds);
+    const T err = reduce_partials(a.part_rr, a.P, lds);
+    if (!(err > st->thresh) || it >= st->max_iter) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->iters = it;
+            st->err = err;
+            st->rho = rho;
+            __hip_atomic_store(&st->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+    }
+    const T beta = (it > 0) ? rho / st->rho_prev : T(0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rho = rho;
+        st->beta = beta;
+        st->err = err;
+        st->iter_b = it;
+    }
+    const T* __restrict__ dprev = (it & 1) ? a.d1 : a.d0;
+    T* __restrict__ dnew = (it & 1) ? a.d0 : a.d1;
+    const T* __restrict__ h = a.h;
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    uint32_t r0, r1;
+    block_row_range(a.m, a.chunk, L, r0, r1);
+    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    T pdq = T(0);
+    for (uint32_t row = r0 + g; row < r1; row += kBlock / LPR) {
+        const uint32_t jb = a.rp[row], je = a.rp[row + 1];
+        T acc = T(0);
+        for (uint32_t j = jb + lane; j < je; j += LPR) {
+            const uint32_t c = ntload(a.ci + j);
+            const T dj = add_rn(mul_rn(beta, dprev[c]), h[c]);
+            acc += ntload(a.val + j) * dj;
+        }
+        acc = group_sum<LPR>(acc);
+        if (lane == 0) {
+            const T di = add_rn(mul_rn(beta, dprev[row]), h[row]);
+            dnew[row] = di;
+            a.q[row] = acc;
+            pdq += di * acc;
+        }
+    }
+    pdq = block_sum_all(pdq, lds);
+    if (threadIdx.x == 0) a.part_dq[L] = pdq;
+}
+
+template <typename T, bool PREC>
+__global__ __launch_bounds__(kBlock) void k_cg_update(CGArgs<T> a) {
+    __shared__ T lds[kWavesPerBlock];
+    __shared__ int s_flag;
+    CGStatus<T>* st = a.st;
+    if (block_done(st, &s_flag)) return;
+    const int it = st->iter_b;
+    const T dq = reduce_partials(a.part_dq, a.P, lds);
+    const T rho = st->rho;
+    const T alpha = rho / dq;
+    const T* __restrict__ d = (it & 1) ? a.d0 : a.d1;
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    uint32_t r0, r1;
+    block_row_range(a.m, a.chunk, L, r0, r1);
+    T prh = T(0), prr = T(0);
+    for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+        const T xi = add_rn(a.x[i], mul_rn(-alpha, d[i]));
+        const T ri = add_rn(a.r[i], mul_rn(-alpha, a.q[i]));
+        const T hi = PREC ? mul_rn(a.dinv[i], ri) : ri;
+        a.x[i] = xi;
+        a.r[i] = ri;
+        a.h[i] = hi;
+        prh += ri * hi;
+        prr += ri * ri;
+    }
+    prh = block_sum_all(prh, lds);
+    prr = block_sum_all(prr, lds);
+    if (threadIdx.x == 0) {
+        a.part_rh[L] = prh;
+        a.part_rr[L] = prr;
+        if (blockIdx.x == 0) {
+            st->rho_prev = rho;
+            st->alpha = alpha;
+            st->iter_a = it + 1;
+        }
+    }
+}
+
+// ---------------------------------------------------- refinement helpers
+
+// out = float(r * s)
+__global__ __launch_bounds__(kBlock) void k_scale_cast(uint64_t n, const double* __restrict__ r,
+                                                       double s, float* __restrict__ out) {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
+        out[i] = float(r[i] * s);
+}
+
+// x += s * double(c)
+__global__ __launch_bounds__(kBlock) void k_ir_update(uint64_t n, double* __restrict__ x,
+                                                      const float* __restrict__ c, double s) {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
+        x[i] = x[i] + s * double(c[i]);
+}
+
+// ------------------------------------------------------------- launchers
+
+template <typename T>
+hipError_t cg_start(const CGArgs<T>& a, int lpr, bool precond, bool x_zero, T eps,
+                    int max_iter, hipStream_t s) {
+    if (a.m == 0 || a.P == 0) return hipErrorInvalidValue;
+    const int xz = x_zero ? 1 : 0;
+    if (precond) {
+        NLHIP_LPR_SWITCH(lpr, (k_cg_init<LPR, T, true><<<a.P, kBlock, 0, s>>>(a, xz)));
+    } else {
+        NLHIP_LPR_SWITCH(lpr, (k_cg_init<LPR, T, false><<<a.P, kBlock, 0, s>>>(a, xz)));
+    }
+    k_cg_init_finish<T><<<1, kBlock, 0, s>>>(a, eps, max_iter);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t cg_step(const CGArgs<T>& a, int lpr, bool precond, hipStream_t s) {
+    NLHIP_LPR_SWITCH(lpr, (k_cg_spmv<LPR, T><<<a.P, kBlock, 0, s>>>(a)));
+    if (precond) k_cg_update<T, true><<<a.P, kBlock, 0, s>>>(a);
+    else k_cg_update<T, false><<<a.P, kBlock, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t cg_step_part(const CGArgs<T>& a, int lpr, bool precond, int part, hipStream_t s) {
+    if (part == 0) {
+        NLHIP_LPR_SWITCH(lpr, (k_cg_spmv<LPR, T><<<a.P, kBlock, 0, s>>>(a)));
+    } else if (precond) {
+        k_cg_update<T, true><<<a.P, kBlock, 0, s>>>(a);
+    } else {
+        k_cg_update<T, false><<<a.P, kBlock, 0, s>>>(a);
+    }
+    return hipGetLastError();
+}
+template hipError_t cg_step_part<double>(const CGArgs<double>&, int, bool, int, hipStream_t);
+template hipError_t cg_step_part<float>(const CGArgs<float>&, int, bool, int, hipStream_t);
+
+template hipError_t cg_start<double>(const CGArgs<double>&, int, bool, bool, double, int,
+                                     hipStream_t);
+template hipError_t cg_start<float>(const CGArgs<float>&, int, bool, bool, float, int,
+                                    hipStream_t);
+template hipError_t cg_step<double>(const CGArgs<double>&, int, bool, hipStream_t);
+template hipError_t cg_step<float>(const CGArgs<float>&, int, bool, hipStream_t);
+
+static uint32_t ew_grid(uint64_t n) {
+    uint64_t g = (n + kBlock - 1) / kBlock;
+    return uint32_t(g > 4096 ? 4096 : (g == 0 ? 1 : g));
+}
+
+hipError_t ir_scale_cast(uint64_t n, const double* r, double s, float* out, hipStream_t st) {
+    k_scale_cast<<<ew_grid(n), kBlock, 0, st>>>(n, r, s, out);
+    return hipGetLastError();
+}
+
+hipError_t ir_update(uint64_t n, double* x, const float* c, double s, hipStream_t st) {
+    k_ir_update<<<ew_grid(n), kBlock, 0, st>>>(n, x, c, s);
+    return hipGetLastError();
+}
+
+}  // namespace nlhip

@@ -1,0 +1,141 @@
+// CRS sparse matrix-vector kernels for gfx950.
+//
+// Layout: rowptr[m+1] (u32), colind[nnz] (u32), val[nnz] (f64 or f32).
+// Each row is handled by a group of LPR lanes (LPR in {4,8,16,32}); a
+// 256-thread workgroup handles 256/LPR rows at a time and walks one
+// contiguous range of rows (XCD-aware: see xcd_logical_block).  The val and
+// colind streams are read once with non-temporal loads; x is gathered through
+// L2.  The group's partial products are combined by a fixed xor butterfly,
+// so every result is deterministic.
+//
+// Follows nlMultMatrixVector on CRS matrices (y_i = sum_j a_ij x_j),
+// called at src/lib/OGF/scene_graph/NL/matrix.cpp:303-305.
+#include "dev_common.h"
+#include "nlhip_kernels.h"
+
+namespace nlhip {
+
+template <int LPR, typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv(uint32_t m, uint32_t chunk,
+                                                 const uint32_t* __restrict__ rp,
+                                                 const uint32_t* __restrict__ ci,
+                                                 const T* __restrict__ val,
+                                                 const T* __restrict__ x,
+                                                 T* __restrict__ y) {
+    uint32_t r0, r1;
+    block_row_range(m, chunk, xcd_logical_block(blockIdx.x, gridDim.x), r0, r1);
+    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    for (uint32_t row = r0 + g; row < r1; row += kBlock / LPR) {
+        const T s = row_product<LPR>(row, rp, ci, val, x, lane);
+        if (lane == 0) y[row] = s;
+    }
+}
+
+// r = b - A x (FP64), with per-block partial sums of r.r and b.b written to
+// part_rr[L], part_bb[L] (L = logical block).
+template <int LPR>
+__global__ __launch_bounds__(kBlock) void k_residual(
+    uint32_t m, uint32_t chunk, const uint32_t* __restrict__ rp,
+    const uint32_t* __restrict__ ci, const double* __restrict__ val,
+    const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
+    double* __restrict__ part_rr, double* __restrict__ part_bb) {
+    __shared__ double lds[kWavesPerBlock];
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    uint32_t r0, r1;
+    block_row_range(m, chunk, L, r0, r1);
+    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    double srr = 0.0, sbb = 0.0;
+    for (uint32_t row = r0 + g; row < r1; row += kBlock / LPR) {
+        const double s = row_product<LPR>(row, rp, ci, val, x, lane);
+        if (lane == 0) {
+            const double bi = b[row];
+            const double ri = bi - s;
+            if (r) r[row] = ri;
+            srr += ri * ri;
+            sbb += bi * bi;
+        }
+    }
+    srr = block_sum_all(srr, lds);
+    sbb = block_sum_all(sbb, lds);
+    if (threadIdx.x == 0) {
+        part_rr[L] = srr;
+        part_bb[L] = sbb;
+    }
+}
+
+// dinv[i] = 1 / a_ii (1 when the row has no or a zero diagonal entry);
+// duplicate diagonal entries are summed.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_jacobi(uint32_t m, const uint32_t* __restrict__ rp,
+                                                   const uint32_t* __restrict__ ci,
+                                                   const double* __restrict__ val,
+                                                   T* __restrict__ dinv) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    double d = 0.0;
+    for (uint32_t j = rp[i]; j < rp[i + 1]; ++j)
+        if (ci[j] == i) d += val[j];
+    const T dt = T(d);
+    dinv[i] = (dt != T(0)) ? T(1) / dt : T(1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_cast_f32(uint64_t n, const double* __restrict__ a,
+                                                     float* __restrict__ b) {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
+        b[i] = float(a[i]);
+}
+
+// ---------------------------------------------------------------- launchers
+
+uint32_t spmv_chunk(uint32_t m, uint32_t grid, int lpr) {
+    const uint32_t rows_per_pass = uint32_t(kBlock / lpr);
+    uint64_t c = (uint64_t(m) + grid - 1) / grid;
+    c = (c + rows_per_pass - 1) / rows_per_pass * rows_per_pass;
+    if (c == 0) c = rows_per_pass;
+    return uint32_t(c);
+}
+
+template <typename T>
+hipError_t spmv(uint32_t m, const uint32_t* rp, const uint32_t* ci, const T* val, const T* x,
+                T* y, int lpr, uint32_t grid, hipStream_t s) {
+    if (m == 0) return hipSuccess;
+    const uint32_t chunk = spmv_chunk(m, grid, lpr);
+    NLHIP_LPR_SWITCH(lpr, (k_spmv<LPR, T><<<grid, kBlock, 0, s>>>(m, chunk, rp, ci, val, x, y)));
+    return hipGetLastError();
+}
+template hipError_t spmv<double>(uint32_t, const uint32_t*, const uint32_t*, const double*,
+                                 const double*, double*, int, uint32_t, hipStream_t);
+template hipError_t spmv<float>(uint32_t, const uint32_t*, const uint32_t*, const float*,
+                                const float*, float*, int, uint32_t, hipStream_t);
+
+hipError_t residual64(uint32_t m, const uint32_t* rp, const uint32_t* ci, const double* val,
+                      const double* x, const double* b, double* r, double* part_rr,
+                      double* part_bb, int lpr, uint32_t grid, hipStream_t s) {
+    const uint32_t chunk = spmv_chunk(m, grid, lpr);
+    NLHIP_LPR_SWITCH(lpr, (k_residual<LPR><<<grid, kBlock, 0, s>>>(m, chunk, rp, ci, val, x, b,
+                                                                   r, part_rr, part_bb)));
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t jacobi_dinv(uint32_t m, const uint32_t* rp, const uint32_t* ci, const double* val,
+                       T* dinv, hipStream_t s) {
+    if (m == 0) return hipSuccess;
+    k_jacobi<T><<<(m + kBlock - 1) / kBlock, kBlock, 0, s>>>(m, rp, ci, val, dinv);
+    return hipGetLastError();
+}
+template hipError_t jacobi_dinv<double>(uint32_t, const uint32_t*, const uint32_t*,
+                                        const double*, double*, hipStream_t);
+template hipError_t jacobi_dinv<float>(uint32_t, const uint32_t*, const uint32_t*,
+                                       const double*, float*, hipStream_t);
+
+hipError_t cast_f64_f32(uint64_t n, const double* a, float* b, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t g = (n + kBlock - 1) / kBlock;
+    if (g > 8192) g = 8192;
+    k_cast_f32<<<uint32_t(g), kBlock, 0, s>>>(n, a, b);
+    return hipGetLastError();
+}
+
+}  // namespace nlhip

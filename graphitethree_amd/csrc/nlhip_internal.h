@@ -1,0 +1,291 @@
+// Host-side runtime of libnlhip.so: device handle, buffers, resident
+// systems and the solve drivers.  C++17, compiled with g++ against the HIP
+// runtime API; kernels are reached through nlhip_kernels.h.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nl.h"
+#include "nl_hip.h"
+#include "nlhip_kernels.h"
+
+namespace nlhip {
+
+// ------------------------------------------------------------------ errors
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char* expr, const char* file, int line);
+
+#define NLHIP_CHECK(expr)                                                   \
+    do {                                                                    \
+        hipError_t nlhip_rc_ = (expr);                                      \
+        if (nlhip_rc_ != hipSuccess) ::nlhip::throw_hip(nlhip_rc_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+void set_last_error(const std::string& msg);
+void log_error(const std::string& msg);  // "[NL] msg" on stderr + last error
+
+// Runs f(); converts exceptions to an error code (never lets one escape).
+template <typename F>
+int guarded(const char* where, F&& f) {
+    try {
+        f();
+        return NLHIP_OK;
+    } catch (const Error& e) {
+        log_error(std::string(where) + ": " + e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        log_error(std::string(where) + ": host allocation failed");
+        return NLHIP_ERR_OOM;
+    } catch (const std::exception& e) {
+        log_error(std::string(where) + ": " + e.what());
+        return NLHIP_ERR_INVALID;
+    }
+}
+
+// ------------------------------------------------------------------ device
+
+struct Device {
+    int id = -1;
+    int cu_count = 0;
+    uint64_t hbm_bytes = 0;
+    std::string name;
+    std::string arch;
+};
+
+// Initialises the device on first call; throws Error(NLHIP_ERR_NO_DEVICE)
+// when no gfx950 device is usable.
+const Device& device();
+bool device_usable();  // no throw
+
+// Fault-injection hook for tests: the next `count` solves fail with
+// NLHIP_ERR_INJECTED before touching the device.
+bool take_injected_failure();
+
+// ------------------------------------------------------------------ buffers
+
+template <typename T>
+class DevBuf {
+  public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) n = 1;
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, n * sizeof(T));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(NLHIP_ERR_OOM, "hipMalloc of " + std::to_string(n * sizeof(T)) +
+                                           " bytes failed: " + hipGetErrorString(e));
+        }
+        p_ = static_cast<T*>(p);
+        n_ = n;
+    }
+    void ensure(size_t n) { if (n_ < n || !p_) alloc(n); }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+    size_t bytes() const { return n_ * sizeof(T); }
+
+  private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+template <typename T>
+class PinnedBuf {
+  public:
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { release(); }
+    void ensure(size_t n) {
+        if (n_ >= n && p_) return;
+        release();
+        if (n == 0) n = 1;
+        void* p = nullptr;
+        hipError_t e = hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(NLHIP_ERR_OOM, std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
+        }
+        p_ = static_cast<T*>(p);
+        n_ = n;
+    }
+    void release() {
+        if (p_) (void)hipHostFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* get() const { return p_; }
+
+  private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// True when p is device (or managed) memory the GPU can address directly.
+bool is_device_pointer(const void* p);
+
+// Copies n elements into a device buffer from a host or device pointer.
+void copy_in(void* dst, const void* src, size_t bytes, hipStream_t s);
+// Copies n elements from device to a host or device pointer.
+void copy_out(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// ------------------------------------------------------------------ stream
+
+class Stream {
+  public:
+    Stream();
+    ~Stream();
+    Stream(const Stream&) = delete;
+    Stream& operator=(const Stream&) = delete;
+    hipStream_t get() const { return s_; }
+    void sync() const { NLHIP_CHECK(hipStreamSynchronize(s_)); }
+
+  private:
+    hipStream_t s_ = nullptr;
+};
+
+class Event {
+  public:
+    Event();
+    ~Event();
+    Event(const Event&) = delete;
+    Event& operator=(const Event&) = delete;
+    hipEvent_t get() const { return e_; }
+    void record(hipStream_t s) { NLHIP_CHECK(hipEventRecord(e_, s)); }
+    void sync() { NLHIP_CHECK(hipEventSynchronize(e_)); }
+
+  private:
+    hipEvent_t e_ = nullptr;
+};
+
+float elapsed_ms(const Event& a, const Event& b);
+
+// ------------------------------------------------------------------ graphs
+
+// A captured sequence of launches replayed as one hipGraph.
+class Graph {
+  public:
+    Graph() = default;
+    ~Graph() { reset(); }
+    Graph(const Graph&) = delete;
+    Graph& operator=(const Graph&) = delete;
+    template <typename F>
+    void capture(hipStream_t s, F&& body) {
+        reset();
+        NLHIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t err = hipSuccess;
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        NLHIP_CHECK(hipStreamEndCapture(s, &graph_));
+        NLHIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+        (void)err;
+    }
+    void launch(hipStream_t s) const { NLHIP_CHECK(hipGraphLaunch(exec_, s)); }
+    bool ready() const { return exec_ != nullptr; }
+    void reset() {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        exec_ = nullptr;
+        graph_ = nullptr;
+    }
+
+  private:
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+};
+
+// ------------------------------------------------------------------ system
+
+template <typename T>
+struct CGWorkspace {
+    uint32_t n = 0, P = 0;
+    DevBuf<T> x, b, r, h, d0, d1, q, part;
+    DevBuf<CGStatus<T>> st;
+    PinnedBuf<CGStatus<T>> st_host;  // two slots
+    Graph graph;
+    int graph_iters = 0;
+    int graph_lpr = 0;
+    bool graph_precond = false;
+    CGArgs<T> graph_args{};  // arguments baked into the captured graph
+    CGArgs<T> args{};
+    void ensure(uint32_t n_, uint32_t P_);
+};
+
+// A CRS matrix resident on the device, with its Jacobi inverse diagonal and
+// (optionally) FP32 copies, its own stream and reusable solve workspaces.
+struct System {
+    uint32_t n = 0;
+    uint64_t nnz = 0;
+    DevBuf<uint32_t> rowptr, colind;
+    DevBuf<double> val, dinv;
+    DevBuf<float> val32, dinv32;
+    int lpr = 8;
+    uint32_t grid = 1;
+    Stream stream;
+    std::mutex mu;
+    CGWorkspace<double> ws64;
+    CGWorkspace<float> ws32;
+    DevBuf<double> scratch;  // residual vector + partials for refinement
+    // Jacobi inverse diagonal supplied by a separate preconditioner object
+    // (nlSolveSystemIterative's P); used instead of dinv while set.
+    const double* dinv_override = nullptr;
+    PinnedBuf<double> host_scalars;
+
+    void ensure_fp32();
+    uint64_t device_bytes() const;
+};
+
+// Validates and uploads; throws Error.
+System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
+                      const double* val, uint32_t flags);
+void default_launch(uint32_t n, uint64_t nnz, int& lpr, uint32_t& grid);
+
+// Solve drivers (System::mu must be held).  b, x: device pointers of length n.
+void solve_fp64(System& S, const double* b_dev, double* x_dev, const nlhip_solve_params& p,
+                nlhip_solve_stats& st);
+void solve_mixed(System& S, const double* b_dev, double* x_dev, const nlhip_solve_params& p,
+                 nlhip_solve_stats& st);
+// |b - A x| / |b| in FP64 on the device.
+double true_rel_residual(System& S, const double* b_dev, const double* x_dev);
+// Full solve with staging of host/device b and x.
+void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
+           nlhip_solve_stats& st);
+
+// Device BiCGSTAB on BLAS-1 kernels (NL_BICGSTAB); returns iterations.
+uint32_t bicgstab_device(System& S, const double* dinv, const double* b_dev, double* x_dev,
+                         double eps, uint32_t max_iter, double* rel2);
+
+}  // namespace nlhip

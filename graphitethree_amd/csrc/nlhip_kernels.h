@@ -1,0 +1,89 @@
+// Host-callable launchers for the HIP kernels (defined in k_*.hip).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+#include "nlhip_types.h"
+
+namespace nlhip {
+
+// Row-group width (lanes per row) dispatch over the compiled instances.
+#define NLHIP_LPR_SWITCH(lpr, CALL)                       \
+    switch (lpr) {                                        \
+        case 4: { constexpr int LPR = 4; CALL; } break;   \
+        case 8: { constexpr int LPR = 8; CALL; } break;   \
+        case 16: { constexpr int LPR = 16; CALL; } break; \
+        case 32: { constexpr int LPR = 32; CALL; } break; \
+        default: return hipErrorInvalidValue;             \
+    }
+
+// Rows per workgroup for an SpMV-shaped launch of `grid` workgroups.
+uint32_t spmv_chunk(uint32_t m, uint32_t grid, int lpr);
+
+template <typename T>
+hipError_t spmv(uint32_t m, const uint32_t* rp, const uint32_t* ci, const T* val, const T* x,
+                T* y, int lpr, uint32_t grid, hipStream_t s);
+
+// r = b - A x (r may be null), partial sums of r.r and b.b per workgroup.
+hipError_t residual64(uint32_t m, const uint32_t* rp, const uint32_t* ci, const double* val,
+                      const double* x, const double* b, double* r, double* part_rr,
+                      double* part_bb, int lpr, uint32_t grid, hipStream_t s);
+
+template <typename T>
+hipError_t jacobi_dinv(uint32_t m, const uint32_t* rp, const uint32_t* ci, const double* val,
+                       T* dinv, hipStream_t s);
+
+hipError_t cast_f64_f32(uint64_t n, const double* a, float* b, hipStream_t s);
+
+// BLAS-1
+hipError_t blas_dscal(uint64_t n, double a, double* x, hipStream_t s);
+hipError_t blas_daxpy(uint64_t n, double a, const double* x, double* y, hipStream_t s);
+hipError_t blas_dmul(uint64_t n, const double* a, const double* x, double* y, hipStream_t s);
+template <typename T>
+hipError_t fill(uint64_t n, T v, T* x, hipStream_t s);
+// out[0] = x.y, out[1] = sqrt(x.y); part holds `grid` doubles.
+hipError_t blas_ddot(uint64_t n, const double* x, const double* y, double* part, uint32_t grid,
+                     double* out, hipStream_t s);
+// out[0] = sum(part[0..P)), out[1] = sqrt(out[0]).
+hipError_t reduce_final(const double* part, uint32_t P, double* out, hipStream_t s);
+
+// Fused PCG.  All arrays are device memory.
+template <typename T>
+struct CGArgs {
+    uint32_t m;      // rows
+    uint32_t chunk;  // rows per workgroup (spmv_chunk(m, P, lpr))
+    uint32_t P;      // workgroups per launch = number of partial sums
+    uint32_t pad;
+    const uint32_t* rp;
+    const uint32_t* ci;
+    const T* val;
+    const T* dinv;   // null: no preconditioner
+    T* x;
+    const T* b;
+    T* r;
+    T* h;
+    T* d0;
+    T* d1;
+    T* q;
+    T* part_dq;
+    T* part_rh;
+    T* part_rr;
+    T* part_bb;
+    CGStatus<T>* st;
+};
+
+template <typename T>
+hipError_t cg_start(const CGArgs<T>& a, int lpr, bool precond, bool x_zero, T eps,
+                    int max_iter, hipStream_t s);
+template <typename T>
+hipError_t cg_step(const CGArgs<T>& a, int lpr, bool precond, hipStream_t s);
+// One of the two launches of an iteration: part 0 = spmv step, 1 = update.
+template <typename T>
+hipError_t cg_step_part(const CGArgs<T>& a, int lpr, bool precond, int part, hipStream_t s);
+
+// Mixed-precision refinement helpers.
+hipError_t ir_scale_cast(uint64_t n, const double* r, double s, float* out, hipStream_t st);
+hipError_t ir_update(uint64_t n, double* x, const float* c, double s, hipStream_t st);
+
+}  // namespace nlhip

@@ -1,0 +1,586 @@
+// Resident systems and the solve drivers: FP64 Jacobi-PCG, FP32-inner /
+// FP64-outer iterative refinement, device BiCGSTAB, SpMV, and the C ABI of
+// the handle-based API (include/nl_hip.h, part 2).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+
+#include "nlhip_internal.h"
+
+namespace nlhip {
+
+// ----------------------------------------------------------------- launch
+
+void default_launch(uint32_t n, uint64_t nnz, int& lpr, uint32_t& grid) {
+    const double avg = n ? double(nnz) / double(n) : 1.0;
+    lpr = avg <= 5.0 ? 4 : avg <= 11.0 ? 8 : avg <= 24.0 ? 16 : 32;
+    const uint32_t rows_per_pass = uint32_t(256 / lpr);  // 256-thread workgroups
+    uint64_t g = (uint64_t(n) + rows_per_pass - 1) / rows_per_pass;
+    uint32_t cap = 2048;
+    if (device_usable()) cap = uint32_t(std::max(8, device().cu_count * 8));
+    grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
+}
+
+// ----------------------------------------------------------------- system
+
+static void validate_crs(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* val) {
+    if (rp[0] != 0) throw Error(NLHIP_ERR_INVALID, "rowptr[0] != 0");
+    for (uint32_t i = 0; i < n; ++i)
+        if (rp[i + 1] < rp[i])
+            throw Error(NLHIP_ERR_INVALID, "rowptr is not monotone at row " + std::to_string(i));
+    const uint64_t nnz = rp[n];
+    int bad_col = 0, bad_val = 0;
+#pragma omp parallel for reduction(| : bad_col, bad_val) schedule(static)
+    for (int64_t j = 0; j < int64_t(nnz); ++j) {
+        bad_col |= (ci[j] >= n);
+        bad_val |= !std::isfinite(val[j]);
+    }
+    if (bad_col) throw Error(NLHIP_ERR_INVALID, "colind out of range");
+    if (bad_val) throw Error(NLHIP_ERR_NONFINITE, "matrix value is NaN or Inf");
+}
+
+System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
+                      const double* val, uint32_t flags) {
+    if (n == 0 || !rowptr) throw Error(NLHIP_ERR_INVALID, "empty system");
+    device();
+    // Validation runs on host copies (a device caller's arrays are fetched).
+    std::vector<uint32_t> h_rp, h_ci;
+    std::vector<double> h_val;
+    const uint32_t* rp = rowptr;
+    const uint32_t* ci = colind;
+    const double* va = val;
+    if (is_device_pointer(rowptr)) {
+        h_rp.resize(size_t(n) + 1);
+        NLHIP_CHECK(hipMemcpy(h_rp.data(), rowptr, (size_t(n) + 1) * 4, hipMemcpyDefault));
+        rp = h_rp.data();
+    }
+    const uint64_t nnz = rp[n];
+    if (nnz > 0 && (!colind || !val)) throw Error(NLHIP_ERR_INVALID, "null colind/val");
+    if (nnz && is_device_pointer(colind)) {
+        h_ci.resize(nnz);
+        NLHIP_CHECK(hipMemcpy(h_ci.data(), colind, nnz * 4, hipMemcpyDefault));
+        ci = h_ci.data();
+    }
+    if (nnz && is_device_pointer(val)) {
+        h_val.resize(nnz);
+        NLHIP_CHECK(hipMemcpy(h_val.data(), val, nnz * 8, hipMemcpyDefault));
+        va = h_val.data();
+    }
+    validate_crs(n, rp, ci, va);
+    auto S = std::make_unique<System>();
+    S->n = n;
+    S->nnz = nnz;
+    S->rowptr.alloc(size_t(n) + 1);
+    S->colind.alloc(nnz);
+    S->val.alloc(nnz);
+    S->dinv.alloc(n);
+    hipStream_t s = S->stream.get();
+    copy_in(S->rowptr.get(), rp, (size_t(n) + 1) * 4, s);
+    copy_in(S->colind.get(), ci, nnz * 4, s);
+    copy_in(S->val.get(), va, nnz * 8, s);
+    NLHIP_CHECK(jacobi_dinv<double>(n, S->rowptr.get(), S->colind.get(), S->val.get(),
+                                    S->dinv.get(), s));
+    default_launch(n, nnz, S->lpr, S->grid);
+    if (flags & NLHIP_SYSTEM_FP32) S->ensure_fp32();
+    S->stream.sync();
+    return S.release();
+}
+
+void System::ensure_fp32() {
+    if (val32.get() && val32.size() >= std::max<uint64_t>(nnz, 1)) return;
+    val32.alloc(nnz);
+    dinv32.alloc(n);
+    hipStream_t s = stream.get();
+    NLHIP_CHECK(cast_f64_f32(nnz, val.get(), val32.get(), s));
+    NLHIP_CHECK(jacobi_dinv<float>(n, rowptr.get(), colind.get(), val.get(), dinv32.get(), s));
+}
+
+uint64_t System::device_bytes() const {
+    return rowptr.bytes() + colind.bytes() + val.bytes() + dinv.bytes() + val32.bytes() +
+           dinv32.bytes() + ws64.x.bytes() * 7 + ws32.x.bytes() * 7 + scratch.bytes();
+}
+
+template <typename T>
+void CGWorkspace<T>::ensure(uint32_t n_, uint32_t P_) {
+    if (n == n_ && P == P_ && x.get()) return;
+    graph.reset();
+    n = n_;
+    P = P_;
+    x.alloc(n);
+    b.alloc(n);
+    r.alloc(n);
+    h.alloc(n);
+    d0.alloc(n);
+    d1.alloc(n);
+    q.alloc(n);
+    part.alloc(size_t(4) * P);
+    st.alloc(1);
+    st_host.ensure(2);
+}
+template struct CGWorkspace<double>;
+template struct CGWorkspace<float>;
+
+template <typename T>
+static void setup_args(System& S, CGWorkspace<T>& ws, const T* val, const T* dinv) {
+    CGArgs<T>& a = ws.args;
+    a.m = S.n;
+    a.P = S.grid;
+    a.chunk = spmv_chunk(S.n, S.grid, S.lpr);
+    a.rp = S.rowptr.get();
+    a.ci = S.colind.get();
+    a.val = val;
+    a.dinv = dinv;
+    a.x = ws.x.get();
+    a.b = ws.b.get();
+    a.r = ws.r.get();
+    a.h = ws.h.get();
+    a.d0 = ws.d0.get();
+    a.d1 = ws.d1.get();
+    a.q = ws.q.get();
+    a.part_dq = ws.part.get();
+    a.part_rh = ws.part.get() + S.grid;
+    a.part_rr = ws.part.get() + 2 * size_t(S.grid);
+    a.part_bb = ws.part.get() + 3 * size_t(S.grid);
+    a.st = ws.st.get();
+}
+
+static uint32_t auto_check_every(const System& S) {
+    // about 2 ms of device work per captured graph, within [8, 128]
+    const double bytes_per_iter = 12.0 * double(S.nnz) + 100.0 * double(S.n);
+    const double iter_s = std::max(bytes_per_iter / 4.0e12, 6.0e-6);
+    return uint32_t(std::clamp(2.0e-3 / iter_s, 8.0, 128.0));
+}
+
+// Runs PCG on ws (args already set, b and x filled) until the device-side
+// stopping test holds.  Returns the final status.
+template <typename T>
+static CGStatus<T> run_cg(System& S, CGWorkspace<T>& ws, bool precond, T eps, int max_iter,
+                          uint32_t K) {
+    hipStream_t s = S.stream.get();
+    NLHIP_CHECK(cg_start<T>(ws.args, S.lpr, precond, false, eps, max_iter, s));
+    if (!ws.graph.ready() || ws.graph_iters != int(K) || ws.graph_lpr != S.lpr ||
+        ws.graph_precond != precond ||
+        std::memcmp(&ws.graph_args, &ws.args, sizeof(CGArgs<T>)) != 0) {
+        ws.graph.capture(s, [&] {
+            for (uint32_t k = 0; k < K; ++k) NLHIP_CHECK(cg_step<T>(ws.args, S.lpr, precond, s));
+        });
+        ws.graph_iters = int(K);
+        ws.graph_lpr = S.lpr;
+        ws.graph_precond = precond;
+        ws.graph_args = ws.args;
+    }
+    Event ev[2];
+    CGStatus<T>* host = ws.st_host.get();
+    host[0].done = host[1].done = 0;
+    const uint64_t max_graphs = uint64_t(max_iter) / K + 4;
+    uint64_t launched = 0;
+    for (;;) {
+        ws.graph.launch(s);
+        const int slot = int(launched & 1);
+        NLHIP_CHECK(hipMemcpyAsync(&host[slot], ws.st.get(), sizeof(CGStatus<T>),
+                                   hipMemcpyDeviceToHost, s));
+        ev[slot].record(s);
+        ++launched;
+        if (launched >= 2) {
+            const int prev = int((launched - 2) & 1);
+            ev[prev].sync();
+            if (host[prev].done) break;
+        }
+        if (launched > max_graphs) {
+            S.stream.sync();
+            if (host[slot].done) break;
+            throw Error(NLHIP_ERR_HIP, "CG did not reach its stopping test");
+        }
+    }
+    S.stream.sync();
+    CGStatus<T> fin;
+    NLHIP_CHECK(hipMemcpy(&fin, ws.st.get(), sizeof(fin), hipMemcpyDeviceToHost));
+    return fin;
+}
+
+// Reads |b - A x|^2 and |b|^2 (FP64) into rr, bb.
+static void residual_norms(System& S, const double* b, const double* x, double* r_out,
+                           double& rr, double& bb) {
+    hipStream_t s = S.stream.get();
+    const uint32_t P = S.grid;
+    S.scratch.ensure(size_t(2) * P + 4);
+    double* part = S.scratch.get();
+    double* out = part + 2 * size_t(P);
+    NLHIP_CHECK(residual64(S.n, S.rowptr.get(), S.colind.get(), S.val.get(), x, b, r_out, part,
+                           part + P, S.lpr, P, s));
+    NLHIP_CHECK(reduce_final(part, P, out, s));
+    NLHIP_CHECK(reduce_final(part + P, P, out + 2, s));
+    S.host_scalars.ensure(4);
+    NLHIP_CHECK(hipMemcpyAsync(S.host_scalars.get(), out, 4 * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+    S.stream.sync();
+    rr = S.host_scalars.get()[0];
+    bb = S.host_scalars.get()[2];
+}
+
+double true_rel_residual(System& S, const double* b, const double* x) {
+    double rr = 0, bb = 0;
+    residual_norms(S, b, x, nullptr, rr, bb);
+    return bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+}
+
+void solve_fp64(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st) {
+    CGWorkspace<double>& ws = S.ws64;
+    const bool precond = p.precond == NL_PRECOND_JACOBI;
+    const double* dinv = S.dinv_override ? S.dinv_override : S.dinv.get();
+    setup_args<double>(S, ws, S.val.get(), precond ? dinv : nullptr);
+    const uint32_t K = p.check_every ? p.check_every : auto_check_every(S);
+    CGStatus<double> fin = run_cg<double>(S, ws, precond, p.eps, int(p.max_iter), K);
+    if (!std::isfinite(fin.bb) || !std::isfinite(fin.err))
+        throw Error(NLHIP_ERR_NONFINITE, "right-hand side or iterate is NaN or Inf");
+    st.iterations = uint32_t(fin.iters);
+    st.outer_iterations = 0;
+    st.cg_rel_residual = fin.bb > 0 ? std::sqrt(fin.err / fin.bb) : std::sqrt(fin.err);
+    st.converged = !(fin.err > fin.thresh) ? 1 : 0;
+}
+
+void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st) {
+    S.ensure_fp32();
+    CGWorkspace<double>& w64 = S.ws64;
+    CGWorkspace<float>& w32 = S.ws32;
+    w32.ensure(S.n, S.grid);
+    const bool precond = p.precond == NL_PRECOND_JACOBI;
+    setup_args<float>(S, w32, S.val32.get(), precond ? S.dinv32.get() : nullptr);
+    hipStream_t s = S.stream.get();
+    const uint32_t K = p.check_every ? p.check_every : auto_check_every(S);
+    const uint32_t inner_cap = p.inner_max_iter ? p.inner_max_iter : p.max_iter;
+    uint32_t total = 0, outer = 0;
+    double rr = 0, bb = 0;
+    for (;;) {
+        residual_norms(S, w64.b.get(), w64.x.get(), w64.r.get(), rr, bb);
+        if (!std::isfinite(rr) || !std::isfinite(bb))
+            throw Error(NLHIP_ERR_NONFINITE, "right-hand side or iterate is NaN or Inf");
+        if (!(rr > (p.eps * p.eps) * bb)) break;
+        if (outer >= p.max_outer || total >= p.max_iter) break;
+        const double rn = std::sqrt(rr);
+        NLHIP_CHECK(ir_scale_cast(S.n, w64.r.get(), 1.0 / rn, w32.b.get(), s));
+        NLHIP_CHECK(fill<float>(S.n, 0.0f, w32.x.get(), s));
+        const uint32_t cap = std::min(inner_cap, p.max_iter - total);
+        CGStatus<float> fin =
+            run_cg<float>(S, w32, precond, float(p.inner_eps), int(cap), K);
+        total += uint32_t(fin.iters);
+        NLHIP_CHECK(ir_update(S.n, w64.x.get(), w32.x.get(), rn, s));
+        ++outer;
+        if (fin.iters == 0) break;
+    }
+    st.iterations = total;
+    st.outer_iterations = outer;
+    st.cg_rel_residual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    st.converged = !(rr > (p.eps * p.eps) * bb) ? 1 : 0;
+}
+
+void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
+           nlhip_solve_stats& st) {
+    std::lock_guard<std::mutex> lock(S.mu);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::memset(&st, 0, sizeof(st));
+    if (take_injected_failure()) throw Error(NLHIP_ERR_INJECTED, "injected failure");
+    if (p.precision != NLHIP_PREC_FP64 && p.precision != NLHIP_PREC_MIXED)
+        throw Error(NLHIP_ERR_INVALID, "unknown precision");
+    if (p.precond != NL_PRECOND_JACOBI && p.precond != NL_PRECOND_NONE)
+        throw Error(NLHIP_ERR_UNSUPPORTED, "only Jacobi or no preconditioner on the GPU");
+    S.ws64.ensure(S.n, S.grid);
+    hipStream_t s = S.stream.get();
+    copy_in(S.ws64.b.get(), b, size_t(S.n) * 8, s);
+    copy_in(S.ws64.x.get(), x, size_t(S.n) * 8, s);
+    Event e0, e1;
+    e0.record(s);
+    if (p.precision == NLHIP_PREC_FP64) solve_fp64(S, p, st);
+    else solve_mixed(S, p, st);
+    e1.record(s);
+    e1.sync();
+    st.device_seconds = 1e-3 * elapsed_ms(e0, e1);
+    st.rel_residual = true_rel_residual(S, S.ws64.b.get(), S.ws64.x.get());
+    copy_out(x, S.ws64.x.get(), size_t(S.n) * 8, s);
+    S.stream.sync();
+    st.wall_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st.status = NLHIP_OK;
+}
+
+// ------------------------------------------------------------- BiCGSTAB
+
+namespace {
+struct DotHelper {
+    System& S;
+    DevBuf<double>& part;
+    double* out;
+    double* host;
+    double operator()(const double* a, const double* b) {
+        hipStream_t s = S.stream.get();
+        NLHIP_CHECK(blas_ddot(S.n, a, b, part.get(), S.grid, out, s));
+        NLHIP_CHECK(hipMemcpyAsync(host, out, sizeof(double), hipMemcpyDeviceToHost, s));
+        S.stream.sync();
+        return *host;
+    }
+};
+}  // namespace
+
+// Same operation sequence as the van der Vorst BiCGSTAB restated in the
+// oracle (right preconditioning by dinv, r = b - A x).
+uint32_t bicgstab_device(System& S, const double* dinv, const double* b, double* x, double eps,
+                         uint32_t max_iter, double* rel2) {
+    const uint32_t n = S.n;
+    hipStream_t s = S.stream.get();
+    DevBuf<double> buf(size_t(8) * n);
+    double *r = buf.get(), *rt = r + n, *p = rt + n, *v = p + n, *ph = v + n, *sv = ph + n,
+           *sh = sv + n, *t = sh + n;
+    DevBuf<double> part(S.grid), out(2);
+    S.host_scalars.ensure(4);
+    DotHelper dot{S, part, out.get(), S.host_scalars.get()};
+    auto spmv64 = [&](const double* in, double* o) {
+        NLHIP_CHECK(spmv<double>(n, S.rowptr.get(), S.colind.get(), S.val.get(), in, o, S.lpr,
+                                 S.grid, s));
+    };
+    auto precond = [&](const double* in, double* o) {
+        if (dinv) NLHIP_CHECK(blas_dmul(n, dinv, in, o, s));
+        else NLHIP_CHECK(hipMemcpyAsync(o, in, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+    };
+    auto copy = [&](const double* in, double* o) {
+        NLHIP_CHECK(hipMemcpyAsync(o, in, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+    };
+    spmv64(x, r);
+    NLHIP_CHECK(blas_dscal(n, -1.0, r, s));
+    NLHIP_CHECK(blas_daxpy(n, 1.0, b, r, s));
+    copy(r, rt);
+    NLHIP_CHECK(fill<double>(n, 0.0, p, s));
+    NLHIP_CHECK(fill<double>(n, 0.0, v, s));
+    double rho = 1.0, alpha = 1.0, omega = 1.0;
+    const double bb = dot(b, b);
+    const double thresh = (eps * eps) * bb;
+    double err = dot(r, r);
+    uint32_t its = 0;
+    while (err > thresh && its < max_iter) {
+        const double rho_new = dot(rt, r);
+        if (rho_new == 0.0) break;
+        const double beta = (rho_new / rho) * (alpha / omega);
+        NLHIP_CHECK(blas_daxpy(n, -omega, v, p, s));
+        NLHIP_CHECK(blas_dscal(n, beta, p, s));
+        NLHIP_CHECK(blas_daxpy(n, 1.0, r, p, s));
+        precond(p, ph);
+        spmv64(ph, v);
+        const double rtv = dot(rt, v);
+        if (rtv == 0.0) break;
+        alpha = rho_new / rtv;
+        copy(r, sv);
+        NLHIP_CHECK(blas_daxpy(n, -alpha, v, sv, s));
+        precond(sv, sh);
+        spmv64(sh, t);
+        const double tt = dot(t, t);
+        NLHIP_CHECK(blas_daxpy(n, alpha, ph, x, s));
+        ++its;
+        if (tt == 0.0) {
+            copy(sv, r);
+            err = dot(r, r);
+            break;
+        }
+        omega = dot(t, sv) / tt;
+        NLHIP_CHECK(blas_daxpy(n, omega, sh, x, s));
+        copy(sv, r);
+        NLHIP_CHECK(blas_daxpy(n, -omega, t, r, s));
+        rho = rho_new;
+        err = dot(r, r);
+        if (omega == 0.0) break;
+    }
+    S.stream.sync();
+    if (rel2) *rel2 = bb > 0.0 ? err / bb : err;
+    return its;
+}
+
+}  // namespace nlhip
+
+using namespace nlhip;
+
+static System* sys(nlhip_system h) {
+    if (!h) throw Error(NLHIP_ERR_INVALID, "null system handle");
+    return reinterpret_cast<System*>(h);
+}
+
+extern "C" {
+
+NLAPI void NLAPIENTRY nlhipSolveParamsDefault(nlhip_solve_params* p) {
+    if (!p) return;
+    p->precision = NLHIP_PREC_FP64;
+    p->precond = NL_PRECOND_JACOBI;
+    p->eps = 1e-10;
+    p->max_iter = 5000;
+    p->check_every = 0;
+    p->inner_eps = 1e-4;
+    p->max_outer = 30;
+    p->inner_max_iter = 0;
+}
+
+NLAPI int NLAPIENTRY nlhipSystemCreate(uint32_t n, const uint32_t* rowptr,
+                                       const uint32_t* colind, const double* val,
+                                       uint32_t flags, nlhip_system* out) {
+    return guarded("nlhipSystemCreate", [&] {
+        if (!out) throw Error(NLHIP_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        System* S = system_create(n, rowptr, colind, val, flags);
+        *out = reinterpret_cast<nlhip_system>(S);
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipSystemDestroy(nlhip_system s) {
+    return guarded("nlhipSystemDestroy", [&] {
+        if (!s) return;
+        System* S = sys(s);
+        { std::lock_guard<std::mutex> lock(S->mu); S->stream.sync(); }
+        delete S;
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipSystemInfo(nlhip_system s, uint32_t* n, uint64_t* nnz,
+                                     uint64_t* device_bytes) {
+    return guarded("nlhipSystemInfo", [&] {
+        System* S = sys(s);
+        if (n) *n = S->n;
+        if (nnz) *nnz = S->nnz;
+        if (device_bytes) *device_bytes = S->device_bytes();
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int lanes_per_row,
+                                          uint32_t workgroups) {
+    return guarded("nlhipSystemSetLaunch", [&] {
+        System* S = sys(s);
+        if (lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16 &&
+            lanes_per_row != 32)
+            throw Error(NLHIP_ERR_INVALID, "lanes_per_row must be 4, 8, 16 or 32");
+        if (workgroups == 0 || workgroups > 65536)
+            throw Error(NLHIP_ERR_INVALID, "workgroups must be in [1, 65536]");
+        std::lock_guard<std::mutex> lock(S->mu);
+        S->lpr = lanes_per_row;
+        S->grid = workgroups;
+        S->ws64.graph.reset();
+        S->ws32.graph.reset();
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipSystemGetLaunch(nlhip_system s, int* lanes_per_row,
+                                          uint32_t* workgroups) {
+    return guarded("nlhipSystemGetLaunch", [&] {
+        System* S = sys(s);
+        if (lanes_per_row) *lanes_per_row = S->lpr;
+        if (workgroups) *workgroups = S->grid;
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipSolve(nlhip_system s, const double* b, double* x,
+                                const nlhip_solve_params* params, nlhip_solve_stats* stats) {
+    nlhip_solve_stats local;
+    nlhip_solve_stats& st = stats ? *stats : local;
+    int rc = guarded("nlhipSolve", [&] {
+        System* S = sys(s);
+        if (!b || !x) throw Error(NLHIP_ERR_INVALID, "null b or x");
+        nlhip_solve_params p;
+        nlhipSolveParamsDefault(&p);
+        if (params) p = *params;
+        solve(*S, b, x, p, st);
+    });
+    st.status = rc;
+    return rc;
+}
+
+NLAPI int NLAPIENTRY nlhipSpMV(nlhip_system s, const double* x, double* y) {
+    return guarded("nlhipSpMV", [&] {
+        System* S = sys(s);
+        std::lock_guard<std::mutex> lock(S->mu);
+        hipStream_t st = S->stream.get();
+        S->ws64.ensure(S->n, S->grid);
+        copy_in(S->ws64.x.get(), x, size_t(S->n) * 8, st);
+        NLHIP_CHECK(spmv<double>(S->n, S->rowptr.get(), S->colind.get(), S->val.get(),
+                                 S->ws64.x.get(), S->ws64.q.get(), S->lpr, S->grid, st));
+        copy_out(y, S->ws64.q.get(), size_t(S->n) * 8, st);
+        S->stream.sync();
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
+                                    double* seconds_per_spmv) {
+    return guarded("nlhipBenchSpMV", [&] {
+        System* S = sys(s);
+        std::lock_guard<std::mutex> lock(S->mu);
+        hipStream_t st = S->stream.get();
+        if (reps == 0) reps = 1;
+        Event e0, e1;
+        if (fp32) {
+            S->ensure_fp32();
+            S->ws32.ensure(S->n, S->grid);
+            NLHIP_CHECK(fill<float>(S->n, 1.0f, S->ws32.x.get(), st));
+            auto run = [&] {
+                NLHIP_CHECK(spmv<float>(S->n, S->rowptr.get(), S->colind.get(), S->val32.get(),
+                                        S->ws32.x.get(), S->ws32.q.get(), S->lpr, S->grid, st));
+            };
+            run();
+            e0.record(st);
+            for (uint32_t k = 0; k < reps; ++k) run();
+            e1.record(st);
+        } else {
+            S->ws64.ensure(S->n, S->grid);
+            NLHIP_CHECK(fill<double>(S->n, 1.0, S->ws64.x.get(), st));
+            auto run = [&] {
+                NLHIP_CHECK(spmv<double>(S->n, S->rowptr.get(), S->colind.get(), S->val.get(),
+                                         S->ws64.x.get(), S->ws64.q.get(), S->lpr, S->grid, st));
+            };
+            run();
+            e0.record(st);
+            for (uint32_t k = 0; k < reps; ++k) run();
+            e1.record(st);
+        }
+        e1.sync();
+        if (seconds_per_spmv) *seconds_per_spmv = 1e-3 * elapsed_ms(e0, e1) / reps;
+    });
+}
+
+// Runs `iters` PCG iterations (FP64, or FP32 with fp32 = 1) on the system
+// with the last right-hand side, launching each kernel directly and timing it
+// with HIP events on the system's stream; returns the mean duration of the
+// spmv step and of the update step.
+NLAPI int NLAPIENTRY nlhipProfileCG(nlhip_system s, int fp32, uint32_t iters,
+                                    double* spmv_step_seconds, double* update_step_seconds) {
+    return guarded("nlhipProfileCG", [&] {
+        System* S = sys(s);
+        std::lock_guard<std::mutex> lock(S->mu);
+        hipStream_t st = S->stream.get();
+        if (iters == 0) iters = 1;
+        auto profile = [&](auto& ws, auto* val, auto* dinv, auto zero) {
+            using T = decltype(zero);
+            ws.ensure(S->n, S->grid);
+            setup_args<T>(*S, ws, val, dinv);
+            NLHIP_CHECK(fill<T>(S->n, T(1), ws.b.get(), st));
+            NLHIP_CHECK(fill<T>(S->n, T(0), ws.x.get(), st));
+            NLHIP_CHECK(cg_start<T>(ws.args, S->lpr, true, false, T(0), int(iters) + 8, st));
+            std::vector<Event> ev(3 * size_t(iters));
+            for (uint32_t k = 0; k < iters; ++k) {
+                ev[3 * k].record(st);
+                NLHIP_CHECK(cg_step_part<T>(ws.args, S->lpr, true, 0, st));
+                ev[3 * k + 1].record(st);
+                NLHIP_CHECK(cg_step_part<T>(ws.args, S->lpr, true, 1, st));
+                ev[3 * k + 2].record(st);
+            }
+            S->stream.sync();
+            double ta = 0, tb = 0;
+            for (uint32_t k = 0; k < iters; ++k) {
+                ta += 1e-3 * elapsed_ms(ev[3 * k], ev[3 * k + 1]);
+                tb += 1e-3 * elapsed_ms(ev[3 * k + 1], ev[3 * k + 2]);
+            }
+            if (spmv_step_seconds) *spmv_step_seconds = ta / iters;
+            if (update_step_seconds) *update_step_seconds = tb / iters;
+        };
+        if (fp32) {
+            S->ensure_fp32();
+            profile(S->ws32, S->val32.get(), S->dinv32.get(), 0.0f);
+        } else {
+            profile(S->ws64, S->val.get(), S->dinv.get(), 0.0);
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+/*
+ * OpenNL HIP extension for AMD Instinct MI355X (gfx950) -- libnlhip.so.
+ *
+ * Part 1 fills the role OpenNL's CUDA extension plays behind
+ * nlInitExtension("CUDA") (state recorded as nl:CUDA in
+ * lib/data/violin.obj:82): device NLBlas, device CRS matrices and a device
+ * Jacobi preconditioner that nlSolveSystemIterative
+ * (src/lib/OGF/scene_graph/NL/matrix.cpp:390-393) accepts in place of the
+ * host ones.  Names follow the nlCUDA* pattern with HIP in place of CUDA;
+ * upstream has no HIP extension, so these are this library's own exports.
+ *
+ * Part 2 is the re-entrant, handle-based API underneath: resident systems,
+ * FP64 and mixed-precision solves, and their statistics.  Every function
+ * returns NLHIP_OK (0) or a negative error code and never throws; the
+ * message of the last error on the calling thread is nlhipLastError().
+ */
+#ifndef OPENNL_HIP_NL_HIP_H
+#define OPENNL_HIP_NL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "nl.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Part 1: OpenNL extension entry points ----------------------------- */
+
+/* nlInitExtension("HIP") also calls this. */
+NLAPI NLboolean NLAPIENTRY nlInitExtension_HIP(void);
+NLAPI NLboolean NLAPIENTRY nlExtensionIsInitialized_HIP(void);
+/* Device-memory NLBlas: Malloc/Free/Memcpy on the GPU, BLAS-1 as kernels,
+ * has_unique_memory = NL_FALSE. */
+NLAPI NLBlas_t NLAPIENTRY nlHIPBlas(void);
+/* Uploads a host CRS (or dynamic) matrix; the result is resident on the
+ * device, its mult_func takes device pointers. */
+NLAPI NLMatrix NLAPIENTRY nlHIPMatrixNewFromCRSMatrix(NLMatrix M);
+/* Jacobi preconditioner of M, computed and kept on the device. */
+NLAPI NLMatrix NLAPIENTRY nlHIPJacobiPreconditionerNewFromCRSMatrix(NLMatrix M);
+
+/* ---- Part 2: handle-based API ------------------------------------------ */
+
+#define NLHIP_OK               0
+#define NLHIP_ERR_INVALID     -1   /* bad argument / malformed CRS          */
+#define NLHIP_ERR_NO_DEVICE   -2   /* no usable gfx950 device               */
+#define NLHIP_ERR_HIP         -3   /* HIP runtime error                     */
+#define NLHIP_ERR_OOM         -4   /* device allocation failed              */
+#define NLHIP_ERR_NONFINITE   -5   /* NaN / Inf in b or in the matrix       */
+#define NLHIP_ERR_UNSUPPORTED -6   /* request outside what is implemented   */
+#define NLHIP_ERR_BUDGET      -7   /* exceeds the configured memory budget  */
+#define NLHIP_ERR_INJECTED    -8   /* failure forced by the test hook       */
+
+#define NLHIP_PREC_FP64  0   /* FP64 Jacobi-PCG                              */
+#define NLHIP_PREC_MIXED 1   /* FP32 inner PCG inside FP64 refinement        */
+
+#define NLHIP_SYSTEM_FP32 1u /* also keep an FP32 copy of the values         */
+
+typedef struct nlhip_system_s* nlhip_system;
+
+typedef struct {
+    int32_t precision;        /* NLHIP_PREC_*                                */
+    int32_t precond;          /* NL_PRECOND_JACOBI or NL_PRECOND_NONE        */
+    double eps;               /* stop when |Ax-b| <= eps |b|                 */
+    uint32_t max_iter;        /* iteration cap (FP64 CG / total inner)       */
+    uint32_t check_every;     /* iterations per captured graph (0: auto)     */
+    double inner_eps;         /* mixed: relative tolerance of each inner CG  */
+    uint32_t max_outer;       /* mixed: refinement steps cap                 */
+    uint32_t inner_max_iter;  /* mixed: cap per inner CG (0: max_iter)       */
+} nlhip_solve_params;
+
+typedef struct {
+    uint32_t iterations;      /* CG iterations (mixed: sum of inner)         */
+    uint32_t outer_iterations;/* mixed: refinement steps                     */
+    int32_t converged;        /* 1 when |Ax-b| <= eps |b| was reached        */
+    int32_t status;           /* NLHIP_OK or error code                      */
+    double rel_residual;      /* |b-Ax|/|b| recomputed in FP64 at the end    */
+    double cg_rel_residual;   /* the solver's own recursive residual         */
+    double device_seconds;    /* HIP-event time of the solve on the device   */
+    double wall_seconds;      /* host time including staging                 */
+} nlhip_solve_stats;
+
+NLAPI void NLAPIENTRY nlhipSolveParamsDefault(nlhip_solve_params* p);
+
+/* Copies a CRS matrix (rowptr[n+1], colind[nnz], val[nnz]; host or device
+ * pointers) to the device.  Checks the structure (monotone rowptr, colind <
+ * n) and that the values are finite. */
+NLAPI int NLAPIENTRY nlhipSystemCreate(uint32_t n, const uint32_t* rowptr,
+                                       const uint32_t* colind, const double* val,
+                                       uint32_t flags, nlhip_system* out);
+NLAPI int NLAPIENTRY nlhipSystemDestroy(nlhip_system s);
+NLAPI int NLAPIENTRY nlhipSystemInfo(nlhip_system s, uint32_t* n, uint64_t* nnz,
+                                     uint64_t* device_bytes);
+/* Launch shape: lanes per row (4, 8, 16, 32) and workgroups per launch. */
+NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int lanes_per_row,
+                                          uint32_t workgroups);
+NLAPI int NLAPIENTRY nlhipSystemGetLaunch(nlhip_system s, int* lanes_per_row,
+                                          uint32_t* workgroups);
+/* Solves A x = b; x holds the initial guess.  b, x host or device. */
+NLAPI int NLAPIENTRY nlhipSolve(nlhip_system s, const double* b, double* x,
+                                const nlhip_solve_params* params,
+                                nlhip_solve_stats* stats);
+/* y = A x (FP64), host or device pointers. */
+NLAPI int NLAPIENTRY nlhipSpMV(nlhip_system s, const double* x, double* y);
+/* Times `reps` back-to-back FP64 (fp32 = 0) or FP32 SpMVs on device-resident
+ * vectors; returns the mean seconds per SpMV measured with HIP events. */
+NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
+                                    double* seconds_per_spmv);
+
+/* Runs `iters` PCG iterations (FP64, or FP32 when fp32 = 1) launching each
+ * kernel directly, and returns the mean HIP-event duration of the fused
+ * SpMV step and of the update step. */
+NLAPI int NLAPIENTRY nlhipProfileCG(nlhip_system s, int fp32, uint32_t iters,
+                                    double* spmv_step_seconds,
+                                    double* update_step_seconds);
+
+NLAPI const char* NLAPIENTRY nlhipLastError(void);
+/* Test hook: the next `count` solves fail with NLHIP_ERR_INJECTED before
+ * touching the device (other requests are unaffected). */
+NLAPI void NLAPIENTRY nlhipInjectFailures(int count);
+/* Device name, CU count and HBM bytes of the device in use. */
+NLAPI int NLAPIENTRY nlhipDeviceInfo(char* name, size_t name_len, int* cu_count,
+                                     uint64_t* hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENNL_HIP_NL_HIP_H */

@@ -1,0 +1,143 @@
+"""GPU kernel parity through the C ABI: CRS SpMV, NLBlas level 1, Jacobi.
+
+Integer-valued fixtures make SpMV and the BLAS bit-exact whatever the
+summation order, so every lanes-per-row / workgroup shape must reproduce
+them exactly."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from graphitethree_amd import capi
+from graphitethree_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_spmv_exact_all_launch_shapes(hip):
+    z = np.load(os.path.join(GOLD, "spmv_exact.npz"))
+    for name in z["names"]:
+        rp, ci, va = z[f"{name}_rowptr"], z[f"{name}_colind"], z[f"{name}_val"]
+        S = capi.System(rp, ci, va)
+        for lpr in (4, 8, 16, 32):
+            for wg in (1, 3, 64, 2048):
+                S.set_launch(lpr, wg)
+                y = S.spmv(z[f"{name}_x"])
+                assert np.array_equal(y, z[f"{name}_y"]), (name, lpr, wg)
+        S.close()
+
+
+def test_spmv_lscm_vs_oracle(hip):
+    Sw = W.lscm_grid(60, seed=3)
+    A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+    x = np.random.default_rng(0).standard_normal(Sw.n)
+    ref = O.spmv(A, x)
+    S = capi.System.from_crs(Sw)
+    y = S.spmv(x)
+    scale = np.abs(Sw.to_scipy()) @ np.abs(x)
+    assert np.all(np.abs(y - ref) <= 16 * 2.0 ** -53 * scale + 1e-300)
+
+
+def test_spmv_through_opennl_api(hip):
+    L = hip
+    Sw = W.lscm_grid(20, seed=1)
+    x = np.random.default_rng(1).standard_normal(Sw.n)
+    ref = O.spmv(O.CRS(Sw.rowptr, Sw.colind, Sw.val), x)
+    # dynamic matrix, then compressed CRS, then device matrix
+    M = L.nlSparseMatrixNew(Sw.n, Sw.n, capi.NL_MATRIX_STORE_ROWS)
+    rows = np.repeat(np.arange(Sw.n), np.diff(Sw.rowptr.astype(np.int64)))
+    for i, j, v in zip(rows, Sw.colind, Sw.val):
+        L.nlSparseMatrixAdd(M, int(i), int(j), float(v))
+    y = np.empty(Sw.n)
+    L.nlMultMatrixVector(M, capi.ptr(x), capi.ptr(y))
+    np.testing.assert_allclose(y, ref, rtol=1e-13, atol=1e-13)
+    L.nlMatrixCompress(C.byref(M))
+    y2 = np.empty(Sw.n)
+    L.nlMultMatrixVector(M, capi.ptr(x), capi.ptr(y2))
+    assert np.array_equal(y, y2)
+    H = L.nlHIPMatrixNewFromCRSMatrix(M)
+    blas = L.nlHIPBlas()
+    t = blas.contents
+    Malloc = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_int, C.c_size_t)(t.Malloc)
+    Memcpy = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                         C.c_size_t)(t.Memcpy)
+    Free = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_size_t, C.c_void_p)(t.Free)
+    nb = 8 * Sw.n
+    dx, dy = Malloc(blas, 1, nb), Malloc(blas, 1, nb)
+    Memcpy(blas, dx, 1, x.ctypes.data, 0, nb)
+    L.nlMultMatrixVector(H, C.c_void_p(dx), C.c_void_p(dy))
+    y3 = np.empty(Sw.n)
+    Memcpy(blas, y3.ctypes.data, 0, dy, 1, nb)
+    assert np.array_equal(y, y3)
+    Free(blas, 1, nb, dx)
+    Free(blas, 1, nb, dy)
+    L.nlDeleteMatrix(H)
+    L.nlDeleteMatrix(M)
+
+
+def _blas_fixture():
+    g = json.load(open(os.path.join(GOLD, "blas_exact.json")))
+    return np.array(g["x"]), np.array(g["y"]), g
+
+
+def test_host_blas_exact(hip):
+    x, y, g = _blas_fixture()
+    blas = hip.nlHostBlas()
+    t = blas.contents
+    n = len(x)
+    assert t.Ddot(blas, n, capi.ptr(x), 1, capi.ptr(y), 1) == g["dot"]
+    assert t.Dnrm2(blas, n, capi.ptr(x), 1) == pytest.approx(np.sqrt(g["nrm2_sq"]), rel=2e-16)
+    yy = y.copy()
+    t.Daxpy(blas, n, g["a"], capi.ptr(x), 1, capi.ptr(yy), 1)
+    assert np.array_equal(yy, np.array(g["axpy"]))
+    xx = x.copy()
+    t.Dscal(blas, n, g["a"], capi.ptr(xx), 1)
+    assert np.array_equal(xx, np.array(g["scal"]))
+    zz = np.zeros(n)
+    t.Dcopy(blas, n, capi.ptr(x), 1, capi.ptr(zz), 1)
+    assert np.array_equal(zz, x)
+
+
+def test_hip_blas_on_device_vectors(hip):
+    x, y, g = _blas_fixture()
+    blas = hip.nlHIPBlas()
+    t = blas.contents
+    assert t.has_unique_memory == 0
+    Malloc = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_int, C.c_size_t)(t.Malloc)
+    Memcpy = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                         C.c_size_t)(t.Memcpy)
+    Free = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_size_t, C.c_void_p)(t.Free)
+    n = len(x)
+    nb = 8 * n
+    dx, dy = Malloc(blas, 1, nb), Malloc(blas, 1, nb)
+    Memcpy(blas, dx, 1, x.ctypes.data, 0, nb)
+    Memcpy(blas, dy, 1, y.ctypes.data, 0, nb)
+    assert t.Ddot(blas, n, dx, 1, dy, 1) == g["dot"]
+    t.Daxpy(blas, n, g["a"], dx, 1, dy, 1)
+    out = np.empty(n)
+    Memcpy(blas, out.ctypes.data, 0, dy, 1, nb)
+    assert np.array_equal(out, np.array(g["axpy"]))
+    t.Dscal(blas, n, g["a"], dx, 1)
+    Memcpy(blas, out.ctypes.data, 0, dx, 1, nb)
+    assert np.array_equal(out, np.array(g["scal"]))
+    Free(blas, 1, nb, dx)
+    Free(blas, 1, nb, dy)
+
+
+def test_jacobi_preconditioner_apply(hip):
+    L = hip
+    Sw = W.lscm_grid(15, seed=2)
+    A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+    M = L.nlCRSMatrixNewFromArrays(Sw.n, Sw.n, capi.ptr(Sw.rowptr), capi.ptr(Sw.colind),
+                                   capi.ptr(Sw.val))
+    P = L.nlNewJacobiPreconditioner(M)
+    x = np.random.default_rng(2).standard_normal(Sw.n)
+    y = np.empty(Sw.n)
+    L.nlMultMatrixVector(P, capi.ptr(x), capi.ptr(y))
+    assert np.array_equal(y, O.jacobi(A) * x)
+    L.nlDeleteMatrix(P)
+    L.nlDeleteMatrix(M)

@@ -1,0 +1,233 @@
+"""Pins the CPU oracle (oracle/nl_oracle.c) before it is trusted as a checker.
+
+Each check is chosen so a plausible slip -- a dropped term, a wrong sign, a
+transposed operand, an off-by-one in the row loop -- fails: exact integer
+fixtures, closed-form solutions, the reference's own example systems
+(lib/programming_examples/Lua/linear_algebra.lua), invariants of CG, and
+scipy library routines.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    if not os.path.exists(O._LIB_PATH):
+        O.build()
+
+
+def crs_from_dense(M):
+    A = sp.csr_array(np.asarray(M, dtype=np.float64))
+    A.sort_indices()
+    return O.CRS(A.indptr, A.indices, A.data)
+
+
+def test_spmv_exact_fixtures():
+    z = np.load(os.path.join(GOLD, "spmv_exact.npz"))
+    for name in z["names"]:
+        A = O.CRS(z[f"{name}_rowptr"], z[f"{name}_colind"], z[f"{name}_val"])
+        for par in (False, True):
+            y = O.spmv(A, z[f"{name}_x"], parallel=par)
+            assert np.array_equal(y, z[f"{name}_y"]), name
+
+
+def test_spmv_dense_brute_force_nonsymmetric():
+    rng = np.random.default_rng(3)
+    D = rng.standard_normal((23, 23)) * (rng.random((23, 23)) < 0.3)
+    x = rng.standard_normal(23)
+    y = O.spmv(crs_from_dense(D), x)
+    ref = np.array([math.fsum(D[i, j] * x[j] for j in range(23)) for i in range(23)])
+    np.testing.assert_allclose(y, ref, rtol=1e-14, atol=1e-14)
+    # a transposed operand would be caught: D is not symmetric
+    assert np.abs(D.T @ x - ref).max() > 1e-3
+
+
+def test_blas_exact():
+    g = json.load(open(os.path.join(GOLD, "blas_exact.json")))
+    x, y, a = np.array(g["x"]), np.array(g["y"]), g["a"]
+    assert O.ddot(x, y) == g["dot"]
+    assert O.ddot(x, y, parallel=True) == g["dot"]
+    assert O.dnrm2(x) == math.sqrt(g["nrm2_sq"])
+    yy = y.copy()
+    O.daxpy(a, x, yy)
+    assert np.array_equal(yy, np.array(g["axpy"]))
+    xx = x.copy()
+    O.dscal(a, xx)
+    assert np.array_equal(xx, np.array(g["scal"]))
+    zz = np.empty_like(x)
+    O.dcopy(x, zz)
+    assert np.array_equal(zz, x)
+
+
+def test_dot_error_bound():
+    rng = np.random.default_rng(11)
+    n = 100000
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    exact = math.fsum(x * y)  # products rounded once; bound covers that too
+    gamma = n * 2.0 ** -53 / (1 - n * 2.0 ** -53)
+    bound = gamma * float(np.sum(np.abs(x * y))) + n * 2.0 ** -53 * float(np.sum(np.abs(x * y)))
+    assert abs(O.ddot(x, y) - exact) <= bound
+
+
+def test_jacobi_closed_form_and_zero_diagonal():
+    A = crs_from_dense([[4.0, 1.0, 0.0], [1.0, 0.0, 2.0], [0.0, 2.0, -8.0]])
+    d = O.jacobi(A)
+    assert np.array_equal(d, np.array([0.25, 1.0, -0.125]))
+
+
+def test_lua_kat_symmetric_cg_jacobi():
+    k = json.load(open(os.path.join(GOLD, "lua_kats.json")))["symmetric"]
+    A = crs_from_dense(k["M"])
+    # solve_symmetric: CG + Jacobi, max_iter = 5 n, eps = 1e-3 (matrix.cpp:439)
+    x, it, rel = O.pcg(A, k["B"], eps=1e-3, max_iter=10)
+    np.testing.assert_allclose(x, k["X"], rtol=0, atol=1e-12)
+    assert it == 2
+
+
+def test_lua_kat_bicgstab():
+    k = json.load(open(os.path.join(GOLD, "lua_kats.json")))["general"]
+    A = crs_from_dense(k["M"])
+    x, it, rel = O.bicgstab(A, k["B"], eps=k["eps"], max_iter=k["max_iter"])
+    np.testing.assert_allclose(x, k["X"], rtol=0, atol=1e-3 * np.linalg.norm(k["X"]))
+    assert rel <= 1e-3
+
+
+def tridiag(n):
+    main = 2.0 * np.ones(n)
+    off = -1.0 * np.ones(n - 1)
+    return sp.diags([off, main, off], [-1, 0, 1], format="csr")
+
+
+def test_pcg_tridiagonal_closed_form():
+    n = 200
+    A = tridiag(n)
+    i = np.arange(1, n + 1)
+    x_true = np.sin(np.pi * i / (n + 1))  # eigenvector: A x = lambda x
+    lam = 2 - 2 * np.cos(np.pi / (n + 1))
+    b = lam * x_true
+    C = O.CRS(A.indptr, A.indices, A.data)
+    x, it, rel = O.pcg(C, b, eps=1e-9, max_iter=1000)
+    # b is an eigenvector of the (Jacobi-scaled) operator: one step leaves
+    # only a rounding-level residual (about u * cond = 2e-12 here)
+    assert it == 1
+    np.testing.assert_allclose(x, x_true, rtol=0, atol=1e-10)
+
+
+def test_pcg_finite_termination_distinct_eigenvalues():
+    vals = np.array([1.0, 2.0, 3.0, 5.0, 7.0, 11.0, 13.0])
+    A = sp.diags(vals, 0, format="csr")
+    C = O.CRS(A.indptr, A.indices, A.data)
+    b = np.ones(7)
+    x, it, rel = O.pcg(C, b, eps=1e-14, max_iter=100, precond=None)
+    assert it <= 7
+    np.testing.assert_allclose(x, 1 / vals, rtol=1e-13)
+
+
+def test_pcg_zero_rhs_and_max_iter():
+    A = tridiag(50)
+    C = O.CRS(A.indptr, A.indices, A.data)
+    x, it, rel = O.pcg(C, np.zeros(50))
+    assert it == 0 and not x.any()
+    b = np.arange(50, dtype=float)
+    x, it, rel = O.pcg(C, b, eps=1e-14, max_iter=3)
+    assert it == 3 and rel > 1e-3
+
+
+def test_pcg_against_scipy_direct_lscm():
+    z = np.load(os.path.join(GOLD, "lscm_k12.npz"))
+    A = O.CRS(z["rowptr"], z["colind"], z["val"])
+    x, it, rel = O.pcg(A, z["b"], eps=1e-12, max_iter=10000)
+    assert rel <= 1e-12
+    assert np.linalg.norm(x - z["x"]) / np.linalg.norm(z["x"]) < 1e-8
+
+
+def test_pcg_variants_agree():
+    z = np.load(os.path.join(GOLD, "lscm_k12.npz"))
+    A = O.CRS(z["rowptr"], z["colind"], z["val"])
+    x0, i0, _ = O.pcg(A, z["b"], variant=0)
+    x1, i1, _ = O.pcg(A, z["b"], variant=1)
+    assert i0 == i1 and np.array_equal(x0, x1)  # same per-row arithmetic
+    x2, i2, _ = O.pcg(A, z["b"], variant=2)
+    assert abs(i2 - i0) <= 2
+    assert np.linalg.norm(x2 - x0) / np.linalg.norm(x0) < 1e-6
+
+
+def test_planted_sign_error_is_caught():
+    """A PCG with the wrong sign on the direction update (d = beta d - h)
+    fails the tridiagonal closed form that the oracle passes."""
+    n = 60
+    A = tridiag(n).toarray()
+    b = np.ones(n)
+    x = np.zeros(n); r = A @ x - b; h = r / 2.0; d = h.copy(); rh = r @ h
+    for _ in range(200):
+        Ad = A @ d; al = rh / (d @ Ad); x -= al * d; r -= al * Ad; h = r / 2.0
+        rn = r @ h; be = rn / rh; rh = rn; d = be * d - h   # planted mistake
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) > 1e-3
+    C = crs_from_dense(A)
+    xo, it, rel = O.pcg(C, b, eps=1e-12, max_iter=200)
+    assert rel <= 1e-12
+
+
+def test_dynamic_matrix_accumulates_like_coo():
+    rng = np.random.default_rng(5)
+    I = rng.integers(0, 30, 500); J = rng.integers(0, 40, 500); V = rng.standard_normal(500)
+    D = O.Dyn(30, 40)
+    for i, j, v in zip(I, J, V):
+        D.add(int(i), int(j), float(v))
+    C = D.compress()
+    ref = sp.coo_array((V, (I, J)), shape=(30, 40)).tocsr()
+    ref.sum_duplicates(); ref.sort_indices()
+    assert np.array_equal(C.rowptr, ref.indptr)
+    assert np.array_equal(C.colind, ref.indices)
+    np.testing.assert_allclose(C.val, ref.data, rtol=1e-13, atol=1e-13)
+
+
+def test_context_least_squares_normal_equations():
+    rng = np.random.default_rng(9)
+    nv, nrows = 8, 20
+    A = rng.standard_normal((nrows, nv)) * (rng.random((nrows, nv)) < 0.5)
+    rhs = rng.standard_normal(nrows)
+    locked = {2: 1.5, 5: -0.25}
+    ctx = O.Context(nv, least_squares=True)
+    for v, val in locked.items():
+        ctx.set_variable(v, val)
+        ctx.lock_variable(v)
+    ctx.begin_matrix()
+    for r in range(nrows):
+        ctx.row([(j, A[r, j]) for j in range(nv) if A[r, j] != 0], rhs[r])
+    M, b = ctx.system()
+    free = [v for v in range(nv) if v not in locked]
+    Af, Al = A[:, free], A[:, list(locked)]
+    xl = np.array(list(locked.values()))
+    Mref = Af.T @ Af
+    bref = Af.T @ (rhs - Al @ xl)
+    Md = M.to_scipy().toarray()
+    np.testing.assert_allclose(Md, Mref, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(b, bref, rtol=1e-12, atol=1e-12)
+    it, rel = ctx.solve(eps=1e-13, max_iter=1000)
+    xs = np.array([ctx.get_variable(v) for v in free])
+    np.testing.assert_allclose(xs, np.linalg.lstsq(Af, rhs - Al @ xl, rcond=None)[0],
+                               rtol=1e-8, atol=1e-9)
+    assert ctx.get_variable(2) == 1.5
+
+
+def test_context_square_system():
+    M = np.array([[4.0, 1.0, 0.0], [1.0, 3.0, 1.0], [0.0, 1.0, 2.0]])
+    ctx = O.Context(3, least_squares=False)
+    ctx.begin_matrix()
+    b = np.array([1.0, 2.0, 3.0])
+    for i in range(3):
+        ctx.row([(j, M[i, j]) for j in range(3) if M[i, j] != 0], b[i])
+    it, rel = ctx.solve(eps=1e-14, max_iter=100)
+    x = np.array([ctx.get_variable(i) for i in range(3)])
+    np.testing.assert_allclose(x, np.linalg.solve(M, b), rtol=1e-12)
