@@ -35,13 +35,12 @@ def spmv_bytes(n: int, nnz: int, sv: int = 8, sx: int = 8) -> float:
     return nnz * (sv + 4) + (n + 1) * 4 + n * sx + n * sx
 
 
-def cg_spmv_step_bytes(n: int, nnz: int, s: int = 8) -> float:
-    """Algorithmic bytes of the fused spmv step of one PCG iteration: values,
-    colind, rowptr; h and d_prev read once; d and q written."""
-    return nnz * (s + 4) + (n + 1) * 4 + 2 * n * s + 2 * n * s
+def cg_direction_bytes(n: int, s: int = 8) -> float:
+    """d = beta d + h: d and h read, d written."""
+    return 3 * n * s
 
 
-def cg_update_step_bytes(n: int, s: int = 8) -> float:
+def cg_update_bytes(n: int, s: int = 8) -> float:
     """x, d, r, q, dinv read; x, r, h written."""
     return 8 * n * s
 
@@ -82,7 +81,7 @@ def main():
     t0 = time.time()
     S = capi.System.from_crs(Sw, fp32=not args.no_mixed)
     lpr, wg = S.launch()
-    log(f"uploaded in {time.time() - t0:.1f}s; launch lanes/row={lpr} workgroups={wg}")
+    log(f"uploaded in {time.time() - t0:.1f}s; launch rows/wavefront={lpr} workgroups={wg}")
 
     # ---- standalone SpMV bandwidth (FP64, and FP32 values)
     t_spmv = S.bench_spmv(fp32=False, reps=50)
@@ -115,14 +114,14 @@ def main():
     solves_per_s = 1e3 / ms_per_step
     iters = int(stats.iterations)
 
-    # ---- dominant kernel timing (HIP events on the system's stream)
-    t_a, t_b = capi.profile_cg(S, iters=50)
-    bytes_a = cg_spmv_step_bytes(n, nnz)
-    bytes_b = cg_update_step_bytes(n)
+    # ---- per-kernel timing of one PCG iteration (HIP events, system stream)
+    t_dir, t_a, t_upd = capi.profile_cg(S, iters=50)
+    bytes_a = spmv_bytes(n, nnz)          # q = A d (+ d.q from the same d)
     achieved = bytes_a / t_a / 1e9
-    log(f"cg spmv step {t_a * 1e3:.3f} ms ({achieved:.0f} GB/s), update step "
-        f"{t_b * 1e3:.3f} ms ({bytes_b / t_b / 1e9:.0f} GB/s)")
-    iter_bytes = bytes_a + bytes_b
+    log(f"cg direction {t_dir * 1e3:.3f} ms ({cg_direction_bytes(n) / t_dir / 1e9:.0f} GB/s), "
+        f"spmv {t_a * 1e3:.3f} ms ({achieved:.0f} GB/s), update {t_upd * 1e3:.3f} ms "
+        f"({cg_update_bytes(n) / t_upd / 1e9:.0f} GB/s)")
+    iter_bytes = bytes_a + cg_direction_bytes(n) + cg_update_bytes(n)
     iter_gbs = iters * iter_bytes / (ms_per_step * 1e-3) / 1e9 if iters else 0.0
 
     result_extra = {}
@@ -193,7 +192,7 @@ def main():
         "config": {"workload": f"lscm_grid_{args.k}", "n": n, "nnz": nnz,
                    "solver": "Jacobi-PCG fp64", "eps": args.eps, "max_iter": args.max_iter,
                    "iterations": iters, "converged": bool(stats.converged),
-                   "rel_residual": stats.rel_residual, "lanes_per_row": lpr,
+                   "rel_residual": stats.rel_residual, "rows_per_wave": lpr,
                    "workgroups": wg},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "spmv_gbs": {"fp64": spmv_gbs, "fp32": spmv32_gbs,
@@ -203,6 +202,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": bytes_a,
                      "launch_ms": t_a * 1e3},
+        "kernels_ms": {"k_cg_dir": t_dir * 1e3, "k_cg_spmv": t_a * 1e3,
+                       "k_cg_update": t_upd * 1e3},
         "cpu_baseline": cpu,
         "device": dev_name,
         **result_extra,

@@ -155,7 +155,7 @@ SIGNATURES = {
                              C.POINTER(SolveStats)]),
     "nlhipSpMV": (C.c_int, [_vp, _vp, _vp]),
     "nlhipBenchSpMV": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
-    "nlhipProfileCG": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p, _f64p]),
+    "nlhipProfileCG": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
     "nlhipLastError": (C.c_char_p, []),
     "nlhipInjectFailures": (None, [C.c_int]),
     "nlhipDeviceInfo": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int),
@@ -246,12 +246,13 @@ class System:
     __del__ = close
 
     def launch(self):
-        lpr, wg = C.c_int(), C.c_uint32()
-        check(lib().nlhipSystemGetLaunch(self._h, C.byref(lpr), C.byref(wg)), "get launch")
-        return int(lpr.value), int(wg.value)
+        """(rows per wavefront, workgroups) of the SpMV engine."""
+        rpw, wg = C.c_int(), C.c_uint32()
+        check(lib().nlhipSystemGetLaunch(self._h, C.byref(rpw), C.byref(wg)), "get launch")
+        return int(rpw.value), int(wg.value)
 
-    def set_launch(self, lanes_per_row: int, workgroups: int):
-        check(lib().nlhipSystemSetLaunch(self._h, lanes_per_row, workgroups), "set launch")
+    def set_launch(self, rows_per_wave: int, workgroups: int):
+        check(lib().nlhipSystemSetLaunch(self._h, rows_per_wave, workgroups), "set launch")
 
     def info(self):
         n, nnz, b = C.c_uint32(), C.c_uint64(), C.c_uint64()
@@ -280,11 +281,10 @@ class System:
 
 
 def profile_cg(system: "System", iters: int = 50, fp32: bool = False):
-    """Mean (spmv step, update step) kernel seconds over `iters` iterations."""
-    a, b = C.c_double(), C.c_double()
-    check(lib().nlhipProfileCG(system.handle, 1 if fp32 else 0, iters, C.byref(a),
-                               C.byref(b)), "nlhipProfileCG")
-    return float(a.value), float(b.value)
+    """Mean (direction, spmv, update) kernel seconds over `iters` iterations."""
+    t = (C.c_double * 3)()
+    check(lib().nlhipProfileCG(system.handle, 1 if fp32 else 0, iters, t), "nlhipProfileCG")
+    return float(t[0]), float(t[1]), float(t[2])
 
 
 def device_info():

@@ -91,4 +91,209 @@ __device__ __forceinline__ void block_row_range(uint32_t m, uint32_t chunk, uint
     r1 = e < m ? uint32_t(e) : m;
 }
 
+// ---- SpMV engines -----------------------------------------------------
+//
+// Both engines walk the rows owned by one workgroup and hand every row sum
+// to an epilogue epi(row, sum), called by exactly one lane per row.
+//
+// Vector engine (RPW < 64 rows per wavefront): LPR = 64/RPW lanes per row,
+// rows [L*sp, L*sp + sp) of the matrix for logical block L, partial products
+// combined by a fixed xor butterfly.
+//
+// Stream engine (RPW = 64): one row per lane.  A wavefront owns a tile of 64
+// consecutive rows, whose nonzeros form one contiguous range; it streams that
+// range in chunks of 64*E entries, each lane loading E independent (val,
+// colind) pairs and gathering x, stages the products in LDS and then lane l
+// sums row l's products in column order.  Block L owns tiles [L*sp, L*sp+sp)
+// (wavefront w of the block takes every 4th of them).  Each row sum is
+// sum_j fl(a_ij x_j) accumulated left to right with one rounding per add.
+
+template <int W>
+struct VectorEngine {
+    static constexpr int LPR = 64 / W;
+    template <typename T, typename Epi>
+    __device__ __forceinline__ static void run(uint32_t m, uint32_t sp, uint32_t L,
+                                               const uint32_t* __restrict__ rp,
+                                               const uint32_t* __restrict__ ci,
+                                               const T* __restrict__ val,
+                                               const T* __restrict__ x, T*, Epi& epi) {
+        uint32_t r0, r1;
+        block_row_range(m, sp, L, r0, r1);
+        const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+        for (uint32_t row = r0 + g; row < r1; row += kBlock / LPR) {
+            const T s = row_product<LPR>(row, rp, ci, val, x, lane);
+            if (lane == 0) epi(row, s);
+        }
+    }
+};
+
+template <int E, bool NT>
+struct StreamEngine {
+    template <typename T>
+    __device__ __forceinline__ static T ld(const T* p) {
+        if constexpr (NT) return ntload(p);
+        else return *p;
+    }
+
+    template <typename T>
+    __device__ __forceinline__ static void load_chunk(const uint32_t* __restrict__ ci,
+                                                      const T* __restrict__ val, uint32_t base,
+                                                      uint32_t J1, int lane, uint32_t (&c)[E],
+                                                      T (&a)[E]) {
+        // column indices first: the gathers wait on them, not on the values
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
+            c[e] = ld(ci + (j < J1 ? j : J1 - 1u));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
+            a[e] = ld(val + (j < J1 ? j : J1 - 1u));
+        }
+    }
+
+    // Rows [R0, R1) of one wavefront: lane l owns row r0 + l of the current
+    // 64-row tile; the wavefront's nonzeros [rp[R0], rp[R1]) are streamed as
+    // one contiguous range in chunks of 64*E entries, the next chunk's loads
+    // issued before the current chunk is summed.
+    template <typename T, typename Epi>
+    __device__ __forceinline__ static void run_rows(uint32_t R0, uint32_t R1,
+                                                    const uint32_t* __restrict__ rp,
+                                                    const uint32_t* __restrict__ ci,
+                                                    const T* __restrict__ val,
+                                                    const T* __restrict__ x, T* __restrict__ st,
+                                                    int lane, Epi& epi) {
+        const uint32_t J1 = rp[R1];
+        uint32_t r0 = R0;
+        uint32_t nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
+        uint32_t tile_end = rp[r0 + nrows];
+        uint32_t rb = (uint32_t(lane) < nrows) ? rp[r0 + uint32_t(lane)] : tile_end;
+        uint32_t re = __shfl_down(rb, 1, kWave);
+        if (uint32_t(lane) + 1 >= nrows) re = tile_end;
+        uint32_t base = __shfl(rb, 0, kWave);
+        T acc = T(0);
+        uint32_t c[E];
+        T a[E];
+        if (base < J1) load_chunk(ci, val, base, J1, lane, c, a);
+        while (base < J1) {
+            const uint32_t craw = base + 64u * E;
+            const uint32_t cend = craw < J1 ? craw : J1;
+            T p[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) p[e] = x[c[e]];
+            // keep every gather ahead of the prefetch in the instruction stream:
+            // loads return in order, so a gather issued late would make its
+            // wait drain the prefetch too
+            __builtin_amdgcn_sched_barrier(0);
+            // next chunk, issued unconditionally (clamped to the last entry)
+            // so that its loads overlap this chunk's products and row sums
+            uint32_t cn[E];
+            T an[E];
+            load_chunk(ci, val, craw, J1, lane, cn, an);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
+                st[e * 64 + lane] = (j < J1) ? mul_rn(a[e], p[e]) : T(0);
+            }
+            __builtin_amdgcn_wave_barrier();
+            // consume this chunk; it may complete several tiles
+            for (;;) {
+                const uint32_t lo = rb > base ? rb : base;
+                const uint32_t hi = re < cend ? re : cend;
+                uint32_t j = lo;
+                for (; j + 4 <= hi; j += 4) {
+                    const T v0 = st[j - base], v1 = st[j + 1 - base];
+                    const T v2 = st[j + 2 - base], v3 = st[j + 3 - base];
+                    acc = add_rn(add_rn(add_rn(add_rn(acc, v0), v1), v2), v3);
+                }
+                for (; j < hi; ++j) acc = add_rn(acc, st[j - base]);
+                if (tile_end > cend) break;
+                if (uint32_t(lane) < nrows) epi(r0 + uint32_t(lane), acc);
+                acc = T(0);
+                r0 += 64u;
+                if (r0 >= R1) return;
+                nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
+                tile_end = rp[r0 + nrows];
+                rb = (uint32_t(lane) < nrows) ? rp[r0 + uint32_t(lane)] : tile_end;
+                re = __shfl_down(rb, 1, kWave);
+                if (uint32_t(lane) + 1 >= nrows) re = tile_end;
+            }
+            __builtin_amdgcn_wave_barrier();
+            base = craw;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                c[e] = cn[e];
+                a[e] = an[e];
+            }
+        }
+        // tiles left without nonzeros (all their rows are empty)
+        for (;;) {
+            if (uint32_t(lane) < nrows) epi(r0 + uint32_t(lane), acc);
+            acc = T(0);
+            r0 += 64u;
+            if (r0 >= R1) return;
+            nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
+        }
+    }
+
+    // Block L owns 64-row tiles [L*sp, L*sp + sp); each of its wavefronts
+    // takes one contiguous quarter of them.
+    template <typename T, typename Epi>
+    __device__ __forceinline__ static void run(uint32_t m, uint32_t sp, uint32_t L,
+                                               const uint32_t* __restrict__ rp,
+                                               const uint32_t* __restrict__ ci,
+                                               const T* __restrict__ val,
+                                               const T* __restrict__ x, T* stage, Epi& epi) {
+        const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+        const uint64_t ntiles = (uint64_t(m) + 63) / 64;
+        uint64_t tb = uint64_t(L) * sp;
+        if (tb > ntiles) tb = ntiles;
+        const uint64_t te = tb + sp < ntiles ? tb + sp : ntiles;
+        const uint64_t q = (te - tb + kWavesPerBlock - 1) / kWavesPerBlock;
+        uint64_t t0 = tb + uint64_t(w) * q;
+        if (t0 > te) t0 = te;
+        const uint64_t t1 = t0 + q < te ? t0 + q : te;
+        if (t0 >= t1) return;
+        const uint32_t R0 = uint32_t(t0 * 64);
+        const uint32_t R1 = uint32_t(t1 * 64 < m ? t1 * 64 : m);
+        run_rows(R0, R1, rp, ci, val, x, stage + w * (64 * E), lane, epi);
+    }
+};
+
+// Engine for a rows-per-wavefront value, and its LDS staging size.
+constexpr int kStreamE = 8;
+
+template <int RPW>
+struct EngineOf {
+    using type = VectorEngine<RPW>;
+};
+template <>
+struct EngineOf<64> {
+    using type = StreamEngine<kStreamE, true>;
+};
+// Variants compared by the launch-shape sweep (not selectable from the ABI).
+template <>
+struct EngineOf<65> {
+    using type = StreamEngine<4, true>;
+};
+template <>
+struct EngineOf<66> {
+    using type = StreamEngine<8, false>;
+};
+template <>
+struct EngineOf<67> {
+    using type = StreamEngine<16, true>;
+};
+
+template <int RPW>
+struct StageSize {
+    static constexpr int value = RPW == 64   ? kWavesPerBlock * 64 * kStreamE
+                                 : RPW == 65 ? kWavesPerBlock * 64 * 4
+                                 : RPW == 66 ? kWavesPerBlock * 64 * 8
+                                 : RPW == 67 ? kWavesPerBlock * 64 * 16
+                                             : 1;
+};
+
 }  // namespace nlhip

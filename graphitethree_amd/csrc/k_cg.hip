@@ -10,17 +10,18 @@
 //     rho' = r.h; beta = rho'/rho; rho = rho'; d = beta d + h; err = r.r
 // and the stopping rule |Ax-b|/|b| < eps of matrix.h:177-179.
 //
-// One iteration is two launches and no host round trip:
-//   spmv step   reduce r.h and r.r partials -> rho, err; stopping test;
-//               beta = rho/rho_prev; q = A d with d = beta d_prev + h formed
-//               on the fly in the gather (d is double-buffered); write d, q;
-//               partials of d.q.
-//   update step reduce d.q partials -> alpha; x -= alpha d; r -= alpha q;
-//               h = P r; partials of r.h and r.r.
-// Every block reduces the partials of the previous launch itself, in a fixed
-// order, so all blocks hold bitwise-identical scalars.  After the stopping
-// test holds, later launches exit at once, so a host can enqueue (or replay
-// a captured graph of) many iterations and read the status afterwards.
+// One iteration is three launches and no host round trip:
+//   direction  reduce the r.h and r.r partials -> rho, err; stopping test;
+//              beta = rho / rho_prev; d = beta d + h (d = h at it 0)
+//   spmv       q = A d and partials of d.q
+//   update     reduce the d.q partials -> alpha; x -= alpha d;
+//              r -= alpha q; h = P r; partials of r.h and r.r
+// Per iteration that moves 12 nnz + 108 n bytes in FP64 (values, colind,
+// rowptr, d gathered once, and the vector streams).  Every block reduces the
+// partials of the previous launch itself, in a fixed order, so all blocks
+// hold bitwise-identical scalars.  Once the stopping test holds, later
+// launches exit at once, so the host can replay a captured graph of many
+// iterations and read the status afterwards.
 #include "dev_common.h"
 #include "nlhip_kernels.h"
 
@@ -28,35 +29,29 @@ namespace nlhip {
 
 template <typename T>
 __device__ __forceinline__ bool block_done(const CGStatus<T>* st, int* s_flag) {
-    if (threadIdx.x == 0) *s_flag = __hip_atomic_load(&st->done, __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0)
+        *s_flag = __hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     return *s_flag != 0;
 }
 
-template <int LPR, typename T, bool PREC>
-__global__ __launch_bounds__(kBlock) void k_cg_init(CGArgs<T> a, int x_zero) {
+template <int RPW, typename T, bool PREC>
+__global__ __launch_bounds__(kBlock) void k_cg_init(CGArgs<T> a) {
+    __shared__ T stage[StageSize<RPW>::value];
     __shared__ T lds[kWavesPerBlock];
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
-    uint32_t r0, r1;
-    block_row_range(a.m, a.chunk, L, r0, r1);
-    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
     T prh = T(0), prr = T(0), pbb = T(0);
-    for (uint32_t row = r0 + g; row < r1; row += kBlock / LPR) {
-        T s = T(0);
-        if (!x_zero) s = row_product<LPR>(row, a.rp, a.ci, a.val, a.x, lane);
-        if (lane == 0) {
-            const T bi = a.b[row];
-            const T ri = add_rn(s, mul_rn(T(-1), bi));
-            const T hi = PREC ? mul_rn(a.dinv[row], ri) : ri;
-            a.r[row] = ri;
-            a.h[row] = hi;
-            a.d0[row] = T(0);
-            prh += ri * hi;
-            prr += ri * ri;
-            pbb += bi * bi;
-        }
-    }
+    auto epi = [&](uint32_t row, T s) {
+        const T bi = a.b[row];
+        const T ri = add_rn(s, mul_rn(T(-1), bi));
+        const T hi = PREC ? mul_rn(a.dinv[row], ri) : ri;
+        a.r[row] = ri;
+        a.h[row] = hi;
+        prh += ri * hi;
+        prr += ri * ri;
+        pbb += bi * bi;
+    };
+    EngineOf<RPW>::type::run(a.m, a.sp, L, a.rp, a.ci, a.val, a.x, stage, epi);
     prh = block_sum_all(prh, lds);
     prr = block_sum_all(prr, lds);
     pbb = block_sum_all(pbb, lds);
@@ -88,8 +83,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_finish(CGArgs<T> a, T eps, i
     }
 }
 
-template <int LPR, typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(CGArgs<T> a) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_dir(CGArgs<T> a) {
     __shared__ T lds[kWavesPerBlock];
     __shared__ int s_flag;
     CGStatus<T>* st = a.st;
@@ -113,30 +108,31 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(CGArgs<T> a) {
         st->err = err;
         st->iter_b = it;
     }
-    const T* __restrict__ dprev = (it & 1) ? a.d1 : a.d0;
-    T* __restrict__ dnew = (it & 1) ? a.d0 : a.d1;
-    const T* __restrict__ h = a.h;
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     uint32_t r0, r1;
     block_row_range(a.m, a.chunk, L, r0, r1);
-    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
-    T pdq = T(0);
-    for (uint32_t row = r0 + g; row < r1; row += kBlock / LPR) {
-        const uint32_t jb = a.rp[row], je = a.rp[row + 1];
-        T acc = T(0);
-        for (uint32_t j = jb + lane; j < je; j += LPR) {
-            const uint32_t c = ntload(a.ci + j);
-            const T dj = add_rn(mul_rn(beta, dprev[c]), h[c]);
-            acc += ntload(a.val + j) * dj;
-        }
-        acc = group_sum<LPR>(acc);
-        if (lane == 0) {
-            const T di = add_rn(mul_rn(beta, dprev[row]), h[row]);
-            dnew[row] = di;
-            a.q[row] = acc;
-            pdq += di * acc;
-        }
+    if (it == 0) {
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) a.d[i] = a.h[i];
+    } else {
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock)
+            a.d[i] = add_rn(mul_rn(beta, a.d[i]), a.h[i]);
     }
+}
+
+template <int RPW, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(CGArgs<T> a) {
+    __shared__ T stage[StageSize<RPW>::value];
+    __shared__ T lds[kWavesPerBlock];
+    __shared__ int s_flag;
+    if (block_done(a.st, &s_flag)) return;
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    const T* __restrict__ d = a.d;
+    T pdq = T(0);
+    auto epi = [&](uint32_t row, T s) {
+        a.q[row] = s;
+        pdq += d[row] * s;
+    };
+    EngineOf<RPW>::type::run(a.m, a.sp, L, a.rp, a.ci, a.val, d, stage, epi);
     pdq = block_sum_all(pdq, lds);
     if (threadIdx.x == 0) a.part_dq[L] = pdq;
 }
@@ -151,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CGArgs<T> a) {
     const T dq = reduce_partials(a.part_dq, a.P, lds);
     const T rho = st->rho;
     const T alpha = rho / dq;
-    const T* __restrict__ d = (it & 1) ? a.d0 : a.d1;
+    const T* __restrict__ d = a.d;
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     uint32_t r0, r1;
     block_row_range(a.m, a.chunk, L, r0, r1);
@@ -200,31 +196,24 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(uint64_t n, double* __rest
 // ------------------------------------------------------------- launchers
 
 template <typename T>
-hipError_t cg_start(const CGArgs<T>& a, int lpr, bool precond, bool x_zero, T eps,
-                    int max_iter, hipStream_t s) {
+hipError_t cg_start(const CGArgs<T>& a, int rpw, bool precond, T eps, int max_iter,
+                    hipStream_t s) {
     if (a.m == 0 || a.P == 0) return hipErrorInvalidValue;
-    const int xz = x_zero ? 1 : 0;
     if (precond) {
-        NLHIP_LPR_SWITCH(lpr, (k_cg_init<LPR, T, true><<<a.P, kBlock, 0, s>>>(a, xz)));
+        NLHIP_RPW_SWITCH(rpw, (k_cg_init<RPW, T, true><<<a.P, kBlock, 0, s>>>(a)));
     } else {
-        NLHIP_LPR_SWITCH(lpr, (k_cg_init<LPR, T, false><<<a.P, kBlock, 0, s>>>(a, xz)));
+        NLHIP_RPW_SWITCH(rpw, (k_cg_init<RPW, T, false><<<a.P, kBlock, 0, s>>>(a)));
     }
     k_cg_init_finish<T><<<1, kBlock, 0, s>>>(a, eps, max_iter);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t cg_step(const CGArgs<T>& a, int lpr, bool precond, hipStream_t s) {
-    NLHIP_LPR_SWITCH(lpr, (k_cg_spmv<LPR, T><<<a.P, kBlock, 0, s>>>(a)));
-    if (precond) k_cg_update<T, true><<<a.P, kBlock, 0, s>>>(a);
-    else k_cg_update<T, false><<<a.P, kBlock, 0, s>>>(a);
-    return hipGetLastError();
-}
-
-template <typename T>
-hipError_t cg_step_part(const CGArgs<T>& a, int lpr, bool precond, int part, hipStream_t s) {
+hipError_t cg_step_part(const CGArgs<T>& a, int rpw, bool precond, int part, hipStream_t s) {
     if (part == 0) {
-        NLHIP_LPR_SWITCH(lpr, (k_cg_spmv<LPR, T><<<a.P, kBlock, 0, s>>>(a)));
+        k_cg_dir<T><<<a.P, kBlock, 0, s>>>(a);
+    } else if (part == 1) {
+        NLHIP_RPW_SWITCH(rpw, (k_cg_spmv<RPW, T><<<a.P, kBlock, 0, s>>>(a)));
     } else if (precond) {
         k_cg_update<T, true><<<a.P, kBlock, 0, s>>>(a);
     } else {
@@ -232,15 +221,22 @@ hipError_t cg_step_part(const CGArgs<T>& a, int lpr, bool precond, int part, hip
     }
     return hipGetLastError();
 }
-template hipError_t cg_step_part<double>(const CGArgs<double>&, int, bool, int, hipStream_t);
-template hipError_t cg_step_part<float>(const CGArgs<float>&, int, bool, int, hipStream_t);
 
-template hipError_t cg_start<double>(const CGArgs<double>&, int, bool, bool, double, int,
-                                     hipStream_t);
-template hipError_t cg_start<float>(const CGArgs<float>&, int, bool, bool, float, int,
-                                    hipStream_t);
+template <typename T>
+hipError_t cg_step(const CGArgs<T>& a, int rpw, bool precond, hipStream_t s) {
+    for (int part = 0; part < 3; ++part) {
+        hipError_t e = cg_step_part<T>(a, rpw, precond, part, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template hipError_t cg_start<double>(const CGArgs<double>&, int, bool, double, int, hipStream_t);
+template hipError_t cg_start<float>(const CGArgs<float>&, int, bool, float, int, hipStream_t);
 template hipError_t cg_step<double>(const CGArgs<double>&, int, bool, hipStream_t);
 template hipError_t cg_step<float>(const CGArgs<float>&, int, bool, hipStream_t);
+template hipError_t cg_step_part<double>(const CGArgs<double>&, int, bool, int, hipStream_t);
+template hipError_t cg_step_part<float>(const CGArgs<float>&, int, bool, int, hipStream_t);
 
 static uint32_t ew_grid(uint64_t n) {
     uint64_t g = (n + kBlock - 1) / kBlock;

@@ -1,12 +1,13 @@
 // CRS sparse matrix-vector kernels for gfx950.
 //
 // Layout: rowptr[m+1] (u32), colind[nnz] (u32), val[nnz] (f64 or f32).
-// Each row is handled by a group of LPR lanes (LPR in {4,8,16,32}); a
-// 256-thread workgroup handles 256/LPR rows at a time and walks one
-// contiguous range of rows (XCD-aware: see xcd_logical_block).  The val and
-// colind streams are read once with non-temporal loads; x is gathered through
-// L2.  The group's partial products are combined by a fixed xor butterfly,
-// so every result is deterministic.
+// Two engines (dev_common.h), selected by rows per wavefront (RPW):
+//   RPW in {2,4,8,16}: 64/RPW lanes per row, xor-butterfly row sums;
+//   RPW = 64: one row per lane, nonzeros streamed 8 per lane per chunk with
+//   non-temporal loads, products staged in LDS, rows summed left to right.
+// Workgroups own contiguous row ranges, remapped so each XCD owns one
+// contiguous slice of the matrix (xcd_logical_block): the x gathers of
+// neighbouring rows then hit the same L2.
 //
 // Follows nlMultMatrixVector on CRS matrices (y_i = sum_j a_ij x_j),
 // called at src/lib/OGF/scene_graph/NL/matrix.cpp:303-305.
@@ -15,46 +16,38 @@
 
 namespace nlhip {
 
-template <int LPR, typename T>
-__global__ __launch_bounds__(kBlock) void k_spmv(uint32_t m, uint32_t chunk,
+template <int RPW, typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv(uint32_t m, uint32_t sp,
                                                  const uint32_t* __restrict__ rp,
                                                  const uint32_t* __restrict__ ci,
                                                  const T* __restrict__ val,
-                                                 const T* __restrict__ x,
-                                                 T* __restrict__ y) {
-    uint32_t r0, r1;
-    block_row_range(m, chunk, xcd_logical_block(blockIdx.x, gridDim.x), r0, r1);
-    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
-    for (uint32_t row = r0 + g; row < r1; row += kBlock / LPR) {
-        const T s = row_product<LPR>(row, rp, ci, val, x, lane);
-        if (lane == 0) y[row] = s;
-    }
+                                                 const T* __restrict__ x, T* __restrict__ y) {
+    __shared__ T stage[StageSize<RPW>::value];
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    auto epi = [&](uint32_t row, T s) { y[row] = s; };
+    EngineOf<RPW>::type::run(m, sp, L, rp, ci, val, x, stage, epi);
 }
 
 // r = b - A x (FP64), with per-block partial sums of r.r and b.b written to
 // part_rr[L], part_bb[L] (L = logical block).
-template <int LPR>
+template <int RPW>
 __global__ __launch_bounds__(kBlock) void k_residual(
-    uint32_t m, uint32_t chunk, const uint32_t* __restrict__ rp,
-    const uint32_t* __restrict__ ci, const double* __restrict__ val,
-    const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
-    double* __restrict__ part_rr, double* __restrict__ part_bb) {
+    uint32_t m, uint32_t sp, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+    const double* __restrict__ val, const double* __restrict__ x,
+    const double* __restrict__ b, double* __restrict__ r, double* __restrict__ part_rr,
+    double* __restrict__ part_bb) {
+    __shared__ double stage[StageSize<RPW>::value];
     __shared__ double lds[kWavesPerBlock];
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
-    uint32_t r0, r1;
-    block_row_range(m, chunk, L, r0, r1);
-    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
     double srr = 0.0, sbb = 0.0;
-    for (uint32_t row = r0 + g; row < r1; row += kBlock / LPR) {
-        const double s = row_product<LPR>(row, rp, ci, val, x, lane);
-        if (lane == 0) {
-            const double bi = b[row];
-            const double ri = bi - s;
-            if (r) r[row] = ri;
-            srr += ri * ri;
-            sbb += bi * bi;
-        }
-    }
+    auto epi = [&](uint32_t row, double s) {
+        const double bi = b[row];
+        const double ri = bi - s;
+        if (r) r[row] = ri;
+        srr += ri * ri;
+        sbb += bi * bi;
+    };
+    EngineOf<RPW>::type::run(m, sp, L, rp, ci, val, x, stage, epi);
     srr = block_sum_all(srr, lds);
     sbb = block_sum_all(sbb, lds);
     if (threadIdx.x == 0) {
@@ -88,8 +81,13 @@ __global__ __launch_bounds__(kBlock) void k_cast_f32(uint64_t n, const double* _
 
 // ---------------------------------------------------------------- launchers
 
-uint32_t spmv_chunk(uint32_t m, uint32_t grid, int lpr) {
-    const uint32_t rows_per_pass = uint32_t(kBlock / lpr);
+uint32_t spmv_work_per_block(uint32_t m, uint32_t grid, int rpw) {
+    if (rpw >= 64) {
+        const uint64_t tiles = (uint64_t(m) + 63) / 64;
+        uint64_t t = (tiles + grid - 1) / grid;
+        return uint32_t(t ? t : 1);
+    }
+    const uint32_t rows_per_pass = uint32_t(kWavesPerBlock * rpw);
     uint64_t c = (uint64_t(m) + grid - 1) / grid;
     c = (c + rows_per_pass - 1) / rows_per_pass * rows_per_pass;
     if (c == 0) c = rows_per_pass;
@@ -98,10 +96,10 @@ uint32_t spmv_chunk(uint32_t m, uint32_t grid, int lpr) {
 
 template <typename T>
 hipError_t spmv(uint32_t m, const uint32_t* rp, const uint32_t* ci, const T* val, const T* x,
-                T* y, int lpr, uint32_t grid, hipStream_t s) {
+                T* y, int rpw, uint32_t grid, hipStream_t s) {
     if (m == 0) return hipSuccess;
-    const uint32_t chunk = spmv_chunk(m, grid, lpr);
-    NLHIP_LPR_SWITCH(lpr, (k_spmv<LPR, T><<<grid, kBlock, 0, s>>>(m, chunk, rp, ci, val, x, y)));
+    const uint32_t sp = spmv_work_per_block(m, grid, rpw);
+    NLHIP_RPW_SWITCH(rpw, (k_spmv<RPW, T><<<grid, kBlock, 0, s>>>(m, sp, rp, ci, val, x, y)));
     return hipGetLastError();
 }
 template hipError_t spmv<double>(uint32_t, const uint32_t*, const uint32_t*, const double*,
@@ -111,9 +109,9 @@ template hipError_t spmv<float>(uint32_t, const uint32_t*, const uint32_t*, cons
 
 hipError_t residual64(uint32_t m, const uint32_t* rp, const uint32_t* ci, const double* val,
                       const double* x, const double* b, double* r, double* part_rr,
-                      double* part_bb, int lpr, uint32_t grid, hipStream_t s) {
-    const uint32_t chunk = spmv_chunk(m, grid, lpr);
-    NLHIP_LPR_SWITCH(lpr, (k_residual<LPR><<<grid, kBlock, 0, s>>>(m, chunk, rp, ci, val, x, b,
+                      double* part_bb, int rpw, uint32_t grid, hipStream_t s) {
+    const uint32_t sp = spmv_work_per_block(m, grid, rpw);
+    NLHIP_RPW_SWITCH(rpw, (k_residual<RPW><<<grid, kBlock, 0, s>>>(m, sp, rp, ci, val, x, b,
                                                                    r, part_rr, part_bb)));
     return hipGetLastError();
 }

@@ -232,12 +232,12 @@ class Graph {
 template <typename T>
 struct CGWorkspace {
     uint32_t n = 0, P = 0;
-    DevBuf<T> x, b, r, h, d0, d1, q, part;
+    DevBuf<T> x, b, r, h, d, q, part;
     DevBuf<CGStatus<T>> st;
     PinnedBuf<CGStatus<T>> st_host;  // two slots
     Graph graph;
     int graph_iters = 0;
-    int graph_lpr = 0;
+    int graph_rpw = 0;
     bool graph_precond = false;
     CGArgs<T> graph_args{};  // arguments baked into the captured graph
     CGArgs<T> args{};
@@ -252,8 +252,8 @@ struct System {
     DevBuf<uint32_t> rowptr, colind;
     DevBuf<double> val, dinv;
     DevBuf<float> val32, dinv32;
-    int lpr = 8;
-    uint32_t grid = 1;
+    int rpw = 64;        // rows per wavefront of the SpMV engine
+    uint32_t grid = 1;   // workgroups per launch
     Stream stream;
     std::mutex mu;
     CGWorkspace<double> ws64;
@@ -271,7 +271,7 @@ struct System {
 // Validates and uploads; throws Error.
 System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
                       const double* val, uint32_t flags);
-void default_launch(uint32_t n, uint64_t nnz, int& lpr, uint32_t& grid);
+void default_launch(uint32_t n, uint64_t nnz, int& rpw, uint32_t& grid);
 
 // Solve drivers (System::mu must be held).  b, x: device pointers of length n.
 void solve_fp64(System& S, const double* b_dev, double* x_dev, const nlhip_solve_params& p,

@@ -8,27 +8,39 @@
 
 namespace nlhip {
 
-// Row-group width (lanes per row) dispatch over the compiled instances.
-#define NLHIP_LPR_SWITCH(lpr, CALL)                       \
-    switch (lpr) {                                        \
-        case 4: { constexpr int LPR = 4; CALL; } break;   \
-        case 8: { constexpr int LPR = 8; CALL; } break;   \
-        case 16: { constexpr int LPR = 16; CALL; } break; \
-        case 32: { constexpr int LPR = 32; CALL; } break; \
+// Rows-per-wavefront dispatch over the compiled SpMV engines:
+// 2, 4, 8, 16 -> 32, 16, 8, 4 lanes per row; 64 -> one row per lane, streamed.
+#define NLHIP_RPW_SWITCH(rpw, CALL)                       \
+    switch (rpw) {                                        \
+        case 2: { constexpr int RPW = 2; CALL; } break;   \
+        case 4: { constexpr int RPW = 4; CALL; } break;   \
+        case 8: { constexpr int RPW = 8; CALL; } break;   \
+        case 16: { constexpr int RPW = 16; CALL; } break; \
+        case 64: { constexpr int RPW = 64; CALL; } break; \
+        case 65: { constexpr int RPW = 65; CALL; } break; \
+        case 66: { constexpr int RPW = 66; CALL; } break; \
+        case 67: { constexpr int RPW = 67; CALL; } break; \
         default: return hipErrorInvalidValue;             \
     }
 
-// Rows per workgroup for an SpMV-shaped launch of `grid` workgroups.
-uint32_t spmv_chunk(uint32_t m, uint32_t grid, int lpr);
+// 65..67 are stream-engine variants (chunk width / load kind) kept for
+// launch-shape sweeps; they compute the same sums as 64.
+inline bool valid_rpw(int rpw) {
+    return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 67);
+}
+
+// SpMV work per workgroup for `grid` workgroups: rows (vector engines,
+// a multiple of the rows one pass covers) or 64-row tiles (stream engine).
+uint32_t spmv_work_per_block(uint32_t m, uint32_t grid, int rpw);
 
 template <typename T>
 hipError_t spmv(uint32_t m, const uint32_t* rp, const uint32_t* ci, const T* val, const T* x,
-                T* y, int lpr, uint32_t grid, hipStream_t s);
+                T* y, int rpw, uint32_t grid, hipStream_t s);
 
 // r = b - A x (r may be null), partial sums of r.r and b.b per workgroup.
 hipError_t residual64(uint32_t m, const uint32_t* rp, const uint32_t* ci, const double* val,
                       const double* x, const double* b, double* r, double* part_rr,
-                      double* part_bb, int lpr, uint32_t grid, hipStream_t s);
+                      double* part_bb, int rpw, uint32_t grid, hipStream_t s);
 
 template <typename T>
 hipError_t jacobi_dinv(uint32_t m, const uint32_t* rp, const uint32_t* ci, const double* val,
@@ -52,9 +64,9 @@ hipError_t reduce_final(const double* part, uint32_t P, double* out, hipStream_t
 template <typename T>
 struct CGArgs {
     uint32_t m;      // rows
-    uint32_t chunk;  // rows per workgroup (spmv_chunk(m, P, lpr))
+    uint32_t chunk;  // rows per workgroup of the element-wise launches
+    uint32_t sp;     // SpMV work per workgroup (spmv_work_per_block)
     uint32_t P;      // workgroups per launch = number of partial sums
-    uint32_t pad;
     const uint32_t* rp;
     const uint32_t* ci;
     const T* val;
@@ -63,8 +75,7 @@ struct CGArgs {
     const T* b;
     T* r;
     T* h;
-    T* d0;
-    T* d1;
+    T* d;
     T* q;
     T* part_dq;
     T* part_rh;
@@ -74,13 +85,14 @@ struct CGArgs {
 };
 
 template <typename T>
-hipError_t cg_start(const CGArgs<T>& a, int lpr, bool precond, bool x_zero, T eps,
-                    int max_iter, hipStream_t s);
+hipError_t cg_start(const CGArgs<T>& a, int rpw, bool precond, T eps, int max_iter,
+                    hipStream_t s);
+// One iteration: direction step, SpMV step, update step.
 template <typename T>
-hipError_t cg_step(const CGArgs<T>& a, int lpr, bool precond, hipStream_t s);
-// One of the two launches of an iteration: part 0 = spmv step, 1 = update.
+hipError_t cg_step(const CGArgs<T>& a, int rpw, bool precond, hipStream_t s);
+// One launch of an iteration: part 0 = direction, 1 = SpMV, 2 = update.
 template <typename T>
-hipError_t cg_step_part(const CGArgs<T>& a, int lpr, bool precond, int part, hipStream_t s);
+hipError_t cg_step_part(const CGArgs<T>& a, int rpw, bool precond, int part, hipStream_t s);
 
 // Mixed-precision refinement helpers.
 hipError_t ir_scale_cast(uint64_t n, const double* r, double s, float* out, hipStream_t st);

@@ -12,13 +12,16 @@ namespace nlhip {
 
 // ----------------------------------------------------------------- launch
 
-void default_launch(uint32_t n, uint64_t nnz, int& lpr, uint32_t& grid) {
-    const double avg = n ? double(nnz) / double(n) : 1.0;
-    lpr = avg <= 5.0 ? 4 : avg <= 11.0 ? 8 : avg <= 24.0 ? 16 : 32;
-    const uint32_t rows_per_pass = uint32_t(256 / lpr);  // 256-thread workgroups
-    uint64_t g = (uint64_t(n) + rows_per_pass - 1) / rows_per_pass;
-    uint32_t cap = 2048;
-    if (device_usable()) cap = uint32_t(std::max(8, device().cu_count * 8));
+// Stream engine (64 rows per wavefront); one 64-row tile per wavefront at
+// least, at most 4 workgroups per CU (which also bounds the number of
+// per-workgroup partial sums every CG launch reduces).
+void default_launch(uint32_t n, uint64_t nnz, int& rpw, uint32_t& grid) {
+    (void)nnz;
+    rpw = 64;
+    const uint64_t tiles = (uint64_t(n) + 63) / 64;
+    const uint64_t g = (tiles + 3) / 4;  // 4 wavefronts per workgroup
+    uint32_t cap = 1024;
+    if (device_usable()) cap = uint32_t(std::max(4, device().cu_count * 4));
     grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
 }
 
@@ -81,7 +84,7 @@ System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind
     copy_in(S->val.get(), va, nnz * 8, s);
     NLHIP_CHECK(jacobi_dinv<double>(n, S->rowptr.get(), S->colind.get(), S->val.get(),
                                     S->dinv.get(), s));
-    default_launch(n, nnz, S->lpr, S->grid);
+    default_launch(n, nnz, S->rpw, S->grid);
     if (flags & NLHIP_SYSTEM_FP32) S->ensure_fp32();
     S->stream.sync();
     return S.release();
@@ -111,8 +114,7 @@ void CGWorkspace<T>::ensure(uint32_t n_, uint32_t P_) {
     b.alloc(n);
     r.alloc(n);
     h.alloc(n);
-    d0.alloc(n);
-    d1.alloc(n);
+    d.alloc(n);
     q.alloc(n);
     part.alloc(size_t(4) * P);
     st.alloc(1);
@@ -126,7 +128,8 @@ static void setup_args(System& S, CGWorkspace<T>& ws, const T* val, const T* din
     CGArgs<T>& a = ws.args;
     a.m = S.n;
     a.P = S.grid;
-    a.chunk = spmv_chunk(S.n, S.grid, S.lpr);
+    a.chunk = uint32_t((uint64_t(S.n) + S.grid - 1) / S.grid);
+    a.sp = spmv_work_per_block(S.n, S.grid, S.rpw);
     a.rp = S.rowptr.get();
     a.ci = S.colind.get();
     a.val = val;
@@ -135,8 +138,7 @@ static void setup_args(System& S, CGWorkspace<T>& ws, const T* val, const T* din
     a.b = ws.b.get();
     a.r = ws.r.get();
     a.h = ws.h.get();
-    a.d0 = ws.d0.get();
-    a.d1 = ws.d1.get();
+    a.d = ws.d.get();
     a.q = ws.q.get();
     a.part_dq = ws.part.get();
     a.part_rh = ws.part.get() + S.grid;
@@ -158,15 +160,15 @@ template <typename T>
 static CGStatus<T> run_cg(System& S, CGWorkspace<T>& ws, bool precond, T eps, int max_iter,
                           uint32_t K) {
     hipStream_t s = S.stream.get();
-    NLHIP_CHECK(cg_start<T>(ws.args, S.lpr, precond, false, eps, max_iter, s));
-    if (!ws.graph.ready() || ws.graph_iters != int(K) || ws.graph_lpr != S.lpr ||
+    NLHIP_CHECK(cg_start<T>(ws.args, S.rpw, precond, eps, max_iter, s));
+    if (!ws.graph.ready() || ws.graph_iters != int(K) || ws.graph_rpw != S.rpw ||
         ws.graph_precond != precond ||
         std::memcmp(&ws.graph_args, &ws.args, sizeof(CGArgs<T>)) != 0) {
         ws.graph.capture(s, [&] {
-            for (uint32_t k = 0; k < K; ++k) NLHIP_CHECK(cg_step<T>(ws.args, S.lpr, precond, s));
+            for (uint32_t k = 0; k < K; ++k) NLHIP_CHECK(cg_step<T>(ws.args, S.rpw, precond, s));
         });
         ws.graph_iters = int(K);
-        ws.graph_lpr = S.lpr;
+        ws.graph_rpw = S.rpw;
         ws.graph_precond = precond;
         ws.graph_args = ws.args;
     }
@@ -208,7 +210,7 @@ static void residual_norms(System& S, const double* b, const double* x, double* 
     double* part = S.scratch.get();
     double* out = part + 2 * size_t(P);
     NLHIP_CHECK(residual64(S.n, S.rowptr.get(), S.colind.get(), S.val.get(), x, b, r_out, part,
-                           part + P, S.lpr, P, s));
+                           part + P, S.rpw, P, s));
     NLHIP_CHECK(reduce_final(part, P, out, s));
     NLHIP_CHECK(reduce_final(part + P, P, out + 2, s));
     S.host_scalars.ensure(4);
@@ -335,7 +337,7 @@ uint32_t bicgstab_device(System& S, const double* dinv, const double* b, double*
     S.host_scalars.ensure(4);
     DotHelper dot{S, part, out.get(), S.host_scalars.get()};
     auto spmv64 = [&](const double* in, double* o) {
-        NLHIP_CHECK(spmv<double>(n, S.rowptr.get(), S.colind.get(), S.val.get(), in, o, S.lpr,
+        NLHIP_CHECK(spmv<double>(n, S.rowptr.get(), S.colind.get(), S.val.get(), in, o, S.rpw,
                                  S.grid, s));
     };
     auto precond = [&](const double* in, double* o) {
@@ -446,28 +448,27 @@ NLAPI int NLAPIENTRY nlhipSystemInfo(nlhip_system s, uint32_t* n, uint64_t* nnz,
     });
 }
 
-NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int lanes_per_row,
+NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,
                                           uint32_t workgroups) {
     return guarded("nlhipSystemSetLaunch", [&] {
         System* S = sys(s);
-        if (lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16 &&
-            lanes_per_row != 32)
-            throw Error(NLHIP_ERR_INVALID, "lanes_per_row must be 4, 8, 16 or 32");
+        if (!valid_rpw(rows_per_wave))
+            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64");
         if (workgroups == 0 || workgroups > 65536)
             throw Error(NLHIP_ERR_INVALID, "workgroups must be in [1, 65536]");
         std::lock_guard<std::mutex> lock(S->mu);
-        S->lpr = lanes_per_row;
+        S->rpw = rows_per_wave;
         S->grid = workgroups;
         S->ws64.graph.reset();
         S->ws32.graph.reset();
     });
 }
 
-NLAPI int NLAPIENTRY nlhipSystemGetLaunch(nlhip_system s, int* lanes_per_row,
+NLAPI int NLAPIENTRY nlhipSystemGetLaunch(nlhip_system s, int* rows_per_wave,
                                           uint32_t* workgroups) {
     return guarded("nlhipSystemGetLaunch", [&] {
         System* S = sys(s);
-        if (lanes_per_row) *lanes_per_row = S->lpr;
+        if (rows_per_wave) *rows_per_wave = S->rpw;
         if (workgroups) *workgroups = S->grid;
     });
 }
@@ -496,7 +497,7 @@ NLAPI int NLAPIENTRY nlhipSpMV(nlhip_system s, const double* x, double* y) {
         S->ws64.ensure(S->n, S->grid);
         copy_in(S->ws64.x.get(), x, size_t(S->n) * 8, st);
         NLHIP_CHECK(spmv<double>(S->n, S->rowptr.get(), S->colind.get(), S->val.get(),
-                                 S->ws64.x.get(), S->ws64.q.get(), S->lpr, S->grid, st));
+                                 S->ws64.x.get(), S->ws64.q.get(), S->rpw, S->grid, st));
         copy_out(y, S->ws64.q.get(), size_t(S->n) * 8, st);
         S->stream.sync();
     });
@@ -516,7 +517,7 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
             NLHIP_CHECK(fill<float>(S->n, 1.0f, S->ws32.x.get(), st));
             auto run = [&] {
                 NLHIP_CHECK(spmv<float>(S->n, S->rowptr.get(), S->colind.get(), S->val32.get(),
-                                        S->ws32.x.get(), S->ws32.q.get(), S->lpr, S->grid, st));
+                                        S->ws32.x.get(), S->ws32.q.get(), S->rpw, S->grid, st));
             };
             run();
             e0.record(st);
@@ -527,7 +528,7 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
             NLHIP_CHECK(fill<double>(S->n, 1.0, S->ws64.x.get(), st));
             auto run = [&] {
                 NLHIP_CHECK(spmv<double>(S->n, S->rowptr.get(), S->colind.get(), S->val.get(),
-                                         S->ws64.x.get(), S->ws64.q.get(), S->lpr, S->grid, st));
+                                         S->ws64.x.get(), S->ws64.q.get(), S->rpw, S->grid, st));
             };
             run();
             e0.record(st);
@@ -540,11 +541,10 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
 }
 
 // Runs `iters` PCG iterations (FP64, or FP32 with fp32 = 1) on the system
-// with the last right-hand side, launching each kernel directly and timing it
-// with HIP events on the system's stream; returns the mean duration of the
-// spmv step and of the update step.
-NLAPI int NLAPIENTRY nlhipProfileCG(nlhip_system s, int fp32, uint32_t iters,
-                                    double* spmv_step_seconds, double* update_step_seconds) {
+// with b = 1, launching each kernel directly and timing it with HIP events on
+// the system's stream; seconds[0..2] = mean duration of the direction, SpMV
+// and update launches.
+NLAPI int NLAPIENTRY nlhipProfileCG(nlhip_system s, int fp32, uint32_t iters, double* seconds) {
     return guarded("nlhipProfileCG", [&] {
         System* S = sys(s);
         std::lock_guard<std::mutex> lock(S->mu);
@@ -556,23 +556,22 @@ NLAPI int NLAPIENTRY nlhipProfileCG(nlhip_system s, int fp32, uint32_t iters,
             setup_args<T>(*S, ws, val, dinv);
             NLHIP_CHECK(fill<T>(S->n, T(1), ws.b.get(), st));
             NLHIP_CHECK(fill<T>(S->n, T(0), ws.x.get(), st));
-            NLHIP_CHECK(cg_start<T>(ws.args, S->lpr, true, false, T(0), int(iters) + 8, st));
-            std::vector<Event> ev(3 * size_t(iters));
+            NLHIP_CHECK(cg_start<T>(ws.args, S->rpw, true, T(0), int(iters) + 8, st));
+            std::vector<Event> ev(4 * size_t(iters));
             for (uint32_t k = 0; k < iters; ++k) {
-                ev[3 * k].record(st);
-                NLHIP_CHECK(cg_step_part<T>(ws.args, S->lpr, true, 0, st));
-                ev[3 * k + 1].record(st);
-                NLHIP_CHECK(cg_step_part<T>(ws.args, S->lpr, true, 1, st));
-                ev[3 * k + 2].record(st);
+                for (int part = 0; part < 3; ++part) {
+                    ev[4 * k + part].record(st);
+                    NLHIP_CHECK(cg_step_part<T>(ws.args, S->rpw, true, part, st));
+                }
+                ev[4 * k + 3].record(st);
             }
             S->stream.sync();
-            double ta = 0, tb = 0;
-            for (uint32_t k = 0; k < iters; ++k) {
-                ta += 1e-3 * elapsed_ms(ev[3 * k], ev[3 * k + 1]);
-                tb += 1e-3 * elapsed_ms(ev[3 * k + 1], ev[3 * k + 2]);
-            }
-            if (spmv_step_seconds) *spmv_step_seconds = ta / iters;
-            if (update_step_seconds) *update_step_seconds = tb / iters;
+            double t[3] = {0, 0, 0};
+            for (uint32_t k = 0; k < iters; ++k)
+                for (int part = 0; part < 3; ++part)
+                    t[part] += 1e-3 * elapsed_ms(ev[4 * k + part], ev[4 * k + part + 1]);
+            for (int part = 0; part < 3; ++part)
+                if (seconds) seconds[part] = t[part] / iters;
         };
         if (fp32) {
             S->ensure_fp32();

@@ -92,10 +92,12 @@ NLAPI int NLAPIENTRY nlhipSystemCreate(uint32_t n, const uint32_t* rowptr,
 NLAPI int NLAPIENTRY nlhipSystemDestroy(nlhip_system s);
 NLAPI int NLAPIENTRY nlhipSystemInfo(nlhip_system s, uint32_t* n, uint64_t* nnz,
                                      uint64_t* device_bytes);
-/* Launch shape: lanes per row (4, 8, 16, 32) and workgroups per launch. */
-NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int lanes_per_row,
+/* Launch shape of the SpMV engine: rows per wavefront (2, 4, 8, 16: that many
+ * rows share a 64-lane wavefront; 64: one row per lane, nonzeros streamed
+ * through LDS) and workgroups per launch. */
+NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,
                                           uint32_t workgroups);
-NLAPI int NLAPIENTRY nlhipSystemGetLaunch(nlhip_system s, int* lanes_per_row,
+NLAPI int NLAPIENTRY nlhipSystemGetLaunch(nlhip_system s, int* rows_per_wave,
                                           uint32_t* workgroups);
 /* Solves A x = b; x holds the initial guess.  b, x host or device. */
 NLAPI int NLAPIENTRY nlhipSolve(nlhip_system s, const double* b, double* x,
@@ -108,12 +110,11 @@ NLAPI int NLAPIENTRY nlhipSpMV(nlhip_system s, const double* x, double* y);
 NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
                                     double* seconds_per_spmv);
 
-/* Runs `iters` PCG iterations (FP64, or FP32 when fp32 = 1) launching each
- * kernel directly, and returns the mean HIP-event duration of the fused
- * SpMV step and of the update step. */
+/* Runs `iters` PCG iterations (FP64, or FP32 when fp32 = 1, b = 1) launching
+ * each kernel directly; seconds[0..2] receive the mean HIP-event duration of
+ * the direction, SpMV and update launches. */
 NLAPI int NLAPIENTRY nlhipProfileCG(nlhip_system s, int fp32, uint32_t iters,
-                                    double* spmv_step_seconds,
-                                    double* update_step_seconds);
+                                    double* seconds);
 
 NLAPI const char* NLAPIENTRY nlhipLastError(void);
 /* Test hook: the next `count` solves fail with NLHIP_ERR_INJECTED before

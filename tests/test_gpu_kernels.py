@@ -23,11 +23,11 @@ def test_spmv_exact_all_launch_shapes(hip):
     for name in z["names"]:
         rp, ci, va = z[f"{name}_rowptr"], z[f"{name}_colind"], z[f"{name}_val"]
         S = capi.System(rp, ci, va)
-        for lpr in (4, 8, 16, 32):
+        for rpw in (2, 4, 8, 16, 64):
             for wg in (1, 3, 64, 2048):
-                S.set_launch(lpr, wg)
+                S.set_launch(rpw, wg)
                 y = S.spmv(z[f"{name}_x"])
-                assert np.array_equal(y, z[f"{name}_y"]), (name, lpr, wg)
+                assert np.array_equal(y, z[f"{name}_y"]), (name, rpw, wg)
         S.close()
 
 
@@ -40,6 +40,25 @@ def test_spmv_lscm_vs_oracle(hip):
     y = S.spmv(x)
     scale = np.abs(Sw.to_scipy()) @ np.abs(x)
     assert np.all(np.abs(y - ref) <= 16 * 2.0 ** -53 * scale + 1e-300)
+
+
+def test_stream_engine_is_bitwise_the_oracle_spmv(hip):
+    """64 rows per wavefront sums each row left to right with one rounding
+    per product and per add: exactly the oracle's row loop."""
+    for Sw in (W.lscm_grid(45, seed=5), W.cotan_system(W.grid_mesh(70, seed=2), nrhs=1)):
+        A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+        x = np.random.default_rng(4).standard_normal(Sw.n)
+        S = capi.System.from_crs(Sw)
+        for wg in (1, 5, 300):
+            S.set_launch(64, wg)
+            assert np.array_equal(S.spmv(x), O.spmv(A, x))
+
+
+def test_set_launch_rejects_bad_shapes(hip):
+    S = capi.System([0, 1], [0], [1.0])
+    for rpw, wg in [(3, 10), (32, 10), (64, 0)]:
+        with pytest.raises(capi.NLHIPError):
+            S.set_launch(rpw, wg)
 
 
 def test_spmv_through_opennl_api(hip):
