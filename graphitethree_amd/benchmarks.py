@@ -1,0 +1,127 @@
+"""Measurements of the configurations beside bench.py's single large system.
+
+    python -m graphitethree_amd.benchmarks atlas [--count 2000] [--seed 1]
+
+atlas: `count` disk-like LSCM charts (N ~ logU[50, 50000] vertices) solved
+  (1) as one batch in FP64, (2) as one batch with FP32 inner PCG inside FP64
+  refinement, and (3) chart by chart with the plain FP64 GPU PCG
+  (nlhipSolve per chart, the way an atlas builder calls the solver once per
+  chart).  Reports solves/s of each, the speed-up of (2) and (1) over (3),
+  convergence counts, and the largest relative difference between the
+  batched and per-chart solutions.
+
+Each subcommand prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import capi
+from . import workloads as W
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def init_device():
+    L = capi.lib()
+    if not L.nlInitExtension(b"HIP"):
+        raise SystemExit("nlInitExtension(\"HIP\") failed: " + capi.last_error())
+    return capi.device_info()
+
+
+def run_atlas(args) -> dict:
+    dev = init_device()
+    t0 = time.time()
+    A = W.atlas(args.count, seed=args.seed, nmin=args.nmin, nmax=args.nmax)
+    log(f"atlas: {A.count} charts, n={A.n}, nnz={A.nnz}, built in {time.time() - t0:.1f}s")
+    B = capi.Batch(packed=(A.row_off, A.rowptr, A.colind, A.val), fp32=True)
+    params = dict(eps=args.eps, max_iter=args.max_iter)
+
+    x64, p64, t64 = B.solve(A.b, **params)            # warm-up (graph capture)
+    x64, p64, t64 = B.solve(A.b, **params)
+    log(f"batched fp64: {t64.device_seconds:.3f}s, max it {t64.iterations}, "
+        f"converged {sum(p.converged for p in p64)}/{A.count}")
+    xm, pm, tm = B.solve(A.b, precision="mixed", inner_eps=args.inner_eps,
+                         max_iter=args.max_iter_mixed, eps=args.eps)
+    xm, pm, tm = B.solve(A.b, precision="mixed", inner_eps=args.inner_eps,
+                         max_iter=args.max_iter_mixed, eps=args.eps)
+    log(f"batched mixed: {tm.device_seconds:.3f}s, converged "
+        f"{sum(p.converged for p in pm)}/{A.count}")
+
+    # plain FP64 GPU CG, one nlhipSolve per chart (matrices resident)
+    charts = list(range(0, A.count, args.baseline_stride))
+    xs64 = B.split(x64)
+    t_plain = 0.0
+    worst_diff = 0.0
+    its_plain = []
+    for k, c in enumerate(charts):
+        S = A.chart(c)
+        sysc = capi.System.from_crs(S)
+        sysc.solve(S.b, **params)                      # warm-up
+        x, st = sysc.solve(S.b, **params)
+        t_plain += st.device_seconds
+        its_plain.append(int(st.iterations))
+        worst_diff = max(worst_diff, float(np.linalg.norm(x - xs64[c]) /
+                                           max(np.linalg.norm(xs64[c]), 1e-300)))
+        sysc.close()
+        if k % 200 == 0:
+            log(f"  per-chart baseline {k}/{len(charts)} ({t_plain:.2f}s so far)")
+    scale = A.count / len(charts)
+    t_plain_all = t_plain * scale
+    res = {
+        "metric": "atlas solves/s (charts solved per second) at 1e-10 rel. residual",
+        "workload": f"atlas_{A.count}_charts_N{int(args.nmin)}-{int(args.nmax)}",
+        "charts": A.count, "n": A.n, "nnz": A.nnz, "eps": args.eps,
+        "device": dev[0],
+        "batched_fp64": {"seconds": t64.device_seconds, "solves_per_s": A.count / t64.device_seconds,
+                         "max_iterations": int(t64.iterations), "max_iter": args.max_iter,
+                         "converged": int(sum(p.converged for p in p64)),
+                         "worst_rel_residual": float(t64.rel_residual)},
+        "batched_mixed": {"seconds": tm.device_seconds, "solves_per_s": A.count / tm.device_seconds,
+                          "max_inner_iterations": int(tm.iterations),
+                          "max_iter": args.max_iter_mixed, "inner_eps": args.inner_eps,
+                          "converged": int(sum(p.converged for p in pm)),
+                          "worst_rel_residual": float(tm.rel_residual),
+                          "max_rel_diff_vs_fp64": float(max(
+                              np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+                              for a, b in zip(B.split(xm), xs64)))},
+        "per_chart_fp64": {"seconds": t_plain_all, "solves_per_s": A.count / t_plain_all,
+                           "sampled_charts": len(charts),
+                           "sample": f"every {args.baseline_stride}th chart, time scaled by "
+                                     f"{scale:g}",
+                           "max_rel_diff_vs_batched": worst_diff,
+                           "iterations_mean": float(np.mean(its_plain))},
+    }
+    res["speedup_mixed_batched_vs_per_chart_fp64"] = t_plain_all / tm.device_seconds
+    res["speedup_fp64_batched_vs_per_chart_fp64"] = t_plain_all / t64.device_seconds
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("atlas")
+    a.add_argument("--count", type=int, default=2000)
+    a.add_argument("--seed", type=int, default=1)
+    a.add_argument("--nmin", type=float, default=50)
+    a.add_argument("--nmax", type=float, default=50000)
+    a.add_argument("--eps", type=float, default=1e-10)
+    a.add_argument("--max-iter", type=int, default=50000)
+    a.add_argument("--max-iter-mixed", type=int, default=200000)
+    a.add_argument("--inner-eps", type=float, default=1e-4)
+    a.add_argument("--baseline-stride", type=int, default=1)
+    args = ap.parse_args(argv)
+    if args.cmd == "atlas":
+        out = run_atlas(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -156,6 +156,13 @@ SIGNATURES = {
     "nlhipSpMV": (C.c_int, [_vp, _vp, _vp]),
     "nlhipBenchSpMV": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
     "nlhipProfileCG": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
+    "nlhipBatchCreate": (C.c_int, [C.c_uint32, _vp, _vp, _vp, _vp, C.c_uint32,
+                                   C.POINTER(_vp)]),
+    "nlhipBatchDestroy": (C.c_int, [_vp]),
+    "nlhipBatchInfo": (C.c_int, [_vp, _u32p, _u32p, C.POINTER(C.c_uint64), _u32p]),
+    "nlhipBatchSetLaunch": (C.c_int, [_vp, C.c_uint32]),
+    "nlhipBatchSolve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
+                                  C.POINTER(SolveStats), C.POINTER(SolveStats)]),
     "nlhipLastError": (C.c_char_p, []),
     "nlhipInjectFailures": (None, [C.c_int]),
     "nlhipDeviceInfo": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int),
@@ -278,6 +285,74 @@ class System:
         check(lib().nlhipBenchSpMV(self._h, 1 if fp32 else 0, reps, C.byref(s)),
               "nlhipBenchSpMV")
         return float(s.value)
+
+
+class Batch:
+    """Many independent systems packed block-diagonally (nlhipBatchCreate)."""
+
+    def __init__(self, systems=None, fp32: bool = False, packed=None):
+        """`systems`: sequence of objects with rowptr, colind, val (local CRS);
+        or `packed` = (row_off, rowptr, colind, val) already block-diagonal."""
+        if packed is not None:
+            self._create(*packed, fp32=fp32)
+            return
+        row_off = [0]
+        rps, cis, vas = [], [], []
+        nnz_off = 0
+        for S in systems:
+            n = len(S.rowptr) - 1
+            rps.append(np.asarray(S.rowptr[:-1], dtype=np.int64) + nnz_off)
+            cis.append(np.asarray(S.colind, dtype=np.int64) + row_off[-1])
+            vas.append(np.asarray(S.val, dtype=np.float64))
+            nnz_off += int(S.rowptr[-1])
+            row_off.append(row_off[-1] + n)
+        self._create(np.array(row_off, dtype=np.uint32),
+                     np.concatenate(rps + [np.array([nnz_off])]).astype(np.uint32),
+                     np.concatenate(cis).astype(np.uint32), np.concatenate(vas), fp32=fp32)
+
+    def _create(self, row_off, rowptr, colind, val, fp32: bool):
+        self.row_off = np.ascontiguousarray(row_off, dtype=np.uint32)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.uint32)
+        self.colind = np.ascontiguousarray(colind, dtype=np.uint32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.nsys = len(self.row_off) - 1
+        self.n = int(self.row_off[-1])
+        h = C.c_void_p()
+        check(lib().nlhipBatchCreate(self.nsys, ptr(self.row_off), ptr(self.rowptr),
+                                     ptr(self.colind), ptr(self.val),
+                                     NLHIP_SYSTEM_FP32 if fp32 else 0, C.byref(h)),
+              "nlhipBatchCreate")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().nlhipBatchDestroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def info(self):
+        ns, n, nnz, t = C.c_uint32(), C.c_uint32(), C.c_uint64(), C.c_uint32()
+        check(lib().nlhipBatchInfo(self._h, C.byref(ns), C.byref(n), C.byref(nnz),
+                                   C.byref(t)), "nlhipBatchInfo")
+        return int(ns.value), int(n.value), int(nnz.value), int(t.value)
+
+    def set_launch(self, workgroups: int):
+        check(lib().nlhipBatchSetLaunch(self._h, workgroups), "nlhipBatchSetLaunch")
+
+    def split(self, v):
+        return [v[self.row_off[s]:self.row_off[s + 1]] for s in range(self.nsys)]
+
+    def solve(self, b, x0=None, **params):
+        """b: concatenated right-hand sides.  Returns (x, per-system stats, total)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        p = default_params(**params)
+        per = (SolveStats * self.nsys)()
+        tot = SolveStats()
+        check(lib().nlhipBatchSolve(self._h, ptr(b), ptr(x), C.byref(p), per, C.byref(tot)),
+              "nlhipBatchSolve")
+        return x, list(per), tot
 
 
 def profile_cg(system: "System", iters: int = 50, fp32: bool = False):

@@ -1,0 +1,466 @@
+// Batched solves of many independent systems (texture-atlas charts): host
+// side of k_batch.hip and its C ABI (include/nl_hip.h, batched part).
+//
+// The systems arrive as one block-diagonal CRS.  They are cut into tiles of
+// kTileRows rows; a fixed grid of workgroups walks the active tiles; a
+// captured hipGraph holds `check_every` iterations followed by a rebuild of
+// the active lists, and the host only reads the number of active systems
+// between graph launches.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+
+#include "nlhip_internal.h"
+
+namespace nlhip {
+
+template <typename T>
+struct BatchWork {
+    DevBuf<T> x, b, r, h, d, q, part, sys;  // part: 4 x ntiles; sys: 6 x nsys
+    DevBuf<int32_t> iters, done, skip;
+    DevBuf<uint32_t> lists, counts;         // act_tiles | act_sys ; 2 counters
+    PinnedBuf<uint32_t> host_counts;        // two slots
+    Graph graph;
+    BatchArgs<T> args{};
+    BatchArgs<T> graph_args{};
+    uint32_t graph_iters = 0;
+    int graph_max_iter = -1;
+    bool graph_precond = false;
+};
+
+struct Batch {
+    uint32_t nsys = 0, n = 0, ntiles = 0;
+    uint64_t nnz = 0;
+    DevBuf<uint32_t> row_off, tile_off, tile_sys, tile_row0, rowptr, colind;
+    DevBuf<double> val, dinv, scale;  // scale: 2 x nsys (scale, unscale)
+    DevBuf<float> val32, dinv32;
+    std::vector<uint32_t> h_row_off;
+    uint32_t grid = 1;
+    Stream stream;
+    std::mutex mu;
+    BatchWork<double> w64;
+    BatchWork<float> w32;
+};
+
+namespace {
+
+template <typename T>
+void work_alloc(Batch& B, BatchWork<T>& w) {
+    if (w.x.get()) return;
+    const size_t n = B.n;
+    w.x.alloc(n);
+    w.b.alloc(n);
+    w.r.alloc(n);
+    w.h.alloc(n);
+    w.d.alloc(n);
+    w.q.alloc(n);
+    w.part.alloc(4 * size_t(B.ntiles));
+    w.sys.alloc(6 * size_t(B.nsys));
+    w.iters.alloc(B.nsys);
+    w.done.alloc(B.nsys);
+    w.skip.alloc(B.nsys);
+    w.lists.alloc(size_t(B.ntiles) + B.nsys);
+    w.counts.alloc(2);
+    w.host_counts.ensure(2);
+    NLHIP_CHECK(fill<int32_t>(B.nsys, 0, w.skip.get(), B.stream.get()));
+    BatchArgs<T>& a = w.args;
+    a.nsys = B.nsys;
+    a.ntiles = B.ntiles;
+    a.row_off = B.row_off.get();
+    a.tile_off = B.tile_off.get();
+    a.tile_sys = B.tile_sys.get();
+    a.tile_row0 = B.tile_row0.get();
+    a.rp = B.rowptr.get();
+    a.ci = B.colind.get();
+    a.x = w.x.get();
+    a.b = w.b.get();
+    a.r = w.r.get();
+    a.h = w.h.get();
+    a.d = w.d.get();
+    a.q = w.q.get();
+    a.part_dq = w.part.get();
+    a.part_rh = w.part.get() + B.ntiles;
+    a.part_rr = w.part.get() + 2 * size_t(B.ntiles);
+    a.part_bb = w.part.get() + 3 * size_t(B.ntiles);
+    a.rho = w.sys.get();
+    a.alpha = w.sys.get() + B.nsys;
+    a.beta = w.sys.get() + 2 * size_t(B.nsys);
+    a.err = w.sys.get() + 3 * size_t(B.nsys);
+    a.thresh = w.sys.get() + 4 * size_t(B.nsys);
+    a.bb = w.sys.get() + 5 * size_t(B.nsys);
+    a.iters = w.iters.get();
+    a.done = w.done.get();
+    a.skip = nullptr;
+    a.act_tiles = w.lists.get();
+    a.act_sys = w.lists.get() + B.ntiles;
+    a.n_act_tiles = w.counts.get();
+    a.n_act_sys = w.counts.get() + 1;
+}
+
+// Runs the batched PCG (arguments set, b and x filled) until no system is
+// active.  Graphs hold K iterations and a list rebuild.
+template <typename T>
+void run_batch(Batch& B, BatchWork<T>& w, bool precond, T eps, int max_iter, uint32_t K) {
+    hipStream_t s = B.stream.get();
+    NLHIP_CHECK(batch_start<T>(w.args, precond, eps, max_iter, B.grid, s));
+    if (!w.graph.ready() || w.graph_iters != K || w.graph_precond != precond ||
+        w.graph_max_iter != max_iter ||
+        std::memcmp(&w.graph_args, &w.args, sizeof(BatchArgs<T>)) != 0) {
+        w.graph.capture(s, [&] {
+            for (uint32_t k = 0; k < K; ++k)
+                NLHIP_CHECK(batch_iteration<T>(w.args, precond, max_iter, B.grid, s));
+            NLHIP_CHECK(batch_compact<T>(w.args, B.grid, s));
+        });
+        w.graph_iters = K;
+        w.graph_precond = precond;
+        w.graph_max_iter = max_iter;
+        w.graph_args = w.args;
+    }
+    Event ev[2];
+    uint32_t* host = w.host_counts.get();
+    const uint64_t max_graphs = uint64_t(std::max(max_iter, 0)) / K + 4;
+    uint64_t launched = 0;
+    for (;;) {
+        w.graph.launch(s);
+        const int slot = int(launched & 1);
+        NLHIP_CHECK(hipMemcpyAsync(&host[slot], w.counts.get() + 1, sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, s));
+        ev[slot].record(s);
+        ++launched;
+        if (launched >= 2) {
+            const int prev = int((launched - 2) & 1);
+            ev[prev].sync();
+            if (host[prev] == 0) break;
+        }
+        if (launched > max_graphs) {
+            B.stream.sync();
+            if (host[slot] == 0) break;
+            throw Error(NLHIP_ERR_HIP, "batched CG did not reach its stopping test");
+        }
+    }
+    B.stream.sync();
+}
+
+uint32_t batch_check_every(const Batch& B) {
+    const double bytes = 12.0 * double(B.nnz) + 108.0 * double(B.n);
+    const double iter_s = std::max(bytes / 4.0e12, 3.0e-5);
+    return uint32_t(std::clamp(4.0e-3 / iter_s, 4.0, 64.0));
+}
+
+void ensure_fp32(Batch& B) {
+    if (B.val32.get()) return;
+    B.val32.alloc(B.nnz);
+    B.dinv32.alloc(B.n);
+    NLHIP_CHECK(cast_f64_f32(B.nnz, B.val.get(), B.val32.get(), B.stream.get()));
+    NLHIP_CHECK(jacobi_dinv<float>(B.n, B.rowptr.get(), B.colind.get(), B.val.get(),
+                                   B.dinv32.get(), B.stream.get()));
+}
+
+void fill_stats(Batch& B, const std::vector<int32_t>& its, const std::vector<double>& rel,
+                const std::vector<double>& cgrel, const std::vector<uint8_t>& conv,
+                const std::vector<uint32_t>* outer, nlhip_solve_stats* per,
+                nlhip_solve_stats* total, double dev_s, double wall_s) {
+    uint64_t it_sum = 0;
+    uint32_t it_max = 0, nconv = 0;
+    double worst = 0.0;
+    for (uint32_t s = 0; s < B.nsys; ++s) {
+        it_sum += uint64_t(its[s]);
+        it_max = std::max(it_max, uint32_t(its[s]));
+        nconv += conv[s];
+        worst = std::max(worst, rel[s]);
+        if (per) {
+            nlhip_solve_stats& st = per[s];
+            std::memset(&st, 0, sizeof(st));
+            st.iterations = uint32_t(its[s]);
+            st.outer_iterations = outer ? (*outer)[s] : 0;
+            st.converged = conv[s];
+            st.rel_residual = rel[s];
+            st.cg_rel_residual = cgrel[s];
+            st.device_seconds = dev_s;
+            st.wall_seconds = wall_s;
+        }
+    }
+    if (total) {
+        std::memset(total, 0, sizeof(*total));
+        total->iterations = it_max;
+        total->outer_iterations = uint32_t(std::min<uint64_t>(it_sum, 0xFFFFFFFFu));
+        total->converged = (nconv == B.nsys) ? 1 : 0;
+        total->rel_residual = worst;
+        total->device_seconds = dev_s;
+        total->wall_seconds = wall_s;
+    }
+}
+
+// FP64 |b - A x| / |b| for every system.
+void true_residuals(Batch& B, std::vector<double>& rel, std::vector<double>* rr_out = nullptr,
+                    std::vector<double>* bb_out = nullptr) {
+    work_alloc(B, B.w64);
+    BatchArgs<double> a = B.w64.args;
+    a.val = B.val.get();
+    NLHIP_CHECK(batch_residual64(a, B.grid, B.stream.get()));
+    std::vector<double> rr(B.nsys), bb(B.nsys);
+    NLHIP_CHECK(hipMemcpyAsync(rr.data(), a.err, B.nsys * 8, hipMemcpyDeviceToHost,
+                               B.stream.get()));
+    NLHIP_CHECK(hipMemcpyAsync(bb.data(), a.bb, B.nsys * 8, hipMemcpyDeviceToHost,
+                               B.stream.get()));
+    B.stream.sync();
+    rel.resize(B.nsys);
+    for (uint32_t s = 0; s < B.nsys; ++s)
+        rel[s] = bb[s] > 0 ? std::sqrt(rr[s] / bb[s]) : std::sqrt(rr[s]);
+    if (rr_out) *rr_out = rr;
+    if (bb_out) *bb_out = bb;
+}
+
+void batch_solve_fp64(Batch& B, const nlhip_solve_params& p, std::vector<int32_t>& its,
+                      std::vector<double>& cgrel, std::vector<uint8_t>& conv) {
+    BatchWork<double>& w = B.w64;
+    const bool precond = p.precond == NL_PRECOND_JACOBI;
+    w.args.val = B.val.get();
+    w.args.dinv = precond ? B.dinv.get() : nullptr;
+    w.args.skip = nullptr;
+    const uint32_t K = p.check_every ? p.check_every : batch_check_every(B);
+    run_batch<double>(B, w, precond, p.eps, int(p.max_iter), K);
+    std::vector<double> err(B.nsys), thr(B.nsys), bb(B.nsys);
+    its.resize(B.nsys);
+    hipStream_t s = B.stream.get();
+    NLHIP_CHECK(hipMemcpyAsync(its.data(), w.iters.get(), B.nsys * 4, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipMemcpyAsync(err.data(), w.args.err, B.nsys * 8, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipMemcpyAsync(thr.data(), w.args.thresh, B.nsys * 8, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipMemcpyAsync(bb.data(), w.args.bb, B.nsys * 8, hipMemcpyDeviceToHost, s));
+    B.stream.sync();
+    cgrel.resize(B.nsys);
+    conv.resize(B.nsys);
+    for (uint32_t k = 0; k < B.nsys; ++k) {
+        if (!std::isfinite(err[k]) || !std::isfinite(bb[k]))
+            throw Error(NLHIP_ERR_NONFINITE, "system " + std::to_string(k) +
+                                                 ": right-hand side or iterate is NaN or Inf");
+        cgrel[k] = bb[k] > 0 ? std::sqrt(err[k] / bb[k]) : std::sqrt(err[k]);
+        conv[k] = !(err[k] > thr[k]) ? 1 : 0;
+    }
+}
+
+void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_t>& its,
+                       std::vector<double>& cgrel, std::vector<uint8_t>& conv,
+                       std::vector<uint32_t>& outer) {
+    ensure_fp32(B);
+    work_alloc(B, B.w32);
+    BatchWork<double>& w64 = B.w64;
+    BatchWork<float>& w32 = B.w32;
+    const bool precond = p.precond == NL_PRECOND_JACOBI;
+    w32.args.val = B.val32.get();
+    w32.args.dinv = precond ? B.dinv32.get() : nullptr;
+    w32.args.skip = w32.skip.get();
+    hipStream_t s = B.stream.get();
+    B.scale.ensure(2 * size_t(B.nsys));
+    const uint32_t K = p.check_every ? p.check_every : batch_check_every(B);
+    const uint32_t inner_cap = p.inner_max_iter ? p.inner_max_iter : p.max_iter;
+    its.assign(B.nsys, 0);
+    outer.assign(B.nsys, 0);
+    conv.assign(B.nsys, 0);
+    cgrel.assign(B.nsys, 0.0);
+    std::vector<int32_t> skip(B.nsys), inner_its(B.nsys);
+    std::vector<double> sc(2 * size_t(B.nsys));
+    for (uint32_t step = 0;; ++step) {
+        std::vector<double> rel, rr, bb;
+        true_residuals(B, rel, &rr, &bb);  // leaves r64 = b - A x in w64.r
+        uint32_t active = 0;
+        for (uint32_t k = 0; k < B.nsys; ++k) {
+            if (!std::isfinite(rr[k]) || !std::isfinite(bb[k]))
+                throw Error(NLHIP_ERR_NONFINITE, "system " + std::to_string(k) +
+                                                     ": right-hand side or iterate is NaN or Inf");
+            conv[k] = !(rr[k] > (p.eps * p.eps) * bb[k]) ? 1 : 0;
+            cgrel[k] = rel[k];
+            const bool budget = uint32_t(its[k]) >= p.max_iter || outer[k] >= p.max_outer;
+            skip[k] = (conv[k] || budget) ? 1 : 0;
+            const double rn = std::sqrt(rr[k]);
+            sc[k] = skip[k] ? 0.0 : 1.0 / rn;
+            sc[B.nsys + k] = skip[k] ? 0.0 : rn;
+            active += skip[k] ? 0 : 1;
+        }
+        if (active == 0) break;
+        NLHIP_CHECK(hipMemcpyAsync(w32.skip.get(), skip.data(), B.nsys * 4, hipMemcpyHostToDevice, s));
+        NLHIP_CHECK(hipMemcpyAsync(B.scale.get(), sc.data(), sc.size() * 8, hipMemcpyHostToDevice, s));
+        NLHIP_CHECK(batch_scale_cast(w64.args, w32.args, B.scale.get(), B.grid, s));
+        run_batch<float>(B, w32, precond, float(p.inner_eps), int(inner_cap), K);
+        NLHIP_CHECK(hipMemcpyAsync(inner_its.data(), w32.iters.get(), B.nsys * 4,
+                                   hipMemcpyDeviceToHost, s));
+        NLHIP_CHECK(batch_ir_update(w64.args, w32.args, B.scale.get() + B.nsys, B.grid, s));
+        B.stream.sync();
+        bool progress = false;
+        for (uint32_t k = 0; k < B.nsys; ++k) {
+            if (skip[k]) continue;
+            its[k] += inner_its[k];
+            outer[k] += 1;
+            progress |= inner_its[k] > 0;
+        }
+        if (!progress) break;
+    }
+}
+
+Batch* batch_create(uint32_t nsys, const uint32_t* row_off, const uint32_t* rowptr,
+                    const uint32_t* colind, const double* val, uint32_t flags) {
+    if (nsys == 0 || !row_off || !rowptr) throw Error(NLHIP_ERR_INVALID, "empty batch");
+    if (row_off[0] != 0) throw Error(NLHIP_ERR_INVALID, "row_off[0] != 0");
+    const uint32_t n = row_off[nsys];
+    if (n == 0) throw Error(NLHIP_ERR_INVALID, "batch without rows");
+    for (uint32_t s = 0; s < nsys; ++s)
+        if (row_off[s + 1] <= row_off[s])
+            throw Error(NLHIP_ERR_INVALID, "system " + std::to_string(s) + " has no rows");
+    if (rowptr[0] != 0) throw Error(NLHIP_ERR_INVALID, "rowptr[0] != 0");
+    for (uint32_t i = 0; i < n; ++i)
+        if (rowptr[i + 1] < rowptr[i]) throw Error(NLHIP_ERR_INVALID, "rowptr is not monotone");
+    const uint64_t nnz = rowptr[n];
+    // every entry must stay inside its own system's diagonal block
+    int bad_col = 0, bad_val = 0;
+#pragma omp parallel for reduction(| : bad_col, bad_val) schedule(dynamic, 16)
+    for (int64_t s = 0; s < int64_t(nsys); ++s) {
+        const uint32_t lo = row_off[s], hi = row_off[s + 1];
+        for (uint64_t j = rowptr[lo]; j < rowptr[hi]; ++j) {
+            bad_col |= (colind[j] < lo || colind[j] >= hi);
+            bad_val |= !std::isfinite(val[j]);
+        }
+    }
+    if (bad_col) throw Error(NLHIP_ERR_INVALID, "an entry couples two systems (not block diagonal)");
+    if (bad_val) throw Error(NLHIP_ERR_NONFINITE, "matrix value is NaN or Inf");
+    device();
+    auto B = std::make_unique<Batch>();
+    B->nsys = nsys;
+    B->n = n;
+    B->nnz = nnz;
+    B->h_row_off.assign(row_off, row_off + nsys + 1);
+    std::vector<uint32_t> tile_off(size_t(nsys) + 1, 0), tile_sys, tile_row0;
+    for (uint32_t s = 0; s < nsys; ++s) {
+        const uint32_t rows = row_off[s + 1] - row_off[s];
+        const uint32_t nt = (rows + kTileRows - 1) / kTileRows;
+        tile_off[s + 1] = tile_off[s] + nt;
+        for (uint32_t t = 0; t < nt; ++t) {
+            tile_sys.push_back(s);
+            tile_row0.push_back(row_off[s] + t * kTileRows);
+        }
+    }
+    B->ntiles = tile_off[nsys];
+    hipStream_t st = B->stream.get();
+    B->row_off.alloc(size_t(nsys) + 1);
+    B->tile_off.alloc(size_t(nsys) + 1);
+    B->tile_sys.alloc(B->ntiles);
+    B->tile_row0.alloc(B->ntiles);
+    B->rowptr.alloc(size_t(n) + 1);
+    B->colind.alloc(nnz);
+    B->val.alloc(nnz);
+    B->dinv.alloc(n);
+    copy_in(B->row_off.get(), row_off, (size_t(nsys) + 1) * 4, st);
+    copy_in(B->tile_off.get(), tile_off.data(), tile_off.size() * 4, st);
+    copy_in(B->tile_sys.get(), tile_sys.data(), tile_sys.size() * 4, st);
+    copy_in(B->tile_row0.get(), tile_row0.data(), tile_row0.size() * 4, st);
+    copy_in(B->rowptr.get(), rowptr, (size_t(n) + 1) * 4, st);
+    copy_in(B->colind.get(), colind, nnz * 4, st);
+    copy_in(B->val.get(), val, nnz * 8, st);
+    NLHIP_CHECK(jacobi_dinv<double>(n, B->rowptr.get(), B->colind.get(), B->val.get(),
+                                    B->dinv.get(), st));
+    B->grid = uint32_t(std::max(4, device().cu_count * 4));
+    if (flags & NLHIP_SYSTEM_FP32) ensure_fp32(*B);
+    work_alloc(*B, B->w64);
+    B->stream.sync();
+    return B.release();
+}
+
+}  // namespace
+
+}  // namespace nlhip
+
+using namespace nlhip;
+
+static Batch* bat(nlhip_batch h) {
+    if (!h) throw Error(NLHIP_ERR_INVALID, "null batch handle");
+    return reinterpret_cast<Batch*>(h);
+}
+
+extern "C" {
+
+NLAPI int NLAPIENTRY nlhipBatchCreate(uint32_t nsys, const uint32_t* row_off,
+                                      const uint32_t* rowptr, const uint32_t* colind,
+                                      const double* val, uint32_t flags, nlhip_batch* out) {
+    return guarded("nlhipBatchCreate", [&] {
+        if (!out) throw Error(NLHIP_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        *out = reinterpret_cast<nlhip_batch>(batch_create(nsys, row_off, rowptr, colind, val, flags));
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipBatchDestroy(nlhip_batch b) {
+    return guarded("nlhipBatchDestroy", [&] {
+        if (!b) return;
+        Batch* B = bat(b);
+        { std::lock_guard<std::mutex> lock(B->mu); B->stream.sync(); }
+        delete B;
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipBatchInfo(nlhip_batch b, uint32_t* nsys, uint32_t* n, uint64_t* nnz,
+                                    uint32_t* tiles) {
+    return guarded("nlhipBatchInfo", [&] {
+        Batch* B = bat(b);
+        if (nsys) *nsys = B->nsys;
+        if (n) *n = B->n;
+        if (nnz) *nnz = B->nnz;
+        if (tiles) *tiles = B->ntiles;
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipBatchSetLaunch(nlhip_batch b, uint32_t workgroups) {
+    return guarded("nlhipBatchSetLaunch", [&] {
+        Batch* B = bat(b);
+        if (workgroups == 0 || workgroups > 65536)
+            throw Error(NLHIP_ERR_INVALID, "workgroups must be in [1, 65536]");
+        std::lock_guard<std::mutex> lock(B->mu);
+        B->grid = workgroups;
+        B->w64.graph.reset();
+        B->w32.graph.reset();
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipBatchSolve(nlhip_batch b, const double* rhs, double* x,
+                                     const nlhip_solve_params* params,
+                                     nlhip_solve_stats* per_system, nlhip_solve_stats* total) {
+    return guarded("nlhipBatchSolve", [&] {
+        Batch* B = bat(b);
+        if (!rhs || !x) throw Error(NLHIP_ERR_INVALID, "null b or x");
+        nlhip_solve_params p;
+        nlhipSolveParamsDefault(&p);
+        if (params) p = *params;
+        if (p.precond != NL_PRECOND_JACOBI && p.precond != NL_PRECOND_NONE)
+            throw Error(NLHIP_ERR_UNSUPPORTED, "only Jacobi or no preconditioner on the GPU");
+        std::lock_guard<std::mutex> lock(B->mu);
+        if (take_injected_failure()) throw Error(NLHIP_ERR_INJECTED, "injected failure");
+        const auto t0 = std::chrono::steady_clock::now();
+        hipStream_t s = B->stream.get();
+        copy_in(B->w64.b.get(), rhs, size_t(B->n) * 8, s);
+        copy_in(B->w64.x.get(), x, size_t(B->n) * 8, s);
+        Event e0, e1;
+        e0.record(s);
+        std::vector<int32_t> its;
+        std::vector<double> cgrel, rel;
+        std::vector<uint8_t> conv;
+        std::vector<uint32_t> outer;
+        if (p.precision == NLHIP_PREC_FP64) {
+            batch_solve_fp64(*B, p, its, cgrel, conv);
+        } else if (p.precision == NLHIP_PREC_MIXED) {
+            batch_solve_mixed(*B, p, its, cgrel, conv, outer);
+        } else {
+            throw Error(NLHIP_ERR_INVALID, "unknown precision");
+        }
+        e1.record(s);
+        e1.sync();
+        const double dev_s = 1e-3 * elapsed_ms(e0, e1);
+        true_residuals(*B, rel);
+        copy_out(x, B->w64.x.get(), size_t(B->n) * 8, s);
+        B->stream.sync();
+        const double wall_s =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        fill_stats(*B, its, rel, cgrel, conv, outer.empty() ? nullptr : &outer, per_system,
+                   total, dev_s, wall_s);
+    });
+}
+
+}  // extern "C"

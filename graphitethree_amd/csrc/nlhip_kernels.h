@@ -94,6 +94,64 @@ hipError_t cg_step(const CGArgs<T>& a, int rpw, bool precond, hipStream_t s);
 template <typename T>
 hipError_t cg_step_part(const CGArgs<T>& a, int rpw, bool precond, int part, hipStream_t s);
 
+// ---- batched systems (k_batch.hip) ----------------------------------------
+
+constexpr uint32_t kTileRows = 256;  // rows per batch tile: 4 wavefronts x 64
+
+template <typename T>
+struct BatchArgs {
+    uint32_t nsys;
+    uint32_t ntiles;
+    const uint32_t* row_off;    // nsys + 1
+    const uint32_t* tile_off;   // nsys + 1: tiles of system s
+    const uint32_t* tile_sys;   // ntiles
+    const uint32_t* tile_row0;  // ntiles: first (global) row of the tile
+    const uint32_t* rp;         // global rowptr (n + 1)
+    const uint32_t* ci;         // global column indices
+    const T* val;
+    const T* dinv;              // null: no preconditioner
+    T* x;
+    const T* b;
+    T* r;
+    T* h;
+    T* d;
+    T* q;
+    T* part_dq;                 // per tile
+    T* part_rh;
+    T* part_rr;
+    T* part_bb;
+    T* rho;                     // per system
+    T* alpha;
+    T* beta;
+    T* err;
+    T* thresh;
+    T* bb;
+    int32_t* iters;
+    int32_t* done;
+    const int32_t* skip;        // per system, may be null: start as done
+    uint32_t* act_tiles;
+    uint32_t* n_act_tiles;
+    uint32_t* act_sys;
+    uint32_t* n_act_sys;
+};
+
+template <typename T>
+hipError_t batch_start(const BatchArgs<T>& a, bool precond, T eps, int max_iter, uint32_t grid,
+                       hipStream_t s);
+template <typename T>
+hipError_t batch_iteration(const BatchArgs<T>& a, bool precond, int max_iter, uint32_t grid,
+                           hipStream_t s);
+template <typename T>
+hipError_t batch_compact(const BatchArgs<T>& a, uint32_t grid, hipStream_t s);
+// r = b - A x (FP64) and per-system |r|^2 -> err[], |b|^2 -> bb[].
+hipError_t batch_residual64(const BatchArgs<double>& a, uint32_t grid, hipStream_t s);
+// b32 = float(r64 * scale[s]), x32 = 0.
+hipError_t batch_scale_cast(const BatchArgs<double>& a64, const BatchArgs<float>& a32,
+                            const double* scale, uint32_t grid, hipStream_t s);
+// x64 += unscale[s] * x32 where a32.skip[s] == 0.
+hipError_t batch_ir_update(const BatchArgs<double>& a64, const BatchArgs<float>& a32,
+                           const double* unscale, uint32_t grid, hipStream_t s);
+
 // Mixed-precision refinement helpers.
 hipError_t ir_scale_cast(uint64_t n, const double* r, double s, float* out, hipStream_t st);
 hipError_t ir_update(uint64_t n, double* x, const float* c, double s, hipStream_t st);

@@ -473,6 +473,79 @@ System* cotan_system(const Mesh& M, double tstep, uint32_t nrhs, int rhs_kind,
     return S;
 }
 
+// A texture atlas: `count` disk-like charts with N ~ logU[Nmin, Nmax]
+// vertices each, their LSCM systems (extreme vertices pinned at their own
+// (x, y)) packed block-diagonally.
+struct Atlas {
+    std::vector<uint32_t> row_off, rowptr, colind, chart_nv;
+    std::vector<double> val, rhs;
+};
+
+Atlas* build_atlas(uint32_t count, uint64_t seed, double Nmin, double Nmax);
+
+}  // namespace
+
+extern "C" void wl_atlas_chart(uint64_t seed, uint32_t idx, double Nmin, double Nmax,
+                               uint32_t* N, uint64_t* chart_seed);
+
+namespace {
+Atlas* build_atlas(uint32_t count, uint64_t seed, double Nmin, double Nmax) {
+    std::vector<System*> sys(count, nullptr);
+    std::vector<uint32_t> nv(count, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long c = 0; c < long(count); ++c) {
+        uint32_t N;
+        uint64_t cs;
+        wl_atlas_chart(seed, uint32_t(c), Nmin, Nmax, &N, &cs);
+        Mesh* M = chart_mesh(N, cs);
+        uint32_t p[2] = {0, 0};
+        {
+            uint32_t lo = 0, hi = 0;
+            for (uint32_t v = 1; v < M->nv(); ++v) {
+                double sv = M->P[3 * size_t(v)] + M->P[3 * size_t(v) + 1];
+                if (sv < M->P[3 * size_t(lo)] + M->P[3 * size_t(lo) + 1]) lo = v;
+                if (sv > M->P[3 * size_t(hi)] + M->P[3 * size_t(hi) + 1]) hi = v;
+            }
+            p[0] = lo;
+            p[1] = hi;
+        }
+        double uv[4] = {M->P[3 * size_t(p[0])], M->P[3 * size_t(p[0]) + 1],
+                        M->P[3 * size_t(p[1])], M->P[3 * size_t(p[1]) + 1]};
+        sys[c] = lscm_system(*M, p, uv);
+        nv[c] = M->nv();
+        delete M;
+    }
+    Atlas* A = new Atlas;
+    A->row_off.push_back(0);
+    uint64_t nnz = 0;
+    for (uint32_t c = 0; c < count; ++c) {
+        A->row_off.push_back(A->row_off.back() + sys[c]->n);
+        nnz += sys[c]->colind.size();
+    }
+    const uint32_t n = A->row_off.back();
+    A->rowptr.resize(size_t(n) + 1);
+    A->colind.resize(nnz);
+    A->val.resize(nnz);
+    A->rhs.resize(n);
+    A->chart_nv = nv;
+    std::vector<uint64_t> nnz_off(count + 1, 0);
+    for (uint32_t c = 0; c < count; ++c) nnz_off[c + 1] = nnz_off[c] + sys[c]->colind.size();
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long c = 0; c < long(count); ++c) {
+        const System& S = *sys[c];
+        const uint32_t r0 = A->row_off[c];
+        const uint64_t z0 = nnz_off[c];
+        for (uint32_t i = 0; i < S.n; ++i) A->rowptr[r0 + i] = uint32_t(z0 + S.rowptr[i]);
+        for (size_t j = 0; j < S.colind.size(); ++j) {
+            A->colind[z0 + j] = r0 + S.colind[j];
+            A->val[z0 + j] = S.val[j];
+        }
+        std::copy(S.rhs.begin(), S.rhs.begin() + S.n, A->rhs.begin() + r0);
+        delete sys[c];
+    }
+    A->rowptr[n] = uint32_t(nnz);
+    return A;
+}
 }  // namespace
 
 extern "C" {
@@ -568,5 +641,29 @@ void wl_atlas_chart(uint64_t seed, uint32_t idx, double Nmin, double Nmax,
     *N = uint32_t(std::llround(std::exp(ln)));
     *chart_seed = splitmix64(s);
 }
+
+void* wl_atlas(uint32_t count, uint64_t seed, double Nmin, double Nmax) {
+    if (count == 0) return nullptr;
+    return build_atlas(count, seed, Nmin, Nmax);
+}
+uint32_t wl_atlas_count(const void* a) {
+    return uint32_t(static_cast<const Atlas*>(a)->row_off.size() - 1);
+}
+uint64_t wl_atlas_nnz(const void* a) { return static_cast<const Atlas*>(a)->colind.size(); }
+const uint32_t* wl_atlas_row_off(const void* a) {
+    return static_cast<const Atlas*>(a)->row_off.data();
+}
+const uint32_t* wl_atlas_rowptr(const void* a) {
+    return static_cast<const Atlas*>(a)->rowptr.data();
+}
+const uint32_t* wl_atlas_colind(const void* a) {
+    return static_cast<const Atlas*>(a)->colind.data();
+}
+const double* wl_atlas_val(const void* a) { return static_cast<const Atlas*>(a)->val.data(); }
+const double* wl_atlas_rhs(const void* a) { return static_cast<const Atlas*>(a)->rhs.data(); }
+const uint32_t* wl_atlas_chart_nv(const void* a) {
+    return static_cast<const Atlas*>(a)->chart_nv.data();
+}
+void wl_atlas_free(void* a) { delete static_cast<Atlas*>(a); }
 
 }  // extern "C"

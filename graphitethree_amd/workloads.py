@@ -53,6 +53,16 @@ def lib() -> C.CDLL:
             "wl_sys_free": (None, [C.c_void_p]),
             "wl_atlas_chart": (None, [C.c_uint64, C.c_uint32, C.c_double, C.c_double,
                                       _u32p, C.POINTER(C.c_uint64)]),
+            "wl_atlas": (C.c_void_p, [C.c_uint32, C.c_uint64, C.c_double, C.c_double]),
+            "wl_atlas_count": (C.c_uint32, [C.c_void_p]),
+            "wl_atlas_nnz": (C.c_uint64, [C.c_void_p]),
+            "wl_atlas_row_off": (_u32p, [C.c_void_p]),
+            "wl_atlas_rowptr": (_u32p, [C.c_void_p]),
+            "wl_atlas_colind": (_u32p, [C.c_void_p]),
+            "wl_atlas_val": (_f64p, [C.c_void_p]),
+            "wl_atlas_rhs": (_f64p, [C.c_void_p]),
+            "wl_atlas_chart_nv": (_u32p, [C.c_void_p]),
+            "wl_atlas_free": (None, [C.c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -196,6 +206,59 @@ def atlas_chart_params(seed: int, idx: int, nmin: float = 50, nmax: float = 5000
     N, s = C.c_uint32(), C.c_uint64()
     lib().wl_atlas_chart(seed, idx, nmin, nmax, C.byref(N), C.byref(s))
     return int(N.value), int(s.value)
+
+
+@dataclass
+class Atlas:
+    """Charts packed block-diagonally: chart c owns rows [row_off[c], row_off[c+1])
+    of the global CRS (global column indices)."""
+    row_off: np.ndarray
+    rowptr: np.ndarray
+    colind: np.ndarray
+    val: np.ndarray
+    b: np.ndarray
+    chart_nv: np.ndarray
+
+    @property
+    def count(self) -> int:
+        return len(self.row_off) - 1
+
+    @property
+    def n(self) -> int:
+        return int(self.row_off[-1])
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowptr[-1])
+
+    def chart(self, c: int) -> SystemCRS:
+        """Chart c as a standalone system (local indices)."""
+        r0, r1 = int(self.row_off[c]), int(self.row_off[c + 1])
+        z0, z1 = int(self.rowptr[r0]), int(self.rowptr[r1])
+        rp = (self.rowptr[r0:r1 + 1].astype(np.int64) - z0).astype(np.uint32)
+        ci = (self.colind[z0:z1].astype(np.int64) - r0).astype(np.uint32)
+        return SystemCRS(rp, ci, self.val[z0:z1].copy(), self.b[None, r0:r1].copy())
+
+
+def atlas(count: int, seed: int = 1, nmin: float = 50, nmax: float = 50000) -> Atlas:
+    """`count` disk-like LSCM charts, N ~ logU[nmin, nmax] vertices each."""
+    L = lib()
+    h = L.wl_atlas(count, seed, nmin, nmax)
+    if not h:
+        raise ValueError("atlas construction failed")
+    try:
+        m = L.wl_atlas_count(h)
+        row_off = _arr(L.wl_atlas_row_off(h), m + 1, np.uint32)
+        n = int(row_off[-1])
+        nnz = int(L.wl_atlas_nnz(h))
+        out = Atlas(row_off, _arr(L.wl_atlas_rowptr(h), n + 1, np.uint32),
+                    _arr(L.wl_atlas_colind(h), nnz, np.uint32),
+                    _arr(L.wl_atlas_val(h), nnz, np.float64),
+                    _arr(L.wl_atlas_rhs(h), n, np.float64),
+                    _arr(L.wl_atlas_chart_nv(h), m, np.uint32))
+    finally:
+        L.wl_atlas_free(h)
+    return out
 
 
 def lscm_grid_size(k: int):

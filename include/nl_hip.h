@@ -116,6 +116,33 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
 NLAPI int NLAPIENTRY nlhipProfileCG(nlhip_system s, int fp32, uint32_t iters,
                                     double* seconds);
 
+/* ---- batched systems (texture-atlas charts) ------------------------------
+ * Many independent systems solved together (mesh_make_atlas solves one LSCM
+ * system per chart, src/lib/OGF/mesh/commands/mesh_grob_surface_commands.cpp:
+ * 724-730).  The systems come packed as one block-diagonal CRS: system s owns
+ * rows and columns [row_off[s], row_off[s+1]); rowptr and colind use global
+ * indices.  Every entry must stay inside its own system's block.  Each
+ * system's result is bitwise independent of the other systems in the batch.
+ */
+typedef struct nlhip_batch_s* nlhip_batch;
+
+NLAPI int NLAPIENTRY nlhipBatchCreate(uint32_t nsys, const uint32_t* row_off,
+                                      const uint32_t* rowptr, const uint32_t* colind,
+                                      const double* val, uint32_t flags,
+                                      nlhip_batch* out);
+NLAPI int NLAPIENTRY nlhipBatchDestroy(nlhip_batch b);
+NLAPI int NLAPIENTRY nlhipBatchInfo(nlhip_batch b, uint32_t* nsys, uint32_t* n,
+                                    uint64_t* nnz, uint32_t* tiles);
+/* Workgroups of the persistent grid that walks the active tiles. */
+NLAPI int NLAPIENTRY nlhipBatchSetLaunch(nlhip_batch b, uint32_t workgroups);
+/* Solves every system; b and x are the concatenated vectors (x holds the
+ * initial guesses).  per_system (nsys entries) and total may be null; in
+ * total, iterations is the largest count and outer_iterations the sum. */
+NLAPI int NLAPIENTRY nlhipBatchSolve(nlhip_batch b, const double* rhs, double* x,
+                                     const nlhip_solve_params* params,
+                                     nlhip_solve_stats* per_system,
+                                     nlhip_solve_stats* total);
+
 NLAPI const char* NLAPIENTRY nlhipLastError(void);
 /* Test hook: the next `count` solves fail with NLHIP_ERR_INJECTED before
  * touching the device (other requests are unaffected). */

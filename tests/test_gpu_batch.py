@@ -1,0 +1,108 @@
+"""Batched solves (texture-atlas charts, one packed block-diagonal system):
+each chart matches the oracle's PCG on that chart alone, a chart's result is
+bitwise the same alone and in any mix or launch shape, and the FP32-inner /
+FP64-refined variant reaches the FP64 residual bar."""
+import numpy as np
+import pytest
+
+from graphitethree_amd import capi
+from graphitethree_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+REL_DIFF = 1e-8
+
+
+def same_iters(a, b) -> bool:
+    a, b = int(a), int(b)
+    return abs(a - b) <= max(2, 0.05 * max(a, b))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def small_atlas():
+    return W.atlas(12, seed=21, nmin=50, nmax=3000)
+
+
+def test_batch_matches_oracle_per_chart(hip, small_atlas):
+    A = small_atlas
+    B = capi.Batch([A.chart(c) for c in range(A.count)])
+    x, per, tot = B.solve(A.b, eps=1e-10, max_iter=5000)
+    xs = B.split(x)
+    for c in range(A.count):
+        S = A.chart(c)
+        xr, it, _ = O.pcg(O.CRS(S.rowptr, S.colind, S.val), S.b, eps=1e-10, max_iter=5000)
+        print(f"chart {c}: n={S.n} it gpu/cpu {per[c].iterations}/{it} "
+              f"rel_res {per[c].rel_residual:.2e} diff {rel(xs[c], xr):.2e}")
+        assert same_iters(per[c].iterations, it)
+        assert per[c].converged == 1 and per[c].rel_residual <= 1.01e-10
+        assert rel(xs[c], xr) <= REL_DIFF
+    assert tot.converged == 1
+    assert tot.iterations == max(p.iterations for p in per)
+
+
+def test_batch_alone_equals_in_mix(hip, small_atlas):
+    A = small_atlas
+    full = capi.Batch([A.chart(c) for c in range(A.count)])
+    x, per, _ = full.solve(A.b)
+    xs = full.split(x)
+    for c in (0, 5, 11):
+        S = A.chart(c)
+        alone = capi.Batch([S])
+        xa, pa, _ = alone.solve(S.b)
+        assert np.array_equal(xa, xs[c]), c
+        assert pa[0].iterations == per[c].iterations
+    # a different mix: reversed order and a subset
+    sub = [A.chart(c) for c in (11, 3, 0)]
+    Bs = capi.Batch(sub)
+    xb, _, _ = Bs.solve(np.concatenate([s.b for s in sub]))
+    parts = Bs.split(xb)
+    assert np.array_equal(parts[0], xs[11]) and np.array_equal(parts[2], xs[0])
+
+
+def test_batch_launch_shape_does_not_change_bits(hip, small_atlas):
+    A = small_atlas
+    B = capi.Batch([A.chart(c) for c in range(A.count)])
+    x0, _, _ = B.solve(A.b)
+    for wg in (1, 7, 3000):
+        B.set_launch(wg)
+        x1, _, _ = B.solve(A.b)
+        assert np.array_equal(x0, x1), wg
+
+
+def test_batch_mixed_precision(hip, small_atlas):
+    A = small_atlas
+    B = capi.Batch([A.chart(c) for c in range(A.count)], fp32=True)
+    x64, p64, _ = B.solve(A.b, eps=1e-10, max_iter=20000)
+    xm, pm, tm = B.solve(A.b, precision="mixed", eps=1e-10, max_iter=200000, inner_eps=1e-4)
+    for c, (a, b) in enumerate(zip(B.split(xm), B.split(x64))):
+        S = A.chart(c)
+        r = O.spmv(O.CRS(S.rowptr, S.colind, S.val), a) - S.b
+        assert np.linalg.norm(r) / np.linalg.norm(S.b) <= 1e-10, c
+        assert pm[c].converged == 1 and pm[c].outer_iterations >= 1
+        assert rel(a, b) < 1e-6
+    assert tm.converged == 1
+
+
+def test_batch_rejects_bad_input(hip):
+    S1 = W.lscm_grid(6, seed=1)
+    S2 = W.lscm_grid(7, seed=2)
+    B = capi.Batch([S1, S2])
+    b = np.concatenate([S1.b, S2.b])
+    b[S1.n + 3] = np.inf
+    with pytest.raises(capi.NLHIPError) as e:
+        B.solve(b)
+    assert e.value.code == capi.NLHIP_ERR_NONFINITE
+    # an entry coupling system 0 to system 1
+    with pytest.raises(capi.NLHIPError) as e:
+        row_off = np.array([0, 1, 2], dtype=np.uint32)
+        h = capi.C.c_void_p()
+        rp = np.array([0, 2, 3], dtype=np.uint32)
+        ci = np.array([0, 1, 1], dtype=np.uint32)
+        va = np.array([1.0, 0.5, 1.0])
+        capi.check(capi.lib().nlhipBatchCreate(2, capi.ptr(row_off), capi.ptr(rp), capi.ptr(ci),
+                                               capi.ptr(va), 0, capi.C.byref(h)), "create")
+    assert e.value.code == capi.NLHIP_ERR_INVALID
