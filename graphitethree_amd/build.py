@@ -32,7 +32,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"
 
 NLHIP_HIP = ["k_spmv.hip", "k_blas.hip", "k_cg.hip", "k_batch.hip", "k_spmm.hip"]
 NLHIP_CPP = ["nl_api.cpp", "nlhip_device.cpp", "nlhip_solver.cpp",
-             "nlhip_batch.cpp", "nlhip_server.cpp", "nlhip_autotune.cpp"]
+             "nlhip_batch.cpp", "nlhip_multi.cpp", "nlhip_server.cpp"]
 
 
 def _run(cmd):

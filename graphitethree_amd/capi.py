@@ -153,6 +153,8 @@ SIGNATURES = {
     "nlhipSystemGetLaunch": (C.c_int, [_vp, C.POINTER(C.c_int), _u32p]),
     "nlhipSolve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
                              C.POINTER(SolveStats)]),
+    "nlhipSolveMulti": (C.c_int, [_vp, C.c_uint32, _vp, _vp, C.POINTER(SolveParams),
+                                  C.POINTER(SolveStats), C.POINTER(SolveStats)]),
     "nlhipSpMV": (C.c_int, [_vp, _vp, _vp]),
     "nlhipBenchSpMV": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
     "nlhipProfileCG": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
@@ -273,6 +275,18 @@ class System:
         st = SolveStats()
         check(lib().nlhipSolve(self._h, ptr(b), ptr(x), C.byref(p), C.byref(st)), "nlhipSolve")
         return x, st
+
+    def solve_multi(self, B, X0=None, **params):
+        """B: (k, n) right-hand sides (row j = system j).  Returns (X, per, total)."""
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        k = B.shape[0]
+        X = np.zeros_like(B) if X0 is None else np.array(X0, dtype=np.float64, copy=True)
+        p = default_params(**params)
+        per = (SolveStats * k)()
+        tot = SolveStats()
+        check(lib().nlhipSolveMulti(self._h, k, ptr(B), ptr(X), C.byref(p), per, C.byref(tot)),
+              "nlhipSolveMulti")
+        return X, list(per), tot
 
     def spmv(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -686,6 +686,33 @@ bool solve_context(Context& C) {
     const NLuint max_iter = C.max_iterations ? C.max_iterations : 5 * n;
     std::vector<double> b(n), x(n);
     bool ok = true;
+    if (solver == NL_CG && C.nb_systems > 1 && C.nb_systems <= 32) {
+        // NL_NB_SYSTEMS right-hand sides share every pass over the matrix
+        const NLuint k = C.nb_systems;
+        std::vector<double> X(size_t(n) * k);
+        for (NLuint j = 0; j < k; ++j) {
+            const double* vals = &C.values[size_t(j) * C.nb_variables];
+            for (NLuint v = 0; v < C.nb_variables; ++v)
+                if (C.free_index[v] != ~NLuint(0)) X[size_t(j) * n + C.free_index[v]] = vals[v];
+        }
+        nlhip_solve_params p;
+        nlhipSolveParamsDefault(&p);
+        p.eps = C.threshold;
+        p.max_iter = max_iter;
+        p.precond = (C.preconditioner == NL_PRECOND_JACOBI) ? NL_PRECOND_JACOBI : NL_PRECOND_NONE;
+        std::vector<nlhip_solve_stats> per(k);
+        nlhip_solve_stats tot;
+        solve_multi(*S, k, C.rhs.data(), X.data(), p, per.data(), &tot);
+        for (NLuint j = 0; j < k; ++j) {
+            C.used_iterations += per[j].iterations;
+            double* out = &C.values[size_t(j) * C.nb_variables];
+            for (NLuint v = 0; v < C.nb_variables; ++v)
+                if (C.free_index[v] != ~NLuint(0)) out[v] = X[size_t(j) * n + C.free_index[v]];
+        }
+        C.error = tot.rel_residual;
+        C.elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return true;
+    }
     for (NLuint k = 0; k < C.nb_systems; ++k) {
         const double* vals = &C.values[size_t(k) * C.nb_variables];
         for (NLuint v = 0; v < C.nb_variables; ++v)

@@ -244,6 +244,22 @@ struct CGWorkspace {
     void ensure(uint32_t n_, uint32_t P_);
 };
 
+// Workspace of a multi right-hand-side solve (k_spmm.hip), n x k interleaved.
+template <typename T>
+struct MultiWork {
+    uint32_t n = 0, k = 0, P = 0;
+    DevBuf<T> x, b, r, h, d, q, part, col;  // part: 4 x P x k; col: 6 x k
+    DevBuf<int32_t> iters, done, skip;
+    DevBuf<uint32_t> n_active;
+    PinnedBuf<uint32_t> host;               // two polling slots
+    Graph graph;
+    MultiArgs<T> args{};
+    MultiArgs<T> graph_args{};
+    uint32_t graph_iters = 0;
+    int graph_max_iter = -1;
+    bool graph_precond = false;
+};
+
 // A CRS matrix resident on the device, with its Jacobi inverse diagonal and
 // (optionally) FP32 copies, its own stream and reusable solve workspaces.
 struct System {
@@ -259,6 +275,9 @@ struct System {
     CGWorkspace<double> ws64;
     CGWorkspace<float> ws32;
     DevBuf<double> scratch;  // residual vector + partials for refinement
+    MultiWork<double> m64;   // multi right-hand-side workspaces
+    MultiWork<float> m32;
+    DevBuf<double> mscale;
     // Jacobi inverse diagonal supplied by a separate preconditioner object
     // (nlSolveSystemIterative's P); used instead of dinv while set.
     const double* dinv_override = nullptr;
@@ -283,6 +302,11 @@ double true_rel_residual(System& S, const double* b_dev, const double* x_dev);
 // Full solve with staging of host/device b and x.
 void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
            nlhip_solve_stats& st);
+
+// k right-hand sides at once (B, X column-major n x k, host or device).
+// per (k entries) and total may be null.
+void solve_multi(System& S, uint32_t k, const double* B, double* X,
+                 const nlhip_solve_params& p, nlhip_solve_stats* per, nlhip_solve_stats* total);
 
 // Device BiCGSTAB on BLAS-1 kernels (NL_BICGSTAB); returns iterations.
 uint32_t bicgstab_device(System& S, const double* dinv, const double* b_dev, double* x_dev,

@@ -152,6 +152,55 @@ hipError_t batch_scale_cast(const BatchArgs<double>& a64, const BatchArgs<float>
 hipError_t batch_ir_update(const BatchArgs<double>& a64, const BatchArgs<float>& a32,
                            const double* unscale, uint32_t grid, hipStream_t s);
 
+// ---- multiple right-hand sides (k_spmm.hip) ---------------------------------
+
+template <typename T>
+struct MultiArgs {
+    uint32_t m;               // rows
+    uint32_t k;               // right-hand sides (<= 32)
+    uint32_t G;               // lanes per row: k rounded up to a power of two
+    uint32_t P;               // workgroups of the row-shaped launches
+    uint32_t rows_per_block;
+    uint32_t pad;
+    const uint32_t* rp;
+    const uint32_t* ci;
+    const T* val;
+    const T* dinv;            // per row; null: no preconditioner
+    T* x;                     // n x k interleaved
+    const T* b;
+    T* r;
+    T* h;
+    T* d;
+    T* q;
+    T* part_dq;               // P x k
+    T* part_rh;
+    T* part_rr;
+    T* part_bb;
+    T* rho;                   // per column
+    T* alpha;
+    T* beta;
+    T* err;
+    T* thresh;
+    T* bb;
+    int32_t* iters;
+    int32_t* done;
+    const int32_t* skip;      // may be null
+    uint32_t* n_active;       // columns still iterating
+};
+
+template <typename T>
+hipError_t multi_start(const MultiArgs<T>& a, bool precond, T eps, int max_iter, hipStream_t s);
+template <typename T>
+hipError_t multi_iteration(const MultiArgs<T>& a, bool precond, int max_iter, hipStream_t s);
+template <typename T>
+hipError_t multi_layout(uint32_t n, uint32_t k, const T* src, T* dst, bool to_interleaved,
+                        hipStream_t s);
+hipError_t multi_residual64(const MultiArgs<double>& a, hipStream_t s);
+hipError_t multi_scale_cast(const MultiArgs<double>& a64, const MultiArgs<float>& a32,
+                            const double* scale, hipStream_t s);
+hipError_t multi_ir_update(const MultiArgs<double>& a64, const MultiArgs<float>& a32,
+                           const double* unscale, hipStream_t s);
+
 // Mixed-precision refinement helpers.
 hipError_t ir_scale_cast(uint64_t n, const double* r, double s, float* out, hipStream_t st);
 hipError_t ir_update(uint64_t n, double* x, const float* c, double s, hipStream_t st);

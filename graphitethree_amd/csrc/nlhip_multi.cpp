@@ -1,0 +1,289 @@
+// Multiple right-hand sides of one resident system (OpenNL's NL_NB_SYSTEMS,
+// reached through mesh_smooth at src/lib/OGF/mesh/commands/
+// mesh_grob_surface_commands.cpp:703): host side of k_spmm.hip.
+//
+// The k recurrences run together and share every pass over the matrix; each
+// column keeps its own scalars and stopping test, so column j follows the
+// same iteration as a single-right-hand-side solve of that column.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "nlhip_internal.h"
+
+namespace nlhip {
+
+namespace {
+
+uint32_t pow2_ceil(uint32_t k) {
+    uint32_t g = 1;
+    while (g < k) g <<= 1;
+    return g;
+}
+
+template <typename T>
+void multi_alloc(System& S, MultiWork<T>& w, uint32_t k) {
+    const uint32_t P = S.grid;
+    if (w.n == S.n && w.k == k && w.P == P && w.x.get()) return;
+    w.graph.reset();
+    w.n = S.n;
+    w.k = k;
+    w.P = P;
+    const size_t nk = size_t(S.n) * k;
+    w.x.alloc(nk);
+    w.b.alloc(nk);
+    w.r.alloc(nk);
+    w.h.alloc(nk);
+    w.d.alloc(nk);
+    w.q.alloc(nk);
+    w.part.alloc(4 * size_t(P) * k);
+    w.col.alloc(6 * size_t(k));
+    w.iters.alloc(k);
+    w.done.alloc(k);
+    w.skip.alloc(k);
+    w.n_active.alloc(1);
+    w.host.ensure(2);
+    MultiArgs<T>& a = w.args;
+    a = MultiArgs<T>{};
+    a.m = S.n;
+    a.k = k;
+    a.G = pow2_ceil(k);
+    a.P = P;
+    const uint32_t rows_per_pass = 256u / a.G;
+    uint64_t rpb = (uint64_t(S.n) + P - 1) / P;
+    rpb = (rpb + rows_per_pass - 1) / rows_per_pass * rows_per_pass;
+    a.rows_per_block = uint32_t(std::max<uint64_t>(rpb, rows_per_pass));
+    a.rp = S.rowptr.get();
+    a.ci = S.colind.get();
+    a.x = w.x.get();
+    a.b = w.b.get();
+    a.r = w.r.get();
+    a.h = w.h.get();
+    a.d = w.d.get();
+    a.q = w.q.get();
+    a.part_dq = w.part.get();
+    a.part_rh = w.part.get() + size_t(P) * k;
+    a.part_rr = w.part.get() + 2 * size_t(P) * k;
+    a.part_bb = w.part.get() + 3 * size_t(P) * k;
+    a.rho = w.col.get();
+    a.alpha = w.col.get() + k;
+    a.beta = w.col.get() + 2 * size_t(k);
+    a.err = w.col.get() + 3 * size_t(k);
+    a.thresh = w.col.get() + 4 * size_t(k);
+    a.bb = w.col.get() + 5 * size_t(k);
+    a.iters = w.iters.get();
+    a.done = w.done.get();
+    a.skip = nullptr;
+    a.n_active = w.n_active.get();
+}
+
+template <typename T>
+void run_multi(System& S, MultiWork<T>& w, bool precond, T eps, int max_iter, uint32_t K) {
+    hipStream_t s = S.stream.get();
+    NLHIP_CHECK(multi_start<T>(w.args, precond, eps, max_iter, s));
+    if (!w.graph.ready() || w.graph_iters != K || w.graph_precond != precond ||
+        w.graph_max_iter != max_iter ||
+        std::memcmp(&w.graph_args, &w.args, sizeof(MultiArgs<T>)) != 0) {
+        w.graph.capture(s, [&] {
+            for (uint32_t i = 0; i < K; ++i)
+                NLHIP_CHECK(multi_iteration<T>(w.args, precond, max_iter, s));
+        });
+        w.graph_iters = K;
+        w.graph_precond = precond;
+        w.graph_max_iter = max_iter;
+        w.graph_args = w.args;
+    }
+    Event ev[2];
+    uint32_t* host = w.host.get();
+    const uint64_t max_graphs = uint64_t(std::max(max_iter, 0)) / K + 4;
+    uint64_t launched = 0;
+    for (;;) {
+        w.graph.launch(s);
+        const int slot = int(launched & 1);
+        NLHIP_CHECK(hipMemcpyAsync(&host[slot], w.n_active.get(), sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, s));
+        ev[slot].record(s);
+        ++launched;
+        if (launched >= 2) {
+            const int prev = int((launched - 2) & 1);
+            ev[prev].sync();
+            if (host[prev] == 0) break;
+        }
+        if (launched > max_graphs) {
+            S.stream.sync();
+            if (host[slot] == 0) break;
+            throw Error(NLHIP_ERR_HIP, "multi right-hand-side CG did not reach its stopping test");
+        }
+    }
+    S.stream.sync();
+}
+
+uint32_t multi_check_every(const System& S, uint32_t k) {
+    const double bytes = 12.0 * double(S.nnz) + 108.0 * double(S.n) * k;
+    const double iter_s = std::max(bytes / 4.0e12, 2.0e-5);
+    return uint32_t(std::clamp(2.0e-3 / iter_s, 4.0, 64.0));
+}
+
+// FP64 |b_j - A x_j|^2 and |b_j|^2 for every column into rr, bb.
+void multi_norms(System& S, uint32_t k, std::vector<double>& rr, std::vector<double>& bb) {
+    MultiArgs<double> a = S.m64.args;
+    a.val = S.val.get();
+    NLHIP_CHECK(multi_residual64(a, S.stream.get()));
+    rr.resize(k);
+    bb.resize(k);
+    NLHIP_CHECK(hipMemcpyAsync(rr.data(), a.err, k * 8, hipMemcpyDeviceToHost, S.stream.get()));
+    NLHIP_CHECK(hipMemcpyAsync(bb.data(), a.bb, k * 8, hipMemcpyDeviceToHost, S.stream.get()));
+    S.stream.sync();
+}
+
+}  // namespace
+
+void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_solve_params& p,
+                 nlhip_solve_stats* per, nlhip_solve_stats* total) {
+    if (k == 0 || k > 32) throw Error(NLHIP_ERR_UNSUPPORTED, "1 to 32 right-hand sides");
+    if (p.precond != NL_PRECOND_JACOBI && p.precond != NL_PRECOND_NONE)
+        throw Error(NLHIP_ERR_UNSUPPORTED, "only Jacobi or no preconditioner on the GPU");
+    std::lock_guard<std::mutex> lock(S.mu);
+    if (take_injected_failure()) throw Error(NLHIP_ERR_INJECTED, "injected failure");
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = S.stream.get();
+    multi_alloc(S, S.m64, k);
+    MultiWork<double>& w = S.m64;
+    const size_t nk = size_t(S.n) * k;
+    // stage column-major B and X through q, then interleave
+    copy_in(w.q.get(), B, nk * 8, s);
+    NLHIP_CHECK(multi_layout<double>(S.n, k, w.q.get(), w.b.get(), true, s));
+    copy_in(w.q.get(), X, nk * 8, s);
+    NLHIP_CHECK(multi_layout<double>(S.n, k, w.q.get(), w.x.get(), true, s));
+    const bool precond = p.precond == NL_PRECOND_JACOBI;
+    const uint32_t K = p.check_every ? p.check_every : multi_check_every(S, k);
+    Event e0, e1;
+    e0.record(s);
+    std::vector<int32_t> its(k, 0);
+    std::vector<uint32_t> outer(k, 0);
+    std::vector<double> cgrel(k, 0.0);
+    std::vector<uint8_t> conv(k, 0);
+    if (p.precision == NLHIP_PREC_FP64) {
+        w.args.val = S.val.get();
+        w.args.dinv = precond ? S.dinv.get() : nullptr;
+        w.args.skip = nullptr;
+        run_multi<double>(S, w, precond, p.eps, int(p.max_iter), K);
+        std::vector<double> err(k), thr(k), bb(k);
+        NLHIP_CHECK(hipMemcpyAsync(its.data(), w.iters.get(), k * 4, hipMemcpyDeviceToHost, s));
+        NLHIP_CHECK(hipMemcpyAsync(err.data(), w.args.err, k * 8, hipMemcpyDeviceToHost, s));
+        NLHIP_CHECK(hipMemcpyAsync(thr.data(), w.args.thresh, k * 8, hipMemcpyDeviceToHost, s));
+        NLHIP_CHECK(hipMemcpyAsync(bb.data(), w.args.bb, k * 8, hipMemcpyDeviceToHost, s));
+        S.stream.sync();
+        for (uint32_t j = 0; j < k; ++j) {
+            if (!std::isfinite(err[j]) || !std::isfinite(bb[j]))
+                throw Error(NLHIP_ERR_NONFINITE, "right-hand side or iterate is NaN or Inf");
+            cgrel[j] = bb[j] > 0 ? std::sqrt(err[j] / bb[j]) : std::sqrt(err[j]);
+            conv[j] = !(err[j] > thr[j]) ? 1 : 0;
+        }
+    } else if (p.precision == NLHIP_PREC_MIXED) {
+        S.ensure_fp32();
+        multi_alloc(S, S.m32, k);
+        MultiWork<float>& w32 = S.m32;
+        w32.args.val = S.val32.get();
+        w32.args.dinv = precond ? S.dinv32.get() : nullptr;
+        w32.args.skip = w32.skip.get();
+        S.mscale.ensure(2 * size_t(k));
+        const uint32_t inner_cap = p.inner_max_iter ? p.inner_max_iter : p.max_iter;
+        std::vector<int32_t> skip(k), inner(k);
+        std::vector<double> sc(2 * size_t(k));
+        for (;;) {
+            std::vector<double> rr, bb;
+            multi_norms(S, k, rr, bb);  // r64 = B - A X in w.r
+            uint32_t active = 0;
+            for (uint32_t j = 0; j < k; ++j) {
+                if (!std::isfinite(rr[j]) || !std::isfinite(bb[j]))
+                    throw Error(NLHIP_ERR_NONFINITE, "right-hand side or iterate is NaN or Inf");
+                conv[j] = !(rr[j] > (p.eps * p.eps) * bb[j]) ? 1 : 0;
+                cgrel[j] = bb[j] > 0 ? std::sqrt(rr[j] / bb[j]) : std::sqrt(rr[j]);
+                const bool budget = uint32_t(its[j]) >= p.max_iter || outer[j] >= p.max_outer;
+                skip[j] = (conv[j] || budget) ? 1 : 0;
+                const double rn = std::sqrt(rr[j]);
+                sc[j] = skip[j] ? 0.0 : 1.0 / rn;
+                sc[k + j] = skip[j] ? 0.0 : rn;
+                active += skip[j] ? 0 : 1;
+            }
+            if (active == 0) break;
+            NLHIP_CHECK(hipMemcpyAsync(w32.skip.get(), skip.data(), k * 4, hipMemcpyHostToDevice, s));
+            NLHIP_CHECK(hipMemcpyAsync(S.mscale.get(), sc.data(), sc.size() * 8,
+                                       hipMemcpyHostToDevice, s));
+            NLHIP_CHECK(multi_scale_cast(w.args, w32.args, S.mscale.get(), s));
+            run_multi<float>(S, w32, precond, float(p.inner_eps), int(inner_cap), K);
+            NLHIP_CHECK(hipMemcpyAsync(inner.data(), w32.iters.get(), k * 4, hipMemcpyDeviceToHost, s));
+            NLHIP_CHECK(multi_ir_update(w.args, w32.args, S.mscale.get() + k, s));
+            S.stream.sync();
+            bool progress = false;
+            for (uint32_t j = 0; j < k; ++j) {
+                if (skip[j]) continue;
+                its[j] += inner[j];
+                outer[j] += 1;
+                progress |= inner[j] > 0;
+            }
+            if (!progress) break;
+        }
+    } else {
+        throw Error(NLHIP_ERR_INVALID, "unknown precision");
+    }
+    e1.record(s);
+    e1.sync();
+    const double dev_s = 1e-3 * elapsed_ms(e0, e1);
+    std::vector<double> rr, bb;
+    multi_norms(S, k, rr, bb);
+    NLHIP_CHECK(multi_layout<double>(S.n, k, w.x.get(), w.q.get(), false, s));
+    copy_out(X, w.q.get(), nk * 8, s);
+    S.stream.sync();
+    const double wall_s =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    uint32_t it_max = 0, nconv = 0;
+    double worst = 0.0;
+    for (uint32_t j = 0; j < k; ++j) {
+        const double rel = bb[j] > 0 ? std::sqrt(rr[j] / bb[j]) : std::sqrt(rr[j]);
+        it_max = std::max(it_max, uint32_t(its[j]));
+        nconv += conv[j];
+        worst = std::max(worst, rel);
+        if (per) {
+            nlhip_solve_stats& st = per[j];
+            std::memset(&st, 0, sizeof(st));
+            st.iterations = uint32_t(its[j]);
+            st.outer_iterations = outer[j];
+            st.converged = conv[j];
+            st.rel_residual = rel;
+            st.cg_rel_residual = cgrel[j];
+            st.device_seconds = dev_s;
+            st.wall_seconds = wall_s;
+        }
+    }
+    if (total) {
+        std::memset(total, 0, sizeof(*total));
+        total->iterations = it_max;
+        total->converged = nconv == k ? 1 : 0;
+        total->rel_residual = worst;
+        total->device_seconds = dev_s;
+        total->wall_seconds = wall_s;
+    }
+}
+
+}  // namespace nlhip
+
+using namespace nlhip;
+
+extern "C" {
+
+NLAPI int NLAPIENTRY nlhipSolveMulti(nlhip_system s, uint32_t k, const double* B, double* X,
+                                     const nlhip_solve_params* params,
+                                     nlhip_solve_stats* per_rhs, nlhip_solve_stats* total) {
+    return guarded("nlhipSolveMulti", [&] {
+        if (!s) throw Error(NLHIP_ERR_INVALID, "null system handle");
+        if (!B || !X) throw Error(NLHIP_ERR_INVALID, "null B or X");
+        nlhip_solve_params p;
+        nlhipSolveParamsDefault(&p);
+        if (params) p = *params;
+        solve_multi(*reinterpret_cast<System*>(s), k, B, X, p, per_rhs, total);
+    });
+}
+
+}  // extern "C"

@@ -103,6 +103,15 @@ NLAPI int NLAPIENTRY nlhipSystemGetLaunch(nlhip_system s, int* rows_per_wave,
 NLAPI int NLAPIENTRY nlhipSolve(nlhip_system s, const double* b, double* x,
                                 const nlhip_solve_params* params,
                                 nlhip_solve_stats* stats);
+/* k right-hand sides at once (1 <= k <= 32), sharing every pass over the
+ * matrix: B and X are column-major n x k (system j at B + j n, the layout of
+ * OpenNL's NL_NB_SYSTEMS); X holds the initial guesses.  Each column has its
+ * own scalars and stopping test.  per_rhs (k entries) and total may be null;
+ * total.iterations is the largest count. */
+NLAPI int NLAPIENTRY nlhipSolveMulti(nlhip_system s, uint32_t k, const double* B,
+                                     double* X, const nlhip_solve_params* params,
+                                     nlhip_solve_stats* per_rhs,
+                                     nlhip_solve_stats* total);
 /* y = A x (FP64), host or device pointers. */
 NLAPI int NLAPIENTRY nlhipSpMV(nlhip_system s, const double* x, double* y);
 /* Times `reps` back-to-back FP64 (fp32 = 0) or FP32 SpMVs on device-resident

@@ -1,0 +1,110 @@
+"""Multiple right-hand sides (OpenNL's NL_NB_SYSTEMS; mesh_smooth's x, y, z):
+every column matches the oracle's single-right-hand-side PCG on that column."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from graphitethree_amd import capi
+from graphitethree_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+REL_DIFF = 1e-8
+
+
+def same_iters(a, b) -> bool:
+    a, b = int(a), int(b)
+    return abs(a - b) <= max(2, 0.05 * max(a, b))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("k", [1, 3, 5, 8, 17, 32])
+def test_multi_rhs_matches_oracle_columns(hip, k):
+    Sw = W.lscm_grid(18, seed=3)
+    rng = np.random.default_rng(k)
+    B = rng.standard_normal((k, Sw.n))
+    S = capi.System.from_crs(Sw)
+    X, per, tot = S.solve_multi(B, eps=1e-10, max_iter=5000)
+    A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+    for j in range(k):
+        xr, it, _ = O.pcg(A, B[j], eps=1e-10, max_iter=5000)
+        assert same_iters(per[j].iterations, it), (j, per[j].iterations, it)
+        assert per[j].converged == 1 and per[j].rel_residual <= 1.01e-10
+        assert rel(X[j], xr) <= REL_DIFF, j
+    assert tot.iterations == max(p.iterations for p in per)
+
+
+def test_multi_rhs_smoothing_xyz(hip):
+    mesh = W.grid_mesh(60, seed=4)
+    Sw = W.cotan_system(mesh, t=1e-3, nrhs=3, rhs="xyz")
+    S = capi.System.from_crs(Sw)
+    X, per, tot = S.solve_multi(Sw.rhs, eps=1e-10, max_iter=1000)
+    A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+    for j in range(3):
+        xr, it, _ = O.pcg(A, Sw.rhs[j], eps=1e-10, max_iter=1000)
+        assert same_iters(per[j].iterations, it)
+        assert rel(X[j], xr) <= REL_DIFF
+        xs, st = S.solve(Sw.rhs[j], eps=1e-10, max_iter=1000)
+        assert rel(X[j], xs) <= REL_DIFF
+
+
+def test_multi_rhs_mixed_precision(hip):
+    mesh = W.grid_mesh(50, seed=5)
+    Sw = W.cotan_system(mesh, t=1e-3, nrhs=4, rhs="gaussian", seed=2)
+    S = capi.System.from_crs(Sw, fp32=True)
+    X, per, tot = S.solve_multi(Sw.rhs, precision="mixed", eps=1e-10, max_iter=1000,
+                                inner_eps=1e-4)
+    A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+    for j in range(4):
+        r = O.spmv(A, X[j]) - Sw.rhs[j]
+        assert np.linalg.norm(r) / np.linalg.norm(Sw.rhs[j]) <= 1e-10
+        assert per[j].converged == 1 and per[j].outer_iterations >= 1
+    assert tot.converged == 1
+
+
+def test_multi_rhs_bad_k(hip):
+    S = capi.System([0, 1], [0], [2.0])
+    for k in (0, 33):
+        B = np.ones((max(k, 1), 1))
+        with pytest.raises(capi.NLHIPError) as e:
+            X = np.zeros_like(B)
+            p = capi.default_params()
+            capi.check(capi.lib().nlhipSolveMulti(S.handle, k, capi.ptr(B), capi.ptr(X),
+                                                  C.byref(p), None, None), "multi")
+        assert e.value.code == capi.NLHIP_ERR_UNSUPPORTED
+
+
+def test_context_nb_systems(hip):
+    """NL_NB_SYSTEMS = 3 through the context API: three right-hand sides of
+    one square symmetric system, compared with three oracle solves."""
+    L = hip
+    Sw = W.cotan_system(W.grid_mesh(12, seed=6), t=1e-3, nrhs=3, rhs="xyz")
+    n = Sw.n
+    L.nlNewContext()
+    L.nlSolverParameteri(capi.NL_NB_VARIABLES, n)
+    L.nlSolverParameteri(capi.NL_NB_SYSTEMS, 3)
+    L.nlSolverParameteri(capi.NL_SOLVER, capi.NL_CG)
+    L.nlSolverParameterd(capi.NL_THRESHOLD, 1e-12)
+    L.nlSolverParameteri(capi.NL_MAX_ITERATIONS, 1000)
+    L.nlBegin(capi.NL_SYSTEM)
+    L.nlBegin(capi.NL_MATRIX)
+    for i in range(n):
+        L.nlBegin(capi.NL_ROW)
+        for jj in range(Sw.rowptr[i], Sw.rowptr[i + 1]):
+            L.nlCoefficient(int(Sw.colind[jj]), float(Sw.val[jj]))
+        for j in range(3):
+            L.nlMultiRightHandSide(j, float(Sw.rhs[j][i]))
+        L.nlEnd(capi.NL_ROW)
+    L.nlEnd(capi.NL_MATRIX)
+    L.nlEnd(capi.NL_SYSTEM)
+    assert L.nlSolve() == capi.NL_TRUE
+    got = np.array([[L.nlMultiGetVariable(i, j) for i in range(n)] for j in range(3)])
+    L.nlDeleteContext(L.nlGetCurrent())
+    A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+    for j in range(3):
+        xr, _, _ = O.pcg(A, Sw.rhs[j], eps=1e-12, max_iter=1000)
+        assert rel(got[j], xr) <= REL_DIFF
