@@ -1,6 +1,8 @@
 """Measurements of the configurations beside bench.py's single large system.
 
     python -m graphitethree_amd.benchmarks atlas [--count 2000] [--seed 1]
+    python -m graphitethree_amd.benchmarks multirhs [--grid 1500] [--ks 3 4 8 16 32]
+    python -m graphitethree_amd.benchmarks serving [--count 120] [--rate 10]
 
 atlas: `count` disk-like LSCM charts (N ~ logU[50, 50000] vertices) solved
   (1) as one batch in FP64, (2) as one batch with FP32 inner PCG inside FP64
@@ -14,6 +16,13 @@ multirhs: the (I + 1e-3 L) cotangent-Laplacian system of a k x k noisy grid
   (default 1500 x 1500, n = 2.25 M) with 3..32 N(0,1) right-hand sides,
   solved as one multi-right-hand-side PCG (FP64, and FP32 inner with FP64
   refinement) against one FP64 PCG per right-hand side.
+
+serving: a stream of requests with Poisson arrivals (rate --rate per second):
+  60% LSCM charts with N ~ logU[1e3, 1e6] vertices, 40% smoothing solves
+  (I + 1e-3 L) x = b with 3 right-hand sides (x, y, z) on grids of similar
+  size; submitted to the solve server (200 GB HBM budget) at their arrival
+  times.  Reports p50 / p99 latency (arrival to completion), solves/s and the
+  server's admission figures.
 
 Each subcommand prints one JSON line on stdout; progress goes to stderr.
 """
@@ -155,6 +164,96 @@ def run_multirhs(args) -> dict:
             "device": dev[0], "eps": args.eps, "rows": rows}
 
 
+def make_requests(count: int, seed: int, nmin: float, nmax: float):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        N = int(round(float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))))
+        if rng.random() < 0.6:
+            S = W.lscm_system(W.chart_mesh(N, seed=seed * 100003 + i))
+            reqs.append(("lscm", S, S.b[None, :]))
+        else:
+            g = max(3, int(round(np.sqrt(N))))
+            S = W.cotan_system(W.grid_mesh(g, seed=seed * 100003 + i), t=1e-3, nrhs=3,
+                               rhs="xyz")
+            reqs.append(("smooth", S, S.rhs))
+    return reqs
+
+
+def run_serving(args) -> dict:
+    import threading
+    dev = init_device()
+    t0 = time.time()
+    reqs = make_requests(args.count, args.seed, args.nmin, args.nmax)
+    log(f"serving: {len(reqs)} requests generated in {time.time() - t0:.1f}s "
+        f"({sum(r[0] == 'lscm' for r in reqs)} LSCM, {sum(r[0] == 'smooth' for r in reqs)} "
+        f"smoothing x3)")
+    rng = np.random.default_rng(args.seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(reqs)))
+    srv = capi.Server(hbm_budget_bytes=int(args.budget_gb * (1 << 30)),
+                      max_active=args.max_active)
+    ids = [None] * len(reqs)
+    results = [None] * len(reqs)
+    submitted = threading.Semaphore(0)
+
+    def waiter():
+        for i in range(len(reqs)):
+            submitted.acquire()
+            try:
+                X, res = srv.wait(ids[i])
+                results[i] = (0, res, X)
+            except capi.NLHIPError as e:
+                results[i] = (e.code, getattr(e, "result", None), None)
+            if i % 20 == 0:
+                log(f"  completed {i + 1}/{len(reqs)}")
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    start = time.perf_counter()
+    for i, (kind, S, B) in enumerate(reqs):
+        delay = arrivals[i] - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        ids[i] = srv.submit(S, B, eps=args.eps, max_iter=args.max_iter)
+        submitted.release()
+    th.join()
+    info = srv.info()
+    lat = np.array([r[1].t_done - r[1].t_submit for r in results if r[0] == 0])
+    wait = np.array([r[1].t_admit - r[1].t_submit for r in results if r[0] == 0])
+    t_first = min(r[1].t_submit for r in results if r[0] == 0)
+    t_last = max(r[1].t_done for r in results if r[0] == 0)
+    conv = sum(int(r[1].stats.converged) for r in results if r[0] == 0)
+    # spot check: a few requests solved alone give the same bits as in the mix
+    same = True
+    for i in (0, len(reqs) // 2, len(reqs) - 1):
+        kind, S, B = reqs[i]
+        alone = capi.Server(max_active=1)
+        Xa, _ = alone.wait(alone.submit(S, B, eps=args.eps, max_iter=args.max_iter))
+        alone.close()
+        same &= bool(np.array_equal(Xa, results[i][2]))
+    srv.close()
+    return {
+        "metric": "serving p50/p99 solve latency and solves/s",
+        "workload": f"serving_{len(reqs)}_requests_rate{args.rate:g}",
+        "device": dev[0], "requests": len(reqs), "rate_per_s": args.rate,
+        "lscm": int(sum(r[0] == "lscm" for r in reqs)),
+        "smoothing_x3": int(sum(r[0] == "smooth" for r in reqs)),
+        "eps": args.eps, "max_iter": args.max_iter,
+        "completed": int(info.completed), "failed": int(info.failed),
+        "converged_to_eps": conv,
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "mean": float(lat.mean()), "max": float(lat.max())},
+        "queue_wait_s": {"p50": float(np.percentile(wait, 50)),
+                         "p99": float(np.percentile(wait, 99))},
+        "solves_per_s": float(len(lat) / (t_last - t_first)),
+        "makespan_s": float(t_last - t_first),
+        "budget_gb": info.budget_bytes / 2**30,
+        "peak_in_use_gb": info.peak_in_use_bytes / 2**30,
+        "peak_running": int(info.peak_running), "max_active": args.max_active,
+        "alone_equals_mix_spot_check": same,
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -176,8 +275,20 @@ def main(argv=None):
     mr.add_argument("--eps", type=float, default=1e-10)
     mr.add_argument("--max-iter", type=int, default=5000)
     mr.add_argument("--inner-eps", type=float, default=1e-4)
+    sv = sub.add_parser("serving")
+    sv.add_argument("--count", type=int, default=120)
+    sv.add_argument("--rate", type=float, default=10.0)
+    sv.add_argument("--seed", type=int, default=1)
+    sv.add_argument("--nmin", type=float, default=1e3)
+    sv.add_argument("--nmax", type=float, default=1e6)
+    sv.add_argument("--eps", type=float, default=1e-10)
+    sv.add_argument("--max-iter", type=int, default=5000)
+    sv.add_argument("--budget-gb", type=float, default=200.0)
+    sv.add_argument("--max-active", type=int, default=8)
     args = ap.parse_args(argv)
-    if args.cmd == "atlas":
+    if args.cmd == "serving":
+        out = run_serving(args)
+    elif args.cmd == "atlas":
         out = run_atlas(args)
     elif args.cmd == "multirhs":
         out = run_multirhs(args)

@@ -86,6 +86,23 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active", C.c_uint32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int32), ("k", C.c_uint32),
+                ("n", C.c_uint32), ("device_bytes", C.c_uint64), ("stats", SolveStats),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("in_use_bytes", C.c_uint64),
+                ("peak_in_use_bytes", C.c_uint64), ("pending", C.c_uint32),
+                ("running", C.c_uint32), ("peak_running", C.c_uint32),
+                ("completed", C.c_uint64), ("failed", C.c_uint64)]
+
+
 _u32p = C.POINTER(C.c_uint32)
 _f64p = C.POINTER(C.c_double)
 _vp = C.c_void_p
@@ -165,6 +182,14 @@ SIGNATURES = {
     "nlhipBatchSetLaunch": (C.c_int, [_vp, C.c_uint32]),
     "nlhipBatchSolve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
                                   C.POINTER(SolveStats), C.POINTER(SolveStats)]),
+    "nlhipServerConfigDefault": (None, [C.POINTER(ServerConfig)]),
+    "nlhipServerCreate": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
+    "nlhipServerSubmit": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32, _vp, _vp,
+                                    C.POINTER(SolveParams), C.POINTER(C.c_uint64)]),
+    "nlhipServerWait": (C.c_int, [_vp, C.c_uint64, _vp, C.POINTER(RequestResult),
+                                  C.c_double]),
+    "nlhipServerInfo": (C.c_int, [_vp, C.POINTER(ServerInfo)]),
+    "nlhipServerDestroy": (C.c_int, [_vp]),
     "nlhipLastError": (C.c_char_p, []),
     "nlhipInjectFailures": (None, [C.c_int]),
     "nlhipDeviceInfo": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int),
@@ -367,6 +392,61 @@ class Batch:
         check(lib().nlhipBatchSolve(self._h, ptr(b), ptr(x), C.byref(p), per, C.byref(tot)),
               "nlhipBatchSolve")
         return x, list(per), tot
+
+
+class Server:
+    """Solve server (nlhipServerCreate): budgeted admission, concurrent runs."""
+
+    def __init__(self, hbm_budget_bytes: int | None = None, max_active: int = 8):
+        c = ServerConfig()
+        lib().nlhipServerConfigDefault(C.byref(c))
+        if hbm_budget_bytes is not None:
+            c.hbm_budget_bytes = hbm_budget_bytes
+        c.max_active = max_active
+        h = C.c_void_p()
+        check(lib().nlhipServerCreate(C.byref(c), C.byref(h)), "nlhipServerCreate")
+        self._h = h
+        self._n = {}
+
+    def submit(self, S, B, X0=None, **params) -> int:
+        """S: CRS system (rowptr, colind, val); B: (k, n) or (n,)."""
+        rp = np.ascontiguousarray(S.rowptr, dtype=np.uint32)
+        ci = np.ascontiguousarray(S.colind, dtype=np.uint32)
+        va = np.ascontiguousarray(S.val, dtype=np.float64)
+        B = np.ascontiguousarray(np.atleast_2d(B), dtype=np.float64)
+        k, n = B.shape
+        x0 = None if X0 is None else np.ascontiguousarray(np.atleast_2d(X0), dtype=np.float64)
+        p = default_params(**params)
+        rid = C.c_uint64()
+        check(lib().nlhipServerSubmit(self._h, n, ptr(rp), ptr(ci), ptr(va), k, ptr(B),
+                                      None if x0 is None else ptr(x0), C.byref(p),
+                                      C.byref(rid)), "nlhipServerSubmit")
+        self._n[int(rid.value)] = (k, n)
+        return int(rid.value)
+
+    def wait(self, rid: int, timeout: float = 0.0):
+        """Returns (X (k, n), RequestResult); raises NLHIPError on failure."""
+        k, n = self._n.pop(rid)
+        X = np.empty((k, n))
+        res = RequestResult()
+        rc = lib().nlhipServerWait(self._h, rid, ptr(X), C.byref(res), timeout)
+        if rc != NLHIP_OK:
+            err = NLHIPError(rc, "nlhipServerWait")
+            err.result = res
+            raise err
+        return X, res
+
+    def info(self) -> ServerInfo:
+        i = ServerInfo()
+        check(lib().nlhipServerInfo(self._h, C.byref(i)), "nlhipServerInfo")
+        return i
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().nlhipServerDestroy(self._h)
+            self._h = None
+
+    __del__ = close
 
 
 def profile_cg(system: "System", iters: int = 50, fp32: bool = False):

@@ -152,6 +152,55 @@ NLAPI int NLAPIENTRY nlhipBatchSolve(nlhip_batch b, const double* rhs, double* x
                                      nlhip_solve_stats* per_system,
                                      nlhip_solve_stats* total);
 
+/* ---- solve server ----------------------------------------------------------
+ * Requests (one system, 1..32 right-hand sides) are copied at submission,
+ * admitted first-come first-served when a slot is free and their device
+ * footprint fits in the remaining budget, and run concurrently, each on its
+ * own resident system and stream.  A request larger than the whole budget
+ * finishes at once with NLHIP_ERR_BUDGET.  A failing request does not affect
+ * the others.  Times are seconds since the server was created.
+ */
+typedef struct nlhip_server_s* nlhip_server;
+
+typedef struct {
+    uint64_t hbm_budget_bytes;  /* default 200 GiB (capped at the device's HBM) */
+    uint32_t max_active;        /* concurrent requests (slots), default 8       */
+} nlhip_server_config;
+
+typedef struct {
+    uint64_t id;
+    int32_t status;
+    uint32_t k;
+    uint32_t n;
+    uint64_t device_bytes;      /* footprint charged against the budget         */
+    nlhip_solve_stats stats;    /* for k > 1: worst column, largest count       */
+    double t_submit, t_admit, t_done;
+} nlhip_request_result;
+
+typedef struct {
+    uint64_t budget_bytes, in_use_bytes, peak_in_use_bytes;
+    uint32_t pending, running, peak_running;
+    uint64_t completed, failed;
+} nlhip_server_info;
+
+NLAPI void NLAPIENTRY nlhipServerConfigDefault(nlhip_server_config* c);
+NLAPI int NLAPIENTRY nlhipServerCreate(const nlhip_server_config* config,
+                                       nlhip_server* out);
+/* B (and X0, may be null) column-major n x k. */
+NLAPI int NLAPIENTRY nlhipServerSubmit(nlhip_server s, uint32_t n, const uint32_t* rowptr,
+                                       const uint32_t* colind, const double* val,
+                                       uint32_t k, const double* B, const double* X0,
+                                       const nlhip_solve_params* params, uint64_t* id);
+/* Blocks until request `id` is finished (timeout_seconds <= 0: no limit),
+ * copies its solution into X (n x k, may be null) and releases it.  Returns
+ * the request's own status. */
+NLAPI int NLAPIENTRY nlhipServerWait(nlhip_server s, uint64_t id, double* X,
+                                     nlhip_request_result* result,
+                                     double timeout_seconds);
+NLAPI int NLAPIENTRY nlhipServerInfo(nlhip_server s, nlhip_server_info* info);
+/* Finishes the admitted and waiting requests, then stops the workers. */
+NLAPI int NLAPIENTRY nlhipServerDestroy(nlhip_server s);
+
 NLAPI const char* NLAPIENTRY nlhipLastError(void);
 /* Test hook: the next `count` solves fail with NLHIP_ERR_INJECTED before
  * touching the device (other requests are unaffected). */
