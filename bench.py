@@ -24,6 +24,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_PROBE_BYTES = 1 << 30
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_cg_spmv_lscm2236_traffic.md")
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per k_cg_spmv launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a plain copy), when
+    it covers this workload; None otherwise."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            kv = dict(line.split(":", 1) for line in f if line.startswith("- ") and ":" in line)
+    except OSError:
+        return None
+    kv = {k[2:].strip(): v.strip() for k, v in kv.items()}
+    if kv.get("workload") != workload:
+        return None
+    try:
+        return float(kv["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def log(*a):
@@ -58,6 +78,9 @@ def main():
                     help="oracle PCG iterations timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only launch the kernels whose HBM counters are read under "
+                         "rocprofv3 --pmc: a 1 GiB plain copy (x3) and 10 PCG iterations")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -82,6 +105,18 @@ def main():
     S = capi.System.from_crs(Sw, fp32=not args.no_mixed)
     lpr, wg = S.launch()
     log(f"uploaded in {time.time() - t0:.1f}s; launch rows/wavefront={lpr} workgroups={wg}")
+
+    if args.traffic_probe:
+        t_copy = capi.bench_copy(COPY_PROBE_BYTES, reps=3)
+        t_dir, t_a, t_upd = capi.profile_cg(S, iters=10)
+        print(json.dumps({"probe": "traffic", "copy_bytes": COPY_PROBE_BYTES, "copy_s": t_copy,
+                          "k_cg_spmv_s": t_a, "spmv_bytes": spmv_bytes(n, nnz)}), flush=True)
+        return
+
+    # ---- plain copy: the bandwidth this device sustains for a streaming kernel
+    t_copy = capi.bench_copy(2 << 30, reps=20)
+    copy_gbs = 2 * (2 << 30) / t_copy / 1e9
+    log(f"copy 2 GiB: {t_copy * 1e3:.3f} ms  {copy_gbs:.0f} GB/s (read + write)")
 
     # ---- standalone SpMV bandwidth (FP64, and FP32 values)
     t_spmv = S.bench_spmv(fp32=False, reps=50)
@@ -200,8 +235,9 @@ def main():
         "iteration_gbs": iter_gbs,
         "roofline": {"bound": "hbm", "kernel": "k_cg_spmv", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": bytes_a,
-                     "launch_ms": t_a * 1e3},
+                     "traffic": measured_traffic(f"lscm_grid_{args.k}"),
+                     "bytes_per_launch": bytes_a, "launch_ms": t_a * 1e3,
+                     "copy_gbs": copy_gbs},
         "kernels_ms": {"k_cg_dir": t_dir * 1e3, "k_cg_spmv": t_a * 1e3,
                        "k_cg_update": t_upd * 1e3},
         "cpu_baseline": cpu,

@@ -37,6 +37,7 @@ NLHIP_ERR_NONFINITE, NLHIP_ERR_UNSUPPORTED, NLHIP_ERR_BUDGET, NLHIP_ERR_INJECTED
     -5, -6, -7, -8)
 NLHIP_PREC_FP64, NLHIP_PREC_MIXED = 0, 1
 NLHIP_SYSTEM_FP32 = 1
+NLHIP_SYSTEM_NO_OFFSETS = 2
 
 
 class NLMatrixStruct(C.Structure):
@@ -174,6 +175,7 @@ SIGNATURES = {
                                   C.POINTER(SolveStats), C.POINTER(SolveStats)]),
     "nlhipSpMV": (C.c_int, [_vp, _vp, _vp]),
     "nlhipBenchSpMV": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
+    "nlhipBenchCopy": (C.c_int, [C.c_uint64, C.c_uint32, _f64p]),
     "nlhipProfileCG": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
     "nlhipBatchCreate": (C.c_int, [C.c_uint32, _vp, _vp, _vp, _vp, C.c_uint32,
                                    C.POINTER(_vp)]),
@@ -253,20 +255,20 @@ def default_params(**kw) -> SolveParams:
 class System:
     """A CRS system resident on the GPU (nlhipSystemCreate)."""
 
-    def __init__(self, rowptr, colind, val, fp32: bool = False):
+    def __init__(self, rowptr, colind, val, fp32: bool = False, offsets: bool = True):
         self.rowptr = np.ascontiguousarray(rowptr, dtype=np.uint32)
         self.colind = np.ascontiguousarray(colind, dtype=np.uint32)
         self.val = np.ascontiguousarray(val, dtype=np.float64)
         self.n = len(self.rowptr) - 1
+        flags = (NLHIP_SYSTEM_FP32 if fp32 else 0) | (0 if offsets else NLHIP_SYSTEM_NO_OFFSETS)
         h = C.c_void_p()
         check(lib().nlhipSystemCreate(self.n, ptr(self.rowptr), ptr(self.colind),
-                                      ptr(self.val), NLHIP_SYSTEM_FP32 if fp32 else 0,
-                                      C.byref(h)), "nlhipSystemCreate")
+                                      ptr(self.val), flags, C.byref(h)), "nlhipSystemCreate")
         self._h = h
 
     @classmethod
-    def from_crs(cls, S, fp32: bool = False) -> "System":
-        return cls(S.rowptr, S.colind, S.val, fp32=fp32)
+    def from_crs(cls, S, fp32: bool = False, offsets: bool = True) -> "System":
+        return cls(S.rowptr, S.colind, S.val, fp32=fp32, offsets=offsets)
 
     @property
     def handle(self):
@@ -454,6 +456,13 @@ def profile_cg(system: "System", iters: int = 50, fp32: bool = False):
     t = (C.c_double * 3)()
     check(lib().nlhipProfileCG(system.handle, 1 if fp32 else 0, iters, t), "nlhipProfileCG")
     return float(t[0]), float(t[1]), float(t[2])
+
+
+def bench_copy(nbytes: int, reps: int = 20) -> float:
+    """Mean seconds of a plain device copy of `nbytes` (HIP events)."""
+    t = C.c_double()
+    check(lib().nlhipBenchCopy(nbytes, reps, C.byref(t)), "nlhipBenchCopy")
+    return float(t.value)
 
 
 def device_info():

@@ -4,6 +4,8 @@
 
 #include <cstdint>
 
+#include "nlhip_types.h"
+
 namespace nlhip {
 
 constexpr int kWave = 64;
@@ -100,13 +102,15 @@ __device__ __forceinline__ void block_row_range(uint32_t m, uint32_t chunk, uint
 // rows [L*sp, L*sp + sp) of the matrix for logical block L, partial products
 // combined by a fixed xor butterfly.
 //
-// Stream engine (RPW = 64): one row per lane.  A wavefront owns a tile of 64
-// consecutive rows, whose nonzeros form one contiguous range; it streams that
+// Stream engine (RPW = 64): one row per lane.  A wavefront owns a run of
+// 64-row tiles, whose nonzeros form one contiguous range; it streams that
 // range in chunks of 64*E entries, each lane loading E independent (val,
 // colind) pairs and gathering x, stages the products in LDS and then lane l
-// sums row l's products in column order.  Block L owns tiles [L*sp, L*sp+sp)
-// (wavefront w of the block takes every 4th of them).  Each row sum is
+// sums row l's products in column order.  Block L owns tiles [L*sp, L*sp+sp),
+// split into four contiguous runs, one per wavefront.  Each row sum is
 // sum_j fl(a_ij x_j) accumulated left to right with one rounding per add.
+// With D16 the column indices are read as 16-bit segment offsets (Cols),
+// 10 instead of 12 bytes per FP64 nonzero; the sums are the same.
 
 template <int W>
 struct VectorEngine {
@@ -114,9 +118,9 @@ struct VectorEngine {
     template <typename T, typename Epi>
     __device__ __forceinline__ static void run(uint32_t m, uint32_t sp, uint32_t L,
                                                const uint32_t* __restrict__ rp,
-                                               const uint32_t* __restrict__ ci,
-                                               const T* __restrict__ val,
+                                               const Cols& cols, const T* __restrict__ val,
                                                const T* __restrict__ x, T*, Epi& epi) {
+        const uint32_t* __restrict__ ci = cols.ci;
         uint32_t r0, r1;
         block_row_range(m, sp, L, r0, r1);
         const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
@@ -127,24 +131,59 @@ struct VectorEngine {
     }
 };
 
-template <int E, bool NT>
+// Logical blocks [lo, hi) that run on the same XCD as logical block L (the
+// inverse view of xcd_logical_block).
+__device__ __forceinline__ void xcd_block_range(uint32_t L, uint32_t G, uint32_t& lo,
+                                                uint32_t& hi) {
+    uint32_t start = 0;
+    for (uint32_t y = 0; y < kNumXcd; ++y) {
+        const uint32_t cnt = (G - y + kNumXcd - 1) / kNumXcd;
+        if (L < start + cnt || y + 1 == kNumXcd) {
+            lo = start;
+            hi = start + cnt;
+            return;
+        }
+        start += cnt;
+    }
+}
+
+template <int E, bool NT, bool D16 = false, int RUN = 0>
 struct StreamEngine {
+    static_assert(64 * E <= int(kSegEntries), "a chunk must span at most two segments");
+
     template <typename T>
     __device__ __forceinline__ static T ld(const T* p) {
         if constexpr (NT) return ntload(p);
         else return *p;
     }
 
+    // Loads the chunk of entries [base, base + 64E) (indices past J1 clamped
+    // to J1 - 1, so every column read is a real one).  base and J1 are
+    // wavefront-uniform.
     template <typename T>
-    __device__ __forceinline__ static void load_chunk(const uint32_t* __restrict__ ci,
+    __device__ __forceinline__ static void load_chunk(const Cols& cols,
                                                       const T* __restrict__ val, uint32_t base,
                                                       uint32_t J1, int lane, uint32_t (&c)[E],
                                                       T (&a)[E]) {
         // column indices first: the gathers wait on them, not on the values
+        if constexpr (D16) {
+            // the clamped indices of this chunk lie in [q, q + 64E) with
+            // q = min(base, J1 - 1), i.e. in segment s0 or s0 + 1
+            const uint32_t q = base < J1 - 1u ? base : J1 - 1u;
+            const uint32_t s0 = q >> kSegShift;
+            const uint32_t b0 = cols.segb[s0], b1 = cols.segb[s0 + 1];
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
-            c[e] = ld(ci + (j < J1 ? j : J1 - 1u));
+            for (int e = 0; e < E; ++e) {
+                const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
+                const uint32_t jj = j < J1 ? j : J1 - 1u;
+                c[e] = ((jj >> kSegShift) == s0 ? b0 : b1) + uint32_t(ld(cols.ci16 + jj));
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
+                c[e] = ld(cols.ci + (j < J1 ? j : J1 - 1u));
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -161,22 +200,21 @@ struct StreamEngine {
     template <typename T, typename Epi>
     __device__ __forceinline__ static void run_rows(uint32_t R0, uint32_t R1,
                                                     const uint32_t* __restrict__ rp,
-                                                    const uint32_t* __restrict__ ci,
-                                                    const T* __restrict__ val,
+                                                    const Cols& cols, const T* __restrict__ val,
                                                     const T* __restrict__ x, T* __restrict__ st,
                                                     int lane, Epi& epi) {
-        const uint32_t J1 = rp[R1];
+        const uint32_t J1 = __builtin_amdgcn_readfirstlane(rp[R1]);
         uint32_t r0 = R0;
         uint32_t nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
         uint32_t tile_end = rp[r0 + nrows];
         uint32_t rb = (uint32_t(lane) < nrows) ? rp[r0 + uint32_t(lane)] : tile_end;
         uint32_t re = __shfl_down(rb, 1, kWave);
         if (uint32_t(lane) + 1 >= nrows) re = tile_end;
-        uint32_t base = __shfl(rb, 0, kWave);
+        uint32_t base = __builtin_amdgcn_readfirstlane(__shfl(rb, 0, kWave));
         T acc = T(0);
         uint32_t c[E];
         T a[E];
-        if (base < J1) load_chunk(ci, val, base, J1, lane, c, a);
+        if (base < J1) load_chunk(cols, val, base, J1, lane, c, a);
         while (base < J1) {
             const uint32_t craw = base + 64u * E;
             const uint32_t cend = craw < J1 ? craw : J1;
@@ -191,7 +229,7 @@ struct StreamEngine {
             // so that its loads overlap this chunk's products and row sums
             uint32_t cn[E];
             T an[E];
-            load_chunk(ci, val, craw, J1, lane, cn, an);
+            load_chunk(cols, val, craw, J1, lane, cn, an);
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
@@ -238,16 +276,34 @@ struct StreamEngine {
         }
     }
 
-    // Block L owns 64-row tiles [L*sp, L*sp + sp); each of its wavefronts
-    // takes one contiguous quarter of them.
+    // Block L owns 64-row tiles [L*sp, L*sp + sp).
+    // RUN = 0: each of its wavefronts takes one contiguous quarter of them.
+    // RUN > 0: the tiles of all blocks on L's XCD are dealt out in runs of RUN
+    // tiles, round-robin over that XCD's wavefronts, so that the wavefronts
+    // of an XCD sweep its rows together and the x entries they gather (a
+    // band around the rows in flight) stay in that XCD's L2.
     template <typename T, typename Epi>
     __device__ __forceinline__ static void run(uint32_t m, uint32_t sp, uint32_t L,
                                                const uint32_t* __restrict__ rp,
-                                               const uint32_t* __restrict__ ci,
-                                               const T* __restrict__ val,
+                                               const Cols& cols, const T* __restrict__ val,
                                                const T* __restrict__ x, T* stage, Epi& epi) {
         const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
         const uint64_t ntiles = (uint64_t(m) + 63) / 64;
+        if constexpr (RUN > 0) {
+            uint32_t lo, hi;
+            xcd_block_range(L, gridDim.x, lo, hi);
+            const uint64_t tb = uint64_t(lo) * sp < ntiles ? uint64_t(lo) * sp : ntiles;
+            const uint64_t te = uint64_t(hi) * sp < ntiles ? uint64_t(hi) * sp : ntiles;
+            const uint64_t nw = uint64_t(hi - lo) * kWavesPerBlock;
+            const uint64_t v = uint64_t(L - lo) * kWavesPerBlock + uint64_t(w);
+            for (uint64_t t0 = tb + v * RUN; t0 < te; t0 += nw * RUN) {
+                const uint64_t t1 = t0 + RUN < te ? t0 + RUN : te;
+                const uint32_t R1 = uint32_t(t1 * 64 < m ? t1 * 64 : m);
+                run_rows(uint32_t(t0 * 64), R1, rp, cols, val, x, stage + w * (64 * E), lane,
+                         epi);
+            }
+            return;
+        }
         uint64_t tb = uint64_t(L) * sp;
         if (tb > ntiles) tb = ntiles;
         const uint64_t te = tb + sp < ntiles ? tb + sp : ntiles;
@@ -258,22 +314,22 @@ struct StreamEngine {
         if (t0 >= t1) return;
         const uint32_t R0 = uint32_t(t0 * 64);
         const uint32_t R1 = uint32_t(t1 * 64 < m ? t1 * 64 : m);
-        run_rows(R0, R1, rp, ci, val, x, stage + w * (64 * E), lane, epi);
+        run_rows(R0, R1, rp, cols, val, x, stage + w * (64 * E), lane, epi);
     }
 };
 
-// Engine for a rows-per-wavefront value, and its LDS staging size.
+// Engine for a rows-per-wavefront code, and its LDS staging size.  Codes
+// 64 and above are stream engines; 64 itself is resolved by the launchers
+// (engine_rpw) to kStreamAuto or kStreamAutoPlain.
 constexpr int kStreamE = 8;
+constexpr int kStreamRun = 4;
 
 template <int RPW>
 struct EngineOf {
     using type = VectorEngine<RPW>;
 };
-template <>
-struct EngineOf<64> {
-    using type = StreamEngine<kStreamE, true>;
-};
-// Variants compared by the launch-shape sweep (not selectable from the ABI).
+// Stream-engine variants for launch-shape sweeps (chunk width, load kind,
+// column format, tile order); all compute the same row sums.
 template <>
 struct EngineOf<65> {
     using type = StreamEngine<4, true>;
@@ -286,10 +342,26 @@ template <>
 struct EngineOf<67> {
     using type = StreamEngine<16, true>;
 };
+template <>
+struct EngineOf<68> {  // 16-bit column offsets, contiguous quarters
+    using type = StreamEngine<kStreamE, true, true>;
+};
+template <>
+struct EngineOf<69> {  // 32-bit indices, XCD-interleaved runs
+    using type = StreamEngine<kStreamE, true, false, kStreamRun>;
+};
+template <>
+struct EngineOf<70> {  // 16-bit column offsets, XCD-interleaved runs
+    using type = StreamEngine<kStreamE, true, true, kStreamRun>;
+};
+template <>
+struct EngineOf<71> {  // 32-bit indices, contiguous quarters
+    using type = StreamEngine<kStreamE, true>;
+};
 
 template <int RPW>
 struct StageSize {
-    static constexpr int value = RPW == 64   ? kWavesPerBlock * 64 * kStreamE
+    static constexpr int value = (RPW == 64 || RPW >= 68) ? kWavesPerBlock * 64 * kStreamE
                                  : RPW == 65 ? kWavesPerBlock * 64 * 4
                                  : RPW == 66 ? kWavesPerBlock * 64 * 8
                                  : RPW == 67 ? kWavesPerBlock * 64 * 16

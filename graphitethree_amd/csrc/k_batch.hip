@@ -37,7 +37,8 @@ __device__ __forceinline__ void tile_spmv(const BatchArgs<T>& a, uint32_t r0, ui
     const uint32_t R0 = r0 + 64u * uint32_t(w);
     if (R0 >= r1) return;
     const uint32_t R1 = (R0 + 64u < r1) ? R0 + 64u : r1;
-    StreamEngine<kStreamE, true>::run_rows(R0, R1, a.rp, a.ci, a.val, x,
+    const Cols cols{a.ci, nullptr, nullptr};
+    StreamEngine<kStreamE, true>::run_rows(R0, R1, a.rp, cols, a.val, x,
                                            stage + w * (64 * kStreamE), lane, epi);
 }
 

@@ -108,6 +108,30 @@ hipError_t blas_ddot(uint64_t n, const double* x, const double* y, double* part,
     return hipGetLastError();
 }
 
+// Plain device copy in 16-byte words (the bandwidth reference for the other
+// kernels): every thread moves four words per trip, loads before stores.
+__global__ __launch_bounds__(kBlock) void k_copy16(uint64_t n16, const uint4* __restrict__ src,
+                                                   uint4* __restrict__ dst) {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride],
+                    d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+hipError_t copy16(uint64_t n16, const void* src, void* dst, uint32_t grid, hipStream_t s) {
+    if (n16 == 0) return hipSuccess;
+    if (grid == 0) return hipErrorInvalidValue;
+    k_copy16<<<grid, kBlock, 0, s>>>(n16, static_cast<const uint4*>(src), static_cast<uint4*>(dst));
+    return hipGetLastError();
+}
+
 hipError_t reduce_final(const double* part, uint32_t P, double* out, hipStream_t s) {
     k_reduce_final<<<1, kBlock, 0, s>>>(part, P, out);
     return hipGetLastError();

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(CGArgs<T> a) {
         prr += ri * ri;
         pbb += bi * bi;
     };
-    EngineOf<RPW>::type::run(a.m, a.sp, L, a.rp, a.ci, a.val, a.x, stage, epi);
+    EngineOf<RPW>::type::run(a.m, a.sp, L, a.rp, a.cols, a.val, a.x, stage, epi);
     prh = block_sum_all(prh, lds);
     prr = block_sum_all(prr, lds);
     pbb = block_sum_all(pbb, lds);
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(CGArgs<T> a) {
         a.q[row] = s;
         pdq += d[row] * s;
     };
-    EngineOf<RPW>::type::run(a.m, a.sp, L, a.rp, a.ci, a.val, d, stage, epi);
+    EngineOf<RPW>::type::run(a.m, a.sp, L, a.rp, a.cols, a.val, d, stage, epi);
     pdq = block_sum_all(pdq, lds);
     if (threadIdx.x == 0) a.part_dq[L] = pdq;
 }
@@ -199,6 +199,7 @@ template <typename T>
 hipError_t cg_start(const CGArgs<T>& a, int rpw, bool precond, T eps, int max_iter,
                     hipStream_t s) {
     if (a.m == 0 || a.P == 0) return hipErrorInvalidValue;
+    rpw = engine_rpw(rpw, a.cols);
     if (precond) {
         NLHIP_RPW_SWITCH(rpw, (k_cg_init<RPW, T, true><<<a.P, kBlock, 0, s>>>(a)));
     } else {
@@ -213,6 +214,7 @@ hipError_t cg_step_part(const CGArgs<T>& a, int rpw, bool precond, int part, hip
     if (part == 0) {
         k_cg_dir<T><<<a.P, kBlock, 0, s>>>(a);
     } else if (part == 1) {
+        rpw = engine_rpw(rpw, a.cols);
         NLHIP_RPW_SWITCH(rpw, (k_cg_spmv<RPW, T><<<a.P, kBlock, 0, s>>>(a)));
     } else if (precond) {
         k_cg_update<T, true><<<a.P, kBlock, 0, s>>>(a);

@@ -18,21 +18,20 @@ namespace nlhip {
 
 template <int RPW, typename T>
 __global__ __launch_bounds__(kBlock) void k_spmv(uint32_t m, uint32_t sp,
-                                                 const uint32_t* __restrict__ rp,
-                                                 const uint32_t* __restrict__ ci,
+                                                 const uint32_t* __restrict__ rp, Cols cols,
                                                  const T* __restrict__ val,
                                                  const T* __restrict__ x, T* __restrict__ y) {
     __shared__ T stage[StageSize<RPW>::value];
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     auto epi = [&](uint32_t row, T s) { y[row] = s; };
-    EngineOf<RPW>::type::run(m, sp, L, rp, ci, val, x, stage, epi);
+    EngineOf<RPW>::type::run(m, sp, L, rp, cols, val, x, stage, epi);
 }
 
 // r = b - A x (FP64), with per-block partial sums of r.r and b.b written to
 // part_rr[L], part_bb[L] (L = logical block).
 template <int RPW>
 __global__ __launch_bounds__(kBlock) void k_residual(
-    uint32_t m, uint32_t sp, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+    uint32_t m, uint32_t sp, const uint32_t* __restrict__ rp, Cols cols,
     const double* __restrict__ val, const double* __restrict__ x,
     const double* __restrict__ b, double* __restrict__ r, double* __restrict__ part_rr,
     double* __restrict__ part_bb) {
@@ -47,7 +46,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(
         srr += ri * ri;
         sbb += bi * bi;
     };
-    EngineOf<RPW>::type::run(m, sp, L, rp, ci, val, x, stage, epi);
+    EngineOf<RPW>::type::run(m, sp, L, rp, cols, val, x, stage, epi);
     srr = block_sum_all(srr, lds);
     sbb = block_sum_all(sbb, lds);
     if (threadIdx.x == 0) {
@@ -95,24 +94,86 @@ uint32_t spmv_work_per_block(uint32_t m, uint32_t grid, int rpw) {
 }
 
 template <typename T>
-hipError_t spmv(uint32_t m, const uint32_t* rp, const uint32_t* ci, const T* val, const T* x,
+hipError_t spmv(uint32_t m, const uint32_t* rp, const Cols& cols, const T* val, const T* x,
                 T* y, int rpw, uint32_t grid, hipStream_t s) {
     if (m == 0) return hipSuccess;
     const uint32_t sp = spmv_work_per_block(m, grid, rpw);
-    NLHIP_RPW_SWITCH(rpw, (k_spmv<RPW, T><<<grid, kBlock, 0, s>>>(m, sp, rp, ci, val, x, y)));
+    rpw = engine_rpw(rpw, cols);
+    NLHIP_RPW_SWITCH(rpw, (k_spmv<RPW, T><<<grid, kBlock, 0, s>>>(m, sp, rp, cols, val, x, y)));
     return hipGetLastError();
 }
-template hipError_t spmv<double>(uint32_t, const uint32_t*, const uint32_t*, const double*,
+template hipError_t spmv<double>(uint32_t, const uint32_t*, const Cols&, const double*,
                                  const double*, double*, int, uint32_t, hipStream_t);
-template hipError_t spmv<float>(uint32_t, const uint32_t*, const uint32_t*, const float*,
+template hipError_t spmv<float>(uint32_t, const uint32_t*, const Cols&, const float*,
                                 const float*, float*, int, uint32_t, hipStream_t);
 
-hipError_t residual64(uint32_t m, const uint32_t* rp, const uint32_t* ci, const double* val,
+hipError_t residual64(uint32_t m, const uint32_t* rp, const Cols& cols, const double* val,
                       const double* x, const double* b, double* r, double* part_rr,
                       double* part_bb, int rpw, uint32_t grid, hipStream_t s) {
     const uint32_t sp = spmv_work_per_block(m, grid, rpw);
-    NLHIP_RPW_SWITCH(rpw, (k_residual<RPW><<<grid, kBlock, 0, s>>>(m, sp, rp, ci, val, x, b,
+    rpw = engine_rpw(rpw, cols);
+    NLHIP_RPW_SWITCH(rpw, (k_residual<RPW><<<grid, kBlock, 0, s>>>(m, sp, rp, cols, val, x, b,
                                                                    r, part_rr, part_bb)));
+    return hipGetLastError();
+}
+
+// segb[s] = min column of segment s; *span = max over segments of the
+// column range.  One workgroup per segment.
+__global__ __launch_bounds__(kBlock) void k_seg_bases(uint64_t nnz, const uint32_t* __restrict__ ci,
+                                                      uint32_t* __restrict__ segb,
+                                                      uint32_t* __restrict__ span) {
+    __shared__ uint32_t lo_s[kWavesPerBlock], hi_s[kWavesPerBlock];
+    const uint64_t j0 = uint64_t(blockIdx.x) << kSegShift;
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (uint64_t j = j0 + threadIdx.x; j < j0 + kSegEntries && j < nnz; j += kBlock) {
+        const uint32_t c = ci[j];
+        lo = c < lo ? c : lo;
+        hi = c > hi ? c : hi;
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const uint32_t l2 = __shfl_xor(lo, o, kWave), h2 = __shfl_xor(hi, o, kWave);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) {
+        lo_s[w] = lo;
+        hi_s[w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kWavesPerBlock; ++k) {
+            lo = lo_s[k] < lo ? lo_s[k] : lo;
+            hi = hi_s[k] > hi ? hi_s[k] : hi;
+        }
+        segb[blockIdx.x] = lo;
+        atomicMax(span, hi - lo);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_encode(uint64_t nnz, const uint32_t* __restrict__ ci,
+                                                       const uint32_t* __restrict__ segb,
+                                                       uint16_t* __restrict__ ci16) {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t j = uint64_t(blockIdx.x) * kBlock + threadIdx.x; j < nnz; j += stride)
+        ci16[j] = uint16_t(ci[j] - segb[j >> kSegShift]);
+}
+
+hipError_t seg_bases(uint64_t nnz, const uint32_t* ci, uint32_t* segb, uint32_t* span,
+                     hipStream_t s) {
+    const uint64_t nseg = (nnz + kSegEntries - 1) >> kSegShift;
+    if (nseg == 0 || nseg > 0x7fffffffu) return hipErrorInvalidValue;
+    k_seg_bases<<<uint32_t(nseg), kBlock, 0, s>>>(nnz, ci, segb, span);
+    return hipGetLastError();
+}
+
+hipError_t seg_encode(uint64_t nnz, const uint32_t* ci, const uint32_t* segb, uint16_t* ci16,
+                      hipStream_t s) {
+    if (nnz == 0) return hipSuccess;
+    uint64_t g = (nnz + kBlock - 1) / kBlock;
+    if (g > 16384) g = 16384;
+    k_seg_encode<<<uint32_t(g), kBlock, 0, s>>>(nnz, ci, segb, ci16);
     return hipGetLastError();
 }
 

@@ -266,6 +266,8 @@ struct System {
     uint32_t n = 0;
     uint64_t nnz = 0;
     DevBuf<uint32_t> rowptr, colind;
+    DevBuf<uint16_t> ci16;   // 16-bit column offsets (empty when the span is too wide)
+    DevBuf<uint32_t> segb;   // their per-segment bases
     DevBuf<double> val, dinv;
     DevBuf<float> val32, dinv32;
     int rpw = 64;        // rows per wavefront of the SpMV engine
@@ -285,6 +287,7 @@ struct System {
 
     void ensure_fp32();
     uint64_t device_bytes() const;
+    Cols cols() const { return Cols{colind.get(), ci16.get(), segb.get()}; }
 };
 
 // Validates and uploads; throws Error.

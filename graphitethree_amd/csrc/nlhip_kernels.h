@@ -8,25 +8,45 @@
 
 namespace nlhip {
 
-// Rows-per-wavefront dispatch over the compiled SpMV engines:
-// 2, 4, 8, 16 -> 32, 16, 8, 4 lanes per row; 64 -> one row per lane, streamed.
+// Rows-per-wavefront dispatch over the compiled SpMV engines (codes of
+// dev_common.h's EngineOf): 2, 4, 8, 16 -> 32, 16, 8, 4 lanes per row;
+// 65..71 -> stream engines (one row per lane).  Launchers pass the code
+// through engine_rpw first, which resolves 64.
 #define NLHIP_RPW_SWITCH(rpw, CALL)                       \
     switch (rpw) {                                        \
         case 2: { constexpr int RPW = 2; CALL; } break;   \
         case 4: { constexpr int RPW = 4; CALL; } break;   \
         case 8: { constexpr int RPW = 8; CALL; } break;   \
         case 16: { constexpr int RPW = 16; CALL; } break; \
-        case 64: { constexpr int RPW = 64; CALL; } break; \
         case 65: { constexpr int RPW = 65; CALL; } break; \
         case 66: { constexpr int RPW = 66; CALL; } break; \
         case 67: { constexpr int RPW = 67; CALL; } break; \
+        case 68: { constexpr int RPW = 68; CALL; } break; \
+        case 69: { constexpr int RPW = 69; CALL; } break; \
+        case 70: { constexpr int RPW = 70; CALL; } break; \
+        case 71: { constexpr int RPW = 71; CALL; } break; \
         default: return hipErrorInvalidValue;             \
     }
 
-// 65..67 are stream-engine variants (chunk width / load kind) kept for
-// launch-shape sweeps; they compute the same sums as 64.
+// Launch shapes a caller may set: 2, 4, 8, 16, and the stream engines 64
+// (the default stream engine) and 65..71 (variants kept for sweeps; they
+// compute the same sums as 64).
 inline bool valid_rpw(int rpw) {
-    return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 67);
+    return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 71);
+}
+
+// Default stream engines: with and without 16-bit column offsets.
+constexpr int kStreamAuto = 70;
+constexpr int kStreamAutoPlain = 69;
+
+// Engine a launcher runs for a requested code: 64 becomes the default stream
+// engine for the matrix's column format; a variant that reads 16-bit offsets
+// falls back to its 32-bit twin when the matrix has none.
+inline int engine_rpw(int rpw, const Cols& cols) {
+    if (rpw == 64) return cols.ci16 ? kStreamAuto : kStreamAutoPlain;
+    if (!cols.ci16 && rpw == 68) return 71;
+    if (!cols.ci16 && rpw == 70) return 69;
+    return rpw;
 }
 
 // SpMV work per workgroup for `grid` workgroups: rows (vector engines,
@@ -34,11 +54,11 @@ inline bool valid_rpw(int rpw) {
 uint32_t spmv_work_per_block(uint32_t m, uint32_t grid, int rpw);
 
 template <typename T>
-hipError_t spmv(uint32_t m, const uint32_t* rp, const uint32_t* ci, const T* val, const T* x,
+hipError_t spmv(uint32_t m, const uint32_t* rp, const Cols& cols, const T* val, const T* x,
                 T* y, int rpw, uint32_t grid, hipStream_t s);
 
 // r = b - A x (r may be null), partial sums of r.r and b.b per workgroup.
-hipError_t residual64(uint32_t m, const uint32_t* rp, const uint32_t* ci, const double* val,
+hipError_t residual64(uint32_t m, const uint32_t* rp, const Cols& cols, const double* val,
                       const double* x, const double* b, double* r, double* part_rr,
                       double* part_bb, int rpw, uint32_t grid, hipStream_t s);
 
@@ -48,10 +68,21 @@ hipError_t jacobi_dinv(uint32_t m, const uint32_t* rp, const uint32_t* ci, const
 
 hipError_t cast_f64_f32(uint64_t n, const double* a, float* b, hipStream_t s);
 
+// 16-bit column offsets (Cols): segb[s] = smallest column of segment s and
+// *span = max over segments of (largest - smallest column), atomically
+// max-combined into *span (set it to 0 first); then ci16[j] = ci[j] -
+// segb[j >> kSegShift] once the span is known to fit in 16 bits.
+hipError_t seg_bases(uint64_t nnz, const uint32_t* ci, uint32_t* segb, uint32_t* span,
+                     hipStream_t s);
+hipError_t seg_encode(uint64_t nnz, const uint32_t* ci, const uint32_t* segb, uint16_t* ci16,
+                      hipStream_t s);
+
 // BLAS-1
 hipError_t blas_dscal(uint64_t n, double a, double* x, hipStream_t s);
 hipError_t blas_daxpy(uint64_t n, double a, const double* x, double* y, hipStream_t s);
 hipError_t blas_dmul(uint64_t n, const double* a, const double* x, double* y, hipStream_t s);
+// dst[0..16 n16) = src[0..16 n16) with `grid` workgroups.
+hipError_t copy16(uint64_t n16, const void* src, void* dst, uint32_t grid, hipStream_t s);
 template <typename T>
 hipError_t fill(uint64_t n, T v, T* x, hipStream_t s);
 // out[0] = x.y, out[1] = sqrt(x.y); part holds `grid` doubles.
@@ -68,7 +99,7 @@ struct CGArgs {
     uint32_t sp;     // SpMV work per workgroup (spmv_work_per_block)
     uint32_t P;      // workgroups per launch = number of partial sums
     const uint32_t* rp;
-    const uint32_t* ci;
+    Cols cols;
     const T* val;
     const T* dinv;   // null: no preconditioner
     T* x;

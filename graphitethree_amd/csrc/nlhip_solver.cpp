@@ -43,6 +43,30 @@ static void validate_crs(uint32_t n, const uint32_t* rp, const uint32_t* ci, con
     if (bad_val) throw Error(NLHIP_ERR_NONFINITE, "matrix value is NaN or Inf");
 }
 
+// 16-bit column offsets for the stream engine: kept when every 1024-entry
+// segment of the nonzero stream spans at most 65535 columns (banded matrices:
+// grids in scan order, LSCM and Laplacian systems of meshes with local vertex
+// order); the 32-bit indices stay for the other engines.
+static void build_col_offsets(System& S) {
+    if (S.nnz == 0) return;
+    const uint64_t nseg = (S.nnz + kSegEntries - 1) >> kSegShift;
+    hipStream_t s = S.stream.get();
+    DevBuf<uint32_t> segb(nseg + 1), span(1);
+    NLHIP_CHECK(fill<int32_t>(1, 0, reinterpret_cast<int32_t*>(span.get()), s));
+    NLHIP_CHECK(seg_bases(S.nnz, S.colind.get(), segb.get(), span.get(), s));
+    uint32_t h_span = 0;
+    NLHIP_CHECK(hipMemcpyAsync(&h_span, span.get(), 4, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipStreamSynchronize(s));
+    if (h_span > 0xffffu) return;
+    NLHIP_CHECK(hipMemcpyAsync(segb.get() + nseg, segb.get() + nseg - 1, 4,
+                               hipMemcpyDeviceToDevice, s));
+    DevBuf<uint16_t> ci16(S.nnz);
+    NLHIP_CHECK(seg_encode(S.nnz, S.colind.get(), segb.get(), ci16.get(), s));
+    NLHIP_CHECK(hipStreamSynchronize(s));
+    S.ci16 = std::move(ci16);
+    S.segb = std::move(segb);
+}
+
 System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
                       const double* val, uint32_t flags) {
     if (n == 0 || !rowptr) throw Error(NLHIP_ERR_INVALID, "empty system");
@@ -84,6 +108,7 @@ System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind
     copy_in(S->val.get(), va, nnz * 8, s);
     NLHIP_CHECK(jacobi_dinv<double>(n, S->rowptr.get(), S->colind.get(), S->val.get(),
                                     S->dinv.get(), s));
+    if (!(flags & NLHIP_SYSTEM_NO_OFFSETS)) build_col_offsets(*S);
     default_launch(n, nnz, S->rpw, S->grid);
     if (flags & NLHIP_SYSTEM_FP32) S->ensure_fp32();
     S->stream.sync();
@@ -100,7 +125,8 @@ void System::ensure_fp32() {
 }
 
 uint64_t System::device_bytes() const {
-    return rowptr.bytes() + colind.bytes() + val.bytes() + dinv.bytes() + val32.bytes() +
+    return rowptr.bytes() + colind.bytes() + ci16.bytes() + segb.bytes() + val.bytes() +
+           dinv.bytes() + val32.bytes() +
            dinv32.bytes() + ws64.x.bytes() * 7 + ws32.x.bytes() * 7 + scratch.bytes();
 }
 
@@ -131,7 +157,7 @@ static void setup_args(System& S, CGWorkspace<T>& ws, const T* val, const T* din
     a.chunk = uint32_t((uint64_t(S.n) + S.grid - 1) / S.grid);
     a.sp = spmv_work_per_block(S.n, S.grid, S.rpw);
     a.rp = S.rowptr.get();
-    a.ci = S.colind.get();
+    a.cols = S.cols();
     a.val = val;
     a.dinv = dinv;
     a.x = ws.x.get();
@@ -209,7 +235,7 @@ static void residual_norms(System& S, const double* b, const double* x, double* 
     S.scratch.ensure(size_t(2) * P + 4);
     double* part = S.scratch.get();
     double* out = part + 2 * size_t(P);
-    NLHIP_CHECK(residual64(S.n, S.rowptr.get(), S.colind.get(), S.val.get(), x, b, r_out, part,
+    NLHIP_CHECK(residual64(S.n, S.rowptr.get(), S.cols(), S.val.get(), x, b, r_out, part,
                            part + P, S.rpw, P, s));
     NLHIP_CHECK(reduce_final(part, P, out, s));
     NLHIP_CHECK(reduce_final(part + P, P, out + 2, s));
@@ -337,7 +363,7 @@ uint32_t bicgstab_device(System& S, const double* dinv, const double* b, double*
     S.host_scalars.ensure(4);
     DotHelper dot{S, part, out.get(), S.host_scalars.get()};
     auto spmv64 = [&](const double* in, double* o) {
-        NLHIP_CHECK(spmv<double>(n, S.rowptr.get(), S.colind.get(), S.val.get(), in, o, S.rpw,
+        NLHIP_CHECK(spmv<double>(n, S.rowptr.get(), S.cols(), S.val.get(), in, o, S.rpw,
                                  S.grid, s));
     };
     auto precond = [&](const double* in, double* o) {
@@ -453,7 +479,7 @@ NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,
     return guarded("nlhipSystemSetLaunch", [&] {
         System* S = sys(s);
         if (!valid_rpw(rows_per_wave))
-            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64");
+            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64..71");
         if (workgroups == 0 || workgroups > 65536)
             throw Error(NLHIP_ERR_INVALID, "workgroups must be in [1, 65536]");
         std::lock_guard<std::mutex> lock(S->mu);
@@ -496,7 +522,7 @@ NLAPI int NLAPIENTRY nlhipSpMV(nlhip_system s, const double* x, double* y) {
         hipStream_t st = S->stream.get();
         S->ws64.ensure(S->n, S->grid);
         copy_in(S->ws64.x.get(), x, size_t(S->n) * 8, st);
-        NLHIP_CHECK(spmv<double>(S->n, S->rowptr.get(), S->colind.get(), S->val.get(),
+        NLHIP_CHECK(spmv<double>(S->n, S->rowptr.get(), S->cols(), S->val.get(),
                                  S->ws64.x.get(), S->ws64.q.get(), S->rpw, S->grid, st));
         copy_out(y, S->ws64.q.get(), size_t(S->n) * 8, st);
         S->stream.sync();
@@ -516,7 +542,7 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
             S->ws32.ensure(S->n, S->grid);
             NLHIP_CHECK(fill<float>(S->n, 1.0f, S->ws32.x.get(), st));
             auto run = [&] {
-                NLHIP_CHECK(spmv<float>(S->n, S->rowptr.get(), S->colind.get(), S->val32.get(),
+                NLHIP_CHECK(spmv<float>(S->n, S->rowptr.get(), S->cols(), S->val32.get(),
                                         S->ws32.x.get(), S->ws32.q.get(), S->rpw, S->grid, st));
             };
             run();
@@ -527,7 +553,7 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
             S->ws64.ensure(S->n, S->grid);
             NLHIP_CHECK(fill<double>(S->n, 1.0, S->ws64.x.get(), st));
             auto run = [&] {
-                NLHIP_CHECK(spmv<double>(S->n, S->rowptr.get(), S->colind.get(), S->val.get(),
+                NLHIP_CHECK(spmv<double>(S->n, S->rowptr.get(), S->cols(), S->val.get(),
                                          S->ws64.x.get(), S->ws64.q.get(), S->rpw, S->grid, st));
             };
             run();
@@ -537,6 +563,38 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
         }
         e1.sync();
         if (seconds_per_spmv) *seconds_per_spmv = 1e-3 * elapsed_ms(e0, e1) / reps;
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipBenchCopy(uint64_t bytes, uint32_t reps, double* seconds_per_copy) {
+    return guarded("nlhipBenchCopy", [&] {
+        if (bytes == 0 || bytes % 16 != 0)
+            throw Error(NLHIP_ERR_INVALID, "copy size must be a positive multiple of 16 bytes");
+        const Device& d = device();
+        if (reps == 0) reps = 1;
+        DevBuf<uint8_t> src(bytes), dst(bytes);
+        Stream stream;
+        hipStream_t st = stream.get();
+        NLHIP_CHECK(fill<int32_t>(bytes / 4, 1, reinterpret_cast<int32_t*>(src.get()), st));
+        // best of a few grid sizes: 4, 16 and 64 workgroups per CU, and one
+        // four-word trip per thread
+        const uint64_t n16 = bytes / 16;
+        const uint64_t one_trip = std::max<uint64_t>(1, (n16 + 4 * 256 - 1) / (4 * 256));
+        double best = 0;
+        for (uint64_t g : {uint64_t(d.cu_count) * 4, uint64_t(d.cu_count) * 16,
+                           uint64_t(d.cu_count) * 64, one_trip}) {
+            const uint32_t grid = uint32_t(std::min<uint64_t>(g, 1u << 30));
+            NLHIP_CHECK(copy16(n16, src.get(), dst.get(), grid, st));
+            Event e0, e1;
+            e0.record(st);
+            for (uint32_t k = 0; k < reps; ++k)
+                NLHIP_CHECK(copy16(n16, src.get(), dst.get(), grid, st));
+            e1.record(st);
+            e1.sync();
+            const double t = 1e-3 * elapsed_ms(e0, e1) / reps;
+            if (best == 0 || t < best) best = t;
+        }
+        if (seconds_per_copy) *seconds_per_copy = best;
     });
 }
 

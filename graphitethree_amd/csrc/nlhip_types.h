@@ -4,6 +4,21 @@
 
 namespace nlhip {
 
+// Column indices of a CRS matrix as the SpMV engines read them.  `ci` holds
+// the plain 32-bit indices.  When the matrix is banded enough, `ci16` holds
+// the same indices as 16-bit offsets from a per-segment base: entry j of the
+// nonzero stream has column segb[j >> kSegShift] + ci16[j], and segb holds one
+// base per 1024-entry segment plus one padding entry (ci16 and segb are null
+// otherwise).
+constexpr int kSegShift = 10;
+constexpr uint32_t kSegEntries = 1u << kSegShift;
+
+struct Cols {
+    const uint32_t* ci;
+    const uint16_t* ci16;
+    const uint32_t* segb;
+};
+
 // Device-resident state of one PCG solve.  Each field has exactly one writer
 // per kernel launch (block 0 of the kernel named in the comment); every other
 // block only reads fields written by an earlier launch.

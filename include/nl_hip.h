@@ -56,6 +56,8 @@ NLAPI NLMatrix NLAPIENTRY nlHIPJacobiPreconditionerNewFromCRSMatrix(NLMatrix M);
 #define NLHIP_PREC_MIXED 1   /* FP32 inner PCG inside FP64 refinement        */
 
 #define NLHIP_SYSTEM_FP32 1u /* also keep an FP32 copy of the values         */
+#define NLHIP_SYSTEM_NO_OFFSETS 2u /* read 32-bit column indices only (no
+                                      16-bit segment offsets)               */
 
 typedef struct nlhip_system_s* nlhip_system;
 
@@ -94,7 +96,9 @@ NLAPI int NLAPIENTRY nlhipSystemInfo(nlhip_system s, uint32_t* n, uint64_t* nnz,
                                      uint64_t* device_bytes);
 /* Launch shape of the SpMV engine: rows per wavefront (2, 4, 8, 16: that many
  * rows share a 64-lane wavefront; 64: one row per lane, nonzeros streamed
- * through LDS) and workgroups per launch. */
+ * through LDS, the default) and workgroups per launch.  65..71 select
+ * variants of the streamed engine for launch-shape sweeps (chunk width,
+ * load kind, column format, tile order); they give the same sums as 64. */
 NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,
                                           uint32_t workgroups);
 NLAPI int NLAPIENTRY nlhipSystemGetLaunch(nlhip_system s, int* rows_per_wave,
@@ -118,6 +122,12 @@ NLAPI int NLAPIENTRY nlhipSpMV(nlhip_system s, const double* x, double* y);
  * vectors; returns the mean seconds per SpMV measured with HIP events. */
 NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
                                     double* seconds_per_spmv);
+
+/* Times `reps` plain device-to-device copies of `bytes` (a multiple of 16)
+ * between two buffers it allocates, for a few grid sizes; returns the mean
+ * seconds per copy of the fastest (HIP events).  The bandwidth reference the
+ * solver kernels are compared with. */
+NLAPI int NLAPIENTRY nlhipBenchCopy(uint64_t bytes, uint32_t reps, double* seconds_per_copy);
 
 /* Runs `iters` PCG iterations (FP64, or FP32 when fp32 = 1, b = 1) launching
  * each kernel directly; seconds[0..2] receive the mean HIP-event duration of

@@ -23,7 +23,7 @@ def test_spmv_exact_all_launch_shapes(hip):
     for name in z["names"]:
         rp, ci, va = z[f"{name}_rowptr"], z[f"{name}_colind"], z[f"{name}_val"]
         S = capi.System(rp, ci, va)
-        for rpw in (2, 4, 8, 16, 64):
+        for rpw in (2, 4, 8, 16, 64, 65, 66, 67, 68, 69, 70, 71):
             for wg in (1, 3, 64, 2048):
                 S.set_launch(rpw, wg)
                 y = S.spmv(z[f"{name}_x"])
@@ -49,9 +49,53 @@ def test_stream_engine_is_bitwise_the_oracle_spmv(hip):
         A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
         x = np.random.default_rng(4).standard_normal(Sw.n)
         S = capi.System.from_crs(Sw)
-        for wg in (1, 5, 300):
-            S.set_launch(64, wg)
-            assert np.array_equal(S.spmv(x), O.spmv(A, x))
+        ref = O.spmv(A, x)
+        for rpw in (64, 65, 66, 67, 68, 69, 70, 71):
+            for wg in (1, 5, 9, 300):
+                S.set_launch(rpw, wg)
+                assert np.array_equal(S.spmv(x), ref), (rpw, wg)
+
+
+def _crs_from_rows(n, rows):
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.concatenate([np.sort(np.asarray(r, dtype=np.int64)) for r in rows]).astype(np.uint32)
+    return rp, ci
+
+
+def test_column_offsets_match_32bit_indices(hip):
+    """The stream engine reading 16-bit segment offsets gives the bits of the
+    oracle's row loop, on ragged banded rows (empty rows, rows longer than a
+    segment, nnz not a multiple of the segment) and on a matrix whose column
+    span is too wide for offsets (which keeps the 32-bit indices)."""
+    rng = np.random.default_rng(11)
+    cases = []
+    n = 70000
+    rows = []
+    for i in range(n):
+        k = 0 if i % 97 == 0 else (2500 if i == 1234 else int(rng.integers(1, 40)))
+        lo, hi = max(0, i - 20000), min(n, i + 20000)
+        rows.append(rng.choice(np.arange(lo, hi), size=min(k, hi - lo), replace=False))
+    cases.append(("banded", n, *_crs_from_rows(n, rows), True))
+    n = 90000
+    rows = [np.unique(np.r_[i, rng.integers(0, n, size=5)]) for i in range(n)]
+    cases.append(("wide", n, *_crs_from_rows(n, rows), False))
+    for name, n, rp, ci, expect_offsets in cases:
+        va = rng.standard_normal(len(ci))
+        x = rng.standard_normal(n)
+        ref = O.spmv(O.CRS(rp, ci, va), x)
+        S = capi.System(rp, ci, va)
+        P = capi.System(rp, ci, va, offsets=False)
+        extra = S.info()[2] - P.info()[2]
+        assert (extra > 0) == expect_offsets, (name, extra)
+        for rpw in (64, 68, 70):
+            for wg in (1, 7, 1024):
+                S.set_launch(rpw, wg)
+                P.set_launch(rpw, wg)
+                assert np.array_equal(S.spmv(x), ref), (name, rpw, wg)
+                assert np.array_equal(P.spmv(x), ref), (name, rpw, wg)
+        S.close()
+        P.close()
 
 
 def test_set_launch_rejects_bad_shapes(hip):
