@@ -56,13 +56,24 @@ def spmv_bytes(n: int, nnz: int, sv: int = 8, sx: int = 8) -> float:
 
 
 def cg_direction_bytes(n: int, s: int = 8) -> float:
-    """d = beta d + h: d and h read, d written."""
-    return 3 * n * s
+    """x -= alpha d; d = beta d + h: x, d and h read, x and d written."""
+    return 5 * n * s
 
 
 def cg_update_bytes(n: int, s: int = 8) -> float:
-    """x, d, r, q, dinv read; x, r, h written."""
-    return 8 * n * s
+    """r -= alpha q; h = D^-1 r: r, q, dinv read; r, h written."""
+    return 5 * n * s
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
 
 
 def main():
@@ -78,6 +89,8 @@ def main():
                     help="oracle PCG iterations timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--rows-per-wave", type=int, default=0,
+                    help="SpMV engine code for the C2 system (0: the library default)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only launch the kernels whose HBM counters are read under "
                          "rocprofv3 --pmc: a 1 GiB plain copy (x3) and 10 PCG iterations")
@@ -103,6 +116,8 @@ def main():
     log(f"LSCM grid {args.k}^2: n={n} nnz={nnz} built in {time.time() - t0:.1f}s")
     t0 = time.time()
     S = capi.System.from_crs(Sw, fp32=not args.no_mixed)
+    if args.rows_per_wave:
+        S.set_launch(args.rows_per_wave, S.launch()[1])
     lpr, wg = S.launch()
     log(f"uploaded in {time.time() - t0:.1f}s; launch rows/wavefront={lpr} workgroups={wg}")
 
@@ -183,13 +198,18 @@ def main():
         secs = O.pcg_timed_iters(A, b, args.cpu_iters, variant=1)
         per_iter = secs / args.cpu_iters
         cpu_solve_s = per_iter * max(iters, 1)
+        secs_all = O.pcg_timed_iters(A, b, args.cpu_iters, variant=2)
         cpu = {"value": 1.0 / cpu_solve_s, "unit": "solves/s", "cores": threads,
                "kind": "port",
                "sample": f"{args.cpu_iters} Jacobi-PCG iterations of the oracle (OpenMP "
                          f"SpMV, serial BLAS-1, OpenNL operation order) on this system, "
-                         f"{secs:.1f}s, scaled to the {iters}-iteration solve"}
+                         f"{secs:.1f}s, scaled to the {iters}-iteration solve",
+               "all_openmp_value": 1.0 / (secs_all / args.cpu_iters * max(iters, 1)),
+               "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+               "omp_proc_bind": os.environ.get("OMP_PROC_BIND", "")}
         log(f"cpu baseline: {per_iter * 1e3:.1f} ms/iter on {threads} threads -> "
-            f"{cpu['value']:.4f} solves/s")
+            f"{cpu['value']:.4f} solves/s (all-OpenMP {secs_all / args.cpu_iters * 1e3:.1f} "
+            f"ms/iter)")
         del A
 
     # ---- C1 (317 x 317, cache-resident) to full convergence, no cap

@@ -358,13 +358,18 @@ template <>
 struct EngineOf<71> {  // 32-bit indices, contiguous quarters
     using type = StreamEngine<kStreamE, true>;
 };
+template <>
+struct EngineOf<72> {  // 16-bit column offsets, 16 entries per lane per chunk
+    using type = StreamEngine<16, true, true>;
+};
 
 template <int RPW>
 struct StageSize {
-    static constexpr int value = (RPW == 64 || RPW >= 68) ? kWavesPerBlock * 64 * kStreamE
+    static constexpr int value = (RPW == 64 || (RPW >= 68 && RPW <= 71))
+                                     ? kWavesPerBlock * 64 * kStreamE
                                  : RPW == 65 ? kWavesPerBlock * 64 * 4
                                  : RPW == 66 ? kWavesPerBlock * 64 * 8
-                                 : RPW == 67 ? kWavesPerBlock * 64 * 16
+                                 : (RPW == 67 || RPW == 72) ? kWavesPerBlock * 64 * 16
                                              : 1;
 };
 

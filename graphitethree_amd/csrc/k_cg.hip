@@ -11,17 +11,22 @@
 // and the stopping rule |Ax-b|/|b| < eps of matrix.h:177-179.
 //
 // One iteration is three launches and no host round trip:
-//   direction  reduce the r.h and r.r partials -> rho, err; stopping test;
+//   direction  reduce the r.h and r.r partials -> rho, err; x -= alpha d
+//              with the previous iteration's alpha and d (the update of x
+//              is deferred to here, where d is read anyway); stopping test;
 //              beta = rho / rho_prev; d = beta d + h (d = h at it 0)
 //   spmv       q = A d and partials of d.q
-//   update     reduce the d.q partials -> alpha; x -= alpha d;
-//              r -= alpha q; h = P r; partials of r.h and r.r
-// Per iteration that moves 12 nnz + 108 n bytes in FP64 (values, colind,
+//   update     reduce the d.q partials -> alpha; r -= alpha q; h = P r;
+//              partials of r.h and r.r
+// Per iteration that moves 12 nnz + 100 n bytes in FP64 with 32-bit column
+// indices, 10 nnz + 100 n with 16-bit offsets (values, column indices,
 // rowptr, d gathered once, and the vector streams).  Every block reduces the
 // partials of the previous launch itself, in a fixed order, so all blocks
-// hold bitwise-identical scalars.  Once the stopping test holds, later
-// launches exit at once, so the host can replay a captured graph of many
-// iterations and read the status afterwards.
+// hold bitwise-identical scalars.  When the stopping test holds, the
+// direction launch still applies the pending x update and raises `stop`; the
+// next launch turns it into `done`, after which every launch exits at once,
+// so the host can replay a captured graph of many iterations and read the
+// status afterwards.
 #include "dev_common.h"
 #include "nlhip_kernels.h"
 
@@ -79,6 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_finish(CGArgs<T> a, T eps, i
         st->iter_b = 0;
         st->iters = 0;
         st->max_iter = max_iter;
+        st->stop = 0;
         __hip_atomic_store(&st->done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -88,34 +94,41 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(CGArgs<T> a) {
     __shared__ T lds[kWavesPerBlock];
     __shared__ int s_flag;
     CGStatus<T>* st = a.st;
+    // `done` is only raised by a later launch, never by this one, so every
+    // block of the launch that stops still applies its part of the x update
     if (block_done(st, &s_flag)) return;
     const int it = st->iter_a;
     const T rho = reduce_partials(a.part_rh, a.P, lds);
     const T err = reduce_partials(a.part_rr, a.P, lds);
-    if (!(err > st->thresh) || it >= st->max_iter) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            st->iters = it;
-            st->err = err;
-            st->rho = rho;
-            __hip_atomic_store(&st->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        return;
-    }
+    const bool stop = !(err > st->thresh) || it >= st->max_iter;
+    const T alpha = st->alpha;  // of iteration it - 1
     const T beta = (it > 0) ? rho / st->rho_prev : T(0);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->rho = rho;
-        st->beta = beta;
         st->err = err;
-        st->iter_b = it;
+        if (stop) {
+            st->iters = it;
+            st->stop = 1;
+        } else {
+            st->beta = beta;
+            st->iter_b = it;
+        }
     }
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     uint32_t r0, r1;
     block_row_range(a.m, a.chunk, L, r0, r1);
     if (it == 0) {
-        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) a.d[i] = a.h[i];
-    } else {
+        if (!stop)
+            for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) a.d[i] = a.h[i];
+    } else if (stop) {
         for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock)
-            a.d[i] = add_rn(mul_rn(beta, a.d[i]), a.h[i]);
+            a.x[i] = add_rn(a.x[i], mul_rn(-alpha, a.d[i]));
+    } else {
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+            const T di = a.d[i];
+            a.x[i] = add_rn(a.x[i], mul_rn(-alpha, di));
+            a.d[i] = add_rn(mul_rn(beta, di), a.h[i]);
+        }
     }
 }
 
@@ -124,7 +137,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(CGArgs<T> a) {
     __shared__ T stage[StageSize<RPW>::value];
     __shared__ T lds[kWavesPerBlock];
     __shared__ int s_flag;
-    if (block_done(a.st, &s_flag)) return;
+    if (threadIdx.x == 0) {
+        CGStatus<T>* st = a.st;
+        int f = __hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!f && st->stop) {
+            f = 1;
+            if (blockIdx.x == 0)
+                __hip_atomic_store(&st->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_flag = f;
+    }
+    __syncthreads();
+    if (s_flag) return;
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const T* __restrict__ d = a.d;
     T pdq = T(0);
@@ -147,16 +171,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CGArgs<T> a) {
     const T dq = reduce_partials(a.part_dq, a.P, lds);
     const T rho = st->rho;
     const T alpha = rho / dq;
-    const T* __restrict__ d = a.d;
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     uint32_t r0, r1;
     block_row_range(a.m, a.chunk, L, r0, r1);
     T prh = T(0), prr = T(0);
     for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
-        const T xi = add_rn(a.x[i], mul_rn(-alpha, d[i]));
         const T ri = add_rn(a.r[i], mul_rn(-alpha, a.q[i]));
         const T hi = PREC ? mul_rn(a.dinv[i], ri) : ri;
-        a.x[i] = xi;
         a.r[i] = ri;
         a.h[i] = hi;
         prh += ri * hi;

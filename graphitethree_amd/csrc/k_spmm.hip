@@ -20,41 +20,89 @@
 
 namespace nlhip {
 
-constexpr int kSpmmU = 4;  // entries per lane per step
+constexpr uint32_t kSpmmChunk = 1024;  // (colind, val) entries staged in LDS at a time
+constexpr uint32_t kSpmmPass = 64;     // rows per pass (at least)
+constexpr int kSpmmU = 8;              // gathers in flight per lane
 
+// Rows of logical block L, in passes of max(64, kBlock / G) consecutive rows:
+// the block copies the pass's (colind, val) range into LDS with coalesced
+// loads (in chunks of kSpmmChunk entries), then each G-lane group sums its
+// rows of the pass from LDS, lane j gathering X[col * k + j].  Row sums are
+// left to right with one rounding per product and per add.  Every thread
+// of the block must call this (it synchronises the block).
 template <int G, typename T, typename Epi>
 __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_per_block,
                                           uint32_t L, const uint32_t* __restrict__ rp,
                                           const uint32_t* __restrict__ ci,
                                           const T* __restrict__ val, const T* __restrict__ X,
-                                          Epi& epi) {
+                                          Epi& epi, const int32_t* done = nullptr) {
+    __shared__ uint32_t s_ci[kSpmmChunk];
+    __shared__ T s_val[kSpmmChunk];
+    constexpr uint32_t groups = kBlock / G;
+    constexpr uint32_t pass = groups > kSpmmPass ? groups : kSpmmPass;
     const uint32_t g = threadIdx.x / G, j = threadIdx.x % G;
-    const uint32_t groups = kBlock / G;
-    const uint64_t rb = uint64_t(L) * rows_per_block;
-    const uint32_t r0 = rb < m ? uint32_t(rb) : m;
+    // columns that have stopped gather nothing (their sums are not used)
+    const bool active_col = j < k && !(done && done[j]);
+    const uint64_t rb0 = uint64_t(L) * rows_per_block;
+    const uint32_t r0 = rb0 < m ? uint32_t(rb0) : m;
     const uint32_t r1 = (uint64_t(r0) + rows_per_block < m) ? r0 + rows_per_block : m;
-    const bool active_col = j < k;
-    for (uint32_t row = r0 + g; row < r1; row += groups) {
-        const uint32_t b = rp[row], e = rp[row + 1];
-        T acc = T(0);
-        for (uint32_t p = b; p < e; p += kSpmmU) {
-            uint32_t c[kSpmmU];
-            T a[kSpmmU], xv[kSpmmU];
+    constexpr uint32_t R = pass / groups;  // rows per group in one pass
+    for (uint32_t a = r0; a < r1; a += pass) {
+        const uint32_t pe = (a + pass < r1) ? a + pass : r1;
+        const uint32_t J0 = rp[a], J1 = rp[pe];
+        uint32_t rb[R], re[R];
+        T acc[R];
 #pragma unroll
-            for (int u = 0; u < kSpmmU; ++u) {
-                const uint32_t q = p + u < e ? p + u : e - 1u;
-                c[u] = ci[q];
-                a[u] = val[q];
+        for (uint32_t t = 0; t < R; ++t) {
+            const uint32_t row = a + t * groups + g;
+            rb[t] = re[t] = 0;
+            if (row < pe) {
+                rb[t] = rp[row];
+                re[t] = rp[row + 1];
             }
-#pragma unroll
-            for (int u = 0; u < kSpmmU; ++u)
-                xv[u] = active_col ? X[uint64_t(c[u]) * k + j] : T(0);
-#pragma unroll
-            for (int u = 0; u < kSpmmU; ++u)
-                if (p + u < e) acc = add_rn(acc, mul_rn(a[u], xv[u]));
+            acc[t] = T(0);
         }
-        if (active_col) epi(row, j, acc);
+        for (uint32_t c0 = J0; c0 < J1; c0 += kSpmmChunk) {
+            const uint32_t c1 = (c0 + kSpmmChunk < J1) ? c0 + kSpmmChunk : J1;
+            __syncthreads();  // the previous chunk has been consumed
+            for (uint32_t q = c0 + threadIdx.x; q < c1; q += kBlock) {
+                s_ci[q - c0] = ci[q];
+                s_val[q - c0] = val[q];
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t t = 0; t < R; ++t) {
+                const uint32_t lo = rb[t] > c0 ? rb[t] : c0, hi = re[t] < c1 ? re[t] : c1;
+                for (uint32_t p = lo; p < hi; p += kSpmmU) {
+                    T av[kSpmmU], xv[kSpmmU];
+#pragma unroll
+                    for (int u = 0; u < kSpmmU; ++u) {
+                        const uint32_t q = (p + u < hi ? p + u : hi - 1u) - c0;
+                        av[u] = s_val[q];
+                        xv[u] = active_col ? X[uint64_t(s_ci[q]) * k + j] : T(0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kSpmmU; ++u)
+                        if (p + u < hi) acc[t] = add_rn(acc[t], mul_rn(av[u], xv[u]));
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t t = 0; t < R; ++t) {
+            const uint32_t row = a + t * groups + g;
+            if (j < k && row < pe) epi(row, j, acc[t]);
+        }
     }
+}
+
+// Rows [r0, r1) of logical block L for the element-wise kernels, with the
+// same G-lane groups as spmm_rows: thread t handles column t % G of rows
+// r0 + t / G, r0 + t / G + kBlock / G, ...
+__device__ __forceinline__ void block_rows(const uint32_t m, const uint32_t rows_per_block,
+                                           uint32_t L, uint32_t& r0, uint32_t& r1) {
+    const uint64_t rb = uint64_t(L) * rows_per_block;
+    r0 = rb < m ? uint32_t(rb) : m;
+    r1 = (uint64_t(r0) + rows_per_block < m) ? r0 + rows_per_block : m;
 }
 
 // Per-workgroup partial sums of k columns: lds[kBlock] scratch; the group
@@ -95,23 +143,32 @@ __global__ __launch_bounds__(kBlock) void k_m_init(MultiArgs<T> a) {
     column_partials(pbb, k, G, lds, a.part_bb + size_t(L) * k);
 }
 
-// One wavefront per column: sum of P partials (stride k), fixed order.
+// One workgroup per column: sum of the P partials of column j (stride k) in
+// a fixed order (thread t takes partials t, t + 256, ..., then the block
+// sums the threads); every thread gets the total.
 template <typename T>
-__device__ __forceinline__ T col_sum(const T* part, uint32_t P, uint32_t k, uint32_t j) {
-    const int lane = threadIdx.x & (kWave - 1);
+__device__ __forceinline__ T col_sum(const T* part, uint32_t P, uint32_t k, uint32_t j,
+                                     T* lds) {
     T v = T(0);
-    for (uint32_t p = uint32_t(lane); p < P; p += kWave) v += part[size_t(p) * k + j];
-    return group_sum<kWave>(v);
+    uint32_t p = threadIdx.x;
+    for (; p + 3 * kBlock < P; p += 4 * kBlock) {
+        const T a = part[size_t(p) * k + j], b = part[size_t(p + kBlock) * k + j];
+        const T c = part[size_t(p + 2 * kBlock) * k + j];
+        const T d = part[size_t(p + 3 * kBlock) * k + j];
+        v = v + a + b + c + d;
+    }
+    for (; p < P; p += kBlock) v += part[size_t(p) * k + j];
+    return block_sum_all(v, lds);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_m_sys_init(MultiArgs<T> a, T eps, int max_iter) {
-    const uint32_t j = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    if (j >= a.k) return;
-    const T rho = col_sum(a.part_rh, a.P, a.k, j);
-    const T err = col_sum(a.part_rr, a.P, a.k, j);
-    const T bb = col_sum(a.part_bb, a.P, a.k, j);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
+    __shared__ T lds[kWavesPerBlock];
+    const uint32_t j = blockIdx.x;
+    const T rho = col_sum(a.part_rh, a.P, a.k, j, lds);
+    const T err = col_sum(a.part_rr, a.P, a.k, j, lds);
+    const T bb = col_sum(a.part_bb, a.P, a.k, j, lds);
+    if (threadIdx.x == 0) {
         const T thresh = (eps * eps) * bb;
         a.bb[j] = bb;
         a.thresh[j] = thresh;
@@ -127,19 +184,31 @@ __global__ __launch_bounds__(kBlock) void k_m_sys_init(MultiArgs<T> a, T eps, in
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_m_dir(MultiArgs<T> a) {
-    const uint32_t k = a.k;
-    const uint64_t total = uint64_t(a.m) * k;
-    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
-    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < total; i += stride) {
-        const uint32_t j = uint32_t(i % k);
-        if (a.done[j]) continue;
-        a.d[i] = (a.iters[j] == 0) ? a.h[i] : add_rn(mul_rn(a.beta[j], a.d[i]), a.h[i]);
+    if (*a.n_active == 0) return;  // every column has stopped
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    const uint32_t k = a.k, G = a.G;
+    const uint32_t g = threadIdx.x / G, j = threadIdx.x % G;
+    if (j >= k || a.done[j]) return;
+    uint32_t r0, r1;
+    block_rows(a.m, a.rows_per_block, L, r0, r1);
+    if (a.iters[j] == 0) {
+        for (uint32_t row = r0 + g; row < r1; row += kBlock / G) {
+            const uint64_t i = uint64_t(row) * k + j;
+            a.d[i] = a.h[i];
+        }
+    } else {
+        const T beta = a.beta[j];
+        for (uint32_t row = r0 + g; row < r1; row += kBlock / G) {
+            const uint64_t i = uint64_t(row) * k + j;
+            a.d[i] = add_rn(mul_rn(beta, a.d[i]), a.h[i]);
+        }
     }
 }
 
 template <int G, typename T>
 __global__ __launch_bounds__(kBlock) void k_m_spmm(MultiArgs<T> a) {
     __shared__ T lds[kBlock];
+    if (*a.n_active == 0) return;  // every column has stopped (block-uniform)
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t k = a.k;
     T pdq = T(0);
@@ -148,29 +217,30 @@ __global__ __launch_bounds__(kBlock) void k_m_spmm(MultiArgs<T> a) {
         a.q[i] = s;
         pdq += a.d[i] * s;
     };
-    spmm_rows<G>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.d, epi);
+    spmm_rows<G>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.d, epi, a.done);
     column_partials(pdq, k, G, lds, a.part_dq + size_t(L) * k);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_m_reduce_alpha(MultiArgs<T> a) {
-    const uint32_t j = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    if (j >= a.k || a.done[j]) return;
-    const T dq = col_sum(a.part_dq, a.P, a.k, j);
-    if ((threadIdx.x & (kWave - 1)) == 0) a.alpha[j] = a.rho[j] / dq;
+    __shared__ T lds[kWavesPerBlock];
+    const uint32_t j = blockIdx.x;
+    if (a.done[j]) return;
+    const T dq = col_sum(a.part_dq, a.P, a.k, j, lds);
+    if (threadIdx.x == 0) a.alpha[j] = a.rho[j] / dq;
 }
 
 template <typename T, bool PREC>
 __global__ __launch_bounds__(kBlock) void k_m_update(MultiArgs<T> a) {
     __shared__ T lds[kBlock];
+    if (*a.n_active == 0) return;  // every column has stopped (block-uniform)
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t k = a.k;
     // element-wise over the same rows as the SpMM block, G lanes per row
     const uint32_t G = a.G;
     const uint32_t g = threadIdx.x / G, j = threadIdx.x % G;
-    const uint64_t rb = uint64_t(L) * a.rows_per_block;
-    const uint32_t r0 = rb < a.m ? uint32_t(rb) : a.m;
-    const uint32_t r1 = (uint64_t(r0) + a.rows_per_block < a.m) ? r0 + a.rows_per_block : a.m;
+    uint32_t r0, r1;
+    block_rows(a.m, a.rows_per_block, L, r0, r1);
     T prh = T(0), prr = T(0);
     if (j < k && !a.done[j]) {
         const T alpha = a.alpha[j];
@@ -192,11 +262,12 @@ __global__ __launch_bounds__(kBlock) void k_m_update(MultiArgs<T> a) {
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_m_reduce_beta(MultiArgs<T> a, int max_iter) {
-    const uint32_t j = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    if (j >= a.k || a.done[j]) return;
-    const T rho_new = col_sum(a.part_rh, a.P, a.k, j);
-    const T err = col_sum(a.part_rr, a.P, a.k, j);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
+    __shared__ T lds[kWavesPerBlock];
+    const uint32_t j = blockIdx.x;
+    if (a.done[j]) return;
+    const T rho_new = col_sum(a.part_rh, a.P, a.k, j, lds);
+    const T err = col_sum(a.part_rr, a.P, a.k, j, lds);
+    if (threadIdx.x == 0) {
         const int it = a.iters[j] + 1;
         a.iters[j] = it;
         a.err[j] = err;
@@ -221,29 +292,26 @@ __global__ void k_m_count(MultiArgs<T> a) {
 
 // ---------------------------------------------------- layout / refinement
 
-// interleaved[i*k + j] = colmajor[j*n + i] (and back)
+// interleaved[i*k + j] = colmajor[j*n + i] (and back), G-lane groups per
+// row (G = k rounded up to a power of two), grid-stride over rows.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_to_interleaved(uint32_t n, uint32_t k,
+__global__ __launch_bounds__(kBlock) void k_to_interleaved(uint32_t n, uint32_t k, uint32_t G,
                                                            const T* __restrict__ cm,
                                                            T* __restrict__ il) {
-    const uint64_t total = uint64_t(n) * k;
-    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
-    for (uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x; t < total; t += stride) {
-        const uint64_t i = t / k, j = t % k;
-        il[t] = cm[j * n + i];
-    }
+    const uint32_t g = threadIdx.x / G, j = threadIdx.x % G, per = kBlock / G;
+    if (j >= k) return;
+    for (uint64_t i = uint64_t(blockIdx.x) * per + g; i < n; i += uint64_t(gridDim.x) * per)
+        il[i * k + j] = cm[uint64_t(j) * n + i];
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_to_colmajor(uint32_t n, uint32_t k,
+__global__ __launch_bounds__(kBlock) void k_to_colmajor(uint32_t n, uint32_t k, uint32_t G,
                                                         const T* __restrict__ il,
                                                         T* __restrict__ cm) {
-    const uint64_t total = uint64_t(n) * k;
-    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
-    for (uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x; t < total; t += stride) {
-        const uint64_t j = t / n, i = t % n;
-        cm[t] = il[i * k + j];
-    }
+    const uint32_t g = threadIdx.x / G, j = threadIdx.x % G, per = kBlock / G;
+    if (j >= k) return;
+    for (uint64_t i = uint64_t(blockIdx.x) * per + g; i < n; i += uint64_t(gridDim.x) * per)
+        cm[uint64_t(j) * n + i] = il[i * k + j];
 }
 
 // R = B - A X (FP64, interleaved), per-column partial |r|^2 and |b|^2.
@@ -267,11 +335,11 @@ __global__ __launch_bounds__(kBlock) void k_m_residual64(MultiArgs<double> a) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_m_norms64(MultiArgs<double> a) {
-    const uint32_t j = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    if (j >= a.k) return;
-    const double rr = col_sum(a.part_rr, a.P, a.k, j);
-    const double bb = col_sum(a.part_bb, a.P, a.k, j);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
+    __shared__ double lds[kWavesPerBlock];
+    const uint32_t j = blockIdx.x;
+    const double rr = col_sum(a.part_rr, a.P, a.k, j, lds);
+    const double bb = col_sum(a.part_bb, a.P, a.k, j, lds);
+    if (threadIdx.x == 0) {
         a.err[j] = rr;
         a.bb[j] = bb;
     }
@@ -281,11 +349,16 @@ __global__ __launch_bounds__(kBlock) void k_m_norms64(MultiArgs<double> a) {
 __global__ __launch_bounds__(kBlock) void k_m_scale_cast(MultiArgs<double> a64,
                                                          MultiArgs<float> a32,
                                                          const double* scale) {
-    const uint32_t k = a64.k;
-    const uint64_t total = uint64_t(a64.m) * k;
-    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
-    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < total; i += stride) {
-        const_cast<float*>(a32.b)[i] = float(a64.r[i] * scale[i % k]);
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    const uint32_t k = a64.k, G = a64.G;
+    const uint32_t g = threadIdx.x / G, j = threadIdx.x % G;
+    if (j >= k) return;
+    const double sj = scale[j];
+    uint32_t r0, r1;
+    block_rows(a64.m, a64.rows_per_block, L, r0, r1);
+    for (uint32_t row = r0 + g; row < r1; row += kBlock / G) {
+        const uint64_t i = uint64_t(row) * k + j;
+        const_cast<float*>(a32.b)[i] = float(a64.r[i] * sj);
         a32.x[i] = 0.0f;
     }
 }
@@ -293,13 +366,16 @@ __global__ __launch_bounds__(kBlock) void k_m_scale_cast(MultiArgs<double> a64,
 __global__ __launch_bounds__(kBlock) void k_m_ir_update(MultiArgs<double> a64,
                                                         MultiArgs<float> a32,
                                                         const double* unscale) {
-    const uint32_t k = a64.k;
-    const uint64_t total = uint64_t(a64.m) * k;
-    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
-    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < total; i += stride) {
-        const uint32_t j = uint32_t(i % k);
-        if (a32.skip[j]) continue;
-        a64.x[i] = a64.x[i] + unscale[j] * double(a32.x[i]);
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    const uint32_t k = a64.k, G = a64.G;
+    const uint32_t g = threadIdx.x / G, j = threadIdx.x % G;
+    if (j >= k || a32.skip[j]) return;
+    const double uj = unscale[j];
+    uint32_t r0, r1;
+    block_rows(a64.m, a64.rows_per_block, L, r0, r1);
+    for (uint32_t row = r0 + g; row < r1; row += kBlock / G) {
+        const uint64_t i = uint64_t(row) * k + j;
+        a64.x[i] = a64.x[i] + uj * double(a32.x[i]);
     }
 }
 
@@ -328,16 +404,15 @@ hipError_t multi_start(const MultiArgs<T>& a, bool precond, T eps, int max_iter,
     } else {
         NLHIP_G_SWITCH(a.G, (k_m_init<G, T, false><<<a.P, kBlock, 0, s>>>(a)));
     }
-    k_m_sys_init<T><<<(a.k + kWavesPerBlock - 1) / kWavesPerBlock, kBlock, 0, s>>>(a, eps,
-                                                                                   max_iter);
+    k_m_sys_init<T><<<a.k, kBlock, 0, s>>>(a, eps, max_iter);
     k_m_count<T><<<1, 64, 0, s>>>(a);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t multi_iteration(const MultiArgs<T>& a, bool precond, int max_iter, hipStream_t s) {
-    const uint32_t cg = (a.k + kWavesPerBlock - 1) / kWavesPerBlock;
-    k_m_dir<T><<<ew_grid_m(uint64_t(a.m) * a.k), kBlock, 0, s>>>(a);
+    const uint32_t cg = a.k;  // one workgroup per column for the reductions
+    k_m_dir<T><<<a.P, kBlock, 0, s>>>(a);
     NLHIP_G_SWITCH(a.G, (k_m_spmm<G, T><<<a.P, kBlock, 0, s>>>(a)));
     k_m_reduce_alpha<T><<<cg, kBlock, 0, s>>>(a);
     if (precond) k_m_update<T, true><<<a.P, kBlock, 0, s>>>(a);
@@ -350,27 +425,30 @@ hipError_t multi_iteration(const MultiArgs<T>& a, bool precond, int max_iter, hi
 template <typename T>
 hipError_t multi_layout(uint32_t n, uint32_t k, const T* src, T* dst, bool to_interleaved,
                         hipStream_t s) {
-    const uint32_t g = ew_grid_m(uint64_t(n) * k);
-    if (to_interleaved) k_to_interleaved<T><<<g, kBlock, 0, s>>>(n, k, src, dst);
-    else k_to_colmajor<T><<<g, kBlock, 0, s>>>(n, k, src, dst);
+    if (k == 0 || k > 32) return hipErrorInvalidValue;
+    uint32_t G = 1;
+    while (G < k) G <<= 1;
+    const uint32_t g = ew_grid_m(uint64_t(n) * G);
+    if (to_interleaved) k_to_interleaved<T><<<g, kBlock, 0, s>>>(n, k, G, src, dst);
+    else k_to_colmajor<T><<<g, kBlock, 0, s>>>(n, k, G, src, dst);
     return hipGetLastError();
 }
 
 hipError_t multi_residual64(const MultiArgs<double>& a, hipStream_t s) {
     NLHIP_G_SWITCH(a.G, (k_m_residual64<G><<<a.P, kBlock, 0, s>>>(a)));
-    k_m_norms64<<<(a.k + kWavesPerBlock - 1) / kWavesPerBlock, kBlock, 0, s>>>(a);
+    k_m_norms64<<<a.k, kBlock, 0, s>>>(a);
     return hipGetLastError();
 }
 
 hipError_t multi_scale_cast(const MultiArgs<double>& a64, const MultiArgs<float>& a32,
                             const double* scale, hipStream_t s) {
-    k_m_scale_cast<<<ew_grid_m(uint64_t(a64.m) * a64.k), kBlock, 0, s>>>(a64, a32, scale);
+    k_m_scale_cast<<<a64.P, kBlock, 0, s>>>(a64, a32, scale);
     return hipGetLastError();
 }
 
 hipError_t multi_ir_update(const MultiArgs<double>& a64, const MultiArgs<float>& a32,
                            const double* unscale, hipStream_t s) {
-    k_m_ir_update<<<ew_grid_m(uint64_t(a64.m) * a64.k), kBlock, 0, s>>>(a64, a32, unscale);
+    k_m_ir_update<<<a64.P, kBlock, 0, s>>>(a64, a32, unscale);
     return hipGetLastError();
 }
 

@@ -311,6 +311,7 @@ void* b_malloc(NLBlas_t blas, NLmemoryType type, size_t size) {
     guarded("NLBlas::Malloc", [&] {
         if (type == NL_DEVICE_MEMORY) {
             device();
+            auto g = no_capture_guard();
             NLHIP_CHECK(hipMalloc(&p, size ? size : 1));
         } else {
             p = std::malloc(size ? size : 1);
@@ -324,15 +325,22 @@ void* b_malloc(NLBlas_t blas, NLmemoryType type, size_t size) {
 
 void b_free(NLBlas_t blas, NLmemoryType type, size_t size, void* p) {
     if (!p) return;
-    if (type == NL_DEVICE_MEMORY) (void)hipFree(p);
-    else std::free(p);
+    if (type == NL_DEVICE_MEMORY) {
+        auto g = no_capture_guard();
+        (void)hipFree(p);
+    } else {
+        std::free(p);
+    }
     blas->used_ram[type] -= std::min<NLulong>(blas->used_ram[type], size);
 }
 
 void b_memcpy(NLBlas_t, void* to, NLmemoryType, const void* from, NLmemoryType, size_t size) {
     guarded("NLBlas::Memcpy", [&] {
         device();
-        NLHIP_CHECK(hipMemcpy(to, from, size, hipMemcpyDefault));
+        Util& u = util();
+        std::lock_guard<std::mutex> lock(u.mu);
+        NLHIP_CHECK(hipMemcpyAsync(to, from, size, hipMemcpyDefault, u.stream.get()));
+        u.stream.sync();
     });
 }
 
@@ -453,7 +461,13 @@ void h_dzero(NLBlas_t, int n, double* x) {
 void d_dcopy(NLBlas_t, int n, const double* x, int incx, double* y, int incy) {
     guarded("nlHIPBlas()->Dcopy", [&] {
         check_inc(incx, incy);
-        if (n > 0) NLHIP_CHECK(hipMemcpy(y, x, sizeof(double) * size_t(n), hipMemcpyDefault));
+        if (n > 0) {
+            Util& u = util();
+            std::lock_guard<std::mutex> lock(u.mu);
+            NLHIP_CHECK(hipMemcpyAsync(y, x, sizeof(double) * size_t(n), hipMemcpyDefault,
+                                       u.stream.get()));
+            u.stream.sync();
+        }
     });
 }
 

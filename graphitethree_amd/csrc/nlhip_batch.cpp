@@ -21,10 +21,9 @@ struct BatchWork {
     DevBuf<int32_t> iters, done, skip;
     DevBuf<uint32_t> lists, counts;         // act_tiles | act_sys ; 2 counters
     PinnedBuf<uint32_t> host_counts;        // two slots
-    Graph graph;
+    GraphLadder graph;
     BatchArgs<T> args{};
     BatchArgs<T> graph_args{};
-    uint32_t graph_iters = 0;
     int graph_max_iter = -1;
     bool graph_precond = false;
 };
@@ -104,25 +103,24 @@ template <typename T>
 void run_batch(Batch& B, BatchWork<T>& w, bool precond, T eps, int max_iter, uint32_t K) {
     hipStream_t s = B.stream.get();
     NLHIP_CHECK(batch_start<T>(w.args, precond, eps, max_iter, B.grid, s));
-    if (!w.graph.ready() || w.graph_iters != K || w.graph_precond != precond ||
-        w.graph_max_iter != max_iter ||
-        std::memcmp(&w.graph_args, &w.args, sizeof(BatchArgs<T>)) != 0) {
-        w.graph.capture(s, [&] {
-            for (uint32_t k = 0; k < K; ++k)
-                NLHIP_CHECK(batch_iteration<T>(w.args, precond, max_iter, B.grid, s));
-            NLHIP_CHECK(batch_compact<T>(w.args, B.grid, s));
-        });
-        w.graph_iters = K;
-        w.graph_precond = precond;
-        w.graph_max_iter = max_iter;
-        w.graph_args = w.args;
-    }
+    if (w.graph_precond != precond || w.graph_max_iter != max_iter ||
+        std::memcmp(&w.graph_args, &w.args, sizeof(BatchArgs<T>)) != 0)
+        w.graph.reset();
+    w.graph.ensure(K, s, [&](uint32_t k) {
+        for (uint32_t i = 0; i < k; ++i)
+            NLHIP_CHECK(batch_iteration<T>(w.args, precond, max_iter, B.grid, s));
+        NLHIP_CHECK(batch_compact<T>(w.args, B.grid, s));
+    });
+    w.graph_precond = precond;
+    w.graph_max_iter = max_iter;
+    w.graph_args = w.args;
     Event ev[2];
     uint32_t* host = w.host_counts.get();
-    const uint64_t max_graphs = uint64_t(std::max(max_iter, 0)) / K + 4;
-    uint64_t launched = 0;
+    const uint64_t limit = uint64_t(std::max(max_iter, 0)) + 3ull * w.graph.cap() + 8;
+    uint64_t launched = 0, covered = 0;
     for (;;) {
-        w.graph.launch(s);
+        w.graph.launch(launched, s);
+        covered += w.graph.iters(launched);
         const int slot = int(launched & 1);
         NLHIP_CHECK(hipMemcpyAsync(&host[slot], w.counts.get() + 1, sizeof(uint32_t),
                                    hipMemcpyDeviceToHost, s));
@@ -133,7 +131,7 @@ void run_batch(Batch& B, BatchWork<T>& w, bool precond, T eps, int max_iter, uin
             ev[prev].sync();
             if (host[prev] == 0) break;
         }
-        if (launched > max_graphs) {
+        if (covered > limit) {
             B.stream.sync();
             if (host[slot] == 0) break;
             throw Error(NLHIP_ERR_HIP, "batched CG did not reach its stopping test");

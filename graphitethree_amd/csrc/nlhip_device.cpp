@@ -109,18 +109,33 @@ void copy_out(void* dst, const void* src, size_t bytes, hipStream_t s) {
     NLHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s));
 }
 
+std::shared_mutex& capture_mutex() {
+    static std::shared_mutex* m = new std::shared_mutex();  // process lifetime
+    return *m;
+}
+
 Stream::Stream() {
     device();
+    auto g = no_capture_guard();
     NLHIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
 }
 
 Stream::~Stream() {
-    if (s_) (void)hipStreamDestroy(s_);
+    if (s_) {
+        auto g = no_capture_guard();
+        (void)hipStreamDestroy(s_);
+    }
 }
 
-Event::Event() { NLHIP_CHECK(hipEventCreate(&e_)); }
+Event::Event() {
+    auto g = no_capture_guard();
+    NLHIP_CHECK(hipEventCreate(&e_));
+}
 Event::~Event() {
-    if (e_) (void)hipEventDestroy(e_);
+    if (e_) {
+        auto g = no_capture_guard();
+        (void)hipEventDestroy(e_);
+    }
 }
 
 float elapsed_ms(const Event& a, const Event& b) {

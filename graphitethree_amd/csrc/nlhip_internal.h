@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstring>
 #include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -73,6 +74,17 @@ bool device_usable();  // no throw
 // NLHIP_ERR_INJECTED before touching the device.
 bool take_injected_failure();
 
+// ------------------------------------------------------------------ capture
+
+// Stream capture and the runtime calls that may not overlap another thread's
+// capture (allocation, stream/event creation, synchronous copies).  Graph
+// capture holds this mutex exclusively from BeginCapture to EndCapture (it
+// only records launches); those calls hold it shared.
+std::shared_mutex& capture_mutex();
+inline std::shared_lock<std::shared_mutex> no_capture_guard() {
+    return std::shared_lock<std::shared_mutex>(capture_mutex());
+}
+
 // ------------------------------------------------------------------ buffers
 
 template <typename T>
@@ -92,7 +104,11 @@ class DevBuf {
         release();
         if (n == 0) n = 1;
         void* p = nullptr;
-        hipError_t e = hipMalloc(&p, n * sizeof(T));
+        hipError_t e;
+        {
+            auto g = no_capture_guard();
+            e = hipMalloc(&p, n * sizeof(T));
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             throw Error(NLHIP_ERR_OOM, "hipMalloc of " + std::to_string(n * sizeof(T)) +
@@ -103,7 +119,10 @@ class DevBuf {
     }
     void ensure(size_t n) { if (n_ < n || !p_) alloc(n); }
     void release() {
-        if (p_) (void)hipFree(p_);
+        if (p_) {
+            auto g = no_capture_guard();
+            (void)hipFree(p_);
+        }
         p_ = nullptr;
         n_ = 0;
     }
@@ -128,7 +147,11 @@ class PinnedBuf {
         release();
         if (n == 0) n = 1;
         void* p = nullptr;
-        hipError_t e = hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault);
+        hipError_t e;
+        {
+            auto g = no_capture_guard();
+            e = hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault);
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             throw Error(NLHIP_ERR_OOM, std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
@@ -137,7 +160,10 @@ class PinnedBuf {
         n_ = n;
     }
     void release() {
-        if (p_) (void)hipHostFree(p_);
+        if (p_) {
+            auto g = no_capture_guard();
+            (void)hipHostFree(p_);
+        }
         p_ = nullptr;
         n_ = 0;
     }
@@ -199,19 +225,20 @@ class Graph {
     template <typename F>
     void capture(hipStream_t s, F&& body) {
         reset();
-        NLHIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        hipError_t err = hipSuccess;
-        try {
-            body();
-        } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(s, &g);
-            if (g) (void)hipGraphDestroy(g);
-            throw;
+        {
+            std::unique_lock<std::shared_mutex> lock(capture_mutex());
+            NLHIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            NLHIP_CHECK(hipStreamEndCapture(s, &graph_));
         }
-        NLHIP_CHECK(hipStreamEndCapture(s, &graph_));
         NLHIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
-        (void)err;
     }
     void launch(hipStream_t s) const { NLHIP_CHECK(hipGraphLaunch(exec_, s)); }
     bool ready() const { return exec_ != nullptr; }
@@ -227,6 +254,53 @@ class Graph {
     hipGraphExec_t exec_ = nullptr;
 };
 
+// Graphs of 4, 8, 16, ... iterations up to a cap K (launched as 4, 4, 8,
+// ..., K, K, ...).  The host learns that a solve stopped one graph late, so
+// a solve that stops after a few iterations replays few launches past its
+// stopping test, and a long one still runs K iterations per graph.
+class GraphLadder {
+  public:
+    void reset() {
+        for (auto& g : g_) g.reset();
+        n_ = 0;
+        cap_ = 0;
+    }
+    // Captures the graphs for a cap of K on stream s (record(k) enqueues k
+    // iterations); keeps them while K is unchanged (reset() when anything
+    // baked into the launches changes).
+    template <typename F>
+    void ensure(uint32_t K, hipStream_t s, F&& record) {
+        if (K == 0) K = 1;
+        if (n_ && K == cap_) return;
+        reset();
+        uint32_t k = K < kFirst ? K : kFirst;
+        for (;;) {
+            iters_[n_] = k;
+            g_[n_].capture(s, [&] { record(k); });
+            ++n_;
+            if (k >= K || n_ == kLevels) break;
+            k = 2 * k < K ? 2 * k : K;
+        }
+        cap_ = K;
+    }
+    bool ready() const { return n_ > 0; }
+    uint32_t cap() const { return cap_; }
+    // Iterations in the i-th launch (0-based) of a solve, and its launch.
+    uint32_t iters(uint64_t i) const { return iters_[level(i)]; }
+    void launch(uint64_t i, hipStream_t s) const { g_[level(i)].launch(s); }
+
+  private:
+    static constexpr int kLevels = 12;
+    static constexpr uint32_t kFirst = 4;
+    int level(uint64_t i) const {
+        return i == 0 ? 0 : int(i - 1 < uint64_t(n_ - 1) ? i - 1 : uint64_t(n_ - 1));
+    }
+    Graph g_[kLevels];
+    uint32_t iters_[kLevels] = {};
+    int n_ = 0;
+    uint32_t cap_ = 0;
+};
+
 // ------------------------------------------------------------------ system
 
 template <typename T>
@@ -235,8 +309,7 @@ struct CGWorkspace {
     DevBuf<T> x, b, r, h, d, q, part;
     DevBuf<CGStatus<T>> st;
     PinnedBuf<CGStatus<T>> st_host;  // two slots
-    Graph graph;
-    int graph_iters = 0;
+    GraphLadder graph;
     int graph_rpw = 0;
     bool graph_precond = false;
     CGArgs<T> graph_args{};  // arguments baked into the captured graph
@@ -252,10 +325,9 @@ struct MultiWork {
     DevBuf<int32_t> iters, done, skip;
     DevBuf<uint32_t> n_active;
     PinnedBuf<uint32_t> host;               // two polling slots
-    Graph graph;
+    GraphLadder graph;
     MultiArgs<T> args{};
     MultiArgs<T> graph_args{};
-    uint32_t graph_iters = 0;
     int graph_max_iter = -1;
     bool graph_precond = false;
 };

@@ -10,7 +10,7 @@ namespace nlhip {
 
 // Rows-per-wavefront dispatch over the compiled SpMV engines (codes of
 // dev_common.h's EngineOf): 2, 4, 8, 16 -> 32, 16, 8, 4 lanes per row;
-// 65..71 -> stream engines (one row per lane).  Launchers pass the code
+// 65..72 -> stream engines (one row per lane).  Launchers pass the code
 // through engine_rpw first, which resolves 64.
 #define NLHIP_RPW_SWITCH(rpw, CALL)                       \
     switch (rpw) {                                        \
@@ -25,19 +25,22 @@ namespace nlhip {
         case 69: { constexpr int RPW = 69; CALL; } break; \
         case 70: { constexpr int RPW = 70; CALL; } break; \
         case 71: { constexpr int RPW = 71; CALL; } break; \
+        case 72: { constexpr int RPW = 72; CALL; } break; \
         default: return hipErrorInvalidValue;             \
     }
 
 // Launch shapes a caller may set: 2, 4, 8, 16, and the stream engines 64
-// (the default stream engine) and 65..71 (variants kept for sweeps; they
+// (the default stream engine) and 65..72 (variants kept for sweeps; they
 // compute the same sums as 64).
 inline bool valid_rpw(int rpw) {
-    return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 71);
+    return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 72);
 }
 
-// Default stream engines: with and without 16-bit column offsets.
-constexpr int kStreamAuto = 70;
-constexpr int kStreamAutoPlain = 69;
+// Default stream engines: with and without 16-bit column offsets (each wave
+// streams one contiguous run of tiles; the XCD-interleaved runs of 69/70
+// gather less but are no faster on large systems and slower on small ones).
+constexpr int kStreamAuto = 68;
+constexpr int kStreamAutoPlain = 71;
 
 // Engine a launcher runs for a requested code: 64 becomes the default stream
 // engine for the matrix's column format; a variant that reads 16-bit offsets
@@ -46,6 +49,7 @@ inline int engine_rpw(int rpw, const Cols& cols) {
     if (rpw == 64) return cols.ci16 ? kStreamAuto : kStreamAutoPlain;
     if (!cols.ci16 && rpw == 68) return 71;
     if (!cols.ci16 && rpw == 70) return 69;
+    if (!cols.ci16 && rpw == 72) return 67;
     return rpw;
 }
 

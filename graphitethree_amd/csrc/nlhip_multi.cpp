@@ -21,9 +21,19 @@ uint32_t pow2_ceil(uint32_t k) {
     return g;
 }
 
+// Workgroups of the multi right-hand-side launches: enough for one pass of
+// G-lane row groups over the matrix, at most 16 per CU (the kernels use few
+// registers, so the extra waves hide the gather latency).
+uint32_t multi_grid(const System& S, uint32_t k) {
+    const uint32_t per = 256u / pow2_ceil(k);
+    const uint64_t passes = (uint64_t(S.n) + per - 1) / per;
+    const uint64_t cap = uint64_t(std::max(1, device().cu_count)) * 16;
+    return uint32_t(std::max<uint64_t>(1, std::min(passes, cap)));
+}
+
 template <typename T>
 void multi_alloc(System& S, MultiWork<T>& w, uint32_t k) {
-    const uint32_t P = S.grid;
+    const uint32_t P = multi_grid(S, k);
     if (w.n == S.n && w.k == k && w.P == P && w.x.get()) return;
     w.graph.reset();
     w.n = S.n;
@@ -81,24 +91,23 @@ template <typename T>
 void run_multi(System& S, MultiWork<T>& w, bool precond, T eps, int max_iter, uint32_t K) {
     hipStream_t s = S.stream.get();
     NLHIP_CHECK(multi_start<T>(w.args, precond, eps, max_iter, s));
-    if (!w.graph.ready() || w.graph_iters != K || w.graph_precond != precond ||
-        w.graph_max_iter != max_iter ||
-        std::memcmp(&w.graph_args, &w.args, sizeof(MultiArgs<T>)) != 0) {
-        w.graph.capture(s, [&] {
-            for (uint32_t i = 0; i < K; ++i)
-                NLHIP_CHECK(multi_iteration<T>(w.args, precond, max_iter, s));
-        });
-        w.graph_iters = K;
-        w.graph_precond = precond;
-        w.graph_max_iter = max_iter;
-        w.graph_args = w.args;
-    }
+    if (w.graph_precond != precond || w.graph_max_iter != max_iter ||
+        std::memcmp(&w.graph_args, &w.args, sizeof(MultiArgs<T>)) != 0)
+        w.graph.reset();
+    w.graph.ensure(K, s, [&](uint32_t k) {
+        for (uint32_t i = 0; i < k; ++i)
+            NLHIP_CHECK(multi_iteration<T>(w.args, precond, max_iter, s));
+    });
+    w.graph_precond = precond;
+    w.graph_max_iter = max_iter;
+    w.graph_args = w.args;
     Event ev[2];
     uint32_t* host = w.host.get();
-    const uint64_t max_graphs = uint64_t(std::max(max_iter, 0)) / K + 4;
-    uint64_t launched = 0;
+    const uint64_t limit = uint64_t(std::max(max_iter, 0)) + 3ull * w.graph.cap() + 8;
+    uint64_t launched = 0, covered = 0;
     for (;;) {
-        w.graph.launch(s);
+        w.graph.launch(launched, s);
+        covered += w.graph.iters(launched);
         const int slot = int(launched & 1);
         NLHIP_CHECK(hipMemcpyAsync(&host[slot], w.n_active.get(), sizeof(uint32_t),
                                    hipMemcpyDeviceToHost, s));
@@ -109,7 +118,7 @@ void run_multi(System& S, MultiWork<T>& w, bool precond, T eps, int max_iter, ui
             ev[prev].sync();
             if (host[prev] == 0) break;
         }
-        if (launched > max_graphs) {
+        if (covered > limit) {
             S.stream.sync();
             if (host[slot] == 0) break;
             throw Error(NLHIP_ERR_HIP, "multi right-hand-side CG did not reach its stopping test");

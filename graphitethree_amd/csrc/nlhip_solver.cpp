@@ -77,21 +77,26 @@ System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind
     const uint32_t* rp = rowptr;
     const uint32_t* ci = colind;
     const double* va = val;
+    auto fetch = [](void* dst, const void* src, size_t bytes) {
+        Stream st;
+        NLHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, st.get()));
+        st.sync();
+    };
     if (is_device_pointer(rowptr)) {
         h_rp.resize(size_t(n) + 1);
-        NLHIP_CHECK(hipMemcpy(h_rp.data(), rowptr, (size_t(n) + 1) * 4, hipMemcpyDefault));
+        fetch(h_rp.data(), rowptr, (size_t(n) + 1) * 4);
         rp = h_rp.data();
     }
     const uint64_t nnz = rp[n];
     if (nnz > 0 && (!colind || !val)) throw Error(NLHIP_ERR_INVALID, "null colind/val");
     if (nnz && is_device_pointer(colind)) {
         h_ci.resize(nnz);
-        NLHIP_CHECK(hipMemcpy(h_ci.data(), colind, nnz * 4, hipMemcpyDefault));
+        fetch(h_ci.data(), colind, nnz * 4);
         ci = h_ci.data();
     }
     if (nnz && is_device_pointer(val)) {
         h_val.resize(nnz);
-        NLHIP_CHECK(hipMemcpy(h_val.data(), val, nnz * 8, hipMemcpyDefault));
+        fetch(h_val.data(), val, nnz * 8);
         va = h_val.data();
     }
     validate_crs(n, rp, ci, va);
@@ -187,24 +192,24 @@ static CGStatus<T> run_cg(System& S, CGWorkspace<T>& ws, bool precond, T eps, in
                           uint32_t K) {
     hipStream_t s = S.stream.get();
     NLHIP_CHECK(cg_start<T>(ws.args, S.rpw, precond, eps, max_iter, s));
-    if (!ws.graph.ready() || ws.graph_iters != int(K) || ws.graph_rpw != S.rpw ||
-        ws.graph_precond != precond ||
-        std::memcmp(&ws.graph_args, &ws.args, sizeof(CGArgs<T>)) != 0) {
-        ws.graph.capture(s, [&] {
-            for (uint32_t k = 0; k < K; ++k) NLHIP_CHECK(cg_step<T>(ws.args, S.rpw, precond, s));
-        });
-        ws.graph_iters = int(K);
-        ws.graph_rpw = S.rpw;
-        ws.graph_precond = precond;
-        ws.graph_args = ws.args;
-    }
+    if (ws.graph_rpw != S.rpw || ws.graph_precond != precond ||
+        std::memcmp(&ws.graph_args, &ws.args, sizeof(CGArgs<T>)) != 0)
+        ws.graph.reset();
+    ws.graph.ensure(K, s, [&](uint32_t k) {
+        for (uint32_t i = 0; i < k; ++i) NLHIP_CHECK(cg_step<T>(ws.args, S.rpw, precond, s));
+    });
+    ws.graph_rpw = S.rpw;
+    ws.graph_precond = precond;
+    ws.graph_args = ws.args;
     Event ev[2];
     CGStatus<T>* host = ws.st_host.get();
     host[0].done = host[1].done = 0;
-    const uint64_t max_graphs = uint64_t(max_iter) / K + 4;
-    uint64_t launched = 0;
+    // the device stops by itself at max_iter; past that the status must say so
+    const uint64_t limit = uint64_t(std::max(max_iter, 0)) + 3ull * ws.graph.cap() + 8;
+    uint64_t launched = 0, covered = 0;
     for (;;) {
-        ws.graph.launch(s);
+        ws.graph.launch(launched, s);
+        covered += ws.graph.iters(launched);
         const int slot = int(launched & 1);
         NLHIP_CHECK(hipMemcpyAsync(&host[slot], ws.st.get(), sizeof(CGStatus<T>),
                                    hipMemcpyDeviceToHost, s));
@@ -215,15 +220,15 @@ static CGStatus<T> run_cg(System& S, CGWorkspace<T>& ws, bool precond, T eps, in
             ev[prev].sync();
             if (host[prev].done) break;
         }
-        if (launched > max_graphs) {
+        if (covered > limit) {
             S.stream.sync();
             if (host[slot].done) break;
             throw Error(NLHIP_ERR_HIP, "CG did not reach its stopping test");
         }
     }
-    S.stream.sync();
     CGStatus<T> fin;
-    NLHIP_CHECK(hipMemcpy(&fin, ws.st.get(), sizeof(fin), hipMemcpyDeviceToHost));
+    NLHIP_CHECK(hipMemcpyAsync(&fin, ws.st.get(), sizeof(fin), hipMemcpyDeviceToHost, s));
+    S.stream.sync();
     return fin;
 }
 
@@ -479,7 +484,7 @@ NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,
     return guarded("nlhipSystemSetLaunch", [&] {
         System* S = sys(s);
         if (!valid_rpw(rows_per_wave))
-            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64..71");
+            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64..72");
         if (workgroups == 0 || workgroups > 65536)
             throw Error(NLHIP_ERR_INVALID, "workgroups must be in [1, 65536]");
         std::lock_guard<std::mutex> lock(S->mu);

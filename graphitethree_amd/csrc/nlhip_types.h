@@ -30,12 +30,12 @@ struct CGStatus {
     T bb;            // b.b                                (init)
     T err;           // r.r when the loop stopped          (spmv step)
     T alpha, beta;   // last step sizes, for diagnostics
-    int32_t iter_a;  // completed iterations, read by the spmv step
+    int32_t iter_a;  // completed iterations, read by the direction step
     int32_t iter_b;  // completed iterations, read by the update step
-    int32_t done;    // 1 once the stopping test held      (spmv step)
+    int32_t done;    // 1 once the loop has stopped        (spmv step)
     int32_t iters;   // iterations performed when done
     int32_t max_iter;
-    int32_t pad;
+    int32_t stop;    // 1 once the stopping test held      (direction step)
 };
 
 }  // namespace nlhip

@@ -23,7 +23,7 @@ def test_spmv_exact_all_launch_shapes(hip):
     for name in z["names"]:
         rp, ci, va = z[f"{name}_rowptr"], z[f"{name}_colind"], z[f"{name}_val"]
         S = capi.System(rp, ci, va)
-        for rpw in (2, 4, 8, 16, 64, 65, 66, 67, 68, 69, 70, 71):
+        for rpw in (2, 4, 8, 16, 64, 65, 66, 67, 68, 69, 70, 71, 72):
             for wg in (1, 3, 64, 2048):
                 S.set_launch(rpw, wg)
                 y = S.spmv(z[f"{name}_x"])
@@ -50,7 +50,7 @@ def test_stream_engine_is_bitwise_the_oracle_spmv(hip):
         x = np.random.default_rng(4).standard_normal(Sw.n)
         S = capi.System.from_crs(Sw)
         ref = O.spmv(A, x)
-        for rpw in (64, 65, 66, 67, 68, 69, 70, 71):
+        for rpw in (64, 65, 66, 67, 68, 69, 70, 71, 72):
             for wg in (1, 5, 9, 300):
                 S.set_launch(rpw, wg)
                 assert np.array_equal(S.spmv(x), ref), (rpw, wg)
@@ -88,7 +88,7 @@ def test_column_offsets_match_32bit_indices(hip):
         P = capi.System(rp, ci, va, offsets=False)
         extra = S.info()[2] - P.info()[2]
         assert (extra > 0) == expect_offsets, (name, extra)
-        for rpw in (64, 68, 70):
+        for rpw in (64, 68, 70, 72):
             for wg in (1, 7, 1024):
                 S.set_launch(rpw, wg)
                 P.set_launch(rpw, wg)
