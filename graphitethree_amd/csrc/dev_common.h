@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
+#include <utility>
 
 #include "nlhip_types.h"
 
@@ -93,10 +95,49 @@ __device__ __forceinline__ void block_row_range(uint32_t m, uint32_t chunk, uint
     r1 = e < m ? uint32_t(e) : m;
 }
 
+// ---- SpMV epilogues ----------------------------------------------------
+//
+// The engines hand every row sum to an epilogue, called by exactly one lane
+// per row: epi.done(row, sum, pv), where pv = epi.pre(row) was fetched when
+// the row's tile started, so that the latency of the per-row operands the
+// epilogue reads (b, D^-1, d, ...) overlaps the row's nonzeros.
+
+struct NoPre {
+    __device__ __forceinline__ int operator()(uint32_t) const { return 0; }
+};
+
+template <typename Pre, typename Fin>
+struct Epilogue {
+    Pre pre;
+    Fin fin;
+    using PV = decltype(std::declval<Pre&>()(0u));
+    template <typename T>
+    __device__ __forceinline__ void done(uint32_t row, T sum, const PV& pv) {
+        if constexpr (std::is_same<Pre, NoPre>::value) fin(row, sum);
+        else fin(row, sum, pv);
+    }
+};
+
+// fin(row, sum)
+template <typename Fin>
+__device__ __forceinline__ Epilogue<NoPre, Fin> plain_epi(Fin fin) {
+    return Epilogue<NoPre, Fin>{NoPre{}, fin};
+}
+// pre(row) -> pv; fin(row, sum, pv)
+template <typename Pre, typename Fin>
+__device__ __forceinline__ Epilogue<Pre, Fin> prefetch_epi(Pre pre, Fin fin) {
+    return Epilogue<Pre, Fin>{pre, fin};
+}
+
+template <typename T>
+struct Pair {
+    T a, b;
+};
+
 // ---- SpMV engines -----------------------------------------------------
 //
 // Both engines walk the rows owned by one workgroup and hand every row sum
-// to an epilogue epi(row, sum), called by exactly one lane per row.
+// to an epilogue (above).
 //
 // Vector engine (RPW < 64 rows per wavefront): LPR = 64/RPW lanes per row,
 // rows [L*sp, L*sp + sp) of the matrix for logical block L, partial products
@@ -126,7 +167,7 @@ struct VectorEngine {
         const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
         for (uint32_t row = r0 + g; row < r1; row += kBlock / LPR) {
             const T s = row_product<LPR>(row, rp, ci, val, x, lane);
-            if (lane == 0) epi(row, s);
+            if (lane == 0) epi.done(row, s, epi.pre(row));
         }
     }
 };
@@ -208,6 +249,8 @@ struct StreamEngine {
         uint32_t nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
         uint32_t tile_end = rp[r0 + nrows];
         uint32_t rb = (uint32_t(lane) < nrows) ? rp[r0 + uint32_t(lane)] : tile_end;
+        typename Epi::PV pv{};
+        if (uint32_t(lane) < nrows) pv = epi.pre(r0 + uint32_t(lane));
         uint32_t re = __shfl_down(rb, 1, kWave);
         if (uint32_t(lane) + 1 >= nrows) re = tile_end;
         uint32_t base = __builtin_amdgcn_readfirstlane(__shfl(rb, 0, kWave));
@@ -248,7 +291,7 @@ struct StreamEngine {
                 }
                 for (; j < hi; ++j) acc = add_rn(acc, st[j - base]);
                 if (tile_end > cend) break;
-                if (uint32_t(lane) < nrows) epi(r0 + uint32_t(lane), acc);
+                if (uint32_t(lane) < nrows) epi.done(r0 + uint32_t(lane), acc, pv);
                 acc = T(0);
                 r0 += 64u;
                 if (r0 >= R1) return;
@@ -257,6 +300,7 @@ struct StreamEngine {
                 rb = (uint32_t(lane) < nrows) ? rp[r0 + uint32_t(lane)] : tile_end;
                 re = __shfl_down(rb, 1, kWave);
                 if (uint32_t(lane) + 1 >= nrows) re = tile_end;
+                if (uint32_t(lane) < nrows) pv = epi.pre(r0 + uint32_t(lane));
             }
             __builtin_amdgcn_wave_barrier();
             base = craw;
@@ -268,11 +312,12 @@ struct StreamEngine {
         }
         // tiles left without nonzeros (all their rows are empty)
         for (;;) {
-            if (uint32_t(lane) < nrows) epi(r0 + uint32_t(lane), acc);
+            if (uint32_t(lane) < nrows) epi.done(r0 + uint32_t(lane), acc, pv);
             acc = T(0);
             r0 += 64u;
             if (r0 >= R1) return;
             nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
+            if (uint32_t(lane) < nrows) pv = epi.pre(r0 + uint32_t(lane));
         }
     }
 

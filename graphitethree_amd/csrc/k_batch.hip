@@ -37,9 +37,13 @@ __device__ __forceinline__ void tile_spmv(const BatchArgs<T>& a, uint32_t r0, ui
     const uint32_t R0 = r0 + 64u * uint32_t(w);
     if (R0 >= r1) return;
     const uint32_t R1 = (R0 + 64u < r1) ? R0 + 64u : r1;
-    const Cols cols{a.ci, nullptr, nullptr};
-    StreamEngine<kStreamE, true>::run_rows(R0, R1, a.rp, cols, a.val, x,
-                                           stage + w * (64 * kStreamE), lane, epi);
+    const Cols cols{a.ci, a.ci16, a.segb};
+    if (a.ci16)
+        StreamEngine<kStreamE, true, true>::run_rows(R0, R1, a.rp, cols, a.val, x,
+                                                     stage + w * (64 * kStreamE), lane, epi);
+    else
+        StreamEngine<kStreamE, true>::run_rows(R0, R1, a.rp, cols, a.val, x,
+                                               stage + w * (64 * kStreamE), lane, epi);
 }
 
 template <typename T, bool PREC>
@@ -52,16 +56,18 @@ __global__ __launch_bounds__(kBlock) void k_b_init(BatchArgs<T> a) {
         uint32_t r0, r1;
         tile_rows(a, t, r0, r1);
         T prh = T(0), prr = T(0), pbb = T(0);
-        auto epi = [&](uint32_t row, T s) {
-            const T bi = a.b[row];
-            const T ri = add_rn(s, mul_rn(T(-1), bi));
-            const T hi = PREC ? mul_rn(a.dinv[row], ri) : ri;
-            a.r[row] = ri;
-            a.h[row] = hi;
-            prh += ri * hi;
-            prr += ri * ri;
-            pbb += bi * bi;
-        };
+        auto epi = prefetch_epi(
+            [&](uint32_t row) { return Pair<T>{a.b[row], PREC ? a.dinv[row] : T(1)}; },
+            [&](uint32_t row, T s, const Pair<T>& pv) {
+                const T bi = pv.a;
+                const T ri = add_rn(s, mul_rn(T(-1), bi));
+                const T hi = PREC ? mul_rn(pv.b, ri) : ri;
+                a.r[row] = ri;
+                a.h[row] = hi;
+                prh += ri * hi;
+                prr += ri * ri;
+                pbb += bi * bi;
+            });
         tile_spmv(a, r0, r1, a.x, stage, epi);
         prh = block_sum_all(prh, lds);
         prr = block_sum_all(prr, lds);
@@ -138,10 +144,11 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(BatchArgs<T> a) {
         uint32_t r0, r1;
         tile_rows(a, t, r0, r1);
         T pdq = T(0);
-        auto epi = [&](uint32_t row, T v) {
-            a.q[row] = v;
-            pdq += a.d[row] * v;
-        };
+        auto epi = prefetch_epi([&](uint32_t row) { return a.d[row]; },
+                                [&](uint32_t row, T v, T dr) {
+                                    a.q[row] = v;
+                                    pdq += dr * v;
+                                });
         tile_spmv(a, r0, r1, a.d, stage, epi);
         pdq = block_sum_all(pdq, lds);
         if (threadIdx.x == 0) a.part_dq[t] = pdq;
@@ -260,13 +267,13 @@ __global__ __launch_bounds__(kBlock) void k_b_residual64(BatchArgs<double> a) {
         uint32_t r0, r1;
         tile_rows(a, t, r0, r1);
         double srr = 0.0, sbb = 0.0;
-        auto epi = [&](uint32_t row, double s) {
-            const double bi = a.b[row];
-            const double ri = bi - s;
-            a.r[row] = ri;
-            srr += ri * ri;
-            sbb += bi * bi;
-        };
+        auto epi = prefetch_epi([&](uint32_t row) { return a.b[row]; },
+                                [&](uint32_t row, double s, double bi) {
+                                    const double ri = bi - s;
+                                    a.r[row] = ri;
+                                    srr += ri * ri;
+                                    sbb += bi * bi;
+                                });
         tile_spmv(a, r0, r1, a.x, stage, epi);
         srr = block_sum_all(srr, lds);
         sbb = block_sum_all(sbb, lds);

@@ -15,7 +15,8 @@
 //              with the previous iteration's alpha and d (the update of x
 //              is deferred to here, where d is read anyway); stopping test;
 //              beta = rho / rho_prev; d = beta d + h (d = h at it 0)
-//   spmv       q = A d and partials of d.q
+//   spmv       q = A d and partials of d.q (up to twice the workgroups of
+//              the other launches: more waves in flight for the gathers)
 //   update     reduce the d.q partials -> alpha; r -= alpha q; h = P r;
 //              partials of r.h and r.r
 // Per iteration that moves 12 nnz + 100 n bytes in FP64 with 32-bit column
@@ -46,16 +47,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(CGArgs<T> a) {
     __shared__ T lds[kWavesPerBlock];
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     T prh = T(0), prr = T(0), pbb = T(0);
-    auto epi = [&](uint32_t row, T s) {
-        const T bi = a.b[row];
-        const T ri = add_rn(s, mul_rn(T(-1), bi));
-        const T hi = PREC ? mul_rn(a.dinv[row], ri) : ri;
-        a.r[row] = ri;
-        a.h[row] = hi;
-        prh += ri * hi;
-        prr += ri * ri;
-        pbb += bi * bi;
-    };
+    auto epi = prefetch_epi(
+        [&](uint32_t row) { return Pair<T>{a.b[row], PREC ? a.dinv[row] : T(1)}; },
+        [&](uint32_t row, T s, const Pair<T>& pv) {
+            const T bi = pv.a;
+            const T ri = add_rn(s, mul_rn(T(-1), bi));
+            const T hi = PREC ? mul_rn(pv.b, ri) : ri;
+            a.r[row] = ri;
+            a.h[row] = hi;
+            prh += ri * hi;
+            prr += ri * ri;
+            pbb += bi * bi;
+        });
     EngineOf<RPW>::type::run(a.m, a.sp, L, a.rp, a.cols, a.val, a.x, stage, epi);
     prh = block_sum_all(prh, lds);
     prr = block_sum_all(prr, lds);
@@ -152,11 +155,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(CGArgs<T> a) {
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const T* __restrict__ d = a.d;
     T pdq = T(0);
-    auto epi = [&](uint32_t row, T s) {
-        a.q[row] = s;
-        pdq += d[row] * s;
-    };
-    EngineOf<RPW>::type::run(a.m, a.sp, L, a.rp, a.cols, a.val, d, stage, epi);
+    auto epi = prefetch_epi([&](uint32_t row) { return d[row]; },
+                            [&](uint32_t row, T s, T dr) {
+                                a.q[row] = s;
+                                pdq += dr * s;
+                            });
+    EngineOf<RPW>::type::run(a.m, a.sps, L, a.rp, a.cols, a.val, d, stage, epi);
     pdq = block_sum_all(pdq, lds);
     if (threadIdx.x == 0) a.part_dq[L] = pdq;
 }
@@ -168,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CGArgs<T> a) {
     CGStatus<T>* st = a.st;
     if (block_done(st, &s_flag)) return;
     const int it = st->iter_b;
-    const T dq = reduce_partials(a.part_dq, a.P, lds);
+    const T dq = reduce_partials(a.part_dq, a.Ps, lds);
     const T rho = st->rho;
     const T alpha = rho / dq;
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
@@ -236,7 +240,7 @@ hipError_t cg_step_part(const CGArgs<T>& a, int rpw, bool precond, int part, hip
         k_cg_dir<T><<<a.P, kBlock, 0, s>>>(a);
     } else if (part == 1) {
         rpw = engine_rpw(rpw, a.cols);
-        NLHIP_RPW_SWITCH(rpw, (k_cg_spmv<RPW, T><<<a.P, kBlock, 0, s>>>(a)));
+        NLHIP_RPW_SWITCH(rpw, (k_cg_spmv<RPW, T><<<a.Ps, kBlock, 0, s>>>(a)));
     } else if (precond) {
         k_cg_update<T, true><<<a.P, kBlock, 0, s>>>(a);
     } else {

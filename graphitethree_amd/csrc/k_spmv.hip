@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(uint32_t m, uint32_t sp,
                                                  const T* __restrict__ x, T* __restrict__ y) {
     __shared__ T stage[StageSize<RPW>::value];
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
-    auto epi = [&](uint32_t row, T s) { y[row] = s; };
+    auto epi = plain_epi([&](uint32_t row, T s) { y[row] = s; });
     EngineOf<RPW>::type::run(m, sp, L, rp, cols, val, x, stage, epi);
 }
 
@@ -39,13 +39,13 @@ __global__ __launch_bounds__(kBlock) void k_residual(
     __shared__ double lds[kWavesPerBlock];
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     double srr = 0.0, sbb = 0.0;
-    auto epi = [&](uint32_t row, double s) {
-        const double bi = b[row];
-        const double ri = bi - s;
-        if (r) r[row] = ri;
-        srr += ri * ri;
-        sbb += bi * bi;
-    };
+    auto epi = prefetch_epi([&](uint32_t row) { return b[row]; },
+                            [&](uint32_t row, double s, double bi) {
+                                const double ri = bi - s;
+                                if (r) r[row] = ri;
+                                srr += ri * ri;
+                                sbb += bi * bi;
+                            });
     EngineOf<RPW>::type::run(m, sp, L, rp, cols, val, x, stage, epi);
     srr = block_sum_all(srr, lds);
     sbb = block_sum_all(sbb, lds);

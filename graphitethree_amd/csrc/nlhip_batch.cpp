@@ -32,6 +32,8 @@ struct Batch {
     uint32_t nsys = 0, n = 0, ntiles = 0;
     uint64_t nnz = 0;
     DevBuf<uint32_t> row_off, tile_off, tile_sys, tile_row0, rowptr, colind;
+    DevBuf<uint16_t> ci16;  // 16-bit column offsets (empty when they do not fit)
+    DevBuf<uint32_t> segb;
     DevBuf<double> val, dinv, scale;  // scale: 2 x nsys (scale, unscale)
     DevBuf<float> val32, dinv32;
     std::vector<uint32_t> h_row_off;
@@ -72,6 +74,8 @@ void work_alloc(Batch& B, BatchWork<T>& w) {
     a.tile_row0 = B.tile_row0.get();
     a.rp = B.rowptr.get();
     a.ci = B.colind.get();
+    a.ci16 = B.ci16.get();
+    a.segb = B.segb.get();
     a.x = w.x.get();
     a.b = w.b.get();
     a.r = w.r.get();
@@ -356,6 +360,8 @@ Batch* batch_create(uint32_t nsys, const uint32_t* row_off, const uint32_t* rowp
     copy_in(B->val.get(), val, nnz * 8, st);
     NLHIP_CHECK(jacobi_dinv<double>(n, B->rowptr.get(), B->colind.get(), B->val.get(),
                                     B->dinv.get(), st));
+    if (!(flags & NLHIP_SYSTEM_NO_OFFSETS))
+        build_col_offsets(nnz, B->colind.get(), st, B->ci16, B->segb);
     B->grid = uint32_t(std::max(4, device().cu_count * 4));
     if (flags & NLHIP_SYSTEM_FP32) ensure_fp32(*B);
     work_alloc(*B, B->w64);

@@ -362,6 +362,11 @@ struct System {
     Cols cols() const { return Cols{colind.get(), ci16.get(), segb.get()}; }
 };
 
+// Builds the 16-bit column offsets of a device colind (see Cols); false when
+// a segment spans more than 65535 columns.
+bool build_col_offsets(uint64_t nnz, const uint32_t* colind, hipStream_t s,
+                       DevBuf<uint16_t>& ci16, DevBuf<uint32_t>& segb);
+
 // Validates and uploads; throws Error.
 System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
                       const double* val, uint32_t flags);

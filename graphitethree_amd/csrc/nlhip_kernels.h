@@ -100,8 +100,10 @@ template <typename T>
 struct CGArgs {
     uint32_t m;      // rows
     uint32_t chunk;  // rows per workgroup of the element-wise launches
-    uint32_t sp;     // SpMV work per workgroup (spmv_work_per_block)
+    uint32_t sp;     // SpMV work per workgroup of the init launch (P workgroups)
     uint32_t P;      // workgroups per launch = number of partial sums
+    uint32_t Ps;     // workgroups of the SpMV step (d.q partials), Ps <= 2 P
+    uint32_t sps;    // SpMV work per workgroup of the SpMV step
     const uint32_t* rp;
     Cols cols;
     const T* val;
@@ -143,6 +145,8 @@ struct BatchArgs {
     const uint32_t* tile_row0;  // ntiles: first (global) row of the tile
     const uint32_t* rp;         // global rowptr (n + 1)
     const uint32_t* ci;         // global column indices
+    const uint16_t* ci16;       // their 16-bit segment offsets, or null
+    const uint32_t* segb;       // segment bases of ci16
     const T* val;
     const T* dinv;              // null: no preconditioner
     T* x;

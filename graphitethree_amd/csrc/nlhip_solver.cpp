@@ -46,25 +46,27 @@ static void validate_crs(uint32_t n, const uint32_t* rp, const uint32_t* ci, con
 // 16-bit column offsets for the stream engine: kept when every 1024-entry
 // segment of the nonzero stream spans at most 65535 columns (banded matrices:
 // grids in scan order, LSCM and Laplacian systems of meshes with local vertex
-// order); the 32-bit indices stay for the other engines.
-static void build_col_offsets(System& S) {
-    if (S.nnz == 0) return;
-    const uint64_t nseg = (S.nnz + kSegEntries - 1) >> kSegShift;
-    hipStream_t s = S.stream.get();
+// order); the 32-bit indices stay for the other engines.  Returns false (and
+// leaves ci16, segb empty) when they do not fit.
+bool build_col_offsets(uint64_t nnz, const uint32_t* colind, hipStream_t s,
+                       DevBuf<uint16_t>& ci16_out, DevBuf<uint32_t>& segb_out) {
+    if (nnz == 0) return false;
+    const uint64_t nseg = (nnz + kSegEntries - 1) >> kSegShift;
     DevBuf<uint32_t> segb(nseg + 1), span(1);
     NLHIP_CHECK(fill<int32_t>(1, 0, reinterpret_cast<int32_t*>(span.get()), s));
-    NLHIP_CHECK(seg_bases(S.nnz, S.colind.get(), segb.get(), span.get(), s));
+    NLHIP_CHECK(seg_bases(nnz, colind, segb.get(), span.get(), s));
     uint32_t h_span = 0;
     NLHIP_CHECK(hipMemcpyAsync(&h_span, span.get(), 4, hipMemcpyDeviceToHost, s));
     NLHIP_CHECK(hipStreamSynchronize(s));
-    if (h_span > 0xffffu) return;
+    if (h_span > 0xffffu) return false;
     NLHIP_CHECK(hipMemcpyAsync(segb.get() + nseg, segb.get() + nseg - 1, 4,
                                hipMemcpyDeviceToDevice, s));
-    DevBuf<uint16_t> ci16(S.nnz);
-    NLHIP_CHECK(seg_encode(S.nnz, S.colind.get(), segb.get(), ci16.get(), s));
+    DevBuf<uint16_t> ci16(nnz);
+    NLHIP_CHECK(seg_encode(nnz, colind, segb.get(), ci16.get(), s));
     NLHIP_CHECK(hipStreamSynchronize(s));
-    S.ci16 = std::move(ci16);
-    S.segb = std::move(segb);
+    ci16_out = std::move(ci16);
+    segb_out = std::move(segb);
+    return true;
 }
 
 System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
@@ -113,7 +115,8 @@ System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind
     copy_in(S->val.get(), va, nnz * 8, s);
     NLHIP_CHECK(jacobi_dinv<double>(n, S->rowptr.get(), S->colind.get(), S->val.get(),
                                     S->dinv.get(), s));
-    if (!(flags & NLHIP_SYSTEM_NO_OFFSETS)) build_col_offsets(*S);
+    if (!(flags & NLHIP_SYSTEM_NO_OFFSETS))
+        build_col_offsets(nnz, S->colind.get(), s, S->ci16, S->segb);
     default_launch(n, nnz, S->rpw, S->grid);
     if (flags & NLHIP_SYSTEM_FP32) S->ensure_fp32();
     S->stream.sync();
@@ -147,7 +150,7 @@ void CGWorkspace<T>::ensure(uint32_t n_, uint32_t P_) {
     h.alloc(n);
     d.alloc(n);
     q.alloc(n);
-    part.alloc(size_t(4) * P);
+    part.alloc(size_t(5) * P);  // d.q (up to 2 P), r.h, r.r, b.b
     st.alloc(1);
     st_host.ensure(2);
 }
@@ -161,6 +164,12 @@ static void setup_args(System& S, CGWorkspace<T>& ws, const T* val, const T* din
     a.P = S.grid;
     a.chunk = uint32_t((uint64_t(S.n) + S.grid - 1) / S.grid);
     a.sp = spmv_work_per_block(S.n, S.grid, S.rpw);
+    // the streamed SpMV step runs with up to twice the workgroups (at least
+    // four 64-row tiles each)
+    const uint64_t quads = (uint64_t(S.n) + 255) / 256;
+    a.Ps = S.rpw >= 64 ? uint32_t(std::max<uint64_t>(S.grid, std::min<uint64_t>(2ull * S.grid, quads)))
+                       : S.grid;
+    a.sps = spmv_work_per_block(S.n, a.Ps, S.rpw);
     a.rp = S.rowptr.get();
     a.cols = S.cols();
     a.val = val;
@@ -171,10 +180,10 @@ static void setup_args(System& S, CGWorkspace<T>& ws, const T* val, const T* din
     a.h = ws.h.get();
     a.d = ws.d.get();
     a.q = ws.q.get();
-    a.part_dq = ws.part.get();
-    a.part_rh = ws.part.get() + S.grid;
-    a.part_rr = ws.part.get() + 2 * size_t(S.grid);
-    a.part_bb = ws.part.get() + 3 * size_t(S.grid);
+    a.part_dq = ws.part.get();                       // Ps <= 2 P entries
+    a.part_rh = ws.part.get() + 2 * size_t(S.grid);
+    a.part_rr = ws.part.get() + 3 * size_t(S.grid);
+    a.part_bb = ws.part.get() + 4 * size_t(S.grid);
     a.st = ws.st.get();
 }
 
