@@ -38,6 +38,7 @@ NLHIP_ERR_NONFINITE, NLHIP_ERR_UNSUPPORTED, NLHIP_ERR_BUDGET, NLHIP_ERR_INJECTED
 NLHIP_PREC_FP64, NLHIP_PREC_MIXED = 0, 1
 NLHIP_SYSTEM_FP32 = 1
 NLHIP_SYSTEM_NO_OFFSETS = 2
+NLHIP_SYSTEM_NO_BLOCKS = 4
 
 
 class NLMatrixStruct(C.Structure):
@@ -255,20 +256,23 @@ def default_params(**kw) -> SolveParams:
 class System:
     """A CRS system resident on the GPU (nlhipSystemCreate)."""
 
-    def __init__(self, rowptr, colind, val, fp32: bool = False, offsets: bool = True):
+    def __init__(self, rowptr, colind, val, fp32: bool = False, offsets: bool = True,
+                 blocks: bool = True):
         self.rowptr = np.ascontiguousarray(rowptr, dtype=np.uint32)
         self.colind = np.ascontiguousarray(colind, dtype=np.uint32)
         self.val = np.ascontiguousarray(val, dtype=np.float64)
         self.n = len(self.rowptr) - 1
-        flags = (NLHIP_SYSTEM_FP32 if fp32 else 0) | (0 if offsets else NLHIP_SYSTEM_NO_OFFSETS)
+        flags = ((NLHIP_SYSTEM_FP32 if fp32 else 0) | (0 if offsets else NLHIP_SYSTEM_NO_OFFSETS)
+                 | (0 if blocks else NLHIP_SYSTEM_NO_BLOCKS))
         h = C.c_void_p()
         check(lib().nlhipSystemCreate(self.n, ptr(self.rowptr), ptr(self.colind),
                                       ptr(self.val), flags, C.byref(h)), "nlhipSystemCreate")
         self._h = h
 
     @classmethod
-    def from_crs(cls, S, fp32: bool = False, offsets: bool = True) -> "System":
-        return cls(S.rowptr, S.colind, S.val, fp32=fp32, offsets=offsets)
+    def from_crs(cls, S, fp32: bool = False, offsets: bool = True,
+                 blocks: bool = True) -> "System":
+        return cls(S.rowptr, S.colind, S.val, fp32=fp32, offsets=offsets, blocks=blocks)
 
     @property
     def handle(self):

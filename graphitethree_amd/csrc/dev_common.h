@@ -363,6 +363,195 @@ struct StreamEngine {
     }
 };
 
+// ---- 2x2 block stream engine ------------------------------------------
+//
+// For matrices in 2x2 block form: rows 2I and 2I+1 have the same columns,
+// which come in pairs (2J, 2J+1) (LSCM normal equations: the u and v
+// unknowns of a vertex).  Block row I stores its blocks as (a(2I,2J),
+// a(2I,2J+1), a(2I+1,2J), a(2I+1,2J+1)) with the block column J as a 16-bit
+// segment offset (Cols::ci16 / Cols::segb over the block stream): 8.5 bytes
+// per FP64 nonzero.  One block row per lane, 64 block rows per wavefront
+// tile, chunks of 64*EB blocks; the four products of every block are staged
+// in LDS and each lane sums its two rows left to right (a(2I,2J) x_2J, then
+// a(2I,2J+1) x_2J+1, for J ascending) -- the order and roundings of the CRS
+// row loop.  `rp` is the block row pointer (m/2 + 1 entries), `val` the
+// block values (4 per block), `m` the number of (scalar) rows.
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Vec2<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+
+template <int EB>
+struct BlockStreamEngine {
+    static_assert(64 * EB <= int(kSegEntries), "a chunk must span at most two segments");
+
+    // block values of block jj: 4 T, 16-byte aligned for double (32-byte
+    // blocks), 16-byte for float
+    template <typename T>
+    __device__ __forceinline__ static void load_vals(const T* __restrict__ val, uint32_t jj,
+                                                     T (&a)[4]) {
+        using V2 = typename Vec2<T>::type;
+        const V2* p = reinterpret_cast<const V2*>(val + 4ull * jj);
+        const V2 lo = __builtin_nontemporal_load(p), hi = __builtin_nontemporal_load(p + 1);
+        a[0] = lo.x;
+        a[1] = lo.y;
+        a[2] = hi.x;
+        a[3] = hi.y;
+    }
+
+    template <typename T>
+    __device__ __forceinline__ static void load_chunk(const Cols& cols,
+                                                      const T* __restrict__ val, uint32_t base,
+                                                      uint32_t J1, int lane, uint32_t (&c)[EB],
+                                                      T (&a)[EB][4]) {
+        const uint32_t q = base < J1 - 1u ? base : J1 - 1u;
+        const uint32_t s0 = q >> kSegShift;
+        const uint32_t b0 = cols.segb[s0], b1 = cols.segb[s0 + 1];
+#pragma unroll
+        for (int e = 0; e < EB; ++e) {
+            const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
+            const uint32_t jj = j < J1 ? j : J1 - 1u;
+            c[e] = ((jj >> kSegShift) == s0 ? b0 : b1) + uint32_t(ntload(cols.ci16 + jj));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int e = 0; e < EB; ++e) {
+            const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
+            load_vals(val, j < J1 ? j : J1 - 1u, a[e]);
+        }
+    }
+
+    // Block rows [R0, R1) of one wavefront (same structure as
+    // StreamEngine::run_rows, one block row per lane).
+    template <typename T, typename Epi>
+    __device__ __forceinline__ static void run_rows(uint32_t R0, uint32_t R1,
+                                                    const uint32_t* __restrict__ rp,
+                                                    const Cols& cols, const T* __restrict__ val,
+                                                    const T* __restrict__ x, T* __restrict__ st,
+                                                    int lane, Epi& epi) {
+        using V2 = typename Vec2<T>::type;
+        const uint32_t J1 = __builtin_amdgcn_readfirstlane(rp[R1]);
+        uint32_t r0 = R0;
+        uint32_t nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
+        uint32_t tile_end = rp[r0 + nrows];
+        uint32_t rb = (uint32_t(lane) < nrows) ? rp[r0 + uint32_t(lane)] : tile_end;
+        uint32_t re = __shfl_down(rb, 1, kWave);
+        if (uint32_t(lane) + 1 >= nrows) re = tile_end;
+        uint32_t base = __builtin_amdgcn_readfirstlane(__shfl(rb, 0, kWave));
+        typename Epi::PV pv0{}, pv1{};
+        if (uint32_t(lane) < nrows) {
+            pv0 = epi.pre(2u * (r0 + uint32_t(lane)));
+            pv1 = epi.pre(2u * (r0 + uint32_t(lane)) + 1u);
+        }
+        T acc0 = T(0), acc1 = T(0);
+        uint32_t c[EB];
+        T a[EB][4];
+        if (base < J1) load_chunk(cols, val, base, J1, lane, c, a);
+        while (base < J1) {
+            const uint32_t craw = base + 64u * EB;
+            const uint32_t cend = craw < J1 ? craw : J1;
+            V2 xp[EB];
+#pragma unroll
+            for (int e = 0; e < EB; ++e) xp[e] = *reinterpret_cast<const V2*>(x + 2ull * c[e]);
+            __builtin_amdgcn_sched_barrier(0);
+            uint32_t cn[EB];
+            T an[EB][4];
+            load_chunk(cols, val, craw, J1, lane, cn, an);
+#pragma unroll
+            for (int e = 0; e < EB; ++e) {
+                const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
+                const bool ok = j < J1;
+                T* o = st + 4 * (e * 64 + lane);
+                o[0] = ok ? mul_rn(a[e][0], xp[e].x) : T(0);
+                o[1] = ok ? mul_rn(a[e][1], xp[e].y) : T(0);
+                o[2] = ok ? mul_rn(a[e][2], xp[e].x) : T(0);
+                o[3] = ok ? mul_rn(a[e][3], xp[e].y) : T(0);
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (;;) {
+                const uint32_t lo = rb > base ? rb : base;
+                const uint32_t hi = re < cend ? re : cend;
+                for (uint32_t j = lo; j < hi; ++j) {
+                    const T* o = st + 4 * (j - base);
+                    acc0 = add_rn(add_rn(acc0, o[0]), o[1]);
+                    acc1 = add_rn(add_rn(acc1, o[2]), o[3]);
+                }
+                if (tile_end > cend) break;
+                if (uint32_t(lane) < nrows) {
+                    epi.done(2u * (r0 + uint32_t(lane)), acc0, pv0);
+                    epi.done(2u * (r0 + uint32_t(lane)) + 1u, acc1, pv1);
+                }
+                acc0 = acc1 = T(0);
+                r0 += 64u;
+                if (r0 >= R1) return;
+                nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
+                tile_end = rp[r0 + nrows];
+                rb = (uint32_t(lane) < nrows) ? rp[r0 + uint32_t(lane)] : tile_end;
+                re = __shfl_down(rb, 1, kWave);
+                if (uint32_t(lane) + 1 >= nrows) re = tile_end;
+                if (uint32_t(lane) < nrows) {
+                    pv0 = epi.pre(2u * (r0 + uint32_t(lane)));
+                    pv1 = epi.pre(2u * (r0 + uint32_t(lane)) + 1u);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            base = craw;
+#pragma unroll
+            for (int e = 0; e < EB; ++e) {
+                c[e] = cn[e];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) a[e][k] = an[e][k];
+            }
+        }
+        for (;;) {
+            if (uint32_t(lane) < nrows) {
+                epi.done(2u * (r0 + uint32_t(lane)), acc0, pv0);
+                epi.done(2u * (r0 + uint32_t(lane)) + 1u, acc1, pv1);
+            }
+            acc0 = acc1 = T(0);
+            r0 += 64u;
+            if (r0 >= R1) return;
+            nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
+            if (uint32_t(lane) < nrows) {
+                pv0 = epi.pre(2u * (r0 + uint32_t(lane)));
+                pv1 = epi.pre(2u * (r0 + uint32_t(lane)) + 1u);
+            }
+        }
+    }
+
+    // Block L owns block-row tiles [L*sp, L*sp + sp) (tiles of 64 block
+    // rows); each wavefront takes a contiguous quarter of them.
+    template <typename T, typename Epi>
+    __device__ __forceinline__ static void run(uint32_t m, uint32_t sp, uint32_t L,
+                                               const uint32_t* __restrict__ rp,
+                                               const Cols& cols, const T* __restrict__ val,
+                                               const T* __restrict__ x, T* stage, Epi& epi) {
+        const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+        const uint32_t mb = m / 2;
+        const uint64_t ntiles = (uint64_t(mb) + 63) / 64;
+        uint64_t tb = uint64_t(L) * sp;
+        if (tb > ntiles) tb = ntiles;
+        const uint64_t te = tb + sp < ntiles ? tb + sp : ntiles;
+        const uint64_t q = (te - tb + kWavesPerBlock - 1) / kWavesPerBlock;
+        uint64_t t0 = tb + uint64_t(w) * q;
+        if (t0 > te) t0 = te;
+        const uint64_t t1 = t0 + q < te ? t0 + q : te;
+        if (t0 >= t1) return;
+        const uint32_t R0 = uint32_t(t0 * 64);
+        const uint32_t R1 = uint32_t(t1 * 64 < mb ? t1 * 64 : mb);
+        run_rows(R0, R1, rp, cols, val, x, stage + w * (64 * EB * 4), lane, epi);
+    }
+};
+
+constexpr int kBlockEB = 2;  // blocks per lane per chunk (8 nonzeros)
+
 // Engine for a rows-per-wavefront code, and its LDS staging size.  Codes
 // 64 and above are stream engines; 64 itself is resolved by the launchers
 // (engine_rpw) to kStreamAuto or kStreamAutoPlain.
@@ -407,6 +596,10 @@ template <>
 struct EngineOf<72> {  // 16-bit column offsets, 16 entries per lane per chunk
     using type = StreamEngine<16, true, true>;
 };
+template <>
+struct EngineOf<73> {  // 2x2 blocks (rp, cols, val are the block arrays)
+    using type = BlockStreamEngine<kBlockEB>;
+};
 
 template <int RPW>
 struct StageSize {
@@ -415,6 +608,7 @@ struct StageSize {
                                  : RPW == 65 ? kWavesPerBlock * 64 * 4
                                  : RPW == 66 ? kWavesPerBlock * 64 * 8
                                  : (RPW == 67 || RPW == 72) ? kWavesPerBlock * 64 * 16
+                                 : RPW == 73 ? kWavesPerBlock * 64 * kBlockEB * 4
                                              : 1;
 };
 

@@ -82,7 +82,8 @@ __global__ __launch_bounds__(kBlock) void k_cast_f32(uint64_t n, const double* _
 
 uint32_t spmv_work_per_block(uint32_t m, uint32_t grid, int rpw) {
     if (rpw >= 64) {
-        const uint64_t tiles = (uint64_t(m) + 63) / 64;
+        const uint64_t rows = rpw == 73 ? m / 2 : m;  // 73: tiles of 64 block rows
+        const uint64_t tiles = (rows + 63) / 64;
         uint64_t t = (tiles + grid - 1) / grid;
         return uint32_t(t ? t : 1);
     }
@@ -159,6 +160,71 @@ __global__ __launch_bounds__(kBlock) void k_seg_encode(uint64_t nnz, const uint3
     for (uint64_t j = uint64_t(blockIdx.x) * kBlock + threadIdx.x; j < nnz; j += stride)
         ci16[j] = uint16_t(ci[j] - segb[j >> kSegShift]);
 }
+
+// 2x2 block form (see BlockStreamEngine): *bad stays 0 when every row pair
+// (2I, 2I+1) has the same even-length column list made of pairs (2J, 2J+1).
+__global__ __launch_bounds__(kBlock) void k_block_check(uint32_t nb,
+                                                        const uint32_t* __restrict__ rp,
+                                                        const uint32_t* __restrict__ ci,
+                                                        uint32_t* __restrict__ bad) {
+    const uint32_t I = blockIdx.x * kBlock + threadIdx.x;
+    if (I >= nb) return;
+    const uint32_t b0 = rp[2 * I], b1 = rp[2 * I + 1], e1 = rp[2 * I + 2];
+    const uint32_t len = b1 - b0;
+    uint32_t fail = (e1 - b1 != len) || (len & 1u);
+    for (uint32_t q = 0; !fail && q < len; q += 2) {
+        const uint32_t c = ci[b0 + q];
+        fail = (c & 1u) || ci[b0 + q + 1] != c + 1u || ci[b1 + q] != c ||
+               ci[b1 + q + 1] != c + 1u;
+    }
+    if (fail) atomicOr(bad, 1u);
+}
+
+// brp[I] = rp[2I] / 4; block (I, q/2): bcol = ci[rp[2I] + q] / 2, values
+// (a(2I,c), a(2I,c+1), a(2I+1,c), a(2I+1,c+1)).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_block_pack(uint32_t nb,
+                                                       const uint32_t* __restrict__ rp,
+                                                       const uint32_t* __restrict__ ci,
+                                                       const double* __restrict__ val,
+                                                       uint32_t* __restrict__ brp,
+                                                       uint32_t* __restrict__ bcol,
+                                                       T* __restrict__ bval) {
+    const uint32_t I = blockIdx.x * kBlock + threadIdx.x;
+    if (I > nb) return;
+    const uint32_t b0 = rp[2 * I];
+    brp[I] = b0 / 4;
+    if (I == nb) return;
+    const uint32_t b1 = rp[2 * I + 1], len = b1 - b0;
+    for (uint32_t q = 0; q < len; q += 2) {
+        const uint64_t k = b0 / 4 + q / 2;
+        if (bcol) bcol[k] = ci[b0 + q] / 2;
+        bval[4 * k + 0] = T(val[b0 + q]);
+        bval[4 * k + 1] = T(val[b0 + q + 1]);
+        bval[4 * k + 2] = T(val[b1 + q]);
+        bval[4 * k + 3] = T(val[b1 + q + 1]);
+    }
+}
+
+hipError_t block_check(uint32_t nb, const uint32_t* rp, const uint32_t* ci, uint32_t* bad,
+                       hipStream_t s) {
+    if (nb == 0) return hipSuccess;
+    k_block_check<<<(nb + kBlock - 1) / kBlock, kBlock, 0, s>>>(nb, rp, ci, bad);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t block_pack(uint32_t nb, const uint32_t* rp, const uint32_t* ci, const double* val,
+                      uint32_t* brp, uint32_t* bcol, T* bval, hipStream_t s) {
+    k_block_pack<T><<<(nb + 1 + kBlock - 1) / kBlock, kBlock, 0, s>>>(nb, rp, ci, val, brp,
+                                                                     bcol, bval);
+    return hipGetLastError();
+}
+template hipError_t block_pack<double>(uint32_t, const uint32_t*, const uint32_t*,
+                                       const double*, uint32_t*, uint32_t*, double*,
+                                       hipStream_t);
+template hipError_t block_pack<float>(uint32_t, const uint32_t*, const uint32_t*,
+                                      const double*, uint32_t*, uint32_t*, float*, hipStream_t);
 
 hipError_t seg_bases(uint64_t nnz, const uint32_t* ci, uint32_t* segb, uint32_t* span,
                      hipStream_t s) {

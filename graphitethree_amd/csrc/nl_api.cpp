@@ -238,8 +238,9 @@ void system_mult(System* S, const double* x, double* y) {
     hipStream_t s = S->stream.get();
     S->ws64.ensure(S->n, S->grid);
     copy_in(S->ws64.x.get(), x, size_t(S->n) * 8, s);
-    NLHIP_CHECK(spmv<double>(S->n, S->rowptr.get(), S->cols(), S->val.get(),
-                             S->ws64.x.get(), S->ws64.q.get(), S->rpw, S->grid, s));
+    const SpmvOps<double> o = S->ops<double>();
+    NLHIP_CHECK(spmv<double>(S->n, o.rp, o.cols, o.val, S->ws64.x.get(), S->ws64.q.get(), o.code,
+                             S->grid, s));
     copy_out(y, S->ws64.q.get(), size_t(S->n) * 8, s);
     S->stream.sync();
 }

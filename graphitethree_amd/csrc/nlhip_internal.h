@@ -9,6 +9,7 @@
 #include <cstring>
 #include <mutex>
 #include <shared_mutex>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -303,9 +304,20 @@ class GraphLadder {
 
 // ------------------------------------------------------------------ system
 
+// What an SpMV launch reads: row pointer, column indices, values, and the
+// resolved engine code (NLHIP_RPW_SWITCH).
+template <typename T>
+struct SpmvOps {
+    const uint32_t* rp;
+    Cols cols;
+    const T* val;
+    int code;
+};
+
 template <typename T>
 struct CGWorkspace {
     uint32_t n = 0, P = 0;
+    int code = 0;  // SpMV engine code the arguments were set up for
     DevBuf<T> x, b, r, h, d, q, part;
     DevBuf<CGStatus<T>> st;
     PinnedBuf<CGStatus<T>> st_host;  // two slots
@@ -340,6 +352,12 @@ struct System {
     DevBuf<uint32_t> rowptr, colind;
     DevBuf<uint16_t> ci16;   // 16-bit column offsets (empty when the span is too wide)
     DevBuf<uint32_t> segb;   // their per-segment bases
+    // 2x2 block form (BlockStreamEngine; empty when the matrix has none):
+    // block row pointer, block columns as 16-bit segment offsets, values
+    DevBuf<uint32_t> brp, bsegb;
+    DevBuf<uint16_t> bci16;
+    DevBuf<double> bval;
+    DevBuf<float> bval32;
     DevBuf<double> val, dinv;
     DevBuf<float> val32, dinv32;
     int rpw = 64;        // rows per wavefront of the SpMV engine
@@ -360,7 +378,30 @@ struct System {
     void ensure_fp32();
     uint64_t device_bytes() const;
     Cols cols() const { return Cols{colind.get(), ci16.get(), segb.get()}; }
+    bool blocks() const { return bval.get() != nullptr; }
+    // Arrays and engine code of the SpMV for the current launch shape (FP64
+    // or, for T = float, the FP32 values; ensure_fp32() first).
+    template <typename T>
+    SpmvOps<T> ops() const {
+        const T* v;
+        const T* bv;
+        if constexpr (std::is_same<T, double>::value) {
+            v = val.get();
+            bv = bval.get();
+        } else {
+            v = val32.get();
+            bv = bval32.get();
+        }
+        if (blocks() && (rpw == 64 || rpw == 73))
+            return SpmvOps<T>{brp.get(), Cols{nullptr, bci16.get(), bsegb.get()}, bv, 73};
+        const int code = rpw == 73 ? 64 : rpw;
+        return SpmvOps<T>{rowptr.get(), cols(), v, engine_rpw(code, cols())};
+    }
 };
+
+// Builds the 2x2 block form of S (brp, bci16, bsegb, bval) when the matrix
+// has it (see BlockStreamEngine) and the block columns fit 16-bit offsets.
+bool build_blocks(System& S);
 
 // Builds the 16-bit column offsets of a device colind (see Cols); false when
 // a segment spans more than 65535 columns.

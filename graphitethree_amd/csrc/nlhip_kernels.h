@@ -10,7 +10,8 @@ namespace nlhip {
 
 // Rows-per-wavefront dispatch over the compiled SpMV engines (codes of
 // dev_common.h's EngineOf): 2, 4, 8, 16 -> 32, 16, 8, 4 lanes per row;
-// 65..72 -> stream engines (one row per lane).  Launchers pass the code
+// 65..72 -> stream engines (one row per lane); 73 -> 2x2 block stream
+// engine (its callers pass the block arrays).  Launchers pass the code
 // through engine_rpw first, which resolves 64.
 #define NLHIP_RPW_SWITCH(rpw, CALL)                       \
     switch (rpw) {                                        \
@@ -26,14 +27,16 @@ namespace nlhip {
         case 70: { constexpr int RPW = 70; CALL; } break; \
         case 71: { constexpr int RPW = 71; CALL; } break; \
         case 72: { constexpr int RPW = 72; CALL; } break; \
+        case 73: { constexpr int RPW = 73; CALL; } break; \
         default: return hipErrorInvalidValue;             \
     }
 
 // Launch shapes a caller may set: 2, 4, 8, 16, and the stream engines 64
-// (the default stream engine) and 65..72 (variants kept for sweeps; they
-// compute the same sums as 64).
+// (the default stream engine) and 65..73 (variants kept for sweeps; they
+// compute the same sums as 64; 73 runs as 64 on matrices without 2x2 block
+// form).
 inline bool valid_rpw(int rpw) {
-    return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 72);
+    return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 73);
 }
 
 // Default stream engines: with and without 16-bit column offsets (each wave
@@ -78,6 +81,14 @@ hipError_t cast_f64_f32(uint64_t n, const double* a, float* b, hipStream_t s);
 // segb[j >> kSegShift] once the span is known to fit in 16 bits.
 hipError_t seg_bases(uint64_t nnz, const uint32_t* ci, uint32_t* segb, uint32_t* span,
                      hipStream_t s);
+// 2x2 block form over nb = m/2 row pairs: check (sets *bad, cleared first by
+// the caller), then pack brp (nb + 1), bcol (nnz/4, may be null) and bval
+// (nnz values, 4 per block).
+hipError_t block_check(uint32_t nb, const uint32_t* rp, const uint32_t* ci, uint32_t* bad,
+                       hipStream_t s);
+template <typename T>
+hipError_t block_pack(uint32_t nb, const uint32_t* rp, const uint32_t* ci, const double* val,
+                      uint32_t* brp, uint32_t* bcol, T* bval, hipStream_t s);
 hipError_t seg_encode(uint64_t nnz, const uint32_t* ci, const uint32_t* segb, uint16_t* ci16,
                       hipStream_t s);
 

@@ -46,9 +46,10 @@ uint64_t footprint(uint32_t n, uint64_t nnz, uint32_t k, bool mixed) {
     const uint64_t parts = 4ull * 8 * 1024;                       // per-workgroup partials
     uint64_t b = 4ull * (n + 1) + 12ull * nnz + 8ull * n;        // rowptr, colind, val, dinv
     b += 2ull * nnz + 4ull * (nnz / 1024 + 2);                    // 16-bit column offsets
+    b += 8ull * nnz + nnz / 2 + 2ull * n + 4ull * (nnz / 4096 + 2); // 2x2 block form
     if (k <= 1) b += 8ull * 8 * n + parts;                        // x b r h d q + scratch
     else b += 7ull * 8 * n * k + parts * k;                       // interleaved vectors
-    if (mixed) b += 4ull * nnz + 4ull * n + (k <= 1 ? 7ull * 4 * n : 6ull * 4 * n * k);
+    if (mixed) b += 8ull * nnz + 4ull * n + (k <= 1 ? 7ull * 4 * n : 6ull * 4 * n * k);
     return b + (256ull << 10);                                     // allocation slack
 }
 

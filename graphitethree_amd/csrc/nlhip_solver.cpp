@@ -69,6 +69,32 @@ bool build_col_offsets(uint64_t nnz, const uint32_t* colind, hipStream_t s,
     return true;
 }
 
+bool build_blocks(System& S) {
+    if (S.n < 2 || (S.n & 1u) || S.nnz == 0 || (S.nnz & 3u)) return false;
+    const uint32_t nb = S.n / 2;
+    const uint64_t nblk = S.nnz / 4;
+    hipStream_t s = S.stream.get();
+    DevBuf<uint32_t> bad(1);
+    NLHIP_CHECK(fill<int32_t>(1, 0, reinterpret_cast<int32_t*>(bad.get()), s));
+    NLHIP_CHECK(block_check(nb, S.rowptr.get(), S.colind.get(), bad.get(), s));
+    uint32_t h_bad = 1;
+    NLHIP_CHECK(hipMemcpyAsync(&h_bad, bad.get(), 4, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipStreamSynchronize(s));
+    if (h_bad) return false;
+    DevBuf<uint32_t> brp(size_t(nb) + 1), bcol(nblk);
+    DevBuf<double> bval(S.nnz);
+    NLHIP_CHECK(block_pack<double>(nb, S.rowptr.get(), S.colind.get(), S.val.get(), brp.get(),
+                                   bcol.get(), bval.get(), s));
+    DevBuf<uint16_t> bci16;
+    DevBuf<uint32_t> bsegb;
+    if (!build_col_offsets(nblk, bcol.get(), s, bci16, bsegb)) return false;
+    S.brp = std::move(brp);
+    S.bval = std::move(bval);
+    S.bci16 = std::move(bci16);
+    S.bsegb = std::move(bsegb);
+    return true;
+}
+
 System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
                       const double* val, uint32_t flags) {
     if (n == 0 || !rowptr) throw Error(NLHIP_ERR_INVALID, "empty system");
@@ -115,8 +141,10 @@ System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind
     copy_in(S->val.get(), va, nnz * 8, s);
     NLHIP_CHECK(jacobi_dinv<double>(n, S->rowptr.get(), S->colind.get(), S->val.get(),
                                     S->dinv.get(), s));
-    if (!(flags & NLHIP_SYSTEM_NO_OFFSETS))
+    if (!(flags & NLHIP_SYSTEM_NO_OFFSETS)) {
         build_col_offsets(nnz, S->colind.get(), s, S->ci16, S->segb);
+        if (!(flags & NLHIP_SYSTEM_NO_BLOCKS)) build_blocks(*S);
+    }
     default_launch(n, nnz, S->rpw, S->grid);
     if (flags & NLHIP_SYSTEM_FP32) S->ensure_fp32();
     S->stream.sync();
@@ -130,10 +158,16 @@ void System::ensure_fp32() {
     hipStream_t s = stream.get();
     NLHIP_CHECK(cast_f64_f32(nnz, val.get(), val32.get(), s));
     NLHIP_CHECK(jacobi_dinv<float>(n, rowptr.get(), colind.get(), val.get(), dinv32.get(), s));
+    if (blocks()) {
+        bval32.alloc(nnz);
+        NLHIP_CHECK(block_pack<float>(n / 2, rowptr.get(), colind.get(), val.get(), brp.get(),
+                                      nullptr, bval32.get(), s));
+    }
 }
 
 uint64_t System::device_bytes() const {
     return rowptr.bytes() + colind.bytes() + ci16.bytes() + segb.bytes() + val.bytes() +
+           brp.bytes() + bsegb.bytes() + bci16.bytes() + bval.bytes() + bval32.bytes() +
            dinv.bytes() + val32.bytes() +
            dinv32.bytes() + ws64.x.bytes() * 7 + ws32.x.bytes() * 7 + scratch.bytes();
 }
@@ -158,21 +192,24 @@ template struct CGWorkspace<double>;
 template struct CGWorkspace<float>;
 
 template <typename T>
-static void setup_args(System& S, CGWorkspace<T>& ws, const T* val, const T* dinv) {
+static void setup_args(System& S, CGWorkspace<T>& ws, const T* dinv) {
     CGArgs<T>& a = ws.args;
+    const SpmvOps<T> o = S.ops<T>();
+    ws.code = o.code;
     a.m = S.n;
     a.P = S.grid;
     a.chunk = uint32_t((uint64_t(S.n) + S.grid - 1) / S.grid);
-    a.sp = spmv_work_per_block(S.n, S.grid, S.rpw);
+    a.sp = spmv_work_per_block(S.n, S.grid, o.code);
     // the streamed SpMV step runs with up to twice the workgroups (at least
     // four 64-row tiles each)
     const uint64_t quads = (uint64_t(S.n) + 255) / 256;
-    a.Ps = S.rpw >= 64 ? uint32_t(std::max<uint64_t>(S.grid, std::min<uint64_t>(2ull * S.grid, quads)))
-                       : S.grid;
-    a.sps = spmv_work_per_block(S.n, a.Ps, S.rpw);
-    a.rp = S.rowptr.get();
-    a.cols = S.cols();
-    a.val = val;
+    a.Ps = o.code >= 64
+               ? uint32_t(std::max<uint64_t>(S.grid, std::min<uint64_t>(2ull * S.grid, quads)))
+               : S.grid;
+    a.sps = spmv_work_per_block(S.n, a.Ps, o.code);
+    a.rp = o.rp;
+    a.cols = o.cols;
+    a.val = o.val;
     a.dinv = dinv;
     a.x = ws.x.get();
     a.b = ws.b.get();
@@ -200,14 +237,14 @@ template <typename T>
 static CGStatus<T> run_cg(System& S, CGWorkspace<T>& ws, bool precond, T eps, int max_iter,
                           uint32_t K) {
     hipStream_t s = S.stream.get();
-    NLHIP_CHECK(cg_start<T>(ws.args, S.rpw, precond, eps, max_iter, s));
-    if (ws.graph_rpw != S.rpw || ws.graph_precond != precond ||
+    NLHIP_CHECK(cg_start<T>(ws.args, ws.code, precond, eps, max_iter, s));
+    if (ws.graph_rpw != ws.code || ws.graph_precond != precond ||
         std::memcmp(&ws.graph_args, &ws.args, sizeof(CGArgs<T>)) != 0)
         ws.graph.reset();
     ws.graph.ensure(K, s, [&](uint32_t k) {
-        for (uint32_t i = 0; i < k; ++i) NLHIP_CHECK(cg_step<T>(ws.args, S.rpw, precond, s));
+        for (uint32_t i = 0; i < k; ++i) NLHIP_CHECK(cg_step<T>(ws.args, ws.code, precond, s));
     });
-    ws.graph_rpw = S.rpw;
+    ws.graph_rpw = ws.code;
     ws.graph_precond = precond;
     ws.graph_args = ws.args;
     Event ev[2];
@@ -249,8 +286,8 @@ static void residual_norms(System& S, const double* b, const double* x, double* 
     S.scratch.ensure(size_t(2) * P + 4);
     double* part = S.scratch.get();
     double* out = part + 2 * size_t(P);
-    NLHIP_CHECK(residual64(S.n, S.rowptr.get(), S.cols(), S.val.get(), x, b, r_out, part,
-                           part + P, S.rpw, P, s));
+    const SpmvOps<double> o = S.ops<double>();
+    NLHIP_CHECK(residual64(S.n, o.rp, o.cols, o.val, x, b, r_out, part, part + P, o.code, P, s));
     NLHIP_CHECK(reduce_final(part, P, out, s));
     NLHIP_CHECK(reduce_final(part + P, P, out + 2, s));
     S.host_scalars.ensure(4);
@@ -271,7 +308,7 @@ void solve_fp64(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st) {
     CGWorkspace<double>& ws = S.ws64;
     const bool precond = p.precond == NL_PRECOND_JACOBI;
     const double* dinv = S.dinv_override ? S.dinv_override : S.dinv.get();
-    setup_args<double>(S, ws, S.val.get(), precond ? dinv : nullptr);
+    setup_args<double>(S, ws, precond ? dinv : nullptr);
     const uint32_t K = p.check_every ? p.check_every : auto_check_every(S);
     CGStatus<double> fin = run_cg<double>(S, ws, precond, p.eps, int(p.max_iter), K);
     if (!std::isfinite(fin.bb) || !std::isfinite(fin.err))
@@ -288,7 +325,7 @@ void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st) 
     CGWorkspace<float>& w32 = S.ws32;
     w32.ensure(S.n, S.grid);
     const bool precond = p.precond == NL_PRECOND_JACOBI;
-    setup_args<float>(S, w32, S.val32.get(), precond ? S.dinv32.get() : nullptr);
+    setup_args<float>(S, w32, precond ? S.dinv32.get() : nullptr);
     hipStream_t s = S.stream.get();
     const uint32_t K = p.check_every ? p.check_every : auto_check_every(S);
     const uint32_t inner_cap = p.inner_max_iter ? p.inner_max_iter : p.max_iter;
@@ -376,9 +413,9 @@ uint32_t bicgstab_device(System& S, const double* dinv, const double* b, double*
     DevBuf<double> part(S.grid), out(2);
     S.host_scalars.ensure(4);
     DotHelper dot{S, part, out.get(), S.host_scalars.get()};
+    const SpmvOps<double> A = S.ops<double>();
     auto spmv64 = [&](const double* in, double* o) {
-        NLHIP_CHECK(spmv<double>(n, S.rowptr.get(), S.cols(), S.val.get(), in, o, S.rpw,
-                                 S.grid, s));
+        NLHIP_CHECK(spmv<double>(n, A.rp, A.cols, A.val, in, o, A.code, S.grid, s));
     };
     auto precond = [&](const double* in, double* o) {
         if (dinv) NLHIP_CHECK(blas_dmul(n, dinv, in, o, s));
@@ -493,7 +530,7 @@ NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,
     return guarded("nlhipSystemSetLaunch", [&] {
         System* S = sys(s);
         if (!valid_rpw(rows_per_wave))
-            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64..72");
+            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64..73");
         if (workgroups == 0 || workgroups > 65536)
             throw Error(NLHIP_ERR_INVALID, "workgroups must be in [1, 65536]");
         std::lock_guard<std::mutex> lock(S->mu);
@@ -536,8 +573,9 @@ NLAPI int NLAPIENTRY nlhipSpMV(nlhip_system s, const double* x, double* y) {
         hipStream_t st = S->stream.get();
         S->ws64.ensure(S->n, S->grid);
         copy_in(S->ws64.x.get(), x, size_t(S->n) * 8, st);
-        NLHIP_CHECK(spmv<double>(S->n, S->rowptr.get(), S->cols(), S->val.get(),
-                                 S->ws64.x.get(), S->ws64.q.get(), S->rpw, S->grid, st));
+        const SpmvOps<double> o = S->ops<double>();
+        NLHIP_CHECK(spmv<double>(S->n, o.rp, o.cols, o.val, S->ws64.x.get(), S->ws64.q.get(),
+                                 o.code, S->grid, st));
         copy_out(y, S->ws64.q.get(), size_t(S->n) * 8, st);
         S->stream.sync();
     });
@@ -555,9 +593,10 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
             S->ensure_fp32();
             S->ws32.ensure(S->n, S->grid);
             NLHIP_CHECK(fill<float>(S->n, 1.0f, S->ws32.x.get(), st));
+            const SpmvOps<float> o = S->ops<float>();
             auto run = [&] {
-                NLHIP_CHECK(spmv<float>(S->n, S->rowptr.get(), S->cols(), S->val32.get(),
-                                        S->ws32.x.get(), S->ws32.q.get(), S->rpw, S->grid, st));
+                NLHIP_CHECK(spmv<float>(S->n, o.rp, o.cols, o.val, S->ws32.x.get(),
+                                        S->ws32.q.get(), o.code, S->grid, st));
             };
             run();
             e0.record(st);
@@ -566,9 +605,10 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
         } else {
             S->ws64.ensure(S->n, S->grid);
             NLHIP_CHECK(fill<double>(S->n, 1.0, S->ws64.x.get(), st));
+            const SpmvOps<double> o = S->ops<double>();
             auto run = [&] {
-                NLHIP_CHECK(spmv<double>(S->n, S->rowptr.get(), S->cols(), S->val.get(),
-                                         S->ws64.x.get(), S->ws64.q.get(), S->rpw, S->grid, st));
+                NLHIP_CHECK(spmv<double>(S->n, o.rp, o.cols, o.val, S->ws64.x.get(),
+                                         S->ws64.q.get(), o.code, S->grid, st));
             };
             run();
             e0.record(st);
@@ -622,18 +662,18 @@ NLAPI int NLAPIENTRY nlhipProfileCG(nlhip_system s, int fp32, uint32_t iters, do
         std::lock_guard<std::mutex> lock(S->mu);
         hipStream_t st = S->stream.get();
         if (iters == 0) iters = 1;
-        auto profile = [&](auto& ws, auto* val, auto* dinv, auto zero) {
+        auto profile = [&](auto& ws, auto* dinv, auto zero) {
             using T = decltype(zero);
             ws.ensure(S->n, S->grid);
-            setup_args<T>(*S, ws, val, dinv);
+            setup_args<T>(*S, ws, dinv);
             NLHIP_CHECK(fill<T>(S->n, T(1), ws.b.get(), st));
             NLHIP_CHECK(fill<T>(S->n, T(0), ws.x.get(), st));
-            NLHIP_CHECK(cg_start<T>(ws.args, S->rpw, true, T(0), int(iters) + 8, st));
+            NLHIP_CHECK(cg_start<T>(ws.args, ws.code, true, T(0), int(iters) + 8, st));
             std::vector<Event> ev(4 * size_t(iters));
             for (uint32_t k = 0; k < iters; ++k) {
                 for (int part = 0; part < 3; ++part) {
                     ev[4 * k + part].record(st);
-                    NLHIP_CHECK(cg_step_part<T>(ws.args, S->rpw, true, part, st));
+                    NLHIP_CHECK(cg_step_part<T>(ws.args, ws.code, true, part, st));
                 }
                 ev[4 * k + 3].record(st);
             }
@@ -647,9 +687,9 @@ NLAPI int NLAPIENTRY nlhipProfileCG(nlhip_system s, int fp32, uint32_t iters, do
         };
         if (fp32) {
             S->ensure_fp32();
-            profile(S->ws32, S->val32.get(), S->dinv32.get(), 0.0f);
+            profile(S->ws32, S->dinv32.get(), 0.0f);
         } else {
-            profile(S->ws64, S->val.get(), S->dinv.get(), 0.0);
+            profile(S->ws64, S->dinv.get(), 0.0);
         }
     });
 }

@@ -57,7 +57,10 @@ NLAPI NLMatrix NLAPIENTRY nlHIPJacobiPreconditionerNewFromCRSMatrix(NLMatrix M);
 
 #define NLHIP_SYSTEM_FP32 1u /* also keep an FP32 copy of the values         */
 #define NLHIP_SYSTEM_NO_OFFSETS 2u /* read 32-bit column indices only (no
-                                      16-bit segment offsets)               */
+                                      16-bit segment offsets, no 2x2 blocks) */
+#define NLHIP_SYSTEM_NO_BLOCKS 4u  /* no 2x2 block form (kept otherwise when
+                                      rows 2I, 2I+1 share columns that come
+                                      in pairs 2J, 2J+1, as in LSCM)       */
 
 typedef struct nlhip_system_s* nlhip_system;
 
@@ -96,7 +99,7 @@ NLAPI int NLAPIENTRY nlhipSystemInfo(nlhip_system s, uint32_t* n, uint64_t* nnz,
                                      uint64_t* device_bytes);
 /* Launch shape of the SpMV engine: rows per wavefront (2, 4, 8, 16: that many
  * rows share a 64-lane wavefront; 64: one row per lane, nonzeros streamed
- * through LDS, the default) and workgroups per launch.  65..72 select
+ * through LDS, the default) and workgroups per launch.  65..73 select
  * variants of the streamed engine for launch-shape sweeps (chunk width,
  * load kind, column format, tile order); they give the same sums as 64. */
 NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,

@@ -23,7 +23,7 @@ def test_spmv_exact_all_launch_shapes(hip):
     for name in z["names"]:
         rp, ci, va = z[f"{name}_rowptr"], z[f"{name}_colind"], z[f"{name}_val"]
         S = capi.System(rp, ci, va)
-        for rpw in (2, 4, 8, 16, 64, 65, 66, 67, 68, 69, 70, 71, 72):
+        for rpw in (2, 4, 8, 16, 64, 65, 66, 67, 68, 69, 70, 71, 72, 73):
             for wg in (1, 3, 64, 2048):
                 S.set_launch(rpw, wg)
                 y = S.spmv(z[f"{name}_x"])
@@ -50,7 +50,7 @@ def test_stream_engine_is_bitwise_the_oracle_spmv(hip):
         x = np.random.default_rng(4).standard_normal(Sw.n)
         S = capi.System.from_crs(Sw)
         ref = O.spmv(A, x)
-        for rpw in (64, 65, 66, 67, 68, 69, 70, 71, 72):
+        for rpw in (64, 65, 66, 67, 68, 69, 70, 71, 72, 73):
             for wg in (1, 5, 9, 300):
                 S.set_launch(rpw, wg)
                 assert np.array_equal(S.spmv(x), ref), (rpw, wg)
@@ -96,6 +96,61 @@ def test_column_offsets_match_32bit_indices(hip):
                 assert np.array_equal(P.spmv(x), ref), (name, rpw, wg)
         S.close()
         P.close()
+
+
+def test_block_form_matches_crs(hip):
+    """LSCM matrices are stored in 2x2 block form (rows 2I, 2I+1 share their
+    columns, which come in pairs): the block engine gives the bits of the
+    oracle's row loop on every launch shape, and a matrix that breaks the
+    pattern in one place keeps the scalar format (same results)."""
+    rng = np.random.default_rng(5)
+    Sw = W.lscm_grid(70, seed=4)
+    x = rng.standard_normal(Sw.n)
+    ref = O.spmv(O.CRS(Sw.rowptr, Sw.colind, Sw.val), x)
+    B = capi.System.from_crs(Sw)
+    C = capi.System.from_crs(Sw, blocks=False)
+    assert B.info()[2] > C.info()[2]          # block arrays were built
+    for rpw in (64, 73, 68, 71):  # the stream engines sum rows left to right
+        for wg in (1, 3, 77, 2048):
+            B.set_launch(rpw, wg)
+            C.set_launch(rpw, wg)
+            assert np.array_equal(B.spmv(x), ref), (rpw, wg)
+            assert np.array_equal(C.spmv(x), ref), (rpw, wg)
+    # break the pair structure of one row: swap two column values' order is
+    # not allowed (sorted rows), so drop one entry of row 0's last pair
+    rp, ci, va = Sw.rowptr.astype(np.int64), Sw.colind.copy(), Sw.val.copy()
+    cut = rp[1] - 1
+    keep = np.ones(len(ci), bool)
+    keep[cut] = False
+    rp2 = rp.copy()
+    rp2[1:] -= 1
+    ci2, va2 = ci[keep], va[keep]
+    ref2 = O.spmv(O.CRS(rp2.astype(np.uint32), ci2, va2), x)
+    D = capi.System(rp2, ci2, va2)
+    E = capi.System(rp2, ci2, va2, blocks=False)
+    assert D.info()[2] == E.info()[2]          # no block form
+    D.set_launch(73, 64)
+    assert np.array_equal(D.spmv(x), ref2)
+    for S in (B, C, D, E):
+        S.close()
+
+
+def test_block_form_solves_like_scalar_form(hip):
+    """PCG with the block engine and with the scalar stream engine: same
+    residual target, iterations within the summation-order tolerance, both
+    within 1e-8 of the oracle."""
+    Sw = W.lscm_grid(40, seed=9)
+    A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+    xo, ito, _ = O.pcg(A, Sw.b, None, 1e-10, 20000)
+    for blocks in (True, False):
+        S = capi.System.from_crs(Sw, blocks=blocks, fp32=True)
+        x, st = S.solve(Sw.b, eps=1e-10, max_iter=20000)
+        assert st.converged and st.rel_residual <= 1e-10
+        assert abs(int(st.iterations) - ito) <= max(2, ito // 20)
+        assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-8
+        xm, sm = S.solve(Sw.b, precision="mixed", eps=1e-10, max_iter=200000)
+        assert sm.rel_residual <= 1e-10
+        S.close()
 
 
 def test_set_launch_rejects_bad_shapes(hip):
