@@ -369,8 +369,8 @@ struct StreamEngine {
 // which come in pairs (2J, 2J+1) (LSCM normal equations: the u and v
 // unknowns of a vertex).  Block row I stores its blocks as (a(2I,2J),
 // a(2I,2J+1), a(2I+1,2J), a(2I+1,2J+1)) with the block column J as a 16-bit
-// segment offset (Cols::ci16 / Cols::segb over the block stream): 8.5 bytes
-// per FP64 nonzero.  One block row per lane, 64 block rows per wavefront
+// segment offset (Cols::ci16 / Cols::segb over the block stream; 8.5 bytes
+// per FP64 nonzero) or, with D16 = false, as a 32-bit index in Cols::ci.  One block row per lane, 64 block rows per wavefront
 // tile, chunks of 64*EB blocks; the four products of every block are staged
 // in LDS and each lane sums its two rows left to right (a(2I,2J) x_2J, then
 // a(2I,2J+1) x_2J+1, for J ascending) -- the order and roundings of the CRS
@@ -388,7 +388,7 @@ struct Vec2<float> {
     typedef float type __attribute__((ext_vector_type(2)));
 };
 
-template <int EB>
+template <int EB, bool D16 = true>
 struct BlockStreamEngine {
     static_assert(64 * EB <= int(kSegEntries), "a chunk must span at most two segments");
 
@@ -411,14 +411,22 @@ struct BlockStreamEngine {
                                                       const T* __restrict__ val, uint32_t base,
                                                       uint32_t J1, int lane, uint32_t (&c)[EB],
                                                       T (&a)[EB][4]) {
-        const uint32_t q = base < J1 - 1u ? base : J1 - 1u;
-        const uint32_t s0 = q >> kSegShift;
-        const uint32_t b0 = cols.segb[s0], b1 = cols.segb[s0 + 1];
+        if constexpr (D16) {
+            const uint32_t q = base < J1 - 1u ? base : J1 - 1u;
+            const uint32_t s0 = q >> kSegShift;
+            const uint32_t b0 = cols.segb[s0], b1 = cols.segb[s0 + 1];
 #pragma unroll
-        for (int e = 0; e < EB; ++e) {
-            const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
-            const uint32_t jj = j < J1 ? j : J1 - 1u;
-            c[e] = ((jj >> kSegShift) == s0 ? b0 : b1) + uint32_t(ntload(cols.ci16 + jj));
+            for (int e = 0; e < EB; ++e) {
+                const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
+                const uint32_t jj = j < J1 ? j : J1 - 1u;
+                c[e] = ((jj >> kSegShift) == s0 ? b0 : b1) + uint32_t(ntload(cols.ci16 + jj));
+            }
+        } else {  // 32-bit block columns in cols.ci
+#pragma unroll
+            for (int e = 0; e < EB; ++e) {
+                const uint32_t j = base + uint32_t(e) * 64u + uint32_t(lane);
+                c[e] = ntload(cols.ci + (j < J1 ? j : J1 - 1u));
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -429,14 +437,19 @@ struct BlockStreamEngine {
     }
 
     // Block rows [R0, R1) of one wavefront (same structure as
-    // StreamEngine::run_rows, one block row per lane).
+    // StreamEngine::run_rows, one block row per lane).  Block row B holds
+    // rows 2B + delta and 2B + 1 + delta, block column J columns 2J + delta
+    // and 2J + 1 + delta (delta = 0 for a whole matrix; a batched system
+    // may sit at any row offset).
     template <typename T, typename Epi>
     __device__ __forceinline__ static void run_rows(uint32_t R0, uint32_t R1,
                                                     const uint32_t* __restrict__ rp,
                                                     const Cols& cols, const T* __restrict__ val,
                                                     const T* __restrict__ x, T* __restrict__ st,
-                                                    int lane, Epi& epi) {
+                                                    int lane, Epi& epi, int32_t delta = 0) {
         using V2 = typename Vec2<T>::type;
+        const T* __restrict__ xs = x + delta;
+        const bool aligned = (delta & 1) == 0;
         const uint32_t J1 = __builtin_amdgcn_readfirstlane(rp[R1]);
         uint32_t r0 = R0;
         uint32_t nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
@@ -445,10 +458,11 @@ struct BlockStreamEngine {
         uint32_t re = __shfl_down(rb, 1, kWave);
         if (uint32_t(lane) + 1 >= nrows) re = tile_end;
         uint32_t base = __builtin_amdgcn_readfirstlane(__shfl(rb, 0, kWave));
+        auto row_of = [&](uint32_t B) { return uint32_t(int64_t(2u * B) + delta); };
         typename Epi::PV pv0{}, pv1{};
         if (uint32_t(lane) < nrows) {
-            pv0 = epi.pre(2u * (r0 + uint32_t(lane)));
-            pv1 = epi.pre(2u * (r0 + uint32_t(lane)) + 1u);
+            pv0 = epi.pre(row_of(r0 + uint32_t(lane)));
+            pv1 = epi.pre(row_of(r0 + uint32_t(lane)) + 1u);
         }
         T acc0 = T(0), acc1 = T(0);
         uint32_t c[EB];
@@ -458,8 +472,17 @@ struct BlockStreamEngine {
             const uint32_t craw = base + 64u * EB;
             const uint32_t cend = craw < J1 ? craw : J1;
             V2 xp[EB];
+            if (aligned) {
 #pragma unroll
-            for (int e = 0; e < EB; ++e) xp[e] = *reinterpret_cast<const V2*>(x + 2ull * c[e]);
+                for (int e = 0; e < EB; ++e)
+                    xp[e] = *reinterpret_cast<const V2*>(xs + 2ull * c[e]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < EB; ++e) {
+                    xp[e].x = xs[2ull * c[e]];
+                    xp[e].y = xs[2ull * c[e] + 1];
+                }
+            }
             __builtin_amdgcn_sched_barrier(0);
             uint32_t cn[EB];
             T an[EB][4];
@@ -485,8 +508,8 @@ struct BlockStreamEngine {
                 }
                 if (tile_end > cend) break;
                 if (uint32_t(lane) < nrows) {
-                    epi.done(2u * (r0 + uint32_t(lane)), acc0, pv0);
-                    epi.done(2u * (r0 + uint32_t(lane)) + 1u, acc1, pv1);
+                    epi.done(row_of(r0 + uint32_t(lane)), acc0, pv0);
+                    epi.done(row_of(r0 + uint32_t(lane)) + 1u, acc1, pv1);
                 }
                 acc0 = acc1 = T(0);
                 r0 += 64u;
@@ -497,8 +520,8 @@ struct BlockStreamEngine {
                 re = __shfl_down(rb, 1, kWave);
                 if (uint32_t(lane) + 1 >= nrows) re = tile_end;
                 if (uint32_t(lane) < nrows) {
-                    pv0 = epi.pre(2u * (r0 + uint32_t(lane)));
-                    pv1 = epi.pre(2u * (r0 + uint32_t(lane)) + 1u);
+                    pv0 = epi.pre(row_of(r0 + uint32_t(lane)));
+                    pv1 = epi.pre(row_of(r0 + uint32_t(lane)) + 1u);
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -512,16 +535,16 @@ struct BlockStreamEngine {
         }
         for (;;) {
             if (uint32_t(lane) < nrows) {
-                epi.done(2u * (r0 + uint32_t(lane)), acc0, pv0);
-                epi.done(2u * (r0 + uint32_t(lane)) + 1u, acc1, pv1);
+                epi.done(row_of(r0 + uint32_t(lane)), acc0, pv0);
+                epi.done(row_of(r0 + uint32_t(lane)) + 1u, acc1, pv1);
             }
             acc0 = acc1 = T(0);
             r0 += 64u;
             if (r0 >= R1) return;
             nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
             if (uint32_t(lane) < nrows) {
-                pv0 = epi.pre(2u * (r0 + uint32_t(lane)));
-                pv1 = epi.pre(2u * (r0 + uint32_t(lane)) + 1u);
+                pv0 = epi.pre(row_of(r0 + uint32_t(lane)));
+                pv1 = epi.pre(row_of(r0 + uint32_t(lane)) + 1u);
             }
         }
     }

@@ -20,20 +20,50 @@
 
 namespace nlhip {
 
+// Rows [r0, r1) of tile t: kTileRows rows, or kBlockTileRows for systems in
+// 2x2 block form (a tile is one pass of the workgroup's 4 wavefronts).  The
+// tiling of a system depends only on that system.
 template <typename T>
 __device__ __forceinline__ void tile_rows(const BatchArgs<T>& a, uint32_t t, uint32_t& r0,
                                           uint32_t& r1) {
     const uint32_t s = a.tile_sys[t];
     r0 = a.tile_row0[t];
     const uint32_t e = a.row_off[s + 1];
-    r1 = (r0 + kTileRows < e) ? r0 + kTileRows : e;
+    const uint32_t tr = (a.sys_pair0 && a.sys_pair0[s] != kNoBlocks) ? kBlockTileRows : kTileRows;
+    r1 = (r0 + tr < e) ? r0 + tr : e;
 }
 
-// Stream SpMV of one tile: wavefront w takes rows [r0 + 64 w, r0 + 64 w + 64).
+template <typename T>
+__device__ __forceinline__ bool tile_blk(const BatchArgs<T>& a, uint32_t t) {
+    return a.sys_pair0 && a.sys_pair0[a.tile_sys[t]] != kNoBlocks;
+}
+
+// Stream SpMV of one tile: wavefront w takes rows [r0 + 64 w, r0 + 64 w + 64)
+// or, for a system in 2x2 block form (blk), the block rows of its row pairs
+// [r0 + 128 w, r0 + 128 w + 128) (r0 - row_off[s] and r1 - row_off[s] are
+// even for such a system).
 template <typename T, typename Epi>
-__device__ __forceinline__ void tile_spmv(const BatchArgs<T>& a, uint32_t r0, uint32_t r1,
-                                          const T* x, T* stage, Epi& epi) {
+__device__ __forceinline__ void tile_spmv(const BatchArgs<T>& a, uint32_t t, uint32_t r0,
+                                          uint32_t r1, bool blk, const T* x, T* stage,
+                                          Epi& epi) {
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (blk) {
+        static_assert(kBlockEB * 4 <= kStreamE, "staging area too small for the block engine");
+        const uint32_t s = a.tile_sys[t], lo = a.row_off[s], p0 = a.sys_pair0[s];
+        const int32_t delta = int32_t(lo - 2u * p0);
+        const uint32_t B0 = p0 + (r0 - lo) / 2 + 64u * uint32_t(w), Bend = p0 + (r1 - lo) / 2;
+        if (B0 >= Bend) return;
+        const uint32_t B1 = (B0 + 64u < Bend) ? B0 + 64u : Bend;
+        const Cols bcols{a.bcol, a.bci16, a.bsegb};
+        T* st = stage + w * (64 * kBlockEB * 4);
+        if (a.bci16)
+            BlockStreamEngine<kBlockEB, true>::run_rows(B0, B1, a.brp, bcols, a.bval, x, st,
+                                                        lane, epi, delta);
+        else
+            BlockStreamEngine<kBlockEB, false>::run_rows(B0, B1, a.brp, bcols, a.bval, x, st,
+                                                         lane, epi, delta);
+        return;
+    }
     const uint32_t R0 = r0 + 64u * uint32_t(w);
     if (R0 >= r1) return;
     const uint32_t R1 = (R0 + 64u < r1) ? R0 + 64u : r1;
@@ -68,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_b_init(BatchArgs<T> a) {
                 prr += ri * ri;
                 pbb += bi * bi;
             });
-        tile_spmv(a, r0, r1, a.x, stage, epi);
+        tile_spmv(a, t, r0, r1, tile_blk(a, t), a.x, stage, epi);
         prh = block_sum_all(prh, lds);
         prr = block_sum_all(prr, lds);
         pbb = block_sum_all(pbb, lds);
@@ -149,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(BatchArgs<T> a) {
                                     a.q[row] = v;
                                     pdq += dr * v;
                                 });
-        tile_spmv(a, r0, r1, a.d, stage, epi);
+        tile_spmv(a, t, r0, r1, tile_blk(a, t), a.d, stage, epi);
         pdq = block_sum_all(pdq, lds);
         if (threadIdx.x == 0) a.part_dq[t] = pdq;
     }
@@ -274,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_b_residual64(BatchArgs<double> a) {
                                     srr += ri * ri;
                                     sbb += bi * bi;
                                 });
-        tile_spmv(a, r0, r1, a.x, stage, epi);
+        tile_spmv(a, t, r0, r1, tile_blk(a, t), a.x, stage, epi);
         srr = block_sum_all(srr, lds);
         sbb = block_sum_all(sbb, lds);
         if (threadIdx.x == 0) {

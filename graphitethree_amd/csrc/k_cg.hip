@@ -15,8 +15,9 @@
 //              with the previous iteration's alpha and d (the update of x
 //              is deferred to here, where d is read anyway); stopping test;
 //              beta = rho / rho_prev; d = beta d + h (d = h at it 0)
-//   spmv       q = A d and partials of d.q (up to twice the workgroups of
-//              the other launches: more waves in flight for the gathers)
+//   spmv       q = A d and partials of d.q (up to kSpmvGridMul times the
+//              workgroups of the other launches: more waves in flight for
+//              the gathers)
 //   update     reduce the d.q partials -> alpha; r -= alpha q; h = P r;
 //              partials of r.h and r.r
 // Per iteration that moves 12 nnz + 100 n bytes in FP64 with 32-bit column

@@ -34,6 +34,11 @@ struct Batch {
     DevBuf<uint32_t> row_off, tile_off, tile_sys, tile_row0, rowptr, colind;
     DevBuf<uint16_t> ci16;  // 16-bit column offsets (empty when they do not fit)
     DevBuf<uint32_t> segb;
+    // 2x2 block form of the systems that have it (empty when none has)
+    DevBuf<uint32_t> sys_pair0, brp, bcol, bsegb;
+    DevBuf<uint16_t> bci16;
+    DevBuf<double> bval;
+    DevBuf<float> bval32;
     DevBuf<double> val, dinv, scale;  // scale: 2 x nsys (scale, unscale)
     DevBuf<float> val32, dinv32;
     std::vector<uint32_t> h_row_off;
@@ -76,6 +81,13 @@ void work_alloc(Batch& B, BatchWork<T>& w) {
     a.ci = B.colind.get();
     a.ci16 = B.ci16.get();
     a.segb = B.segb.get();
+    a.sys_pair0 = B.sys_pair0.get();
+    a.brp = B.brp.get();
+    a.bcol = B.bcol.get();
+    a.bci16 = B.bci16.get();
+    a.bsegb = B.bsegb.get();
+    if constexpr (std::is_same<T, double>::value) a.bval = B.bval.get();
+    else a.bval = B.bval32.get();
     a.x = w.x.get();
     a.b = w.b.get();
     a.r = w.r.get();
@@ -157,6 +169,10 @@ void ensure_fp32(Batch& B) {
     NLHIP_CHECK(cast_f64_f32(B.nnz, B.val.get(), B.val32.get(), B.stream.get()));
     NLHIP_CHECK(jacobi_dinv<float>(B.n, B.rowptr.get(), B.colind.get(), B.val.get(),
                                    B.dinv32.get(), B.stream.get()));
+    if (B.bval.get()) {
+        B.bval32.alloc(B.bval.size());
+        NLHIP_CHECK(cast_f64_f32(B.bval.size(), B.bval.get(), B.bval32.get(), B.stream.get()));
+    }
 }
 
 void fill_stats(Batch& B, const std::vector<int32_t>& its, const std::vector<double>& rel,
@@ -300,6 +316,79 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
     }
 }
 
+// 2x2 block form per system (see BlockStreamEngine): system s takes it when
+// it has an even number of rows and every row pair (row_off[s] + 2p, +1) of
+// it has the same even-length column list made of pairs (c, c + 1) with
+// c - row_off[s] even.  Whether a system does depends only on that system
+// (not on its offset in the batch), so its tiles, and its result, are the
+// same in any batch.  Built on the host (the arrays are host arrays here);
+// block columns as 16-bit offsets when the whole block stream fits them,
+// 32-bit otherwise (same sums either way).
+void build_batch_blocks(Batch& B, const uint32_t* row_off, const uint32_t* rowptr,
+                        const uint32_t* colind, const double* val, std::vector<int32_t>& blk) {
+    const uint32_t nsys = B.nsys;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t s = 0; s < int64_t(nsys); ++s) {
+        const uint32_t lo = row_off[s], hi = row_off[s + 1];
+        bool ok = ((hi - lo) % 2 == 0);
+        for (uint32_t i = lo; ok && i < hi; i += 2) {
+            const uint32_t b0 = rowptr[i], b1 = rowptr[i + 1], e1 = rowptr[i + 2];
+            const uint32_t len = b1 - b0;
+            ok = (e1 - b1 == len) && (len % 2 == 0);
+            for (uint32_t q = 0; ok && q < len; q += 2) {
+                const uint32_t c = colind[b0 + q];
+                ok = (c - lo) % 2 == 0 && colind[b0 + q + 1] == c + 1 && colind[b1 + q] == c &&
+                     colind[b1 + q + 1] == c + 1;
+            }
+        }
+        blk[size_t(s)] = ok ? 1 : 0;
+    }
+    std::vector<uint32_t> pair0(nsys, kNoBlocks);
+    uint32_t np = 0;
+    for (uint32_t s = 0; s < nsys; ++s)
+        if (blk[s]) {
+            pair0[s] = np;
+            np += (row_off[s + 1] - row_off[s]) / 2;
+        }
+    if (np == 0) return;
+    std::vector<uint32_t> h_brp(size_t(np) + 1, 0);
+    for (uint32_t s = 0; s < nsys; ++s)
+        if (blk[s])
+            for (uint32_t i = row_off[s], B = pair0[s]; i < row_off[s + 1]; i += 2, ++B)
+                h_brp[B + 1] = (rowptr[i + 1] - rowptr[i]) / 2;
+    for (uint32_t I = 0; I < np; ++I) h_brp[I + 1] += h_brp[I];
+    const uint64_t nblk = h_brp[np];
+    std::vector<uint32_t> h_bcol(nblk);
+    std::vector<double> h_bval(4 * nblk);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t s = 0; s < int64_t(nsys); ++s) {
+        if (!blk[size_t(s)]) continue;
+        const uint32_t lo = row_off[s], p0 = pair0[size_t(s)];
+        for (uint32_t i = lo, Bk = p0; i < row_off[s + 1]; i += 2, ++Bk) {
+            const uint32_t b0 = rowptr[i], b1 = rowptr[i + 1], len = b1 - b0;
+            uint64_t k = h_brp[Bk];
+            for (uint32_t q = 0; q < len; q += 2, ++k) {
+                h_bcol[k] = p0 + (colind[b0 + q] - lo) / 2;
+                h_bval[4 * k + 0] = val[b0 + q];
+                h_bval[4 * k + 1] = val[b0 + q + 1];
+                h_bval[4 * k + 2] = val[b1 + q];
+                h_bval[4 * k + 3] = val[b1 + q + 1];
+            }
+        }
+    }
+    hipStream_t st = B.stream.get();
+    B.sys_pair0.alloc(nsys);
+    B.brp.alloc(size_t(np) + 1);
+    B.bcol.alloc(nblk);
+    B.bval.alloc(4 * nblk);
+    copy_in(B.sys_pair0.get(), pair0.data(), size_t(nsys) * 4, st);
+    copy_in(B.brp.get(), h_brp.data(), h_brp.size() * 4, st);
+    copy_in(B.bcol.get(), h_bcol.data(), nblk * 4, st);
+    copy_in(B.bval.get(), h_bval.data(), nblk * 32, st);
+    if (build_col_offsets(nblk, B.bcol.get(), st, B.bci16, B.bsegb)) B.bcol.release();
+    NLHIP_CHECK(hipStreamSynchronize(st));  // host staging arrays go out of scope
+}
+
 Batch* batch_create(uint32_t nsys, const uint32_t* row_off, const uint32_t* rowptr,
                     const uint32_t* colind, const double* val, uint32_t flags) {
     if (nsys == 0 || !row_off || !rowptr) throw Error(NLHIP_ERR_INVALID, "empty batch");
@@ -331,37 +420,42 @@ Batch* batch_create(uint32_t nsys, const uint32_t* row_off, const uint32_t* rowp
     B->n = n;
     B->nnz = nnz;
     B->h_row_off.assign(row_off, row_off + nsys + 1);
-    std::vector<uint32_t> tile_off(size_t(nsys) + 1, 0), tile_sys, tile_row0;
-    for (uint32_t s = 0; s < nsys; ++s) {
-        const uint32_t rows = row_off[s + 1] - row_off[s];
-        const uint32_t nt = (rows + kTileRows - 1) / kTileRows;
-        tile_off[s + 1] = tile_off[s] + nt;
-        for (uint32_t t = 0; t < nt; ++t) {
-            tile_sys.push_back(s);
-            tile_row0.push_back(row_off[s] + t * kTileRows);
-        }
-    }
-    B->ntiles = tile_off[nsys];
     hipStream_t st = B->stream.get();
     B->row_off.alloc(size_t(nsys) + 1);
-    B->tile_off.alloc(size_t(nsys) + 1);
-    B->tile_sys.alloc(B->ntiles);
-    B->tile_row0.alloc(B->ntiles);
     B->rowptr.alloc(size_t(n) + 1);
     B->colind.alloc(nnz);
     B->val.alloc(nnz);
     B->dinv.alloc(n);
     copy_in(B->row_off.get(), row_off, (size_t(nsys) + 1) * 4, st);
-    copy_in(B->tile_off.get(), tile_off.data(), tile_off.size() * 4, st);
-    copy_in(B->tile_sys.get(), tile_sys.data(), tile_sys.size() * 4, st);
-    copy_in(B->tile_row0.get(), tile_row0.data(), tile_row0.size() * 4, st);
     copy_in(B->rowptr.get(), rowptr, (size_t(n) + 1) * 4, st);
     copy_in(B->colind.get(), colind, nnz * 4, st);
     copy_in(B->val.get(), val, nnz * 8, st);
     NLHIP_CHECK(jacobi_dinv<double>(n, B->rowptr.get(), B->colind.get(), B->val.get(),
                                     B->dinv.get(), st));
-    if (!(flags & NLHIP_SYSTEM_NO_OFFSETS))
+    std::vector<int32_t> blk(nsys, 0);
+    if (!(flags & NLHIP_SYSTEM_NO_OFFSETS)) {
         build_col_offsets(nnz, B->colind.get(), st, B->ci16, B->segb);
+        if (!(flags & NLHIP_SYSTEM_NO_BLOCKS))
+            build_batch_blocks(*B, row_off, rowptr, colind, val, blk);
+    }
+    std::vector<uint32_t> tile_off(size_t(nsys) + 1, 0), tile_sys, tile_row0;
+    for (uint32_t s = 0; s < nsys; ++s) {
+        const uint32_t rows = row_off[s + 1] - row_off[s];
+        const uint32_t TR = blk[s] ? kBlockTileRows : kTileRows;
+        const uint32_t nt = (rows + TR - 1) / TR;
+        tile_off[s + 1] = tile_off[s] + nt;
+        for (uint32_t t = 0; t < nt; ++t) {
+            tile_sys.push_back(s);
+            tile_row0.push_back(row_off[s] + t * TR);
+        }
+    }
+    B->ntiles = tile_off[nsys];
+    B->tile_off.alloc(size_t(nsys) + 1);
+    B->tile_sys.alloc(B->ntiles);
+    B->tile_row0.alloc(B->ntiles);
+    copy_in(B->tile_off.get(), tile_off.data(), tile_off.size() * 4, st);
+    copy_in(B->tile_sys.get(), tile_sys.data(), tile_sys.size() * 4, st);
+    copy_in(B->tile_row0.get(), tile_row0.data(), tile_row0.size() * 4, st);
     B->grid = uint32_t(std::max(4, device().cu_count * 4));
     if (flags & NLHIP_SYSTEM_FP32) ensure_fp32(*B);
     work_alloc(*B, B->w64);

@@ -392,15 +392,19 @@ struct System {
             v = val32.get();
             bv = bval32.get();
         }
-        if (blocks() && (rpw == 64 || rpw == 73))
+        if (blocks() && (rpw == 73 || (rpw == 64 && n >= kBlockMinRows)))
             return SpmvOps<T>{brp.get(), Cols{nullptr, bci16.get(), bsegb.get()}, bv, 73};
         const int code = rpw == 73 ? 64 : rpw;
         return SpmvOps<T>{rowptr.get(), cols(), v, engine_rpw(code, cols())};
     }
 };
 
-// Builds the 2x2 block form of S (brp, bci16, bsegb, bval) when the matrix
-// has it (see BlockStreamEngine) and the block columns fit 16-bit offsets.
+// Builds the 2x2 block form (brp, bci16, bsegb, bval; see
+// BlockStreamEngine) of a device CRS matrix when it has that form and its
+// block columns fit 16-bit offsets; false otherwise.
+bool build_block_form(uint32_t n, uint64_t nnz, const uint32_t* rowptr, const uint32_t* colind,
+                      const double* val, hipStream_t s, DevBuf<uint32_t>& brp,
+                      DevBuf<uint16_t>& bci16, DevBuf<uint32_t>& bsegb, DevBuf<double>& bval);
 bool build_blocks(System& S);
 
 // Builds the 16-bit column offsets of a device colind (see Cols); false when

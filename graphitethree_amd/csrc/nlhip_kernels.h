@@ -39,6 +39,10 @@ inline bool valid_rpw(int rpw) {
     return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 73);
 }
 
+// Smallest system (rows) on which code 64 runs the 2x2 block engine: below
+// it the scalar engine's shorter chunk chain per wavefront is faster.
+constexpr uint32_t kBlockMinRows = 131072;
+
 // Default stream engines: with and without 16-bit column offsets (each wave
 // streams one contiguous run of tiles; the XCD-interleaved runs of 69/70
 // gather less but are no faster on large systems and slower on small ones).
@@ -106,6 +110,10 @@ hipError_t blas_ddot(uint64_t n, const double* x, const double* y, double* part,
 // out[0] = sum(part[0..P)), out[1] = sqrt(out[0]).
 hipError_t reduce_final(const double* part, uint32_t P, double* out, hipStream_t s);
 
+// The SpMV step of the fused PCG runs on up to this many times the
+// workgroups of the other launches (more waves in flight for its gathers).
+constexpr uint32_t kSpmvGridMul = 2;
+
 // Fused PCG.  All arrays are device memory.
 template <typename T>
 struct CGArgs {
@@ -113,7 +121,7 @@ struct CGArgs {
     uint32_t chunk;  // rows per workgroup of the element-wise launches
     uint32_t sp;     // SpMV work per workgroup of the init launch (P workgroups)
     uint32_t P;      // workgroups per launch = number of partial sums
-    uint32_t Ps;     // workgroups of the SpMV step (d.q partials), Ps <= 2 P
+    uint32_t Ps;     // workgroups of the SpMV step (d.q partials), Ps <= kSpmvGridMul P
     uint32_t sps;    // SpMV work per workgroup of the SpMV step
     const uint32_t* rp;
     Cols cols;
@@ -144,7 +152,9 @@ hipError_t cg_step_part(const CGArgs<T>& a, int rpw, bool precond, int part, hip
 
 // ---- batched systems (k_batch.hip) ----------------------------------------
 
-constexpr uint32_t kTileRows = 256;  // rows per batch tile: 4 wavefronts x 64
+constexpr uint32_t kTileRows = 256;  // rows per batch tile: 4 wavefronts x 64 rows
+constexpr uint32_t kBlockTileRows = 512;  // with 2x2 blocks: 4 wavefronts x 64 block rows
+constexpr uint32_t kNoBlocks = 0xffffffffu;
 
 template <typename T>
 struct BatchArgs {
@@ -158,6 +168,19 @@ struct BatchArgs {
     const uint32_t* ci;         // global column indices
     const uint16_t* ci16;       // their 16-bit segment offsets, or null
     const uint32_t* segb;       // segment bases of ci16
+    // 2x2 block form of the systems that have it (BlockStreamEngine):
+    // sys_pair0[s] = first block row of system s (kNoBlocks: scalar form;
+    // the array is null when no system has the form); the block rows of s
+    // are its row pairs (row_off[s] + 2p, +1), i.e. block row B holds rows
+    // 2B + delta_s with delta_s = row_off[s] - 2 sys_pair0[s]; brp over all
+    // block rows; block columns as 16-bit offsets (bci16, bsegb) or, when
+    // they do not fit, 32-bit (bcol)
+    const uint32_t* sys_pair0;
+    const uint32_t* brp;
+    const uint32_t* bcol;
+    const uint16_t* bci16;
+    const uint32_t* bsegb;
+    const T* bval;
     const T* val;
     const T* dinv;              // null: no preconditioner
     T* x;

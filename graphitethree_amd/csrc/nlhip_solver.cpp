@@ -12,14 +12,17 @@ namespace nlhip {
 
 // ----------------------------------------------------------------- launch
 
-// Stream engine (64 rows per wavefront); one 64-row tile per wavefront at
-// least, at most 4 workgroups per CU (which also bounds the number of
-// per-workgroup partial sums every CG launch reduces).
+// Stream engine (code 64: the 2x2 block engine on block-form matrices of at
+// least kBlockMinRows rows, the scalar one otherwise); rows per workgroup
+// chosen by size (measured on LSCM systems of 300 to 10 M rows: small,
+// latency-bound systems run fastest with more, emptier workgroups, large
+// ones with fewer, fuller ones), at most 4 workgroups per CU (which also
+// bounds the number of per-workgroup partial sums every CG launch reduces).
 void default_launch(uint32_t n, uint64_t nnz, int& rpw, uint32_t& grid) {
     (void)nnz;
     rpw = 64;
-    const uint64_t tiles = (uint64_t(n) + 63) / 64;
-    const uint64_t g = (tiles + 3) / 4;  // 4 wavefronts per workgroup
+    const uint64_t per = n < 32768u ? 128 : (n < kBlockMinRows ? 256 : 512);
+    const uint64_t g = (uint64_t(n) + per - 1) / per;
     uint32_t cap = 1024;
     if (device_usable()) cap = uint32_t(std::max(4, device().cu_count * 4));
     grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
@@ -69,30 +72,36 @@ bool build_col_offsets(uint64_t nnz, const uint32_t* colind, hipStream_t s,
     return true;
 }
 
-bool build_blocks(System& S) {
-    if (S.n < 2 || (S.n & 1u) || S.nnz == 0 || (S.nnz & 3u)) return false;
-    const uint32_t nb = S.n / 2;
-    const uint64_t nblk = S.nnz / 4;
-    hipStream_t s = S.stream.get();
+bool build_block_form(uint32_t n, uint64_t nnz, const uint32_t* rowptr, const uint32_t* colind,
+                      const double* val, hipStream_t s, DevBuf<uint32_t>& brp_out,
+                      DevBuf<uint16_t>& bci16_out, DevBuf<uint32_t>& bsegb_out,
+                      DevBuf<double>& bval_out) {
+    if (n < 2 || (n & 1u) || nnz == 0 || (nnz & 3u)) return false;
+    const uint32_t nb = n / 2;
+    const uint64_t nblk = nnz / 4;
     DevBuf<uint32_t> bad(1);
     NLHIP_CHECK(fill<int32_t>(1, 0, reinterpret_cast<int32_t*>(bad.get()), s));
-    NLHIP_CHECK(block_check(nb, S.rowptr.get(), S.colind.get(), bad.get(), s));
+    NLHIP_CHECK(block_check(nb, rowptr, colind, bad.get(), s));
     uint32_t h_bad = 1;
     NLHIP_CHECK(hipMemcpyAsync(&h_bad, bad.get(), 4, hipMemcpyDeviceToHost, s));
     NLHIP_CHECK(hipStreamSynchronize(s));
     if (h_bad) return false;
     DevBuf<uint32_t> brp(size_t(nb) + 1), bcol(nblk);
-    DevBuf<double> bval(S.nnz);
-    NLHIP_CHECK(block_pack<double>(nb, S.rowptr.get(), S.colind.get(), S.val.get(), brp.get(),
-                                   bcol.get(), bval.get(), s));
+    DevBuf<double> bval(nnz);
+    NLHIP_CHECK(block_pack<double>(nb, rowptr, colind, val, brp.get(), bcol.get(), bval.get(), s));
     DevBuf<uint16_t> bci16;
     DevBuf<uint32_t> bsegb;
     if (!build_col_offsets(nblk, bcol.get(), s, bci16, bsegb)) return false;
-    S.brp = std::move(brp);
-    S.bval = std::move(bval);
-    S.bci16 = std::move(bci16);
-    S.bsegb = std::move(bsegb);
+    brp_out = std::move(brp);
+    bval_out = std::move(bval);
+    bci16_out = std::move(bci16);
+    bsegb_out = std::move(bsegb);
     return true;
+}
+
+bool build_blocks(System& S) {
+    return build_block_form(S.n, S.nnz, S.rowptr.get(), S.colind.get(), S.val.get(),
+                            S.stream.get(), S.brp, S.bci16, S.bsegb, S.bval);
 }
 
 System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
@@ -184,7 +193,7 @@ void CGWorkspace<T>::ensure(uint32_t n_, uint32_t P_) {
     h.alloc(n);
     d.alloc(n);
     q.alloc(n);
-    part.alloc(size_t(5) * P);  // d.q (up to 2 P), r.h, r.r, b.b
+    part.alloc(size_t(kSpmvGridMul + 3) * P);  // d.q (up to kSpmvGridMul P), r.h, r.r, b.b
     st.alloc(1);
     st_host.ensure(2);
 }
@@ -200,12 +209,13 @@ static void setup_args(System& S, CGWorkspace<T>& ws, const T* dinv) {
     a.P = S.grid;
     a.chunk = uint32_t((uint64_t(S.n) + S.grid - 1) / S.grid);
     a.sp = spmv_work_per_block(S.n, S.grid, o.code);
-    // the streamed SpMV step runs with up to twice the workgroups (at least
-    // four 64-row tiles each)
-    const uint64_t quads = (uint64_t(S.n) + 255) / 256;
-    a.Ps = o.code >= 64
-               ? uint32_t(std::max<uint64_t>(S.grid, std::min<uint64_t>(2ull * S.grid, quads)))
-               : S.grid;
+    // the streamed SpMV step runs with up to kSpmvGridMul times the
+    // workgroups (at least four wavefront tiles each)
+    const uint64_t tile_rows = o.code == 73 ? 128 : 64;
+    const uint64_t quads = (uint64_t(S.n) + 4 * tile_rows - 1) / (4 * tile_rows);
+    a.Ps = o.code >= 64 ? uint32_t(std::max<uint64_t>(
+                              S.grid, std::min<uint64_t>(uint64_t(kSpmvGridMul) * S.grid, quads)))
+                        : S.grid;
     a.sps = spmv_work_per_block(S.n, a.Ps, o.code);
     a.rp = o.rp;
     a.cols = o.cols;
@@ -217,10 +227,10 @@ static void setup_args(System& S, CGWorkspace<T>& ws, const T* dinv) {
     a.h = ws.h.get();
     a.d = ws.d.get();
     a.q = ws.q.get();
-    a.part_dq = ws.part.get();                       // Ps <= 2 P entries
-    a.part_rh = ws.part.get() + 2 * size_t(S.grid);
-    a.part_rr = ws.part.get() + 3 * size_t(S.grid);
-    a.part_bb = ws.part.get() + 4 * size_t(S.grid);
+    a.part_dq = ws.part.get();  // Ps <= kSpmvGridMul P entries
+    a.part_rh = ws.part.get() + size_t(kSpmvGridMul) * S.grid;
+    a.part_rr = ws.part.get() + size_t(kSpmvGridMul + 1) * S.grid;
+    a.part_bb = ws.part.get() + size_t(kSpmvGridMul + 2) * S.grid;
     a.st = ws.st.get();
 }
 

@@ -63,6 +63,31 @@ def test_batch_alone_equals_in_mix(hip, small_atlas):
     assert np.array_equal(parts[0], xs[11]) and np.array_equal(parts[2], xs[0])
 
 
+def test_batch_mixed_forms_at_any_offset(hip, small_atlas):
+    """Charts in 2x2 block form keep their bits when packed after a system
+    with an odd number of rows (their rows then start at odd offsets) and
+    next to systems without the form; the scalar system keeps its bits too;
+    every system matches the oracle."""
+    A = small_atlas
+    odd = W.cotan_system(W.grid_mesh(9, seed=3), nrhs=1)          # n = 81
+    assert odd.n % 2 == 1
+    charts = [A.chart(c) for c in (2, 7)]
+    mix = [odd, charts[0], odd, charts[1]]
+    B = capi.Batch(mix)
+    xb, pb, _ = B.solve(np.concatenate([S.b for S in mix]))
+    parts = B.split(xb)
+    for S, xp in zip(mix, parts):
+        alone = capi.Batch([S])
+        xa, _, _ = alone.solve(S.b)
+        assert np.array_equal(xa, xp)
+        xo, _, _ = O.pcg(O.CRS(S.rowptr, S.colind, S.val), S.b, None, 1e-10, 50000)
+        assert rel(xp, xo) <= REL_DIFF
+    Bm = capi.Batch(mix, fp32=True)
+    xm, pm, _ = Bm.solve(np.concatenate([S.b for S in mix]), precision="mixed", eps=1e-10,
+                         max_iter=200000, inner_eps=1e-4)
+    assert all(p.rel_residual <= 1.01e-10 for p in pm)
+
+
 def test_batch_launch_shape_does_not_change_bits(hip, small_atlas):
     A = small_atlas
     B = capi.Batch([A.chart(c) for c in range(A.count)])

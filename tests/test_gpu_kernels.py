@@ -144,6 +144,8 @@ def test_block_form_solves_like_scalar_form(hip):
     xo, ito, _ = O.pcg(A, Sw.b, None, 1e-10, 20000)
     for blocks in (True, False):
         S = capi.System.from_crs(Sw, blocks=blocks, fp32=True)
+        if blocks:  # small systems default to the scalar engine
+            S.set_launch(73, S.launch()[1])
         x, st = S.solve(Sw.b, eps=1e-10, max_iter=20000)
         assert st.converged and st.rel_residual <= 1e-10
         assert abs(int(st.iterations) - ito) <= max(2, ito // 20)
