@@ -756,6 +756,20 @@ bool solve_context(Context& C) {
             copy_out(x.data(), S->ws64.x.get(), size_t(n) * 8, s);
             S->stream.sync();
             C.error = std::sqrt(rel2);
+        } else if (solver == NL_GMRES) {
+            std::lock_guard<std::mutex> lock(S->mu);
+            S->ws64.ensure(n, S->grid);
+            hipStream_t s = S->stream.get();
+            copy_in(S->ws64.b.get(), b.data(), size_t(n) * 8, s);
+            copy_in(S->ws64.x.get(), x.data(), size_t(n) * 8, s);
+            double rel2 = 0.0;
+            const double* dinv = (C.preconditioner == NL_PRECOND_JACOBI) ? S->dinv.get() : nullptr;
+            C.used_iterations += gmres_device(*S, dinv, S->ws64.b.get(), S->ws64.x.get(),
+                                              C.threshold, max_iter,
+                                              C.inner_iterations ? C.inner_iterations : 5, &rel2);
+            copy_out(x.data(), S->ws64.x.get(), size_t(n) * 8, s);
+            S->stream.sync();
+            C.error = std::sqrt(rel2);
         } else {
             throw Error(NLHIP_ERR_UNSUPPORTED, "nlSolve: solver not available on the GPU");
         }
@@ -908,7 +922,6 @@ NLAPI NLuint NLAPIENTRY nlSolveSystemIterative(NLBlas_t blas, NLMatrix M, NLMatr
                                                NLdouble* b, NLdouble* x, NLenum solver,
                                                double eps, NLuint max_iter, NLuint inner_iter) {
     NLuint its = 0;
-    (void)inner_iter;
     guarded("nlSolveSystemIterative", [&] {
         if (!b || !x) throw Error(NLHIP_ERR_INVALID, "null b or x");
         System* S = system_of(M);
@@ -947,8 +960,21 @@ NLAPI NLuint NLAPIENTRY nlSolveSystemIterative(NLBlas_t blas, NLMatrix M, NLMatr
             copy_out(x, S->ws64.x.get(), size_t(S->n) * 8, s);
             S->stream.sync();
             if (blas) blas->flops += NLulong(its) * (4 * S->nnz + 20 * NLulong(S->n));
+        } else if (solver == NL_GMRES) {
+            std::lock_guard<std::mutex> lock(S->mu);
+            S->ws64.ensure(S->n, S->grid);
+            hipStream_t s = S->stream.get();
+            copy_in(S->ws64.b.get(), b, size_t(S->n) * 8, s);
+            copy_in(S->ws64.x.get(), x, size_t(S->n) * 8, s);
+            const NLuint m = inner_iter ? inner_iter : 5;  // matrix.cpp:385 passes 5
+            its = gmres_device(*S, dinv, S->ws64.b.get(), S->ws64.x.get(), eps, max_iter, m,
+                               nullptr);
+            copy_out(x, S->ws64.x.get(), size_t(S->n) * 8, s);
+            S->stream.sync();
+            if (blas) blas->flops += NLulong(its) * (2 * S->nnz + NLulong(2 * m + 8) * S->n);
         } else {
-            throw Error(NLHIP_ERR_UNSUPPORTED, "solver not available on the GPU (CG, BiCGSTAB)");
+            throw Error(NLHIP_ERR_UNSUPPORTED,
+                        "solver not available on the GPU (CG, BiCGSTAB, GMRES)");
         }
     });
     return its;

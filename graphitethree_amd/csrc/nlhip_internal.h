@@ -436,5 +436,9 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X,
 // Device BiCGSTAB on BLAS-1 kernels (NL_BICGSTAB); returns iterations.
 uint32_t bicgstab_device(System& S, const double* dinv, const double* b_dev, double* x_dev,
                          double eps, uint32_t max_iter, double* rel2);
+// Restarted GMRES(m) on the device (right-preconditioned with dinv when not
+// null); returns the number of Arnoldi steps.
+uint32_t gmres_device(System& S, const double* dinv, const double* b_dev, double* x_dev,
+                      double eps, uint32_t max_iter, uint32_t m, double* rel2);
 
 }  // namespace nlhip

@@ -482,6 +482,101 @@ uint32_t bicgstab_device(System& S, const double* dinv, const double* b, double*
     return its;
 }
 
+// Restarted GMRES(m) (Saad & Schultz 1986), modified Gram-Schmidt Arnoldi,
+// Givens rotations, right preconditioning with dinv (null = none).  Same
+// method and stopping rule as the oracle's or_gmres; each inner product is
+// reduced on the device and read back (a host-driven loop, like BiCGSTAB).
+uint32_t gmres_device(System& S, const double* dinv, const double* b, double* x, double eps,
+                      uint32_t max_iter, uint32_t m, double* rel2) {
+    if (m < 1) m = 1;
+    const uint32_t n = S.n;
+    hipStream_t s = S.stream.get();
+    DevBuf<double> Vb(size_t(m + 1) * n), buf(size_t(3) * n);
+    double* V = Vb.get();
+    double *w = buf.get(), *z = w + n, *u = z + n;
+    DevBuf<double> part(S.grid), out(2);
+    S.host_scalars.ensure(4);
+    DotHelper dot{S, part, out.get(), S.host_scalars.get()};
+    const SpmvOps<double> A = S.ops<double>();
+    auto spmv64 = [&](const double* in, double* o) {
+        NLHIP_CHECK(spmv<double>(n, A.rp, A.cols, A.val, in, o, A.code, S.grid, s));
+    };
+    auto precond = [&](const double* in, double* o) {
+        if (dinv) NLHIP_CHECK(blas_dmul(n, dinv, in, o, s));
+        else NLHIP_CHECK(hipMemcpyAsync(o, in, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+    };
+    auto residual = [&] {  // V_0 = b - A x
+        spmv64(x, V);
+        NLHIP_CHECK(blas_dscal(n, -1.0, V, s));
+        NLHIP_CHECK(blas_daxpy(n, 1.0, b, V, s));
+    };
+    std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), g(size_t(m) + 1), y(m);
+    const double bb = dot(b, b);
+    const double thresh = (eps * eps) * bb;
+    residual();
+    double err = dot(V, V);
+    uint32_t its = 0;
+    while (err > thresh && its < max_iter) {
+        const double beta = std::sqrt(err);
+        NLHIP_CHECK(blas_dscal(n, 1.0 / beta, V, s));
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        uint32_t k = 0;
+        bool breakdown = false;
+        for (uint32_t j = 0; j < m && its < max_iter; ++j) {
+            double* vj = V + size_t(j) * n;
+            precond(vj, z);
+            spmv64(z, w);
+            for (uint32_t i = 0; i <= j; ++i) {
+                const double h = dot(w, V + size_t(i) * n);
+                H[size_t(i) * m + j] = h;
+                NLHIP_CHECK(blas_daxpy(n, -h, V + size_t(i) * n, w, s));
+            }
+            const double hn = std::sqrt(dot(w, w));
+            H[size_t(j + 1) * m + j] = hn;
+            if (hn != 0.0) {
+                double* vn = V + size_t(j + 1) * n;
+                NLHIP_CHECK(hipMemcpyAsync(vn, w, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+                NLHIP_CHECK(blas_dscal(n, 1.0 / hn, vn, s));
+            } else {
+                breakdown = true;
+            }
+            for (uint32_t i = 0; i < j; ++i) {
+                const double h0 = H[size_t(i) * m + j], h1 = H[size_t(i + 1) * m + j];
+                H[size_t(i) * m + j] = cs[i] * h0 + sn[i] * h1;
+                H[size_t(i + 1) * m + j] = -sn[i] * h0 + cs[i] * h1;
+            }
+            const double h0 = H[size_t(j) * m + j], h1 = H[size_t(j + 1) * m + j];
+            const double den = std::hypot(h0, h1);
+            cs[j] = den != 0.0 ? h0 / den : 1.0;
+            sn[j] = den != 0.0 ? h1 / den : 0.0;
+            H[size_t(j) * m + j] = den;
+            H[size_t(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            ++its;
+            k = j + 1;
+            if (g[j + 1] * g[j + 1] <= thresh || breakdown) break;
+        }
+        for (int i = int(k) - 1; i >= 0; --i) {
+            double t = g[size_t(i)];
+            for (uint32_t l = uint32_t(i) + 1; l < k; ++l) t -= H[size_t(i) * m + l] * y[l];
+            y[size_t(i)] = H[size_t(i) * m + size_t(i)] != 0.0 ? t / H[size_t(i) * m + size_t(i)]
+                                                                : 0.0;
+        }
+        NLHIP_CHECK(fill<double>(n, 0.0, u, s));
+        for (uint32_t i = 0; i < k; ++i) NLHIP_CHECK(blas_daxpy(n, y[i], V + size_t(i) * n, u, s));
+        precond(u, z);
+        NLHIP_CHECK(blas_daxpy(n, 1.0, z, x, s));
+        residual();
+        err = dot(V, V);
+        if (breakdown) break;
+    }
+    S.stream.sync();
+    if (rel2) *rel2 = bb > 0.0 ? err / bb : err;
+    return its;
+}
+
 }  // namespace nlhip
 
 using namespace nlhip;

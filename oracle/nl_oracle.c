@@ -14,6 +14,9 @@
  *                           termination, b = 0 -> 0 iterations, scipy solve
  *   or_bicgstab             linear_algebra.lua KAT ([[1,2],[3,4]] ->
  *                           [-4,4.5]), scipy solve on nonsymmetric systems
+ *   or_gmres                numpy solve on nonsymmetric systems, finite
+ *                           termination with m = n, non-increasing true
+ *                           residual over restarts, scipy's gmres solution
  *   or_dyn_*                duplicate accumulation, scipy COO->CSR
  *   or_ctx_*                dense A^T A from numpy, flat-grid LSCM closed form
  */
@@ -320,6 +323,94 @@ int or_bicgstab(uint32_t n, const uint32_t* rp, const uint32_t* ci,
     }
     if (out_rel2) *out_rel2 = (bb > 0.0) ? err / bb : err;
     free(r); free(rt); free(p); free(v); free(ph); free(s); free(sh); free(t);
+    return its;
+}
+
+/* ---------------------------------------------------------------- GMRES */
+
+/* Restarted GMRES(m) (Saad & Schultz 1986; Saad, "Iterative Methods for
+ * Sparse Linear Systems", 2nd ed., Alg. 6.9 with Givens rotations as in
+ * sec. 6.5.3), modified Gram-Schmidt Arnoldi, right preconditioning with
+ * dinv (x = D^-1 u, A D^-1 u = b).  Stops when the Arnoldi residual
+ * estimate |g_{j+1}|^2 <= eps^2 b.b, after max_iter Arnoldi steps in all, or
+ * on a breakdown; a restart recomputes the true residual b - A x. */
+int or_gmres(uint32_t n, const uint32_t* rp, const uint32_t* ci,
+             const double* a, const double* dinv, const double* b, double* x,
+             double eps, int max_iter, int m, double* out_rel2) {
+    if (m < 1) m = 1;
+    size_t sz = sizeof(double) * (size_t)n;
+    double* V = malloc(sz * (size_t)(m + 1));
+    double *w = malloc(sz), *z = malloc(sz), *u = malloc(sz);
+    double* H = calloc((size_t)(m + 1) * (size_t)m, sizeof(double)); /* H[i*m + j] */
+    double *cs = malloc(sizeof(double) * (size_t)m), *sn = malloc(sizeof(double) * (size_t)m);
+    double *g = malloc(sizeof(double) * (size_t)(m + 1)), *y = malloc(sizeof(double) * (size_t)m);
+    double bb = or_ddot(n, b, b);
+    double thresh = (eps * eps) * bb;
+    /* r = b - A x in V_0 */
+    or_spmv(n, rp, ci, a, x, V);
+    or_dscal(n, -1.0, V);
+    or_daxpy(n, 1.0, b, V);
+    double err = or_ddot(n, V, V);
+    int its = 0;
+    while (err > thresh && its < max_iter) {
+        double beta = sqrt(err);
+        or_dscal(n, 1.0 / beta, V);
+        memset(g, 0, sizeof(double) * (size_t)(m + 1));
+        g[0] = beta;
+        int k = 0;
+        int breakdown = 0;
+        for (int j = 0; j < m && its < max_iter; ++j) {
+            double* vj = V + (size_t)j * n;
+            precond(n, dinv, vj, z, 0);
+            or_spmv(n, rp, ci, a, z, w);
+            for (int i = 0; i <= j; ++i) {
+                double h = or_ddot(n, w, V + (size_t)i * n);
+                H[i * m + j] = h;
+                or_daxpy(n, -h, V + (size_t)i * n, w);
+            }
+            double hn = sqrt(or_ddot(n, w, w));
+            H[(j + 1) * m + j] = hn;
+            if (hn != 0.0) {
+                or_dcopy(n, w, V + (size_t)(j + 1) * n);
+                or_dscal(n, 1.0 / hn, V + (size_t)(j + 1) * n);
+            } else {
+                breakdown = 1;
+            }
+            for (int i = 0; i < j; ++i) {  /* earlier rotations on column j */
+                double h0 = H[i * m + j], h1 = H[(i + 1) * m + j];
+                H[i * m + j] = cs[i] * h0 + sn[i] * h1;
+                H[(i + 1) * m + j] = -sn[i] * h0 + cs[i] * h1;
+            }
+            double h0 = H[j * m + j], h1 = H[(j + 1) * m + j];
+            double den = hypot(h0, h1);
+            cs[j] = den != 0.0 ? h0 / den : 1.0;
+            sn[j] = den != 0.0 ? h1 / den : 0.0;
+            H[j * m + j] = den;
+            H[(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            ++its;
+            k = j + 1;
+            if (g[j + 1] * g[j + 1] <= thresh || breakdown) break;
+        }
+        /* y = H[0:k,0:k]^-1 g[0:k]; x += D^-1 V y */
+        for (int i = k - 1; i >= 0; --i) {
+            double t = g[i];
+            for (int l = i + 1; l < k; ++l) t -= H[i * m + l] * y[l];
+            y[i] = H[i * m + i] != 0.0 ? t / H[i * m + i] : 0.0;
+        }
+        memset(u, 0, sz);
+        for (int i = 0; i < k; ++i) or_daxpy(n, y[i], V + (size_t)i * n, u);
+        precond(n, dinv, u, z, 0);
+        or_daxpy(n, 1.0, z, x);
+        or_spmv(n, rp, ci, a, x, V);
+        or_dscal(n, -1.0, V);
+        or_daxpy(n, 1.0, b, V);
+        err = or_ddot(n, V, V);
+        if (breakdown) break;
+    }
+    if (out_rel2) *out_rel2 = (bb > 0.0) ? err / bb : err;
+    free(V); free(w); free(z); free(u); free(H); free(cs); free(sn); free(g); free(y);
     return its;
 }
 

@@ -71,6 +71,15 @@ int or_bicgstab(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
                 const double* val, const double* dinv, const double* b,
                 double* x, double eps, int max_iter, double* out_rel2);
 
+/* ---- GMRES(m) (NL_GMRES, matrix.cpp:377-379, with inner_iter = m = 5 at
+ * matrix.cpp:385): restarted, modified Gram-Schmidt, right-preconditioned
+ * with dinv (NULL = none); returns the number of Arnoldi steps.  Upstream's
+ * own GMRES (geogram, absent here) is not available to compare with: this
+ * is the textbook method. */
+int or_gmres(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
+             const double* val, const double* dinv, const double* b, double* x,
+             double eps, int max_iter, int m, double* out_rel2);
+
 /* ---- Dynamic matrix: duplicate (i,j) entries accumulate
  * (scene_graph/NL/matrix.h:111-112,122-123), then compression to CRS with
  * each row's columns in increasing order. */

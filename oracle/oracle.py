@@ -52,6 +52,8 @@ def lib() -> C.CDLL:
                                                 C.c_int]),
             "or_bicgstab": (C.c_int, [C.c_uint32, _u32p, _u32p, _f64p, _f64p,
                                       _f64p, _f64p, C.c_double, C.c_int, _f64p]),
+            "or_gmres": (C.c_int, [C.c_uint32, _u32p, _u32p, _f64p, _f64p,
+                                   _f64p, _f64p, C.c_double, C.c_int, C.c_int, _f64p]),
             "or_dyn_new": (C.c_void_p, [C.c_uint32, C.c_uint32]),
             "or_dyn_free": (None, [C.c_void_p]),
             "or_dyn_add": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_double]),
@@ -188,6 +190,18 @@ def bicgstab(A: CRS, b, x0=None, eps=1e-10, max_iter=5000, precond=None):
     it = lib().or_bicgstab(*A.args(), None if dinv is None else _p(dinv, C.c_double),
                            _p(b, C.c_double), _p(x, C.c_double), float(eps),
                            int(max_iter), C.byref(rel2))
+    return x, int(it), float(np.sqrt(rel2.value))
+
+
+def gmres(A: CRS, b, x0=None, eps=1e-10, max_iter=5000, m=5, precond=None):
+    """Restarted GMRES(m).  Returns (x, Arnoldi steps, relative residual)."""
+    b = _f64(b)
+    x = np.zeros(A.m) if x0 is None else _f64(x0).copy()
+    dinv = jacobi(A) if precond == "jacobi" else None
+    rel2 = C.c_double(0.0)
+    it = lib().or_gmres(*A.args(), None if dinv is None else _p(dinv, C.c_double),
+                        _p(b, C.c_double), _p(x, C.c_double), float(eps),
+                        int(max_iter), int(m), C.byref(rel2))
     return x, int(it), float(np.sqrt(rel2.value))
 
 

@@ -179,6 +179,45 @@ def test_solve_system_iterative_host_api(hip):
     L.nlDeleteMatrix(M)
 
 
+def test_gmres_host_api_and_wrapper(hip):
+    """nlSolveSystemIterative(NL_GMRES, inner_iter = m) against the oracle's
+    GMRES(m), with and without Jacobi, on a nonsymmetric system; and the
+    wrapper's solve_iterative(..., "GMRES") on the general lua KAT."""
+    L = hip
+    import scipy.sparse as sp
+    n = 400
+    rng = np.random.default_rng(17)
+    A = sp.random(n, n, density=6.0 / n, random_state=17, format="csr")
+    A = (A + sp.diags(4.0 + rng.random(n))).tocsr()
+    A.sort_indices()
+    rp, ci, va = A.indptr.astype(np.uint32), A.indices.astype(np.uint32), A.data
+    b = rng.standard_normal(n)
+    M = L.nlCRSMatrixNewFromArrays(n, n, capi.ptr(rp), capi.ptr(ci), capi.ptr(va))
+    P = L.nlNewJacobiPreconditioner(M)
+    for prec, pname in ((None, None), (P, "jacobi")):
+        x = np.zeros(n)
+        its = L.nlSolveSystemIterative(L.nlHostBlas(), M, prec, capi.ptr(b), capi.ptr(x),
+                                       capi.NL_GMRES, 1e-11, 5000, 5)
+        xr, itr, relr = O.gmres(O.CRS(rp, ci, va), b, eps=1e-11, max_iter=5000, m=5,
+                                precond=pname)
+        assert its > 0 and same_iters(its, itr), (its, itr)
+        assert np.linalg.norm(A @ x - b) <= 1.01e-11 * np.linalg.norm(b)
+        assert rel(x, xr) <= REL_DIFF
+    L.nlDeleteMatrix(P)
+    L.nlDeleteMatrix(M)
+    k = json.load(open(os.path.join(GOLD, "lua_kats.json")))["general"]
+    NL = nl.Library()
+    B = NL.create_vector(2)
+    B[0], B[1] = k["B"]
+    Mw = NL.create_matrix(2, 2)
+    for i in range(2):
+        for j in range(2):
+            Mw.add_coefficient(i, j, k["M"][i][j])
+    X = nl.Vector(2)
+    Mw.solve_iterative(B, X, solver="GMRES", precond="None", max_iter=1000, eps=1e-12)
+    np.testing.assert_allclose(X.data, k["X"], atol=1e-9)
+
+
 def test_hip_extension_objects(hip):
     L = hip
     Sw = W.lscm_grid(22, seed=3)

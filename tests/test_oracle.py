@@ -102,6 +102,63 @@ def test_lua_kat_bicgstab():
     assert rel <= 1e-3
 
 
+def _nonsymmetric(n, seed):
+    rng = np.random.default_rng(seed)
+    A = sp.random(n, n, density=6.0 / n, random_state=seed, format="csr")
+    A = A + sp.diags(4.0 + rng.random(n))  # diagonally dominant enough, nonsymmetric
+    A.sort_indices()
+    return A.tocsr()
+
+
+def test_gmres_nonsymmetric_matches_direct_solve():
+    A = _nonsymmetric(120, 3)
+    b = np.random.default_rng(4).standard_normal(120)
+    x_ref = np.linalg.solve(A.toarray(), b)
+    C = O.CRS(A.indptr, A.indices, A.data)
+    for precond in (None, "jacobi"):
+        x, it, rel = O.gmres(C, b, eps=1e-12, max_iter=2000, m=5, precond=precond)
+        assert rel <= 1e-12
+        assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-10, precond
+
+
+def test_gmres_finite_termination_and_monotone_restarts():
+    # GMRES(n) on an n x n system terminates in at most n steps
+    n = 10
+    rng = np.random.default_rng(9)
+    Ad = rng.standard_normal((n, n)) + 4 * np.eye(n)
+    A = sp.csr_matrix(Ad)
+    b = rng.standard_normal(n)
+    C = O.CRS(A.indptr, A.indices, A.data)
+    x, it, rel = O.gmres(C, b, eps=1e-13, max_iter=100, m=n)
+    assert it <= n and rel <= 1e-12
+    # restarted GMRES minimises the residual over each cycle: the true
+    # residual after k cycles never increases with k
+    A2 = _nonsymmetric(200, 5)
+    b2 = np.random.default_rng(6).standard_normal(200)
+    C2 = O.CRS(A2.indptr, A2.indices, A2.data)
+    rels = [O.gmres(C2, b2, eps=1e-30, max_iter=3 * k, m=3)[2] for k in range(1, 12)]
+    assert all(r1 <= r0 * (1 + 1e-12) for r0, r1 in zip(rels, rels[1:]))
+    assert rels[-1] < rels[0]
+
+
+def test_gmres_agrees_with_scipy_and_superlu():
+    from scipy.sparse.linalg import gmres as sp_gmres, splu
+    A = _nonsymmetric(150, 11)
+    b = np.random.default_rng(12).standard_normal(150)
+    xs, info = sp_gmres(A, b, restart=5, rtol=1e-12, maxiter=2000)
+    assert info == 0
+    x, it, rel = O.gmres(O.CRS(A.indptr, A.indices, A.data), b, eps=1e-12, max_iter=5000, m=5)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) <= 1e-9
+    # the SPD LSCM fixture: GMRES(30) + Jacobi reaches the direct solution
+    z = np.load(os.path.join(GOLD, "lscm_k12.npz"))
+    M = sp.csr_matrix((z["val"], z["colind"], z["rowptr"])).tocsc()
+    x_lu = splu(M).solve(z["b"])
+    x, it, rel = O.gmres(O.CRS(z["rowptr"], z["colind"], z["val"]), z["b"], eps=1e-11,
+                         max_iter=200000, m=30, precond="jacobi")
+    assert rel <= 1e-11
+    assert np.linalg.norm(x - x_lu) / np.linalg.norm(x_lu) <= 1e-6
+
+
 def tridiag(n):
     main = 2.0 * np.ones(n)
     off = -1.0 * np.ones(n - 1)
