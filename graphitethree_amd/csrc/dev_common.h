@@ -573,7 +573,7 @@ struct BlockStreamEngine {
     }
 };
 
-constexpr int kBlockEB = 2;  // blocks per lane per chunk (8 nonzeros)
+constexpr int kBlockEB = 4;  // blocks per lane per chunk (16 nonzeros)
 
 // Engine for a rows-per-wavefront code, and its LDS staging size.  Codes
 // 64 and above are stream engines; 64 itself is resolved by the launchers
@@ -623,6 +623,18 @@ template <>
 struct EngineOf<73> {  // 2x2 blocks (rp, cols, val are the block arrays)
     using type = BlockStreamEngine<kBlockEB>;
 };
+template <>
+struct EngineOf<74> {  // 2x2 blocks, 2 blocks per lane per chunk
+    using type = BlockStreamEngine<2>;
+};
+template <>
+struct EngineOf<75> {  // 2x2 blocks, 1 block per lane per chunk
+    using type = BlockStreamEngine<1>;
+};
+template <>
+struct EngineOf<76> {  // 2x2 blocks, 8 blocks per lane per chunk
+    using type = BlockStreamEngine<8>;
+};
 
 template <int RPW>
 struct StageSize {
@@ -632,6 +644,9 @@ struct StageSize {
                                  : RPW == 66 ? kWavesPerBlock * 64 * 8
                                  : (RPW == 67 || RPW == 72) ? kWavesPerBlock * 64 * 16
                                  : RPW == 73 ? kWavesPerBlock * 64 * kBlockEB * 4
+                                 : RPW == 74 ? kWavesPerBlock * 64 * 2 * 4
+                                 : RPW == 75 ? kWavesPerBlock * 64 * 1 * 4
+                                 : RPW == 76 ? kWavesPerBlock * 64 * 8 * 4
                                              : 1;
 };
 

@@ -33,6 +33,10 @@ __device__ __forceinline__ void tile_rows(const BatchArgs<T>& a, uint32_t t, uin
     r1 = (r0 + tr < e) ? r0 + tr : e;
 }
 
+// LDS staging per workgroup (elements): enough for either engine.
+constexpr int kBatchStage =
+    kWavesPerBlock * 64 * (kStreamE > kBlockEB * 4 ? kStreamE : kBlockEB * 4);
+
 template <typename T>
 __device__ __forceinline__ bool tile_blk(const BatchArgs<T>& a, uint32_t t) {
     return a.sys_pair0 && a.sys_pair0[a.tile_sys[t]] != kNoBlocks;
@@ -48,7 +52,7 @@ __device__ __forceinline__ void tile_spmv(const BatchArgs<T>& a, uint32_t t, uin
                                           Epi& epi) {
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     if (blk) {
-        static_assert(kBlockEB * 4 <= kStreamE, "staging area too small for the block engine");
+        static_assert(kBlockEB * 4 * 64 * kWavesPerBlock <= kBatchStage, "staging area too small");
         const uint32_t s = a.tile_sys[t], lo = a.row_off[s], p0 = a.sys_pair0[s];
         const int32_t delta = int32_t(lo - 2u * p0);
         const uint32_t B0 = p0 + (r0 - lo) / 2 + 64u * uint32_t(w), Bend = p0 + (r1 - lo) / 2;
@@ -78,7 +82,7 @@ __device__ __forceinline__ void tile_spmv(const BatchArgs<T>& a, uint32_t t, uin
 
 template <typename T, bool PREC>
 __global__ __launch_bounds__(kBlock) void k_b_init(BatchArgs<T> a) {
-    __shared__ T stage[kWavesPerBlock * 64 * kStreamE];
+    __shared__ T stage[kBatchStage];
     __shared__ T lds[kWavesPerBlock];
     const uint32_t n = *a.n_act_tiles;
     for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
@@ -164,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_b_dir(BatchArgs<T> a) {
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_spmv(BatchArgs<T> a) {
-    __shared__ T stage[kWavesPerBlock * 64 * kStreamE];
+    __shared__ T stage[kBatchStage];
     __shared__ T lds[kWavesPerBlock];
     const uint32_t n = *a.n_act_tiles;
     for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
@@ -291,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_b_all_active(BatchArgs<T> a) {
 
 // r = b - A x per tile (FP64), partials of r.r and b.b.
 __global__ __launch_bounds__(kBlock) void k_b_residual64(BatchArgs<double> a) {
-    __shared__ double stage[kWavesPerBlock * 64 * kStreamE];
+    __shared__ double stage[kBatchStage];
     __shared__ double lds[kWavesPerBlock];
     for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
         uint32_t r0, r1;

@@ -392,9 +392,10 @@ struct System {
             v = val32.get();
             bv = bval32.get();
         }
-        if (blocks() && (rpw == 73 || (rpw == 64 && n >= kBlockMinRows)))
-            return SpmvOps<T>{brp.get(), Cols{nullptr, bci16.get(), bsegb.get()}, bv, 73};
-        const int code = rpw == 73 ? 64 : rpw;
+        if (blocks() && (rpw >= 73 || (rpw == 64 && n >= kBlockMinRows)))
+            return SpmvOps<T>{brp.get(), Cols{nullptr, bci16.get(), bsegb.get()}, bv,
+                              rpw >= 73 ? rpw : 73};
+        const int code = rpw >= 73 ? 64 : rpw;
         return SpmvOps<T>{rowptr.get(), cols(), v, engine_rpw(code, cols())};
     }
 };

@@ -211,7 +211,7 @@ static void setup_args(System& S, CGWorkspace<T>& ws, const T* dinv) {
     a.sp = spmv_work_per_block(S.n, S.grid, o.code);
     // the streamed SpMV step runs with up to kSpmvGridMul times the
     // workgroups (at least four wavefront tiles each)
-    const uint64_t tile_rows = o.code == 73 ? 128 : 64;
+    const uint64_t tile_rows = o.code >= 73 ? 128 : 64;
     const uint64_t quads = (uint64_t(S.n) + 4 * tile_rows - 1) / (4 * tile_rows);
     a.Ps = o.code >= 64 ? uint32_t(std::max<uint64_t>(
                               S.grid, std::min<uint64_t>(uint64_t(kSpmvGridMul) * S.grid, quads)))
@@ -635,7 +635,7 @@ NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,
     return guarded("nlhipSystemSetLaunch", [&] {
         System* S = sys(s);
         if (!valid_rpw(rows_per_wave))
-            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64..73");
+            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64..76");
         if (workgroups == 0 || workgroups > 65536)
             throw Error(NLHIP_ERR_INVALID, "workgroups must be in [1, 65536]");
         std::lock_guard<std::mutex> lock(S->mu);

@@ -99,9 +99,10 @@ NLAPI int NLAPIENTRY nlhipSystemInfo(nlhip_system s, uint32_t* n, uint64_t* nnz,
                                      uint64_t* device_bytes);
 /* Launch shape of the SpMV engine: rows per wavefront (2, 4, 8, 16: that many
  * rows share a 64-lane wavefront; 64: one row per lane, nonzeros streamed
- * through LDS, the default) and workgroups per launch.  65..73 select
+ * through LDS, the default) and workgroups per launch.  65..76 select
  * variants of the streamed engine for launch-shape sweeps (chunk width,
- * load kind, column format, tile order); they give the same sums as 64. */
+ * load kind, column format, tile order, 2x2 blocks); they give the same sums
+ * as 64. */
 NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,
                                           uint32_t workgroups);
 NLAPI int NLAPIENTRY nlhipSystemGetLaunch(nlhip_system s, int* rows_per_wave,
