@@ -33,9 +33,12 @@ __device__ __forceinline__ void tile_rows(const BatchArgs<T>& a, uint32_t t, uin
     r1 = (r0 + tr < e) ? r0 + tr : e;
 }
 
-// LDS staging per workgroup (elements): enough for either engine.
+// Blocks per lane per chunk of the batched block engine (the single-system
+// default, 4, was slower on the atlas's small charts), and the LDS staging
+// per workgroup (elements), enough for either engine.
+constexpr int kBatchEB = 2;
 constexpr int kBatchStage =
-    kWavesPerBlock * 64 * (kStreamE > kBlockEB * 4 ? kStreamE : kBlockEB * 4);
+    kWavesPerBlock * 64 * (kStreamE > kBatchEB * 4 ? kStreamE : kBatchEB * 4);
 
 template <typename T>
 __device__ __forceinline__ bool tile_blk(const BatchArgs<T>& a, uint32_t t) {
@@ -52,19 +55,19 @@ __device__ __forceinline__ void tile_spmv(const BatchArgs<T>& a, uint32_t t, uin
                                           Epi& epi) {
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     if (blk) {
-        static_assert(kBlockEB * 4 * 64 * kWavesPerBlock <= kBatchStage, "staging area too small");
+        static_assert(kBatchEB * 4 * 64 * kWavesPerBlock <= kBatchStage, "staging area too small");
         const uint32_t s = a.tile_sys[t], lo = a.row_off[s], p0 = a.sys_pair0[s];
         const int32_t delta = int32_t(lo - 2u * p0);
         const uint32_t B0 = p0 + (r0 - lo) / 2 + 64u * uint32_t(w), Bend = p0 + (r1 - lo) / 2;
         if (B0 >= Bend) return;
         const uint32_t B1 = (B0 + 64u < Bend) ? B0 + 64u : Bend;
         const Cols bcols{a.bcol, a.bci16, a.bsegb};
-        T* st = stage + w * (64 * kBlockEB * 4);
+        T* st = stage + w * (64 * kBatchEB * 4);
         if (a.bci16)
-            BlockStreamEngine<kBlockEB, true>::run_rows(B0, B1, a.brp, bcols, a.bval, x, st,
+            BlockStreamEngine<kBatchEB, true>::run_rows(B0, B1, a.brp, bcols, a.bval, x, st,
                                                         lane, epi, delta);
         else
-            BlockStreamEngine<kBlockEB, false>::run_rows(B0, B1, a.brp, bcols, a.bval, x, st,
+            BlockStreamEngine<kBatchEB, false>::run_rows(B0, B1, a.brp, bcols, a.bval, x, st,
                                                          lane, epi, delta);
         return;
     }

@@ -1,15 +1,25 @@
 // Solve server: admits requests (one system, one or more right-hand sides)
 // under an HBM byte budget and a slot limit, runs the admitted ones
-// concurrently -- each on its own resident system and HIP stream, so their
-// kernels overlap on the device -- and hands back solutions with per-request
-// timestamps for latency accounting.
+// concurrently -- each solve on its own HIP stream, so their kernels overlap
+// on the device -- and hands back solutions with per-request timestamps for
+// latency accounting.
 //
 // Admission is first-come first-served: the oldest waiting request is
 // admitted as soon as a slot is free and its device footprint fits in what
 // the budget has left; a request larger than the whole budget is rejected
-// with NLHIP_ERR_BUDGET instead of queueing forever.  Every request's kernels
-// depend only on that request (its own matrix, vectors, launch shape), so a
-// result is bitwise the same whether it ran alone or next to others.
+// with NLHIP_ERR_BUDGET instead of queueing forever.
+//
+// Batching: a small single-right-hand-side request (n <= kBatchRows) is
+// solved by the batched solver, together with the other waiting small
+// requests that have the same solve parameters (taken in arrival order,
+// within the budget and kBatchSystems / kBatchTotalRows); a batch takes one
+// slot.  Such a request always goes this way, alone as a batch of one, and
+// a batched system's result does not depend on the rest of its batch, so
+// every result is bitwise the same whether it ran alone or next to others.
+// Larger and multi-right-hand-side requests run on their own resident
+// system.  Validation and the fault-injection hook act on each request
+// before batching, so a failing request fails alone.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -26,9 +36,14 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+constexpr uint32_t kBatchRows = 1u << 18;        // largest request that is batched
+constexpr uint32_t kBatchSystems = 256;          // requests per batch
+constexpr uint64_t kBatchTotalRows = 1ull << 23; // rows per batch
+
 struct Request {
     uint64_t id = 0;
     uint32_t n = 0, k = 1;
+    bool batchable = false;
     std::vector<uint32_t> rowptr, colind;
     std::vector<double> val, B, X;
     nlhip_solve_params params{};
@@ -65,7 +80,8 @@ struct Server {
     std::map<uint64_t, std::shared_ptr<Request>> all;
     uint64_t next_id = 1;
     uint64_t in_use = 0, peak_in_use = 0;
-    uint32_t running = 0, peak_running = 0;
+    uint32_t running = 0, peak_running = 0;  // requests on the device
+    uint32_t solves = 0;                     // solves (single or batched) in flight
     uint64_t completed = 0, failed = 0;
     bool stop = false;
     std::vector<std::thread> workers;
@@ -74,36 +90,63 @@ struct Server {
 
     void worker() {
         for (;;) {
-            std::shared_ptr<Request> R;
+            std::vector<std::shared_ptr<Request>> group;
+            uint64_t bytes = 0;
             {
                 std::unique_lock<std::mutex> lock(mu);
                 auto admissible = [&] {
-                    return !pending.empty() && running < slots &&
+                    return !pending.empty() && solves < slots &&
                            in_use + pending.front()->bytes <= budget;
                 };
                 cv_work.wait(lock, [&] { return (stop && pending.empty()) || admissible(); });
                 if (!admissible()) return;  // stopping and nothing left to run
-                R = pending.front();
+                group.push_back(pending.front());
                 pending.pop_front();
-                in_use += R->bytes;
+                bytes = group[0]->bytes;
+                if (group[0]->batchable) {
+                    // compatible waiting requests, in arrival order
+                    uint64_t rows = group[0]->n;
+                    for (auto it = pending.begin();
+                         it != pending.end() && group.size() < kBatchSystems;) {
+                        const Request& C = **it;
+                        if (C.batchable && rows + C.n <= kBatchTotalRows &&
+                            in_use + bytes + C.bytes <= budget &&
+                            std::memcmp(&C.params, &group[0]->params, sizeof(C.params)) == 0) {
+                            rows += C.n;
+                            bytes += C.bytes;
+                            group.push_back(*it);
+                            it = pending.erase(it);
+                        } else {
+                            ++it;
+                        }
+                    }
+                }
+                in_use += bytes;
                 peak_in_use = std::max(peak_in_use, in_use);
-                ++running;
+                ++solves;
+                running += uint32_t(group.size());
                 peak_running = std::max(peak_running, running);
-                R->t_admit = now();
+                const double t = now();
+                for (auto& R : group) R->t_admit = t;
             }
-            run(*R);
+            if (group[0]->batchable) run_batch(group);
+            else run(*group[0]);
             {
                 std::lock_guard<std::mutex> lock(mu);
-                in_use -= R->bytes;
-                --running;
-                R->t_done = now();
-                R->finished = true;
-                (R->status == NLHIP_OK ? completed : failed) += 1;
-                // inputs are no longer needed; keep only the solution
-                R->rowptr = {};
-                R->colind = {};
-                R->val = {};
-                R->B = {};
+                in_use -= bytes;
+                --solves;
+                running -= uint32_t(group.size());
+                const double t = now();
+                for (auto& R : group) {
+                    R->t_done = t;
+                    R->finished = true;
+                    (R->status == NLHIP_OK ? completed : failed) += 1;
+                    // inputs are no longer needed; keep only the solution
+                    R->rowptr = {};
+                    R->colind = {};
+                    R->val = {};
+                    R->B = {};
+                }
             }
             cv_work.notify_all();
             cv_done.notify_all();
@@ -121,6 +164,76 @@ struct Server {
             }
         });
         R.stats.status = R.status;
+    }
+
+    // Validates every request (a failing one fails alone), then solves the
+    // others as one batch and hands each its own solution and statistics.
+    void run_batch(std::vector<std::shared_ptr<Request>>& group) {
+        std::vector<Request*> ok;
+        for (auto& R : group) {
+            R->status = guarded("nlhip server request", [&] {
+                if (take_injected_failure())
+                    throw Error(NLHIP_ERR_INJECTED, "injected failure");
+                validate(*R);
+            });
+            R->stats.status = R->status;
+            if (R->status == NLHIP_OK) ok.push_back(R.get());
+        }
+        if (ok.empty()) return;
+        const uint32_t ns = uint32_t(ok.size());
+        std::vector<uint32_t> row_off(size_t(ns) + 1, 0), rowptr, colind;
+        std::vector<double> val, b, x;
+        for (uint32_t i = 0; i < ns; ++i) row_off[i + 1] = row_off[i] + ok[i]->n;
+        rowptr.reserve(size_t(row_off[ns]) + 1);
+        uint64_t nnz = 0;
+        for (Request* R : ok) {
+            for (uint32_t r = 0; r < R->n; ++r) rowptr.push_back(uint32_t(nnz + R->rowptr[r]));
+            nnz += R->rowptr[R->n];
+            if (nnz > 0xffffffffull) break;
+        }
+        rowptr.push_back(uint32_t(nnz));
+        colind.reserve(nnz);
+        val.reserve(nnz);
+        for (uint32_t i = 0; i < ns; ++i) {
+            for (uint32_t c : ok[i]->colind) colind.push_back(c + row_off[i]);
+            val.insert(val.end(), ok[i]->val.begin(), ok[i]->val.end());
+            b.insert(b.end(), ok[i]->B.begin(), ok[i]->B.end());
+            x.insert(x.end(), ok[i]->X.begin(), ok[i]->X.end());
+        }
+        std::vector<nlhip_solve_stats> per(ns);
+        nlhip_batch Bh = nullptr;
+        const nlhip_solve_params& p = ok[0]->params;
+        int rc = nnz > 0xffffffffull ? NLHIP_ERR_UNSUPPORTED : NLHIP_OK;
+        if (rc == NLHIP_OK)
+            rc = nlhipBatchCreate(ns, row_off.data(), rowptr.data(), colind.data(), val.data(),
+                                  p.precision == NLHIP_PREC_MIXED ? NLHIP_SYSTEM_FP32 : 0u, &Bh);
+        if (rc == NLHIP_OK) rc = nlhipBatchSolve(Bh, b.data(), x.data(), &p, per.data(), nullptr);
+        if (Bh) nlhipBatchDestroy(Bh);
+        for (uint32_t i = 0; i < ns; ++i) {
+            Request& R = *ok[i];
+            R.status = rc;
+            if (rc == NLHIP_OK) {
+                R.stats = per[i];
+                std::copy(x.begin() + row_off[i], x.begin() + row_off[i + 1], R.X.begin());
+            }
+            R.stats.status = rc;
+        }
+    }
+
+    // The checks system creation makes, per request: CRS structure, finite
+    // values and right-hand side.
+    static void validate(const Request& R) {
+        if (R.rowptr.empty() || R.rowptr[0] != 0) throw Error(NLHIP_ERR_INVALID, "rowptr[0] != 0");
+        for (uint32_t i = 0; i < R.n; ++i)
+            if (R.rowptr[i + 1] < R.rowptr[i])
+                throw Error(NLHIP_ERR_INVALID, "rowptr is not monotone");
+        for (uint32_t c : R.colind)
+            if (c >= R.n) throw Error(NLHIP_ERR_INVALID, "colind out of range");
+        for (double v : R.val)
+            if (!std::isfinite(v)) throw Error(NLHIP_ERR_NONFINITE, "matrix value is NaN or Inf");
+        for (double v : R.B)
+            if (!std::isfinite(v))
+                throw Error(NLHIP_ERR_NONFINITE, "right-hand side is NaN or Inf");
     }
 };
 
@@ -182,6 +295,7 @@ NLAPI int NLAPIENTRY nlhipServerSubmit(nlhip_server h, uint32_t n, const uint32_
         nlhipSolveParamsDefault(&R->params);
         if (params) R->params = *params;
         R->bytes = footprint(n, nnz, k, R->params.precision == NLHIP_PREC_MIXED);
+        R->batchable = (k == 1 && n <= kBatchRows);
         std::lock_guard<std::mutex> lock(S->mu);
         if (S->stop) throw Error(NLHIP_ERR_INVALID, "server is shutting down");
         R->id = S->next_id++;

@@ -22,11 +22,11 @@ def requests():
 
 
 def solve_alone(S, B):
-    sysc = capi.System.from_crs(S)
-    if B.shape[0] == 1:
-        x, _ = sysc.solve(B[0], eps=1e-10, max_iter=5000)
-        return x[None, :]
-    X, _, _ = sysc.solve_multi(B, eps=1e-10, max_iter=5000)
+    """The same request on a server of its own."""
+    srv = capi.Server(max_active=1)
+    X, res = srv.wait(srv.submit(S, B, eps=1e-10, max_iter=5000), timeout=120)
+    srv.close()
+    assert res.status == 0
     return X
 
 
@@ -43,9 +43,33 @@ def test_server_mix_equals_alone(hip):
         assert res.status == 0 and res.stats.converged == 1
         assert res.t_submit <= res.t_admit <= res.t_done
         assert np.array_equal(X, solve_alone(S, B))
-    S, B = reqs[1]
-    xr, _, _ = O.pcg(O.CRS(S.rowptr, S.colind, S.val), B[0], eps=1e-10, max_iter=5000)
-    assert np.linalg.norm(results[1][0][0] - xr) / np.linalg.norm(xr) <= 1e-8
+    for i in (1, 4):
+        S, B = reqs[i]
+        xr, _, _ = O.pcg(O.CRS(S.rowptr, S.colind, S.val), B[0], eps=1e-10, max_iter=5000)
+        assert np.linalg.norm(results[i][0][0] - xr) / np.linalg.norm(xr) <= 1e-8
+    S, B = reqs[6]  # three right-hand sides, run on its own system
+    for j in range(3):
+        xr, _, _ = O.pcg(O.CRS(S.rowptr, S.colind, S.val), B[j], eps=1e-10, max_iter=5000)
+        assert np.linalg.norm(results[6][0][j] - xr) / np.linalg.norm(xr) <= 1e-8
+
+
+def test_server_batches_small_requests(hip):
+    """Small single-right-hand-side requests waiting together are solved as
+    one batch: with one slot, several requests are on the device at once,
+    and each result matches the oracle."""
+    reqs = [W.lscm_system(W.chart_mesh(N, seed=400 + i))
+            for i, N in enumerate((200, 900, 1500, 300, 2200, 600))]
+    srv = capi.Server(max_active=1)
+    ids = [srv.submit(S, S.b, eps=1e-10, max_iter=5000) for S in reqs]
+    out = [srv.wait(i, timeout=120) for i in ids]
+    info = srv.info()
+    srv.close()
+    assert info.completed == len(reqs) and info.failed == 0
+    assert info.peak_running >= 2
+    for S, (X, res) in zip(reqs, out):
+        assert res.status == 0 and res.stats.converged == 1
+        xr, _, _ = O.pcg(O.CRS(S.rowptr, S.colind, S.val), S.b, eps=1e-10, max_iter=5000)
+        assert np.linalg.norm(X[0] - xr) / np.linalg.norm(xr) <= 1e-8
 
 
 def test_server_budget_admission(hip):
