@@ -1,28 +1,26 @@
 // Batched Jacobi-PCG over many independent systems (the per-chart solves of
 // a texture atlas, src/lib/OGF/mesh/commands/mesh_grob_surface_commands.cpp:
-// 724-730), run as one fixed grid of persistent workgroups.
+// 724-730), run as one fixed grid of workgroups per launch.
 //
 // Layout: the systems are packed as one block-diagonal CRS (global row and
 // column indices), rows of system s in [row_off[s], row_off[s+1]).  Each
 // system is cut into tiles of kTileRows rows that never straddle systems.
-// A workgroup walks the device-side list of active tiles (grid stride), so
-// one launch covers every unconverged system whatever their sizes.
+// Workgroups walk the device-side list of active tiles (a contiguous share
+// each in the iteration kernels, grid stride elsewhere), so one launch
+// covers every unconverged system whatever their sizes.
 //
-// Per iteration and per system the arithmetic is the CG of k_cg.hip:
-//   direction d = beta d + h;  spmv q = A d (+ tile partials of d.q);
-//   reduce-A alpha = rho / sum(d.q);  update x, r, h (+ partials r.h, r.r);
-//   reduce-B rho', err, stopping test, beta = rho' / rho.
-// System scalars are reduced by one wavefront per system over that system's
-// tile partials in a fixed order, so a system's result depends only on the
-// system itself: it is bitwise the same solved alone or in any batch.
+// Per iteration and per system the arithmetic is the CG of k_cg.hip, in the
+// same three launches (direction, SpMV, update; see below).  System scalars
+// are reduced over that system's tile partials in a fixed order, so a
+// system's result depends only on the system itself: it is bitwise the same
+// solved alone or in any batch.
 #include "dev_common.h"
 #include "nlhip_kernels.h"
 
 namespace nlhip {
 
 // Rows [r0, r1) of tile t: kTileRows rows, or kBlockTileRows for systems in
-// 2x2 block form (a tile is one pass of the workgroup's 4 wavefronts).  The
-// tiling of a system depends only on that system.
+// 2x2 block form.  The tiling of a system depends only on that system.
 template <typename T>
 __device__ __forceinline__ void tile_rows(const BatchArgs<T>& a, uint32_t t, uint32_t& r0,
                                           uint32_t& r1) {
@@ -33,55 +31,134 @@ __device__ __forceinline__ void tile_rows(const BatchArgs<T>& a, uint32_t t, uin
     r1 = (r0 + tr < e) ? r0 + tr : e;
 }
 
-// Blocks per lane per chunk of the batched block engine (the single-system
-// default, 4, was slower on the atlas's small charts), and the LDS staging
-// per workgroup (elements), enough for either engine.
+// Blocks per lane per chunk of the batched block engine: 4 in the kernel
+// built for block-form systems only, 2 in the kernel that also holds the
+// scalar engine (whose registers would otherwise cut occupancy to 3
+// wavefronts per SIMD); and the LDS staging per workgroup (elements),
+// enough for any of them.
 constexpr int kBatchEB = 2;
-constexpr int kBatchStage =
-    kWavesPerBlock * 64 * (kStreamE > kBatchEB * 4 ? kStreamE : kBatchEB * 4);
+constexpr int kBatchEBBlock = 4;
+constexpr int kBatchStageE = kStreamE > kBatchEBBlock * 4 ? kStreamE : kBatchEBBlock * 4;
+constexpr int kBatchStage = kWavesPerBlock * 64 * kBatchStageE;
 
 template <typename T>
 __device__ __forceinline__ bool tile_blk(const BatchArgs<T>& a, uint32_t t) {
     return a.sys_pair0 && a.sys_pair0[a.tile_sys[t]] != kNoBlocks;
 }
 
-// Stream SpMV of one tile: wavefront w takes rows [r0 + 64 w, r0 + 64 w + 64)
-// or, for a system in 2x2 block form (blk), the block rows of its row pairs
-// [r0 + 128 w, r0 + 128 w + 128) (r0 - row_off[s] and r1 - row_off[s] are
-// even for such a system).
+// Stream SpMV of rows [r0, r1) of system s (first row lo, first block row
+// p0, or kNoBlocks for the scalar form): wavefront w takes kTileWaveRows
+// rows from r0 + w kTileWaveRows or, in 2x2 block form, kTileWaveRows
+// block rows (row pairs) from row r0 + 2 w kTileWaveRows (r0 - lo and
+// r1 - lo are even for such a system).
+//
+// F (kFormAny, kFormBlock16, kFormBlock32) says which forms the kernel is
+// built for: any, or only the 2x2 block form with 16-bit or 32-bit block
+// columns (a batch whose systems all have that form; such a kernel holds
+// one engine and needs fewer registers, so more wavefronts fit on a CU).
+constexpr int kFormAny = 0, kFormBlock16 = 1, kFormBlock32 = 2;
+
+template <typename T, int F = kFormAny, typename Epi>
+__device__ __forceinline__ void tile_spmv_rows(const BatchArgs<T>& a, uint32_t r0, uint32_t r1,
+                                               uint32_t lo, uint32_t p0, const T* x, T* stage,
+                                               Epi& epi) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (F != kFormAny || p0 != kNoBlocks) {
+        constexpr int EB = F == kFormAny ? kBatchEB : kBatchEBBlock;
+        static_assert(EB * 4 * 64 * kWavesPerBlock <= kBatchStage, "staging area too small");
+        const int32_t delta = int32_t(lo - 2u * p0);
+        const uint32_t B0 = p0 + (r0 - lo) / 2 + kTileWaveRows * uint32_t(w);
+        const uint32_t Bend = p0 + (r1 - lo) / 2;
+        if (B0 >= Bend) return;
+        const uint32_t B1 = (B0 + kTileWaveRows < Bend) ? B0 + kTileWaveRows : Bend;
+        const Cols bcols{a.bcol, a.bci16, a.bsegb};
+        T* st = stage + w * (64 * EB * 4);
+        if (F == kFormBlock16 || (F == kFormAny && a.bci16))
+            BlockStreamEngine<EB, true>::run_rows(B0, B1, a.brp, bcols, a.bval, x, st, lane,
+                                                  epi, delta);
+        else
+            BlockStreamEngine<EB, false>::run_rows(B0, B1, a.brp, bcols, a.bval, x, st, lane,
+                                                   epi, delta);
+        return;
+    }
+    if constexpr (F == kFormAny) {
+        const uint32_t R0 = r0 + kTileWaveRows * uint32_t(w);
+        if (R0 >= r1) return;
+        const uint32_t R1 = (R0 + kTileWaveRows < r1) ? R0 + kTileWaveRows : r1;
+        const Cols cols{a.ci, a.ci16, a.segb};
+        if (a.ci16)
+            StreamEngine<kStreamE, true, true>::run_rows(R0, R1, a.rp, cols, a.val, x,
+                                                         stage + w * (64 * kStreamE), lane, epi);
+        else
+            StreamEngine<kStreamE, true>::run_rows(R0, R1, a.rp, cols, a.val, x,
+                                                   stage + w * (64 * kStreamE), lane, epi);
+    }
+}
+
 template <typename T, typename Epi>
 __device__ __forceinline__ void tile_spmv(const BatchArgs<T>& a, uint32_t t, uint32_t r0,
                                           uint32_t r1, bool blk, const T* x, T* stage,
                                           Epi& epi) {
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    if (blk) {
-        static_assert(kBatchEB * 4 * 64 * kWavesPerBlock <= kBatchStage, "staging area too small");
-        const uint32_t s = a.tile_sys[t], lo = a.row_off[s], p0 = a.sys_pair0[s];
-        const int32_t delta = int32_t(lo - 2u * p0);
-        const uint32_t B0 = p0 + (r0 - lo) / 2 + 64u * uint32_t(w), Bend = p0 + (r1 - lo) / 2;
-        if (B0 >= Bend) return;
-        const uint32_t B1 = (B0 + 64u < Bend) ? B0 + 64u : Bend;
-        const Cols bcols{a.bcol, a.bci16, a.bsegb};
-        T* st = stage + w * (64 * kBatchEB * 4);
-        if (a.bci16)
-            BlockStreamEngine<kBatchEB, true>::run_rows(B0, B1, a.brp, bcols, a.bval, x, st,
-                                                        lane, epi, delta);
-        else
-            BlockStreamEngine<kBatchEB, false>::run_rows(B0, B1, a.brp, bcols, a.bval, x, st,
-                                                         lane, epi, delta);
-        return;
-    }
-    const uint32_t R0 = r0 + 64u * uint32_t(w);
-    if (R0 >= r1) return;
-    const uint32_t R1 = (R0 + 64u < r1) ? R0 + 64u : r1;
-    const Cols cols{a.ci, a.ci16, a.segb};
-    if (a.ci16)
-        StreamEngine<kStreamE, true, true>::run_rows(R0, R1, a.rp, cols, a.val, x,
-                                                     stage + w * (64 * kStreamE), lane, epi);
-    else
-        StreamEngine<kStreamE, true>::run_rows(R0, R1, a.rp, cols, a.val, x,
-                                               stage + w * (64 * kStreamE), lane, epi);
+    const uint32_t s = a.tile_sys[t];
+    tile_spmv_rows(a, r0, r1, a.row_off[s], blk ? a.sys_pair0[s] : kNoBlocks, x, stage, epi);
 }
+
+// Metadata of one entry of the active-tile list.  In the iteration kernels
+// lane j of every wavefront loads the entry k + j, so a workgroup fetches
+// the metadata of 64 tiles in three dependent loads instead of three per
+// tile, and reads one tile's with v_readlane.
+struct TileDesc {
+    uint32_t t;         // tile
+    uint32_t s;         // its system
+    uint32_t r0, r1;    // its rows
+    uint32_t lo;        // first row of s
+    uint32_t p0;        // first block row of s, or kNoBlocks
+    uint32_t to0, to1;  // tiles of s
+    uint32_t flags;     // 1: done[s], 2: stop[s] (when asked for)
+};
+
+template <typename T>
+__device__ __forceinline__ TileDesc tile_desc_load(const BatchArgs<T>& a, uint32_t k,
+                                                   uint32_t k1, bool with_stop) {
+    TileDesc d{};
+    if (k < k1) {
+        d.t = a.act_tiles[k];
+        d.s = a.tile_sys[d.t];
+        d.r0 = a.tile_row0[d.t];
+        d.lo = a.row_off[d.s];
+        const uint32_t e = a.row_off[d.s + 1];
+        d.p0 = a.sys_pair0 ? a.sys_pair0[d.s] : kNoBlocks;
+        const uint32_t tr = d.p0 != kNoBlocks ? kBlockTileRows : kTileRows;
+        d.r1 = (d.r0 + tr < e) ? d.r0 + tr : e;
+        d.to0 = a.tile_off[d.s];
+        d.to1 = a.tile_off[d.s + 1];
+        d.flags = (a.done[d.s] ? 1u : 0u) | ((with_stop && a.stop[d.s]) ? 2u : 0u);
+    }
+    return d;
+}
+
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, int j) {
+    return uint32_t(__builtin_amdgcn_readlane(int(v), j));
+}
+
+__device__ __forceinline__ TileDesc tile_desc_at(const TileDesc& v, int j) {
+    TileDesc d;
+    d.t = lane_value(v.t, j);
+    d.s = lane_value(v.s, j);
+    d.r0 = lane_value(v.r0, j);
+    d.r1 = lane_value(v.r1, j);
+    d.lo = lane_value(v.lo, j);
+    d.p0 = lane_value(v.p0, j);
+    d.to0 = lane_value(v.to0, j);
+    d.to1 = lane_value(v.to1, j);
+    d.flags = lane_value(v.flags, j);
+    return d;
+}
+
+// Rows of a tile per thread in the iteration kernels (loads of all of them
+// are issued before the stores).
+constexpr int kTileRowsPerThread = int(kBlockTileRows) / kBlock;
+static_assert(kTileRows <= kBlockTileRows && kBlockTileRows % kBlock == 0, "tile shape");
 
 template <typename T, bool PREC>
 __global__ __launch_bounds__(kBlock) void k_b_init(BatchArgs<T> a) {
@@ -120,12 +197,16 @@ __global__ __launch_bounds__(kBlock) void k_b_init(BatchArgs<T> a) {
 // One wavefront per system: sum of part[tile_off[s] .. tile_off[s+1]) in a
 // fixed order (lane-strided, then the xor butterfly).
 template <typename T>
-__device__ __forceinline__ T sys_sum(const BatchArgs<T>& a, const T* part, uint32_t s) {
+__device__ __forceinline__ T tiles_sum(const T* part, uint32_t to0, uint32_t to1) {
     const int lane = threadIdx.x & (kWave - 1);
     T v = T(0);
-    for (uint32_t t = a.tile_off[s] + uint32_t(lane); t < a.tile_off[s + 1]; t += kWave)
-        v += part[t];
+    for (uint32_t t = to0 + uint32_t(lane); t < to1; t += kWave) v += part[t];
     return group_sum<kWave>(v);
+}
+
+template <typename T>
+__device__ __forceinline__ T sys_sum(const BatchArgs<T>& a, const T* part, uint32_t s) {
+    return tiles_sum(part, a.tile_off[s], a.tile_off[s + 1]);
 }
 
 template <typename T>
@@ -140,116 +221,192 @@ __global__ __launch_bounds__(kBlock) void k_b_sys_init(BatchArgs<T> a, T eps, in
         a.bb[s] = bb;
         a.thresh[s] = thresh;
         a.rho[s] = rho;
+        a.rho_prev[s] = T(1);
         a.err[s] = err;
         a.beta[s] = T(0);
         a.alpha[s] = T(0);
         a.iters[s] = 0;
+        a.stop[s] = 0;
         const bool skip = a.skip && a.skip[s];
         a.done[s] = (skip || !(err > thresh) || max_iter <= 0) ? 1 : 0;
     }
 }
 
+// Three launches per iteration, as the single-system CG (k_cg.hip): the
+// system scalars are reduced from the tile partials by every workgroup that
+// needs them (sys_sum, a fixed order), and only the workgroup of a system's
+// first tile writes them.
+//
+//   direction  rho, err from the r.h, r.r partials; x -= alpha d with the
+//              previous alpha (the deferred update of x); stopping test
+//              (raises stop[s]); beta = rho / rho_prev; d = beta d + h
+//              (d = h at iteration 0, whose test ran in k_b_sys_init)
+//   spmv       stop[s] -> done[s]; q = A d, tile partials of d.q
+//   update     alpha = rho / sum(d.q); r -= alpha q; h = P r; partials of
+//              r.h and r.r
+// Each workgroup takes a contiguous share of the active-tile list.  The
+// list keeps a system's tiles adjacent (k_b_compact appends them in order
+// within each wavefront), so a workgroup meets a system once or twice per
+// launch and reduces its scalars only then, instead of once per tile.
+// Shares follow the XCD-contiguous logical block order, so the tiles of a
+// system run on one or two XCDs and its SpMV gathers stay in their L2.
+__device__ __forceinline__ void tile_share(uint32_t n, uint32_t& k0, uint32_t& k1) {
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    k0 = uint32_t(uint64_t(n) * L / gridDim.x);
+    k1 = uint32_t(uint64_t(n) * (L + 1) / gridDim.x);
+}
+
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_dir(BatchArgs<T> a) {
-    const uint32_t n = *a.n_act_tiles;
-    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
-        const uint32_t t = a.act_tiles[k];
-        const uint32_t s = a.tile_sys[t];
-        if (a.done[s]) continue;
-        uint32_t r0, r1;
-        tile_rows(a, t, r0, r1);
-        const int it = a.iters[s];
-        const T beta = a.beta[s];
-        if (it == 0) {
-            for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) a.d[i] = a.h[i];
-        } else {
-            for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock)
-                a.d[i] = add_rn(mul_rn(beta, a.d[i]), a.h[i]);
+__global__ __launch_bounds__(kBlock) void k_b_dir(BatchArgs<T> a, int max_iter) {
+    uint32_t k0, k1;
+    tile_share(*a.n_act_tiles, k0, k1);
+    const int lane = threadIdx.x & (kWave - 1);
+    uint32_t cs = ~0u;  // system whose scalars are held below
+    int it = 0;
+    bool stop = false;
+    T rho = T(0), err = T(0), alpha = T(0), beta = T(0);
+    for (uint32_t kb = k0; kb < k1; kb += kWave) {
+        // stop[s] is raised in this launch, so it is not read here
+        const TileDesc mine = tile_desc_load(a, kb + uint32_t(lane), k1, false);
+        const int nb = int(k1 - kb < uint32_t(kWave) ? k1 - kb : uint32_t(kWave));
+        for (int j = 0; j < nb; ++j) {
+            const TileDesc td = tile_desc_at(mine, j);
+            if (td.flags & 1u) continue;
+            if (td.s != cs) {
+                cs = td.s;
+                it = a.iters[td.s];
+                if (it > 0) {
+                    rho = tiles_sum(a.part_rh, td.to0, td.to1);
+                    err = tiles_sum(a.part_rr, td.to0, td.to1);
+                    stop = !(err > a.thresh[td.s]) || it >= max_iter;
+                    alpha = a.alpha[td.s];
+                    beta = rho / a.rho_prev[td.s];
+                }
+            }
+            T xv[kTileRowsPerThread], dv[kTileRowsPerThread], hv[kTileRowsPerThread];
+            if (it == 0) {
+#pragma unroll
+                for (int q = 0; q < kTileRowsPerThread; ++q) {
+                    const uint32_t i = td.r0 + threadIdx.x + uint32_t(q * kBlock);
+                    if (i < td.r1) hv[q] = a.h[i];
+                }
+#pragma unroll
+                for (int q = 0; q < kTileRowsPerThread; ++q) {
+                    const uint32_t i = td.r0 + threadIdx.x + uint32_t(q * kBlock);
+                    if (i < td.r1) a.d[i] = hv[q];
+                }
+                continue;
+            }
+            if (td.t == td.to0 && threadIdx.x == 0) {
+                a.err[td.s] = err;
+                a.rho[td.s] = rho;
+                if (stop) a.stop[td.s] = 1;
+                else a.beta[td.s] = beta;
+            }
+#pragma unroll
+            for (int q = 0; q < kTileRowsPerThread; ++q) {
+                const uint32_t i = td.r0 + threadIdx.x + uint32_t(q * kBlock);
+                if (i < td.r1) {
+                    xv[q] = a.x[i];
+                    dv[q] = a.d[i];
+                    if (!stop) hv[q] = a.h[i];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kTileRowsPerThread; ++q) {
+                const uint32_t i = td.r0 + threadIdx.x + uint32_t(q * kBlock);
+                if (i < td.r1) {
+                    a.x[i] = add_rn(xv[q], mul_rn(-alpha, dv[q]));
+                    if (!stop) a.d[i] = add_rn(mul_rn(beta, dv[q]), hv[q]);
+                }
+            }
         }
     }
 }
 
-template <typename T>
+template <typename T, int F>
 __global__ __launch_bounds__(kBlock) void k_b_spmv(BatchArgs<T> a) {
     __shared__ T stage[kBatchStage];
     __shared__ T lds[kWavesPerBlock];
-    const uint32_t n = *a.n_act_tiles;
-    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
-        const uint32_t t = a.act_tiles[k];
-        const uint32_t s = a.tile_sys[t];
-        if (a.done[s]) continue;
-        uint32_t r0, r1;
-        tile_rows(a, t, r0, r1);
-        T pdq = T(0);
-        auto epi = prefetch_epi([&](uint32_t row) { return a.d[row]; },
-                                [&](uint32_t row, T v, T dr) {
-                                    a.q[row] = v;
-                                    pdq += dr * v;
-                                });
-        tile_spmv(a, t, r0, r1, tile_blk(a, t), a.d, stage, epi);
-        pdq = block_sum_all(pdq, lds);
-        if (threadIdx.x == 0) a.part_dq[t] = pdq;
+    uint32_t k0, k1;
+    tile_share(*a.n_act_tiles, k0, k1);
+    const int lane = threadIdx.x & (kWave - 1);
+    for (uint32_t kb = k0; kb < k1; kb += kWave) {
+        const TileDesc mine = tile_desc_load(a, kb + uint32_t(lane), k1, true);
+        const int nb = int(k1 - kb < uint32_t(kWave) ? k1 - kb : uint32_t(kWave));
+        for (int j = 0; j < nb; ++j) {
+            const TileDesc td = tile_desc_at(mine, j);
+            if (td.flags & 1u) continue;
+            if (td.flags & 2u) {
+                if (td.t == td.to0 && threadIdx.x == 0) a.done[td.s] = 1;
+                continue;
+            }
+            T pdq = T(0);
+            auto epi = prefetch_epi([&](uint32_t row) { return a.d[row]; },
+                                    [&](uint32_t row, T v, T dr) {
+                                        a.q[row] = v;
+                                        pdq += dr * v;
+                                    });
+            tile_spmv_rows<T, F>(a, td.r0, td.r1, td.lo, td.p0, a.d, stage, epi);
+            pdq = block_sum_all(pdq, lds);
+            if (threadIdx.x == 0) a.part_dq[td.t] = pdq;
+        }
     }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_reduce_alpha(BatchArgs<T> a) {
-    const uint32_t ks = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    if (ks >= *a.n_act_sys) return;
-    const uint32_t s = a.act_sys[ks];
-    if (a.done[s]) return;
-    const T dq = sys_sum(a, a.part_dq, s);
-    if ((threadIdx.x & (kWave - 1)) == 0) a.alpha[s] = a.rho[s] / dq;
 }
 
 template <typename T, bool PREC>
 __global__ __launch_bounds__(kBlock) void k_b_update(BatchArgs<T> a) {
     __shared__ T lds[kWavesPerBlock];
-    const uint32_t n = *a.n_act_tiles;
-    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
-        const uint32_t t = a.act_tiles[k];
-        const uint32_t s = a.tile_sys[t];
-        if (a.done[s]) continue;
-        uint32_t r0, r1;
-        tile_rows(a, t, r0, r1);
-        const T alpha = a.alpha[s];
-        T prh = T(0), prr = T(0);
-        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
-            const T xi = add_rn(a.x[i], mul_rn(-alpha, a.d[i]));
-            const T ri = add_rn(a.r[i], mul_rn(-alpha, a.q[i]));
-            const T hi = PREC ? mul_rn(a.dinv[i], ri) : ri;
-            a.x[i] = xi;
-            a.r[i] = ri;
-            a.h[i] = hi;
-            prh += ri * hi;
-            prr += ri * ri;
-        }
-        prh = block_sum_all(prh, lds);
-        prr = block_sum_all(prr, lds);
-        if (threadIdx.x == 0) {
-            a.part_rh[t] = prh;
-            a.part_rr[t] = prr;
-        }
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_reduce_beta(BatchArgs<T> a, int max_iter) {
-    const uint32_t ks = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    if (ks >= *a.n_act_sys) return;
-    const uint32_t s = a.act_sys[ks];
-    if (a.done[s]) return;
-    const T rho_new = sys_sum(a, a.part_rh, s);
-    const T err = sys_sum(a, a.part_rr, s);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        const int it = a.iters[s] + 1;
-        a.iters[s] = it;
-        a.err[s] = err;
-        if (!(err > a.thresh[s]) || it >= max_iter) {
-            a.done[s] = 1;
-        } else {
-            a.beta[s] = rho_new / a.rho[s];
-            a.rho[s] = rho_new;
+    uint32_t k0, k1;
+    tile_share(*a.n_act_tiles, k0, k1);
+    const int lane = threadIdx.x & (kWave - 1);
+    uint32_t cs = ~0u;
+    T rho = T(0), alpha = T(0);
+    for (uint32_t kb = k0; kb < k1; kb += kWave) {
+        const TileDesc mine = tile_desc_load(a, kb + uint32_t(lane), k1, false);
+        const int nb = int(k1 - kb < uint32_t(kWave) ? k1 - kb : uint32_t(kWave));
+        for (int j = 0; j < nb; ++j) {
+            const TileDesc td = tile_desc_at(mine, j);
+            if (td.flags & 1u) continue;
+            if (td.s != cs) {
+                cs = td.s;
+                rho = a.rho[td.s];
+                alpha = rho / tiles_sum(a.part_dq, td.to0, td.to1);
+            }
+            T rv[kTileRowsPerThread], qv[kTileRowsPerThread], pv[kTileRowsPerThread];
+#pragma unroll
+            for (int q = 0; q < kTileRowsPerThread; ++q) {
+                const uint32_t i = td.r0 + threadIdx.x + uint32_t(q * kBlock);
+                if (i < td.r1) {
+                    rv[q] = a.r[i];
+                    qv[q] = a.q[i];
+                    if (PREC) pv[q] = a.dinv[i];
+                }
+            }
+            T prh = T(0), prr = T(0);
+#pragma unroll
+            for (int q = 0; q < kTileRowsPerThread; ++q) {
+                const uint32_t i = td.r0 + threadIdx.x + uint32_t(q * kBlock);
+                if (i < td.r1) {
+                    const T ri = add_rn(rv[q], mul_rn(-alpha, qv[q]));
+                    const T hi = PREC ? mul_rn(pv[q], ri) : ri;
+                    a.r[i] = ri;
+                    a.h[i] = hi;
+                    prh += ri * hi;
+                    prr += ri * ri;
+                }
+            }
+            prh = block_sum_all(prh, lds);
+            prr = block_sum_all(prr, lds);
+            if (threadIdx.x == 0) {
+                a.part_rh[td.t] = prh;
+                a.part_rr[td.t] = prr;
+                if (td.t == td.to0) {
+                    a.alpha[td.s] = alpha;
+                    a.rho_prev[td.s] = rho;
+                    a.iters[td.s] = a.iters[td.s] + 1;
+                }
+            }
         }
     }
 }
@@ -381,13 +538,12 @@ hipError_t batch_start(const BatchArgs<T>& a, bool precond, T eps, int max_iter,
 template <typename T>
 hipError_t batch_iteration(const BatchArgs<T>& a, bool precond, int max_iter, uint32_t grid,
                            hipStream_t s) {
-    const uint32_t sg = (a.nsys + kWavesPerBlock - 1) / kWavesPerBlock;
-    k_b_dir<T><<<grid, kBlock, 0, s>>>(a);
-    k_b_spmv<T><<<grid, kBlock, 0, s>>>(a);
-    k_b_reduce_alpha<T><<<sg, kBlock, 0, s>>>(a);
+    k_b_dir<T><<<grid, kBlock, 0, s>>>(a, max_iter);
+    if (a.all_blocks && a.bci16) k_b_spmv<T, kFormBlock16><<<grid, kBlock, 0, s>>>(a);
+    else if (a.all_blocks) k_b_spmv<T, kFormBlock32><<<grid, kBlock, 0, s>>>(a);
+    else k_b_spmv<T, kFormAny><<<grid, kBlock, 0, s>>>(a);
     if (precond) k_b_update<T, true><<<grid, kBlock, 0, s>>>(a);
     else k_b_update<T, false><<<grid, kBlock, 0, s>>>(a);
-    k_b_reduce_beta<T><<<sg, kBlock, 0, s>>>(a, max_iter);
     return hipGetLastError();
 }
 

@@ -17,8 +17,8 @@ namespace nlhip {
 
 template <typename T>
 struct BatchWork {
-    DevBuf<T> x, b, r, h, d, q, part, sys;  // part: 4 x ntiles; sys: 6 x nsys
-    DevBuf<int32_t> iters, done, skip;
+    DevBuf<T> x, b, r, h, d, q, part, sys;  // part: 4 x ntiles; sys: 7 x nsys
+    DevBuf<int32_t> iters, done, stop, skip;
     DevBuf<uint32_t> lists, counts;         // act_tiles | act_sys ; 2 counters
     PinnedBuf<uint32_t> host_counts;        // two slots
     GraphLadder graph;
@@ -36,6 +36,7 @@ struct Batch {
     DevBuf<uint32_t> segb;
     // 2x2 block form of the systems that have it (empty when none has)
     DevBuf<uint32_t> sys_pair0, brp, bcol, bsegb;
+    bool all_blocks = false;  // every system has the block form
     DevBuf<uint16_t> bci16;
     DevBuf<double> bval;
     DevBuf<float> bval32;
@@ -62,9 +63,10 @@ void work_alloc(Batch& B, BatchWork<T>& w) {
     w.d.alloc(n);
     w.q.alloc(n);
     w.part.alloc(4 * size_t(B.ntiles));
-    w.sys.alloc(6 * size_t(B.nsys));
+    w.sys.alloc(7 * size_t(B.nsys));
     w.iters.alloc(B.nsys);
     w.done.alloc(B.nsys);
+    w.stop.alloc(B.nsys);
     w.skip.alloc(B.nsys);
     w.lists.alloc(size_t(B.ntiles) + B.nsys);
     w.counts.alloc(2);
@@ -82,6 +84,7 @@ void work_alloc(Batch& B, BatchWork<T>& w) {
     a.ci16 = B.ci16.get();
     a.segb = B.segb.get();
     a.sys_pair0 = B.sys_pair0.get();
+    a.all_blocks = B.all_blocks ? 1u : 0u;
     a.brp = B.brp.get();
     a.bcol = B.bcol.get();
     a.bci16 = B.bci16.get();
@@ -104,8 +107,10 @@ void work_alloc(Batch& B, BatchWork<T>& w) {
     a.err = w.sys.get() + 3 * size_t(B.nsys);
     a.thresh = w.sys.get() + 4 * size_t(B.nsys);
     a.bb = w.sys.get() + 5 * size_t(B.nsys);
+    a.rho_prev = w.sys.get() + 6 * size_t(B.nsys);
     a.iters = w.iters.get();
     a.done = w.done.get();
+    a.stop = w.stop.get();
     a.skip = nullptr;
     a.act_tiles = w.lists.get();
     a.act_sys = w.lists.get() + B.ntiles;
@@ -351,6 +356,7 @@ void build_batch_blocks(Batch& B, const uint32_t* row_off, const uint32_t* rowpt
             np += (row_off[s + 1] - row_off[s]) / 2;
         }
     if (np == 0) return;
+    B.all_blocks = std::all_of(blk.begin(), blk.begin() + nsys, [](int32_t v) { return v != 0; });
     std::vector<uint32_t> h_brp(size_t(np) + 1, 0);
     for (uint32_t s = 0; s < nsys; ++s)
         if (blk[s])
@@ -456,7 +462,12 @@ Batch* batch_create(uint32_t nsys, const uint32_t* row_off, const uint32_t* rowp
     copy_in(B->tile_off.get(), tile_off.data(), tile_off.size() * 4, st);
     copy_in(B->tile_sys.get(), tile_sys.data(), tile_sys.size() * 4, st);
     copy_in(B->tile_row0.get(), tile_row0.data(), tile_row0.size() * 4, st);
-    B->grid = uint32_t(std::max(4, device().cu_count * 4));
+    // 64 workgroups per CU, each taking a contiguous share of the active
+    // tiles: several times what fits on a CU at once, so the dispatcher
+    // balances shares of unequal cost; fewer when the batch has fewer tiles
+    // (a small batch, e.g. one served request, launches no idle workgroups)
+    B->grid = uint32_t(std::max<uint64_t>(
+        1, std::min<uint64_t>(uint64_t(std::max(4, device().cu_count * 64)), B->ntiles)));
     if (flags & NLHIP_SYSTEM_FP32) ensure_fp32(*B);
     work_alloc(*B, B->w64);
     B->stream.sync();

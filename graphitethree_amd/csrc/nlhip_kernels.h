@@ -155,8 +155,13 @@ hipError_t cg_step_part(const CGArgs<T>& a, int rpw, bool precond, int part, hip
 
 // ---- batched systems (k_batch.hip) ----------------------------------------
 
-constexpr uint32_t kTileRows = 256;  // rows per batch tile: 4 wavefronts x 64 rows
-constexpr uint32_t kBlockTileRows = 512;  // with 2x2 blocks: 4 wavefronts x 64 block rows
+// A batch tile is what one workgroup (4 wavefronts) takes in one step:
+// each wavefront streams kTileWaveRows consecutive rows, or block rows for a
+// system in 2x2 block form (longer runs keep a wavefront's loads pipelined).
+constexpr uint32_t kTileDepth = 4;
+constexpr uint32_t kTileWaveRows = 64 * kTileDepth;
+constexpr uint32_t kTileRows = 4 * kTileWaveRows;       // scalar form
+constexpr uint32_t kBlockTileRows = 2 * kTileRows;      // 2x2 block form
 constexpr uint32_t kNoBlocks = 0xffffffffu;
 
 template <typename T>
@@ -179,6 +184,7 @@ struct BatchArgs {
     // block rows; block columns as 16-bit offsets (bci16, bsegb) or, when
     // they do not fit, 32-bit (bcol)
     const uint32_t* sys_pair0;
+    uint32_t all_blocks;        // every system has the block form
     const uint32_t* brp;
     const uint32_t* bcol;
     const uint16_t* bci16;
@@ -196,7 +202,8 @@ struct BatchArgs {
     T* part_rh;
     T* part_rr;
     T* part_bb;
-    T* rho;                     // per system
+    T* rho;                     // per system: r.h of the current iteration
+    T* rho_prev;                //   r.h of the previous one
     T* alpha;
     T* beta;
     T* err;
@@ -204,6 +211,7 @@ struct BatchArgs {
     T* bb;
     int32_t* iters;
     int32_t* done;
+    int32_t* stop;              // stopping test held (turned into done next launch)
     const int32_t* skip;        // per system, may be null: start as done
     uint32_t* act_tiles;
     uint32_t* n_act_tiles;
