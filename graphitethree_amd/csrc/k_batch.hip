@@ -46,11 +46,12 @@ __device__ __forceinline__ bool tile_blk(const BatchArgs<T>& a, uint32_t t) {
     return a.sys_pair0 && a.sys_pair0[a.tile_sys[t]] != kNoBlocks;
 }
 
-// Stream SpMV of rows [r0, r1) of system s (first row lo, first block row
-// p0, or kNoBlocks for the scalar form): wavefront w takes kTileWaveRows
-// rows from r0 + w kTileWaveRows or, in 2x2 block form, kTileWaveRows
-// block rows (row pairs) from row r0 + 2 w kTileWaveRows (r0 - lo and
-// r1 - lo are even for such a system).
+// Stream SpMV of passes [first, first + R) of a tile with rows [r0, r1) of
+// system s (first row lo, first block row p0, or kNoBlocks for the scalar
+// form), by one wavefront: pass i covers kTilePassRows rows from
+// r0 + i kTilePassRows or, in 2x2 block form, kTilePassRows block rows
+// (row pairs) from row r0 + 2 i kTilePassRows (r0 - lo and r1 - lo are even
+// for such a system).  Returns the first row of the run.
 //
 // F (kFormAny, kFormBlock16, kFormBlock32) says which forms the kernel is
 // built for: any, or only the 2x2 block form with 16-bit or 32-bit block
@@ -59,18 +60,18 @@ __device__ __forceinline__ bool tile_blk(const BatchArgs<T>& a, uint32_t t) {
 constexpr int kFormAny = 0, kFormBlock16 = 1, kFormBlock32 = 2;
 
 template <typename T, int F = kFormAny, typename Epi>
-__device__ __forceinline__ void tile_spmv_rows(const BatchArgs<T>& a, uint32_t r0, uint32_t r1,
-                                               uint32_t lo, uint32_t p0, const T* x, T* stage,
-                                               Epi& epi) {
+__device__ __forceinline__ void wave_spmv_rows(const BatchArgs<T>& a, uint32_t r0, uint32_t r1,
+                                               uint32_t lo, uint32_t p0, uint32_t first,
+                                               uint32_t R, const T* x, T* stage, Epi& epi) {
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     if (F != kFormAny || p0 != kNoBlocks) {
         constexpr int EB = F == kFormAny ? kBatchEB : kBatchEBBlock;
         static_assert(EB * 4 * 64 * kWavesPerBlock <= kBatchStage, "staging area too small");
         const int32_t delta = int32_t(lo - 2u * p0);
-        const uint32_t B0 = p0 + (r0 - lo) / 2 + kTileWaveRows * uint32_t(w);
+        const uint32_t B0 = p0 + (r0 - lo) / 2 + kTilePassRows * first;
         const uint32_t Bend = p0 + (r1 - lo) / 2;
         if (B0 >= Bend) return;
-        const uint32_t B1 = (B0 + kTileWaveRows < Bend) ? B0 + kTileWaveRows : Bend;
+        const uint32_t B1 = (B0 + kTilePassRows * R < Bend) ? B0 + kTilePassRows * R : Bend;
         const Cols bcols{a.bcol, a.bci16, a.bsegb};
         T* st = stage + w * (64 * EB * 4);
         if (F == kFormBlock16 || (F == kFormAny && a.bci16))
@@ -82,9 +83,9 @@ __device__ __forceinline__ void tile_spmv_rows(const BatchArgs<T>& a, uint32_t r
         return;
     }
     if constexpr (F == kFormAny) {
-        const uint32_t R0 = r0 + kTileWaveRows * uint32_t(w);
+        const uint32_t R0 = r0 + kTilePassRows * first;
         if (R0 >= r1) return;
-        const uint32_t R1 = (R0 + kTileWaveRows < r1) ? R0 + kTileWaveRows : r1;
+        const uint32_t R1 = (R0 + kTilePassRows * R < r1) ? R0 + kTilePassRows * R : r1;
         const Cols cols{a.ci, a.ci16, a.segb};
         if (a.ci16)
             StreamEngine<kStreamE, true, true>::run_rows(R0, R1, a.rp, cols, a.val, x,
@@ -95,6 +96,14 @@ __device__ __forceinline__ void tile_spmv_rows(const BatchArgs<T>& a, uint32_t r
     }
 }
 
+// The whole tile by the workgroup: wavefront w takes pass w.
+template <typename T, int F = kFormAny, typename Epi>
+__device__ __forceinline__ void tile_spmv_rows(const BatchArgs<T>& a, uint32_t r0, uint32_t r1,
+                                               uint32_t lo, uint32_t p0, const T* x, T* stage,
+                                               Epi& epi) {
+    wave_spmv_rows<T, F>(a, r0, r1, lo, p0, threadIdx.x / kWave, 1u, x, stage, epi);
+}
+
 template <typename T, typename Epi>
 __device__ __forceinline__ void tile_spmv(const BatchArgs<T>& a, uint32_t t, uint32_t r0,
                                           uint32_t r1, bool blk, const T* x, T* stage,
@@ -102,6 +111,9 @@ __device__ __forceinline__ void tile_spmv(const BatchArgs<T>& a, uint32_t t, uin
     const uint32_t s = a.tile_sys[t];
     tile_spmv_rows(a, r0, r1, a.row_off[s], blk ? a.sys_pair0[s] : kNoBlocks, x, stage, epi);
 }
+
+__device__ __forceinline__ double fma_rn(double a, double b, double c) { return __fma_rn(a, b, c); }
+__device__ __forceinline__ float fma_rn(float a, float b, float c) { return __fmaf_rn(a, b, c); }
 
 // Metadata of one entry of the active-tile list.  In the iteration kernels
 // lane j of every wavefront loads the entry k + j, so a workgroup fetches
@@ -324,32 +336,77 @@ __global__ __launch_bounds__(kBlock) void k_b_dir(BatchArgs<T> a, int max_iter) 
     }
 }
 
+// SpMV of the active tiles.  A wavefront streams R consecutive passes of one
+// tile (R = 1, 2 or 4, chosen per launch from the number of active tiles):
+// long runs keep a wavefront's loads in flight when tiles are plentiful,
+// R = 1 spreads few tiles over all wavefronts.  4 / R wavefronts share a
+// tile and a workgroup step covers R tiles.  Whatever R, a tile's d.q
+// partial is formed in one order: per pass the sum over its lanes (each
+// lane summing its rows in order), then the 4 pass sums in pass order, as
+// block_sum_all does when wavefront w takes pass w.
 template <typename T, int F>
 __global__ __launch_bounds__(kBlock) void k_b_spmv(BatchArgs<T> a) {
     __shared__ T stage[kBatchStage];
-    __shared__ T lds[kWavesPerBlock];
-    uint32_t k0, k1;
-    tile_share(*a.n_act_tiles, k0, k1);
+    __shared__ T pass_sum[kWavesPerBlock][4];  // [tile slot][pass]
+    const uint32_t n = *a.n_act_tiles;
+    const uint32_t R = n >= a.spmv_r4_tiles ? 4u : (n >= a.spmv_r2_tiles ? 2u : 1u);
     const int lane = threadIdx.x & (kWave - 1);
+    // wavefront index, known to the compiler as uniform across the wavefront
+    const uint32_t w = uint32_t(__builtin_amdgcn_readfirstlane(int(threadIdx.x / kWave)));
+    const uint32_t wpt = 4u / R;  // wavefronts per tile
+    const uint32_t slot = w / wpt, first = (w % wpt) * R;
+    // contiguous share of whole steps (R tiles each)
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    const uint64_t steps = (uint64_t(n) + R - 1) / R;
+    const uint32_t k0 = uint32_t(steps * L / gridDim.x) * R;
+    const uint32_t k1e = uint32_t(steps * (L + 1) / gridDim.x) * R;
+    const uint32_t k1 = k1e < n ? k1e : n;
     for (uint32_t kb = k0; kb < k1; kb += kWave) {
         const TileDesc mine = tile_desc_load(a, kb + uint32_t(lane), k1, true);
         const int nb = int(k1 - kb < uint32_t(kWave) ? k1 - kb : uint32_t(kWave));
-        for (int j = 0; j < nb; ++j) {
-            const TileDesc td = tile_desc_at(mine, j);
-            if (td.flags & 1u) continue;
-            if (td.flags & 2u) {
-                if (td.t == td.to0 && threadIdx.x == 0) a.done[td.s] = 1;
-                continue;
+        for (int j = 0; j < nb; j += int(R)) {
+            const bool have = j + int(slot) < nb;
+            TileDesc td{};
+            if (have) td = tile_desc_at(mine, j + int(slot));
+            const bool work = have && (td.flags & 3u) == 0u;
+            if (have && td.flags == 2u && td.t == td.to0 && first == 0 && lane == 0)
+                a.done[td.s] = 1;
+            T acc0 = T(0), acc1 = T(0), acc2 = T(0), acc3 = T(0);
+            if (work) {
+                const bool blk = F != kFormAny || td.p0 != kNoBlocks;
+                const uint32_t base = td.r0 + (blk ? 2u : 1u) * kTilePassRows * first;
+                const uint32_t shift = blk ? 7u : 6u;  // log2 of rows per pass
+                auto epi = prefetch_epi([&](uint32_t row) { return a.d[row]; },
+                                        [&](uint32_t row, T v, T dr) {
+                                            a.q[row] = v;
+                                            // the lane's own pass accumulator
+                                            // (fma with 0 for the others
+                                            // leaves them unchanged)
+                                            const uint32_t pp = (row - base) >> shift;
+                                            acc0 = fma_rn(dr, pp == 0 ? v : T(0), acc0);
+                                            acc1 = fma_rn(dr, pp == 1 ? v : T(0), acc1);
+                                            acc2 = fma_rn(dr, pp == 2 ? v : T(0), acc2);
+                                            acc3 = fma_rn(dr, pp == 3 ? v : T(0), acc3);
+                                        });
+                wave_spmv_rows<T, F>(a, td.r0, td.r1, td.lo, td.p0, first, R, a.d, stage, epi);
             }
-            T pdq = T(0);
-            auto epi = prefetch_epi([&](uint32_t row) { return a.d[row]; },
-                                    [&](uint32_t row, T v, T dr) {
-                                        a.q[row] = v;
-                                        pdq += dr * v;
-                                    });
-            tile_spmv_rows<T, F>(a, td.r0, td.r1, td.lo, td.p0, a.d, stage, epi);
-            pdq = block_sum_all(pdq, lds);
-            if (threadIdx.x == 0) a.part_dq[td.t] = pdq;
+            acc0 = group_sum<kWave>(acc0);
+            if (R > 1) acc1 = group_sum<kWave>(acc1);
+            if (R == 4) {
+                acc2 = group_sum<kWave>(acc2);
+                acc3 = group_sum<kWave>(acc3);
+                if (work && lane == 0) a.part_dq[td.t] = ((acc0 + acc1) + acc2) + acc3;
+            } else {
+                if (work && lane == 0) {
+                    pass_sum[slot][first] = acc0;
+                    if (R == 2) pass_sum[slot][first + 1] = acc1;
+                }
+                __syncthreads();
+                if (work && first == 0 && lane == 0)
+                    a.part_dq[td.t] = ((pass_sum[slot][0] + pass_sum[slot][1]) +
+                                       pass_sum[slot][2]) + pass_sum[slot][3];
+                __syncthreads();
+            }
         }
     }
 }

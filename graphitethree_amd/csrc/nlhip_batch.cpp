@@ -52,6 +52,9 @@ struct Batch {
 
 namespace {
 
+constexpr uint32_t kSpmvR2PerCu = 16;
+constexpr uint32_t kSpmvR4PerCu = 32;
+
 template <typename T>
 void work_alloc(Batch& B, BatchWork<T>& w) {
     if (w.x.get()) return;
@@ -85,6 +88,12 @@ void work_alloc(Batch& B, BatchWork<T>& w) {
     a.segb = B.segb.get();
     a.sys_pair0 = B.sys_pair0.get();
     a.all_blocks = B.all_blocks ? 1u : 0u;
+    // SpMV run length: with R passes per wavefront there are 4 n / R
+    // wavefront tasks for n active tiles; longer runs once that is well
+    // above the wavefronts a launch keeps resident (about 16 per CU)
+    const uint32_t cu = uint32_t(std::max(1, device().cu_count));
+    a.spmv_r2_tiles = kSpmvR2PerCu * cu;
+    a.spmv_r4_tiles = kSpmvR4PerCu * cu;
     a.brp = B.brp.get();
     a.bcol = B.bcol.get();
     a.bci16 = B.bci16.get();

@@ -155,12 +155,10 @@ hipError_t cg_step_part(const CGArgs<T>& a, int rpw, bool precond, int part, hip
 
 // ---- batched systems (k_batch.hip) ----------------------------------------
 
-// A batch tile is what one workgroup (4 wavefronts) takes in one step:
-// each wavefront streams kTileWaveRows consecutive rows, or block rows for a
-// system in 2x2 block form (longer runs keep a wavefront's loads pipelined).
-constexpr uint32_t kTileDepth = 4;
-constexpr uint32_t kTileWaveRows = 64 * kTileDepth;
-constexpr uint32_t kTileRows = 4 * kTileWaveRows;       // scalar form
+// A batch tile is 4 passes of 64 rows, or of 64 block rows (row pairs) for
+// a system in 2x2 block form; its partial sums are formed pass by pass.
+constexpr uint32_t kTilePassRows = 64;
+constexpr uint32_t kTileRows = 4 * kTilePassRows;       // scalar form
 constexpr uint32_t kBlockTileRows = 2 * kTileRows;      // 2x2 block form
 constexpr uint32_t kNoBlocks = 0xffffffffu;
 
@@ -185,6 +183,8 @@ struct BatchArgs {
     // they do not fit, 32-bit (bcol)
     const uint32_t* sys_pair0;
     uint32_t all_blocks;        // every system has the block form
+    uint32_t spmv_r2_tiles;     // active tiles from which SpMV wavefronts
+    uint32_t spmv_r4_tiles;     //   stream 2 (4) passes of a tile
     const uint32_t* brp;
     const uint32_t* bcol;
     const uint16_t* bci16;
