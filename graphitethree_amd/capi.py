@@ -183,6 +183,7 @@ SIGNATURES = {
     "nlhipBatchDestroy": (C.c_int, [_vp]),
     "nlhipBatchInfo": (C.c_int, [_vp, _u32p, _u32p, C.POINTER(C.c_uint64), _u32p]),
     "nlhipBatchSetLaunch": (C.c_int, [_vp, C.c_uint32]),
+    "nlhipBatchSetSpmvPasses": (C.c_int, [_vp, C.c_uint32]),
     "nlhipBatchSolve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
                                   C.POINTER(SolveStats), C.POINTER(SolveStats)]),
     "nlhipServerConfigDefault": (None, [C.POINTER(ServerConfig)]),
@@ -384,6 +385,10 @@ class Batch:
 
     def set_launch(self, workgroups: int):
         check(lib().nlhipBatchSetLaunch(self._h, workgroups), "nlhipBatchSetLaunch")
+
+    def set_spmv_passes(self, passes: int):
+        """SpMV passes per wavefront: 1, 2, 4, or 0 (by active tiles)."""
+        check(lib().nlhipBatchSetSpmvPasses(self._h, passes), "nlhipBatchSetSpmvPasses")
 
     def split(self, v):
         return [v[self.row_off[s]:self.row_off[s + 1]] for s in range(self.nsys)]

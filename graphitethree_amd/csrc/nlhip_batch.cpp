@@ -44,6 +44,7 @@ struct Batch {
     DevBuf<float> val32, dinv32;
     std::vector<uint32_t> h_row_off;
     uint32_t grid = 1;
+    uint32_t spmv_passes = 0;  // 0: by active tiles, else 1, 2 or 4
     Stream stream;
     std::mutex mu;
     BatchWork<double> w64;
@@ -54,6 +55,19 @@ namespace {
 
 constexpr uint32_t kSpmvR2PerCu = 16;
 constexpr uint32_t kSpmvR4PerCu = 32;
+
+// SpMV run length: with R passes per wavefront there are 4 n / R wavefront
+// tasks for n active tiles; longer runs once that is well above the
+// wavefronts a launch keeps resident (about 16 per CU).  A forced length
+// (nlhipBatchSetSpmvPasses) applies from any number of tiles.
+template <typename T>
+void set_spmv_runs(const Batch& B, BatchArgs<T>& a) {
+    const uint32_t cu = uint32_t(std::max(1, device().cu_count));
+    constexpr uint32_t kNever = 0xffffffffu;
+    const uint32_t p = B.spmv_passes;
+    a.spmv_r2_tiles = p == 0 ? kSpmvR2PerCu * cu : (p >= 2 ? 0u : kNever);
+    a.spmv_r4_tiles = p == 0 ? kSpmvR4PerCu * cu : (p == 4 ? 0u : kNever);
+}
 
 template <typename T>
 void work_alloc(Batch& B, BatchWork<T>& w) {
@@ -88,12 +102,7 @@ void work_alloc(Batch& B, BatchWork<T>& w) {
     a.segb = B.segb.get();
     a.sys_pair0 = B.sys_pair0.get();
     a.all_blocks = B.all_blocks ? 1u : 0u;
-    // SpMV run length: with R passes per wavefront there are 4 n / R
-    // wavefront tasks for n active tiles; longer runs once that is well
-    // above the wavefronts a launch keeps resident (about 16 per CU)
-    const uint32_t cu = uint32_t(std::max(1, device().cu_count));
-    a.spmv_r2_tiles = kSpmvR2PerCu * cu;
-    a.spmv_r4_tiles = kSpmvR4PerCu * cu;
+    set_spmv_runs(B, a);
     a.brp = B.brp.get();
     a.bcol = B.bcol.get();
     a.bci16 = B.bci16.get();
@@ -533,6 +542,20 @@ NLAPI int NLAPIENTRY nlhipBatchSetLaunch(nlhip_batch b, uint32_t workgroups) {
             throw Error(NLHIP_ERR_INVALID, "workgroups must be in [1, 65536]");
         std::lock_guard<std::mutex> lock(B->mu);
         B->grid = workgroups;
+        B->w64.graph.reset();
+        B->w32.graph.reset();
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipBatchSetSpmvPasses(nlhip_batch b, uint32_t passes) {
+    return guarded("nlhipBatchSetSpmvPasses", [&] {
+        Batch* B = bat(b);
+        if (passes != 0 && passes != 1 && passes != 2 && passes != 4)
+            throw Error(NLHIP_ERR_INVALID, "passes must be 0, 1, 2 or 4");
+        std::lock_guard<std::mutex> lock(B->mu);
+        B->spmv_passes = passes;
+        set_spmv_runs(*B, B->w64.args);
+        set_spmv_runs(*B, B->w32.args);
         B->w64.graph.reset();
         B->w32.graph.reset();
     });

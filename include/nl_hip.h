@@ -156,8 +156,12 @@ NLAPI int NLAPIENTRY nlhipBatchCreate(uint32_t nsys, const uint32_t* row_off,
 NLAPI int NLAPIENTRY nlhipBatchDestroy(nlhip_batch b);
 NLAPI int NLAPIENTRY nlhipBatchInfo(nlhip_batch b, uint32_t* nsys, uint32_t* n,
                                     uint64_t* nnz, uint32_t* tiles);
-/* Workgroups of the persistent grid that walks the active tiles. */
+/* Workgroups of the grid that walks the active tiles. */
 NLAPI int NLAPIENTRY nlhipBatchSetLaunch(nlhip_batch b, uint32_t workgroups);
+/* Passes of a tile (64 rows or block rows each, 4 per tile) that one
+ * wavefront of the SpMV streams: 1, 2 or 4, or 0 (the default) to choose
+ * per launch from the number of active tiles.  Results do not depend on it. */
+NLAPI int NLAPIENTRY nlhipBatchSetSpmvPasses(nlhip_batch b, uint32_t passes);
 /* Solves every system; b and x are the concatenated vectors (x holds the
  * initial guesses).  per_system (nsys entries) and total may be null; in
  * total, iterations is the largest count and outer_iterations the sum. */

@@ -98,6 +98,25 @@ def test_batch_launch_shape_does_not_change_bits(hip, small_atlas):
         assert np.array_equal(x0, x1), wg
 
 
+def test_batch_spmv_passes_do_not_change_bits(hip, small_atlas):
+    """Tile partials are formed pass by pass, so the SpMV run length per
+    wavefront (1, 2 or 4 passes, normally chosen by the number of active
+    tiles) changes no bit, in FP64 and in the FP32 inner solves."""
+    A = W.atlas(6, seed=5, nmin=2000, nmax=20000)
+    B = capi.Batch(packed=(A.row_off, A.rowptr, A.colind, A.val), fp32=True)
+    x0, p0, _ = B.solve(A.b)
+    xm0, _, _ = B.solve(A.b, precision="mixed", max_iter=200000, inner_eps=1e-4)
+    for passes in (1, 2, 4, 0):
+        B.set_spmv_passes(passes)
+        x1, p1, _ = B.solve(A.b)
+        assert np.array_equal(x0, x1), passes
+        assert [p.iterations for p in p0] == [p.iterations for p in p1]
+        xm1, _, _ = B.solve(A.b, precision="mixed", max_iter=200000, inner_eps=1e-4)
+        assert np.array_equal(xm0, xm1), passes
+    with pytest.raises(capi.NLHIPError):
+        B.set_spmv_passes(3)
+
+
 def test_batch_mixed_precision(hip, small_atlas):
     A = small_atlas
     B = capi.Batch([A.chart(c) for c in range(A.count)], fp32=True)
