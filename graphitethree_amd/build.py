@@ -22,17 +22,22 @@ OBJ = os.path.join(ROOT, "_build", "obj")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-# Packed-FP32 instructions are disabled for all device code.
-HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+# Packed-FP32 instructions are disabled for all device code.  No
+# floating-point contraction: hipcc's default would fuse a * b + c into one
+# FMA (HIP's __dmul_rn / __dadd_rn are plain operators), while the
+# element-wise updates and the assembly sums are specified as a separately
+# rounded multiply and add.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
              "-Wno-unused-command-line-argument", f"-I{INCLUDE}", f"-I{CSRC}"]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}",
               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
-NLHIP_HIP = ["k_spmv.hip", "k_blas.hip", "k_cg.hip", "k_batch.hip", "k_spmm.hip"]
+NLHIP_HIP = ["k_spmv.hip", "k_blas.hip", "k_cg.hip", "k_batch.hip", "k_spmm.hip",
+             "k_assemble.hip"]
 NLHIP_CPP = ["nl_api.cpp", "nlhip_device.cpp", "nlhip_solver.cpp",
-             "nlhip_batch.cpp", "nlhip_multi.cpp", "nlhip_server.cpp"]
+             "nlhip_batch.cpp", "nlhip_multi.cpp", "nlhip_server.cpp", "nlhip_assemble.cpp"]
 
 
 def _run(cmd):

@@ -166,6 +166,10 @@ SIGNATURES = {
     "nlhipSystemCreate": (C.c_int, [C.c_uint32, _vp, _vp, _vp, C.c_uint32,
                                     C.POINTER(_vp)]),
     "nlhipSystemDestroy": (C.c_int, [_vp]),
+    "nlhipSystemGetCRS": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "nlhipLeastSquaresCreate": (C.c_int, [C.c_uint32, C.c_uint32, _vp, _vp, _vp, C.c_uint32,
+                                          _vp, _vp, _vp, C.c_uint32, C.POINTER(_vp), _vp,
+                                          _u32p]),
     "nlhipSystemInfo": (C.c_int, [_vp, _u32p, C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)]),
     "nlhipSystemSetLaunch": (C.c_int, [_vp, C.c_int, C.c_uint32]),
@@ -274,6 +278,50 @@ class System:
     def from_crs(cls, S, fp32: bool = False, offsets: bool = True,
                  blocks: bool = True) -> "System":
         return cls(S.rowptr, S.colind, S.val, fp32=fp32, offsets=offsets, blocks=blocks)
+
+    @classmethod
+    def least_squares(cls, nb_variables: int, row_ptr, var, coef, row_rhs=None, locked=None,
+                      values=None, fp32: bool = False):
+        """Normal equations of the rows assembled on the device
+        (nlhipLeastSquaresCreate).  row_rhs: (k, rows) or (rows,); values:
+        (k, nb_variables) or (nb_variables,).  Returns (system, rhs (k, nfree))."""
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        var = np.ascontiguousarray(var, dtype=np.uint32)
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        nrows = len(row_ptr) - 1
+        k = 1
+        if row_rhs is not None:
+            row_rhs = np.ascontiguousarray(np.atleast_2d(row_rhs), dtype=np.float64)
+            k = row_rhs.shape[0]
+        if values is not None:
+            values = np.ascontiguousarray(np.atleast_2d(values), dtype=np.float64)
+            k = max(k, values.shape[0])
+        if locked is not None:
+            locked = np.ascontiguousarray(locked, dtype=np.uint8)
+        nfree_locked = nb_variables - (int(np.count_nonzero(locked)) if locked is not None else 0)
+        rhs = np.zeros((k, max(nfree_locked, 1)))
+        h = C.c_void_p()
+        nfree = C.c_uint32()
+        flags = NLHIP_SYSTEM_FP32 if fp32 else 0
+        check(lib().nlhipLeastSquaresCreate(
+            nb_variables, nrows, ptr(row_ptr), ptr(var), ptr(coef), k,
+            ptr(row_rhs) if row_rhs is not None else None,
+            ptr(locked) if locked is not None else None,
+            ptr(values) if values is not None else None, flags, C.byref(h), ptr(rhs),
+            C.byref(nfree)), "nlhipLeastSquaresCreate")
+        S = cls.__new__(cls)
+        S._h = h
+        S.n = int(nfree.value)
+        return S, rhs[:, :S.n]
+
+    def crs(self):
+        """The resident CRS (rowptr, colind, val) copied back."""
+        n, nnz, _ = self.info()
+        rp = np.empty(n + 1, dtype=np.uint32)
+        ci = np.empty(max(nnz, 1), dtype=np.uint32)
+        va = np.empty(max(nnz, 1), dtype=np.float64)
+        check(lib().nlhipSystemGetCRS(self._h, ptr(rp), ptr(ci), ptr(va)), "nlhipSystemGetCRS")
+        return rp, ci[:nnz], va[:nnz]
 
     @property
     def handle(self):

@@ -65,8 +65,10 @@ __device__ __forceinline__ T ntload(const T* p) {
     return __builtin_nontemporal_load(p);
 }
 
-// Separately rounded multiply and add (no contraction), so element-wise
-// updates round exactly like a scalar x = x + (a * y) loop.
+// Separately rounded multiply and add, so element-wise updates round exactly
+// like a scalar x = x + (a * y) loop.  HIP's __dmul_rn / __dadd_rn are plain
+// operators, so this holds because device code is compiled with
+// -ffp-contract=off (no multiply-add fusion anywhere).
 __device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
 __device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
 __device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }

@@ -112,9 +112,6 @@ __device__ __forceinline__ void tile_spmv(const BatchArgs<T>& a, uint32_t t, uin
     tile_spmv_rows(a, r0, r1, a.row_off[s], blk ? a.sys_pair0[s] : kNoBlocks, x, stage, epi);
 }
 
-__device__ __forceinline__ double fma_rn(double a, double b, double c) { return __fma_rn(a, b, c); }
-__device__ __forceinline__ float fma_rn(float a, float b, float c) { return __fmaf_rn(a, b, c); }
-
 // Metadata of one entry of the active-tile list.  In the iteration kernels
 // lane j of every wavefront loads the entry k + j, so a workgroup fetches
 // the metadata of 64 tiles in three dependent loads instead of three per
@@ -379,14 +376,15 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(BatchArgs<T> a) {
                 auto epi = prefetch_epi([&](uint32_t row) { return a.d[row]; },
                                         [&](uint32_t row, T v, T dr) {
                                             a.q[row] = v;
-                                            // the lane's own pass accumulator
-                                            // (fma with 0 for the others
-                                            // leaves them unchanged)
+                                            // the accumulator of the row's pass
+                                            // (selects, not branches, keep the
+                                            // four in registers)
                                             const uint32_t pp = (row - base) >> shift;
-                                            acc0 = fma_rn(dr, pp == 0 ? v : T(0), acc0);
-                                            acc1 = fma_rn(dr, pp == 1 ? v : T(0), acc1);
-                                            acc2 = fma_rn(dr, pp == 2 ? v : T(0), acc2);
-                                            acc3 = fma_rn(dr, pp == 3 ? v : T(0), acc3);
+                                            const T p = dr * v;
+                                            acc0 = pp == 0 ? acc0 + p : acc0;
+                                            acc1 = pp == 1 ? acc1 + p : acc1;
+                                            acc2 = pp == 2 ? acc2 + p : acc2;
+                                            acc3 = pp == 3 ? acc3 + p : acc3;
                                         });
                 wave_spmv_rows<T, F>(a, td.r0, td.r1, td.lo, td.p0, first, R, a.d, stage, epi);
             }

@@ -581,6 +581,15 @@ struct Context {
     std::vector<std::pair<NLuint, double>> row;
     std::vector<double> row_rhs;
     double row_scaling = 1.0;
+    // Least-squares rows recorded for the device assembly (scaled as
+    // end_row scales them): entries over ls_rp, right-hand sides row-major
+    // (nb_systems per row).  ls_device is false once the matrix went the
+    // host way (an nlAddIJCoefficient in between); ls_sys is the assembled
+    // system, ls_nnz its nonzeros.
+    bool ls_device = false;
+    std::vector<uint32_t> ls_rp, ls_var;
+    std::vector<double> ls_coef, ls_rhs;
+    std::unique_ptr<System> ls_sys;
     NLuint used_iterations = 0;
     double error = 0.0;
     double elapsed = 0.0;
@@ -626,24 +635,22 @@ void begin_matrix(Context& C) {
     C.M = dyn_new(f, f);
     C.rhs.assign(size_t(f) * C.nb_systems, 0.0);
     C.current_row = 0;
+    C.ls_device = C.least_squares;
+    C.ls_rp.assign(1, 0);
+    C.ls_var.clear();
+    C.ls_coef.clear();
+    C.ls_rhs.clear();
+    C.ls_sys.reset();
 }
 
-// One row sum_k a_k x_{v_k} = rhs: locked variables move to the right-hand
-// side in coefficient order, then the least-squares normal equations receive
-// a_f1 a_f2 and a_f (rhs - locked part), or the row becomes row cur_row.
-void end_row(Context& C) {
-    double w = C.row_scaling;
-    if (C.normalize_rows) {
-        double s2 = 0.0;
-        for (auto& c : C.row) s2 += c.second * c.second;
-        if (s2 > 0.0) w = w / std::sqrt(s2);
-    }
-    std::vector<std::pair<NLuint, double>> row = C.row;
-    if (w != 1.0)
-        for (auto& c : row) c.second = c.second * w;
+// One row sum_k a_k x_{v_k} = rhs (coefficients and right-hand sides already
+// scaled): locked variables move to the right-hand side in coefficient
+// order, then the least-squares normal equations receive a_f1 a_f2 and
+// a_f (rhs - locked part), or the row becomes row cur_row.
+void host_row(Context& C, const std::vector<std::pair<NLuint, double>>& row, const double* rr) {
     std::vector<double> S(C.nb_systems);
     for (NLuint k = 0; k < C.nb_systems; ++k) {
-        double s = (w != 1.0) ? C.row_rhs[k] * w : C.row_rhs[k];
+        double s = rr[k];
         for (auto& c : row) {
             if (C.locked[c.first]) {
                 double p = c.second * C.values[size_t(k) * C.nb_variables + c.first];
@@ -680,7 +687,72 @@ void end_row(Context& C) {
         for (NLuint k = 0; k < C.nb_systems; ++k) C.rhs[size_t(k) * C.nb_free + r] = S[k];
         C.M->version++;
     }
+}
+
+// Row scaling (nlRowScaling, NL_NORMALIZE_ROWS), then the row is recorded
+// for the device assembly (least squares) or added on the host.
+void end_row(Context& C) {
+    double w = C.row_scaling;
+    if (C.normalize_rows) {
+        double s2 = 0.0;
+        for (auto& c : C.row) s2 += c.second * c.second;
+        if (s2 > 0.0) w = w / std::sqrt(s2);
+    }
+    std::vector<std::pair<NLuint, double>> row = C.row;
+    if (w != 1.0)
+        for (auto& c : row) c.second = c.second * w;
+    std::vector<double> rr(C.nb_systems);
+    for (NLuint k = 0; k < C.nb_systems; ++k)
+        rr[k] = (w != 1.0) ? C.row_rhs[k] * w : C.row_rhs[k];
+    if (C.least_squares && C.ls_device) {
+        if (C.ls_var.size() + row.size() > 0xffffffffull)
+            throw Error(NLHIP_ERR_UNSUPPORTED, "more than 2^32 - 1 row coefficients");
+        for (auto& c : row) {
+            C.ls_var.push_back(c.first);
+            C.ls_coef.push_back(c.second);
+        }
+        C.ls_rp.push_back(uint32_t(C.ls_var.size()));
+        C.ls_rhs.insert(C.ls_rhs.end(), rr.begin(), rr.end());
+        C.ls_sys.reset();
+    } else {
+        host_row(C, row, rr.data());
+    }
     ++C.current_row;
+}
+
+// The recorded rows go through the host accumulation instead (a matrix that
+// also receives nlAddIJCoefficient keeps OpenNL's order of additions).
+void ls_to_host(Context& C) {
+    if (!C.ls_device) return;
+    C.ls_device = false;
+    const size_t nrows = C.ls_rp.size() - 1;
+    std::vector<std::pair<NLuint, double>> row;
+    for (size_t r = 0; r < nrows; ++r) {
+        row.clear();
+        for (uint32_t e = C.ls_rp[r]; e < C.ls_rp[r + 1]; ++e)
+            row.emplace_back(C.ls_var[e], C.ls_coef[e]);
+        host_row(C, row, &C.ls_rhs[r * C.nb_systems]);
+    }
+    C.ls_rp.assign(1, 0);
+    C.ls_var.clear();
+    C.ls_coef.clear();
+    C.ls_rhs.clear();
+}
+
+// The normal equations of the recorded rows, assembled on the device; the
+// right-hand sides land in C.rhs.
+System& ls_system(Context& C) {
+    if (C.ls_sys) return *C.ls_sys;
+    const uint32_t nrows = uint32_t(C.ls_rp.size() - 1);
+    std::vector<double> rhs_t(size_t(C.nb_systems) * nrows);  // systems x rows
+    for (uint32_t r = 0; r < nrows; ++r)
+        for (NLuint k = 0; k < C.nb_systems; ++k)
+            rhs_t[size_t(k) * nrows + r] = C.ls_rhs[size_t(r) * C.nb_systems + k];
+    uint32_t nfree = 0;
+    C.ls_sys.reset(ls_create(C.nb_variables, nrows, C.ls_rp.data(), C.ls_var.data(),
+                             C.ls_coef.data(), C.nb_systems, rhs_t.data(), C.locked.data(),
+                             C.values.data(), 0, C.rhs.data(), &nfree));
+    return *C.ls_sys;
 }
 
 bool solve_context(Context& C) {
@@ -692,10 +764,17 @@ bool solve_context(Context& C) {
         C.error = 0.0;
         return true;
     }
-    std::vector<NLuint> rp, ci;
-    std::vector<double> va;
-    dyn_to_arrays(C.M, rp, ci, va);
-    std::unique_ptr<System> S(system_create(n, rp.data(), ci.data(), va.data(), 0));
+    std::unique_ptr<System> owned;
+    System* S = nullptr;
+    if (C.ls_device) {
+        S = &ls_system(C);
+    } else {
+        std::vector<NLuint> rp, ci;
+        std::vector<double> va;
+        dyn_to_arrays(C.M, rp, ci, va);
+        owned.reset(system_create(n, rp.data(), ci.data(), va.data(), 0));
+        S = owned.get();
+    }
     NLenum solver = C.solver;
     if (solver == NL_SOLVER_DEFAULT) solver = (C.least_squares || C.symmetric) ? NL_CG : NL_BICGSTAB;
     const NLuint max_iter = C.max_iterations ? C.max_iterations : 5 * n;
@@ -1138,7 +1217,10 @@ NLAPI void NLAPIENTRY nlGetIntegerv(NLenum pname, NLint* params) {
             case NL_USED_ITERATIONS: *params = NLint(C.used_iterations); break;
             case NL_PRECONDITIONER: *params = NLint(C.preconditioner); break;
             case NL_INNER_ITERATIONS: *params = NLint(C.inner_iterations); break;
-            case NL_NNZ: *params = C.M ? NLint(nlSparseMatrixNNZ(C.M)) : 0; break;
+            case NL_NNZ:
+                if (C.ls_device && C.nb_free) *params = NLint(ls_system(C).nnz);
+                else *params = C.M ? NLint(nlSparseMatrixNNZ(C.M)) : 0;
+                break;
             default: throw Error(NLHIP_ERR_INVALID, "unknown parameter");
         }
     });
@@ -1298,6 +1380,7 @@ NLAPI void NLAPIENTRY nlAddIJCoefficient(NLuint i, NLuint j, NLdouble value) {
             throw Error(NLHIP_ERR_INVALID, "nlAddIJCoefficient outside NL_MATRIX");
         check_var(C, i, "nlAddIJCoefficient");
         check_var(C, j, "nlAddIJCoefficient");
+        ls_to_host(C);
         const NLuint fi = C.free_index[i], fj = C.free_index[j];
         if (fi == ~NLuint(0)) return;
         if (fj == ~NLuint(0)) {

@@ -416,6 +416,39 @@ bool build_col_offsets(uint64_t nnz, const uint32_t* colind, hipStream_t s,
 // Validates and uploads; throws Error.
 System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
                       const double* val, uint32_t flags);
+// Takes over a device CRS built by this library (already valid: sorted
+// columns in range, finite values) and finishes the system as
+// system_create does.  Work on `producer` must be complete.
+System* system_adopt(uint32_t n, uint64_t nnz, DevBuf<uint32_t>&& rowptr,
+                     DevBuf<uint32_t>&& colind, DevBuf<double>&& val, uint32_t flags);
+
+// Least-squares normal equations on the device (k_assemble.hip).  Device
+// arrays: rows rp[nrows + 1] over entries (var, coef)[nent]; row right-hand
+// sides row_rhs[nsys][nrows] (null: 0); free_index[nvars] (0xffffffff for a
+// locked variable); values[nsys][nvars] of the locked variables (null: 0).
+struct LSInput {
+    uint32_t nrows = 0, nvars = 0, nsys = 1, nfree = 0;
+    uint64_t nent = 0;
+    const uint32_t* rp = nullptr;
+    const uint32_t* var = nullptr;
+    const double* coef = nullptr;
+    const double* row_rhs = nullptr;
+    const uint32_t* free_index = nullptr;
+    const double* values = nullptr;
+};
+struct LSOutput {
+    DevBuf<uint32_t> rowptr, colind;  // CRS over the free variables, sorted columns
+    DevBuf<double> val;
+    DevBuf<double> rhs;               // nsys x nfree
+    uint64_t nnz = 0;
+};
+void ls_assemble_device(const LSInput& in, LSOutput& out, hipStream_t s);
+// nlhipLeastSquaresCreate: host or device inputs, checks, assembly, system;
+// rhs (nsys x nfree, host or device) may be null.
+System* ls_create(uint32_t nvars, uint32_t nrows, const uint32_t* row_ptr, const uint32_t* var,
+                  const double* coef, uint32_t nsys, const double* row_rhs,
+                  const uint8_t* locked, const double* values, uint32_t flags, double* rhs,
+                  uint32_t* nfree_out);
 void default_launch(uint32_t n, uint64_t nnz, int& rpw, uint32_t& grid);
 
 // Solve drivers (System::mu must be held).  b, x: device pointers of length n.

@@ -104,6 +104,26 @@ bool build_blocks(System& S) {
                             S.stream.get(), S.brp, S.bci16, S.bsegb, S.bval);
 }
 
+namespace {
+
+// Jacobi diagonal, column offsets, block form, launch shape and FP32 copy of
+// a system whose CRS is resident.
+void finish_system(System& S, uint32_t flags) {
+    hipStream_t s = S.stream.get();
+    S.dinv.alloc(S.n);
+    NLHIP_CHECK(jacobi_dinv<double>(S.n, S.rowptr.get(), S.colind.get(), S.val.get(),
+                                    S.dinv.get(), s));
+    if (!(flags & NLHIP_SYSTEM_NO_OFFSETS)) {
+        build_col_offsets(S.nnz, S.colind.get(), s, S.ci16, S.segb);
+        if (!(flags & NLHIP_SYSTEM_NO_BLOCKS)) build_blocks(S);
+    }
+    default_launch(S.n, S.nnz, S.rpw, S.grid);
+    if (flags & NLHIP_SYSTEM_FP32) S.ensure_fp32();
+    S.stream.sync();
+}
+
+}  // namespace
+
 System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
                       const double* val, uint32_t flags) {
     if (n == 0 || !rowptr) throw Error(NLHIP_ERR_INVALID, "empty system");
@@ -143,20 +163,24 @@ System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind
     S->rowptr.alloc(size_t(n) + 1);
     S->colind.alloc(nnz);
     S->val.alloc(nnz);
-    S->dinv.alloc(n);
     hipStream_t s = S->stream.get();
     copy_in(S->rowptr.get(), rp, (size_t(n) + 1) * 4, s);
     copy_in(S->colind.get(), ci, nnz * 4, s);
     copy_in(S->val.get(), va, nnz * 8, s);
-    NLHIP_CHECK(jacobi_dinv<double>(n, S->rowptr.get(), S->colind.get(), S->val.get(),
-                                    S->dinv.get(), s));
-    if (!(flags & NLHIP_SYSTEM_NO_OFFSETS)) {
-        build_col_offsets(nnz, S->colind.get(), s, S->ci16, S->segb);
-        if (!(flags & NLHIP_SYSTEM_NO_BLOCKS)) build_blocks(*S);
-    }
-    default_launch(n, nnz, S->rpw, S->grid);
-    if (flags & NLHIP_SYSTEM_FP32) S->ensure_fp32();
-    S->stream.sync();
+    finish_system(*S, flags);
+    return S.release();
+}
+
+System* system_adopt(uint32_t n, uint64_t nnz, DevBuf<uint32_t>&& rowptr,
+                     DevBuf<uint32_t>&& colind, DevBuf<double>&& val, uint32_t flags) {
+    if (n == 0) throw Error(NLHIP_ERR_INVALID, "empty system");
+    auto S = std::make_unique<System>();
+    S->n = n;
+    S->nnz = nnz;
+    S->rowptr = std::move(rowptr);
+    S->colind = std::move(colind);
+    S->val = std::move(val);
+    finish_system(*S, flags);
     return S.release();
 }
 

@@ -97,6 +97,33 @@ NLAPI int NLAPIENTRY nlhipSystemCreate(uint32_t n, const uint32_t* rowptr,
 NLAPI int NLAPIENTRY nlhipSystemDestroy(nlhip_system s);
 NLAPI int NLAPIENTRY nlhipSystemInfo(nlhip_system s, uint32_t* n, uint64_t* nnz,
                                      uint64_t* device_bytes);
+/* Copies the resident CRS out (rowptr[n+1], colind[nnz], val[nnz]; host or
+ * device pointers, any may be null). */
+NLAPI int NLAPIENTRY nlhipSystemGetCRS(nlhip_system s, uint32_t* rowptr, uint32_t* colind,
+                                       double* val);
+
+/* Least-squares normal equations assembled on the device: the system a
+ * least-squares OpenNL context builds from the same rows (replaces the
+ * per-row accumulation of nlEnd(NL_ROW) under NL_LEAST_SQUARES and the
+ * compression of nlEnd(NL_MATRIX); geogram's nl_context / NLRowColumn code,
+ * driven at voxel_grob_transport_commands.cpp:236-279 and by the LSCM calls
+ * at mesh_grob_surface_commands.cpp:724-779).  Row r is
+ *   sum over e in [row_ptr[r], row_ptr[r+1]) of coef[e] x[var[e]] = row_rhs[s][r]
+ * for each of nb_systems right-hand sides (row_rhs: nb_systems x nb_rows, or
+ * NULL for zeros).  Locked variables (locked[v] != 0; NULL: none) keep
+ * values[s][v] (nb_systems x nb_variables) and move to the right-hand side.
+ * The system is over the free variables in index order; rhs (nb_systems x
+ * nb_free, host or device, may be NULL) receives its right-hand sides, and
+ * nb_free their number.  Every sum is formed in the order the context forms
+ * it, so the CRS and rhs are bitwise those of the host assembly.  Inputs are
+ * host or device arrays; flags as nlhipSystemCreate. */
+NLAPI int NLAPIENTRY nlhipLeastSquaresCreate(uint32_t nb_variables, uint32_t nb_rows,
+                                             const uint32_t* row_ptr, const uint32_t* var,
+                                             const double* coef, uint32_t nb_systems,
+                                             const double* row_rhs, const uint8_t* locked,
+                                             const double* values, uint32_t flags,
+                                             nlhip_system* out, double* rhs,
+                                             uint32_t* nb_free);
 /* Launch shape of the SpMV engine: rows per wavefront (2, 4, 8, 16: that many
  * rows share a 64-lane wavefront; 64: one row per lane, nonzeros streamed
  * through LDS, the default) and workgroups per launch.  65..76 select

@@ -147,8 +147,33 @@ def test_context_state_errors_without_gpu(capfd):
     L.nlEnd(capi.NL_MATRIX)
     L.nlEnd(capi.NL_SYSTEM)
     v = C.c_int()
+    # least-squares rows are assembled on the device: no device, an error
     L.nlGetIntegerv(capi.NL_NNZ, C.byref(v))
-    assert v.value == 1   # one free-free entry (1,1)
+    assert "no HIP device" in capi.last_error()
     assert L.nlGetVariable(0) == 1.0 and L.nlVariableIsLocked(0)
     L.nlDeleteContext(ctx)
     assert not L.nlGetCurrent()
+
+
+def test_context_rows_then_ij_coefficients_accumulate_on_host():
+    """An nlAddIJCoefficient after least-squares rows sends the rows through
+    the host accumulation first (OpenNL's order of additions)."""
+    L = capi.lib()
+    ctx = L.nlNewContext()
+    L.nlSolverParameteri(capi.NL_NB_VARIABLES, 3)
+    L.nlSolverParameteri(capi.NL_LEAST_SQUARES, capi.NL_TRUE)
+    L.nlBegin(capi.NL_SYSTEM)
+    L.nlLockVariable(0)
+    L.nlSetVariable(0, 1.0)
+    L.nlBegin(capi.NL_MATRIX)
+    L.nlBegin(capi.NL_ROW)
+    L.nlCoefficient(0, 1.0)
+    L.nlCoefficient(1, -1.0)
+    L.nlEnd(capi.NL_ROW)
+    L.nlAddIJCoefficient(1, 1, 0.0)
+    L.nlEnd(capi.NL_MATRIX)
+    L.nlEnd(capi.NL_SYSTEM)
+    v = C.c_int()
+    L.nlGetIntegerv(capi.NL_NNZ, C.byref(v))
+    assert v.value == 1   # one free-free entry (1,1)
+    L.nlDeleteContext(ctx)

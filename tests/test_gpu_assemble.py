@@ -1,0 +1,202 @@
+"""Least-squares normal equations assembled on the device
+(nlhipLeastSquaresCreate, and the context API's nlBegin/nlEnd(NL_ROW) rows,
+which now go through it): the CRS and right-hand sides are bitwise those the
+oracle's restated OpenNL least-squares context builds from the same rows,
+and those of the host LSCM assembly; malformed input is rejected."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from graphitethree_amd import capi
+from graphitethree_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def random_rows(rng, nvars, nrows, maxlen=8):
+    lens = rng.integers(0, maxlen + 1, nrows)
+    rp = np.zeros(nrows + 1, dtype=np.uint32)
+    rp[1:] = np.cumsum(lens)
+    ne = int(rp[-1])
+    var = rng.integers(0, nvars, ne).astype(np.uint32)   # repeats within a row too
+    coef = rng.standard_normal(ne)
+    coef[rng.random(ne) < 0.05] = 0.0
+    coef[rng.random(ne) < 0.02] = -0.0
+    return rp, var, coef
+
+
+def oracle_system(nvars, rp, var, coef, rhs, locked, values):
+    ctx = O.Context(nvars, least_squares=True)
+    for v in range(nvars):
+        if locked[v]:
+            ctx.set_variable(v, values[v])
+            ctx.lock_variable(v)
+    ctx.begin_matrix()
+    for r in range(len(rp) - 1):
+        ctx.row(list(zip(var[rp[r]:rp[r + 1]], coef[rp[r]:rp[r + 1]])), rhs[r])
+    return ctx.system()
+
+
+def assert_same(S, rhs, crs_ref, rhs_ref):
+    rp, ci, va = S.crs()
+    assert np.array_equal(rp, crs_ref.rowptr)
+    assert np.array_equal(ci, crs_ref.colind)
+    assert np.array_equal(bits(va), bits(crs_ref.val))
+    assert np.array_equal(bits(rhs), bits(rhs_ref))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_random_rows_match_oracle_context(hip, seed):
+    rng = np.random.default_rng(seed)
+    nvars, nrows = 300, 900
+    rp, var, coef = random_rows(rng, nvars, nrows)
+    rhs = rng.standard_normal(nrows)
+    locked = (rng.random(nvars) < 0.1).astype(np.uint8)
+    values = rng.standard_normal(nvars)
+    S, b = capi.System.least_squares(nvars, rp, var, coef, row_rhs=rhs, locked=locked,
+                                     values=values)
+    crs, b_ref = oracle_system(nvars, rp, var, coef, rhs, locked, values)
+    assert S.n == nvars - int(locked.sum())
+    assert_same(S, b[0], crs, b_ref)
+    S.close()
+
+
+def test_several_right_hand_sides(hip):
+    """NL_NB_SYSTEMS: one matrix, a right-hand side per system (each system
+    with its own locked values)."""
+    rng = np.random.default_rng(7)
+    nvars, nrows, k = 120, 400, 3
+    rp, var, coef = random_rows(rng, nvars, nrows, maxlen=6)
+    rhs = rng.standard_normal((k, nrows))
+    locked = (rng.random(nvars) < 0.15).astype(np.uint8)
+    values = rng.standard_normal((k, nvars))
+    S, b = capi.System.least_squares(nvars, rp, var, coef, row_rhs=rhs, locked=locked,
+                                     values=values)
+    for j in range(k):
+        crs, b_ref = oracle_system(nvars, rp, var, coef, rhs[j], locked, values[j])
+        assert_same(S, b[j], crs, b_ref)
+    S.close()
+
+
+def test_lscm_rows_match_host_assembly(hip):
+    """The LSCM rows of a chart give bitwise the system the host LSCM
+    assembly (workloads.cpp, one vertex's rows from its triangles in order)
+    builds."""
+    mesh = W.chart_mesh(3000, seed=4)
+    pins = mesh.extreme_pins()
+    uv = (mesh.points[pins[0], 0], mesh.points[pins[0], 1],
+          mesh.points[pins[1], 0], mesh.points[pins[1], 1])
+    coeffs, ok = mesh.lscm_rows()
+    tri = mesh.triangles[ok]
+    vars6 = (2 * tri[:, [0, 0, 1, 1, 2, 2]] + np.array([0, 1, 0, 1, 0, 1])).astype(np.uint32)
+    c = coeffs[ok]
+    var = np.stack([vars6, vars6], axis=1).reshape(-1)           # real row, imaginary row
+    coef = np.stack([c[:, :6], c[:, 6:]], axis=1).reshape(-1)
+    nrows = 2 * len(tri)
+    rp = (6 * np.arange(nrows + 1)).astype(np.uint32)
+    nvars = 2 * mesh.nv
+    locked = np.zeros(nvars, dtype=np.uint8)
+    values = np.zeros(nvars)
+    for p, (u, v) in zip(pins, [(uv[0], uv[1]), (uv[2], uv[3])]):
+        locked[2 * p] = locked[2 * p + 1] = 1
+        values[2 * p], values[2 * p + 1] = u, v
+    S, b = capi.System.least_squares(nvars, rp, var, coef, locked=locked, values=values)
+    ref = W.lscm_system(mesh, pins=pins, pin_uv=uv)
+    assert_same(S, b[0], O.CRS(ref.rowptr, ref.colind, ref.val, ref.n), ref.b)
+    x, st = S.solve(b[0], eps=1e-10, max_iter=20000)
+    assert st.rel_residual <= 1e-10
+    S.close()
+
+
+def test_context_rows_assemble_on_device(hip):
+    """nlBegin/nlEnd(NL_ROW) in least-squares mode: NL_NNZ and the solution
+    come from the device assembly of the recorded rows."""
+    L = hip
+    rng = np.random.default_rng(11)
+    nvars, nrows = 200, 700
+    rp, var, coef = random_rows(rng, nvars, nrows, maxlen=5)
+    rhs = rng.standard_normal(nrows)
+    locked = np.zeros(nvars, dtype=np.uint8)
+    locked[:5] = 1
+    values = rng.standard_normal(nvars)
+    L.nlNewContext()
+    L.nlSolverParameteri(capi.NL_NB_VARIABLES, nvars)
+    L.nlSolverParameteri(capi.NL_LEAST_SQUARES, capi.NL_TRUE)
+    L.nlBegin(capi.NL_SYSTEM)
+    for v in range(5):
+        L.nlSetVariable(v, values[v])
+        L.nlLockVariable(v)
+    L.nlBegin(capi.NL_MATRIX)
+    for r in range(nrows):
+        L.nlBegin(capi.NL_ROW)
+        for e in range(rp[r], rp[r + 1]):
+            L.nlCoefficient(int(var[e]), float(coef[e]))
+        L.nlRightHandSide(float(rhs[r]))
+        L.nlEnd(capi.NL_ROW)
+    L.nlEnd(capi.NL_MATRIX)
+    L.nlEnd(capi.NL_SYSTEM)
+    nnz = C.c_int()
+    L.nlGetIntegerv(capi.NL_NNZ, C.byref(nnz))
+    crs, _ = oracle_system(nvars, rp, var, coef, rhs, locked, values)
+    assert nnz.value == len(crs.colind)
+    L.nlDeleteContext(L.nlGetCurrent())
+
+
+def test_sums_follow_row_order(hip):
+    """Contributions that do not associate: (1e16 + 1) - 1e16 = 0 in row
+    order, 1 in others.  Entry (0, 1) and b[0] must be 0."""
+    rp = np.array([0, 2, 4, 6], dtype=np.uint32)
+    var = np.array([0, 1, 0, 1, 0, 1], dtype=np.uint32)
+    coef = np.array([1e8, 1e8, 1.0, 1.0, 1e8, -1e8])
+    rhs = np.array([1e8, 1.0, -1e8])
+    S, b = capi.System.least_squares(2, rp, var, coef, row_rhs=rhs)
+    crs, b_ref = oracle_system(2, rp, var, coef, rhs, np.zeros(2, np.uint8), np.zeros(2))
+    rp_out, ci, va = S.crs()
+    print("device M", va, "b", b[0], "oracle M", crs.val, "b", b_ref)
+    assert va[1] == 0.0 and b[0][0] == 0.0
+    assert_same(S, b[0], crs, b_ref)
+    S.close()
+
+
+def test_edge_rows(hip):
+    """Empty rows, rows of locked variables only, a free variable in no row
+    (an empty matrix row), repeated variables in a row."""
+    nvars = 6
+    rp = np.array([0, 0, 2, 4, 7, 7], dtype=np.uint32)
+    var = np.array([0, 1, 2, 2, 3, 1, 3], dtype=np.uint32)
+    coef = np.array([1.5, -2.0, 0.5, 0.25, -1.0, 3.0, 2.0])
+    rhs = np.array([1.0, 2.0, -1.0, 0.5, 7.0])
+    locked = np.array([1, 0, 0, 0, 0, 0], dtype=np.uint8)
+    values = np.array([2.0, 0, 0, 0, 0, 0])
+    S, b = capi.System.least_squares(nvars, rp, var, coef, row_rhs=rhs, locked=locked,
+                                     values=values)
+    crs, b_ref = oracle_system(nvars, rp, var, coef, rhs, locked, values)
+    assert_same(S, b[0], crs, b_ref)
+    rp_out, _, _ = S.crs()
+    assert rp_out[3] == rp_out[5]        # variables 4, 5 (free 3, 4): in no row
+    S.close()
+
+
+def test_rejects_bad_input(hip):
+    rp = np.array([0, 2], dtype=np.uint32)
+    with pytest.raises(capi.NLHIPError):          # variable out of range
+        capi.System.least_squares(2, rp, np.array([0, 5], np.uint32), np.array([1.0, 1.0]))
+    with pytest.raises(capi.NLHIPError):          # NaN coefficient
+        capi.System.least_squares(2, rp, np.array([0, 1], np.uint32),
+                                  np.array([1.0, np.nan]))
+    with pytest.raises(capi.NLHIPError):          # row_ptr not monotone
+        capi.System.least_squares(2, np.array([0, 2, 1], np.uint32),
+                                  np.array([0, 1], np.uint32), np.array([1.0, 1.0]))
+    with pytest.raises(capi.NLHIPError):          # every variable locked
+        capi.System.least_squares(2, rp, np.array([0, 1], np.uint32), np.array([1.0, 1.0]),
+                                  locked=np.array([1, 1], np.uint8), values=np.zeros(2))
+    with pytest.raises(capi.NLHIPError):          # infinite locked value
+        capi.System.least_squares(2, rp, np.array([0, 1], np.uint32), np.array([1.0, 1.0]),
+                                  locked=np.array([1, 0], np.uint8),
+                                  values=np.array([np.inf, 0.0]))
