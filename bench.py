@@ -76,6 +76,75 @@ def cpu_model() -> str:
     return ""
 
 
+def lscm_rows_arrays(mesh):
+    """Rows of the LSCM least-squares problem, as the context receives them:
+    per valid triangle a real and an imaginary row over (u0, v0, u1, v1, u2, v2)."""
+    coeffs, ok = mesh.lscm_rows()
+    tri = mesh.triangles[ok]
+    vars6 = (2 * tri[:, [0, 0, 1, 1, 2, 2]] + np.array([0, 1, 0, 1, 0, 1])).astype(np.uint32)
+    c = coeffs[ok]
+    var = np.stack([vars6, vars6], axis=1).reshape(-1)
+    coef = np.stack([c[:, :6], c[:, 6:]], axis=1).reshape(-1)
+    rp = (6 * np.arange(2 * len(tri) + 1, dtype=np.uint64)).astype(np.uint32)
+    return rp, var, coef
+
+
+def assembly_measurement(args, Sw, capi, W):
+    k = args.k
+    mesh = W.grid_mesh(k, seed=args.seed)
+    pins, uv = (0, k * k - 1), (0.0, 0.0, 1.0, 1.0)     # as lscm_grid
+    rp, var, coef = lscm_rows_arrays(mesh)
+    nrows = len(rp) - 1
+    nvars = 2 * mesh.nv
+    locked = np.zeros(nvars, dtype=np.uint8)
+    values = np.zeros(nvars)
+    for p, (u, v) in zip(pins, [(uv[0], uv[1]), (uv[2], uv[3])]):
+        locked[2 * p] = locked[2 * p + 1] = 1
+        values[2 * p], values[2 * p + 1] = u, v
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        Sa, ba = capi.System.least_squares(nvars, rp, var, coef, locked=locked, values=values)
+        times.append(time.perf_counter() - t0)
+        if _ == 0:
+            rpa, cia, vaa = Sa.crs()
+            same = (np.array_equal(rpa, Sw.rowptr) and np.array_equal(cia, Sw.colind)
+                    and np.array_equal(vaa.view(np.uint64), Sw.val.view(np.uint64))
+                    and np.array_equal(ba[0].view(np.uint64), Sw.b.view(np.uint64)))
+            del rpa, cia, vaa
+        Sa.close()
+    t_dev = min(times)
+    t0 = time.perf_counter()
+    W.lscm_system(mesh, pins=pins, pin_uv=uv)
+    t_gather = time.perf_counter() - t0
+    res = {"rows": nrows, "entries": int(rp[-1]), "n": Sw.n, "nnz": Sw.nnz,
+           "device_s": t_dev, "bitwise_equal_host_assembly": bool(same),
+           "host_gather_s": t_gather,
+           "host_gather_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    log(f"assembly: device {t_dev:.3f}s (rows in from host, CRS resident), bitwise = host "
+        f"assembly: {same}; host gather {t_gather:.2f}s")
+    if args.cpu_iters > 0 and args.assembly_sample_rows > 0:
+        from oracle import oracle as O
+        ns = min(nrows, args.assembly_sample_rows)
+        ctx = O.Context(nvars, least_squares=True)
+        for p, (u, v) in zip(pins, [(uv[0], uv[1]), (uv[2], uv[3])]):
+            ctx.set_variable(2 * p, u); ctx.set_variable(2 * p + 1, v)
+            ctx.lock_variable(2 * p); ctx.lock_variable(2 * p + 1)
+        ctx.begin_matrix()
+        t0 = time.perf_counter()
+        ctx.rows(rp[:ns + 1], var[:int(rp[ns])], coef[:int(rp[ns])])
+        t_s = time.perf_counter() - t0
+        del ctx
+        res["cpu_oracle_s"] = t_s * nrows / ns
+        res["cpu_oracle_sample"] = (f"first {ns} of {nrows} rows through the oracle's "
+                                    f"least-squares context (OpenNL's row-by-row "
+                                    f"accumulation, one core): {t_s:.2f}s, scaled by rows; "
+                                    f"the compression to CRS is not counted")
+        log(f"assembly cpu (oracle context, 1 core): {t_s:.2f}s for {ns} rows -> "
+            f"{res['cpu_oracle_s']:.1f}s for {nrows}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +158,10 @@ def main():
                     help="oracle PCG iterations timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-assembly", action="store_true",
+                    help="skip the least-squares assembly measurement")
+    ap.add_argument("--assembly-sample-rows", type=int, default=2_000_000,
+                    help="rows of the CPU (oracle) assembly sample")
     ap.add_argument("--rows-per-wave", type=int, default=0,
                     help="SpMV engine code for the C2 system (0: the library default)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -230,6 +303,13 @@ def main():
         log(f"317^2: {s1.iterations} it to {s1.rel_residual:.2e} in "
             f"{np.median(l1):.3f}s; capped 5000: {s1c.device_seconds:.3f}s")
         S1.close()
+
+    # ---- least-squares assembly of the same system from its LSCM rows:
+    # device (nlhipLeastSquaresCreate, host rows in, resident CRS out) against
+    # the oracle's row-by-row context (OpenNL's algorithm, one core) on a
+    # sample of the rows, and the multithreaded host gather that built Sw
+    if not args.no_assembly:
+        result_extra["assembly"] = assembly_measurement(args, Sw, capi, W)
 
     out = {
         "metric": "CG solves/s + SpMV HBM GB/s at 1e-10 rel. residual; p50/p99 solve latency",

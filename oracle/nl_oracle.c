@@ -614,6 +614,19 @@ void or_ctx_end_row(or_ctx* C) {
     ++C->cur_row;
 }
 
+/* Rows r < nrows of arrays (entries [rp[r], rp[r+1]) of var/a, right-hand
+ * side rhs[r], or 0 when rhs is NULL), each through begin_row /
+ * coefficient / right_hand_side / end_row. */
+void or_ctx_rows(or_ctx* C, uint32_t nrows, const uint32_t* rp, const uint32_t* var,
+                 const double* a, const double* rhs) {
+    for (uint32_t r = 0; r < nrows; ++r) {
+        or_ctx_begin_row(C);
+        for (uint32_t e = rp[r]; e < rp[r + 1]; ++e) or_ctx_coefficient(C, var[e], a[e]);
+        if (rhs) or_ctx_right_hand_side(C, rhs[r]);
+        or_ctx_end_row(C);
+    }
+}
+
 uint32_t or_ctx_nb_free(const or_ctx* C) { return C->nb_free; }
 uint64_t or_ctx_nnz(const or_ctx* C) { return or_dyn_nnz(C->M); }
 

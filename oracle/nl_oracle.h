@@ -111,6 +111,8 @@ void     or_ctx_begin_row(or_ctx* C);
 void     or_ctx_coefficient(or_ctx* C, uint32_t var, double a);
 void     or_ctx_right_hand_side(or_ctx* C, double v);
 void     or_ctx_end_row(or_ctx* C);
+void     or_ctx_rows(or_ctx* C, uint32_t nrows, const uint32_t* rp, const uint32_t* var,
+                     const double* a, const double* rhs);
 uint32_t or_ctx_nb_free(const or_ctx* C);
 uint64_t or_ctx_nnz(const or_ctx* C);
 /* Compressed system over the free variables. */

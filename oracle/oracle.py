@@ -69,6 +69,8 @@ def lib() -> C.CDLL:
             "or_ctx_coefficient": (None, [C.c_void_p, C.c_uint32, C.c_double]),
             "or_ctx_right_hand_side": (None, [C.c_void_p, C.c_double]),
             "or_ctx_end_row": (None, [C.c_void_p]),
+            "or_ctx_rows": (None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]),
             "or_ctx_nb_free": (C.c_uint32, [C.c_void_p]),
             "or_ctx_nnz": (C.c_uint64, [C.c_void_p]),
             "or_ctx_system": (None, [C.c_void_p, _u32p, _u32p, _f64p, _f64p]),
@@ -262,6 +264,15 @@ class Context:
         if rhs != 0.0:
             L.or_ctx_right_hand_side(self._h, float(rhs))
         L.or_ctx_end_row(self._h)
+
+    def rows(self, rp, var, coef, rhs=None):
+        """Rows from arrays (as many row() calls, looped in C)."""
+        rp = _u32(rp)
+        var = _u32(var)
+        coef = _f64(coef)
+        rhs = None if rhs is None else _f64(rhs)
+        lib().or_ctx_rows(self._h, len(rp) - 1, rp.ctypes.data, var.ctypes.data,
+                          coef.ctypes.data, None if rhs is None else rhs.ctypes.data)
 
     def system(self):
         n = int(lib().or_ctx_nb_free(self._h))

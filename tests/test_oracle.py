@@ -15,6 +15,7 @@ import pytest
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
+from graphitethree_amd import workloads as W
 from oracle import oracle as O
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -276,6 +277,55 @@ def test_context_least_squares_normal_equations():
     np.testing.assert_allclose(xs, np.linalg.lstsq(Af, rhs - Al @ xl, rcond=None)[0],
                                rtol=1e-8, atol=1e-9)
     assert ctx.get_variable(2) == 1.5
+
+
+def test_context_rows_from_arrays_equal_row_calls():
+    rng = np.random.default_rng(4)
+    nv, nrows = 40, 120
+    lens = rng.integers(0, 6, nrows)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    var = rng.integers(0, nv, rp[-1]).astype(np.uint32)
+    coef = rng.standard_normal(rp[-1])
+    rhs = rng.standard_normal(nrows)
+    a, b = O.Context(nv), O.Context(nv)
+    for c in (a, b):
+        c.set_variable(3, 0.5)
+        c.lock_variable(3)
+        c.begin_matrix()
+    for r in range(nrows):
+        a.row(list(zip(var[rp[r]:rp[r + 1]], coef[rp[r]:rp[r + 1]])), rhs[r])
+    b.rows(rp, var, coef, rhs)
+    (Ma, ba), (Mb, bb) = a.system(), b.system()
+    assert np.array_equal(Ma.rowptr, Mb.rowptr) and np.array_equal(Ma.colind, Mb.colind)
+    assert np.array_equal(Ma.val, Mb.val) and np.array_equal(ba, bb)
+
+
+def test_context_lscm_rows_equal_host_lscm_assembly():
+    """Two independent assemblies of LSCM normal equations: the context
+    (row by row, OpenNL's order) and workloads.cpp's per-vertex gather give
+    the same bits."""
+    mesh = W.chart_mesh(1500, seed=8)
+    pins = mesh.extreme_pins()
+    uv = (mesh.points[pins[0], 0], mesh.points[pins[0], 1],
+          mesh.points[pins[1], 0], mesh.points[pins[1], 1])
+    coeffs, ok = mesh.lscm_rows()
+    tri = mesh.triangles[ok]
+    vars6 = (2 * tri[:, [0, 0, 1, 1, 2, 2]] + np.array([0, 1, 0, 1, 0, 1])).astype(np.uint32)
+    c = coeffs[ok]
+    var = np.stack([vars6, vars6], axis=1).reshape(-1)
+    coef = np.stack([c[:, :6], c[:, 6:]], axis=1).reshape(-1)
+    rp = (6 * np.arange(2 * len(tri) + 1)).astype(np.uint32)
+    ctx = O.Context(2 * mesh.nv)
+    for p, (u, v) in zip(pins, [(uv[0], uv[1]), (uv[2], uv[3])]):
+        ctx.set_variable(2 * p, u); ctx.set_variable(2 * p + 1, v)
+        ctx.lock_variable(2 * p); ctx.lock_variable(2 * p + 1)
+    ctx.begin_matrix()
+    ctx.rows(rp, var, coef)
+    M, b = ctx.system()
+    ref = W.lscm_system(mesh, pins=pins, pin_uv=uv)
+    assert np.array_equal(M.rowptr, ref.rowptr) and np.array_equal(M.colind, ref.colind)
+    assert np.array_equal(M.val.view(np.uint64), ref.val.view(np.uint64))
+    assert np.array_equal(b.view(np.uint64), ref.b.view(np.uint64))
 
 
 def test_context_square_system():
