@@ -114,15 +114,30 @@ def assembly_measurement(args, Sw, capi, W):
             del rpa, cia, vaa
         Sa.close()
     t_dev = min(times)
+    # the whole LSCM system from the mesh on the device (rows computed there)
+    times_m = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        Sm, bm = capi.System.lscm(mesh.points, mesh.triangles, pins, uv)
+        times_m.append(time.perf_counter() - t0)
+        if _ == 0:
+            rpa, cia, vaa = Sm.crs()
+            same_m = (np.array_equal(rpa, Sw.rowptr) and np.array_equal(cia, Sw.colind)
+                      and np.array_equal(vaa.view(np.uint64), Sw.val.view(np.uint64))
+                      and np.array_equal(bm.view(np.uint64), Sw.b.view(np.uint64)))
+            del rpa, cia, vaa
+        Sm.close()
     t0 = time.perf_counter()
     W.lscm_system(mesh, pins=pins, pin_uv=uv)
     t_gather = time.perf_counter() - t0
     res = {"rows": nrows, "entries": int(rp[-1]), "n": Sw.n, "nnz": Sw.nnz,
            "device_s": t_dev, "bitwise_equal_host_assembly": bool(same),
+           "device_from_mesh_s": min(times_m), "from_mesh_bitwise_equal": bool(same_m),
            "host_gather_s": t_gather,
            "host_gather_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
     log(f"assembly: device {t_dev:.3f}s (rows in from host, CRS resident), bitwise = host "
-        f"assembly: {same}; host gather {t_gather:.2f}s")
+        f"assembly: {same}; from the mesh {min(times_m):.3f}s ({same_m}); host gather "
+        f"{t_gather:.2f}s")
     if args.cpu_iters > 0 and args.assembly_sample_rows > 0:
         from oracle import oracle as O
         ns = min(nrows, args.assembly_sample_rows)

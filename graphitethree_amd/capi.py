@@ -167,6 +167,9 @@ SIGNATURES = {
                                     C.POINTER(_vp)]),
     "nlhipSystemDestroy": (C.c_int, [_vp]),
     "nlhipSystemGetCRS": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "nlhipLSCMCreate": (C.c_int, [C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32, C.c_double,
+                                  C.c_double, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
+                                  C.POINTER(_vp), _vp, _u32p]),
     "nlhipLeastSquaresCreate": (C.c_int, [C.c_uint32, C.c_uint32, _vp, _vp, _vp, C.c_uint32,
                                           _vp, _vp, _vp, C.c_uint32, C.POINTER(_vp), _vp,
                                           _u32p]),
@@ -313,6 +316,25 @@ class System:
         S._h = h
         S.n = int(nfree.value)
         return S, rhs[:, :S.n]
+
+    @classmethod
+    def lscm(cls, points, triangles, pins, pin_uv, fp32: bool = False):
+        """LSCM system of a mesh built on the device (nlhipLSCMCreate).
+        Returns (system, rhs)."""
+        P = np.ascontiguousarray(points, dtype=np.float64)
+        T = np.ascontiguousarray(triangles, dtype=np.uint32)
+        nv, nt = len(P), len(T)
+        rhs = np.zeros(max(2 * nv - 4, 1))
+        h = C.c_void_p()
+        nfree = C.c_uint32()
+        check(lib().nlhipLSCMCreate(nv, ptr(P), nt, ptr(T), int(pins[0]), float(pin_uv[0]),
+                                    float(pin_uv[1]), int(pins[1]), float(pin_uv[2]),
+                                    float(pin_uv[3]), NLHIP_SYSTEM_FP32 if fp32 else 0,
+                                    C.byref(h), ptr(rhs), C.byref(nfree)), "nlhipLSCMCreate")
+        S = cls.__new__(cls)
+        S._h = h
+        S.n = int(nfree.value)
+        return S, rhs[:S.n]
 
     def crs(self):
         """The resident CRS (rowptr, colind, val) copied back."""

@@ -178,6 +178,93 @@ __global__ void k_ls_rhs(uint32_t nfree, const uint32_t* occ_off, const uint32_t
     }
 }
 
+// ---- LSCM rows on the device
+
+// Triangle t's 12 LSCM coefficients (Levy, Petitjean, Ray, Maillot 2002):
+// the conformality of the map in the triangle's orthonormal frame (X along
+// p1 - p0, Y = N x X), W0 = z2 - z1, W1 = z0 - z2, W2 = z1 - z0, scaled by
+// 1/sqrt(|e1 x e2|): real row (Wr, -Wi) then imaginary row (Wi, Wr) over
+// (u0, v0, u1, v1, u2, v2).  ok[t] = 0 for a degenerate triangle.  Every
+// operation is the host builder's (workloads.cpp, lscm_coeffs), in its
+// order, correctly rounded and unfused, so the coefficients are its bits.
+__global__ void k_lscm_coeffs(uint32_t nt, const double* P, const uint32_t* T, double* c,
+                              uint32_t* ok) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride) {
+        const uint32_t* q = T + 3 * size_t(t);
+        const double* p0 = P + 3 * size_t(q[0]);
+        const double* p1 = P + 3 * size_t(q[1]);
+        const double* p2 = P + 3 * size_t(q[2]);
+        const double e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        const double e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+        const double nr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                              e1[0] * e2[1] - e1[1] * e2[0]};
+        const double dT = sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+        const double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+        double* out = c + 12 * size_t(t);
+        if (!(dT > 0.0) || !(l1 > 0.0)) {
+            ok[t] = 0;
+            for (int j = 0; j < 12; ++j) out[j] = 0.0;
+            continue;
+        }
+        const double X[3] = {e1[0] / l1, e1[1] / l1, e1[2] / l1};
+        const double Nn[3] = {nr[0] / dT, nr[1] / dT, nr[2] / dT};
+        const double Y[3] = {Nn[1] * X[2] - Nn[2] * X[1], Nn[2] * X[0] - Nn[0] * X[2],
+                             Nn[0] * X[1] - Nn[1] * X[0]};
+        const double z1x = l1, z1y = 0.0;
+        const double z2x = e2[0] * X[0] + e2[1] * X[1] + e2[2] * X[2];
+        const double z2y = e2[0] * Y[0] + e2[1] * Y[1] + e2[2] * Y[2];
+        const double Wr[3] = {z2x - z1x, -z2x, z1x};
+        const double Wi[3] = {z2y - z1y, -z2y, z1y};
+        const double sq = sqrt(dT);
+        for (int j = 0; j < 3; ++j) {
+            const double cr = Wr[j] / sq, ci = Wi[j] / sq;
+            out[2 * j] = cr;
+            out[2 * j + 1] = -ci;
+            out[6 + 2 * j] = ci;
+            out[6 + 2 * j + 1] = cr;
+        }
+        ok[t] = 1;
+    }
+}
+
+// Rows of the valid triangles in triangle order: triangle t at slot s (an
+// exclusive count of the valid ones before it) gives rows 2s (real) and
+// 2s + 1 (imaginary), 6 entries each.
+__global__ void k_lscm_rows(uint32_t nt, const uint32_t* T, const double* c, const uint32_t* ok,
+                            const uint32_t* slot, uint32_t* var, double* coef) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride) {
+        if (!ok[t]) continue;
+        const uint32_t* q = T + 3 * size_t(t);
+        const size_t base = 12 * size_t(slot[t]);
+        for (int e = 0; e < 12; ++e) {
+            const int k = e % 6;
+            var[base + e] = 2u * q[k / 2] + uint32_t(k & 1);
+            coef[base + e] = c[12 * size_t(t) + e];
+        }
+    }
+}
+
+__global__ void k_lscm_check(uint32_t nv, uint32_t nt, const double* P, const uint32_t* T,
+                             uint32_t* bad) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t flags = 0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride)
+        for (int k = 0; k < 3; ++k)
+            if (T[3 * size_t(t) + k] >= nv) flags |= 2u;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride)
+        for (int k = 0; k < 3; ++k)
+            if (!isfinite(P[3 * size_t(v) + k])) flags |= 4u;
+    if (flags) atomicOr(bad, flags);
+}
+
+__global__ void k_row_ptr6(uint32_t nrows, uint32_t* rp) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r <= nrows; r += stride)
+        rp[r] = 6u * r;
+}
+
 uint32_t grid_for(uint64_t n) {
     const uint64_t g = (n + kBlock - 1) / kBlock;
     return uint32_t(g < 1 ? 1 : (g > 65536 ? 65536 : g));
@@ -194,6 +281,44 @@ void with_temp(F&& call, hipStream_t s) {
 }
 
 }  // namespace
+
+void lscm_rows_device(uint32_t nv, const double* P, uint32_t nt, const uint32_t* T,
+                      LSRows& rows, hipStream_t s) {
+    {
+        DevBuf<uint32_t> bad(1);
+        NLHIP_CHECK(hipMemsetAsync(bad.get(), 0, 4, s));
+        k_lscm_check<<<grid_for(nt > nv ? nt : nv), kBlock, 0, s>>>(nv, nt, P, T, bad.get());
+        NLHIP_CHECK(hipGetLastError());
+        uint32_t h = 0;
+        NLHIP_CHECK(hipMemcpyAsync(&h, bad.get(), 4, hipMemcpyDeviceToHost, s));
+        NLHIP_CHECK(hipStreamSynchronize(s));
+        if (h & 2u) throw Error(NLHIP_ERR_INVALID, "triangle vertex index out of range");
+        if (h & 4u) throw Error(NLHIP_ERR_NONFINITE, "vertex coordinate is NaN or Inf");
+    }
+    DevBuf<double> c(12 * size_t(nt ? nt : 1));
+    DevBuf<uint32_t> ok(size_t(nt) + 1), slot(size_t(nt) + 1);
+    NLHIP_CHECK(hipMemsetAsync(ok.get(), 0, (size_t(nt) + 1) * 4, s));
+    if (nt) k_lscm_coeffs<<<grid_for(nt), kBlock, 0, s>>>(nt, P, T, c.get(), ok.get());
+    NLHIP_CHECK(hipGetLastError());
+    with_temp([&](void* t, size_t& b) {
+        return hipcub::DeviceScan::ExclusiveSum(t, b, ok.get(), slot.get(), size_t(nt) + 1, s);
+    }, s);
+    uint32_t nvalid = 0;
+    NLHIP_CHECK(hipMemcpyAsync(&nvalid, slot.get() + nt, 4, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipStreamSynchronize(s));
+    if (uint64_t(nvalid) * 12 > 0xffffffffull)
+        throw Error(NLHIP_ERR_UNSUPPORTED, "more than 2^32 - 1 row coefficients");
+    rows.nrows = 2 * nvalid;
+    rows.rp.alloc(size_t(rows.nrows) + 1);
+    rows.var.alloc(12 * size_t(nvalid ? nvalid : 1));
+    rows.coef.alloc(12 * size_t(nvalid ? nvalid : 1));
+    k_row_ptr6<<<grid_for(size_t(rows.nrows) + 1), kBlock, 0, s>>>(rows.nrows, rows.rp.get());
+    if (nt)
+        k_lscm_rows<<<grid_for(nt), kBlock, 0, s>>>(nt, T, c.get(), ok.get(), slot.get(),
+                                                    rows.var.get(), rows.coef.get());
+    NLHIP_CHECK(hipGetLastError());
+    NLHIP_CHECK(hipStreamSynchronize(s));
+}
 
 void ls_assemble_device(const LSInput& in, LSOutput& out, hipStream_t s) {
     const uint32_t nrows = in.nrows, nvars = in.nvars, nsys = in.nsys, nfree = in.nfree;

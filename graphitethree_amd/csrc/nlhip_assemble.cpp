@@ -2,6 +2,7 @@
 // read-back of a resident system's CRS.  The sums are those of a
 // least-squares OpenNL context fed the same rows (k_assemble.hip).
 #include <algorithm>
+#include <cmath>
 #include <memory>
 
 #include "nlhip_internal.h"
@@ -79,6 +80,63 @@ System* ls_create(uint32_t nvars, uint32_t nrows, const uint32_t* row_ptr, const
                         std::move(out.val), flags);
 }
 
+System* lscm_create(uint32_t nv, const double* points, uint32_t nt, const uint32_t* triangles,
+                    const uint32_t pin[2], const double uv[4], uint32_t flags, double* rhs,
+                    uint32_t* nfree_out) {
+    if (nv < 3 || nt == 0 || !points || !triangles)
+        throw Error(NLHIP_ERR_INVALID, "empty mesh");
+    if (pin[0] == pin[1] || pin[0] >= nv || pin[1] >= nv)
+        throw Error(NLHIP_ERR_INVALID, "pins must be two distinct vertices");
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(uv[k])) throw Error(NLHIP_ERR_NONFINITE, "pin position is NaN or Inf");
+    if (uint64_t(nv) * 2 > 0xffffffffull) throw Error(NLHIP_ERR_UNSUPPORTED, "too many vertices");
+    device();
+    Stream st;
+    hipStream_t s = st.get();
+    DevBuf<double> d_p;
+    DevBuf<uint32_t> d_t;
+    const double* P = on_device(points, 3 * size_t(nv), d_p, s);
+    const uint32_t* T = on_device(triangles, 3 * size_t(nt), d_t, s);
+    LSRows rows;
+    lscm_rows_device(nv, P, nt, T, rows, s);
+    const uint32_t nvars = 2 * nv;
+    std::vector<uint32_t> h_free(nvars);
+    std::vector<double> h_values(nvars, 0.0);
+    uint32_t nfree = 0;
+    for (uint32_t v = 0; v < nvars; ++v) {
+        const bool lk = (v / 2 == pin[0]) || (v / 2 == pin[1]);
+        h_free[v] = lk ? 0xffffffffu : nfree++;
+    }
+    h_values[2 * pin[0]] = uv[0];
+    h_values[2 * pin[0] + 1] = uv[1];
+    h_values[2 * pin[1]] = uv[2];
+    h_values[2 * pin[1] + 1] = uv[3];
+    if (nfree_out) *nfree_out = nfree;
+    DevBuf<uint32_t> d_free(nvars);
+    DevBuf<double> d_values(nvars);
+    copy_in(d_free.get(), h_free.data(), size_t(nvars) * 4, s);
+    copy_in(d_values.get(), h_values.data(), size_t(nvars) * 8, s);
+    LSInput in;
+    in.nrows = rows.nrows;
+    in.nvars = nvars;
+    in.nsys = 1;
+    in.nfree = nfree;
+    in.nent = 6ull * rows.nrows;
+    in.rp = rows.rp.get();
+    in.var = rows.var.get();
+    in.coef = rows.coef.get();
+    in.free_index = d_free.get();
+    in.values = d_values.get();
+    LSOutput out;
+    ls_assemble_device(in, out, s);
+    if (rhs) {
+        NLHIP_CHECK(hipMemcpyAsync(rhs, out.rhs.get(), size_t(nfree) * 8, hipMemcpyDefault, s));
+        NLHIP_CHECK(hipStreamSynchronize(s));
+    }
+    return system_adopt(nfree, out.nnz, std::move(out.rowptr), std::move(out.colind),
+                        std::move(out.val), flags);
+}
+
 }  // namespace nlhip
 
 using namespace nlhip;
@@ -102,6 +160,22 @@ NLAPI int NLAPIENTRY nlhipLeastSquaresCreate(uint32_t nb_variables, uint32_t nb_
         *out = nullptr;
         System* S = ls_create(nb_variables, nb_rows, row_ptr, var, coef, nb_systems, row_rhs,
                               locked, values, flags, rhs, nb_free);
+        *out = reinterpret_cast<nlhip_system>(S);
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipLSCMCreate(uint32_t nb_vertices, const double* points,
+                                     uint32_t nb_triangles, const uint32_t* triangles,
+                                     uint32_t pin0, double u0, double v0, uint32_t pin1,
+                                     double u1, double v1, uint32_t flags, nlhip_system* out,
+                                     double* rhs, uint32_t* nb_free) {
+    return guarded("nlhipLSCMCreate", [&] {
+        if (!out) throw Error(NLHIP_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        const uint32_t pin[2] = {pin0, pin1};
+        const double uv[4] = {u0, v0, u1, v1};
+        System* S = lscm_create(nb_vertices, points, nb_triangles, triangles, pin, uv, flags,
+                                rhs, nb_free);
         *out = reinterpret_cast<nlhip_system>(S);
     });
 }

@@ -443,6 +443,16 @@ struct LSOutput {
     uint64_t nnz = 0;
 };
 void ls_assemble_device(const LSInput& in, LSOutput& out, hipStream_t s);
+// LSCM rows of a triangle mesh on the device (points nv x 3, triangles nt x 3,
+// device arrays): two rows of 6 entries per non-degenerate triangle, in
+// triangle order, over the variables (2 v, 2 v + 1) = (u_v, v_v).
+struct LSRows {
+    uint32_t nrows = 0;
+    DevBuf<uint32_t> rp, var;
+    DevBuf<double> coef;
+};
+void lscm_rows_device(uint32_t nv, const double* points, uint32_t nt, const uint32_t* triangles,
+                      LSRows& rows, hipStream_t s);
 // nlhipLeastSquaresCreate: host or device inputs, checks, assembly, system;
 // rhs (nsys x nfree, host or device) may be null.
 System* ls_create(uint32_t nvars, uint32_t nrows, const uint32_t* row_ptr, const uint32_t* var,

@@ -124,6 +124,19 @@ NLAPI int NLAPIENTRY nlhipLeastSquaresCreate(uint32_t nb_variables, uint32_t nb_
                                              const double* values, uint32_t flags,
                                              nlhip_system* out, double* rhs,
                                              uint32_t* nb_free);
+/* The LSCM system of a triangle mesh built entirely on the device: per
+ * non-degenerate triangle the two conformality rows of Levy et al. 2002
+ * (what geogram's LSCM hands to a least-squares context for
+ * mesh_make_atlas / parameterize_chart, mesh_grob_surface_commands.cpp:
+ * 724-779), vertices pin0 and pin1 locked at (u0, v0) and (u1, v1), then
+ * nlhipLeastSquaresCreate's assembly.  Variables are (u_v, v_v) = (2v, 2v+1);
+ * the system is over the free ones in that order; rhs (nb_free) may be NULL.
+ * points: nb_vertices x 3, triangles: nb_triangles x 3; host or device. */
+NLAPI int NLAPIENTRY nlhipLSCMCreate(uint32_t nb_vertices, const double* points,
+                                     uint32_t nb_triangles, const uint32_t* triangles,
+                                     uint32_t pin0, double u0, double v0, uint32_t pin1,
+                                     double u1, double v1, uint32_t flags, nlhip_system* out,
+                                     double* rhs, uint32_t* nb_free);
 /* Launch shape of the SpMV engine: rows per wavefront (2, 4, 8, 16: that many
  * rows share a 64-lane wavefront; 64: one row per lane, nonzeros streamed
  * through LDS, the default) and workgroups per launch.  65..76 select

@@ -114,6 +114,45 @@ def test_lscm_rows_match_host_assembly(hip):
     S.close()
 
 
+@pytest.mark.parametrize("kind", ["chart", "grid"])
+def test_lscm_system_built_on_device(hip, kind):
+    """nlhipLSCMCreate: rows and normal equations on the device, bitwise the
+    host builder's system (coefficients included: sqrt and divisions
+    correctly rounded, nothing fused)."""
+    mesh = W.chart_mesh(5000, seed=2) if kind == "chart" else W.grid_mesh(60, seed=3)
+    pins = mesh.extreme_pins()
+    uv = (0.25, -1.0, 3.0, 2.5)
+    S, b = capi.System.lscm(mesh.points, mesh.triangles, pins, uv)
+    ref = W.lscm_system(mesh, pins=pins, pin_uv=uv)
+    assert_same(S, b, O.CRS(ref.rowptr, ref.colind, ref.val, ref.n), ref.b)
+    S.close()
+
+
+def test_lscm_degenerate_triangles_are_skipped(hip):
+    """Zero-area triangles contribute no rows: the system equals the one
+    built without them."""
+    mesh = W.grid_mesh(40, seed=6)
+    P, T = mesh.points.copy(), mesh.triangles.copy()
+    T[5] = [T[5][0], T[5][0], T[5][1]]          # repeated vertex
+    P[T[9][2]] = P[T[9][0]]                     # coincident vertices
+    pins = mesh.extreme_pins()
+    uv = (0.0, 0.0, 1.0, 1.0)
+    S1, b1 = capi.System.lscm(P, T, pins, uv)
+    keep = np.ones(len(T), dtype=bool)
+    keep[[5, 9]] = False
+    S2, b2 = capi.System.lscm(P, T[keep], pins, uv)
+    rp1, ci1, va1 = S1.crs()
+    rp2, ci2, va2 = S2.crs()
+    assert np.array_equal(rp1, rp2) and np.array_equal(ci1, ci2)
+    assert np.array_equal(bits(va1), bits(va2)) and np.array_equal(bits(b1), bits(b2))
+    with pytest.raises(capi.NLHIPError):        # vertex index out of range
+        capi.System.lscm(P, np.array([[0, 1, len(P)]]), pins, uv)
+    with pytest.raises(capi.NLHIPError):        # the two pins coincide
+        capi.System.lscm(P, T, (pins[0], pins[0]), uv)
+    S1.close()
+    S2.close()
+
+
 def test_context_rows_assemble_on_device(hip):
     """nlBegin/nlEnd(NL_ROW) in least-squares mode: NL_NNZ and the solution
     come from the device assembly of the recorded rows."""
