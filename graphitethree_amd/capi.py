@@ -88,6 +88,12 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class EigenStats(C.Structure):
+    _fields_ = [("outer_iterations", C.c_uint32), ("converged", C.c_uint32),
+                ("inner_iterations", C.c_uint64), ("max_rel_residual", C.c_double),
+                ("seconds", C.c_double)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active", C.c_uint32)]
 
@@ -167,6 +173,8 @@ SIGNATURES = {
                                     C.POINTER(_vp)]),
     "nlhipSystemDestroy": (C.c_int, [_vp]),
     "nlhipSystemGetCRS": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "nlhipEigenSolve": (C.c_int, [_vp, _vp, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
+                                  _vp, _vp, C.POINTER(EigenStats)]),
     "nlhipLSCMCreate": (C.c_int, [C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32, C.c_double,
                                   C.c_double, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
                                   C.POINTER(_vp), _vp, _u32p]),
@@ -335,6 +343,19 @@ class System:
         S._h = h
         S.n = int(nfree.value)
         return S, rhs[:S.n]
+
+    def eigen(self, k: int, mass=None, shift: float = 0.0, eps: float = 1e-8,
+              max_iter: int = 1000):
+        """k eigenpairs of A x = lambda M x nearest above `shift`
+        (nlhipEigenSolve).  Returns (values (k,), vectors (k, n), stats)."""
+        vals = np.zeros(k)
+        vecs = np.zeros((k, self.n))
+        m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+        st = EigenStats()
+        check(lib().nlhipEigenSolve(self._h, ptr(m) if m is not None else None, k, float(shift),
+                                    float(eps), int(max_iter), ptr(vals), ptr(vecs),
+                                    C.byref(st)), "nlhipEigenSolve")
+        return vals, vecs, st
 
     def crs(self):
         """The resident CRS (rowptr, colind, val) copied back."""

@@ -379,7 +379,7 @@ System* lscm_system(const Mesh& M, const uint32_t pin[2], const double uv[4]) {
 // edges, L_ii = -sum_j L_ij).  Entries of each row accumulate over the
 // vertex's incident triangles in increasing order.
 System* cotan_system(const Mesh& M, double tstep, uint32_t nrhs, int rhs_kind,
-                     uint64_t seed) {
+                     uint64_t seed, bool identity = true) {
     const uint32_t nv = M.nv(), nt = M.nt();
     System* S = new System;
     S->n = nv;
@@ -451,7 +451,7 @@ System* cotan_system(const Mesh& M, double tstep, uint32_t nrhs, int rhs_kind,
             }
             uint32_t pos = rb;
             while (S->colind[pos] != a) ++pos;
-            S->val[pos] = S->val[pos] + (1.0 + tstep * diag);
+            S->val[pos] = S->val[pos] + (identity ? 1.0 + tstep * diag : tstep * diag);
         }
     }
     S->rhs.assign(size_t(nv) * nrhs, 0.0);
@@ -603,6 +603,27 @@ void* wl_system_lscm(const void* mp, uint32_t pin0, double u0, double v0,
 
 // A = I + t L; rhs_kind 0: columns x, y, z, x, ... of the vertices;
 // rhs_kind 1: standard normal entries drawn from `seed`.
+// The cotangent stiffness matrix L (no identity term; rows sum to 0) and the
+// lumped mass of each vertex (a third of the area of its triangles).
+void* wl_system_laplacian(const void* mp, double* mass) {
+    const Mesh& M = *static_cast<const Mesh*>(mp);
+    for (uint32_t v = 0; v < M.nv(); ++v) mass[v] = 0.0;
+    for (uint32_t t = 0; t < M.nt(); ++t) {
+        const uint32_t* q = &M.T[3 * size_t(t)];
+        const double* p0 = &M.P[3 * size_t(q[0])];
+        const double* p1 = &M.P[3 * size_t(q[1])];
+        const double* p2 = &M.P[3 * size_t(q[2])];
+        double e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        double e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+        double cx = e1[1] * e2[2] - e1[2] * e2[1];
+        double cy = e1[2] * e2[0] - e1[0] * e2[2];
+        double cz = e1[0] * e2[1] - e1[1] * e2[0];
+        double a3 = std::sqrt(cx * cx + cy * cy + cz * cz) / 6.0;
+        for (int k = 0; k < 3; ++k) mass[q[k]] += a3;
+    }
+    return cotan_system(M, 1.0, 1, 0, 0, false);
+}
+
 void* wl_system_cotan(const void* mp, double t, uint32_t nrhs, int rhs_kind,
                       uint64_t seed) {
     return cotan_system(*static_cast<const Mesh*>(mp), t, nrhs, rhs_kind, seed);

@@ -42,6 +42,7 @@ def lib() -> C.CDLL:
                                             C.c_double]),
             "wl_system_cotan": (C.c_void_p, [C.c_void_p, C.c_double, C.c_uint32,
                                              C.c_int, C.c_uint64]),
+            "wl_system_laplacian": (C.c_void_p, [C.c_void_p, _f64p]),
             "wl_sys_n": (C.c_uint32, [C.c_void_p]),
             "wl_sys_nnz": (C.c_uint64, [C.c_void_p]),
             "wl_sys_nrhs": (C.c_uint32, [C.c_void_p]),
@@ -194,6 +195,14 @@ def cotan_system(mesh: Mesh, t: float = 1e-3, nrhs: int = 3, rhs: str = "xyz",
     """(I + t L) with L the cotangent Laplacian; rhs 'xyz' or 'gaussian'."""
     kind = 0 if rhs == "xyz" else 1
     return _system_from(lib().wl_system_cotan(mesh._h, t, nrhs, kind, seed), False)
+
+
+def laplacian(mesh: Mesh):
+    """(L, mass): the cotangent stiffness matrix (symmetric, rows sum to 0)
+    and the lumped mass of each vertex, for L x = lambda M x."""
+    mass = np.zeros(mesh.nv, dtype=np.float64)
+    S = _system_from(lib().wl_system_laplacian(mesh._h, mass.ctypes.data_as(_f64p)), False)
+    return S, mass
 
 
 def lscm_grid(k: int, seed: int = 1, flat: bool = False) -> SystemCRS:
