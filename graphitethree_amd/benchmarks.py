@@ -3,6 +3,7 @@
     python -m graphitethree_amd.benchmarks atlas [--count 2000] [--seed 1]
     python -m graphitethree_amd.benchmarks multirhs [--grid 1500] [--ks 3 4 8 16 32]
     python -m graphitethree_amd.benchmarks serving [--count 120] [--rate 10]
+    python -m graphitethree_amd.benchmarks eigen [--grid 700] [--k 10]
 
 atlas: `count` disk-like LSCM charts (N ~ logU[50, 50000] vertices) solved
   (1) as one batch in FP64, (2) as one batch with FP32 inner PCG inside FP64
@@ -16,6 +17,10 @@ multirhs: the (I + 1e-3 L) cotangent-Laplacian system of a k x k noisy grid
   (default 1500 x 1500, n = 2.25 M) with 3..32 N(0,1) right-hand sides,
   solved as one multi-right-hand-side PCG (FP64, and FP32 inner with FP64
   refinement) against one FP64 PCG per right-hand side.
+
+eigen: the k smallest eigenpairs of a grid mesh's cotangent Laplacian with
+  lumped mass (manifold harmonics), nlhipEigenSolve against scipy's
+  shift-invert ARPACK with a SuperLU factorization on the host.
 
 serving: a stream of requests with Poisson arrivals (rate --rate per second):
   60% LSCM charts with N ~ logU[1e3, 1e6] vertices, 40% smoothing solves
@@ -254,6 +259,46 @@ def run_serving(args) -> dict:
     }
 
 
+def run_eigen(args) -> dict:
+    """k smallest eigenpairs of a mesh Laplacian with lumped mass (manifold
+    harmonics): the device solver against scipy's shift-invert ARPACK with a
+    SuperLU factorization (the reference's ARPACK-extension configuration)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    dev = init_device()
+    mesh = W.grid_mesh(args.grid, seed=args.seed)
+    L, mass = W.laplacian(mesh)
+    log(f"eigen: grid {args.grid}^2, n={L.n}, nnz={L.nnz}, k={args.k}")
+    S = capi.System.from_crs(L)
+    vals, vecs, st = S.eigen(args.k, mass=mass, shift=args.shift, eps=args.eps)
+    t_gpu = st.seconds
+    vals, vecs, st = S.eigen(args.k, mass=mass, shift=args.shift, eps=args.eps)
+    t_gpu = min(t_gpu, st.seconds)
+    log(f"  device: {t_gpu:.3f}s, {st.outer_iterations} outer / {st.inner_iterations} block-PCG "
+        f"iterations, {st.converged}/{args.k} below {args.eps:g}")
+    res = {"metric": "eigenpairs: seconds for k smallest of L x = lambda M x",
+           "workload": f"laplacian_grid_{args.grid}_k{args.k}", "device": dev[0],
+           "n": L.n, "nnz": L.nnz, "k": args.k, "shift": args.shift, "eps": args.eps,
+           "device_s": t_gpu, "outer_iterations": int(st.outer_iterations),
+           "block_pcg_iterations": int(st.inner_iterations), "converged": int(st.converged),
+           "max_rel_residual": st.max_rel_residual, "eigenvalues": [float(v) for v in vals]}
+    if not args.no_cpu:
+        A = L.to_scipy().tocsc()
+        M = sp.diags(mass).tocsc()
+        t0 = time.perf_counter()
+        ref = np.sort(spla.eigsh(A, k=args.k, M=M, sigma=args.shift, which="LM",
+                                 tol=args.eps)[0])
+        t_cpu = time.perf_counter() - t0
+        res["cpu_arpack_superlu_s"] = t_cpu
+        res["max_rel_diff_eigenvalues"] = float(np.max(np.abs(vals - ref) /
+                                                       np.maximum(np.abs(ref), 1e-12)))
+        res["speedup"] = t_cpu / t_gpu
+        log(f"  cpu (scipy eigsh shift-invert, SuperLU): {t_cpu:.2f}s, speedup "
+            f"{t_cpu / t_gpu:.1f}x")
+    S.close()
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -275,6 +320,13 @@ def main(argv=None):
     mr.add_argument("--eps", type=float, default=1e-10)
     mr.add_argument("--max-iter", type=int, default=5000)
     mr.add_argument("--inner-eps", type=float, default=1e-4)
+    ev = sub.add_parser("eigen")
+    ev.add_argument("--grid", type=int, default=300)
+    ev.add_argument("--k", type=int, default=10)
+    ev.add_argument("--shift", type=float, default=-1.0)
+    ev.add_argument("--eps", type=float, default=1e-8)
+    ev.add_argument("--seed", type=int, default=1)
+    ev.add_argument("--no-cpu", action="store_true")
     sv = sub.add_parser("serving")
     sv.add_argument("--count", type=int, default=120)
     sv.add_argument("--rate", type=float, default=10.0)
@@ -292,6 +344,8 @@ def main(argv=None):
         out = run_atlas(args)
     elif args.cmd == "multirhs":
         out = run_multirhs(args)
+    elif args.cmd == "eigen":
+        out = run_eigen(args)
     print(json.dumps(out), flush=True)
 
 
