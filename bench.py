@@ -173,6 +173,10 @@ def main():
                     help="oracle PCG iterations timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--workgroups", type=int, default=0,
+                    help="SpMV workgroups (with --rows-per-wave: a fixed launch shape)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape instead of the autotuner's")
     ap.add_argument("--no-assembly", action="store_true",
                     help="skip the least-squares assembly measurement")
     ap.add_argument("--assembly-sample-rows", type=int, default=2_000_000,
@@ -204,8 +208,13 @@ def main():
     log(f"LSCM grid {args.k}^2: n={n} nnz={nnz} built in {time.time() - t0:.1f}s")
     t0 = time.time()
     S = capi.System.from_crs(Sw, fp32=not args.no_mixed)
-    if args.rows_per_wave:
-        S.set_launch(args.rows_per_wave, S.launch()[1])
+    tuned = None
+    if args.rows_per_wave or args.workgroups:
+        S.set_launch(args.rows_per_wave or S.launch()[0], args.workgroups or S.launch()[1])
+    elif not args.no_autotune and not args.traffic_probe:
+        tuned = S.autotune(iters=20)
+        log(f"autotuned: engine {tuned[0]}, {tuned[1]} workgroups, "
+            f"{tuned[2] * 1e3:.3f} ms per CG iteration")
     lpr, wg = S.launch()
     log(f"uploaded in {time.time() - t0:.1f}s; launch rows/wavefront={lpr} workgroups={wg}")
 
@@ -343,7 +352,7 @@ def main():
                    "solver": "Jacobi-PCG fp64", "eps": args.eps, "max_iter": args.max_iter,
                    "iterations": iters, "converged": bool(stats.converged),
                    "rel_residual": stats.rel_residual, "rows_per_wave": lpr,
-                   "workgroups": wg},
+                   "workgroups": wg, "autotuned": tuned is not None},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "spmv_gbs": {"fp64": spmv_gbs, "fp32": spmv32_gbs,
                      "bytes_fp64": spmv_bytes(n, nnz)},

@@ -63,6 +63,10 @@ def run_atlas(args) -> dict:
     B = capi.Batch(packed=(A.row_off, A.rowptr, A.colind, A.val), fp32=True)
     params = dict(eps=args.eps, max_iter=args.max_iter)
 
+    tuned = B.autotune(A.b, iters=args.tune_iters) if args.tune_iters else None
+    if tuned:
+        log(f"autotuned: {tuned[0]} workgroups, SpMV passes {tuned[1] or 'by active tiles'}, "
+            f"{tuned[2] * 1e3:.3f} ms per iteration over the first {args.tune_iters}")
     x64, p64, t64 = B.solve(A.b, **params)            # warm-up (graph capture)
     x64, p64, t64 = B.solve(A.b, **params)
     log(f"batched fp64: {t64.device_seconds:.3f}s, max it {t64.iterations}, "
@@ -99,6 +103,8 @@ def run_atlas(args) -> dict:
         "workload": f"atlas_{A.count}_charts_N{int(args.nmin)}-{int(args.nmax)}",
         "charts": A.count, "n": A.n, "nnz": A.nnz, "eps": args.eps,
         "device": dev[0],
+        "autotuned": None if not tuned else {"workgroups": tuned[0], "spmv_passes": tuned[1],
+                                             "ms_per_iteration_first_iters": tuned[2] * 1e3},
         "batched_fp64": {"seconds": t64.device_seconds, "solves_per_s": A.count / t64.device_seconds,
                          "max_iterations": int(t64.iterations), "max_iter": args.max_iter,
                          "converged": int(sum(p.converged for p in p64)),
@@ -312,6 +318,8 @@ def main(argv=None):
     a.add_argument("--max-iter-mixed", type=int, default=200000)
     a.add_argument("--inner-eps", type=float, default=1e-4)
     a.add_argument("--baseline-stride", type=int, default=1)
+    a.add_argument("--tune-iters", type=int, default=50,
+                   help="iterations per candidate of the batch autotuner (0: no tuning)")
     mr = sub.add_parser("multirhs")
     mr.add_argument("--grid", type=int, default=1500)
     mr.add_argument("--t", type=float, default=1e-3)

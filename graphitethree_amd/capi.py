@@ -185,6 +185,9 @@ SIGNATURES = {
                                   C.POINTER(C.c_uint64)]),
     "nlhipSystemSetLaunch": (C.c_int, [_vp, C.c_int, C.c_uint32]),
     "nlhipSystemGetLaunch": (C.c_int, [_vp, C.POINTER(C.c_int), _u32p]),
+    "nlhipSystemAutotune": (C.c_int, [_vp, C.c_int, C.c_uint32, C.POINTER(C.c_int), _u32p,
+                                      _f64p]),
+    "nlhipBatchAutotune": (C.c_int, [_vp, _vp, C.c_uint32, _u32p, _u32p, _f64p]),
     "nlhipSolve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
                              C.POINTER(SolveStats)]),
     "nlhipSolveMulti": (C.c_int, [_vp, C.c_uint32, _vp, _vp, C.POINTER(SolveParams),
@@ -386,6 +389,14 @@ class System:
     def set_launch(self, rows_per_wave: int, workgroups: int):
         check(lib().nlhipSystemSetLaunch(self._h, rows_per_wave, workgroups), "set launch")
 
+    def autotune(self, fp32: bool = False, iters: int = 20):
+        """Times the launch shapes (nlhipSystemAutotune); keeps and returns the
+        fastest (rows per wavefront, workgroups, seconds per CG iteration)."""
+        rpw, wg, t = C.c_int(), C.c_uint32(), C.c_double()
+        check(lib().nlhipSystemAutotune(self._h, 1 if fp32 else 0, iters, C.byref(rpw),
+                                        C.byref(wg), C.byref(t)), "nlhipSystemAutotune")
+        return int(rpw.value), int(wg.value), float(t.value)
+
     def info(self):
         n, nnz, b = C.c_uint32(), C.c_uint64(), C.c_uint64()
         check(lib().nlhipSystemInfo(self._h, C.byref(n), C.byref(nnz), C.byref(b)), "info")
@@ -476,6 +487,15 @@ class Batch:
 
     def set_launch(self, workgroups: int):
         check(lib().nlhipBatchSetLaunch(self._h, workgroups), "nlhipBatchSetLaunch")
+
+    def autotune(self, b, iters: int = 50):
+        """Times grid and SpMV run length (nlhipBatchAutotune); keeps and returns
+        the fastest (workgroups, passes, seconds per iteration)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        wg, ps, t = C.c_uint32(), C.c_uint32(), C.c_double()
+        check(lib().nlhipBatchAutotune(self._h, ptr(b), iters, C.byref(wg), C.byref(ps),
+                                       C.byref(t)), "nlhipBatchAutotune")
+        return int(wg.value), int(ps.value), float(t.value)
 
     def set_spmv_passes(self, passes: int):
         """SpMV passes per wavefront: 1, 2, 4, or 0 (by active tiles)."""

@@ -561,6 +561,81 @@ NLAPI int NLAPIENTRY nlhipBatchSetSpmvPasses(nlhip_batch b, uint32_t passes) {
     });
 }
 
+// Autotuner: times `iters` FP64 iterations of the batched PCG from x = 0 on
+// `rhs` for workgroup counts of 4, 16 and 64 per CU and for each SpMV run
+// length (1, 2, 4 passes per wavefront, or chosen per launch by the active
+// tiles), and keeps the fastest.  The first iterations have every system
+// active, so a fixed run length must beat the adaptive one (and any shape
+// the current one) by 2 % to be taken.  None of these changes a bit of the
+// result.
+NLAPI int NLAPIENTRY nlhipBatchAutotune(nlhip_batch b, const double* rhs, uint32_t iters,
+                                        uint32_t* workgroups, uint32_t* passes,
+                                        double* seconds) {
+    return guarded("nlhipBatchAutotune", [&] {
+        Batch* B = bat(b);
+        if (!rhs) throw Error(NLHIP_ERR_INVALID, "null b");
+        if (iters == 0) iters = 50;
+        std::lock_guard<std::mutex> lock(B->mu);
+        hipStream_t s = B->stream.get();
+        copy_in(B->w64.b.get(), rhs, size_t(B->n) * 8, s);
+        nlhip_solve_params p;
+        nlhipSolveParamsDefault(&p);
+        p.eps = 0.0;  // run every iteration
+        p.max_iter = iters;
+        const uint32_t cu = uint32_t(std::max(1, device().cu_count));
+        std::vector<uint32_t> grids;
+        for (uint32_t m : {4u, 16u, 64u}) {
+            const uint32_t g = uint32_t(std::max<uint64_t>(
+                1, std::min<uint64_t>(uint64_t(cu) * m, B->ntiles)));
+            if (std::find(grids.begin(), grids.end(), g) == grids.end()) grids.push_back(g);
+        }
+        const uint32_t grid0 = B->grid, passes0 = B->spmv_passes;
+        uint32_t best_g = grid0, best_p = passes0;
+        double best = 1e300;
+        std::vector<int32_t> its;
+        std::vector<double> cgrel;
+        std::vector<uint8_t> conv;
+        // the current shape first, then the others
+        std::vector<std::pair<uint32_t, uint32_t>> shapes{{grid0, passes0}};
+        for (uint32_t g : grids)
+            for (uint32_t ps : {0u, 1u, 2u, 4u})
+                if (!(g == grid0 && ps == passes0)) shapes.emplace_back(g, ps);
+        for (const auto& shape : shapes) {
+            const uint32_t g = shape.first, ps = shape.second;
+            B->grid = g;
+            B->spmv_passes = ps;
+            set_spmv_runs(*B, B->w64.args);
+            set_spmv_runs(*B, B->w32.args);
+            B->w64.graph.reset();
+            B->w32.graph.reset();
+            double t = 0.0;
+            for (int rep = 0; rep < 2; ++rep) {  // the first captures the graphs
+                NLHIP_CHECK(hipMemsetAsync(B->w64.x.get(), 0, size_t(B->n) * 8, s));
+                Event e0, e1;
+                e0.record(s);
+                batch_solve_fp64(*B, p, its, cgrel, conv);
+                e1.record(s);
+                e1.sync();
+                t = 1e-3 * elapsed_ms(e0, e1);
+            }
+            if (best == 1e300 || t < 0.98 * best) {
+                best = t;
+                best_g = g;
+                best_p = ps;
+            }
+        }
+        B->grid = best_g;
+        B->spmv_passes = best_p;
+        set_spmv_runs(*B, B->w64.args);
+        set_spmv_runs(*B, B->w32.args);
+        B->w64.graph.reset();
+        B->w32.graph.reset();
+        if (workgroups) *workgroups = best_g;
+        if (passes) *passes = best_p;
+        if (seconds) *seconds = best / iters;
+    });
+}
+
 NLAPI int NLAPIENTRY nlhipBatchSolve(nlhip_batch b, const double* rhs, double* x,
                                      const nlhip_solve_params* params,
                                      nlhip_solve_stats* per_system, nlhip_solve_stats* total) {

@@ -413,6 +413,15 @@ bool build_blocks(System& S);
 bool build_col_offsets(uint64_t nnz, const uint32_t* colind, hipStream_t s,
                        DevBuf<uint16_t>& ci16, DevBuf<uint32_t>& segb);
 
+// Launch-shape autotuning of a system's CG (S.mu held).
+struct TuneResult {
+    int rpw;
+    uint32_t grid;
+    double seconds;  // per CG iteration
+};
+void profile_cg(System& S, bool fp32, uint32_t iters, double t[3]);
+TuneResult autotune(System& S, bool fp32, uint32_t iters);
+
 // Validates and uploads; throws Error.
 System* system_create(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
                       const double* val, uint32_t flags);

@@ -605,6 +605,97 @@ uint32_t gmres_device(System& S, const double* dinv, const double* b, double* x,
 
 using namespace nlhip;
 
+namespace nlhip {
+
+// Mean seconds of the three CG launches (direction, SpMV, update) over
+// `iters` iterations of the system's launch shape (HIP events on its stream;
+// S.mu held).
+void profile_cg(System& S, bool fp32, uint32_t iters, double t[3]) {
+    hipStream_t st = S.stream.get();
+    if (iters == 0) iters = 1;
+    auto profile = [&](auto& ws, auto* dinv, auto zero) {
+        using T = decltype(zero);
+        ws.ensure(S.n, S.grid);
+        setup_args<T>(S, ws, dinv);
+        NLHIP_CHECK(fill<T>(S.n, T(1), ws.b.get(), st));
+        NLHIP_CHECK(fill<T>(S.n, T(0), ws.x.get(), st));
+        NLHIP_CHECK(cg_start<T>(ws.args, ws.code, true, T(0), int(iters) + 8, st));
+        std::vector<Event> ev(4 * size_t(iters));
+        for (uint32_t k = 0; k < iters; ++k) {
+            for (int part = 0; part < 3; ++part) {
+                ev[4 * k + part].record(st);
+                NLHIP_CHECK(cg_step_part<T>(ws.args, ws.code, true, part, st));
+            }
+            ev[4 * k + 3].record(st);
+        }
+        S.stream.sync();
+        double acc[3] = {0, 0, 0};
+        for (uint32_t k = 0; k < iters; ++k)
+            for (int part = 0; part < 3; ++part)
+                acc[part] += 1e-3 * elapsed_ms(ev[4 * k + part], ev[4 * k + part + 1]);
+        for (int part = 0; part < 3; ++part) t[part] = acc[part] / iters;
+    };
+    if (fp32) {
+        S.ensure_fp32();
+        profile(S.ws32, S.dinv32.get(), 0.0f);
+    } else {
+        profile(S.ws64, S.dinv.get(), 0.0);
+    }
+}
+
+// Autotuner: times `iters` CG iterations (the three launches) for each
+// candidate launch shape and keeps the fastest.  Candidates: the streamed
+// SpMV engines, which give the same row sums (the 2x2 block engine with 1,
+// 2, 4 or 8 blocks per lane per chunk when the matrix has the block form;
+// the scalar engine with 16-bit or 32-bit columns, contiguous or
+// XCD-interleaved), times workgroup counts of 1, 2, 4 and 8 per CU (the
+// dot products are summed per workgroup, so the count fixes their order:
+// a tuned system gives the same bits on every solve).  A shape replaces the
+// current one only when 2 % faster; the shape found stays with the system.
+TuneResult autotune(System& S, bool fp32, uint32_t iters) {
+    std::vector<int> codes;
+    if (S.blocks()) codes = {73, 74, 75, 76};
+    if (S.ci16.get()) {
+        codes.push_back(68);
+        codes.push_back(70);
+    }
+    codes.push_back(71);
+    codes.push_back(69);
+    const uint32_t cu = uint32_t(std::max(1, device().cu_count));
+    const uint64_t max_grid = std::max<uint64_t>(1, (uint64_t(S.n) + 63) / 64);
+    std::vector<uint32_t> grids;
+    for (uint32_t m : {1u, 2u, 4u, 8u}) {
+        const uint32_t g = uint32_t(std::min<uint64_t>(uint64_t(cu) * m, max_grid));
+        if (std::find(grids.begin(), grids.end(), g) == grids.end()) grids.push_back(g);
+    }
+    const int rpw0 = S.rpw;
+    const uint32_t grid0 = S.grid;
+    TuneResult best{rpw0, grid0, 1e300};
+    double t[3];
+    // the current shape first; another replaces the best only when 2 %
+    // faster (timing noise does not move the order of the sums)
+    std::vector<std::pair<int, uint32_t>> shapes{{rpw0, grid0}};
+    for (int code : codes)
+        for (uint32_t g : grids)
+            if (!(code == rpw0 && g == grid0)) shapes.emplace_back(code, g);
+    for (const auto& shape : shapes) {
+        S.rpw = shape.first;
+        S.grid = shape.second;
+        profile_cg(S, fp32, 2, t);  // warm-up (argument set-up, caches)
+        profile_cg(S, fp32, iters, t);
+        const double it_s = t[0] + t[1] + t[2];
+        if (best.seconds == 1e300 || it_s < 0.98 * best.seconds)
+            best = TuneResult{shape.first, shape.second, it_s};
+    }
+    S.rpw = best.rpw;
+    S.grid = best.grid;
+    S.ws64.graph.reset();
+    S.ws32.graph.reset();
+    return best;
+}
+
+}  // namespace nlhip
+
 static System* sys(nlhip_system h) {
     if (!h) throw Error(NLHIP_ERR_INVALID, "null system handle");
     return reinterpret_cast<System*>(h);
@@ -789,37 +880,23 @@ NLAPI int NLAPIENTRY nlhipProfileCG(nlhip_system s, int fp32, uint32_t iters, do
     return guarded("nlhipProfileCG", [&] {
         System* S = sys(s);
         std::lock_guard<std::mutex> lock(S->mu);
-        hipStream_t st = S->stream.get();
-        if (iters == 0) iters = 1;
-        auto profile = [&](auto& ws, auto* dinv, auto zero) {
-            using T = decltype(zero);
-            ws.ensure(S->n, S->grid);
-            setup_args<T>(*S, ws, dinv);
-            NLHIP_CHECK(fill<T>(S->n, T(1), ws.b.get(), st));
-            NLHIP_CHECK(fill<T>(S->n, T(0), ws.x.get(), st));
-            NLHIP_CHECK(cg_start<T>(ws.args, ws.code, true, T(0), int(iters) + 8, st));
-            std::vector<Event> ev(4 * size_t(iters));
-            for (uint32_t k = 0; k < iters; ++k) {
-                for (int part = 0; part < 3; ++part) {
-                    ev[4 * k + part].record(st);
-                    NLHIP_CHECK(cg_step_part<T>(ws.args, ws.code, true, part, st));
-                }
-                ev[4 * k + 3].record(st);
-            }
-            S->stream.sync();
-            double t[3] = {0, 0, 0};
-            for (uint32_t k = 0; k < iters; ++k)
-                for (int part = 0; part < 3; ++part)
-                    t[part] += 1e-3 * elapsed_ms(ev[4 * k + part], ev[4 * k + part + 1]);
-            for (int part = 0; part < 3; ++part)
-                if (seconds) seconds[part] = t[part] / iters;
-        };
-        if (fp32) {
-            S->ensure_fp32();
-            profile(S->ws32, S->dinv32.get(), 0.0f);
-        } else {
-            profile(S->ws64, S->dinv.get(), 0.0);
-        }
+        double t[3];
+        profile_cg(*S, fp32 != 0, iters, t);
+        if (seconds)
+            for (int part = 0; part < 3; ++part) seconds[part] = t[part];
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipSystemAutotune(nlhip_system s, int fp32, uint32_t iters,
+                                         int* rows_per_wave, uint32_t* workgroups,
+                                         double* seconds_per_iteration) {
+    return guarded("nlhipSystemAutotune", [&] {
+        System* S = sys(s);
+        std::lock_guard<std::mutex> lock(S->mu);
+        const TuneResult r = autotune(*S, fp32 != 0, iters ? iters : 20);
+        if (rows_per_wave) *rows_per_wave = r.rpw;
+        if (workgroups) *workgroups = r.grid;
+        if (seconds_per_iteration) *seconds_per_iteration = r.seconds;
     });
 }
 

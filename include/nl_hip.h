@@ -171,6 +171,16 @@ NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,
                                           uint32_t workgroups);
 NLAPI int NLAPIENTRY nlhipSystemGetLaunch(nlhip_system s, int* rows_per_wave,
                                           uint32_t* workgroups);
+/* Autotuner: times `iters` CG iterations (0: 20) for each streamed SpMV
+ * engine (2x2 blocks with 1, 2, 4, 8 blocks per lane per chunk; scalar with
+ * 16/32-bit columns, contiguous or XCD-interleaved; same row sums) and 1, 2,
+ * 4, 8 workgroups per CU, and keeps the fastest as the system's launch
+ * shape (returned, with the seconds per iteration).  The workgroup count
+ * fixes the order of the dot-product sums, so a tuned system's solves are
+ * reproducible; they stay within the solver's tolerance of the untuned. */
+NLAPI int NLAPIENTRY nlhipSystemAutotune(nlhip_system s, int fp32, uint32_t iters,
+                                         int* rows_per_wave, uint32_t* workgroups,
+                                         double* seconds_per_iteration);
 /* Solves A x = b; x holds the initial guess.  b, x host or device. */
 NLAPI int NLAPIENTRY nlhipSolve(nlhip_system s, const double* b, double* x,
                                 const nlhip_solve_params* params,
@@ -226,6 +236,12 @@ NLAPI int NLAPIENTRY nlhipBatchSetLaunch(nlhip_batch b, uint32_t workgroups);
  * wavefront of the SpMV streams: 1, 2 or 4, or 0 (the default) to choose
  * per launch from the number of active tiles.  Results do not depend on it. */
 NLAPI int NLAPIENTRY nlhipBatchSetSpmvPasses(nlhip_batch b, uint32_t passes);
+/* Autotuner: times `iters` (0: 50) FP64 iterations on rhs for workgroup
+ * counts of 4, 16, 64 per CU and each SpMV run length, keeps the fastest
+ * (returned, with seconds per iteration).  No result bit depends on it. */
+NLAPI int NLAPIENTRY nlhipBatchAutotune(nlhip_batch b, const double* rhs, uint32_t iters,
+                                        uint32_t* workgroups, uint32_t* passes,
+                                        double* seconds);
 /* Solves every system; b and x are the concatenated vectors (x holds the
  * initial guesses).  per_system (nsys entries) and total may be null; in
  * total, iterations is the largest count and outer_iterations the sum. */

@@ -117,6 +117,17 @@ def test_batch_spmv_passes_do_not_change_bits(hip, small_atlas):
         B.set_spmv_passes(3)
 
 
+def test_batch_autotune_changes_no_bit(hip, small_atlas):
+    A = small_atlas
+    B = capi.Batch([A.chart(c) for c in range(A.count)])
+    x0, p0, _ = B.solve(A.b)
+    wg, passes, t = B.autotune(A.b, iters=20)
+    assert wg >= 1 and passes in (0, 1, 2, 4) and t > 0
+    x1, p1, _ = B.solve(A.b)
+    assert np.array_equal(x0, x1)
+    assert [p.iterations for p in p0] == [p.iterations for p in p1]
+
+
 def test_batch_mixed_precision(hip, small_atlas):
     A = small_atlas
     B = capi.Batch([A.chart(c) for c in range(A.count)], fp32=True)

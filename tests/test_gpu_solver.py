@@ -341,3 +341,20 @@ def test_context_api_periodic_voxel_least_squares(hip):
     cpu = np.array([ctx.get_variable(i) for i in range(N)])
     assert rel(gpu, cpu) <= REL_DIFF
     assert np.array_equal(gpu[locked], data[locked])
+
+
+def test_autotuned_system_matches_oracle(hip):
+    """The autotuner picks a streamed engine and a workgroup count by timing;
+    the tuned solve stays within the tolerance of the oracle and gives the
+    same bits on every solve."""
+    Sw = W.lscm_grid(60, seed=7)
+    S = capi.System.from_crs(Sw)
+    rpw, wg, t = S.autotune(iters=10)
+    assert rpw in (68, 69, 70, 71, 73, 74, 75, 76) and wg >= 1 and t > 0
+    assert S.launch() == (rpw, wg)
+    x1, st = S.solve(Sw.b, eps=1e-10, max_iter=20000)
+    x2, _ = S.solve(Sw.b, eps=1e-10, max_iter=20000)
+    assert np.array_equal(x1, x2)
+    xr, itr, _ = O.pcg(O.CRS(Sw.rowptr, Sw.colind, Sw.val), Sw.b, eps=1e-10, max_iter=20000)
+    assert same_iters(st.iterations, itr) and rel(x1, xr) <= REL_DIFF
+    S.close()
