@@ -1,12 +1,14 @@
 // Kernels of the generalized symmetric eigensolver (A x = lambda M x, M a
-// positive diagonal): block shift-invert subspace iteration with
+// positive diagonal): Chebyshev-filtered subspace iteration with
 // Rayleigh-Ritz, the GPU counterpart of the ARPACK shift-invert solves
 // behind spectral LSCM and manifold harmonics (src/lib/OGF/mesh/commands/
 // mesh_grob_surface_commands.cpp:761-763, mesh_grob_spectral_commands.cpp:
-// 74-116).  Blocks of P vectors are column-major n x P.  Every reduction has
-// a fixed order (per workgroup in row order, then workgroups in index
-// order), so results do not depend on timing.
+// 74-116).  Blocks of P vectors are interleaved n x P (row r's P entries
+// contiguous), the layout of the multi-vector SpMM.  Every reduction has a
+// fixed order (per workgroup in row order, then workgroups in index order),
+// so results do not depend on timing.
 #include "dev_common.h"
+#include "dev_spmm.h"
 #include "nlhip_internal.h"
 
 namespace nlhip {
@@ -15,39 +17,41 @@ namespace {
 
 constexpr uint32_t kGramTile = 64;  // rows per LDS tile of the Gram kernel
 
-// val += -sigma mass[row] on each row's diagonal entry; *missing = 1 when a
-// row has none.
-__global__ void k_eig_shift_diag(uint32_t n, const uint32_t* rp, const uint32_t* ci,
-                                 double* val, const double* mass, double sigma,
-                                 uint32_t* missing) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
-        bool found = false;
-        for (uint32_t k = rp[r]; k < rp[r + 1]; ++k) {
-            if (ci[k] == r) {
-                val[k] = add_rn(val[k], mul_rn(-sigma, mass ? mass[r] : 1.0));
-                found = true;
-                break;
-            }
-        }
-        if (!found) atomicOr(missing, 1u);
-    }
+// AY = A Y.
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_eig_spmm(uint32_t n, uint32_t rpb, const uint32_t* rp,
+                                                     const uint32_t* ci, const double* val,
+                                                     const double* Y, double* AY) {
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    auto epi = [&](uint32_t row, uint32_t j, double s) { AY[size_t(row) * P + j] = s; };
+    spmm_rows<P>(n, P, rpb, L, rp, ci, val, Y, epi);
 }
 
-// Y[:, j] = M X[:, j]
-__global__ void k_eig_scale_rows(uint32_t n, uint32_t p, const double* mass, const double* X,
-                                 double* Y) {
-    const uint64_t total = uint64_t(n) * p;
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride)
-        Y[i] = mass ? mul_rn(mass[i % n], X[i]) : X[i];
+// One step of the Chebyshev recurrence on T = M^-1 A:
+//   Ynew = (T Y - c Y) s1 - s2 Xold       (s2 = 0 and no Xold on the first)
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_eig_cheb(uint32_t n, uint32_t rpb, const uint32_t* rp,
+                                                     const uint32_t* ci, const double* val,
+                                                     const double* minv, const double* Y,
+                                                     const double* Xold, double* Ynew, double c,
+                                                     double s1, double s2) {
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    auto epi = [&](uint32_t row, uint32_t j, double s) {
+        const size_t i = size_t(row) * P + j;
+        const double t = minv ? mul_rn(s, minv[row]) : s;
+        double v = mul_rn(add_rn(t, -mul_rn(c, Y[i])), s1);
+        if (Xold) v = add_rn(v, -mul_rn(s2, Xold[i]));
+        Ynew[i] = v;
+    };
+    spmm_rows<P>(n, P, rpb, L, rp, ci, val, Y, epi);
 }
 
 // Per workgroup b: part[b][i][j] = sum over its rows r (in order) of
-// Y[r, i] Z[r, j].  Rows of workgroup b: [b rpb, (b + 1) rpb).
+// Y[r, i] (M Z)[r, j]  (mass null: M = I).  Rows of b: [b rpb, (b+1) rpb).
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_eig_gram(uint32_t n, const double* Y, const double* Z,
-                                                     uint32_t rpb, double* part) {
+                                                     const double* mass, uint32_t rpb,
+                                                     double* part) {
     __shared__ double ys[kGramTile][P];
     __shared__ double zs[kGramTile][P];
     constexpr int E = (P * P + kBlock - 1) / kBlock;  // entries per thread
@@ -59,10 +63,11 @@ __global__ __launch_bounds__(kBlock) void k_eig_gram(uint32_t n, const double* Y
     for (uint32_t t0 = r0; t0 < r1; t0 += kGramTile) {
         const uint32_t rows = (t0 + kGramTile < r1) ? kGramTile : r1 - t0;
         for (uint32_t idx = threadIdx.x; idx < kGramTile * P; idx += kBlock) {
-            const uint32_t col = idx / kGramTile, rr = idx % kGramTile;
+            const uint32_t rr = idx / P, col = idx % P;
             const bool in = rr < rows;
-            ys[rr][col] = in ? Y[size_t(col) * n + t0 + rr] : 0.0;
-            zs[rr][col] = in ? Z[size_t(col) * n + t0 + rr] : 0.0;
+            const size_t i = size_t(t0 + rr) * P + col;
+            ys[rr][col] = in ? Y[i] : 0.0;
+            zs[rr][col] = in ? (mass ? mul_rn(mass[t0 + rr], Z[i]) : Z[i]) : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -103,20 +108,19 @@ __global__ __launch_bounds__(kBlock) void k_eig_combine(uint32_t n, const double
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
         double y[P];
 #pragma unroll
-        for (int i = 0; i < P; ++i) y[i] = Y[size_t(i) * n + r];
+        for (int i = 0; i < P; ++i) y[i] = Y[size_t(r) * P + i];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
             double s = 0.0;
 #pragma unroll
             for (int i = 0; i < P; ++i) s = add_rn(s, mul_rn(y[i], cs[i * P + j]));
-            X[size_t(j) * n + r] = s;
+            X[size_t(r) * P + j] = s;
         }
     }
 }
 
-// Per workgroup b and column j (rows [b rpb, (b + 1) rpb), in order):
-// part[b][j] = sum (AX - theta_j M X)^2, part[b][P + j] = sum AX^2,
-// part[b][2P + j] = sum (M X)^2.
+// Per workgroup b and column j (rows in order): part[b][j] = sum (AX -
+// theta_j M X)^2, part[b][P + j] = sum AX^2, part[b][2P + j] = sum (M X)^2.
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_eig_residual(uint32_t n, const double* AX,
                                                          const double* X, const double* mass,
@@ -129,8 +133,9 @@ __global__ __launch_bounds__(kBlock) void k_eig_residual(uint32_t n, const doubl
         double sr = 0.0, sa = 0.0, sm = 0.0;
         const double th = theta[j];
         for (uint32_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
-            const double ax = AX[size_t(j) * n + r];
-            const double mx = mass ? mul_rn(mass[r], X[size_t(j) * n + r]) : X[size_t(j) * n + r];
+            const size_t i = size_t(r) * P + j;
+            const double ax = AX[i];
+            const double mx = mass ? mul_rn(mass[r], X[i]) : X[i];
             const double d = add_rn(ax, -mul_rn(th, mx));
             sr = add_rn(sr, mul_rn(d, d));
             sa = add_rn(sa, mul_rn(ax, ax));
@@ -147,51 +152,84 @@ __global__ __launch_bounds__(kBlock) void k_eig_residual(uint32_t n, const doubl
     }
 }
 
+// Gershgorin bound of the spectrum of M^-1 A: max over rows of
+// sum |a_ij| / m_i (the maximum of non-negative doubles, taken on their bit
+// patterns, which order the same way).
+__global__ void k_eig_bound(uint32_t n, const uint32_t* rp, const double* val, const double* minv,
+                            unsigned long long* out) {
+    unsigned long long best = 0;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (uint32_t k = rp[r]; k < rp[r + 1]; ++k) s = add_rn(s, fabs(val[k]));
+        if (minv) s = mul_rn(s, minv[r]);
+        const unsigned long long b = __double_as_longlong(s);
+        best = b > best ? b : best;
+    }
+    if (best) atomicMax(out, best);
+}
+
+__global__ void k_eig_inverse(uint32_t n, const double* m, double* minv) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+        minv[r] = 1.0 / m[r];
+}
+
 uint32_t blocks_for(uint64_t n) {
     const uint64_t g = (n + kBlock - 1) / kBlock;
     return uint32_t(g < 1 ? 1 : (g > 65536 ? 65536 : g));
 }
 
-}  // namespace
-
-void eig_shift_diag(uint32_t n, const uint32_t* rp, const uint32_t* ci, double* val,
-                    const double* mass, double sigma, hipStream_t s) {
-    DevBuf<uint32_t> missing(1);
-    NLHIP_CHECK(hipMemsetAsync(missing.get(), 0, 4, s));
-    k_eig_shift_diag<<<blocks_for(n), kBlock, 0, s>>>(n, rp, ci, val, mass, sigma,
-                                                      missing.get());
-    NLHIP_CHECK(hipGetLastError());
-    uint32_t h = 0;
-    NLHIP_CHECK(hipMemcpyAsync(&h, missing.get(), 4, hipMemcpyDeviceToHost, s));
-    NLHIP_CHECK(hipStreamSynchronize(s));
-    if (h) throw Error(NLHIP_ERR_INVALID, "a row of the matrix has no diagonal entry");
-}
-
-void eig_scale_rows(uint32_t n, uint32_t p, const double* mass, const double* X, double* Y,
-                    hipStream_t s) {
-    k_eig_scale_rows<<<blocks_for(uint64_t(n) * p), kBlock, 0, s>>>(n, p, mass, X, Y);
-    NLHIP_CHECK(hipGetLastError());
-}
-
 // Rows per workgroup of the blocked reductions: whole tiles, at most 1024
 // workgroups.
-static uint32_t rows_per_group(uint32_t n) {
+uint32_t rows_per_group(uint32_t n) {
     uint32_t rpb = (n + 1023) / 1024;
     rpb = (rpb + kGramTile - 1) / kGramTile * kGramTile;
     return rpb < kGramTile ? kGramTile : rpb;
 }
 
-void eig_gram(uint32_t n, uint32_t p, const double* Y, const double* Z, double* G,
-              DevBuf<double>& part, hipStream_t s) {
+}  // namespace
+
+#define NLHIP_EIG_P(P, CALL)                                                        \
+    switch (P) {                                                                    \
+        case 8: { constexpr int PP = 8; CALL; } break;                              \
+        case 16: { constexpr int PP = 16; CALL; } break;                            \
+        case 32: { constexpr int PP = 32; CALL; } break;                            \
+        default: throw Error(NLHIP_ERR_INVALID, "block size must be 8, 16 or 32"); \
+    }
+
+EigShape eig_shape(uint32_t n, uint32_t p, int cu) {
+    // G = p lanes per row, passes of 64 rows; at most 16 workgroups per CU
+    const uint32_t per = 256u / p;
+    const uint64_t passes = (uint64_t(n) + per - 1) / per;
+    const uint64_t cap = uint64_t(cu > 0 ? cu : 1) * 16;
+    EigShape e;
+    e.grid = uint32_t(passes < 1 ? 1 : (passes < cap ? passes : cap));
+    uint64_t rpb = (uint64_t(n) + e.grid - 1) / e.grid;
+    rpb = (rpb + kSpmmPass - 1) / kSpmmPass * kSpmmPass;
+    e.rows_per_block = uint32_t(rpb < kSpmmPass ? kSpmmPass : rpb);
+    return e;
+}
+
+void eig_spmm(const EigShape& e, uint32_t n, uint32_t p, const uint32_t* rp, const uint32_t* ci,
+              const double* val, const double* Y, double* AY, hipStream_t s) {
+    NLHIP_EIG_P(p, (k_eig_spmm<PP><<<e.grid, kBlock, 0, s>>>(n, e.rows_per_block, rp, ci, val,
+                                                              Y, AY)));
+    NLHIP_CHECK(hipGetLastError());
+}
+
+void eig_cheb(const EigShape& e, uint32_t n, uint32_t p, const uint32_t* rp, const uint32_t* ci,
+              const double* val, const double* minv, const double* Y, const double* Xold,
+              double* Ynew, double c, double s1, double s2, hipStream_t s) {
+    NLHIP_EIG_P(p, (k_eig_cheb<PP><<<e.grid, kBlock, 0, s>>>(n, e.rows_per_block, rp, ci, val,
+                                                              minv, Y, Xold, Ynew, c, s1, s2)));
+    NLHIP_CHECK(hipGetLastError());
+}
+
+void eig_gram(uint32_t n, uint32_t p, const double* Y, const double* Z, const double* mass,
+              double* G, DevBuf<double>& part, hipStream_t s) {
     const uint32_t rpb = rows_per_group(n);
     const uint32_t nblk = (n + rpb - 1) / rpb;
     part.ensure(size_t(nblk) * p * p);
-    switch (p) {
-        case 8: k_eig_gram<8><<<nblk, kBlock, 0, s>>>(n, Y, Z, rpb, part.get()); break;
-        case 16: k_eig_gram<16><<<nblk, kBlock, 0, s>>>(n, Y, Z, rpb, part.get()); break;
-        case 32: k_eig_gram<32><<<nblk, kBlock, 0, s>>>(n, Y, Z, rpb, part.get()); break;
-        default: throw Error(NLHIP_ERR_INVALID, "block size must be 8, 16 or 32");
-    }
+    NLHIP_EIG_P(p, (k_eig_gram<PP><<<nblk, kBlock, 0, s>>>(n, Y, Z, mass, rpb, part.get())));
     k_eig_reduce<<<blocks_for(p * p), kBlock, 0, s>>>(nblk, p * p, part.get(), G);
     NLHIP_CHECK(hipGetLastError());
 }
@@ -199,28 +237,38 @@ void eig_gram(uint32_t n, uint32_t p, const double* Y, const double* Z, double* 
 void eig_combine(uint32_t n, uint32_t p, const double* Y, const double* C, double* X,
                  hipStream_t s) {
     const uint32_t g = blocks_for(n) < 4096 ? blocks_for(n) : 4096;
-    switch (p) {
-        case 8: k_eig_combine<8><<<g, kBlock, 0, s>>>(n, Y, C, X); break;
-        case 16: k_eig_combine<16><<<g, kBlock, 0, s>>>(n, Y, C, X); break;
-        case 32: k_eig_combine<32><<<g, kBlock, 0, s>>>(n, Y, C, X); break;
-        default: throw Error(NLHIP_ERR_INVALID, "block size must be 8, 16 or 32");
-    }
+    NLHIP_EIG_P(p, (k_eig_combine<PP><<<g, kBlock, 0, s>>>(n, Y, C, X)));
     NLHIP_CHECK(hipGetLastError());
 }
 
-void eig_residual(uint32_t n, uint32_t p, const double* AX, const double* X,
-                  const double* mass, const double* theta, double* out,
-                  DevBuf<double>& part, hipStream_t s) {
+void eig_residual(uint32_t n, uint32_t p, const double* AX, const double* X, const double* mass,
+                  const double* theta, double* out, DevBuf<double>& part, hipStream_t s) {
     const uint32_t rpb = rows_per_group(n);
     const uint32_t nblk = (n + rpb - 1) / rpb;
     part.ensure(size_t(nblk) * 3 * p);
-    switch (p) {
-        case 8: k_eig_residual<8><<<nblk, kBlock, 0, s>>>(n, AX, X, mass, theta, rpb, part.get()); break;
-        case 16: k_eig_residual<16><<<nblk, kBlock, 0, s>>>(n, AX, X, mass, theta, rpb, part.get()); break;
-        case 32: k_eig_residual<32><<<nblk, kBlock, 0, s>>>(n, AX, X, mass, theta, rpb, part.get()); break;
-        default: throw Error(NLHIP_ERR_INVALID, "block size must be 8, 16 or 32");
-    }
+    NLHIP_EIG_P(p, (k_eig_residual<PP><<<nblk, kBlock, 0, s>>>(n, AX, X, mass, theta, rpb,
+                                                                part.get())));
     k_eig_reduce<<<blocks_for(3 * p), kBlock, 0, s>>>(nblk, 3 * p, part.get(), out);
+    NLHIP_CHECK(hipGetLastError());
+}
+
+double eig_bound(uint32_t n, const uint32_t* rp, const double* val, const double* minv,
+                 hipStream_t s) {
+    DevBuf<unsigned long long> out(1);
+    NLHIP_CHECK(hipMemsetAsync(out.get(), 0, 8, s));
+    k_eig_bound<<<blocks_for(n) < 4096 ? blocks_for(n) : 4096, kBlock, 0, s>>>(n, rp, val, minv,
+                                                                              out.get());
+    NLHIP_CHECK(hipGetLastError());
+    unsigned long long h = 0;
+    NLHIP_CHECK(hipMemcpyAsync(&h, out.get(), 8, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipStreamSynchronize(s));
+    double b;
+    std::memcpy(&b, &h, 8);
+    return b;
+}
+
+void eig_inverse(uint32_t n, const double* m, double* minv, hipStream_t s) {
+    k_eig_inverse<<<blocks_for(n) < 4096 ? blocks_for(n) : 4096, kBlock, 0, s>>>(n, m, minv);
     NLHIP_CHECK(hipGetLastError());
 }
 

@@ -1,21 +1,23 @@
 // Generalized symmetric eigenpairs A x = lambda M x (M a positive diagonal,
-// the lumped mass; identity when absent) nearest above a shift sigma: block
-// shift-invert subspace iteration with Rayleigh-Ritz.  This replaces the
-// ARPACK shift-invert solves that geogram runs for spectral LSCM and
-// manifold harmonics (src/lib/OGF/mesh/commands/
+// the lumped mass; identity when absent): the k smallest, for a shift below
+// the spectrum.  This replaces the ARPACK shift-invert solves that geogram
+// runs for spectral LSCM and manifold harmonics (src/lib/OGF/mesh/commands/
 // mesh_grob_surface_commands.cpp:761-763, mesh_grob_spectral_commands.cpp:
-// 74-116); the factorized solve ARPACK needs becomes a block Jacobi-PCG
-// (nlhipSolveMulti's kernels) on A - sigma M.
+// 74-116), by Chebyshev-filtered subspace iteration (Zhou and Saad): no
+// linear solves, only multi-vector SpMMs.
 //
-// Per outer iteration, for a block X of P M-orthonormal vectors:
-//   Y  = (A - sigma M)^-1 M X     (block PCG, warm-started from X / (theta - sigma),
-//                                   to a tolerance that follows the outer residual)
+// On a block X of P M-orthonormal Ritz vectors with Ritz values theta:
+//   Y  = p_m(M^-1 A) X, p_m the degree-m Chebyshev polynomial that damps
+//        [theta_{P-1}, b] (b a Gershgorin bound of the spectrum) and is 1 at
+//        the smallest Ritz value; three-term recurrence, one fused SpMM per
+//        degree
 //   H  = Y^T A Y,  G = Y^T M Y     (fixed-order device reductions)
 //   H c = theta G c                (host, P x P: Cholesky of the scaled G,
 //                                   cyclic Jacobi on L^-1 H L^-T)
-//   X  = Y C, AX = (A Y) C; residuals |AX - theta M X| / (max(|theta|, |sigma|) |MX|)
+//   X  = Y C, AX = (A Y) C; residuals |AX - theta M X| / (max(|theta|,
+//        |shift|) |MX|)
 // until the k wanted pairs (k < P) are below eps.  P is 8, 16 or 32: the
-// extra vectors speed convergence of the wanted ones.
+// extra vectors keep the damped interval away from the wanted eigenvalues.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -26,17 +28,21 @@
 
 namespace nlhip {
 
-void eig_shift_diag(uint32_t n, const uint32_t* rp, const uint32_t* ci, double* val,
-                    const double* mass, double sigma, hipStream_t s);
-void eig_scale_rows(uint32_t n, uint32_t p, const double* mass, const double* X, double* Y,
-                    hipStream_t s);
-void eig_gram(uint32_t n, uint32_t p, const double* Y, const double* Z, double* G,
-              DevBuf<double>& part, hipStream_t s);
+EigShape eig_shape(uint32_t n, uint32_t p, int cu);
+void eig_spmm(const EigShape& e, uint32_t n, uint32_t p, const uint32_t* rp, const uint32_t* ci,
+              const double* val, const double* Y, double* AY, hipStream_t s);
+void eig_cheb(const EigShape& e, uint32_t n, uint32_t p, const uint32_t* rp, const uint32_t* ci,
+              const double* val, const double* minv, const double* Y, const double* Xold,
+              double* Ynew, double c, double s1, double s2, hipStream_t s);
+void eig_gram(uint32_t n, uint32_t p, const double* Y, const double* Z, const double* mass,
+              double* G, DevBuf<double>& part, hipStream_t s);
 void eig_combine(uint32_t n, uint32_t p, const double* Y, const double* C, double* X,
                  hipStream_t s);
-void eig_residual(uint32_t n, uint32_t p, const double* AX, const double* X,
-                  const double* mass, const double* theta, double* out,
-                  DevBuf<double>& part, hipStream_t s);
+void eig_residual(uint32_t n, uint32_t p, const double* AX, const double* X, const double* mass,
+                  const double* theta, double* out, DevBuf<double>& part, hipStream_t s);
+double eig_bound(uint32_t n, const uint32_t* rp, const double* val, const double* minv,
+                 hipStream_t s);
+void eig_inverse(uint32_t n, const double* m, double* minv, hipStream_t s);
 
 namespace {
 
@@ -171,21 +177,22 @@ void start_block(uint32_t n, uint32_t p, std::vector<double>& X) {
 
 }  // namespace
 
+// Chebyshev degree per outer iteration.
+constexpr int kChebDegree = 40;
+
 void eigen_solve(System& A, const double* mass_in, uint32_t k, double sigma, double eps,
                  uint32_t max_iter, double* evals, double* evecs, nlhip_eigen_stats& st) {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t n = A.n;
     if (k == 0 || k > 24) throw Error(NLHIP_ERR_INVALID, "nb_eigens must be in [1, 24]");
-    if (k >= n) throw Error(NLHIP_ERR_INVALID, "nb_eigens must be below the system size");
     if (!std::isfinite(sigma) || !(eps > 0.0)) throw Error(NLHIP_ERR_INVALID, "bad shift or eps");
-    uint32_t P = k <= 4 ? 8 : (k <= 10 ? 16 : 32);
-    if (P > n) P = n >= 8 ? 8 : n;
-    if (P != 8 && P != 16 && P != 32) throw Error(NLHIP_ERR_INVALID, "system too small");
+    uint32_t P = k <= 4 ? 8 : (k <= 8 ? 16 : 32);
+    if (P >= n) throw Error(NLHIP_ERR_INVALID, "system too small for the eigenvector block");
     Stream stream;
     hipStream_t s = stream.get();
-    // mass on the device (checked positive and finite)
-    DevBuf<double> d_mass;
+    DevBuf<double> d_mass, d_minv;
     const double* mass = nullptr;
+    const double* minv = nullptr;
     if (mass_in) {
         std::vector<double> hm(n);
         NLHIP_CHECK(hipMemcpyAsync(hm.data(), mass_in, size_t(n) * 8, hipMemcpyDefault, s));
@@ -194,83 +201,75 @@ void eigen_solve(System& A, const double* mass_in, uint32_t k, double sigma, dou
             if (!(m > 0.0) || !std::isfinite(m))
                 throw Error(NLHIP_ERR_INVALID, "mass entries must be positive and finite");
         d_mass.alloc(n);
+        d_minv.alloc(n);
         copy_in(d_mass.get(), hm.data(), size_t(n) * 8, s);
+        eig_inverse(n, d_mass.get(), d_minv.get(), s);
         mass = d_mass.get();
+        minv = d_minv.get();
     }
-    // B = A - sigma M, a resident system of its own (Jacobi on its diagonal)
-    std::unique_ptr<System> B;
-    {
-        std::lock_guard<std::mutex> lock(A.mu);
-        DevBuf<uint32_t> rp(size_t(n) + 1), ci(A.nnz);
-        DevBuf<double> va(A.nnz);
-        hipStream_t as = A.stream.get();
-        NLHIP_CHECK(hipMemcpyAsync(rp.get(), A.rowptr.get(), (size_t(n) + 1) * 4,
-                                   hipMemcpyDeviceToDevice, as));
-        NLHIP_CHECK(hipMemcpyAsync(ci.get(), A.colind.get(), A.nnz * 4, hipMemcpyDeviceToDevice, as));
-        NLHIP_CHECK(hipMemcpyAsync(va.get(), A.val.get(), A.nnz * 8, hipMemcpyDeviceToDevice, as));
-        A.stream.sync();
-        eig_shift_diag(n, rp.get(), ci.get(), va.get(), mass, sigma, s);
-        B.reset(system_adopt(n, A.nnz, std::move(rp), std::move(ci), std::move(va), 0));
-    }
+    const uint32_t* rp = A.rowptr.get();
+    const uint32_t* ci = A.colind.get();
+    const double* va = A.val.get();
+    const EigShape shape = eig_shape(n, P, device().cu_count);
+    const double b_up = eig_bound(n, rp, va, minv, s);
+    if (!(b_up > 0.0) || !std::isfinite(b_up))
+        throw Error(NLHIP_ERR_INVALID, "matrix without a positive spectrum bound");
     const size_t blk = size_t(n) * P;
-    DevBuf<double> X(blk), Y(blk), MX(blk), AY(blk), AX(blk), part, dG(size_t(P) * P),
-        dH(size_t(P) * P), dC(size_t(P) * P), dth(P), dres(3 * size_t(P));
+    DevBuf<double> buf[3] = {DevBuf<double>(blk), DevBuf<double>(blk), DevBuf<double>(blk)};
+    DevBuf<double> AY(blk), AX(blk), part, dG(size_t(P) * P), dH(size_t(P) * P),
+        dC(size_t(P) * P), dth(P), dres(3 * size_t(P));
     {
         std::vector<double> h;
         start_block(n, P, h);
-        copy_in(X.get(), h.data(), blk * 8, s);
+        copy_in(buf[0].get(), h.data(), blk * 8, s);
     }
     std::vector<double> theta(P, 0.0), C, G(size_t(P) * P), H(size_t(P) * P);
-    // Inner solves are inexact: their relative residual follows the outer
-    // one (a hundredth of the largest residual of the wanted pairs, from
-    // 1e-2 down to eps / 100), which is all subspace iteration needs and
-    // saves most of the block-PCG iterations on ill-conditioned A - sigma M.
-    nlhip_solve_params ip;
-    nlhipSolveParamsDefault(&ip);
-    ip.max_iter = 20000;
-    const double inner_floor = std::max(1e-14, eps * 1e-2);
-    double outer_res = 1.0;
-    st = nlhip_eigen_stats{};
-    bool have_theta = false;
     std::vector<double> res(3 * size_t(P));
-    const SpmvOps<double> o = A.ops<double>();
-    for (uint32_t it = 0; it < max_iter; ++it) {
-        // Y = (A - sigma M)^-1 M X, from X / (theta - sigma)
-        eig_scale_rows(n, P, mass, X.get(), MX.get(), s);
-        if (have_theta) {
-            std::vector<double> Cg(size_t(P) * P, 0.0);
-            for (uint32_t j = 0; j < P; ++j) {
-                const double gap = theta[j] - sigma;
-                Cg[size_t(j) * P + j] = gap > 0.0 ? 1.0 / gap : 0.0;
-            }
-            copy_in(dC.get(), Cg.data(), Cg.size() * 8, s);
-            eig_combine(n, P, X.get(), dC.get(), Y.get(), s);
-        } else {
-            NLHIP_CHECK(hipMemsetAsync(Y.get(), 0, blk * 8, s));
-        }
-        NLHIP_CHECK(hipStreamSynchronize(s));
-        ip.eps = std::max(inner_floor, std::min(1e-2, 1e-2 * outer_res));
-        nlhip_solve_stats tot{};
-        solve_multi(*B, P, MX.get(), Y.get(), ip, nullptr, &tot);
-        st.inner_iterations += tot.iterations;
-        // A Y, M Y, Gram matrices
-        for (uint32_t j = 0; j < P; ++j)
-            NLHIP_CHECK(spmv<double>(n, o.rp, o.cols, o.val, Y.get() + size_t(j) * n,
-                                     AY.get() + size_t(j) * n, o.code, A.grid, s));
-        eig_scale_rows(n, P, mass, Y.get(), MX.get(), s);
-        eig_gram(n, P, Y.get(), AY.get(), dH.get(), part, s);
-        eig_gram(n, P, Y.get(), MX.get(), dG.get(), part, s);
+    st = nlhip_eigen_stats{};
+    int x_at = 0;  // buffer holding the current block X
+    // Rayleigh-Ritz on the block in buf[y]: X = Y C into buf[x_to], AX
+    auto rayleigh_ritz_step = [&](int y, int x_to) {
+        eig_spmm(shape, n, P, rp, ci, va, buf[y].get(), AY.get(), s);
+        eig_gram(n, P, buf[y].get(), AY.get(), nullptr, dH.get(), part, s);
+        eig_gram(n, P, buf[y].get(), buf[y].get(), mass, dG.get(), part, s);
         copy_out(H.data(), dH.get(), H.size() * 8, s);
         copy_out(G.data(), dG.get(), G.size() * 8, s);
         NLHIP_CHECK(hipStreamSynchronize(s));
         if (!rayleigh_ritz(int(P), H, G, theta, C))
-            throw Error(NLHIP_ERR_HIP, "eigensolver: the iterated block lost rank");
-        have_theta = true;
+            throw Error(NLHIP_ERR_HIP, "eigensolver: the filtered block lost rank");
         copy_in(dC.get(), C.data(), C.size() * 8, s);
-        eig_combine(n, P, Y.get(), dC.get(), X.get(), s);
+        eig_combine(n, P, buf[y].get(), dC.get(), buf[x_to].get(), s);
         eig_combine(n, P, AY.get(), dC.get(), AX.get(), s);
+    };
+    rayleigh_ritz_step(0, 1);
+    x_at = 1;
+    for (uint32_t it = 0; it < max_iter; ++it) {
+        // filter: damp [a, b_up], normalised at the smallest Ritz value
+        const double a = theta[P - 1];
+        const double lo = std::min(theta[0], sigma);
+        if (!(b_up > a) || !(a > lo)) throw Error(NLHIP_ERR_HIP, "eigensolver: degenerate filter interval");
+        const double e = 0.5 * (b_up - a), c = 0.5 * (b_up + a);
+        double sg = e / (lo - c);
+        const double sg1 = sg;
+        int prev = x_at, cur = (x_at + 1) % 3, nxt = (x_at + 2) % 3;
+        eig_cheb(shape, n, P, rp, ci, va, minv, buf[prev].get(), nullptr, buf[cur].get(), c,
+                 sg1 / e, 0.0, s);
+        for (int d = 2; d <= kChebDegree; ++d) {
+            const double sg2 = 1.0 / (2.0 / sg1 - sg);
+            eig_cheb(shape, n, P, rp, ci, va, minv, buf[cur].get(), buf[prev].get(),
+                     buf[nxt].get(), c, 2.0 * sg2 / e, sg * sg2, s);
+            sg = sg2;
+            const int t = prev;
+            prev = cur;
+            cur = nxt;
+            nxt = t;
+        }
+        st.inner_iterations += kChebDegree;
+        // Rayleigh-Ritz into a buffer other than the filtered one
+        x_at = (cur + 1) % 3;
+        rayleigh_ritz_step(cur, x_at);
         copy_in(dth.get(), theta.data(), P * 8, s);
-        eig_residual(n, P, AX.get(), X.get(), mass, dth.get(), dres.get(), part, s);
+        eig_residual(n, P, AX.get(), buf[x_at].get(), mass, dth.get(), dres.get(), part, s);
         copy_out(res.data(), dres.get(), res.size() * 8, s);
         NLHIP_CHECK(hipStreamSynchronize(s));
         st.outer_iterations = it + 1;
@@ -278,22 +277,31 @@ void eigen_solve(System& A, const double* mass_in, uint32_t k, double sigma, dou
         st.max_rel_residual = 0.0;
         // |A x - theta M x| / (max(|theta|, |sigma|) |M x|): relative to the
         // eigenvalue, or to the shift for eigenvalues near 0 (the constant
-        // vector of a Laplacian)
+        // vector of a Laplacian).  A pair also counts as converged when its
+        // residual is at the rounding level of the operator, 1000 u b |M x|
+        // (b the spectrum bound): an eigenvalue near 0 cannot get below it.
         const double big = std::fabs(theta[P - 1]);
+        const double floor_rel = 1e3 * 2.220446049250313e-16 * b_up;
         for (uint32_t j = 0; j < k; ++j) {
             double scale = std::max(std::fabs(theta[j]), std::fabs(sigma));
             if (scale == 0.0) scale = big;
-            const double den = scale * std::sqrt(res[2 * P + j]);
-            const double r = den > 0.0 ? std::sqrt(res[j]) / den : 0.0;
+            const double mx = std::sqrt(res[2 * P + j]);
+            const double rn = std::sqrt(res[j]);
+            const double r = scale * mx > 0.0 ? rn / (scale * mx) : 0.0;
             st.max_rel_residual = std::max(st.max_rel_residual, r);
-            if (r <= eps) ++st.converged;
+            if (r <= eps || rn <= floor_rel * mx) ++st.converged;
         }
-        outer_res = st.max_rel_residual;
         if (st.converged == k) break;
     }
     for (uint32_t j = 0; j < k; ++j) evals[j] = theta[j];
     if (evecs) {
-        NLHIP_CHECK(hipMemcpyAsync(evecs, X.get(), size_t(n) * k * 8, hipMemcpyDefault, s));
+        // interleaved block -> the first k columns, column-major
+        std::vector<double> h(blk), out(size_t(n) * k);
+        copy_out(h.data(), buf[x_at].get(), blk * 8, s);
+        NLHIP_CHECK(hipStreamSynchronize(s));
+        for (uint32_t r = 0; r < n; ++r)
+            for (uint32_t j = 0; j < k; ++j) out[size_t(j) * n + r] = h[size_t(r) * P + j];
+        NLHIP_CHECK(hipMemcpyAsync(evecs, out.data(), out.size() * 8, hipMemcpyDefault, s));
         NLHIP_CHECK(hipStreamSynchronize(s));
     }
     st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -413,6 +413,11 @@ bool build_blocks(System& S);
 bool build_col_offsets(uint64_t nnz, const uint32_t* colind, hipStream_t s,
                        DevBuf<uint16_t>& ci16, DevBuf<uint32_t>& segb);
 
+// Launch shape of the eigensolver's multi-vector SpMM (k_eigen.hip).
+struct EigShape {
+    uint32_t grid = 1, rows_per_block = 64;
+};
+
 // Launch-shape autotuning of a system's CG (S.mu held).
 struct TuneResult {
     int rpw;

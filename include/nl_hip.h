@@ -97,22 +97,23 @@ NLAPI int NLAPIENTRY nlhipSystemCreate(uint32_t n, const uint32_t* rowptr,
 NLAPI int NLAPIENTRY nlhipSystemDestroy(nlhip_system s);
 NLAPI int NLAPIENTRY nlhipSystemInfo(nlhip_system s, uint32_t* n, uint64_t* nnz,
                                      uint64_t* device_bytes);
-/* Generalized symmetric eigenpairs A x = lambda M x nearest above `shift`
- * (the smallest ones when shift is below the spectrum): replaces the ARPACK
- * shift-invert solves behind spectral LSCM and manifold harmonics
+/* Generalized symmetric eigenpairs A x = lambda M x: the nb_eigens
+ * smallest, `shift` a lower bound of the spectrum (A - shift M positive
+ * definite, as for a shift-invert solve).  Replaces the ARPACK shift-invert
+ * solves behind spectral LSCM and manifold harmonics
  * (mesh_grob_surface_commands.cpp:761-763, mesh_grob_spectral_commands.cpp:
- * 74-116), by block shift-invert subspace iteration whose solves with
- * A - shift M are block Jacobi-PCG on the device.  A symmetric with every
- * diagonal entry stored, A - shift M positive definite; mass: the diagonal
- * of M (n positive values, host or device), NULL for the identity.
+ * 74-116), by Chebyshev-filtered subspace iteration on the device (no
+ * linear solves).  A symmetric; mass: the diagonal of M (n positive values,
+ * host or device), NULL for the identity.
  * nb_eigens in [1, 24]; eps bounds |A x - lambda M x| / (max(|lambda|,
- * |shift|) |M x|) for each returned pair; max_iter outer iterations (0:
- * 1000).  eigenvalues (nb_eigens, ascending) and eigenvectors (n x nb_eigens
+ * |shift|) |M x|) for each returned pair, or the residual is at the
+ * rounding level of the operator (1000 u |M^-1 A| |M x|, for eigenvalues
+ * near 0); max_iter outer iterations (0: 1000), each a degree-40 filter.  eigenvalues (nb_eigens, ascending) and eigenvectors (n x nb_eigens
  * column-major, M-orthonormal, host or device, may be NULL). */
 typedef struct {
     uint32_t outer_iterations;
     uint32_t converged;        /* pairs below eps                       */
-    uint64_t inner_iterations; /* block PCG iterations, all outer steps */
+    uint64_t inner_iterations; /* filter degrees (block SpMMs), all steps */
     double max_rel_residual;
     double seconds;
 } nlhip_eigen_stats;
