@@ -89,66 +89,80 @@ __global__ __launch_bounds__(kBlock) void k_eig_gram(uint32_t n, const double* Y
     }
 }
 
-// G[e] = sum over workgroups b (in order) of part[b][e].
+// G[e] = sum over workgroups b (in order) of part[b][e]; the loads of 8
+// partials are issued before they are added.
 __global__ void k_eig_reduce(uint32_t nblk, uint32_t m, const double* part, double* G) {
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
         double s = 0.0;
-        for (uint32_t b = 0; b < nblk; ++b) s = add_rn(s, part[size_t(b) * m + e]);
+        uint32_t b = 0;
+        for (; b + 8 <= nblk; b += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = part[size_t(b + u) * m + e];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s = add_rn(s, v[u]);
+        }
+        for (; b < nblk; ++b) s = add_rn(s, part[size_t(b) * m + e]);
         G[e] = s;
     }
 }
 
 // X[r, j] = sum over i (in order) of Y[r, i] C[i][j]  (C row-major P x P).
+// A group of P lanes takes a row (lane j: column j, coalesced loads and
+// stores); Y[r, i] comes from lane i by a shuffle.
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_eig_combine(uint32_t n, const double* Y,
                                                         const double* C, double* X) {
     __shared__ double cs[P * P];
     for (int e = threadIdx.x; e < P * P; e += kBlock) cs[e] = C[e];
     __syncthreads();
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-        double y[P];
+    const uint32_t j = threadIdx.x % P;
+    constexpr uint32_t rows_per_block = kBlock / P;
+    for (uint32_t r0 = blockIdx.x * rows_per_block; r0 < n; r0 += gridDim.x * rows_per_block) {
+        const uint32_t r = r0 + threadIdx.x / P;
+        const bool in = r < n;
+        const double y = in ? Y[size_t(r) * P + j] : 0.0;
+        double s = 0.0;
 #pragma unroll
-        for (int i = 0; i < P; ++i) y[i] = Y[size_t(r) * P + i];
-#pragma unroll
-        for (int j = 0; j < P; ++j) {
-            double s = 0.0;
-#pragma unroll
-            for (int i = 0; i < P; ++i) s = add_rn(s, mul_rn(y[i], cs[i * P + j]));
-            X[size_t(r) * P + j] = s;
-        }
+        for (int i = 0; i < P; ++i) s = add_rn(s, mul_rn(__shfl(y, i, P), cs[i * P + j]));
+        if (in) X[size_t(r) * P + j] = s;
     }
 }
 
 // Per workgroup b and column j (rows in order): part[b][j] = sum (AX -
 // theta_j M X)^2, part[b][P + j] = sum AX^2, part[b][2P + j] = sum (M X)^2.
+// Thread t takes column t % P of rows r0 + t / P, r0 + t / P + kBlock / P,
+// ... (coalesced); the kBlock / P threads of a column are then added in
+// thread order.
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_eig_residual(uint32_t n, const double* AX,
                                                          const double* X, const double* mass,
                                                          const double* theta, uint32_t rpb,
                                                          double* part) {
-    __shared__ double lds[kWavesPerBlock];
+    __shared__ double lds[3][kBlock];
     const uint32_t r0 = blockIdx.x * rpb;
     const uint32_t r1 = (r0 + rpb < n) ? r0 + rpb : n;
-    for (int j = 0; j < P; ++j) {
-        double sr = 0.0, sa = 0.0, sm = 0.0;
-        const double th = theta[j];
-        for (uint32_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
-            const size_t i = size_t(r) * P + j;
-            const double ax = AX[i];
-            const double mx = mass ? mul_rn(mass[r], X[i]) : X[i];
-            const double d = add_rn(ax, -mul_rn(th, mx));
-            sr = add_rn(sr, mul_rn(d, d));
-            sa = add_rn(sa, mul_rn(ax, ax));
-            sm = add_rn(sm, mul_rn(mx, mx));
-        }
-        sr = block_sum_all(sr, lds);
-        sa = block_sum_all(sa, lds);
-        sm = block_sum_all(sm, lds);
-        if (threadIdx.x == 0) {
-            part[size_t(blockIdx.x) * 3 * P + j] = sr;
-            part[size_t(blockIdx.x) * 3 * P + P + j] = sa;
-            part[size_t(blockIdx.x) * 3 * P + 2 * P + j] = sm;
-        }
+    const uint32_t j = threadIdx.x % P;
+    const double th = theta[j];
+    double sr = 0.0, sa = 0.0, sm = 0.0;
+    for (uint32_t r = r0 + threadIdx.x / P; r < r1; r += kBlock / P) {
+        const size_t i = size_t(r) * P + j;
+        const double ax = AX[i];
+        const double mx = mass ? mul_rn(mass[r], X[i]) : X[i];
+        const double d = add_rn(ax, -mul_rn(th, mx));
+        sr = add_rn(sr, mul_rn(d, d));
+        sa = add_rn(sa, mul_rn(ax, ax));
+        sm = add_rn(sm, mul_rn(mx, mx));
+    }
+    lds[0][threadIdx.x] = sr;
+    lds[1][threadIdx.x] = sa;
+    lds[2][threadIdx.x] = sm;
+    __syncthreads();
+    if (threadIdx.x < 3 * P) {
+        const uint32_t q = threadIdx.x / P, c = threadIdx.x % P;
+        double t = 0.0;
+        for (uint32_t u = c; u < kBlock; u += P) t = add_rn(t, lds[q][u]);
+        part[size_t(blockIdx.x) * 3 * P + q * P + c] = t;
     }
 }
 
