@@ -59,8 +59,12 @@ def run_atlas(args) -> dict:
     dev = init_device()
     t0 = time.time()
     A = W.atlas(args.count, seed=args.seed, nmin=args.nmin, nmax=args.nmax)
-    log(f"atlas: {A.count} charts, n={A.n}, nnz={A.nnz}, built in {time.time() - t0:.1f}s")
+    t_host_build = time.time() - t0
+    log(f"atlas: {A.count} charts, n={A.n}, nnz={A.nnz}, built in {t_host_build:.1f}s")
+    assembly = None if args.no_device_assembly else atlas_device_assembly(args, A)
+    t0 = time.perf_counter()
     B = capi.Batch(packed=(A.row_off, A.rowptr, A.colind, A.val), fp32=True)
+    t_packed_create = time.perf_counter() - t0
     params = dict(eps=args.eps, max_iter=args.max_iter)
 
     tuned = B.autotune(A.b, iters=args.tune_iters) if args.tune_iters else None
@@ -124,9 +128,45 @@ def run_atlas(args) -> dict:
                            "max_rel_diff_vs_batched": worst_diff,
                            "iterations_mean": float(np.mean(its_plain))},
     }
+    res["create"] = {"host_build_seconds": t_host_build,
+                     "host_build": "chart meshes, per-chart LSCM assembly and packing on "
+                                   "the host (OpenMP), then nlhipBatchCreate",
+                     "batch_create_from_packed_seconds": t_packed_create,
+                     "device_assembly": assembly}
     res["speedup_mixed_batched_vs_per_chart_fp64"] = t_plain_all / tm.device_seconds
     res["speedup_fp64_batched_vs_per_chart_fp64"] = t_plain_all / t64.device_seconds
     return res
+
+
+def atlas_device_assembly(args, A) -> dict:
+    """The same atlas built on the device from its chart meshes
+    (nlhipBatchLSCMCreate: LSCM rows, normal equations, block form and tiles);
+    checked bitwise against the host-packed CRS."""
+    t0 = time.time()
+    M = W.atlas_mesh(args.count, seed=args.seed, nmin=args.nmin, nmax=args.nmax)
+    t_mesh = time.time() - t0
+    times = []
+    same = None
+    for rep in range(2):                       # the first call also sizes the sort
+        t0 = time.perf_counter()
+        Bd, rhs = capi.Batch.lscm_atlas(M.vertex_off, M.points, M.triangles, M.pins, M.pin_uv,
+                                        fp32=True)
+        times.append(time.perf_counter() - t0)
+        if rep == 0:
+            ro, rp, ci, va = Bd.crs()
+            same = bool(np.array_equal(ro, A.row_off) and np.array_equal(rp, A.rowptr) and
+                        np.array_equal(ci, A.colind) and
+                        np.array_equal(va.view(np.uint64), A.val.view(np.uint64)) and
+                        np.array_equal(rhs.view(np.uint64), A.b.view(np.uint64)))
+            del ro, rp, ci, va
+        Bd.close()
+    log(f"device atlas assembly: {times[-1]:.3f}s (first call {times[0]:.3f}s), "
+        f"bitwise equal to the host packing: {same}")
+    return {"seconds": times[-1], "first_call_seconds": times[0],
+            "mesh_generation_seconds": t_mesh, "vertices": int(M.vertex_off[-1]),
+            "triangles": int(len(M.triangles)), "bitwise_equal_to_host": same,
+            "scope": "host mesh arrays in, batch resident out: upload, LSCM rows, "
+                     "normal equations, right-hand sides, block form, tiles"}
 
 
 def run_multirhs(args) -> dict:
@@ -318,6 +358,7 @@ def main(argv=None):
     a.add_argument("--max-iter-mixed", type=int, default=200000)
     a.add_argument("--inner-eps", type=float, default=1e-4)
     a.add_argument("--baseline-stride", type=int, default=1)
+    a.add_argument("--no-device-assembly", action="store_true")
     a.add_argument("--tune-iters", type=int, default=50,
                    help="iterations per candidate of the batch autotuner (0: no tuning)")
     mr = sub.add_parser("multirhs")

@@ -199,6 +199,9 @@ SIGNATURES = {
     "nlhipBatchCreate": (C.c_int, [C.c_uint32, _vp, _vp, _vp, _vp, C.c_uint32,
                                    C.POINTER(_vp)]),
     "nlhipBatchDestroy": (C.c_int, [_vp]),
+    "nlhipBatchGetCRS": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "nlhipBatchLSCMCreate": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint32, _vp, _vp, _vp,
+                                       C.c_uint32, C.POINTER(_vp), _vp, _u32p]),
     "nlhipBatchInfo": (C.c_int, [_vp, _u32p, _u32p, C.POINTER(C.c_uint64), _u32p]),
     "nlhipBatchSetLaunch": (C.c_int, [_vp, C.c_uint32]),
     "nlhipBatchSetSpmvPasses": (C.c_int, [_vp, C.c_uint32]),
@@ -471,6 +474,40 @@ class Batch:
                                      NLHIP_SYSTEM_FP32 if fp32 else 0, C.byref(h)),
               "nlhipBatchCreate")
         self._h = h
+
+    @classmethod
+    def lscm_atlas(cls, vertex_off, points, triangles, pins, pin_uv, fp32: bool = False):
+        """An atlas's LSCM systems built on the device from its chart meshes
+        (nlhipBatchLSCMCreate).  Returns (batch, rhs)."""
+        vo = np.ascontiguousarray(vertex_off, dtype=np.uint32)
+        P = np.ascontiguousarray(points, dtype=np.float64)
+        T = np.ascontiguousarray(triangles, dtype=np.uint32)
+        pn = np.ascontiguousarray(pins, dtype=np.uint32)
+        uv = np.ascontiguousarray(pin_uv, dtype=np.float64)
+        nc = len(vo) - 1
+        rhs = np.zeros(max(2 * len(P), 1))
+        h = C.c_void_p()
+        nfree = C.c_uint32()
+        check(lib().nlhipBatchLSCMCreate(nc, ptr(vo), ptr(P), len(T), ptr(T), ptr(pn), ptr(uv),
+                                         NLHIP_SYSTEM_FP32 if fp32 else 0, C.byref(h), ptr(rhs),
+                                         C.byref(nfree)), "nlhipBatchLSCMCreate")
+        B = cls.__new__(cls)
+        B._h = h
+        B.nsys, B.n = nc, int(nfree.value)
+        B.row_off = np.empty(nc + 1, dtype=np.uint32)
+        check(lib().nlhipBatchGetCRS(h, ptr(B.row_off), None, None, None), "nlhipBatchGetCRS")
+        return B, rhs[:B.n]
+
+    def crs(self):
+        """(row_off, rowptr, colind, val) the batch holds (nlhipBatchGetCRS)."""
+        nsys, n, nnz, _ = self.info()
+        ro = np.empty(nsys + 1, dtype=np.uint32)
+        rp = np.empty(n + 1, dtype=np.uint32)
+        ci = np.empty(max(nnz, 1), dtype=np.uint32)
+        va = np.empty(max(nnz, 1), dtype=np.float64)
+        check(lib().nlhipBatchGetCRS(self._h, ptr(ro), ptr(rp), ptr(ci), ptr(va)),
+              "nlhipBatchGetCRS")
+        return ro, rp, ci[:nnz], va[:nnz]
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

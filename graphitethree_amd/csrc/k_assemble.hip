@@ -259,6 +259,29 @@ __global__ void k_lscm_check(uint32_t nv, uint32_t nt, const double* P, const ui
     if (flags) atomicOr(bad, flags);
 }
 
+// *bad = 1 when a triangle's vertices lie in different charts (chart c owns
+// vertices [voff[c], voff[c+1])).
+__global__ void k_tri_chart_check(uint32_t nt, const uint32_t* T, const uint32_t* voff,
+                                  uint32_t nc, uint32_t* bad) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t flag = 0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride) {
+        uint32_t chart[3];
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t v = T[3 * size_t(t) + k];
+            uint32_t lo = 0, hi = nc;  // last c with voff[c] <= v
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (voff[mid] <= v) lo = mid;
+                else hi = mid;
+            }
+            chart[k] = lo;
+        }
+        flag |= (chart[0] != chart[1] || chart[0] != chart[2]) ? 1u : 0u;
+    }
+    if (flag) atomicOr(bad, 1u);
+}
+
 __global__ void k_row_ptr6(uint32_t nrows, uint32_t* rp) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r <= nrows; r += stride)
@@ -318,6 +341,18 @@ void lscm_rows_device(uint32_t nv, const double* P, uint32_t nt, const uint32_t*
                                                     rows.var.get(), rows.coef.get());
     NLHIP_CHECK(hipGetLastError());
     NLHIP_CHECK(hipStreamSynchronize(s));
+}
+
+void check_triangle_charts(uint32_t nt, const uint32_t* T, const uint32_t* voff, uint32_t nc,
+                           hipStream_t s) {
+    DevBuf<uint32_t> bad(1);
+    NLHIP_CHECK(hipMemsetAsync(bad.get(), 0, 4, s));
+    if (nt) k_tri_chart_check<<<grid_for(nt), kBlock, 0, s>>>(nt, T, voff, nc, bad.get());
+    NLHIP_CHECK(hipGetLastError());
+    uint32_t h = 0;
+    NLHIP_CHECK(hipMemcpyAsync(&h, bad.get(), 4, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipStreamSynchronize(s));
+    if (h) throw Error(NLHIP_ERR_INVALID, "a triangle spans two charts");
 }
 
 void ls_assemble_device(const LSInput& in, LSOutput& out, hipStream_t s) {

@@ -6,6 +6,7 @@
 // captured hipGraph holds `check_every` iterations followed by a rebuild of
 // the active lists, and the host only reads the number of active systems
 // between graph launches.
+#include <cstring>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -413,6 +414,8 @@ void build_batch_blocks(Batch& B, const uint32_t* row_off, const uint32_t* rowpt
     NLHIP_CHECK(hipStreamSynchronize(st));  // host staging arrays go out of scope
 }
 
+void batch_finish(Batch& B, const std::vector<int32_t>& blk, uint32_t flags);
+
 Batch* batch_create(uint32_t nsys, const uint32_t* row_off, const uint32_t* rowptr,
                     const uint32_t* colind, const double* val, uint32_t flags) {
     if (nsys == 0 || !row_off || !rowptr) throw Error(NLHIP_ERR_INVALID, "empty batch");
@@ -462,6 +465,87 @@ Batch* batch_create(uint32_t nsys, const uint32_t* row_off, const uint32_t* rowp
         if (!(flags & NLHIP_SYSTEM_NO_BLOCKS))
             build_batch_blocks(*B, row_off, rowptr, colind, val, blk);
     }
+    batch_finish(*B, blk, flags);
+    return B.release();
+}
+
+// The 2x2 block form of a resident batch whose systems all start at even
+// rows, checked and packed on the device for the whole matrix at once (the
+// single-system kernels): it holds when every global row pair (2I, 2I+1)
+// has the pair structure, and then each system has it with its pairs at
+// global pairs, sys_pair0[s] = row_off[s] / 2.  False (no block form) when
+// a pair fails or a system starts at an odd row.
+bool build_batch_blocks_device(Batch& B, std::vector<int32_t>& blk) {
+    for (uint32_t s = 0; s <= B.nsys; ++s)
+        if (B.h_row_off[s] & 1u) return false;
+    hipStream_t st = B.stream.get();
+    const uint32_t nb = B.n / 2;
+    {
+        DevBuf<uint32_t> bad(1);
+        NLHIP_CHECK(hipMemsetAsync(bad.get(), 0, 4, st));
+        NLHIP_CHECK(block_check(nb, B.rowptr.get(), B.colind.get(), bad.get(), st));
+        uint32_t h = 0;
+        NLHIP_CHECK(hipMemcpyAsync(&h, bad.get(), 4, hipMemcpyDeviceToHost, st));
+        NLHIP_CHECK(hipStreamSynchronize(st));
+        if (h) return false;
+    }
+    const uint64_t nblk = B.nnz / 4;
+    std::vector<uint32_t> pair0(B.nsys);
+    for (uint32_t s = 0; s < B.nsys; ++s) pair0[s] = B.h_row_off[s] / 2;
+    B.sys_pair0.alloc(B.nsys);
+    B.brp.alloc(size_t(nb) + 1);
+    B.bcol.alloc(nblk);
+    B.bval.alloc(4 * nblk);
+    copy_in(B.sys_pair0.get(), pair0.data(), size_t(B.nsys) * 4, st);
+    NLHIP_CHECK(block_pack<double>(nb, B.rowptr.get(), B.colind.get(), B.val.get(), B.brp.get(),
+                                   B.bcol.get(), B.bval.get(), st));
+    if (build_col_offsets(nblk, B.bcol.get(), st, B.bci16, B.bsegb)) B.bcol.release();
+    NLHIP_CHECK(hipStreamSynchronize(st));
+    B.all_blocks = true;
+    std::fill(blk.begin(), blk.end(), 1);
+    return true;
+}
+
+// Takes over a device CRS built by this library (systems [h_row_off[s],
+// h_row_off[s+1]), entries inside their own diagonal blocks, sorted, finite)
+// and finishes the batch as batch_create does.
+Batch* batch_adopt(uint32_t nsys, const uint32_t* h_row_off, uint64_t nnz,
+                   DevBuf<uint32_t>&& rowptr, DevBuf<uint32_t>&& colind, DevBuf<double>&& val,
+                   uint32_t flags) {
+    if (nsys == 0) throw Error(NLHIP_ERR_INVALID, "empty batch");
+    for (uint32_t s = 0; s < nsys; ++s)
+        if (h_row_off[s + 1] <= h_row_off[s])
+            throw Error(NLHIP_ERR_INVALID, "system " + std::to_string(s) + " has no rows");
+    auto B = std::make_unique<Batch>();
+    B->nsys = nsys;
+    B->n = h_row_off[nsys];
+    B->nnz = nnz;
+    B->h_row_off.assign(h_row_off, h_row_off + nsys + 1);
+    hipStream_t st = B->stream.get();
+    B->row_off.alloc(size_t(nsys) + 1);
+    copy_in(B->row_off.get(), h_row_off, (size_t(nsys) + 1) * 4, st);
+    B->rowptr = std::move(rowptr);
+    B->colind = std::move(colind);
+    B->val = std::move(val);
+    B->dinv.alloc(B->n);
+    NLHIP_CHECK(jacobi_dinv<double>(B->n, B->rowptr.get(), B->colind.get(), B->val.get(),
+                                    B->dinv.get(), st));
+    std::vector<int32_t> blk(nsys, 0);
+    if (!(flags & NLHIP_SYSTEM_NO_OFFSETS)) {
+        build_col_offsets(nnz, B->colind.get(), st, B->ci16, B->segb);
+        if (!(flags & NLHIP_SYSTEM_NO_BLOCKS)) build_batch_blocks_device(*B, blk);
+    }
+    batch_finish(*B, blk, flags);
+    return B.release();
+}
+
+// Tiles, launch grid, FP32 copy and FP64 work buffers of a batch whose
+// matrix (and block form, per blk) is resident.
+void batch_finish(Batch& Bref, const std::vector<int32_t>& blk, uint32_t flags) {
+    Batch* B = &Bref;
+    const uint32_t nsys = B->nsys;
+    const uint32_t* row_off = B->h_row_off.data();
+    hipStream_t st = B->stream.get();
     std::vector<uint32_t> tile_off(size_t(nsys) + 1, 0), tile_sys, tile_row0;
     for (uint32_t s = 0; s < nsys; ++s) {
         const uint32_t rows = row_off[s + 1] - row_off[s];
@@ -489,7 +573,6 @@ Batch* batch_create(uint32_t nsys, const uint32_t* row_off, const uint32_t* rowp
     if (flags & NLHIP_SYSTEM_FP32) ensure_fp32(*B);
     work_alloc(*B, B->w64);
     B->stream.sync();
-    return B.release();
 }
 
 }  // namespace
@@ -532,6 +615,25 @@ NLAPI int NLAPIENTRY nlhipBatchInfo(nlhip_batch b, uint32_t* nsys, uint32_t* n, 
         if (n) *n = B->n;
         if (nnz) *nnz = B->nnz;
         if (tiles) *tiles = B->ntiles;
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipBatchGetCRS(nlhip_batch b, uint32_t* row_off, uint32_t* rowptr,
+                                      uint32_t* colind, double* val) {
+    return guarded("nlhipBatchGetCRS", [&] {
+        Batch* B = bat(b);
+        std::lock_guard<std::mutex> lock(B->mu);
+        hipStream_t s = B->stream.get();
+        if (row_off)
+            std::memcpy(row_off, B->h_row_off.data(), B->h_row_off.size() * 4);
+        if (rowptr)
+            NLHIP_CHECK(hipMemcpyAsync(rowptr, B->rowptr.get(), (size_t(B->n) + 1) * 4,
+                                       hipMemcpyDefault, s));
+        if (colind && B->nnz)
+            NLHIP_CHECK(hipMemcpyAsync(colind, B->colind.get(), B->nnz * 4, hipMemcpyDefault, s));
+        if (val && B->nnz)
+            NLHIP_CHECK(hipMemcpyAsync(val, B->val.get(), B->nnz * 8, hipMemcpyDefault, s));
+        NLHIP_CHECK(hipStreamSynchronize(s));
     });
 }
 
@@ -633,6 +735,100 @@ NLAPI int NLAPIENTRY nlhipBatchAutotune(nlhip_batch b, const double* rhs, uint32
         if (workgroups) *workgroups = best_g;
         if (passes) *passes = best_p;
         if (seconds) *seconds = best / iters;
+    });
+}
+
+// An atlas built on the device: the LSCM systems of many charts (one mesh
+// whose vertices and triangles are grouped by chart) as one batch.
+NLAPI int NLAPIENTRY nlhipBatchLSCMCreate(uint32_t nb_charts, const uint32_t* vertex_off,
+                                          const double* points, uint32_t nb_triangles,
+                                          const uint32_t* triangles, const uint32_t* pins,
+                                          const double* pin_uv, uint32_t flags,
+                                          nlhip_batch* out, double* rhs, uint32_t* n_out) {
+    return guarded("nlhipBatchLSCMCreate", [&] {
+        if (!out) throw Error(NLHIP_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        if (nb_charts == 0 || !vertex_off || !points || !triangles || !pins || !pin_uv)
+            throw Error(NLHIP_ERR_INVALID, "empty atlas");
+        device();
+        Stream stream;
+        hipStream_t s = stream.get();
+        // chart layout and pins (host copies; these arrays are small)
+        std::vector<uint32_t> voff(size_t(nb_charts) + 1), pin(2 * size_t(nb_charts));
+        std::vector<double> uv(4 * size_t(nb_charts));
+        NLHIP_CHECK(hipMemcpyAsync(voff.data(), vertex_off, voff.size() * 4, hipMemcpyDefault, s));
+        NLHIP_CHECK(hipMemcpyAsync(pin.data(), pins, pin.size() * 4, hipMemcpyDefault, s));
+        NLHIP_CHECK(hipMemcpyAsync(uv.data(), pin_uv, uv.size() * 8, hipMemcpyDefault, s));
+        NLHIP_CHECK(hipStreamSynchronize(s));
+        if (voff[0] != 0) throw Error(NLHIP_ERR_INVALID, "vertex_off[0] != 0");
+        const uint32_t nv = voff[nb_charts];
+        if (uint64_t(nv) * 2 > 0xffffffffull) throw Error(NLHIP_ERR_UNSUPPORTED, "too many vertices");
+        std::vector<uint32_t> row_off(size_t(nb_charts) + 1, 0);
+        std::vector<uint32_t> free_index(2 * size_t(nv));
+        std::vector<double> values(2 * size_t(nv), 0.0);
+        std::vector<uint8_t> locked(nv, 0);
+        for (uint32_t c = 0; c < nb_charts; ++c) {
+            const uint32_t lo = voff[c], hi = voff[c + 1];
+            if (hi < lo + 3) throw Error(NLHIP_ERR_INVALID, "a chart needs at least 3 vertices");
+            const uint32_t p0 = pin[2 * c], p1 = pin[2 * c + 1];
+            if (p0 == p1 || p0 < lo || p0 >= hi || p1 < lo || p1 >= hi)
+                throw Error(NLHIP_ERR_INVALID, "pins must be two distinct vertices of their chart");
+            for (int k = 0; k < 4; ++k)
+                if (!std::isfinite(uv[4 * c + k]))
+                    throw Error(NLHIP_ERR_NONFINITE, "pin position is NaN or Inf");
+            locked[p0] = locked[p1] = 1;
+            values[2 * size_t(p0)] = uv[4 * c];
+            values[2 * size_t(p0) + 1] = uv[4 * c + 1];
+            values[2 * size_t(p1)] = uv[4 * c + 2];
+            values[2 * size_t(p1) + 1] = uv[4 * c + 3];
+            row_off[c + 1] = row_off[c] + 2 * (hi - lo - 2);
+        }
+        uint32_t f = 0;
+        for (uint32_t v = 0; v < nv; ++v)
+            for (int k = 0; k < 2; ++k) free_index[2 * size_t(v) + k] = locked[v] ? 0xffffffffu : f++;
+        // rows and normal equations on the device
+        DevBuf<double> d_p;
+        DevBuf<uint32_t> d_t, d_voff(voff.size()), d_free(free_index.size());
+        DevBuf<double> d_values(values.size());
+        const double* P = points;
+        const uint32_t* T = triangles;
+        if (!is_device_pointer(points)) {
+            d_p.alloc(3 * size_t(nv));
+            copy_in(d_p.get(), points, 3 * size_t(nv) * 8, s);
+            P = d_p.get();
+        }
+        if (!is_device_pointer(triangles)) {
+            d_t.alloc(3 * size_t(nb_triangles ? nb_triangles : 1));
+            copy_in(d_t.get(), triangles, 3 * size_t(nb_triangles) * 4, s);
+            T = d_t.get();
+        }
+        copy_in(d_voff.get(), voff.data(), voff.size() * 4, s);
+        copy_in(d_free.get(), free_index.data(), free_index.size() * 4, s);
+        copy_in(d_values.get(), values.data(), values.size() * 8, s);
+        LSRows rows;
+        lscm_rows_device(nv, P, nb_triangles, T, rows, s);   // checks T < nv, finite P
+        check_triangle_charts(nb_triangles, T, d_voff.get(), nb_charts, s);
+        LSInput in;
+        in.nrows = rows.nrows;
+        in.nvars = 2 * nv;
+        in.nsys = 1;
+        in.nfree = f;
+        in.nent = 6ull * rows.nrows;
+        in.rp = rows.rp.get();
+        in.var = rows.var.get();
+        in.coef = rows.coef.get();
+        in.free_index = d_free.get();
+        in.values = d_values.get();
+        LSOutput o;
+        ls_assemble_device(in, o, s);
+        if (rhs) {
+            NLHIP_CHECK(hipMemcpyAsync(rhs, o.rhs.get(), size_t(f) * 8, hipMemcpyDefault, s));
+            NLHIP_CHECK(hipStreamSynchronize(s));
+        }
+        if (n_out) *n_out = f;
+        Batch* B = batch_adopt(nb_charts, row_off.data(), o.nnz, std::move(o.rowptr),
+                               std::move(o.colind), std::move(o.val), flags);
+        *out = reinterpret_cast<nlhip_batch>(B);
     });
 }
 

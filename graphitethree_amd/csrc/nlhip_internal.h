@@ -467,6 +467,10 @@ struct LSRows {
 };
 void lscm_rows_device(uint32_t nv, const double* points, uint32_t nt, const uint32_t* triangles,
                       LSRows& rows, hipStream_t s);
+// Throws when a triangle's vertices lie in different charts (device arrays;
+// chart c owns vertices [voff[c], voff[c+1])).
+void check_triangle_charts(uint32_t nt, const uint32_t* T, const uint32_t* voff, uint32_t nc,
+                           hipStream_t s);
 // nlhipLeastSquaresCreate: host or device inputs, checks, assembly, system;
 // rhs (nsys x nfree, host or device) may be null.
 System* ls_create(uint32_t nvars, uint32_t nrows, const uint32_t* row_ptr, const uint32_t* var,

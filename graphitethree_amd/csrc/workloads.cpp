@@ -483,10 +483,21 @@ struct Atlas {
 
 Atlas* build_atlas(uint32_t count, uint64_t seed, double Nmin, double Nmax);
 
+// The meshes of the same atlas, concatenated: chart c owns vertices
+// [vertex_off[c], vertex_off[c+1]) and the triangles over them (global ids),
+// with its two pins (global ids) and their (u0, v0, u1, v1).
+struct AtlasMesh {
+    std::vector<uint32_t> vertex_off, tri_off, T, pins;
+    std::vector<double> P, pin_uv;
+};
+
+AtlasMesh* build_atlas_mesh(uint32_t count, uint64_t seed, double Nmin, double Nmax);
+
 }  // namespace
 
 extern "C" void wl_atlas_chart(uint64_t seed, uint32_t idx, double Nmin, double Nmax,
                                uint32_t* N, uint64_t* chart_seed);
+extern "C" void wl_mesh_extreme_pins(const void* mp, uint32_t* p0, uint32_t* p1);
 
 namespace {
 Atlas* build_atlas(uint32_t count, uint64_t seed, double Nmin, double Nmax) {
@@ -544,6 +555,45 @@ Atlas* build_atlas(uint32_t count, uint64_t seed, double Nmin, double Nmax) {
         delete sys[c];
     }
     A->rowptr[n] = uint32_t(nnz);
+    return A;
+}
+
+AtlasMesh* build_atlas_mesh(uint32_t count, uint64_t seed, double Nmin, double Nmax) {
+    std::vector<Mesh*> mesh(count, nullptr);
+    std::vector<uint32_t> pin(2 * size_t(count));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long c = 0; c < long(count); ++c) {
+        uint32_t N;
+        uint64_t cs;
+        wl_atlas_chart(seed, uint32_t(c), Nmin, Nmax, &N, &cs);
+        mesh[c] = chart_mesh(N, cs);
+        wl_mesh_extreme_pins(mesh[c], &pin[2 * c], &pin[2 * c + 1]);
+    }
+    AtlasMesh* A = new AtlasMesh;
+    A->vertex_off.assign(size_t(count) + 1, 0);
+    A->tri_off.assign(size_t(count) + 1, 0);
+    for (uint32_t c = 0; c < count; ++c) {
+        A->vertex_off[c + 1] = A->vertex_off[c] + mesh[c]->nv();
+        A->tri_off[c + 1] = A->tri_off[c] + mesh[c]->nt();
+    }
+    A->P.resize(3 * size_t(A->vertex_off[count]));
+    A->T.resize(3 * size_t(A->tri_off[count]));
+    A->pins.resize(2 * size_t(count));
+    A->pin_uv.resize(4 * size_t(count));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long c = 0; c < long(count); ++c) {
+        const Mesh& M = *mesh[c];
+        const uint32_t v0 = A->vertex_off[c];
+        std::copy(M.P.begin(), M.P.end(), A->P.begin() + 3 * size_t(v0));
+        for (size_t j = 0; j < M.T.size(); ++j) A->T[3 * size_t(A->tri_off[c]) + j] = v0 + M.T[j];
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t p = pin[2 * c + k];
+            A->pins[2 * c + k] = v0 + p;
+            A->pin_uv[4 * c + 2 * k] = M.P[3 * size_t(p)];
+            A->pin_uv[4 * c + 2 * k + 1] = M.P[3 * size_t(p) + 1];
+        }
+        delete mesh[c];
+    }
     return A;
 }
 }  // namespace
@@ -686,5 +736,35 @@ const uint32_t* wl_atlas_chart_nv(const void* a) {
     return static_cast<const Atlas*>(a)->chart_nv.data();
 }
 void wl_atlas_free(void* a) { delete static_cast<Atlas*>(a); }
+
+void* wl_atlas_mesh(uint32_t count, uint64_t seed, double Nmin, double Nmax) {
+    if (count == 0) return nullptr;
+    return build_atlas_mesh(count, seed, Nmin, Nmax);
+}
+uint32_t wl_atlas_mesh_nv(const void* a) {
+    return static_cast<const AtlasMesh*>(a)->vertex_off.back();
+}
+uint32_t wl_atlas_mesh_nt(const void* a) {
+    return static_cast<const AtlasMesh*>(a)->tri_off.back();
+}
+const uint32_t* wl_atlas_mesh_vertex_off(const void* a) {
+    return static_cast<const AtlasMesh*>(a)->vertex_off.data();
+}
+const uint32_t* wl_atlas_mesh_tri_off(const void* a) {
+    return static_cast<const AtlasMesh*>(a)->tri_off.data();
+}
+const double* wl_atlas_mesh_points(const void* a) {
+    return static_cast<const AtlasMesh*>(a)->P.data();
+}
+const uint32_t* wl_atlas_mesh_triangles(const void* a) {
+    return static_cast<const AtlasMesh*>(a)->T.data();
+}
+const uint32_t* wl_atlas_mesh_pins(const void* a) {
+    return static_cast<const AtlasMesh*>(a)->pins.data();
+}
+const double* wl_atlas_mesh_pin_uv(const void* a) {
+    return static_cast<const AtlasMesh*>(a)->pin_uv.data();
+}
+void wl_atlas_mesh_free(void* a) { delete static_cast<AtlasMesh*>(a); }
 
 }  // extern "C"

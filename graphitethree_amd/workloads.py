@@ -64,6 +64,16 @@ def lib() -> C.CDLL:
             "wl_atlas_rhs": (_f64p, [C.c_void_p]),
             "wl_atlas_chart_nv": (_u32p, [C.c_void_p]),
             "wl_atlas_free": (None, [C.c_void_p]),
+            "wl_atlas_mesh": (C.c_void_p, [C.c_uint32, C.c_uint64, C.c_double, C.c_double]),
+            "wl_atlas_mesh_nv": (C.c_uint32, [C.c_void_p]),
+            "wl_atlas_mesh_nt": (C.c_uint32, [C.c_void_p]),
+            "wl_atlas_mesh_vertex_off": (_u32p, [C.c_void_p]),
+            "wl_atlas_mesh_tri_off": (_u32p, [C.c_void_p]),
+            "wl_atlas_mesh_points": (_f64p, [C.c_void_p]),
+            "wl_atlas_mesh_triangles": (_u32p, [C.c_void_p]),
+            "wl_atlas_mesh_pins": (_u32p, [C.c_void_p]),
+            "wl_atlas_mesh_pin_uv": (_f64p, [C.c_void_p]),
+            "wl_atlas_mesh_free": (None, [C.c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -267,6 +277,42 @@ def atlas(count: int, seed: int = 1, nmin: float = 50, nmax: float = 50000) -> A
                     _arr(L.wl_atlas_chart_nv(h), m, np.uint32))
     finally:
         L.wl_atlas_free(h)
+    return out
+
+
+@dataclass
+class AtlasMesh:
+    """The meshes behind `atlas(count, seed, ...)`, concatenated: chart c owns
+    vertices [vertex_off[c], vertex_off[c+1]) and triangles
+    [tri_off[c], tri_off[c+1]) (global vertex ids); pins (count, 2) global
+    vertex ids with pin_uv (count, 4) = (u0, v0, u1, v1)."""
+    vertex_off: np.ndarray
+    tri_off: np.ndarray
+    points: np.ndarray     # (nv, 3)
+    triangles: np.ndarray  # (nt, 3)
+    pins: np.ndarray
+    pin_uv: np.ndarray
+
+    @property
+    def count(self) -> int:
+        return len(self.vertex_off) - 1
+
+
+def atlas_mesh(count: int, seed: int = 1, nmin: float = 50, nmax: float = 50000) -> AtlasMesh:
+    L = lib()
+    h = L.wl_atlas_mesh(count, seed, nmin, nmax)
+    if not h:
+        raise ValueError("atlas construction failed")
+    try:
+        nv, nt = L.wl_atlas_mesh_nv(h), L.wl_atlas_mesh_nt(h)
+        out = AtlasMesh(_arr(L.wl_atlas_mesh_vertex_off(h), count + 1, np.uint32),
+                        _arr(L.wl_atlas_mesh_tri_off(h), count + 1, np.uint32),
+                        _arr(L.wl_atlas_mesh_points(h), 3 * nv, np.float64).reshape(nv, 3),
+                        _arr(L.wl_atlas_mesh_triangles(h), 3 * nt, np.uint32).reshape(nt, 3),
+                        _arr(L.wl_atlas_mesh_pins(h), 2 * count, np.uint32).reshape(count, 2),
+                        _arr(L.wl_atlas_mesh_pin_uv(h), 4 * count, np.float64).reshape(count, 4))
+    finally:
+        L.wl_atlas_mesh_free(h)
     return out
 
 

@@ -228,9 +228,30 @@ NLAPI int NLAPIENTRY nlhipBatchCreate(uint32_t nsys, const uint32_t* row_off,
                                       const uint32_t* rowptr, const uint32_t* colind,
                                       const double* val, uint32_t flags,
                                       nlhip_batch* out);
+/* A texture atlas built on the device (mesh_make_atlas, mesh_grob_surface_
+ * commands.cpp:724-730): the LSCM systems of nb_charts charts of one mesh,
+ * as nlhipLSCMCreate builds each (rows of every non-degenerate triangle,
+ * two pinned vertices per chart, device least-squares assembly), packed as
+ * a batch.  Chart c owns vertices [vertex_off[c], vertex_off[c+1]) (host
+ * array) and the triangles over them; points (3 per vertex) and triangles
+ * (3 global vertex ids each) host or device; pins: 2 global vertex ids per
+ * chart, pin_uv: (u0, v0, u1, v1) per chart (host).  System c is over the
+ * free (u, v) of chart c in vertex order; rhs (n values, host or device,
+ * may be NULL) receives the right-hand sides and n_out their number. */
+NLAPI int NLAPIENTRY nlhipBatchLSCMCreate(uint32_t nb_charts, const uint32_t* vertex_off,
+                                          const double* points, uint32_t nb_triangles,
+                                          const uint32_t* triangles, const uint32_t* pins,
+                                          const double* pin_uv, uint32_t flags,
+                                          nlhip_batch* out, double* rhs, uint32_t* n_out);
 NLAPI int NLAPIENTRY nlhipBatchDestroy(nlhip_batch b);
 NLAPI int NLAPIENTRY nlhipBatchInfo(nlhip_batch b, uint32_t* nsys, uint32_t* n,
                                     uint64_t* nnz, uint32_t* tiles);
+/* The packed CRS the batch holds, copied out (each pointer host or device,
+ * may be NULL): row_off nsys + 1, rowptr n + 1, colind and val nnz values.
+ * No counterpart in the reference (its matrices are host arrays); this is how
+ * a caller inspects a batch the device assembled. */
+NLAPI int NLAPIENTRY nlhipBatchGetCRS(nlhip_batch b, uint32_t* row_off, uint32_t* rowptr,
+                                      uint32_t* colind, double* val);
 /* Workgroups of the grid that walks the active tiles. */
 NLAPI int NLAPIENTRY nlhipBatchSetLaunch(nlhip_batch b, uint32_t workgroups);
 /* Passes of a tile (64 rows or block rows each, 4 per tile) that one

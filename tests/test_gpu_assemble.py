@@ -239,3 +239,52 @@ def test_rejects_bad_input(hip):
         capi.System.least_squares(2, rp, np.array([0, 1], np.uint32), np.array([1.0, 1.0]),
                                   locked=np.array([1, 0], np.uint8),
                                   values=np.array([np.inf, 0.0]))
+
+
+def test_atlas_built_on_device_matches_host_packing(hip):
+    """nlhipBatchLSCMCreate over the concatenated chart meshes gives the
+    batch the host builds chart by chart and packs (same CRS and right-hand
+    sides, bitwise), and solves to the same bits."""
+    count, seed, nmin, nmax = 14, 3, 40, 3000
+    A = W.atlas(count, seed=seed, nmin=nmin, nmax=nmax)
+    M = W.atlas_mesh(count, seed=seed, nmin=nmin, nmax=nmax)
+    assert np.array_equal(np.diff(M.vertex_off), A.chart_nv)
+    B, rhs = capi.Batch.lscm_atlas(M.vertex_off, M.points, M.triangles, M.pins, M.pin_uv)
+    ro, rp, ci, va = B.crs()
+    assert np.array_equal(ro, A.row_off)
+    assert np.array_equal(rp, A.rowptr)
+    assert np.array_equal(ci, A.colind)
+    assert np.array_equal(bits(va), bits(A.val))
+    assert np.array_equal(bits(rhs), bits(A.b))
+    H = capi.Batch(packed=(A.row_off, A.rowptr, A.colind, A.val))
+    x1, p1, _ = B.solve(rhs, eps=1e-10, max_iter=20000)
+    x2, p2, _ = H.solve(A.b, eps=1e-10, max_iter=20000)
+    assert all(p.converged for p in p1)
+    assert np.array_equal(bits(x1), bits(x2))
+    B.close()
+    H.close()
+
+
+def test_atlas_rejects_bad_charts(hip):
+    M = W.atlas_mesh(3, seed=2, nmin=30, nmax=60)
+    vo, P, T, pins, uv = M.vertex_off, M.points, M.triangles.copy(), M.pins.copy(), M.pin_uv
+    T2 = T.copy()
+    T2[0, 0] = vo[1] + 1                       # a triangle of chart 0 reaching into chart 1
+    with pytest.raises(capi.NLHIPError):
+        capi.Batch.lscm_atlas(vo, P, T2, pins, uv)
+    p2 = pins.copy()
+    p2[1, 0] = vo[0]                           # a pin of chart 1 inside chart 0
+    with pytest.raises(capi.NLHIPError):
+        capi.Batch.lscm_atlas(vo, P, T, p2, uv)
+    p3 = pins.copy()
+    p3[0, 1] = p3[0, 0]                        # the same vertex pinned twice
+    with pytest.raises(capi.NLHIPError):
+        capi.Batch.lscm_atlas(vo, P, T, p3, uv)
+    uv2 = uv.copy()
+    uv2[2, 3] = np.nan
+    with pytest.raises(capi.NLHIPError):
+        capi.Batch.lscm_atlas(vo, P, T, pins, uv2)
+    vo2 = vo.copy()
+    vo2[1] = 2                                 # a chart of two vertices
+    with pytest.raises(capi.NLHIPError):
+        capi.Batch.lscm_atlas(vo2, P, T, pins, uv)

@@ -144,3 +144,30 @@ def test_generators_are_deterministic():
     c = W.lscm_grid(10, seed=43)
     assert np.array_equal(a.val, b.val) and np.array_equal(a.b, b.b)
     assert not np.array_equal(a.val, c.val)
+
+
+def test_atlas_mesh_is_the_atlas_charts():
+    """The concatenated chart meshes are the meshes the packed atlas is built
+    from: same vertex counts, triangles inside their charts, pins at the
+    extreme vertices, and chart 0's mesh rebuilt alone gives chart 0's system."""
+    count, seed = 6, 9
+    A = W.atlas(count, seed=seed, nmin=40, nmax=400)
+    M = W.atlas_mesh(count, seed=seed, nmin=40, nmax=400)
+    assert np.array_equal(np.diff(M.vertex_off), A.chart_nv)
+    for c in range(count):
+        v0, v1 = int(M.vertex_off[c]), int(M.vertex_off[c + 1])
+        t0, t1 = int(M.tri_off[c]), int(M.tri_off[c + 1])
+        tri = M.triangles[t0:t1]
+        assert tri.min() >= v0 and tri.max() < v1
+        N, cs = W.atlas_chart_params(seed, c, 40, 400)
+        m = W.chart_mesh(N, cs)
+        assert np.array_equal(m.points, M.points[v0:v1])
+        assert np.array_equal(m.triangles, tri - v0)
+        assert tuple(M.pins[c] - v0) == m.extreme_pins()
+        assert np.array_equal(M.pin_uv[c], [m.points[M.pins[c][0] - v0, 0],
+                                            m.points[M.pins[c][0] - v0, 1],
+                                            m.points[M.pins[c][1] - v0, 0],
+                                            m.points[M.pins[c][1] - v0, 1]])
+        S = W.lscm_system(m)
+        Sc = A.chart(c)
+        assert np.array_equal(S.val, Sc.val) and np.array_equal(S.b[0], Sc.b[0])
