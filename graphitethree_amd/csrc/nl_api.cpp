@@ -22,6 +22,7 @@ namespace {
 
 constexpr uint32_t kMagicHIP = 0x48495043u;     // device CRS matrix
 constexpr uint32_t kMagicJacobi = 0x4a41434fu;  // Jacobi preconditioner
+constexpr uint32_t kMagicSSOR = 0x53534f52u;    // SSOR preconditioner
 
 // Device mirror of a host matrix (dynamic or CRS); rebuilt when stale.
 struct Mirror {
@@ -46,6 +47,15 @@ struct JacobiMatrix {
     NLMultMatrixVectorFunc mult_func;
     uint32_t magic;
     DevBuf<double>* dinv;
+};
+
+struct SSORMatrix {
+    NLuint m, n;
+    NLenum type;
+    NLDestroyMatrixFunc destroy_func;
+    NLMultMatrixVectorFunc mult_func;
+    uint32_t magic;
+    Ssor* ssor;
 };
 
 // Utility stream and staging buffers for operations not tied to a system
@@ -224,12 +234,37 @@ System* system_of(NLMatrix M) {
     throw Error(NLHIP_ERR_UNSUPPORTED, "matrix type has no GPU implementation");
 }
 
+SSORMatrix* as_ssor(NLMatrix P) {
+    if (P && P->type == NL_MATRIX_OTHER) {
+        SSORMatrix* Q = reinterpret_cast<SSORMatrix*>(P);
+        if (Q->magic == kMagicSSOR) return Q;
+    }
+    return nullptr;
+}
+
 JacobiMatrix* as_jacobi(NLMatrix P) {
     if (P && P->type == NL_MATRIX_OTHER) {
         JacobiMatrix* J = reinterpret_cast<JacobiMatrix*>(P);
         if (J->magic == kMagicJacobi) return J;
     }
     return nullptr;
+}
+
+// The preconditioner object P of nlSolveSystemIterative for a system of
+// size n: Jacobi or SSOR (or none when P is null).
+Precond precond_of(NLMatrix P, uint32_t n) {
+    Precond pc;
+    if (!P) return pc;
+    if (SSORMatrix* Q = as_ssor(P)) {
+        if (Q->n != n) throw Error(NLHIP_ERR_INVALID, "preconditioner size mismatch");
+        pc.ssor = Q->ssor;
+        return pc;
+    }
+    JacobiMatrix* J = as_jacobi(P);
+    if (!J) throw Error(NLHIP_ERR_UNSUPPORTED, "only Jacobi and SSOR preconditioners run on the GPU");
+    if (J->n != n) throw Error(NLHIP_ERR_INVALID, "preconditioner size mismatch");
+    pc.dinv = J->dinv->get();
+    return pc;
 }
 
 // y = A x for a device system; x, y host or device.
@@ -279,6 +314,38 @@ void jac_mult(NLMatrix M, const double* x, double* y) {
         copy_out(y, u.b.get(), size_t(J->n) * 8, s);
         u.stream.sync();
     });
+}
+
+void ssor_matrix_destroy(NLMatrix M) {
+    SSORMatrix* Q = reinterpret_cast<SSORMatrix*>(M);
+    ssor_destroy(Q->ssor);
+    Q->ssor = nullptr;
+}
+
+void ssor_matrix_mult(NLMatrix M, const double* x, double* y) {
+    guarded("nlMultMatrixVector", [&] {
+        SSORMatrix* Q = reinterpret_cast<SSORMatrix*>(M);
+        Util& u = util();
+        std::lock_guard<std::mutex> lock(u.mu);
+        hipStream_t s = u.stream.get();
+        u.ensure(Q->n);
+        copy_in(u.a.get(), x, size_t(Q->n) * 8, s);
+        ssor_apply(*Q->ssor, u.a.get(), u.b.get(), s);
+        copy_out(y, u.b.get(), size_t(Q->n) * 8, s);
+        u.stream.sync();
+    });
+}
+
+NLMatrix ssor_new(NLMatrix M, double omega) {
+    System* S = system_of(M);
+    auto Q = std::make_unique<SSORMatrix>();
+    Q->m = Q->n = S->n;
+    Q->type = NL_MATRIX_OTHER;
+    Q->destroy_func = ssor_matrix_destroy;
+    Q->mult_func = ssor_matrix_mult;
+    Q->magic = kMagicSSOR;
+    Q->ssor = ssor_create(*S, omega);
+    return reinterpret_cast<NLMatrix>(Q.release());
 }
 
 NLMatrix jacobi_new(NLMatrix M) {
@@ -829,8 +896,9 @@ bool solve_context(Context& C) {
             copy_in(S->ws64.b.get(), b.data(), size_t(n) * 8, s);
             copy_in(S->ws64.x.get(), x.data(), size_t(n) * 8, s);
             double rel2 = 0.0;
-            const double* dinv = (C.preconditioner == NL_PRECOND_JACOBI) ? S->dinv.get() : nullptr;
-            C.used_iterations += bicgstab_device(*S, dinv, S->ws64.b.get(), S->ws64.x.get(),
+            Precond pc;
+            pc.dinv = (C.preconditioner == NL_PRECOND_JACOBI) ? S->dinv.get() : nullptr;
+            C.used_iterations += bicgstab_device(*S, pc, S->ws64.b.get(), S->ws64.x.get(),
                                                  C.threshold, max_iter, &rel2);
             copy_out(x.data(), S->ws64.x.get(), size_t(n) * 8, s);
             S->stream.sync();
@@ -842,8 +910,9 @@ bool solve_context(Context& C) {
             copy_in(S->ws64.b.get(), b.data(), size_t(n) * 8, s);
             copy_in(S->ws64.x.get(), x.data(), size_t(n) * 8, s);
             double rel2 = 0.0;
-            const double* dinv = (C.preconditioner == NL_PRECOND_JACOBI) ? S->dinv.get() : nullptr;
-            C.used_iterations += gmres_device(*S, dinv, S->ws64.b.get(), S->ws64.x.get(),
+            Precond pc;
+            pc.dinv = (C.preconditioner == NL_PRECOND_JACOBI) ? S->dinv.get() : nullptr;
+            C.used_iterations += gmres_device(*S, pc, S->ws64.b.get(), S->ws64.x.get(),
                                               C.threshold, max_iter,
                                               C.inner_iterations ? C.inner_iterations : 5, &rel2);
             copy_out(x.data(), S->ws64.x.get(), size_t(n) * 8, s);
@@ -953,6 +1022,10 @@ NLAPI void NLAPIENTRY nlDeleteMatrix(NLMatrix M) {
             delete J;
             return;
         }
+        if (SSORMatrix* Q = as_ssor(M)) {
+            delete Q;
+            return;
+        }
     }
     std::free(M);
 }
@@ -968,9 +1041,10 @@ NLAPI NLMatrix NLAPIENTRY nlNewJacobiPreconditioner(NLMatrix M) {
     return r;
 }
 
-NLAPI NLMatrix NLAPIENTRY nlNewSSORPreconditioner(NLMatrix, NLdouble) {
-    log_error("nlNewSSORPreconditioner: SSOR has no GPU implementation in this library");
-    return nullptr;
+NLAPI NLMatrix NLAPIENTRY nlNewSSORPreconditioner(NLMatrix M, NLdouble omega) {
+    NLMatrix r = nullptr;
+    guarded("nlNewSSORPreconditioner", [&] { r = ssor_new(M, omega); });
+    return r;
 }
 
 // ---- BLAS ---------------------------------------------------------------
@@ -1004,14 +1078,19 @@ NLAPI NLuint NLAPIENTRY nlSolveSystemIterative(NLBlas_t blas, NLMatrix M, NLMatr
     guarded("nlSolveSystemIterative", [&] {
         if (!b || !x) throw Error(NLHIP_ERR_INVALID, "null b or x");
         System* S = system_of(M);
-        const double* dinv = nullptr;
-        if (P) {
-            JacobiMatrix* J = as_jacobi(P);
-            if (!J) throw Error(NLHIP_ERR_UNSUPPORTED, "only Jacobi preconditioners run on the GPU");
-            if (J->n != S->n) throw Error(NLHIP_ERR_INVALID, "preconditioner size mismatch");
-            dinv = J->dinv->get();
-        }
-        if (solver == NL_CG || solver == NL_SOLVER_DEFAULT) {
+        const Precond pc = precond_of(P, S->n);
+        const double* dinv = pc.dinv;
+        if ((solver == NL_CG || solver == NL_SOLVER_DEFAULT) && pc.ssor) {
+            std::lock_guard<std::mutex> lock(S->mu);
+            S->ws64.ensure(S->n, S->grid);
+            hipStream_t s = S->stream.get();
+            copy_in(S->ws64.b.get(), b, size_t(S->n) * 8, s);
+            copy_in(S->ws64.x.get(), x, size_t(S->n) * 8, s);
+            its = pcg_device(*S, pc, S->ws64.b.get(), S->ws64.x.get(), eps, max_iter, nullptr);
+            copy_out(x, S->ws64.x.get(), size_t(S->n) * 8, s);
+            S->stream.sync();
+            if (blas) blas->flops += NLulong(its) * (4 * S->nnz + 12 * NLulong(S->n));
+        } else if (solver == NL_CG || solver == NL_SOLVER_DEFAULT) {
             nlhip_solve_params p;
             nlhipSolveParamsDefault(&p);
             p.eps = eps;
@@ -1034,7 +1113,7 @@ NLAPI NLuint NLAPIENTRY nlSolveSystemIterative(NLBlas_t blas, NLMatrix M, NLMatr
             hipStream_t s = S->stream.get();
             copy_in(S->ws64.b.get(), b, size_t(S->n) * 8, s);
             copy_in(S->ws64.x.get(), x, size_t(S->n) * 8, s);
-            its = bicgstab_device(*S, dinv, S->ws64.b.get(), S->ws64.x.get(), eps, max_iter,
+            its = bicgstab_device(*S, pc, S->ws64.b.get(), S->ws64.x.get(), eps, max_iter,
                                   nullptr);
             copy_out(x, S->ws64.x.get(), size_t(S->n) * 8, s);
             S->stream.sync();
@@ -1046,7 +1125,7 @@ NLAPI NLuint NLAPIENTRY nlSolveSystemIterative(NLBlas_t blas, NLMatrix M, NLMatr
             copy_in(S->ws64.b.get(), b, size_t(S->n) * 8, s);
             copy_in(S->ws64.x.get(), x, size_t(S->n) * 8, s);
             const NLuint m = inner_iter ? inner_iter : 5;  // matrix.cpp:385 passes 5
-            its = gmres_device(*S, dinv, S->ws64.b.get(), S->ws64.x.get(), eps, max_iter, m,
+            its = gmres_device(*S, pc, S->ws64.b.get(), S->ws64.x.get(), eps, max_iter, m,
                                nullptr);
             copy_out(x, S->ws64.x.get(), size_t(S->n) * 8, s);
             S->stream.sync();

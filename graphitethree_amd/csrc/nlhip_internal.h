@@ -495,12 +495,62 @@ void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
 void solve_multi(System& S, uint32_t k, const double* B, double* X,
                  const nlhip_solve_params& p, nlhip_solve_stats* per, nlhip_solve_stats* total);
 
+// ------------------------------------------------------------------ SSOR
+
+// Arguments of the SSOR sweeps (k_ssor.hip): the matrix, d_i and d_i / omega,
+// the rows of each sweep sorted by level with the level offsets, and the
+// vectors x (input), w (forward sweep) and y (output).
+struct SsorArgs {
+    const uint32_t* rp;
+    const uint32_t* ci;
+    const double* val;
+    const double* diag;
+    const double* dscale;
+    const uint32_t* rows_lo;
+    const uint32_t* off_lo;
+    const uint32_t* rows_up;
+    const uint32_t* off_up;
+    const double* x;
+    double* w;
+    double* y;
+    double omega, post;  // post = 2 - omega
+    uint32_t n;
+};
+hipError_t ssor_diag(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* val,
+                     double omega, double* diag, double* dscale, uint32_t* bad, hipStream_t s);
+// Levels [l0, l1) of one sweep; `rows` rows in all (one level: the grid
+// covers them; several levels: one workgroup).
+hipError_t ssor_sweep(const SsorArgs& a, bool upper, uint32_t l0, uint32_t l1, uint32_t rows,
+                      hipStream_t s);
+hipError_t ssor_scale(const SsorArgs& a, hipStream_t s);
+
+// An SSOR(omega) preconditioner of a resident matrix (its own copy of the
+// CRS, level schedules built on the host, the sweeps captured as a graph).
+class Ssor;
+Ssor* ssor_create(System& S, double omega);
+void ssor_destroy(Ssor* P);
+uint32_t ssor_n(const Ssor& P);
+// y = SSOR^-1 x for device vectors, on stream s.
+void ssor_apply(Ssor& P, const double* x, double* y, hipStream_t s);
+
+// The preconditioner of the host-driven solvers: Jacobi (dinv), SSOR, or
+// none.
+struct Precond {
+    const double* dinv = nullptr;
+    Ssor* ssor = nullptr;
+};
+void precond_apply(const Precond& P, uint32_t n, const double* in, double* out, hipStream_t s);
+
 // Device BiCGSTAB on BLAS-1 kernels (NL_BICGSTAB); returns iterations.
-uint32_t bicgstab_device(System& S, const double* dinv, const double* b_dev, double* x_dev,
+uint32_t bicgstab_device(System& S, const Precond& P, const double* b_dev, double* x_dev,
                          double eps, uint32_t max_iter, double* rel2);
-// Restarted GMRES(m) on the device (right-preconditioned with dinv when not
-// null); returns the number of Arnoldi steps.
-uint32_t gmres_device(System& S, const double* dinv, const double* b_dev, double* x_dev,
+// Restarted GMRES(m) on the device (right-preconditioned when P is set);
+// returns the number of Arnoldi steps.
+uint32_t gmres_device(System& S, const Precond& P, const double* b_dev, double* x_dev,
                       double eps, uint32_t max_iter, uint32_t m, double* rel2);
+// PCG with any Precond, host-driven like BiCGSTAB (the fused CG kernels
+// apply Jacobi only); returns iterations.
+uint32_t pcg_device(System& S, const Precond& P, const double* b_dev, double* x_dev, double eps,
+                    uint32_t max_iter, double* rel2);
 
 }  // namespace nlhip

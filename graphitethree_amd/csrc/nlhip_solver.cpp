@@ -436,8 +436,8 @@ struct DotHelper {
 }  // namespace
 
 // Same operation sequence as the van der Vorst BiCGSTAB restated in the
-// oracle (right preconditioning by dinv, r = b - A x).
-uint32_t bicgstab_device(System& S, const double* dinv, const double* b, double* x, double eps,
+// oracle (right preconditioning by P, r = b - A x).
+uint32_t bicgstab_device(System& S, const Precond& P, const double* b, double* x, double eps,
                          uint32_t max_iter, double* rel2) {
     const uint32_t n = S.n;
     hipStream_t s = S.stream.get();
@@ -451,10 +451,7 @@ uint32_t bicgstab_device(System& S, const double* dinv, const double* b, double*
     auto spmv64 = [&](const double* in, double* o) {
         NLHIP_CHECK(spmv<double>(n, A.rp, A.cols, A.val, in, o, A.code, S.grid, s));
     };
-    auto precond = [&](const double* in, double* o) {
-        if (dinv) NLHIP_CHECK(blas_dmul(n, dinv, in, o, s));
-        else NLHIP_CHECK(hipMemcpyAsync(o, in, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
-    };
+    auto precond = [&](const double* in, double* o) { precond_apply(P, n, in, o, s); };
     auto copy = [&](const double* in, double* o) {
         NLHIP_CHECK(hipMemcpyAsync(o, in, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
     };
@@ -506,11 +503,57 @@ uint32_t bicgstab_device(System& S, const double* dinv, const double* b, double*
     return its;
 }
 
+// PCG with any preconditioner, in the operation order of the oracle's
+// pcg_core (r = A x - b, x -= alpha d; OpenNL's CG behind
+// nlSolveSystemIterative, scene_graph/NL/matrix.cpp:390-393), each inner
+// product reduced on the device and read back.  The fused CG (solve) stays
+// the path for Jacobi and no preconditioner; this one serves SSOR.
+uint32_t pcg_device(System& S, const Precond& P, const double* b, double* x, double eps,
+                    uint32_t max_iter, double* rel2) {
+    const uint32_t n = S.n;
+    hipStream_t s = S.stream.get();
+    DevBuf<double> buf(size_t(4) * n);
+    double *r = buf.get(), *d = r + n, *h = d + n, *Ad = h + n;
+    DevBuf<double> part(S.grid), out(2);
+    S.host_scalars.ensure(4);
+    DotHelper dot{S, part, out.get(), S.host_scalars.get()};
+    const SpmvOps<double> A = S.ops<double>();
+    auto spmv64 = [&](const double* in, double* o) {
+        NLHIP_CHECK(spmv<double>(n, A.rp, A.cols, A.val, in, o, A.code, S.grid, s));
+    };
+    spmv64(x, r);
+    NLHIP_CHECK(blas_daxpy(n, -1.0, b, r, s));
+    precond_apply(P, n, r, d, s);
+    NLHIP_CHECK(hipMemcpyAsync(h, d, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+    double rh = dot(r, h);
+    double err = dot(r, r);
+    const double bb = dot(b, b);
+    const double thresh = (eps * eps) * bb;
+    uint32_t its = 0;
+    while (err > thresh && its < max_iter) {
+        spmv64(d, Ad);
+        const double alpha = rh / dot(d, Ad);
+        NLHIP_CHECK(blas_daxpy(n, -alpha, d, x, s));
+        NLHIP_CHECK(blas_daxpy(n, -alpha, Ad, r, s));
+        precond_apply(P, n, r, h, s);
+        const double rh_new = dot(r, h);
+        const double beta = rh_new / rh;
+        rh = rh_new;
+        NLHIP_CHECK(blas_dscal(n, beta, d, s));
+        NLHIP_CHECK(blas_daxpy(n, 1.0, h, d, s));
+        ++its;
+        err = dot(r, r);
+    }
+    S.stream.sync();
+    if (rel2) *rel2 = bb > 0.0 ? err / bb : err;
+    return its;
+}
+
 // Restarted GMRES(m) (Saad & Schultz 1986), modified Gram-Schmidt Arnoldi,
-// Givens rotations, right preconditioning with dinv (null = none).  Same
+// Givens rotations, right preconditioning with P (Jacobi, SSOR or none).  Same
 // method and stopping rule as the oracle's or_gmres; each inner product is
 // reduced on the device and read back (a host-driven loop, like BiCGSTAB).
-uint32_t gmres_device(System& S, const double* dinv, const double* b, double* x, double eps,
+uint32_t gmres_device(System& S, const Precond& P, const double* b, double* x, double eps,
                       uint32_t max_iter, uint32_t m, double* rel2) {
     if (m < 1) m = 1;
     const uint32_t n = S.n;
@@ -525,10 +568,7 @@ uint32_t gmres_device(System& S, const double* dinv, const double* b, double* x,
     auto spmv64 = [&](const double* in, double* o) {
         NLHIP_CHECK(spmv<double>(n, A.rp, A.cols, A.val, in, o, A.code, S.grid, s));
     };
-    auto precond = [&](const double* in, double* o) {
-        if (dinv) NLHIP_CHECK(blas_dmul(n, dinv, in, o, s));
-        else NLHIP_CHECK(hipMemcpyAsync(o, in, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
-    };
+    auto precond = [&](const double* in, double* o) { precond_apply(P, n, in, o, s); };
     auto residual = [&] {  // V_0 = b - A x
         spmv64(x, V);
         NLHIP_CHECK(blas_dscal(n, -1.0, V, s));

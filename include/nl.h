@@ -200,7 +200,12 @@ NLAPI NLMatrix NLAPIENTRY nlMatrixFactorize(NLMatrix M, NLenum solver);
  * ====================================================================== */
 /* Jacobi: y = D^-1 x, with 1 where a row has no or a zero diagonal. */
 NLAPI NLMatrix NLAPIENTRY nlNewJacobiPreconditioner(NLMatrix M);
-/* SSOR has no GPU implementation here: returns NULL with an error. */
+/* SSOR(omega) preconditioner of a square matrix (matrix.cpp:366-368 passes
+ * omega = 1.5): y = omega (2 - omega) (D + omega U)^-1 D (D + omega L)^-1 x,
+ * two triangular sweeps run on the device level by level, bitwise the
+ * sequential sweeps.  The matrix is copied (later changes to M do not
+ * reach P); NULL with an error when a diagonal entry is zero or omega is 0.
+ * nlSolveSystemIterative accepts it for NL_CG, NL_BICGSTAB and NL_GMRES. */
 NLAPI NLMatrix NLAPIENTRY nlNewSSORPreconditioner(NLMatrix M, NLdouble omega);
 
 /* ========================================================================

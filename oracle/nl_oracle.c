@@ -17,6 +17,14 @@
  *   or_gmres                numpy solve on nonsymmetric systems, finite
  *                           termination with m = n, non-increasing true
  *                           residual over restarts, scipy's gmres solution
+ *   or_ssor                 dense omega (2 - omega) (D + omega U)^-1 D
+ *                           (D + omega L)^-1 x by scipy triangular solves,
+ *                           omega = 1 symmetric Gauss-Seidel by hand on a
+ *                           3x3 system, diagonal matrices (y = x / d);
+ *                           OpenNL's own operation order is unpinned
+ *                           (its source is absent)
+ *   or_solve_ssor           the SSOR-preconditioned CG / BiCGSTAB / GMRES
+ *                           against numpy solves
  *   or_dyn_*                duplicate accumulation, scipy COO->CSR
  *   or_ctx_*                dense A^T A from numpy, flat-grid LSCM closed form
  */
@@ -145,8 +153,24 @@ void or_jacobi(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
     }
 }
 
-static void precond(uint32_t n, const double* dinv, const double* r, double* h,
+/* A preconditioner as the solvers below apply it: Jacobi (dinv), SSOR
+ * (ssor != 0, the matrix and omega), or none (both unset). */
+typedef struct {
+    const double* dinv;
+    int ssor;
+    const uint32_t* rp;
+    const uint32_t* ci;
+    const double* a;
+    double omega;
+} prec_t;
+
+static void precond(const prec_t* P, uint32_t n, const double* r, double* h,
                     int par) {
+    if (P->ssor) {
+        or_ssor(n, P->rp, P->ci, P->a, P->omega, r, h);
+        return;
+    }
+    const double* dinv = P->dinv;
     if (dinv == NULL) {
         or_dcopy(n, r, h);
         return;
@@ -158,6 +182,47 @@ static void precond(uint32_t n, const double* dinv, const double* r, double* h,
     } else {
         for (uint32_t i = 0; i < n; ++i) h[i] = dinv[i] * r[i];
     }
+}
+
+/* SSOR (nlNewSSORPreconditioner(M, omega); omega = 1.5 at
+ * scene_graph/NL/matrix.cpp:366-368).  OpenNL's source is not in the
+ * reference tree; this is the published SSOR preconditioner written as
+ * OpenNL applies it (a forward sweep, a diagonal scaling, a backward sweep
+ * and a final scaling by 2 - omega):
+ *   w_i = (x_i - sum_{j<i} a_ij w_j) * omega / d_i      i = 0 .. n-1
+ *   w_i = w_i * (d_i / omega)
+ *   y_i = (w_i - sum_{j>i} a_ij y_j) * omega / d_i      i = n-1 .. 0
+ *   y_i = (2 - omega) * y_i
+ * i.e. y = omega (2 - omega) (D + omega U)^-1 D (D + omega L)^-1 x with
+ * A = L + D + U; each sum runs over row i's entries in CRS order, one
+ * rounding per product and per addition; d_i is the row's diagonal entry
+ * (the sum of its j == i entries). */
+void or_ssor(uint32_t n, const uint32_t* rp, const uint32_t* ci,
+             const double* a, double omega, const double* x, double* y) {
+    double* d = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    double* w = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    for (uint32_t i = 0; i < n; ++i) {
+        double di = 0.0;
+        for (uint32_t e = rp[i]; e < rp[i + 1]; ++e)
+            if (ci[e] == i) di = di + a[e];
+        d[i] = di;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        double S = 0.0;
+        for (uint32_t e = rp[i]; e < rp[i + 1]; ++e)
+            if (ci[e] < i) S = S + a[e] * w[ci[e]];
+        w[i] = (x[i] - S) * omega / d[i];
+    }
+    for (uint32_t i = 0; i < n; ++i) w[i] = w[i] * (d[i] / omega);
+    for (uint32_t k = n; k-- > 0;) {
+        double S = 0.0;
+        for (uint32_t e = rp[k]; e < rp[k + 1]; ++e)
+            if (ci[e] > k) S = S + a[e] * y[ci[e]];
+        y[k] = (w[k] - S) * omega / d[k];
+    }
+    for (uint32_t i = 0; i < n; ++i) y[i] = (2.0 - omega) * y[i];
+    free(d);
+    free(w);
 }
 
 /* ------------------------------------------------------------------- PCG */
@@ -207,7 +272,7 @@ static void vscal(variant_t v, uint32_t n, double a, double* x) {
  * x0 comes from the caller (the wrapper zeroes it, matrix.cpp:387).
  */
 static int pcg_core(uint32_t n, const uint32_t* rp, const uint32_t* ci,
-                    const double* a, const double* dinv, const double* b,
+                    const double* a, const prec_t* P, const double* b,
                     double* x, double eps, int max_iter, variant_t v,
                     int check, double* out_rel2) {
     double* r = (double*)malloc(sizeof(double) * (size_t)n);
@@ -216,7 +281,7 @@ static int pcg_core(uint32_t n, const uint32_t* rp, const uint32_t* ci,
     double* Ad = (double*)malloc(sizeof(double) * (size_t)n);
     vspmv(v, n, rp, ci, a, x, r);
     vaxpy(v, n, -1.0, b, r);
-    precond(n, dinv, r, d, v.blas_par);
+    precond(P, n, r, d, v.blas_par);
     or_dcopy(n, d, h);
     double rh = vdot(v, n, r, h);
     double err = vdot(v, n, r, r);
@@ -229,7 +294,7 @@ static int pcg_core(uint32_t n, const uint32_t* rp, const uint32_t* ci,
         double alpha = rh / dAd;
         vaxpy(v, n, -alpha, d, x);
         vaxpy(v, n, -alpha, Ad, r);
-        precond(n, dinv, r, h, v.blas_par);
+        precond(P, n, r, h, v.blas_par);
         double rh_new = vdot(v, n, r, h);
         double beta = rh_new / rh;
         rh = rh_new;
@@ -249,7 +314,8 @@ static int pcg_core(uint32_t n, const uint32_t* rp, const uint32_t* ci,
 int or_pcg(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
            const double* val, const double* dinv, const double* b, double* x,
            double eps, int max_iter, int variant, double* out_rel2) {
-    return pcg_core(n, rowptr, colind, val, dinv, b, x, eps, max_iter,
+    prec_t P = {dinv, 0, NULL, NULL, NULL, 0.0};
+    return pcg_core(n, rowptr, colind, val, &P, b, x, eps, max_iter,
                     decode(variant), 1, out_rel2);
 }
 
@@ -259,7 +325,8 @@ double or_pcg_timed_iters(uint32_t n, const uint32_t* rowptr,
                           int iters, int variant) {
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
-    pcg_core(n, rowptr, colind, val, dinv, b, x, 0.0, iters, decode(variant),
+    prec_t P = {dinv, 0, NULL, NULL, NULL, 0.0};
+    pcg_core(n, rowptr, colind, val, &P, b, x, 0.0, iters, decode(variant),
              0, NULL);
     clock_gettime(CLOCK_MONOTONIC, &t1);
     return (double)(t1.tv_sec - t0.tv_sec) +
@@ -268,9 +335,9 @@ double or_pcg_timed_iters(uint32_t n, const uint32_t* rowptr,
 
 /* -------------------------------------------------------------- BiCGSTAB */
 
-int or_bicgstab(uint32_t n, const uint32_t* rp, const uint32_t* ci,
-                const double* a, const double* dinv, const double* b,
-                double* x, double eps, int max_iter, double* out_rel2) {
+static int bicgstab_core(uint32_t n, const uint32_t* rp, const uint32_t* ci,
+                         const double* a, const prec_t* P, const double* b,
+                         double* x, double eps, int max_iter, double* out_rel2) {
     size_t sz = sizeof(double) * (size_t)n;
     double *r = malloc(sz), *rt = malloc(sz), *p = malloc(sz), *v = malloc(sz);
     double *ph = malloc(sz), *s = malloc(sz), *sh = malloc(sz), *t = malloc(sz);
@@ -294,7 +361,7 @@ int or_bicgstab(uint32_t n, const uint32_t* rp, const uint32_t* ci,
         or_daxpy(n, -omega, v, p);
         or_dscal(n, beta, p);
         or_daxpy(n, 1.0, r, p);
-        precond(n, dinv, p, ph, 0);
+        precond(P, n, p, ph, 0);
         or_spmv(n, rp, ci, a, ph, v);
         double rtv = or_ddot(n, rt, v);
         if (rtv == 0.0) break;
@@ -302,7 +369,7 @@ int or_bicgstab(uint32_t n, const uint32_t* rp, const uint32_t* ci,
         /* s = r - alpha v */
         or_dcopy(n, r, s);
         or_daxpy(n, -alpha, v, s);
-        precond(n, dinv, s, sh, 0);
+        precond(P, n, s, sh, 0);
         or_spmv(n, rp, ci, a, sh, t);
         double tt = or_ddot(n, t, t);
         or_daxpy(n, alpha, ph, x);
@@ -326,6 +393,13 @@ int or_bicgstab(uint32_t n, const uint32_t* rp, const uint32_t* ci,
     return its;
 }
 
+int or_bicgstab(uint32_t n, const uint32_t* rp, const uint32_t* ci,
+                const double* a, const double* dinv, const double* b,
+                double* x, double eps, int max_iter, double* out_rel2) {
+    prec_t P = {dinv, 0, NULL, NULL, NULL, 0.0};
+    return bicgstab_core(n, rp, ci, a, &P, b, x, eps, max_iter, out_rel2);
+}
+
 /* ---------------------------------------------------------------- GMRES */
 
 /* Restarted GMRES(m) (Saad & Schultz 1986; Saad, "Iterative Methods for
@@ -334,9 +408,9 @@ int or_bicgstab(uint32_t n, const uint32_t* rp, const uint32_t* ci,
  * dinv (x = D^-1 u, A D^-1 u = b).  Stops when the Arnoldi residual
  * estimate |g_{j+1}|^2 <= eps^2 b.b, after max_iter Arnoldi steps in all, or
  * on a breakdown; a restart recomputes the true residual b - A x. */
-int or_gmres(uint32_t n, const uint32_t* rp, const uint32_t* ci,
-             const double* a, const double* dinv, const double* b, double* x,
-             double eps, int max_iter, int m, double* out_rel2) {
+static int gmres_core(uint32_t n, const uint32_t* rp, const uint32_t* ci,
+                      const double* a, const prec_t* P, const double* b, double* x,
+                      double eps, int max_iter, int m, double* out_rel2) {
     if (m < 1) m = 1;
     size_t sz = sizeof(double) * (size_t)n;
     double* V = malloc(sz * (size_t)(m + 1));
@@ -361,7 +435,7 @@ int or_gmres(uint32_t n, const uint32_t* rp, const uint32_t* ci,
         int breakdown = 0;
         for (int j = 0; j < m && its < max_iter; ++j) {
             double* vj = V + (size_t)j * n;
-            precond(n, dinv, vj, z, 0);
+            precond(P, n, vj, z, 0);
             or_spmv(n, rp, ci, a, z, w);
             for (int i = 0; i <= j; ++i) {
                 double h = or_ddot(n, w, V + (size_t)i * n);
@@ -401,7 +475,7 @@ int or_gmres(uint32_t n, const uint32_t* rp, const uint32_t* ci,
         }
         memset(u, 0, sz);
         for (int i = 0; i < k; ++i) or_daxpy(n, y[i], V + (size_t)i * n, u);
-        precond(n, dinv, u, z, 0);
+        precond(P, n, u, z, 0);
         or_daxpy(n, 1.0, z, x);
         or_spmv(n, rp, ci, a, x, V);
         or_dscal(n, -1.0, V);
@@ -412,6 +486,23 @@ int or_gmres(uint32_t n, const uint32_t* rp, const uint32_t* ci,
     if (out_rel2) *out_rel2 = (bb > 0.0) ? err / bb : err;
     free(V); free(w); free(z); free(u); free(H); free(cs); free(sn); free(g); free(y);
     return its;
+}
+
+int or_gmres(uint32_t n, const uint32_t* rp, const uint32_t* ci,
+             const double* a, const double* dinv, const double* b, double* x,
+             double eps, int max_iter, int m, double* out_rel2) {
+    prec_t P = {dinv, 0, NULL, NULL, NULL, 0.0};
+    return gmres_core(n, rp, ci, a, &P, b, x, eps, max_iter, m, out_rel2);
+}
+
+int or_solve_ssor(int solver, uint32_t n, const uint32_t* rp, const uint32_t* ci,
+                  const double* a, double omega, const double* b, double* x,
+                  double eps, int max_iter, int m, double* out_rel2) {
+    prec_t P = {NULL, 1, rp, ci, a, omega};
+    if (solver == 0)
+        return pcg_core(n, rp, ci, a, &P, b, x, eps, max_iter, decode(0), 1, out_rel2);
+    if (solver == 1) return bicgstab_core(n, rp, ci, a, &P, b, x, eps, max_iter, out_rel2);
+    return gmres_core(n, rp, ci, a, &P, b, x, eps, max_iter, m, out_rel2);
 }
 
 /* -------------------------------------------------------- dynamic matrix */

@@ -80,6 +80,19 @@ int or_gmres(uint32_t n, const uint32_t* rowptr, const uint32_t* colind,
              const double* val, const double* dinv, const double* b, double* x,
              double eps, int max_iter, int m, double* out_rel2);
 
+/* ---- SSOR preconditioner (nlNewSSORPreconditioner, omega = 1.5 at
+ * matrix.cpp:366-368): y = omega (2 - omega) (D + omega U)^-1 D
+ * (D + omega L)^-1 x, as two triangular sweeps over the CRS rows (see
+ * nl_oracle.c).  The diagonal must be nonzero. */
+void or_ssor(uint32_t n, const uint32_t* rp, const uint32_t* ci,
+             const double* a, double omega, const double* x, double* y);
+
+/* CG (solver 0), BiCGSTAB (1) or GMRES(m) (2) as above, preconditioned by
+ * SSOR(omega) instead of Jacobi. */
+int or_solve_ssor(int solver, uint32_t n, const uint32_t* rp, const uint32_t* ci,
+                  const double* a, double omega, const double* b, double* x,
+                  double eps, int max_iter, int m, double* out_rel2);
+
 /* ---- Dynamic matrix: duplicate (i,j) entries accumulate
  * (scene_graph/NL/matrix.h:111-112,122-123), then compression to CRS with
  * each row's columns in increasing order. */

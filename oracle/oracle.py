@@ -54,6 +54,10 @@ def lib() -> C.CDLL:
                                       _f64p, _f64p, C.c_double, C.c_int, _f64p]),
             "or_gmres": (C.c_int, [C.c_uint32, _u32p, _u32p, _f64p, _f64p,
                                    _f64p, _f64p, C.c_double, C.c_int, C.c_int, _f64p]),
+            "or_ssor": (None, [C.c_uint32, _u32p, _u32p, _f64p, C.c_double, _f64p, _f64p]),
+            "or_solve_ssor": (C.c_int, [C.c_int, C.c_uint32, _u32p, _u32p, _f64p,
+                                        C.c_double, _f64p, _f64p, C.c_double, C.c_int,
+                                        C.c_int, _f64p]),
             "or_dyn_new": (C.c_void_p, [C.c_uint32, C.c_uint32]),
             "or_dyn_free": (None, [C.c_void_p]),
             "or_dyn_add": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_double]),
@@ -204,6 +208,31 @@ def gmres(A: CRS, b, x0=None, eps=1e-10, max_iter=5000, m=5, precond=None):
     it = lib().or_gmres(*A.args(), None if dinv is None else _p(dinv, C.c_double),
                         _p(b, C.c_double), _p(x, C.c_double), float(eps),
                         int(max_iter), int(m), C.byref(rel2))
+    return x, int(it), float(np.sqrt(rel2.value))
+
+
+def ssor(A: CRS, x, omega: float = 1.5) -> np.ndarray:
+    """y = SSOR(omega)^-1 x: forward sweep, diagonal scaling, backward sweep,
+    times (2 - omega)."""
+    x = _f64(x)
+    y = np.zeros(A.m)
+    lib().or_ssor(*A.args(), float(omega), _p(x, C.c_double), _p(y, C.c_double))
+    return y
+
+
+SOLVERS = {"cg": 0, "bicgstab": 1, "gmres": 2}
+
+
+def solve_ssor(A: CRS, b, solver: str = "cg", x0=None, eps=1e-10, max_iter=5000, m=5,
+               omega: float = 1.5):
+    """CG / BiCGSTAB / GMRES(m) preconditioned by SSOR(omega).
+    Returns (x, iterations, relative residual)."""
+    b = _f64(b)
+    x = np.zeros(A.m) if x0 is None else _f64(x0).copy()
+    rel2 = C.c_double(0.0)
+    it = lib().or_solve_ssor(SOLVERS[solver], *A.args(), float(omega), _p(b, C.c_double),
+                             _p(x, C.c_double), float(eps), int(max_iter), int(m),
+                             C.byref(rel2))
     return x, int(it), float(np.sqrt(rel2.value))
 
 

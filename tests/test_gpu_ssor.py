@@ -1,0 +1,160 @@
+"""SSOR preconditioner on the device (nlNewSSORPreconditioner; the wrapper's
+solve_iterative(..., "SSOR") passes omega = 1.5, scene_graph/NL/matrix.cpp:
+366-368): one application is bitwise the oracle's sequential sweeps, on
+patterns whose level schedules take every launch form (long chains of
+one-row levels, wide levels spread over the grid, mixed); CG, BiCGSTAB and
+GMRES preconditioned by it agree with the oracle's; bad input is refused."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from graphitethree_amd import capi, nl
+from graphitethree_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+REL_DIFF = 1e-8
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def same_iters(a, b) -> bool:
+    a, b = int(a), int(b)
+    return abs(a - b) <= max(2, 0.05 * max(a, b))
+
+
+def tridiag(n):
+    return sp.diags([-np.ones(n - 1), 2.0 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1]).tocsr()
+
+
+def laplacian2d(k, shift=1e-3):
+    T = tridiag(k)
+    return (sp.kron(sp.eye(k), T) + sp.kron(T, sp.eye(k)) + shift * sp.eye(k * k)).tocsr()
+
+
+def nonsymmetric(n, seed):
+    rng = np.random.default_rng(seed)
+    A = sp.random(n, n, density=6.0 / n, random_state=seed, format="csr")
+    return (A + sp.diags(4.0 + rng.random(n))).tocsr()
+
+
+def patterns():
+    rng = np.random.default_rng(3)
+    blocks = sp.kron(sp.eye(3000), sp.csr_matrix(rng.random((3, 3)) + 3 * np.eye(3)))
+    Sw = W.lscm_grid(16, seed=2)
+    lscm = sp.csr_matrix((Sw.val, Sw.colind, Sw.rowptr), shape=(Sw.n, Sw.n))
+    return {
+        "chain": tridiag(3000),                # 3000 one-row levels per sweep
+        "wide": blocks.tocsr(),                # 3 levels of 3000 rows
+        "laplacian": laplacian2d(60),          # levels growing then shrinking
+        "lscm": lscm,
+        "nonsymmetric": nonsymmetric(2000, 5),
+    }
+
+
+def nl_matrix(L, A):
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    rp, ci, va = A.indptr.astype(np.uint32), A.indices.astype(np.uint32), A.data.copy()
+    M = L.nlCRSMatrixNewFromArrays(A.shape[0], A.shape[0], capi.ptr(rp), capi.ptr(ci),
+                                   capi.ptr(va))
+    return M, O.CRS(rp, ci, va)
+
+
+@pytest.mark.parametrize("omega", [1.5, 1.0, 0.8])
+def test_ssor_application_is_bitwise_the_oracle(hip, omega):
+    L = hip
+    rng = np.random.default_rng(9)
+    for name, A in patterns().items():
+        M, C = nl_matrix(L, A)
+        P = L.nlNewSSORPreconditioner(M, omega)
+        assert P, name
+        for _ in range(2):
+            x = rng.standard_normal(C.m)
+            y = np.zeros(C.m)
+            L.nlMultMatrixVector(P, capi.ptr(x), capi.ptr(y))
+            assert np.array_equal(bits(y), bits(O.ssor(C, x, omega))), (name, omega)
+        L.nlDeleteMatrix(P)
+        L.nlDeleteMatrix(M)
+
+
+def test_ssor_preconditioned_cg_matches_oracle(hip):
+    L = hip
+    for A in (laplacian2d(50), patterns()["lscm"]):
+        M, C = nl_matrix(L, A)
+        P = L.nlNewSSORPreconditioner(M, 1.5)
+        b = np.random.default_rng(1).standard_normal(C.m)
+        x = np.zeros(C.m)
+        its = L.nlSolveSystemIterative(L.nlHostBlas(), M, P, capi.ptr(b), capi.ptr(x),
+                                       capi.NL_CG, 1e-10, 5000, 5)
+        xr, itr, relr = O.solve_ssor(C, b, "cg", eps=1e-10, max_iter=5000)
+        assert relr <= 1e-10 and same_iters(its, itr), (its, itr)
+        assert rel(x, xr) <= REL_DIFF
+        # fewer iterations than Jacobi-CG on the same system
+        _, itj, _ = O.pcg(C, b, eps=1e-10, max_iter=20000)
+        assert its < itj
+        L.nlDeleteMatrix(P)
+        L.nlDeleteMatrix(M)
+
+
+def test_ssor_preconditioned_bicgstab_and_gmres_match_oracle(hip):
+    L = hip
+    A = nonsymmetric(600, 21)
+    M, C = nl_matrix(L, A)
+    P = L.nlNewSSORPreconditioner(M, 1.5)
+    b = np.random.default_rng(2).standard_normal(C.m)
+    for solver, name in ((capi.NL_BICGSTAB, "bicgstab"), (capi.NL_GMRES, "gmres")):
+        x = np.zeros(C.m)
+        its = L.nlSolveSystemIterative(L.nlHostBlas(), M, P, capi.ptr(b), capi.ptr(x),
+                                       solver, 1e-11, 3000, 5)
+        xr, itr, _ = O.solve_ssor(C, b, name, eps=1e-11, max_iter=3000, m=5)
+        assert its > 0 and same_iters(its, itr), (name, its, itr)
+        assert np.linalg.norm(A @ x - b) <= 1.01e-11 * np.linalg.norm(b), name
+        assert rel(x, xr) <= REL_DIFF, name
+    L.nlDeleteMatrix(P)
+    L.nlDeleteMatrix(M)
+
+
+def test_wrapper_solve_iterative_with_ssor(hip):
+    A = laplacian2d(20)
+    A.sort_indices()
+    NL = nl.Library()
+    n = A.shape[0]
+    Mw = NL.create_matrix(n, n)
+    coo = A.tocoo()
+    for i, j, v in zip(coo.row, coo.col, coo.data):
+        Mw.add_coefficient(int(i), int(j), float(v))
+    B = NL.create_vector(n)
+    rhs = np.random.default_rng(4).standard_normal(n)
+    for i in range(n):
+        B[i] = rhs[i]
+    X = NL.create_vector(n)
+    Mw.solve_iterative(B, X, "CG", "SSOR", 2000, 1e-10)
+    x = np.array(X.data, dtype=np.float64)
+    assert rel(x, np.linalg.solve(A.toarray(), rhs)) <= 1e-8
+
+
+def test_ssor_rejects_bad_input(hip):
+    L = hip
+    A = tridiag(10).tolil()
+    A[4, 4] = 0.0
+    M, _ = nl_matrix(L, A.tocsr())
+    assert not L.nlNewSSORPreconditioner(M, 1.5)      # a zero diagonal entry
+    L.nlDeleteMatrix(M)
+    M, _ = nl_matrix(L, tridiag(10))
+    assert not L.nlNewSSORPreconditioner(M, 0.0)      # omega = 0
+    P = L.nlNewSSORPreconditioner(M, 1.5)
+    M2, _ = nl_matrix(L, tridiag(12))
+    b, x = np.ones(12), np.zeros(12)
+    L.nlSolveSystemIterative(L.nlHostBlas(), M2, P, capi.ptr(b), capi.ptr(x), capi.NL_CG,
+                             1e-10, 100, 5)
+    assert np.all(x == 0.0)                           # size mismatch: nothing solved
+    L.nlDeleteMatrix(P)
+    L.nlDeleteMatrix(M)
+    L.nlDeleteMatrix(M2)

@@ -338,3 +338,80 @@ def test_context_square_system():
     it, rel = ctx.solve(eps=1e-14, max_iter=100)
     x = np.array([ctx.get_variable(i) for i in range(3)])
     np.testing.assert_allclose(x, np.linalg.solve(M, b), rtol=1e-12)
+
+
+# ---- SSOR (nlNewSSORPreconditioner, omega = 1.5 at matrix.cpp:366-368)
+
+def _ssor_dense(M, x, omega):
+    """omega (2 - omega) (D + omega U)^-1 D (D + omega L)^-1 x by dense
+    triangular solves (another route to the same operator)."""
+    import scipy.linalg as sla
+    M = np.asarray(M, dtype=np.float64)
+    D = np.diag(np.diag(M))
+    L, U = np.tril(M, -1), np.triu(M, 1)
+    t = sla.solve_triangular(D + omega * L, x, lower=True)
+    return omega * (2.0 - omega) * sla.solve_triangular(D + omega * U, D @ t, lower=False)
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.5, 0.7])
+def test_ssor_equals_dense_triangular_solves(omega):
+    rng = np.random.default_rng(11)
+    for A in (_nonsymmetric(90, 5), (tridiag(70) + sp.diags(rng.random(70))).tocsr()):
+        A.sort_indices()
+        x = rng.standard_normal(A.shape[0])
+        y = O.ssor(O.CRS(A.indptr, A.indices, A.data), x, omega)
+        ref = _ssor_dense(A.toarray(), x, omega)
+        np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-13 * np.abs(ref).max())
+
+
+def test_ssor_small_cases_by_hand():
+    # diagonal matrix: y = omega (2 - omega) x / d
+    d = np.array([2.0, 4.0, 8.0])
+    x = np.array([1.0, -3.0, 5.0])
+    y = O.ssor(crs_from_dense(np.diag(d)), x, 1.5)
+    np.testing.assert_allclose(y, 0.75 * x / d, rtol=1e-15)
+    # omega = 1: symmetric Gauss-Seidel, M = (D + L) D^-1 (D + U); exact
+    # rationals for A = [[4,1,0],[1,4,1],[0,1,4]] and x = (1, 2, 3)
+    from fractions import Fraction as F
+    A = [[4, 1, 0], [1, 4, 1], [0, 1, 4]]
+    xs = [F(1), F(2), F(3)]
+    w = [F(0)] * 3
+    for i in range(3):          # (D + L) w = x
+        w[i] = (xs[i] - sum(A[i][j] * w[j] for j in range(i))) / A[i][i]
+    z = [A[i][i] * w[i] for i in range(3)]
+    y_ex = [F(0)] * 3
+    for i in reversed(range(3)):  # (D + U) y = D w
+        y_ex[i] = (z[i] - sum(A[i][j] * y_ex[j] for j in range(i + 1, 3))) / A[i][i]
+    y = O.ssor(crs_from_dense(A), np.array([1.0, 2.0, 3.0]), 1.0)
+    np.testing.assert_allclose(y, [float(v) for v in y_ex], rtol=1e-15)
+    # and M y = x for the same M
+    Ad = np.array(A, dtype=float)
+    Msgs = (np.diag(np.diag(Ad)) + np.tril(Ad, -1)) @ np.diag(1 / np.diag(Ad)) @ \
+        (np.diag(np.diag(Ad)) + np.triu(Ad, 1))
+    np.testing.assert_allclose(Msgs @ y, [1.0, 2.0, 3.0], rtol=1e-14)
+
+
+def test_ssor_preconditioned_solvers():
+    # SPD: 2D Laplacian plus a small shift; SSOR-CG matches the direct solve
+    # and needs fewer iterations than Jacobi-CG
+    k = 30
+    T = tridiag(k)
+    A = (sp.kron(sp.eye(k), T) + sp.kron(T, sp.eye(k)) + 1e-3 * sp.eye(k * k)).tocsr()
+    A.sort_indices()
+    C = O.CRS(A.indptr, A.indices, A.data)
+    b = np.random.default_rng(2).standard_normal(k * k)
+    x_ref = np.linalg.solve(A.toarray(), b)
+    x, it_ssor, rel = O.solve_ssor(C, b, "cg", eps=1e-10, max_iter=5000)
+    assert rel <= 1e-10
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-7
+    _, it_jac, _ = O.pcg(C, b, eps=1e-10, max_iter=5000)
+    assert it_ssor < 0.7 * it_jac, (it_ssor, it_jac)
+    # nonsymmetric: BiCGSTAB and GMRES(5) with SSOR
+    N = _nonsymmetric(150, 8)
+    CN = O.CRS(N.indptr, N.indices, N.data)
+    bn = np.random.default_rng(3).standard_normal(150)
+    xn_ref = np.linalg.solve(N.toarray(), bn)
+    for solver in ("bicgstab", "gmres"):
+        xs, it, rel = O.solve_ssor(CN, bn, solver, eps=1e-11, max_iter=3000)
+        assert rel <= 1e-10, solver
+        assert np.linalg.norm(xs - xn_ref) / np.linalg.norm(xn_ref) <= 1e-8, solver
