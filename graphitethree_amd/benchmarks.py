@@ -305,6 +305,64 @@ def run_serving(args) -> dict:
     }
 
 
+def run_ssor(args) -> dict:
+    """SSOR-preconditioned CG through nlSolveSystemIterative (the wrapper's
+    solve_iterative(..., "CG", "SSOR")) on a cotangent smoothing system,
+    against Jacobi-CG on the device and the oracle's SSOR-CG on one host
+    core."""
+    import scipy.sparse as sp
+    from oracle import oracle as O
+    dev = init_device()
+    L = capi.lib()
+    mesh = W.grid_mesh(args.grid, seed=args.seed)
+    Sw = W.cotan_system(mesh, t=args.t, nrhs=1)
+    del mesh
+    n = Sw.n
+    log(f"ssor: cotan {args.grid}^2 t={args.t:g}, n={n}, nnz={Sw.nnz}")
+    M = L.nlCRSMatrixNewFromArrays(n, n, capi.ptr(Sw.rowptr), capi.ptr(Sw.colind),
+                                   capi.ptr(Sw.val))
+    b = Sw.b.copy()
+    t0 = time.perf_counter()
+    P = L.nlNewSSORPreconditioner(M, 1.5)
+    t_create = time.perf_counter() - t0
+    if not P:
+        raise RuntimeError("nlNewSSORPreconditioner failed")
+    rows = {}
+    for name, prec in (("ssor", P), ("jacobi", None)):
+        if name == "jacobi":
+            prec = L.nlNewJacobiPreconditioner(M)
+        for rep in range(2):
+            x = np.zeros(n)
+            t0 = time.perf_counter()
+            its = L.nlSolveSystemIterative(L.nlHostBlas(), M, prec, capi.ptr(b), capi.ptr(x),
+                                           capi.NL_CG, args.eps, args.max_iter, 5)
+            dt = time.perf_counter() - t0
+        A = sp.csr_matrix((Sw.val, Sw.colind, Sw.rowptr), shape=(n, n))
+        rr = float(np.linalg.norm(A @ x - b) / np.linalg.norm(b))
+        rows[name] = {"seconds": dt, "iterations": int(its), "rel_residual": rr,
+                      "ms_per_iteration": 1e3 * dt / max(int(its), 1)}
+        log(f"{name}-CG: {its} iterations, {dt:.3f}s, residual {rr:.2e}")
+        if name == "jacobi":
+            L.nlDeleteMatrix(prec)
+    L.nlDeleteMatrix(P)
+    L.nlDeleteMatrix(M)
+    out = {"metric": "SSOR-preconditioned CG solve time (nlSolveSystemIterative)",
+           "workload": f"cotan_{args.grid}_t{args.t:g}", "n": n, "nnz": Sw.nnz,
+           "device": dev[0], "eps": args.eps, "omega": 1.5,
+           "ssor_create_seconds": t_create, "ssor_cg": rows["ssor"],
+           "jacobi_cg_device": rows["jacobi"]}
+    if not args.no_cpu:
+        C = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+        t0 = time.perf_counter()
+        xc, itc, relc = O.solve_ssor(C, b, "cg", eps=args.eps, max_iter=args.max_iter)
+        dt = time.perf_counter() - t0
+        out["cpu_ssor_cg"] = {"seconds": dt, "iterations": itc, "rel_residual": relc,
+                              "cores": 1, "kind": "port"}
+        out["speedup_vs_cpu_ssor_cg"] = dt / rows["ssor"]["seconds"]
+        log(f"cpu ssor-CG: {itc} iterations, {dt:.3f}s")
+    return out
+
+
 def run_eigen(args) -> dict:
     """k smallest eigenpairs of a mesh Laplacian with lumped mass (manifold
     harmonics): the device solver against scipy's shift-invert ARPACK with a
@@ -376,6 +434,13 @@ def main(argv=None):
     ev.add_argument("--eps", type=float, default=1e-8)
     ev.add_argument("--seed", type=int, default=1)
     ev.add_argument("--no-cpu", action="store_true")
+    so = sub.add_parser("ssor")
+    so.add_argument("--grid", type=int, default=300)
+    so.add_argument("--t", type=float, default=1.0)
+    so.add_argument("--seed", type=int, default=1)
+    so.add_argument("--eps", type=float, default=1e-10)
+    so.add_argument("--max-iter", type=int, default=5000)
+    so.add_argument("--no-cpu", action="store_true")
     sv = sub.add_parser("serving")
     sv.add_argument("--count", type=int, default=120)
     sv.add_argument("--rate", type=float, default=10.0)
@@ -395,6 +460,8 @@ def main(argv=None):
         out = run_multirhs(args)
     elif args.cmd == "eigen":
         out = run_eigen(args)
+    elif args.cmd == "ssor":
+        out = run_ssor(args)
     print(json.dumps(out), flush=True)
 
 

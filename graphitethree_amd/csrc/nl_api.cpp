@@ -847,7 +847,10 @@ bool solve_context(Context& C) {
     const NLuint max_iter = C.max_iterations ? C.max_iterations : 5 * n;
     std::vector<double> b(n), x(n);
     bool ok = true;
-    if (solver == NL_CG && C.nb_systems > 1 && C.nb_systems <= 32) {
+    // NL_PRECOND_SSOR: SSOR(NL_OMEGA) of the assembled matrix
+    std::unique_ptr<Ssor, void (*)(Ssor*)> ssor(nullptr, ssor_destroy);
+    if (C.preconditioner == NL_PRECOND_SSOR) ssor.reset(ssor_create(*S, C.omega));
+    if (solver == NL_CG && C.nb_systems > 1 && C.nb_systems <= 32 && !ssor) {
         // NL_NB_SYSTEMS right-hand sides share every pass over the matrix
         const NLuint k = C.nb_systems;
         std::vector<double> X(size_t(n) * k);
@@ -879,7 +882,21 @@ bool solve_context(Context& C) {
         for (NLuint v = 0; v < C.nb_variables; ++v)
             if (C.free_index[v] != ~NLuint(0)) x[C.free_index[v]] = vals[v];
         std::copy(C.rhs.begin() + size_t(k) * n, C.rhs.begin() + size_t(k + 1) * n, b.begin());
-        if (solver == NL_CG) {
+        if (solver == NL_CG && ssor) {
+            std::lock_guard<std::mutex> lock(S->mu);
+            S->ws64.ensure(n, S->grid);
+            hipStream_t s = S->stream.get();
+            copy_in(S->ws64.b.get(), b.data(), size_t(n) * 8, s);
+            copy_in(S->ws64.x.get(), x.data(), size_t(n) * 8, s);
+            double rel2 = 0.0;
+            Precond pc;
+            pc.ssor = ssor.get();
+            C.used_iterations += pcg_device(*S, pc, S->ws64.b.get(), S->ws64.x.get(), C.threshold,
+                                            max_iter, &rel2);
+            copy_out(x.data(), S->ws64.x.get(), size_t(n) * 8, s);
+            S->stream.sync();
+            C.error = std::sqrt(rel2);
+        } else if (solver == NL_CG) {
             nlhip_solve_params p;
             nlhipSolveParamsDefault(&p);
             p.eps = C.threshold;
@@ -898,6 +915,7 @@ bool solve_context(Context& C) {
             double rel2 = 0.0;
             Precond pc;
             pc.dinv = (C.preconditioner == NL_PRECOND_JACOBI) ? S->dinv.get() : nullptr;
+            pc.ssor = ssor.get();
             C.used_iterations += bicgstab_device(*S, pc, S->ws64.b.get(), S->ws64.x.get(),
                                                  C.threshold, max_iter, &rel2);
             copy_out(x.data(), S->ws64.x.get(), size_t(n) * 8, s);
@@ -912,6 +930,7 @@ bool solve_context(Context& C) {
             double rel2 = 0.0;
             Precond pc;
             pc.dinv = (C.preconditioner == NL_PRECOND_JACOBI) ? S->dinv.get() : nullptr;
+            pc.ssor = ssor.get();
             C.used_iterations += gmres_device(*S, pc, S->ws64.b.get(), S->ws64.x.get(),
                                               C.threshold, max_iter,
                                               C.inner_iterations ? C.inner_iterations : 5, &rel2);

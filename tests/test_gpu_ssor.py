@@ -158,3 +158,54 @@ def test_ssor_rejects_bad_input(hip):
     L.nlDeleteMatrix(P)
     L.nlDeleteMatrix(M)
     L.nlDeleteMatrix(M2)
+
+
+def test_context_api_with_ssor(hip):
+    """nlSolverParameteri(NL_PRECONDITIONER, NL_PRECOND_SSOR) with NL_OMEGA:
+    the LSCM call sequence of mesh_compute_LSCM through nlBegin/nlEnd(NL_ROW),
+    against the oracle's SSOR-CG on the oracle context's normal equations."""
+    L = hip
+    mesh = W.chart_mesh(300, seed=4)
+    pins = mesh.extreme_pins()
+    uv = [(mesh.points[p, 0], mesh.points[p, 1]) for p in pins]
+    coeffs, ok = mesh.lscm_rows()
+    ctx = O.Context(2 * mesh.nv, least_squares=True)
+    L.nlNewContext()
+    L.nlSolverParameteri(capi.NL_NB_VARIABLES, 2 * mesh.nv)
+    L.nlSolverParameteri(capi.NL_LEAST_SQUARES, capi.NL_TRUE)
+    L.nlSolverParameteri(capi.NL_SOLVER, capi.NL_CG)
+    L.nlSolverParameteri(capi.NL_PRECONDITIONER, capi.NL_PRECOND_SSOR)
+    L.nlSolverParameterd(capi.NL_OMEGA, 1.5)
+    L.nlSolverParameterd(capi.NL_THRESHOLD, 1e-10)
+    L.nlSolverParameteri(capi.NL_MAX_ITERATIONS, 5000)
+    L.nlBegin(capi.NL_SYSTEM)
+    for p, (u, v) in zip(pins, uv):
+        for k, val in ((0, u), (1, v)):
+            L.nlSetVariable(2 * p + k, val)
+            L.nlLockVariable(2 * p + k)
+            ctx.set_variable(2 * p + k, val)
+            ctx.lock_variable(2 * p + k)
+    L.nlBegin(capi.NL_MATRIX)
+    ctx.begin_matrix()
+    for t in range(mesh.nt):
+        if not ok[t]:
+            continue
+        tri = mesh.triangles[t]
+        var = [2 * int(tri[e // 2]) + (e & 1) for e in range(6)]
+        for row in (coeffs[t, :6], coeffs[t, 6:]):
+            L.nlBegin(capi.NL_ROW)
+            for v_, a in zip(var, row):
+                L.nlCoefficient(v_, float(a))
+            L.nlEnd(capi.NL_ROW)
+            ctx.row(list(zip(var, row)))
+    L.nlEnd(capi.NL_MATRIX)
+    L.nlEnd(capi.NL_SYSTEM)
+    assert L.nlSolve() == capi.NL_TRUE
+    gpu = np.array([L.nlGetVariable(i) for i in range(2 * mesh.nv)])
+    L.nlDeleteContext(L.nlGetCurrent())
+    A, b = ctx.system()
+    xr, itr, relr = O.solve_ssor(A, b, "cg", eps=1e-10, max_iter=5000)
+    free = [i for i in range(2 * mesh.nv) if i // 2 not in pins]
+    assert relr <= 1e-10
+    assert rel(gpu[free], xr) <= REL_DIFF
+    assert gpu[2 * pins[0]] == uv[0][0]
