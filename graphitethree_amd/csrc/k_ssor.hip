@@ -9,9 +9,10 @@
 // (a row's level is one more than the deepest row it reads), so the rows of
 // one level are independent: a level with many rows is one launch over the
 // whole grid, and a run of consecutive small levels is one workgroup that
-// steps through them with a barrier between levels.  Every row sums its own
-// entries in CRS order with one rounding per product and per addition, so
-// the result is that of the sequential sweep, bit for bit.
+// steps through them with a barrier between levels.  Each sweep keeps, in
+// level order, the entries each row reads, packed in CRS order.  Every row
+// sums them in that order with one rounding per product and per addition,
+// so the result is that of the sequential sweep, bit for bit.
 #include "dev_common.h"
 #include "nlhip_internal.h"
 
@@ -36,30 +37,48 @@ __global__ void k_ssor_diag(uint32_t n, const uint32_t* rp, const uint32_t* ci, 
     if (flag) atomicOr(bad, 1u);
 }
 
+constexpr int kSsorU = 8;  // packed entries loaded (and gathers issued) at a time
+
+// The row at position k of a sweep: its packed entries come in groups of
+// kSsorU (all loads of a group, then all gathers, then the additions in
+// order), so a row costs one round trip per group instead of one per entry.
 template <bool UPPER>
-__device__ __forceinline__ void ssor_row(const SsorArgs& a, uint32_t i) {
-    double S = 0.0;
+__device__ __forceinline__ void ssor_row(const SsorArgs& a, const SsorSweepArgs& sw, uint32_t k) {
+    const uint32_t i = sw.rows[k];
+    const uint32_t e0 = sw.eoff[k], e1 = sw.eoff[k + 1];
     const double* v = UPPER ? a.y : a.w;
-    for (uint32_t e = a.rp[i]; e < a.rp[i + 1]; ++e) {
-        const uint32_t j = a.ci[e];
-        if (UPPER ? j > i : j < i) S = add_rn(S, mul_rn(a.val[e], v[j]));
-    }
     const double xi = UPPER ? mul_rn(a.w[i], a.dscale[i]) : a.x[i];
+    const double di = a.diag[i];
+    double S = 0.0;
+    for (uint32_t e = e0; e < e1; e += kSsorU) {
+        uint32_t c[kSsorU];
+        double av[kSsorU], g[kSsorU];
+#pragma unroll
+        for (int u = 0; u < kSsorU; ++u) {
+            const uint32_t q = e + u < e1 ? e + u : e1 - 1u;
+            c[u] = sw.col[q];
+            av[u] = sw.val[q];
+        }
+#pragma unroll
+        for (int u = 0; u < kSsorU; ++u) g[u] = v[c[u]];
+#pragma unroll
+        for (int u = 0; u < kSsorU; ++u)
+            if (e + uint32_t(u) < e1) S = add_rn(S, mul_rn(av[u], g[u]));
+    }
     const double t = mul_rn(add_rn(xi, -S), a.omega);
-    (UPPER ? a.y : a.w)[i] = t / a.diag[i];
+    (UPPER ? a.y : a.w)[i] = t / di;
 }
 
 // Levels [l0, l1): one level over the whole grid, or (one workgroup) several
 // levels in order with a barrier after each.
 template <bool UPPER>
 __global__ __launch_bounds__(kBlock) void k_ssor_sweep(SsorArgs a, uint32_t l0, uint32_t l1) {
-    const uint32_t* rows = UPPER ? a.rows_up : a.rows_lo;
-    const uint32_t* off = UPPER ? a.off_up : a.off_lo;
+    const SsorSweepArgs& sw = UPPER ? a.up : a.lo;
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t l = l0; l < l1; ++l) {
-        const uint32_t b = off[l], e = off[l + 1];
+        const uint32_t b = sw.off[l], e = sw.off[l + 1];
         for (uint32_t k = b + blockIdx.x * kBlock + threadIdx.x; k < e; k += stride)
-            ssor_row<UPPER>(a, rows[k]);
+            ssor_row<UPPER>(a, sw, k);
         if (l + 1 < l1) __syncthreads();
     }
 }

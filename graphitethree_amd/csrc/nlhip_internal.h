@@ -497,23 +497,24 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X,
 
 // ------------------------------------------------------------------ SSOR
 
-// Arguments of the SSOR sweeps (k_ssor.hip): the matrix, d_i and d_i / omega,
-// the rows of each sweep sorted by level with the level offsets, and the
-// vectors x (input), w (forward sweep) and y (output).
-struct SsorArgs {
-    const uint32_t* rp;
-    const uint32_t* ci;
+// Arguments of the SSOR sweeps (k_ssor.hip).  Each sweep lists its rows by
+// level (positions), and for each position the entries that sweep reads
+// (j < i forward, j > i backward) packed in CRS order.
+struct SsorSweepArgs {
+    const uint32_t* rows;  // row at each position
+    const uint32_t* off;   // first position of each level (levels + 1)
+    const uint32_t* eoff;  // first packed entry of each position (n + 1)
+    const uint32_t* col;
     const double* val;
-    const double* diag;
-    const double* dscale;
-    const uint32_t* rows_lo;
-    const uint32_t* off_lo;
-    const uint32_t* rows_up;
-    const uint32_t* off_up;
-    const double* x;
-    double* w;
-    double* y;
-    double omega, post;  // post = 2 - omega
+};
+struct SsorArgs {
+    SsorSweepArgs lo, up;
+    const double* diag;    // d_i
+    const double* dscale;  // d_i / omega
+    const double* x;       // input
+    double* w;             // forward sweep
+    double* y;             // output
+    double omega, post;    // post = 2 - omega
     uint32_t n;
 };
 hipError_t ssor_diag(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* val,

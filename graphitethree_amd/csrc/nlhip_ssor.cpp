@@ -42,6 +42,15 @@ void level_schedule(uint32_t n, const std::vector<uint32_t>& rp, const std::vect
 
 }  // namespace
 
+// One sweep on the device: rows by level and their packed entries.
+struct SweepBufs {
+    DevBuf<uint32_t> rows, off, eoff, col;
+    DevBuf<double> val;
+    SsorSweepArgs args() const {
+        return SsorSweepArgs{rows.get(), off.get(), eoff.get(), col.get(), val.get()};
+    }
+};
+
 class Ssor {
   public:
     struct Step {
@@ -49,8 +58,8 @@ class Ssor {
     };
     uint32_t n = 0;
     double omega = 1.5;
-    DevBuf<uint32_t> rp, ci, rows_lo, off_lo, rows_up, off_up;
-    DevBuf<double> val, diag, dscale, x, w, y;
+    SweepBufs lo, up;
+    DevBuf<double> diag, dscale, x, w, y;
     std::vector<Step> lo_steps, up_steps;
     Stream stream;
     Graph graph;
@@ -58,15 +67,10 @@ class Ssor {
 
     SsorArgs args() const {
         SsorArgs a{};
-        a.rp = rp.get();
-        a.ci = ci.get();
-        a.val = val.get();
+        a.lo = lo.args();
+        a.up = up.args();
         a.diag = diag.get();
         a.dscale = dscale.get();
-        a.rows_lo = rows_lo.get();
-        a.off_lo = off_lo.get();
-        a.rows_up = rows_up.get();
-        a.off_up = off_up.get();
         a.x = x.get();
         a.w = w.get();
         a.y = y.get();
@@ -102,6 +106,44 @@ std::vector<Ssor::Step> plan(const std::vector<uint32_t>& off) {
 
 }  // namespace
 
+namespace {
+
+// Level schedule of one sweep, its positions' packed entries, and the
+// launch plan; uploaded on stream s.
+std::vector<Ssor::Step> build_sweep(uint32_t n, const std::vector<uint32_t>& rp,
+                                    const std::vector<uint32_t>& ci, const std::vector<double>& va,
+                                    bool upper, SweepBufs& B, hipStream_t s) {
+    std::vector<uint32_t> rows, off;
+    level_schedule(n, rp, ci, upper, rows, off);
+    std::vector<uint32_t> eoff(size_t(n) + 1, 0), col;
+    std::vector<double> val;
+    col.reserve(ci.size() / 2 + 1);
+    val.reserve(ci.size() / 2 + 1);
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t i = rows[k];
+        for (uint32_t e = rp[i]; e < rp[i + 1]; ++e)
+            if (upper ? ci[e] > i : ci[e] < i) {
+                col.push_back(ci[e]);
+                val.push_back(va[e]);
+            }
+        eoff[k + 1] = uint32_t(col.size());
+    }
+    B.rows.alloc(std::max<size_t>(rows.size(), 1));
+    B.off.alloc(off.size());
+    B.eoff.alloc(eoff.size());
+    B.col.alloc(std::max<size_t>(col.size(), 1));
+    B.val.alloc(std::max<size_t>(val.size(), 1));
+    copy_in(B.rows.get(), rows.data(), rows.size() * 4, s);
+    copy_in(B.off.get(), off.data(), off.size() * 4, s);
+    copy_in(B.eoff.get(), eoff.data(), eoff.size() * 4, s);
+    copy_in(B.col.get(), col.data(), col.size() * 4, s);
+    copy_in(B.val.get(), val.data(), val.size() * 8, s);
+    NLHIP_CHECK(hipStreamSynchronize(s));  // the host arrays go out of scope
+    return plan(off);
+}
+
+}  // namespace
+
 Ssor* ssor_create(System& S, double omega) {
     if (!std::isfinite(omega) || omega == 0.0)
         throw Error(NLHIP_ERR_INVALID, "SSOR omega must be finite and nonzero");
@@ -112,50 +154,31 @@ Ssor* ssor_create(System& S, double omega) {
     P->n = n;
     P->omega = omega;
     hipStream_t s = P->stream.get();
-    // the pattern on the host for the level schedules
+    // the matrix on the host (a copy: the preconditioner outlives changes to M)
     std::vector<uint32_t> rp(size_t(n) + 1), ci(nnz);
+    std::vector<double> va(nnz);
     NLHIP_CHECK(hipMemcpyAsync(rp.data(), S.rowptr.get(), rp.size() * 4, hipMemcpyDeviceToHost, s));
-    if (nnz)
-        NLHIP_CHECK(hipMemcpyAsync(ci.data(), S.colind.get(), nnz * 4, hipMemcpyDeviceToHost, s));
-    // the matrix itself (a copy: the preconditioner outlives changes to M)
-    P->rp.alloc(size_t(n) + 1);
-    P->ci.alloc(std::max<uint64_t>(nnz, 1));
-    P->val.alloc(std::max<uint64_t>(nnz, 1));
-    NLHIP_CHECK(hipMemcpyAsync(P->rp.get(), S.rowptr.get(), (size_t(n) + 1) * 4,
-                               hipMemcpyDeviceToDevice, s));
     if (nnz) {
-        NLHIP_CHECK(hipMemcpyAsync(P->ci.get(), S.colind.get(), nnz * 4, hipMemcpyDeviceToDevice, s));
-        NLHIP_CHECK(hipMemcpyAsync(P->val.get(), S.val.get(), nnz * 8, hipMemcpyDeviceToDevice, s));
+        NLHIP_CHECK(hipMemcpyAsync(ci.data(), S.colind.get(), nnz * 4, hipMemcpyDeviceToHost, s));
+        NLHIP_CHECK(hipMemcpyAsync(va.data(), S.val.get(), nnz * 8, hipMemcpyDeviceToHost, s));
     }
     P->diag.alloc(n);
     P->dscale.alloc(n);
     {
         DevBuf<uint32_t> bad(1);
         NLHIP_CHECK(hipMemsetAsync(bad.get(), 0, 4, s));
-        NLHIP_CHECK(ssor_diag(n, P->rp.get(), P->ci.get(), P->val.get(), omega, P->diag.get(),
-                              P->dscale.get(), bad.get(), s));
+        NLHIP_CHECK(ssor_diag(n, S.rowptr.get(), S.colind.get(), S.val.get(), omega,
+                              P->diag.get(), P->dscale.get(), bad.get(), s));
         uint32_t h = 0;
         NLHIP_CHECK(hipMemcpyAsync(&h, bad.get(), 4, hipMemcpyDeviceToHost, s));
         P->stream.sync();
         if (h) throw Error(NLHIP_ERR_INVALID, "SSOR needs a nonzero, finite diagonal entry in every row");
     }
-    std::vector<uint32_t> rows, off;
-    level_schedule(n, rp, ci, false, rows, off);
-    P->rows_lo.alloc(std::max<size_t>(rows.size(), 1));
-    P->off_lo.alloc(off.size());
-    copy_in(P->rows_lo.get(), rows.data(), rows.size() * 4, s);
-    copy_in(P->off_lo.get(), off.data(), off.size() * 4, s);
-    P->lo_steps = plan(off);
-    level_schedule(n, rp, ci, true, rows, off);
-    P->rows_up.alloc(std::max<size_t>(rows.size(), 1));
-    P->off_up.alloc(off.size());
-    copy_in(P->rows_up.get(), rows.data(), rows.size() * 4, s);
-    copy_in(P->off_up.get(), off.data(), off.size() * 4, s);
-    P->up_steps = plan(off);
+    P->lo_steps = build_sweep(n, rp, ci, va, false, P->lo, s);
+    P->up_steps = build_sweep(n, rp, ci, va, true, P->up, s);
     P->x.alloc(n);
     P->w.alloc(n);
     P->y.alloc(n);
-    P->stream.sync();
     const SsorArgs a = P->args();
     P->graph.capture(s, [&] {
         for (const Ssor::Step& st : P->lo_steps)
