@@ -10,22 +10,37 @@ constexpr uint32_t kSpmmChunk = 1024;  // (colind, val) entries staged in LDS at
 constexpr uint32_t kSpmmPass = 64;     // rows per pass (at least)
 constexpr int kSpmmU = 8;              // gathers in flight per lane
 
+// Rows of one pass of the engine for G lanes per row.
+template <int G>
+constexpr uint32_t spmm_pass_rows() {
+    return uint32_t(kBlock / G) > kSpmmPass ? uint32_t(kBlock / G) : kSpmmPass;
+}
+
 // Rows of logical block L, in passes of max(64, kBlock / G) consecutive rows:
 // the block copies the pass's (colind, val) range into LDS with coalesced
-// loads (in chunks of kSpmmChunk entries), then each G-lane group sums its
-// rows of the pass from LDS, lane j gathering X[col * k + j].  Row sums are
-// left to right with one rounding per product and per add.  Every thread
-// of the block must call this (it synchronises the block).
-template <int G, typename T, typename Epi>
+// loads, then each G-lane group sums its rows of the pass from LDS, lane j
+// gathering X[col * k + j].  Row sums are left to right with one rounding
+// per product and per add.  For each row, lane j calls pre(row, j) (loads
+// the epilogue needs that do not depend on the sum) and then
+// fin(row, j, sum, pre value), row by row in the same order whatever the
+// path.  Every thread of the block must call this (it synchronises the
+// block).
+//
+// ONE: every pass's entries fit in one LDS chunk (the caller checked it,
+// spmm_one_chunk): a group finishes each row before the next, holding
+// one accumulator and issuing the row's epilogue loads with its gathers.
+// Otherwise passes are staged in chunks and the group keeps one accumulator
+// per row of the pass until the pass ends.
+template <int G, bool ONE, typename T, typename Pre, typename Fin>
 __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_per_block,
                                           uint32_t L, const uint32_t* __restrict__ rp,
                                           const uint32_t* __restrict__ ci,
                                           const T* __restrict__ val, const T* __restrict__ X,
-                                          Epi& epi, const int32_t* done = nullptr) {
+                                          Pre& pre, Fin& fin, const int32_t* done = nullptr) {
     __shared__ uint32_t s_ci[kSpmmChunk];
     __shared__ T s_val[kSpmmChunk];
     constexpr uint32_t groups = kBlock / G;
-    constexpr uint32_t pass = groups > kSpmmPass ? groups : kSpmmPass;
+    constexpr uint32_t pass = spmm_pass_rows<G>();
     const uint32_t g = threadIdx.x / G, j = threadIdx.x % G;
     // columns that have stopped gather nothing (their sums are not used)
     const bool active_col = j < k && !(done && done[j]);
@@ -33,50 +48,71 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
     const uint32_t r0 = rb0 < m ? uint32_t(rb0) : m;
     const uint32_t r1 = (uint64_t(r0) + rows_per_block < m) ? r0 + rows_per_block : m;
     constexpr uint32_t R = pass / groups;  // rows per group in one pass
+    // U gathers of entries [p, hi) (LDS offsets), added in order to acc
+    auto gather = [&](uint32_t p, uint32_t hi, T& acc) {
+        T av[kSpmmU], xv[kSpmmU];
+#pragma unroll
+        for (int u = 0; u < kSpmmU; ++u) {
+            const uint32_t q = p + u < hi ? p + u : hi - 1u;
+            av[u] = s_val[q];
+            xv[u] = active_col ? X[uint64_t(s_ci[q]) * k + j] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kSpmmU; ++u)
+            if (p + u < hi) acc = add_rn(acc, mul_rn(av[u], xv[u]));
+    };
     for (uint32_t a = r0; a < r1; a += pass) {
         const uint32_t pe = (a + pass < r1) ? a + pass : r1;
         const uint32_t J0 = rp[a], J1 = rp[pe];
-        uint32_t rb[R], re[R];
-        T acc[R];
-#pragma unroll
-        for (uint32_t t = 0; t < R; ++t) {
-            const uint32_t row = a + t * groups + g;
-            rb[t] = re[t] = 0;
-            if (row < pe) {
-                rb[t] = rp[row];
-                re[t] = rp[row + 1];
-            }
-            acc[t] = T(0);
-        }
-        for (uint32_t c0 = J0; c0 < J1; c0 += kSpmmChunk) {
-            const uint32_t c1 = (c0 + kSpmmChunk < J1) ? c0 + kSpmmChunk : J1;
-            __syncthreads();  // the previous chunk has been consumed
-            for (uint32_t q = c0 + threadIdx.x; q < c1; q += kBlock) {
-                s_ci[q - c0] = ci[q];
-                s_val[q - c0] = val[q];
+        if constexpr (ONE) {
+            __syncthreads();  // the previous pass has been consumed
+            for (uint32_t q = J0 + threadIdx.x; q < J1; q += kBlock) {
+                s_ci[q - J0] = ci[q];
+                s_val[q - J0] = val[q];
             }
             __syncthreads();
+            for (uint32_t t = 0; t < R; ++t) {
+                const uint32_t row = a + t * groups + g;
+                if (row >= pe) break;
+                const uint32_t lo = rp[row] - J0, hi = rp[row + 1] - J0;
+                decltype(pre(row, j)) pv{};
+                if (j < k) pv = pre(row, j);
+                T acc = T(0);
+                for (uint32_t p = lo; p < hi; p += kSpmmU) gather(p, hi, acc);
+                if (j < k) fin(row, j, acc, pv);
+            }
+        } else {
+            uint32_t rb[R], re[R];
+            T acc[R];
 #pragma unroll
             for (uint32_t t = 0; t < R; ++t) {
-                const uint32_t lo = rb[t] > c0 ? rb[t] : c0, hi = re[t] < c1 ? re[t] : c1;
-                for (uint32_t p = lo; p < hi; p += kSpmmU) {
-                    T av[kSpmmU], xv[kSpmmU];
+                const uint32_t row = a + t * groups + g;
+                rb[t] = re[t] = 0;
+                if (row < pe) {
+                    rb[t] = rp[row];
+                    re[t] = rp[row + 1];
+                }
+                acc[t] = T(0);
+            }
+            for (uint32_t c0 = J0; c0 < J1; c0 += kSpmmChunk) {
+                const uint32_t c1 = (c0 + kSpmmChunk < J1) ? c0 + kSpmmChunk : J1;
+                __syncthreads();  // the previous chunk has been consumed
+                for (uint32_t q = c0 + threadIdx.x; q < c1; q += kBlock) {
+                    s_ci[q - c0] = ci[q];
+                    s_val[q - c0] = val[q];
+                }
+                __syncthreads();
 #pragma unroll
-                    for (int u = 0; u < kSpmmU; ++u) {
-                        const uint32_t q = (p + u < hi ? p + u : hi - 1u) - c0;
-                        av[u] = s_val[q];
-                        xv[u] = active_col ? X[uint64_t(s_ci[q]) * k + j] : T(0);
-                    }
-#pragma unroll
-                    for (int u = 0; u < kSpmmU; ++u)
-                        if (p + u < hi) acc[t] = add_rn(acc[t], mul_rn(av[u], xv[u]));
+                for (uint32_t t = 0; t < R; ++t) {
+                    const uint32_t lo = rb[t] > c0 ? rb[t] : c0, hi = re[t] < c1 ? re[t] : c1;
+                    for (uint32_t p = lo; p < hi; p += kSpmmU) gather(p - c0, hi - c0, acc[t]);
                 }
             }
-        }
 #pragma unroll
-        for (uint32_t t = 0; t < R; ++t) {
-            const uint32_t row = a + t * groups + g;
-            if (j < k && row < pe) epi(row, j, acc[t]);
+            for (uint32_t t = 0; t < R; ++t) {
+                const uint32_t row = a + t * groups + g;
+                if (j < k && row < pe) fin(row, j, acc[t], pre(row, j));
+            }
         }
     }
 }

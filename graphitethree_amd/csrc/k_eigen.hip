@@ -18,32 +18,41 @@ namespace {
 constexpr uint32_t kGramTile = 64;  // rows per LDS tile of the Gram kernel
 
 // AY = A Y.
-template <int P>
+template <int P, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_eig_spmm(uint32_t n, uint32_t rpb, const uint32_t* rp,
                                                      const uint32_t* ci, const double* val,
                                                      const double* Y, double* AY) {
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
-    auto epi = [&](uint32_t row, uint32_t j, double s) { AY[size_t(row) * P + j] = s; };
-    spmm_rows<P>(n, P, rpb, L, rp, ci, val, Y, epi);
+    auto pre = [](uint32_t, uint32_t) { return 0; };
+    auto fin = [&](uint32_t row, uint32_t j, double s, int) { AY[size_t(row) * P + j] = s; };
+    spmm_rows<P, ONE>(n, P, rpb, L, rp, ci, val, Y, pre, fin);
 }
 
 // One step of the Chebyshev recurrence on T = M^-1 A:
 //   Ynew = (T Y - c Y) s1 - s2 Xold       (s2 = 0 and no Xold on the first)
-template <int P>
+// The row's minv, Y and Xold are loaded with its gathers.
+struct ChebPre {
+    double minv, y, xold;
+};
+
+template <int P, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_eig_cheb(uint32_t n, uint32_t rpb, const uint32_t* rp,
                                                      const uint32_t* ci, const double* val,
                                                      const double* minv, const double* Y,
                                                      const double* Xold, double* Ynew, double c,
                                                      double s1, double s2) {
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
-    auto epi = [&](uint32_t row, uint32_t j, double s) {
+    auto pre = [&](uint32_t row, uint32_t j) {
         const size_t i = size_t(row) * P + j;
-        const double t = minv ? mul_rn(s, minv[row]) : s;
-        double v = mul_rn(add_rn(t, -mul_rn(c, Y[i])), s1);
-        if (Xold) v = add_rn(v, -mul_rn(s2, Xold[i]));
-        Ynew[i] = v;
+        return ChebPre{minv ? minv[row] : 1.0, Y[i], Xold ? Xold[i] : 0.0};
     };
-    spmm_rows<P>(n, P, rpb, L, rp, ci, val, Y, epi);
+    auto fin = [&](uint32_t row, uint32_t j, double s, const ChebPre& pv) {
+        const double t = minv ? mul_rn(s, pv.minv) : s;
+        double v = mul_rn(add_rn(t, -mul_rn(c, pv.y)), s1);
+        if (Xold) v = add_rn(v, -mul_rn(s2, pv.xold));
+        Ynew[size_t(row) * P + j] = v;
+    };
+    spmm_rows<P, ONE>(n, P, rpb, L, rp, ci, val, Y, pre, fin);
 }
 
 // Per workgroup b: part[b][i][j] = sum over its rows r (in order) of
@@ -225,16 +234,26 @@ EigShape eig_shape(uint32_t n, uint32_t p, int cu) {
 
 void eig_spmm(const EigShape& e, uint32_t n, uint32_t p, const uint32_t* rp, const uint32_t* ci,
               const double* val, const double* Y, double* AY, hipStream_t s) {
-    NLHIP_EIG_P(p, (k_eig_spmm<PP><<<e.grid, kBlock, 0, s>>>(n, e.rows_per_block, rp, ci, val,
-                                                              Y, AY)));
+    if (e.one_chunk) {
+        NLHIP_EIG_P(p, (k_eig_spmm<PP, true><<<e.grid, kBlock, 0, s>>>(n, e.rows_per_block, rp, ci,
+                                                                    val, Y, AY)));
+    } else {
+        NLHIP_EIG_P(p, (k_eig_spmm<PP, false><<<e.grid, kBlock, 0, s>>>(n, e.rows_per_block, rp,
+                                                                     ci, val, Y, AY)));
+    }
     NLHIP_CHECK(hipGetLastError());
 }
 
 void eig_cheb(const EigShape& e, uint32_t n, uint32_t p, const uint32_t* rp, const uint32_t* ci,
               const double* val, const double* minv, const double* Y, const double* Xold,
               double* Ynew, double c, double s1, double s2, hipStream_t s) {
-    NLHIP_EIG_P(p, (k_eig_cheb<PP><<<e.grid, kBlock, 0, s>>>(n, e.rows_per_block, rp, ci, val,
-                                                              minv, Y, Xold, Ynew, c, s1, s2)));
+    if (e.one_chunk) {
+        NLHIP_EIG_P(p, (k_eig_cheb<PP, true><<<e.grid, kBlock, 0, s>>>(
+                           n, e.rows_per_block, rp, ci, val, minv, Y, Xold, Ynew, c, s1, s2)));
+    } else {
+        NLHIP_EIG_P(p, (k_eig_cheb<PP, false><<<e.grid, kBlock, 0, s>>>(
+                           n, e.rows_per_block, rp, ci, val, minv, Y, Xold, Ynew, c, s1, s2)));
+    }
     NLHIP_CHECK(hipGetLastError());
 }
 

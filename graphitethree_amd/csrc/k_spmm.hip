@@ -15,8 +15,11 @@
 // partials of d_j.q_j; a reduction launch (alpha_j); update X, R, H with
 // partials of r_j.h_j and r_j.r_j; a reduction launch (stopping test, beta_j).
 // Columns whose stopping test held are left untouched.
+#include <algorithm>
+
 #include "dev_common.h"
 #include "dev_spmm.h"
+#include "nlhip_internal.h"
 #include "nlhip_kernels.h"
 
 namespace nlhip {
@@ -46,24 +49,27 @@ __device__ __forceinline__ void column_partials(T v, uint32_t k, uint32_t G, T* 
     }
 }
 
-template <int G, typename T, bool PREC>
+template <int G, bool ONE, typename T, bool PREC>
 __global__ __launch_bounds__(kBlock) void k_m_init(MultiArgs<T> a) {
     __shared__ T lds[kBlock];
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t k = a.k;
     T prh = T(0), prr = T(0), pbb = T(0);
-    auto epi = [&](uint32_t row, uint32_t j, T s) {
+    auto pre = [&](uint32_t row, uint32_t j) {
+        return Pair<T>{a.b[uint64_t(row) * k + j], PREC ? a.dinv[row] : T(1)};
+    };
+    auto fin = [&](uint32_t row, uint32_t j, T s, const Pair<T>& pv) {
         const uint64_t i = uint64_t(row) * k + j;
-        const T bi = a.b[i];
+        const T bi = pv.a;
         const T ri = add_rn(s, mul_rn(T(-1), bi));
-        const T hi = PREC ? mul_rn(a.dinv[row], ri) : ri;
+        const T hi = PREC ? mul_rn(pv.b, ri) : ri;
         a.r[i] = ri;
         a.h[i] = hi;
         prh += ri * hi;
         prr += ri * ri;
         pbb += bi * bi;
     };
-    spmm_rows<G>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.x, epi);
+    spmm_rows<G, ONE>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.x, pre, fin);
     column_partials(prh, k, G, lds, a.part_rh + size_t(L) * k);
     column_partials(prr, k, G, lds, a.part_rr + size_t(L) * k);
     column_partials(pbb, k, G, lds, a.part_bb + size_t(L) * k);
@@ -131,19 +137,19 @@ __global__ __launch_bounds__(kBlock) void k_m_dir(MultiArgs<T> a) {
     }
 }
 
-template <int G, typename T>
+template <int G, bool ONE, typename T>
 __global__ __launch_bounds__(kBlock) void k_m_spmm(MultiArgs<T> a) {
     __shared__ T lds[kBlock];
     if (*a.n_active == 0) return;  // every column has stopped (block-uniform)
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t k = a.k;
     T pdq = T(0);
-    auto epi = [&](uint32_t row, uint32_t j, T s) {
-        const uint64_t i = uint64_t(row) * k + j;
-        a.q[i] = s;
-        pdq += a.d[i] * s;
+    auto pre = [&](uint32_t row, uint32_t j) { return a.d[uint64_t(row) * k + j]; };
+    auto fin = [&](uint32_t row, uint32_t j, T s, T di) {
+        a.q[uint64_t(row) * k + j] = s;
+        pdq += di * s;
     };
-    spmm_rows<G>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.d, epi, a.done);
+    spmm_rows<G, ONE>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.d, pre, fin, a.done);
     column_partials(pdq, k, G, lds, a.part_dq + size_t(L) * k);
 }
 
@@ -241,21 +247,20 @@ __global__ __launch_bounds__(kBlock) void k_to_colmajor(uint32_t n, uint32_t k, 
 }
 
 // R = B - A X (FP64, interleaved), per-column partial |r|^2 and |b|^2.
-template <int G>
+template <int G, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_m_residual64(MultiArgs<double> a) {
     __shared__ double lds[kBlock];
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t k = a.k;
     double prr = 0.0, pbb = 0.0;
-    auto epi = [&](uint32_t row, uint32_t j, double s) {
-        const uint64_t i = uint64_t(row) * k + j;
-        const double bi = a.b[i];
+    auto pre = [&](uint32_t row, uint32_t j) { return a.b[uint64_t(row) * k + j]; };
+    auto fin = [&](uint32_t row, uint32_t j, double s, double bi) {
         const double ri = bi - s;
-        a.r[i] = ri;
+        a.r[uint64_t(row) * k + j] = ri;
         prr += ri * ri;
         pbb += bi * bi;
     };
-    spmm_rows<G>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.x, epi);
+    spmm_rows<G, ONE>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.x, pre, fin);
     column_partials(prr, k, G, lds, a.part_rr + size_t(L) * k);
     column_partials(pbb, k, G, lds, a.part_bb + size_t(L) * k);
 }
@@ -323,12 +328,48 @@ static uint32_t ew_grid_m(uint64_t n) {
     return uint32_t(g > 4096 ? 4096 : (g == 0 ? 1 : g));
 }
 
+namespace {
+
+__global__ void k_window_max(uint32_t m, const uint32_t* rp, uint32_t pass, uint32_t* out) {
+    const uint32_t nw = (m + 7) / 8;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t best = 0;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += stride) {
+        const uint32_t a = 8 * w;
+        const uint32_t e = (uint64_t(a) + pass < m) ? a + pass : m;
+        const uint32_t c = rp[e] - rp[a];
+        best = c > best ? c : best;
+    }
+    atomicMax(out, best);
+}
+
+}  // namespace
+
+// The largest number of entries in `pass` consecutive rows starting at a
+// multiple of 8 rows: every pass spmm_rows takes starts at such a row
+// (blocks hold multiples of 8 rows for every G).
+bool spmm_one_chunk(uint32_t m, const uint32_t* rp, uint32_t G, hipStream_t s) {
+    const uint32_t pass = std::max<uint32_t>(kBlock / G, kSpmmPass);
+    DevBuf<uint32_t> out(1);
+    NLHIP_CHECK(hipMemsetAsync(out.get(), 0, 4, s));
+    k_window_max<<<ew_grid_m((uint64_t(m) + 7) / 8), kBlock, 0, s>>>(m, rp, pass, out.get());
+    NLHIP_CHECK(hipGetLastError());
+    uint32_t h = 0;
+    NLHIP_CHECK(hipMemcpyAsync(&h, out.get(), 4, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipStreamSynchronize(s));
+    return h <= kSpmmChunk;
+}
+
 template <typename T>
 hipError_t multi_start(const MultiArgs<T>& a, bool precond, T eps, int max_iter, hipStream_t s) {
-    if (precond) {
-        NLHIP_G_SWITCH(a.G, (k_m_init<G, T, true><<<a.P, kBlock, 0, s>>>(a)));
+    if (precond && a.one_chunk) {
+        NLHIP_G_SWITCH(a.G, (k_m_init<G, true, T, true><<<a.P, kBlock, 0, s>>>(a)));
+    } else if (precond) {
+        NLHIP_G_SWITCH(a.G, (k_m_init<G, false, T, true><<<a.P, kBlock, 0, s>>>(a)));
+    } else if (a.one_chunk) {
+        NLHIP_G_SWITCH(a.G, (k_m_init<G, true, T, false><<<a.P, kBlock, 0, s>>>(a)));
     } else {
-        NLHIP_G_SWITCH(a.G, (k_m_init<G, T, false><<<a.P, kBlock, 0, s>>>(a)));
+        NLHIP_G_SWITCH(a.G, (k_m_init<G, false, T, false><<<a.P, kBlock, 0, s>>>(a)));
     }
     k_m_sys_init<T><<<a.k, kBlock, 0, s>>>(a, eps, max_iter);
     k_m_count<T><<<1, 64, 0, s>>>(a);
@@ -339,7 +380,11 @@ template <typename T>
 hipError_t multi_iteration(const MultiArgs<T>& a, bool precond, int max_iter, hipStream_t s) {
     const uint32_t cg = a.k;  // one workgroup per column for the reductions
     k_m_dir<T><<<a.P, kBlock, 0, s>>>(a);
-    NLHIP_G_SWITCH(a.G, (k_m_spmm<G, T><<<a.P, kBlock, 0, s>>>(a)));
+    if (a.one_chunk) {
+        NLHIP_G_SWITCH(a.G, (k_m_spmm<G, true, T><<<a.P, kBlock, 0, s>>>(a)));
+    } else {
+        NLHIP_G_SWITCH(a.G, (k_m_spmm<G, false, T><<<a.P, kBlock, 0, s>>>(a)));
+    }
     k_m_reduce_alpha<T><<<cg, kBlock, 0, s>>>(a);
     if (precond) k_m_update<T, true><<<a.P, kBlock, 0, s>>>(a);
     else k_m_update<T, false><<<a.P, kBlock, 0, s>>>(a);
@@ -361,7 +406,11 @@ hipError_t multi_layout(uint32_t n, uint32_t k, const T* src, T* dst, bool to_in
 }
 
 hipError_t multi_residual64(const MultiArgs<double>& a, hipStream_t s) {
-    NLHIP_G_SWITCH(a.G, (k_m_residual64<G><<<a.P, kBlock, 0, s>>>(a)));
+    if (a.one_chunk) {
+        NLHIP_G_SWITCH(a.G, (k_m_residual64<G, true><<<a.P, kBlock, 0, s>>>(a)));
+    } else {
+        NLHIP_G_SWITCH(a.G, (k_m_residual64<G, false><<<a.P, kBlock, 0, s>>>(a)));
+    }
     k_m_norms64<<<a.k, kBlock, 0, s>>>(a);
     return hipGetLastError();
 }

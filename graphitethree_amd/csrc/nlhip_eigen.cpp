@@ -210,7 +210,8 @@ void eigen_solve(System& A, const double* mass_in, uint32_t k, double sigma, dou
     const uint32_t* rp = A.rowptr.get();
     const uint32_t* ci = A.colind.get();
     const double* va = A.val.get();
-    const EigShape shape = eig_shape(n, P, device().cu_count);
+    EigShape shape = eig_shape(n, P, device().cu_count);
+    shape.one_chunk = spmm_one_chunk(n, rp, P, s);
     const double b_up = eig_bound(n, rp, va, minv, s);
     if (!(b_up > 0.0) || !std::isfinite(b_up))
         throw Error(NLHIP_ERR_INVALID, "matrix without a positive spectrum bound");

@@ -416,6 +416,7 @@ bool build_col_offsets(uint64_t nnz, const uint32_t* colind, hipStream_t s,
 // Launch shape of the eigensolver's multi-vector SpMM (k_eigen.hip).
 struct EigShape {
     uint32_t grid = 1, rows_per_block = 64;
+    bool one_chunk = false;  // spmm_one_chunk for the block width
 };
 
 // Launch-shape autotuning of a system's CG (S.mu held).
@@ -494,6 +495,11 @@ void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
 // per (k entries) and total may be null.
 void solve_multi(System& S, uint32_t k, const double* B, double* X,
                  const nlhip_solve_params& p, nlhip_solve_stats* per, nlhip_solve_stats* total);
+
+// True when every pass of the SpMM row engine with G lanes per row (G =
+// columns rounded up to a power of two) fits one LDS chunk on the matrix
+// with row pointer rp (device), so the engine can finish rows one by one.
+bool spmm_one_chunk(uint32_t m, const uint32_t* rp, uint32_t G, hipStream_t s);
 
 // ------------------------------------------------------------------ SSOR
 

@@ -245,7 +245,7 @@ struct MultiArgs {
     uint32_t G;               // lanes per row: k rounded up to a power of two
     uint32_t P;               // workgroups of the row-shaped launches
     uint32_t rows_per_block;
-    uint32_t pad;
+    uint32_t one_chunk;       // every SpMM pass fits one LDS chunk (spmm_window_max)
     const uint32_t* rp;
     const uint32_t* ci;
     const T* val;

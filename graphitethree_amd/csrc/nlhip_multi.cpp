@@ -63,6 +63,7 @@ void multi_alloc(System& S, MultiWork<T>& w, uint32_t k) {
     uint64_t rpb = (uint64_t(S.n) + P - 1) / P;
     rpb = (rpb + rows_per_pass - 1) / rows_per_pass * rows_per_pass;
     a.rows_per_block = uint32_t(std::max<uint64_t>(rpb, rows_per_pass));
+    a.one_chunk = spmm_one_chunk(S.n, S.rowptr.get(), a.G, S.stream.get()) ? 1u : 0u;
     a.rp = S.rowptr.get();
     a.ci = S.colind.get();
     a.x = w.x.get();
