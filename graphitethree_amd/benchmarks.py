@@ -65,6 +65,8 @@ def run_atlas(args) -> dict:
     t0 = time.perf_counter()
     B = capi.Batch(packed=(A.row_off, A.rowptr, A.colind, A.val), fp32=True)
     t_packed_create = time.perf_counter() - t0
+    if args.direct_launches:
+        B.set_graphs(False)
     params = dict(eps=args.eps, max_iter=args.max_iter)
 
     tuned = B.autotune(A.b, iters=args.tune_iters) if args.tune_iters else None
@@ -136,6 +138,25 @@ def run_atlas(args) -> dict:
     res["speedup_mixed_batched_vs_per_chart_fp64"] = t_plain_all / tm.device_seconds
     res["speedup_fp64_batched_vs_per_chart_fp64"] = t_plain_all / t64.device_seconds
     return res
+
+
+def run_atlas_trace(args) -> dict:
+    """The batched FP64 atlas solve with every kernel launched directly (no
+    graph replay), for kernel tracing: per-kernel time and HBM counters of
+    the batch kernels.  One warm-up solve, one timed solve."""
+    dev = init_device()
+    A = W.atlas(args.count, seed=args.seed, nmin=args.nmin, nmax=args.nmax)
+    log(f"atlas: {A.count} charts, n={A.n}, nnz={A.nnz}")
+    B = capi.Batch(packed=(A.row_off, A.rowptr, A.colind, A.val))
+    B.set_graphs(False)
+    B.solve(A.b, eps=args.eps, max_iter=args.max_iter)
+    x, per, tot = B.solve(A.b, eps=args.eps, max_iter=args.max_iter)
+    log(f"batched fp64, direct launches: {tot.device_seconds:.3f}s, max it {tot.iterations}")
+    return {"workload": f"atlas_{A.count}_charts_N{int(args.nmin)}-{int(args.nmax)}",
+            "device": dev[0], "n": A.n, "nnz": A.nnz, "launches": "direct",
+            "seconds": tot.device_seconds, "max_iterations": int(tot.iterations),
+            "converged": int(sum(p.converged for p in per)),
+            "spmv_bytes_all_active": 8.5 * A.nnz + 16.0 * A.n}
 
 
 def atlas_device_assembly(args, A) -> dict:
@@ -417,6 +438,16 @@ def main(argv=None):
     a.add_argument("--inner-eps", type=float, default=1e-4)
     a.add_argument("--baseline-stride", type=int, default=1)
     a.add_argument("--no-device-assembly", action="store_true")
+    a.add_argument("--direct-launches", action="store_true",
+                   help="no graph replay (kernel tracing); same results")
+    at = sub.add_parser("atlas-trace",
+                        help="the batched FP64 atlas solve alone, launched directly")
+    at.add_argument("--count", type=int, default=2000)
+    at.add_argument("--seed", type=int, default=1)
+    at.add_argument("--nmin", type=float, default=50)
+    at.add_argument("--nmax", type=float, default=50000)
+    at.add_argument("--eps", type=float, default=1e-10)
+    at.add_argument("--max-iter", type=int, default=50000)
     a.add_argument("--tune-iters", type=int, default=50,
                    help="iterations per candidate of the batch autotuner (0: no tuning)")
     mr = sub.add_parser("multirhs")
@@ -456,6 +487,8 @@ def main(argv=None):
         out = run_serving(args)
     elif args.cmd == "atlas":
         out = run_atlas(args)
+    elif args.cmd == "atlas-trace":
+        out = run_atlas_trace(args)
     elif args.cmd == "multirhs":
         out = run_multirhs(args)
     elif args.cmd == "eigen":

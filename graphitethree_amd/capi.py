@@ -204,6 +204,7 @@ SIGNATURES = {
                                        C.c_uint32, C.POINTER(_vp), _vp, _u32p]),
     "nlhipBatchInfo": (C.c_int, [_vp, _u32p, _u32p, C.POINTER(C.c_uint64), _u32p]),
     "nlhipBatchSetLaunch": (C.c_int, [_vp, C.c_uint32]),
+    "nlhipBatchSetGraphs": (C.c_int, [_vp, C.c_int]),
     "nlhipBatchSetSpmvPasses": (C.c_int, [_vp, C.c_uint32]),
     "nlhipBatchSolve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
                                   C.POINTER(SolveStats), C.POINTER(SolveStats)]),
@@ -524,6 +525,10 @@ class Batch:
 
     def set_launch(self, workgroups: int):
         check(lib().nlhipBatchSetLaunch(self._h, workgroups), "nlhipBatchSetLaunch")
+
+    def set_graphs(self, on: bool):
+        """False: enqueue every launch directly (nlhipBatchSetGraphs)."""
+        check(lib().nlhipBatchSetGraphs(self._h, 1 if on else 0), "nlhipBatchSetGraphs")
 
     def autotune(self, b, iters: int = 50):
         """Times grid and SpMV run length (nlhipBatchAutotune); keeps and returns

@@ -46,6 +46,7 @@ struct Batch {
     std::vector<uint32_t> h_row_off;
     uint32_t grid = 1;
     uint32_t spmv_passes = 0;  // 0: by active tiles, else 1, 2 or 4
+    bool graphs = true;        // false: the same launches enqueued directly
     Stream stream;
     std::mutex mu;
     BatchWork<double> w64;
@@ -159,7 +160,13 @@ void run_batch(Batch& B, BatchWork<T>& w, bool precond, T eps, int max_iter, uin
     const uint64_t limit = uint64_t(std::max(max_iter, 0)) + 3ull * w.graph.cap() + 8;
     uint64_t launched = 0, covered = 0;
     for (;;) {
-        w.graph.launch(launched, s);
+        if (B.graphs) {
+            w.graph.launch(launched, s);
+        } else {
+            for (uint32_t i = 0; i < w.graph.iters(launched); ++i)
+                NLHIP_CHECK(batch_iteration<T>(w.args, precond, max_iter, B.grid, s));
+            NLHIP_CHECK(batch_compact<T>(w.args, B.grid, s));
+        }
         covered += w.graph.iters(launched);
         const int slot = int(launched & 1);
         NLHIP_CHECK(hipMemcpyAsync(&host[slot], w.counts.get() + 1, sizeof(uint32_t),
@@ -634,6 +641,14 @@ NLAPI int NLAPIENTRY nlhipBatchGetCRS(nlhip_batch b, uint32_t* row_off, uint32_t
         if (val && B->nnz)
             NLHIP_CHECK(hipMemcpyAsync(val, B->val.get(), B->nnz * 8, hipMemcpyDefault, s));
         NLHIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipBatchSetGraphs(nlhip_batch b, int on) {
+    return guarded("nlhipBatchSetGraphs", [&] {
+        Batch* B = bat(b);
+        std::lock_guard<std::mutex> lock(B->mu);
+        B->graphs = on != 0;
     });
 }
 

@@ -252,6 +252,11 @@ NLAPI int NLAPIENTRY nlhipBatchInfo(nlhip_batch b, uint32_t* nsys, uint32_t* n,
  * a caller inspects a batch the device assembled. */
 NLAPI int NLAPIENTRY nlhipBatchGetCRS(nlhip_batch b, uint32_t* row_off, uint32_t* rowptr,
                                       uint32_t* colind, double* val);
+/* on = 0: solves enqueue every kernel launch directly instead of replaying
+ * captured graphs (for tracing tools that cannot follow graph launches);
+ * the same launches in the same order, so the same bits.  No counterpart in
+ * the reference. */
+NLAPI int NLAPIENTRY nlhipBatchSetGraphs(nlhip_batch b, int on);
 /* Workgroups of the grid that walks the active tiles. */
 NLAPI int NLAPIENTRY nlhipBatchSetLaunch(nlhip_batch b, uint32_t workgroups);
 /* Passes of a tile (64 rows or block rows each, 4 per tile) that one

@@ -161,3 +161,16 @@ def test_batch_rejects_bad_input(hip):
         capi.check(capi.lib().nlhipBatchCreate(2, capi.ptr(row_off), capi.ptr(rp), capi.ptr(ci),
                                                capi.ptr(va), 0, capi.C.byref(h)), "create")
     assert e.value.code == capi.NLHIP_ERR_INVALID
+
+
+def test_batch_direct_launches_give_the_same_bits(hip, small_atlas):
+    A = small_atlas
+    B = capi.Batch(packed=(A.row_off, A.rowptr, A.colind, A.val))
+    x1, _, _ = B.solve(A.b, eps=1e-10, max_iter=20000)
+    B.set_graphs(False)
+    x2, _, _ = B.solve(A.b, eps=1e-10, max_iter=20000)
+    B.set_graphs(True)
+    x3, _, _ = B.solve(A.b, eps=1e-10, max_iter=20000)
+    assert np.array_equal(x1.view(np.uint64), x2.view(np.uint64))
+    assert np.array_equal(x1.view(np.uint64), x3.view(np.uint64))
+    B.close()
