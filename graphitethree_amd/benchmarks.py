@@ -83,6 +83,15 @@ def run_atlas(args) -> dict:
                          max_iter=args.max_iter_mixed, eps=args.eps)
     log(f"batched mixed: {tm.device_seconds:.3f}s, converged "
         f"{sum(p.converged for p in pm)}/{A.count}")
+    fb_params = dict(precision="mixed-fallback", inner_eps=args.inner_eps,
+                     inner_max_iter=args.fallback_inner_iters,
+                     max_iter=args.max_iter_mixed, eps=args.eps)
+    B.solve(A.b, **fb_params)
+    xf, pf, tf = B.solve(A.b, **fb_params)
+    nfb = B.fallbacks()
+    log(f"batched mixed, FP64 fallback after {args.fallback_inner_iters} inner iterations: "
+        f"{tf.device_seconds:.3f}s, {nfb} of {A.count} charts finished in FP64, converged "
+        f"{sum(p.converged for p in pf)}/{A.count}")
 
     # plain FP64 GPU CG, one nlhipSolve per chart (matrices resident)
     charts = list(range(0, A.count, args.baseline_stride))
@@ -130,6 +139,17 @@ def run_atlas(args) -> dict:
                            "max_rel_diff_vs_batched": worst_diff,
                            "iterations_mean": float(np.mean(its_plain))},
     }
+    res["batched_mixed_fp64_fallback"] = {
+        "seconds": tf.device_seconds, "solves_per_s": A.count / tf.device_seconds,
+        "inner_max_iter": args.fallback_inner_iters, "charts_finished_in_fp64": nfb,
+        "converged": int(sum(p.converged for p in pf)),
+        "worst_rel_residual": float(tf.rel_residual),
+        "max_rel_diff_vs_fp64": float(max(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+                                          for a, b in zip(B.split(xf), xs64))),
+        "policy": "FP32 inner PCG in FP64 refinement; a chart whose FP32 inner solve uses "
+                  "inner_max_iter without reaching inner_eps finishes with FP64 CG (DSGESV's "
+                  "fallback)"}
+    res["speedup_mixed_fallback_batched_vs_per_chart_fp64"] = t_plain_all / tf.device_seconds
     res["create"] = {"host_build_seconds": t_host_build,
                      "host_build": "chart meshes, per-chart LSCM assembly and packing on "
                                    "the host (OpenMP), then nlhipBatchCreate",
@@ -438,6 +458,8 @@ def main(argv=None):
     a.add_argument("--inner-eps", type=float, default=1e-4)
     a.add_argument("--baseline-stride", type=int, default=1)
     a.add_argument("--no-device-assembly", action="store_true")
+    a.add_argument("--fallback-inner-iters", type=int, default=300,
+                   help="inner_max_iter of the mixed solve with FP64 fallback")
     a.add_argument("--direct-launches", action="store_true",
                    help="no graph replay (kernel tracing); same results")
     at = sub.add_parser("atlas-trace",

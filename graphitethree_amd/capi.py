@@ -35,7 +35,7 @@ NLHIP_OK = 0
 NLHIP_ERR_INVALID, NLHIP_ERR_NO_DEVICE, NLHIP_ERR_HIP, NLHIP_ERR_OOM = -1, -2, -3, -4
 NLHIP_ERR_NONFINITE, NLHIP_ERR_UNSUPPORTED, NLHIP_ERR_BUDGET, NLHIP_ERR_INJECTED = (
     -5, -6, -7, -8)
-NLHIP_PREC_FP64, NLHIP_PREC_MIXED = 0, 1
+NLHIP_PREC_FP64, NLHIP_PREC_MIXED, NLHIP_PREC_MIXED_FALLBACK = 0, 1, 2
 NLHIP_SYSTEM_FP32 = 1
 NLHIP_SYSTEM_NO_OFFSETS = 2
 NLHIP_SYSTEM_NO_BLOCKS = 4
@@ -205,6 +205,7 @@ SIGNATURES = {
     "nlhipBatchInfo": (C.c_int, [_vp, _u32p, _u32p, C.POINTER(C.c_uint64), _u32p]),
     "nlhipBatchSetLaunch": (C.c_int, [_vp, C.c_uint32]),
     "nlhipBatchSetGraphs": (C.c_int, [_vp, C.c_int]),
+    "nlhipBatchFallbacks": (C.c_int, [_vp, _u32p]),
     "nlhipBatchSetSpmvPasses": (C.c_int, [_vp, C.c_uint32]),
     "nlhipBatchSolve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
                                   C.POINTER(SolveStats), C.POINTER(SolveStats)]),
@@ -269,7 +270,8 @@ def default_params(**kw) -> SolveParams:
     lib().nlhipSolveParamsDefault(C.byref(p))
     for k, v in kw.items():
         if k == "precision" and isinstance(v, str):
-            v = {"fp64": NLHIP_PREC_FP64, "mixed": NLHIP_PREC_MIXED}[v]
+            v = {"fp64": NLHIP_PREC_FP64, "mixed": NLHIP_PREC_MIXED,
+                 "mixed-fallback": NLHIP_PREC_MIXED_FALLBACK}[v]
         if k == "precond" and isinstance(v, str):
             v = {"jacobi": NL_PRECOND_JACOBI, "none": NL_PRECOND_NONE}[v.lower()]
         setattr(p, k, v)
@@ -525,6 +527,12 @@ class Batch:
 
     def set_launch(self, workgroups: int):
         check(lib().nlhipBatchSetLaunch(self._h, workgroups), "nlhipBatchSetLaunch")
+
+    def fallbacks(self) -> int:
+        """Systems the last "mixed-fallback" solve finished in FP64."""
+        n = C.c_uint32()
+        check(lib().nlhipBatchFallbacks(self._h, C.byref(n)), "nlhipBatchFallbacks")
+        return int(n.value)
 
     def set_graphs(self, on: bool):
         """False: enqueue every launch directly (nlhipBatchSetGraphs)."""

@@ -47,6 +47,7 @@ struct Batch {
     uint32_t grid = 1;
     uint32_t spmv_passes = 0;  // 0: by active tiles, else 1, 2 or 4
     bool graphs = true;        // false: the same launches enqueued directly
+    uint32_t last_fallbacks = 0;  // systems the last fallback solve finished in FP64
     Stream stream;
     std::mutex mu;
     BatchWork<double> w64;
@@ -291,7 +292,7 @@ void batch_solve_fp64(Batch& B, const nlhip_solve_params& p, std::vector<int32_t
 
 void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_t>& its,
                        std::vector<double>& cgrel, std::vector<uint8_t>& conv,
-                       std::vector<uint32_t>& outer) {
+                       std::vector<uint32_t>& outer, std::vector<uint8_t>* fell_back = nullptr) {
     ensure_fp32(B);
     work_alloc(B, B.w32);
     BatchWork<double>& w64 = B.w64;
@@ -321,7 +322,8 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
             conv[k] = !(rr[k] > (p.eps * p.eps) * bb[k]) ? 1 : 0;
             cgrel[k] = rel[k];
             const bool budget = uint32_t(its[k]) >= p.max_iter || outer[k] >= p.max_outer;
-            skip[k] = (conv[k] || budget) ? 1 : 0;
+            const bool gone = fell_back && (*fell_back)[k];
+            skip[k] = (conv[k] || budget || gone) ? 1 : 0;
             const double rn = std::sqrt(rr[k]);
             sc[k] = skip[k] ? 0.0 : 1.0 / rn;
             sc[B.nsys + k] = skip[k] ? 0.0 : rn;
@@ -342,8 +344,59 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
             its[k] += inner_its[k];
             outer[k] += 1;
             progress |= inner_its[k] > 0;
+            // an inner solve that used its whole cap without reaching
+            // inner_eps: this system continues in FP64
+            if (fell_back && uint32_t(inner_its[k]) >= inner_cap) (*fell_back)[k] = 1;
         }
         if (!progress) break;
+    }
+}
+
+// Mixed precision with a fallback (the policy of LAPACK's DSGESV: when the
+// FP32 inner solver does not converge, finish in FP64): systems whose FP32
+// inner solve reaches inner_max_iter without reaching inner_eps stop
+// refining and are solved by the FP64 batched CG from their current iterate;
+// the others refine to eps as in batch_solve_mixed.
+void batch_solve_mixed_fallback(Batch& B, const nlhip_solve_params& p, std::vector<int32_t>& its,
+                                std::vector<double>& cgrel, std::vector<uint8_t>& conv,
+                                std::vector<uint32_t>& outer, uint32_t& nfallback) {
+    if (p.inner_max_iter == 0)
+        throw Error(NLHIP_ERR_INVALID, "the fallback needs inner_max_iter > 0");
+    std::vector<uint8_t> fb(B.nsys, 0);
+    // systems that fall back are dropped from the refinement at once: the
+    // FP32 loop skips them from the next outer step on
+    nlhip_solve_params pm = p;
+    batch_solve_mixed(B, pm, its, cgrel, conv, outer, &fb);
+    nfallback = 0;
+    std::vector<int32_t> skip(B.nsys);
+    for (uint32_t k = 0; k < B.nsys; ++k) {
+        const bool go = fb[k] && !conv[k];
+        skip[k] = go ? 0 : 1;
+        nfallback += go ? 1u : 0u;
+    }
+    if (nfallback == 0) return;
+    BatchWork<double>& w = B.w64;
+    hipStream_t s = B.stream.get();
+    NLHIP_CHECK(hipMemcpyAsync(w.skip.get(), skip.data(), B.nsys * 4, hipMemcpyHostToDevice, s));
+    const bool precond = p.precond == NL_PRECOND_JACOBI;
+    w.args.val = B.val.get();
+    w.args.dinv = precond ? B.dinv.get() : nullptr;
+    w.args.skip = w.skip.get();
+    const uint32_t K = p.check_every ? p.check_every : batch_check_every(B);
+    run_batch<double>(B, w, precond, p.eps, int(p.max_iter), K);
+    w.args.skip = nullptr;
+    std::vector<int32_t> its64(B.nsys);
+    std::vector<double> err(B.nsys), thr(B.nsys), bb(B.nsys);
+    NLHIP_CHECK(hipMemcpyAsync(its64.data(), w.iters.get(), B.nsys * 4, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipMemcpyAsync(err.data(), w.args.err, B.nsys * 8, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipMemcpyAsync(thr.data(), w.args.thresh, B.nsys * 8, hipMemcpyDeviceToHost, s));
+    NLHIP_CHECK(hipMemcpyAsync(bb.data(), w.args.bb, B.nsys * 8, hipMemcpyDeviceToHost, s));
+    B.stream.sync();
+    for (uint32_t k = 0; k < B.nsys; ++k) {
+        if (skip[k]) continue;
+        its[k] += its64[k];
+        cgrel[k] = bb[k] > 0 ? std::sqrt(err[k] / bb[k]) : std::sqrt(err[k]);
+        conv[k] = !(err[k] > thr[k]) ? 1 : 0;
     }
 }
 
@@ -644,6 +697,14 @@ NLAPI int NLAPIENTRY nlhipBatchGetCRS(nlhip_batch b, uint32_t* row_off, uint32_t
     });
 }
 
+NLAPI int NLAPIENTRY nlhipBatchFallbacks(nlhip_batch b, uint32_t* n) {
+    return guarded("nlhipBatchFallbacks", [&] {
+        Batch* B = bat(b);
+        std::lock_guard<std::mutex> lock(B->mu);
+        if (n) *n = B->last_fallbacks;
+    });
+}
+
 NLAPI int NLAPIENTRY nlhipBatchSetGraphs(nlhip_batch b, int on) {
     return guarded("nlhipBatchSetGraphs", [&] {
         Batch* B = bat(b);
@@ -874,6 +935,10 @@ NLAPI int NLAPIENTRY nlhipBatchSolve(nlhip_batch b, const double* rhs, double* x
             batch_solve_fp64(*B, p, its, cgrel, conv);
         } else if (p.precision == NLHIP_PREC_MIXED) {
             batch_solve_mixed(*B, p, its, cgrel, conv, outer);
+        } else if (p.precision == NLHIP_PREC_MIXED_FALLBACK) {
+            uint32_t nfb = 0;
+            batch_solve_mixed_fallback(*B, p, its, cgrel, conv, outer, nfb);
+            B->last_fallbacks = nfb;
         } else {
             throw Error(NLHIP_ERR_INVALID, "unknown precision");
         }

@@ -206,7 +206,7 @@ struct Server {
         int rc = nnz > 0xffffffffull ? NLHIP_ERR_UNSUPPORTED : NLHIP_OK;
         if (rc == NLHIP_OK)
             rc = nlhipBatchCreate(ns, row_off.data(), rowptr.data(), colind.data(), val.data(),
-                                  p.precision == NLHIP_PREC_MIXED ? NLHIP_SYSTEM_FP32 : 0u, &Bh);
+                                  p.precision != NLHIP_PREC_FP64 ? NLHIP_SYSTEM_FP32 : 0u, &Bh);
         if (rc == NLHIP_OK) rc = nlhipBatchSolve(Bh, b.data(), x.data(), &p, per.data(), nullptr);
         if (Bh) nlhipBatchDestroy(Bh);
         for (uint32_t i = 0; i < ns; ++i) {
@@ -294,7 +294,7 @@ NLAPI int NLAPIENTRY nlhipServerSubmit(nlhip_server h, uint32_t n, const uint32_
         else R->X.assign(size_t(n) * k, 0.0);
         nlhipSolveParamsDefault(&R->params);
         if (params) R->params = *params;
-        R->bytes = footprint(n, nnz, k, R->params.precision == NLHIP_PREC_MIXED);
+        R->bytes = footprint(n, nnz, k, R->params.precision != NLHIP_PREC_FP64);
         R->batchable = (k == 1 && n <= kBatchRows);
         std::lock_guard<std::mutex> lock(S->mu);
         if (S->stop) throw Error(NLHIP_ERR_INVALID, "server is shutting down");

@@ -54,6 +54,12 @@ NLAPI NLMatrix NLAPIENTRY nlHIPJacobiPreconditionerNewFromCRSMatrix(NLMatrix M);
 
 #define NLHIP_PREC_FP64  0   /* FP64 Jacobi-PCG                              */
 #define NLHIP_PREC_MIXED 1   /* FP32 inner PCG inside FP64 refinement        */
+#define NLHIP_PREC_MIXED_FALLBACK 2 /* batches: as MIXED, but a system whose
+                                       FP32 inner solve reaches inner_max_iter
+                                       without reaching inner_eps finishes
+                                       with FP64 CG from its current iterate
+                                       (LAPACK DSGESV's policy); no
+                                       counterpart in the reference */
 
 #define NLHIP_SYSTEM_FP32 1u /* also keep an FP32 copy of the values         */
 #define NLHIP_SYSTEM_NO_OFFSETS 2u /* read 32-bit column indices only (no
@@ -252,6 +258,8 @@ NLAPI int NLAPIENTRY nlhipBatchInfo(nlhip_batch b, uint32_t* nsys, uint32_t* n,
  * a caller inspects a batch the device assembled. */
 NLAPI int NLAPIENTRY nlhipBatchGetCRS(nlhip_batch b, uint32_t* row_off, uint32_t* rowptr,
                                       uint32_t* colind, double* val);
+/* Systems the last NLHIP_PREC_MIXED_FALLBACK solve finished in FP64. */
+NLAPI int NLAPIENTRY nlhipBatchFallbacks(nlhip_batch b, uint32_t* n);
 /* on = 0: solves enqueue every kernel launch directly instead of replaying
  * captured graphs (for tracing tools that cannot follow graph launches);
  * the same launches in the same order, so the same bits.  No counterpart in

@@ -142,6 +142,36 @@ def test_batch_mixed_precision(hip, small_atlas):
     assert tm.converged == 1
 
 
+def test_batch_mixed_with_fp64_fallback(hip, small_atlas):
+    """NLHIP_PREC_MIXED_FALLBACK: systems whose FP32 inner solve uses its
+    whole cap finish with FP64 CG; every system reaches 1e-10 in FP64
+    terms; with a cap no inner solve reaches, the result is the plain mixed
+    solve, bit for bit."""
+    A = small_atlas
+    B = capi.Batch([A.chart(c) for c in range(A.count)], fp32=True)
+    x64, _, _ = B.solve(A.b, eps=1e-10, max_iter=20000)
+    xf, pf, tf = B.solve(A.b, precision="mixed-fallback", eps=1e-10, max_iter=200000,
+                         inner_eps=1e-4, inner_max_iter=40)
+    nfb = B.fallbacks()
+    assert 0 < nfb <= A.count
+    for c, (a, b) in enumerate(zip(B.split(xf), B.split(x64))):
+        S = A.chart(c)
+        r = O.spmv(O.CRS(S.rowptr, S.colind, S.val), a) - S.b
+        assert np.linalg.norm(r) / np.linalg.norm(S.b) <= 1.01e-10, c
+        assert pf[c].converged == 1
+        assert rel(a, b) < 1e-6
+    assert tf.converged == 1
+    xm, _, _ = B.solve(A.b, precision="mixed", eps=1e-10, max_iter=200000, inner_eps=1e-4,
+                       inner_max_iter=100000)
+    xn, _, _ = B.solve(A.b, precision="mixed-fallback", eps=1e-10, max_iter=200000,
+                       inner_eps=1e-4, inner_max_iter=100000)
+    assert B.fallbacks() == 0
+    assert np.array_equal(xm.view(np.uint64), xn.view(np.uint64))
+    with pytest.raises(capi.NLHIPError):
+        B.solve(A.b, precision="mixed-fallback", inner_max_iter=0)
+    B.close()
+
+
 def test_batch_rejects_bad_input(hip):
     S1 = W.lscm_grid(6, seed=1)
     S2 = W.lscm_grid(7, seed=2)
