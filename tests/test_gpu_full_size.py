@@ -1,0 +1,64 @@
+"""The benchmark's own system at full size (LSCM normal equations of the
+2236 x 2236 height-field grid: 9,999,388 rows, 139,919,888 nonzeros):
+properties that do not depend on the size, checked where the oracle cannot
+solve the whole problem in seconds.
+
+* SpMV rows are bitwise the oracle's row loop (a sample of rows, first and
+  last ones included, through the oracle's own CRS SpMV on those rows);
+* 30 Jacobi-PCG iterations from x0 = 0 give the oracle's iterate (same
+  count, relative difference far below the 1e-8 bar);
+* the residual the solver reports is |b - A x| / |b| recomputed by the
+  oracle's SpMV.
+"""
+import numpy as np
+import pytest
+
+from graphitethree_amd import capi
+from graphitethree_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c2():
+    Sw = W.lscm_grid(2236, seed=1)
+    assert Sw.n == 9_999_388 and Sw.nnz == 139_919_888
+    return Sw
+
+
+def sub_rows(Sw, rows):
+    """The CRS of the given rows (all columns), for the oracle's SpMV."""
+    rp = Sw.rowptr.astype(np.int64)
+    lens = rp[rows + 1] - rp[rows]
+    sub_rp = np.zeros(len(rows) + 1, dtype=np.int64)
+    sub_rp[1:] = np.cumsum(lens)
+    idx = np.concatenate([np.arange(rp[r], rp[r + 1]) for r in rows])
+    return O.CRS(sub_rp.astype(np.uint32), Sw.colind[idx], Sw.val[idx], ncols=Sw.n)
+
+
+def test_spmv_rows_bitwise_at_full_size(hip, c2):
+    Sw = c2
+    S = capi.System.from_crs(Sw)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(Sw.n)
+    y = S.spmv(x)
+    rows = np.unique(np.concatenate([[0, 1, Sw.n - 2, Sw.n - 1],
+                                     rng.integers(0, Sw.n, 4000)]))
+    ref = O.spmv(sub_rows(Sw, rows), x)
+    assert np.array_equal(y[rows].view(np.uint64), ref.view(np.uint64))
+    S.close()
+
+
+def test_cg_iterates_and_residual_at_full_size(hip, c2):
+    Sw = c2
+    S = capi.System.from_crs(Sw)
+    x, st = S.solve(Sw.b, eps=1e-10, max_iter=30)
+    A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+    xr, it, _ = O.pcg(A, Sw.b, eps=1e-10, max_iter=30, variant=1)
+    assert int(st.iterations) == 30 and it == 30
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-10
+    r = Sw.b - O.spmv(A, x, parallel=True)
+    rel = np.linalg.norm(r) / np.linalg.norm(Sw.b)
+    assert abs(st.rel_residual - rel) <= 1e-6 * rel
+    S.close()
