@@ -1,7 +1,7 @@
-"""The benchmark's own system at full size (LSCM normal equations of the
-2236 x 2236 height-field grid: 9,999,388 rows, 139,919,888 nonzeros):
-properties that do not depend on the size, checked where the oracle cannot
-solve the whole problem in seconds.
+"""The benchmarks' own workloads at full size, where the oracle cannot solve
+the whole problem in seconds: properties that do not depend on the size.
+The LSCM normal equations of the 2236 x 2236 height-field grid (9,999,388
+rows, 139,919,888 nonzeros):
 
 * SpMV rows are bitwise the oracle's row loop (a sample of rows, first and
   last ones included, through the oracle's own CRS SpMV on those rows);
@@ -9,6 +9,7 @@ solve the whole problem in seconds.
   count, relative difference far below the 1e-8 bar);
 * the residual the solver reports is |b - A x| / |b| recomputed by the
   oracle's SpMV.
+The 2000-chart atlas: see test_atlas_at_full_size.
 """
 import numpy as np
 import pytest
@@ -62,3 +63,33 @@ def test_cg_iterates_and_residual_at_full_size(hip, c2):
     rel = np.linalg.norm(r) / np.linalg.norm(Sw.b)
     assert abs(st.rel_residual - rel) <= 1e-6 * rel
     S.close()
+
+
+def test_atlas_at_full_size(hip):
+    """The 2000-chart atlas of the batched benchmark (N ~ logU[50, 50000]):
+    the batch built on the device from the chart meshes is bitwise the host
+    packing, every chart reaches 1e-10 by the oracle's SpMV, and sampled
+    charts solved alone agree with their share of the batched solution."""
+    A = W.atlas(2000, seed=1, nmin=50, nmax=50000)
+    M = W.atlas_mesh(2000, seed=1, nmin=50, nmax=50000)
+    Bd, rhs = capi.Batch.lscm_atlas(M.vertex_off, M.points, M.triangles, M.pins, M.pin_uv)
+    ro, rp, ci, va = Bd.crs()
+    assert np.array_equal(ro, A.row_off) and np.array_equal(rp, A.rowptr)
+    assert np.array_equal(ci, A.colind)
+    assert np.array_equal(va.view(np.uint64), A.val.view(np.uint64))
+    assert np.array_equal(rhs.view(np.uint64), A.b.view(np.uint64))
+    del ro, rp, ci, va, M
+    x, per, tot = Bd.solve(rhs, eps=1e-10, max_iter=50000)
+    assert tot.converged == 1
+    r = A.b - O.spmv(O.CRS(A.rowptr, A.colind, A.val), x, parallel=True)
+    xs = Bd.split(x)
+    for c in range(A.count):
+        r0, r1 = int(A.row_off[c]), int(A.row_off[c + 1])
+        assert np.linalg.norm(r[r0:r1]) <= 1.01e-10 * np.linalg.norm(A.b[r0:r1]), c
+    for c in range(0, A.count, 250):
+        S = A.chart(c)
+        Sc = capi.System.from_crs(S)
+        xc, st = Sc.solve(S.b, eps=1e-10, max_iter=50000)
+        assert np.linalg.norm(xc - xs[c]) / np.linalg.norm(xc) <= 1e-8, c
+        Sc.close()
+    Bd.close()
