@@ -9,7 +9,7 @@ rows, 139,919,888 nonzeros):
   count, relative difference far below the 1e-8 bar);
 * the residual the solver reports is |b - A x| / |b| recomputed by the
   oracle's SpMV.
-The 2000-chart atlas: see test_atlas_at_full_size.
+The 2000-chart atlas and the 1500 x 1500 multi-RHS system: see their tests.
 """
 import numpy as np
 import pytest
@@ -93,3 +93,27 @@ def test_atlas_at_full_size(hip):
         assert np.linalg.norm(xc - xs[c]) / np.linalg.norm(xc) <= 1e-8, c
         Sc.close()
     Bd.close()
+
+
+def test_multi_rhs_at_full_size(hip):
+    """The multi-RHS stress system ((I + 1e-3 L) on a 1500 x 1500 grid, 32
+    N(0,1) right-hand sides): the block solve, FP64 and FP32-inner, reaches
+    1e-10 on every column by the oracle's SpMV and agrees with one FP64
+    solve per column."""
+    mesh = W.grid_mesh(1500, seed=1)
+    Sw = W.cotan_system(mesh, t=1e-3, nrhs=32, rhs="gaussian", seed=1)
+    del mesh
+    S = capi.System.from_crs(Sw, fp32=True)
+    A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+    X, per, tot = S.solve_multi(Sw.rhs, eps=1e-10, max_iter=5000)
+    Xm, perm, totm = S.solve_multi(Sw.rhs, precision="mixed", eps=1e-10, max_iter=5000,
+                                   inner_eps=1e-4)
+    for j in range(32):
+        b = Sw.rhs[j]
+        for Y in (X, Xm):
+            r = b - O.spmv(A, Y[j], parallel=True)
+            assert np.linalg.norm(r) <= 1.01e-10 * np.linalg.norm(b), j
+        if j % 8 == 0:
+            xj, _ = S.solve(b, eps=1e-10, max_iter=5000)
+            assert np.linalg.norm(X[j] - xj) / np.linalg.norm(xj) <= 1e-8, j
+    S.close()
