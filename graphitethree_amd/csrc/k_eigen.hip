@@ -79,62 +79,104 @@ __global__ __launch_bounds__(kBlock) void k_eig_gram(uint32_t n, const double* Y
             zs[rr][col] = in ? (mass ? mul_rn(mass[t0 + rr], Z[i]) : Z[i]) : 0.0;
         }
         __syncthreads();
+        if constexpr (P == 32) {
+            // entries (i0, j0), (i0, j0 + 16), (i0 + 16, j0), (i0 + 16, j0 + 16):
+            // four LDS reads per row for four products
+            const int i0 = threadIdx.x / 16, j0 = threadIdx.x % 16;
+            for (uint32_t rr = 0; rr < rows; ++rr) {
+                const double y0 = ys[rr][i0], y1 = ys[rr][i0 + 16];
+                const double z0 = zs[rr][j0], z1 = zs[rr][j0 + 16];
+                acc[0] = add_rn(acc[0], mul_rn(y0, z0));
+                acc[1] = add_rn(acc[1], mul_rn(y0, z1));
+                acc[2] = add_rn(acc[2], mul_rn(y1, z0));
+                acc[3] = add_rn(acc[3], mul_rn(y1, z1));
+            }
+        } else {
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int e = threadIdx.x + q * kBlock;
-            if (e < P * P) {
-                const int i = e / P, j = e % P;
-                double a = acc[q];
-                for (uint32_t rr = 0; rr < rows; ++rr) a = add_rn(a, mul_rn(ys[rr][i], zs[rr][j]));
-                acc[q] = a;
+            for (int q = 0; q < E; ++q) {
+                const int e = threadIdx.x + q * kBlock;
+                if (e < P * P) {
+                    const int i = e / P, j = e % P;
+                    double a = acc[q];
+                    for (uint32_t rr = 0; rr < rows; ++rr)
+                        a = add_rn(a, mul_rn(ys[rr][i], zs[rr][j]));
+                    acc[q] = a;
+                }
             }
         }
         __syncthreads();
     }
+    if constexpr (P == 32) {
+        const int i0 = threadIdx.x / 16, j0 = threadIdx.x % 16;
+        double* out = part + size_t(blockIdx.x) * P * P;
+        out[i0 * P + j0] = acc[0];
+        out[i0 * P + j0 + 16] = acc[1];
+        out[(i0 + 16) * P + j0] = acc[2];
+        out[(i0 + 16) * P + j0 + 16] = acc[3];
+    } else {
 #pragma unroll
-    for (int q = 0; q < E; ++q) {
-        const int e = threadIdx.x + q * kBlock;
-        if (e < P * P) part[size_t(blockIdx.x) * P * P + e] = acc[q];
+        for (int q = 0; q < E; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e < P * P) part[size_t(blockIdx.x) * P * P + e] = acc[q];
+        }
     }
 }
 
-// G[e] = sum over workgroups b (in order) of part[b][e]; the loads of 8
-// partials are issued before they are added.
-__global__ void k_eig_reduce(uint32_t nblk, uint32_t m, const double* part, double* G) {
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        uint32_t b = 0;
-        for (; b + 8 <= nblk; b += 8) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = part[size_t(b + u) * m + e];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) s = add_rn(s, v[u]);
+// G[e] = sum over workgroups b of part[b][e] in a fixed order: a workgroup
+// takes 16 consecutive entries; its thread (u, e) adds partials b = u, u + 16,
+// u + 32, ... in order (16 consecutive entries per load: coalesced), then
+// the 16 sums of an entry are added in u order.
+__global__ __launch_bounds__(kBlock) void k_eig_reduce(uint32_t nblk, uint32_t m,
+                                                       const double* part, double* G) {
+    __shared__ double lds[16][16];
+    const uint32_t el = threadIdx.x % 16, u = threadIdx.x / 16;
+    const uint32_t e = blockIdx.x * 16 + el;
+    double s = 0.0;
+    if (e < m) {
+        uint32_t b = u;
+        for (; b + 48 < nblk; b += 64) {
+            const double v0 = part[size_t(b) * m + e], v1 = part[size_t(b + 16) * m + e];
+            const double v2 = part[size_t(b + 32) * m + e], v3 = part[size_t(b + 48) * m + e];
+            s = add_rn(add_rn(add_rn(add_rn(s, v0), v1), v2), v3);
         }
-        for (; b < nblk; ++b) s = add_rn(s, part[size_t(b) * m + e]);
-        G[e] = s;
+        for (; b < nblk; b += 16) s = add_rn(s, part[size_t(b) * m + e]);
+    }
+    lds[u][el] = s;
+    __syncthreads();
+    if (u == 0 && e < m) {
+        double t = 0.0;
+        for (int q = 0; q < 16; ++q) t = add_rn(t, lds[q][el]);
+        G[e] = t;
     }
 }
 
 // X[r, j] = sum over i (in order) of Y[r, i] C[i][j]  (C row-major P x P).
-// A group of P lanes takes a row (lane j: column j, coalesced loads and
-// stores); Y[r, i] comes from lane i by a shuffle.
+// Lane j of a group of P lanes holds column j of C in registers; the
+// workgroup stages a tile of Y rows in LDS (coalesced), and each group reads
+// its row's Y[r, i] from LDS (one address for the group) and writes X[r, j]
+// (coalesced).
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_eig_combine(uint32_t n, const double* Y,
                                                         const double* C, double* X) {
-    __shared__ double cs[P * P];
-    for (int e = threadIdx.x; e < P * P; e += kBlock) cs[e] = C[e];
-    __syncthreads();
-    const uint32_t j = threadIdx.x % P;
-    constexpr uint32_t rows_per_block = kBlock / P;
-    for (uint32_t r0 = blockIdx.x * rows_per_block; r0 < n; r0 += gridDim.x * rows_per_block) {
-        const uint32_t r = r0 + threadIdx.x / P;
-        const bool in = r < n;
-        const double y = in ? Y[size_t(r) * P + j] : 0.0;
-        double s = 0.0;
+    constexpr uint32_t kTile = 64;  // rows staged at a time
+    __shared__ double ys[kTile * P];
+    const uint32_t j = threadIdx.x % P, g = threadIdx.x / P;
+    constexpr uint32_t groups = kBlock / P;
+    double c[P];
 #pragma unroll
-        for (int i = 0; i < P; ++i) s = add_rn(s, mul_rn(__shfl(y, i, P), cs[i * P + j]));
-        if (in) X[size_t(r) * P + j] = s;
+    for (int i = 0; i < P; ++i) c[i] = C[i * P + j];
+    for (uint32_t t0 = blockIdx.x * kTile; t0 < n; t0 += gridDim.x * kTile) {
+        const uint32_t rows = (t0 + kTile < n) ? kTile : n - t0;
+        __syncthreads();  // the previous tile has been consumed
+        for (uint32_t e = threadIdx.x; e < rows * P; e += kBlock) ys[e] = Y[size_t(t0) * P + e];
+        __syncthreads();
+        for (uint32_t rr = g; rr < rows; rr += groups) {
+            const double* y = ys + rr * P;
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < P; ++i) acc = add_rn(acc, mul_rn(y[i], c[i]));
+            X[size_t(t0 + rr) * P + j] = acc;
+        }
     }
 }
 
@@ -263,13 +305,14 @@ void eig_gram(uint32_t n, uint32_t p, const double* Y, const double* Z, const do
     const uint32_t nblk = (n + rpb - 1) / rpb;
     part.ensure(size_t(nblk) * p * p);
     NLHIP_EIG_P(p, (k_eig_gram<PP><<<nblk, kBlock, 0, s>>>(n, Y, Z, mass, rpb, part.get())));
-    k_eig_reduce<<<blocks_for(p * p), kBlock, 0, s>>>(nblk, p * p, part.get(), G);
+    k_eig_reduce<<<(p * p + 15) / 16, kBlock, 0, s>>>(nblk, p * p, part.get(), G);
     NLHIP_CHECK(hipGetLastError());
 }
 
 void eig_combine(uint32_t n, uint32_t p, const double* Y, const double* C, double* X,
                  hipStream_t s) {
-    const uint32_t g = blocks_for(n) < 4096 ? blocks_for(n) : 4096;
+    const uint32_t tiles = (n + 63) / 64;  // k_eig_combine's tiles of 64 rows
+    const uint32_t g = tiles < 2048 ? (tiles ? tiles : 1) : 2048;
     NLHIP_EIG_P(p, (k_eig_combine<PP><<<g, kBlock, 0, s>>>(n, Y, C, X)));
     NLHIP_CHECK(hipGetLastError());
 }
@@ -281,7 +324,7 @@ void eig_residual(uint32_t n, uint32_t p, const double* AX, const double* X, con
     part.ensure(size_t(nblk) * 3 * p);
     NLHIP_EIG_P(p, (k_eig_residual<PP><<<nblk, kBlock, 0, s>>>(n, AX, X, mass, theta, rpb,
                                                                 part.get())));
-    k_eig_reduce<<<blocks_for(3 * p), kBlock, 0, s>>>(nblk, 3 * p, part.get(), out);
+    k_eig_reduce<<<(3 * p + 15) / 16, kBlock, 0, s>>>(nblk, 3 * p, part.get(), out);
     NLHIP_CHECK(hipGetLastError());
 }
 
