@@ -475,6 +475,7 @@ void build_batch_blocks(Batch& B, const uint32_t* row_off, const uint32_t* rowpt
 }
 
 void batch_finish(Batch& B, const std::vector<int32_t>& blk, uint32_t flags);
+bool build_batch_blocks_device(Batch& B, std::vector<int32_t>& blk);
 
 Batch* batch_create(uint32_t nsys, const uint32_t* row_off, const uint32_t* rowptr,
                     const uint32_t* colind, const double* val, uint32_t flags) {
@@ -522,7 +523,9 @@ Batch* batch_create(uint32_t nsys, const uint32_t* row_off, const uint32_t* rowp
     std::vector<int32_t> blk(nsys, 0);
     if (!(flags & NLHIP_SYSTEM_NO_OFFSETS)) {
         build_col_offsets(nnz, B->colind.get(), st, B->ci16, B->segb);
-        if (!(flags & NLHIP_SYSTEM_NO_BLOCKS))
+        // every system in block form: checked and packed on the device for
+        // the whole matrix at once; otherwise system by system on the host
+        if (!(flags & NLHIP_SYSTEM_NO_BLOCKS) && !build_batch_blocks_device(*B, blk))
             build_batch_blocks(*B, row_off, rowptr, colind, val, blk);
     }
     batch_finish(*B, blk, flags);
