@@ -3,9 +3,15 @@
  *
  * A plain-C, single-threaded (optionally OpenMP) restatement of the OpenNL
  * solve path that Graphite calls through its OGF::NL wrapper:
- * CRS storage and SpMV, the NLBlas level-1 routines, the Jacobi
- * preconditioner, Jacobi-preconditioned CG, BiCGSTAB, and the least-squares
- * context API (normal equations with locked variables moved to the RHS).
+ * CRS storage and SpMV, the NLBlas level-1 routines, the Jacobi and SSOR
+ * preconditioners, preconditioned CG, BiCGSTAB and GMRES(m), the dynamic
+ * matrix, and the least-squares context API (normal equations with locked
+ * variables moved to the RHS).
+ *
+ * Parity unpinned against OpenNL itself (its source is absent; each is
+ * pinned by the mathematics instead, see nl_oracle.c): the Jacobi fallback
+ * for a zero diagonal, GMRES's use of the preconditioner, and the summation
+ * order inside OpenNL's SSOR sweeps.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load this library, and only as the checker / CPU baseline.  The product
