@@ -177,3 +177,15 @@ def test_context_rows_then_ij_coefficients_accumulate_on_host():
     L.nlGetIntegerv(capi.NL_NNZ, C.byref(v))
     assert v.value == 1   # one free-free entry (1,1)
     L.nlDeleteContext(ctx)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N > 1 exits with a message before touching a
+    device (this build runs on one GPU)."""
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode != 0
+    assert "one GPU" in (p.stdout + p.stderr)
