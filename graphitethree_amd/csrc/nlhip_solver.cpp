@@ -353,7 +353,12 @@ void solve_fp64(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st) {
     st.converged = !(fin.err > fin.thresh) ? 1 : 0;
 }
 
-void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st) {
+// FP32 inner PCG inside FP64 refinement.  fallback (NLHIP_PREC_MIXED_
+// FALLBACK): an inner solve that uses its whole cap (inner_max_iter) ends
+// the refinement and the solve finishes with FP64 CG from the current
+// iterate (LAPACK DSGESV's policy).
+void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st,
+                 bool fallback = false) {
     S.ensure_fp32();
     CGWorkspace<double>& w64 = S.ws64;
     CGWorkspace<float>& w32 = S.ws32;
@@ -381,6 +386,17 @@ void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st) 
         NLHIP_CHECK(ir_update(S.n, w64.x.get(), w32.x.get(), rn, s));
         ++outer;
         if (fin.iters == 0) break;
+        if (fallback && uint32_t(fin.iters) >= cap && total < p.max_iter) {
+            nlhip_solve_params p64 = p;
+            p64.max_iter = p.max_iter - total;
+            nlhip_solve_stats s64{};
+            solve_fp64(S, p64, s64);
+            st.iterations = total + s64.iterations;
+            st.outer_iterations = outer;
+            st.cg_rel_residual = s64.cg_rel_residual;
+            st.converged = s64.converged;
+            return;
+        }
     }
     st.iterations = total;
     st.outer_iterations = outer;
@@ -394,8 +410,11 @@ void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
     const auto t0 = std::chrono::steady_clock::now();
     std::memset(&st, 0, sizeof(st));
     if (take_injected_failure()) throw Error(NLHIP_ERR_INJECTED, "injected failure");
-    if (p.precision != NLHIP_PREC_FP64 && p.precision != NLHIP_PREC_MIXED)
+    if (p.precision != NLHIP_PREC_FP64 && p.precision != NLHIP_PREC_MIXED &&
+        p.precision != NLHIP_PREC_MIXED_FALLBACK)
         throw Error(NLHIP_ERR_INVALID, "unknown precision");
+    if (p.precision == NLHIP_PREC_MIXED_FALLBACK && p.inner_max_iter == 0)
+        throw Error(NLHIP_ERR_INVALID, "the fallback needs inner_max_iter > 0");
     if (p.precond != NL_PRECOND_JACOBI && p.precond != NL_PRECOND_NONE)
         throw Error(NLHIP_ERR_UNSUPPORTED, "only Jacobi or no preconditioner on the GPU");
     S.ws64.ensure(S.n, S.grid);
@@ -405,7 +424,7 @@ void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
     Event e0, e1;
     e0.record(s);
     if (p.precision == NLHIP_PREC_FP64) solve_fp64(S, p, st);
-    else solve_mixed(S, p, st);
+    else solve_mixed(S, p, st, p.precision == NLHIP_PREC_MIXED_FALLBACK);
     e1.record(s);
     e1.sync();
     st.device_seconds = 1e-3 * elapsed_ms(e0, e1);

@@ -54,11 +54,12 @@ NLAPI NLMatrix NLAPIENTRY nlHIPJacobiPreconditionerNewFromCRSMatrix(NLMatrix M);
 
 #define NLHIP_PREC_FP64  0   /* FP64 Jacobi-PCG                              */
 #define NLHIP_PREC_MIXED 1   /* FP32 inner PCG inside FP64 refinement        */
-#define NLHIP_PREC_MIXED_FALLBACK 2 /* batches: as MIXED, but a system whose
-                                       FP32 inner solve reaches inner_max_iter
-                                       without reaching inner_eps finishes
-                                       with FP64 CG from its current iterate
-                                       (LAPACK DSGESV's policy); no
+#define NLHIP_PREC_MIXED_FALLBACK 2 /* as MIXED, but a system whose FP32
+                                       inner solve reaches inner_max_iter
+                                       (> 0) without reaching inner_eps
+                                       finishes with FP64 CG from its current
+                                       iterate (LAPACK DSGESV's policy);
+                                       single systems and batches; no
                                        counterpart in the reference */
 
 #define NLHIP_SYSTEM_FP32 1u /* also keep an FP32 copy of the values         */

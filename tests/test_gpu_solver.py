@@ -156,6 +156,31 @@ def test_mixed_precision_refinement(hip):
     assert rel(xm, x64) < 1e-5
 
 
+def test_mixed_with_fp64_fallback(hip):
+    """NLHIP_PREC_MIXED_FALLBACK on one system: an FP32 inner solve that uses
+    its whole cap hands over to FP64 CG from the current iterate; a cap no
+    inner solve reaches gives the plain mixed solve, bit for bit."""
+    Sw = W.lscm_grid(30, seed=2)
+    S = capi.System.from_crs(Sw, fp32=True)
+    x64, _ = S.solve(Sw.b, eps=1e-10, max_iter=20000)
+    xf, sf = S.solve(Sw.b, precision="mixed-fallback", eps=1e-10, max_iter=200000,
+                     inner_eps=1e-4, inner_max_iter=20)
+    assert sf.converged == 1 and sf.rel_residual <= 1.01e-10
+    assert sf.outer_iterations == 1           # the first inner solve hit its cap
+    A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+    r = O.spmv(A, xf) - Sw.b
+    assert np.linalg.norm(r) / np.linalg.norm(Sw.b) <= 1.01e-10
+    assert rel(xf, x64) < 1e-6
+    xm, _ = S.solve(Sw.b, precision="mixed", eps=1e-10, max_iter=200000, inner_eps=1e-4,
+                    inner_max_iter=100000)
+    xn, sn = S.solve(Sw.b, precision="mixed-fallback", eps=1e-10, max_iter=200000,
+                     inner_eps=1e-4, inner_max_iter=100000)
+    assert np.array_equal(xm.view(np.uint64), xn.view(np.uint64))
+    with pytest.raises(capi.NLHIPError):
+        S.solve(Sw.b, precision="mixed-fallback", inner_max_iter=0)
+    S.close()
+
+
 def test_solve_system_iterative_host_api(hip):
     L = hip
     Sw = W.lscm_grid(25, seed=6)
