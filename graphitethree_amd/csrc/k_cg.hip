@@ -128,7 +128,24 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(CGArgs<T> a) {
         for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock)
             a.x[i] = add_rn(a.x[i], mul_rn(-alpha, a.d[i]));
     } else {
-        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+        // four rows per thread at a time: every load before any store (the
+        // arrays may alias as far as the compiler knows)
+        uint32_t i = r0 + threadIdx.x;
+        for (; i + 3 * kBlock < r1; i += 4 * kBlock) {
+            T dv[4], xv[4], hv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                dv[q] = a.d[i + q * kBlock];
+                xv[q] = a.x[i + q * kBlock];
+                hv[q] = a.h[i + q * kBlock];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a.x[i + q * kBlock] = add_rn(xv[q], mul_rn(-alpha, dv[q]));
+                a.d[i + q * kBlock] = add_rn(mul_rn(beta, dv[q]), hv[q]);
+            }
+        }
+        for (; i < r1; i += kBlock) {
             const T di = a.d[i];
             a.x[i] = add_rn(a.x[i], mul_rn(-alpha, di));
             a.d[i] = add_rn(mul_rn(beta, di), a.h[i]);
@@ -180,7 +197,28 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CGArgs<T> a) {
     uint32_t r0, r1;
     block_row_range(a.m, a.chunk, L, r0, r1);
     T prh = T(0), prr = T(0);
-    for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+    // four rows per thread at a time, loads first; the partial sums take the
+    // rows in the same order as one at a time
+    uint32_t i = r0 + threadIdx.x;
+    for (; i + 3 * kBlock < r1; i += 4 * kBlock) {
+        T rv[4], qv[4], dv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            rv[q] = a.r[i + q * kBlock];
+            qv[q] = a.q[i + q * kBlock];
+            dv[q] = PREC ? a.dinv[i + q * kBlock] : T(1);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const T ri = add_rn(rv[q], mul_rn(-alpha, qv[q]));
+            const T hi = PREC ? mul_rn(dv[q], ri) : ri;
+            a.r[i + q * kBlock] = ri;
+            a.h[i + q * kBlock] = hi;
+            prh += ri * hi;
+            prr += ri * ri;
+        }
+    }
+    for (; i < r1; i += kBlock) {
         const T ri = add_rn(a.r[i], mul_rn(-alpha, a.q[i]));
         const T hi = PREC ? mul_rn(a.dinv[i], ri) : ri;
         a.r[i] = ri;
