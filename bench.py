@@ -352,7 +352,11 @@ def main():
                    "solver": "Jacobi-PCG fp64", "eps": args.eps, "max_iter": args.max_iter,
                    "iterations": iters, "converged": bool(stats.converged),
                    "rel_residual": stats.rel_residual, "rows_per_wave": lpr,
-                   "workgroups": wg, "autotuned": tuned is not None},
+                   "workgroups": wg, "autotuned": tuned is not None,
+                   "step": "one Jacobi-PCG solve from x0 = 0 with the stopping rule "
+                           "|Ax - b| <= eps |b| or max_iter iterations (the 317 x 317 "
+                           "configuration's cap; this system needs far more to reach eps, "
+                           "so every step runs the full max_iter iterations)"},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "spmv_gbs": {"fp64": spmv_gbs, "fp32": spmv32_gbs,
                      "bytes_fp64": spmv_bytes(n, nnz)},
