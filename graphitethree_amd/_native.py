@@ -7,7 +7,10 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+# GRAPHITETHREE_AMD_LIB_DIR points at another build of the same libraries
+# (the sanitizer build the CPU tests run against).
+LIB_DIR = os.environ.get("GRAPHITETHREE_AMD_LIB_DIR") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "lib")
 _loaded: dict[str, C.CDLL] = {}
 
 

@@ -2,8 +2,13 @@
 
 * libnlworkloads.so  host C++ (g++, OpenMP): synthetic LSCM / cotan systems.
 * libnlhip.so        the product: OpenNL C API + HIP kernels for gfx950.
+* --sanitize: a copy of libnlhip.so whose host C++ objects (the C ABI, the
+  dynamic matrix, the context, the solve drivers, the server) are built with
+  AddressSanitizer and UndefinedBehaviorSanitizer, linked with the normal
+  device objects, into _build/sanitize/ (tests/test_sanitizers.py runs the
+  CPU tests against it).
 
-Usage: python -m graphitethree_amd.build [--only workloads|nlhip]
+Usage: python -m graphitethree_amd.build [--only workloads|nlhip] [--sanitize]
 """
 from __future__ import annotations
 
@@ -82,6 +87,43 @@ def build_nlhip(jobs: int = 8):
     return out
 
 
+SANITIZE_DIR = os.path.join(ROOT, "_build", "sanitize")
+SANITIZE_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                  "-fno-omit-frame-pointer"]
+
+
+def build_sanitized(jobs: int = 8):
+    """libnlhip.so with ASan + UBSan host objects (g++ only; the device
+    objects of the normal build are linked in unchanged), and
+    libnlworkloads.so with the same instrumentation.  Returns the path of
+    libnlhip.so."""
+    for s in NLHIP_HIP:
+        if not os.path.exists(os.path.join(OBJ, s + ".o")):
+            build_nlhip(jobs)
+            break
+    obj = os.path.join(SANITIZE_DIR, "obj")
+    os.makedirs(obj, exist_ok=True)
+    flags = [f for f in HOST_FLAGS if f != "-O3"] + ["-O1", "-g"] + SANITIZE_FLAGS
+
+    def compile_one(s):
+        o = os.path.join(obj, s + ".o")
+        _run(["g++", *flags, "-fopenmp", "-c", os.path.join(CSRC, s), "-o", o])
+        return o
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, NLHIP_CPP))
+    wl = os.path.join(SANITIZE_DIR, "libnlworkloads.so")
+    _run(["g++", "-O1", "-g", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off",
+          *SANITIZE_FLAGS, "-shared", "-o", wl + ".tmp", os.path.join(CSRC, "workloads.cpp")])
+    os.replace(wl + ".tmp", wl)
+    out = os.path.join(SANITIZE_DIR, "libnlhip.so")
+    _run(["g++", "-shared", "-o", out + ".tmp", *objs,
+          *[os.path.join(OBJ, s + ".o") for s in NLHIP_HIP], *SANITIZE_FLAGS, "-fopenmp",
+          "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
     build_workloads()
     build_nlhip()
@@ -90,7 +132,11 @@ def build_all():
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", choices=["workloads", "nlhip"])
+    ap.add_argument("--sanitize", action="store_true")
     a = ap.parse_args()
+    if a.sanitize:
+        print(build_sanitized())
+        raise SystemExit(0)
     if a.only in (None, "workloads"):
         print(build_workloads())
     if a.only in (None, "nlhip"):

@@ -75,14 +75,15 @@ class SolveParams(C.Structure):
     _fields_ = [("precision", C.c_int32), ("precond", C.c_int32), ("eps", C.c_double),
                 ("max_iter", C.c_uint32), ("check_every", C.c_uint32),
                 ("inner_eps", C.c_double), ("max_outer", C.c_uint32),
-                ("inner_max_iter", C.c_uint32)]
+                ("inner_max_iter", C.c_uint32), ("confirm", C.c_uint32)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("outer_iterations", C.c_uint32),
                 ("converged", C.c_int32), ("status", C.c_int32),
                 ("rel_residual", C.c_double), ("cg_rel_residual", C.c_double),
-                ("device_seconds", C.c_double), ("wall_seconds", C.c_double)]
+                ("device_seconds", C.c_double), ("wall_seconds", C.c_double),
+                ("extra_iterations", C.c_uint32), ("reserved", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -190,6 +191,10 @@ SIGNATURES = {
     "nlhipBatchAutotune": (C.c_int, [_vp, _vp, C.c_uint32, _u32p, _u32p, _f64p]),
     "nlhipSolve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
                              C.POINTER(SolveStats)]),
+    "nlhipSolveBegin": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams)]),
+    "nlhipSolveAdvance": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_int),
+                                    C.POINTER(SolveStats)]),
+    "nlhipSolveEnd": (C.c_int, [_vp, _vp, C.POINTER(SolveStats)]),
     "nlhipSolveMulti": (C.c_int, [_vp, C.c_uint32, _vp, _vp, C.POINTER(SolveParams),
                                   C.POINTER(SolveStats), C.POINTER(SolveStats)]),
     "nlhipSpMV": (C.c_int, [_vp, _vp, _vp]),
@@ -414,6 +419,31 @@ class System:
         p = default_params(**params)
         st = SolveStats()
         check(lib().nlhipSolve(self._h, ptr(b), ptr(x), C.byref(p), C.byref(st)), "nlhipSolve")
+        return x, st
+
+    def solve_begin(self, b, x0=None, **params):
+        """Opens a sliced FP64 solve (nlhipSolveBegin); advance it with
+        solve_advance and close it with solve_end."""
+        self._sliced_b = np.ascontiguousarray(b, dtype=np.float64)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        p = default_params(**params)
+        check(lib().nlhipSolveBegin(self._h, ptr(self._sliced_b),
+                                    None if x0 is None else ptr(x0), C.byref(p)),
+              "nlhipSolveBegin")
+
+    def solve_advance(self, iters: int):
+        """At most `iters` more iterations; returns (finished, stats so far)."""
+        fin = C.c_int()
+        st = SolveStats()
+        check(lib().nlhipSolveAdvance(self._h, int(iters), C.byref(fin), C.byref(st)),
+              "nlhipSolveAdvance")
+        return bool(fin.value), st
+
+    def solve_end(self):
+        """Closes the sliced solve; returns (x, stats)."""
+        x = np.empty(self.n)
+        st = SolveStats()
+        check(lib().nlhipSolveEnd(self._h, ptr(x), C.byref(st)), "nlhipSolveEnd")
         return x, st
 
     def solve_multi(self, B, X0=None, **params):

@@ -237,7 +237,8 @@ __global__ __launch_bounds__(kBlock) void k_b_sys_init(BatchArgs<T> a, T eps, in
         a.iters[s] = 0;
         a.stop[s] = 0;
         const bool skip = a.skip && a.skip[s];
-        a.done[s] = (skip || !(err > thresh) || max_iter <= 0) ? 1 : 0;
+        const int cap = a.cap && a.cap[s] < max_iter ? a.cap[s] : max_iter;
+        a.done[s] = (skip || !(err > thresh) || cap <= 0) ? 1 : 0;
     }
 }
 
@@ -287,7 +288,9 @@ __global__ __launch_bounds__(kBlock) void k_b_dir(BatchArgs<T> a, int max_iter) 
                 if (it > 0) {
                     rho = tiles_sum(a.part_rh, td.to0, td.to1);
                     err = tiles_sum(a.part_rr, td.to0, td.to1);
-                    stop = !(err > a.thresh[td.s]) || it >= max_iter;
+                    const int cap =
+                        a.cap && a.cap[td.s] < max_iter ? a.cap[td.s] : max_iter;
+                    stop = !(err > a.thresh[td.s]) || it >= cap;
                     alpha = a.alpha[td.s];
                     beta = rho / a.rho_prev[td.s];
                 }

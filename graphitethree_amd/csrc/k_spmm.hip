@@ -212,6 +212,25 @@ __global__ __launch_bounds__(kBlock) void k_m_reduce_beta(MultiArgs<T> a, int ma
     }
 }
 
+// Re-opens the columns flagged in `again` (their loop stopped on the
+// stopping test; the host has lowered their thresholds): the reduction the
+// stopping launch skipped (rho_new from the r.h partials, beta = rho_new /
+// rho, rho = rho_new) and done = 0.  The next launches continue those
+// columns exactly as if their threshold had been the lower one from the
+// start.  k_m_count follows.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_m_reopen(MultiArgs<T> a, const int32_t* again) {
+    __shared__ T lds[kWavesPerBlock];
+    const uint32_t j = blockIdx.x;
+    if (!again[j]) return;  // block-uniform
+    const T rho_new = col_sum(a.part_rh, a.P, a.k, j, lds);
+    if (threadIdx.x == 0) {
+        a.beta[j] = rho_new / a.rho[j];
+        a.rho[j] = rho_new;
+        a.done[j] = 0;
+    }
+}
+
 // active column count -> *a.n_active (single thread, after reduce_beta)
 template <typename T>
 __global__ void k_m_count(MultiArgs<T> a) {
@@ -402,6 +421,12 @@ hipError_t multi_layout(uint32_t n, uint32_t k, const T* src, T* dst, bool to_in
     const uint32_t g = ew_grid_m(uint64_t(n) * G);
     if (to_interleaved) k_to_interleaved<T><<<g, kBlock, 0, s>>>(n, k, G, src, dst);
     else k_to_colmajor<T><<<g, kBlock, 0, s>>>(n, k, G, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t multi_reopen(const MultiArgs<double>& a, const int32_t* again, hipStream_t s) {
+    k_m_reopen<double><<<a.k, kBlock, 0, s>>>(a, again);
+    k_m_count<double><<<1, 64, 0, s>>>(a);
     return hipGetLastError();
 }
 

@@ -1116,14 +1116,7 @@ NLAPI NLuint NLAPIENTRY nlSolveSystemIterative(NLBlas_t blas, NLMatrix M, NLMatr
             p.max_iter = max_iter;
             p.precond = dinv ? NL_PRECOND_JACOBI : NL_PRECOND_NONE;
             nlhip_solve_stats st;
-            S->dinv_override = dinv;
-            try {
-                solve(*S, b, x, p, st);
-            } catch (...) {
-                S->dinv_override = nullptr;
-                throw;
-            }
-            S->dinv_override = nullptr;
+            solve(*S, b, x, p, st, dinv);
             its = st.iterations;
             if (blas) blas->flops += NLulong(its) * (2 * S->nnz + 12 * NLulong(S->n));
         } else if (solver == NL_BICGSTAB) {

@@ -19,7 +19,8 @@ namespace nlhip {
 template <typename T>
 struct BatchWork {
     DevBuf<T> x, b, r, h, d, q, part, sys;  // part: 4 x ntiles; sys: 7 x nsys
-    DevBuf<int32_t> iters, done, stop, skip;
+    DevBuf<int32_t> iters, done, stop, skip, cap;
+    DevBuf<T> chk;                          // 2 x nsys: |b - A x|^2, |b|^2 (confirm)
     DevBuf<uint32_t> lists, counts;         // act_tiles | act_sys ; 2 counters
     PinnedBuf<uint32_t> host_counts;        // two slots
     GraphLadder graph;
@@ -88,6 +89,8 @@ void work_alloc(Batch& B, BatchWork<T>& w) {
     w.done.alloc(B.nsys);
     w.stop.alloc(B.nsys);
     w.skip.alloc(B.nsys);
+    w.cap.alloc(B.nsys);
+    w.chk.alloc(2 * size_t(B.nsys));
     w.lists.alloc(size_t(B.ntiles) + B.nsys);
     w.counts.alloc(2);
     w.host_counts.ensure(2);
@@ -133,18 +136,20 @@ void work_alloc(Batch& B, BatchWork<T>& w) {
     a.done = w.done.get();
     a.stop = w.stop.get();
     a.skip = nullptr;
+    a.cap = nullptr;
     a.act_tiles = w.lists.get();
     a.act_sys = w.lists.get() + B.ntiles;
     a.n_act_tiles = w.counts.get();
     a.n_act_sys = w.counts.get() + 1;
 }
 
-// Runs the batched PCG (arguments set, b and x filled) until no system is
-// active.  Graphs hold K iterations and a list rebuild.
+// Runs the batched PCG (started, or re-opened with its lists rebuilt) until
+// no system is active.  Graphs hold K iterations and a list rebuild;
+// `budget` bounds the iterations any system can still run.
 template <typename T>
-void run_batch(Batch& B, BatchWork<T>& w, bool precond, T eps, int max_iter, uint32_t K) {
+void batch_loop(Batch& B, BatchWork<T>& w, bool precond, int max_iter, uint32_t K,
+                uint64_t budget) {
     hipStream_t s = B.stream.get();
-    NLHIP_CHECK(batch_start<T>(w.args, precond, eps, max_iter, B.grid, s));
     if (w.graph_precond != precond || w.graph_max_iter != max_iter ||
         std::memcmp(&w.graph_args, &w.args, sizeof(BatchArgs<T>)) != 0)
         w.graph.reset();
@@ -158,7 +163,7 @@ void run_batch(Batch& B, BatchWork<T>& w, bool precond, T eps, int max_iter, uin
     w.graph_args = w.args;
     Event ev[2];
     uint32_t* host = w.host_counts.get();
-    const uint64_t limit = uint64_t(std::max(max_iter, 0)) + 3ull * w.graph.cap() + 8;
+    const uint64_t limit = budget + 3ull * w.graph.cap() + 8;
     uint64_t launched = 0, covered = 0;
     for (;;) {
         if (B.graphs) {
@@ -188,6 +193,12 @@ void run_batch(Batch& B, BatchWork<T>& w, bool precond, T eps, int max_iter, uin
     B.stream.sync();
 }
 
+template <typename T>
+void run_batch(Batch& B, BatchWork<T>& w, bool precond, T eps, int max_iter, uint32_t K) {
+    NLHIP_CHECK(batch_start<T>(w.args, precond, eps, max_iter, B.grid, B.stream.get()));
+    batch_loop<T>(B, w, precond, max_iter, K, uint64_t(std::max(max_iter, 0)));
+}
+
 uint32_t batch_check_every(const Batch& B) {
     const double bytes = 12.0 * double(B.nnz) + 108.0 * double(B.n);
     const double iter_s = std::max(bytes / 4.0e12, 3.0e-5);
@@ -209,16 +220,18 @@ void ensure_fp32(Batch& B) {
 
 void fill_stats(Batch& B, const std::vector<int32_t>& its, const std::vector<double>& rel,
                 const std::vector<double>& cgrel, const std::vector<uint8_t>& conv,
-                const std::vector<uint32_t>* outer, nlhip_solve_stats* per,
-                nlhip_solve_stats* total, double dev_s, double wall_s) {
+                const std::vector<uint32_t>* outer, const std::vector<uint32_t>& extra,
+                nlhip_solve_stats* per, nlhip_solve_stats* total, double dev_s, double wall_s) {
     uint64_t it_sum = 0;
-    uint32_t it_max = 0, nconv = 0;
+    uint32_t it_max = 0, nconv = 0, extra_max = 0;
     double worst = 0.0;
     for (uint32_t s = 0; s < B.nsys; ++s) {
         it_sum += uint64_t(its[s]);
         it_max = std::max(it_max, uint32_t(its[s]));
         nconv += conv[s];
         worst = std::max(worst, rel[s]);
+        const uint32_t ex = extra.empty() ? 0u : extra[s];
+        extra_max = std::max(extra_max, ex);
         if (per) {
             nlhip_solve_stats& st = per[s];
             std::memset(&st, 0, sizeof(st));
@@ -229,6 +242,7 @@ void fill_stats(Batch& B, const std::vector<int32_t>& its, const std::vector<dou
             st.cg_rel_residual = cgrel[s];
             st.device_seconds = dev_s;
             st.wall_seconds = wall_s;
+            st.extra_iterations = ex;
         }
     }
     if (total) {
@@ -239,6 +253,7 @@ void fill_stats(Batch& B, const std::vector<int32_t>& its, const std::vector<dou
         total->rel_residual = worst;
         total->device_seconds = dev_s;
         total->wall_seconds = wall_s;
+        total->extra_iterations = extra_max;
     }
 }
 
@@ -262,32 +277,138 @@ void true_residuals(Batch& B, std::vector<double>& rel, std::vector<double>* rr_
     if (bb_out) *bb_out = bb;
 }
 
-void batch_solve_fp64(Batch& B, const nlhip_solve_params& p, std::vector<int32_t>& its,
-                      std::vector<double>& cgrel, std::vector<uint8_t>& conv) {
+// |b - A x|^2 and |b|^2 per system without touching what a stopped FP64 CG
+// needs to continue (r, the r.h / r.r tile partials, the per-system
+// scalars): the residual goes to q and the d.q partials, the norms to chk.
+void confirm_norms(Batch& B, std::vector<double>& rr, std::vector<double>& bb) {
     BatchWork<double>& w = B.w64;
+    BatchArgs<double> a = w.args;
+    a.val = B.val.get();
+    a.r = w.q.get();
+    a.part_rr = a.part_dq;
+    a.err = w.chk.get();
+    a.bb = w.chk.get() + B.nsys;
+    NLHIP_CHECK(batch_residual64(a, B.grid, B.stream.get()));
+    rr.resize(B.nsys);
+    bb.resize(B.nsys);
+    NLHIP_CHECK(hipMemcpyAsync(rr.data(), a.err, B.nsys * 8, hipMemcpyDeviceToHost,
+                               B.stream.get()));
+    NLHIP_CHECK(hipMemcpyAsync(bb.data(), a.bb, B.nsys * 8, hipMemcpyDeviceToHost,
+                               B.stream.get()));
+    B.stream.sync();
+}
+
+constexpr int kMaxConfirmRounds = 8;
+
+// FP64 batched PCG from the current x of every system that skip (host, may
+// be null) does not exclude, each system stopping at its own cap (host, may
+// be null: p.max_iter).  With p.confirm, a system whose recursive residual
+// met eps but whose true residual |b - A x| did not continues the same
+// recurrence (threshold scaled down by the ratio measured, alpha = 0, its
+// loop re-opened) until the true residual meets eps, its cap is reached or
+// the true residual stops decreasing; the other systems stay stopped, so
+// each system's iterates are those of its solve alone.  Results for the
+// systems run: its, cgrel (recursive), conv, extra (iterations after the
+// recursive test first held).
+void batch_fp64(Batch& B, const nlhip_solve_params& p, const std::vector<int32_t>* skip,
+                const std::vector<int32_t>* cap, std::vector<int32_t>& its,
+                std::vector<double>& cgrel, std::vector<uint8_t>& conv,
+                std::vector<uint32_t>& extra) {
+    BatchWork<double>& w = B.w64;
+    hipStream_t s = B.stream.get();
+    const uint32_t ns = B.nsys;
     const bool precond = p.precond == NL_PRECOND_JACOBI;
     w.args.val = B.val.get();
     w.args.dinv = precond ? B.dinv.get() : nullptr;
     w.args.skip = nullptr;
+    w.args.cap = nullptr;
+    if (skip) {
+        NLHIP_CHECK(hipMemcpyAsync(w.skip.get(), skip->data(), ns * 4, hipMemcpyHostToDevice, s));
+        w.args.skip = w.skip.get();
+    }
+    if (cap) {
+        NLHIP_CHECK(hipMemcpyAsync(w.cap.get(), cap->data(), ns * 4, hipMemcpyHostToDevice, s));
+        w.args.cap = w.cap.get();
+    }
     const uint32_t K = p.check_every ? p.check_every : batch_check_every(B);
     run_batch<double>(B, w, precond, p.eps, int(p.max_iter), K);
-    std::vector<double> err(B.nsys), thr(B.nsys), bb(B.nsys);
-    its.resize(B.nsys);
-    hipStream_t s = B.stream.get();
-    NLHIP_CHECK(hipMemcpyAsync(its.data(), w.iters.get(), B.nsys * 4, hipMemcpyDeviceToHost, s));
-    NLHIP_CHECK(hipMemcpyAsync(err.data(), w.args.err, B.nsys * 8, hipMemcpyDeviceToHost, s));
-    NLHIP_CHECK(hipMemcpyAsync(thr.data(), w.args.thresh, B.nsys * 8, hipMemcpyDeviceToHost, s));
-    NLHIP_CHECK(hipMemcpyAsync(bb.data(), w.args.bb, B.nsys * 8, hipMemcpyDeviceToHost, s));
-    B.stream.sync();
-    cgrel.resize(B.nsys);
-    conv.resize(B.nsys);
-    for (uint32_t k = 0; k < B.nsys; ++k) {
+    std::vector<double> err(ns), thr(ns), bb(ns);
+    its.resize(ns);
+    auto read = [&] {
+        NLHIP_CHECK(hipMemcpyAsync(its.data(), w.iters.get(), ns * 4, hipMemcpyDeviceToHost, s));
+        NLHIP_CHECK(hipMemcpyAsync(err.data(), w.args.err, ns * 8, hipMemcpyDeviceToHost, s));
+        NLHIP_CHECK(hipMemcpyAsync(thr.data(), w.args.thresh, ns * 8, hipMemcpyDeviceToHost, s));
+        NLHIP_CHECK(hipMemcpyAsync(bb.data(), w.args.bb, ns * 8, hipMemcpyDeviceToHost, s));
+        B.stream.sync();
+    };
+    read();
+    auto ran = [&](uint32_t k) { return !(skip && (*skip)[k]); };
+    auto cap_of = [&](uint32_t k) {
+        return cap ? std::min<int64_t>((*cap)[k], int64_t(p.max_iter)) : int64_t(p.max_iter);
+    };
+    std::vector<uint32_t> first(ns, ~0u);
+    for (uint32_t k = 0; k < ns; ++k) {
+        if (!ran(k)) continue;
         if (!std::isfinite(err[k]) || !std::isfinite(bb[k]))
             throw Error(NLHIP_ERR_NONFINITE, "system " + std::to_string(k) +
                                                  ": right-hand side or iterate is NaN or Inf");
-        cgrel[k] = bb[k] > 0 ? std::sqrt(err[k] / bb[k]) : std::sqrt(err[k]);
-        conv[k] = !(err[k] > thr[k]) ? 1 : 0;
+        if (!(err[k] > thr[k])) first[k] = uint32_t(its[k]);
     }
+    std::vector<double> rr, tb;
+    if (p.confirm) {
+        std::vector<double> last(ns, -1.0), alpha(ns);
+        std::vector<int32_t> stop(ns), done(ns);
+        for (int round = 0;; ++round) {
+            confirm_norms(B, rr, tb);
+            std::vector<uint32_t> again;
+            for (uint32_t k = 0; k < ns; ++k) {
+                if (!ran(k) || err[k] > thr[k]) continue;
+                if (first[k] == ~0u) first[k] = uint32_t(its[k]);
+                const double target = (p.eps * p.eps) * tb[k];
+                if (rr[k] > target && its[k] < cap_of(k) && err[k] > 0.0 &&
+                    (last[k] < 0.0 || rr[k] <= 0.99 * last[k]) && round < kMaxConfirmRounds)
+                    again.push_back(k);
+            }
+            if (again.empty()) break;
+            NLHIP_CHECK(hipMemcpyAsync(alpha.data(), w.args.alpha, ns * 8, hipMemcpyDeviceToHost, s));
+            NLHIP_CHECK(hipMemcpyAsync(stop.data(), w.stop.get(), ns * 4, hipMemcpyDeviceToHost, s));
+            NLHIP_CHECK(hipMemcpyAsync(done.data(), w.done.get(), ns * 4, hipMemcpyDeviceToHost, s));
+            B.stream.sync();
+            for (uint32_t k : again) {
+                const double target = (p.eps * p.eps) * tb[k];
+                thr[k] = err[k] * (target / rr[k]) * 0.5;
+                alpha[k] = 0.0;  // the launch that stopped applied x -= alpha d
+                stop[k] = 0;
+                done[k] = 0;
+                last[k] = rr[k];
+            }
+            NLHIP_CHECK(hipMemcpyAsync(w.args.thresh, thr.data(), ns * 8, hipMemcpyHostToDevice, s));
+            NLHIP_CHECK(hipMemcpyAsync(w.args.alpha, alpha.data(), ns * 8, hipMemcpyHostToDevice, s));
+            NLHIP_CHECK(hipMemcpyAsync(w.stop.get(), stop.data(), ns * 4, hipMemcpyHostToDevice, s));
+            NLHIP_CHECK(hipMemcpyAsync(w.done.get(), done.data(), ns * 4, hipMemcpyHostToDevice, s));
+            NLHIP_CHECK(batch_compact<double>(w.args, B.grid, s));
+            batch_loop<double>(B, w, precond, int(p.max_iter), K, p.max_iter);
+            read();
+        }
+    }
+    cgrel.resize(ns);
+    conv.resize(ns);
+    extra.resize(ns);
+    for (uint32_t k = 0; k < ns; ++k) {
+        if (!ran(k)) continue;
+        cgrel[k] = bb[k] > 0 ? std::sqrt(err[k] / bb[k]) : std::sqrt(err[k]);
+        conv[k] = p.confirm ? (first[k] != ~0u && !(rr[k] > (p.eps * p.eps) * tb[k]) ? 1 : 0)
+                            : (!(err[k] > thr[k]) ? 1 : 0);
+        extra[k] = first[k] == ~0u ? 0u : uint32_t(its[k]) - first[k];
+    }
+    w.args.skip = nullptr;
+    w.args.cap = nullptr;
+}
+
+void batch_solve_fp64(Batch& B, const nlhip_solve_params& p, std::vector<int32_t>& its,
+                      std::vector<double>& cgrel, std::vector<uint8_t>& conv,
+                      std::vector<uint32_t>& extra) {
+    batch_fp64(B, p, nullptr, nullptr, its, cgrel, conv, extra);
 }
 
 void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_t>& its,
@@ -301,6 +422,7 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
     w32.args.val = B.val32.get();
     w32.args.dinv = precond ? B.dinv32.get() : nullptr;
     w32.args.skip = w32.skip.get();
+    w32.args.cap = w32.cap.get();
     hipStream_t s = B.stream.get();
     B.scale.ensure(2 * size_t(B.nsys));
     const uint32_t K = p.check_every ? p.check_every : batch_check_every(B);
@@ -309,7 +431,7 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
     outer.assign(B.nsys, 0);
     conv.assign(B.nsys, 0);
     cgrel.assign(B.nsys, 0.0);
-    std::vector<int32_t> skip(B.nsys), inner_its(B.nsys);
+    std::vector<int32_t> skip(B.nsys), inner_its(B.nsys), cap(B.nsys);
     std::vector<double> sc(2 * size_t(B.nsys));
     for (uint32_t step = 0;; ++step) {
         std::vector<double> rel, rr, bb;
@@ -324,6 +446,9 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
             const bool budget = uint32_t(its[k]) >= p.max_iter || outer[k] >= p.max_outer;
             const bool gone = fell_back && (*fell_back)[k];
             skip[k] = (conv[k] || budget || gone) ? 1 : 0;
+            // each inner solve is capped by what is left of the system's budget
+            cap[k] = int32_t(std::min<uint32_t>(inner_cap, p.max_iter - std::min<uint32_t>(
+                                                                      p.max_iter, its[k])));
             const double rn = std::sqrt(rr[k]);
             sc[k] = skip[k] ? 0.0 : 1.0 / rn;
             sc[B.nsys + k] = skip[k] ? 0.0 : rn;
@@ -331,6 +456,7 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
         }
         if (active == 0) break;
         NLHIP_CHECK(hipMemcpyAsync(w32.skip.get(), skip.data(), B.nsys * 4, hipMemcpyHostToDevice, s));
+        NLHIP_CHECK(hipMemcpyAsync(w32.cap.get(), cap.data(), B.nsys * 4, hipMemcpyHostToDevice, s));
         NLHIP_CHECK(hipMemcpyAsync(B.scale.get(), sc.data(), sc.size() * 8, hipMemcpyHostToDevice, s));
         NLHIP_CHECK(batch_scale_cast(w64.args, w32.args, B.scale.get(), B.grid, s));
         run_batch<float>(B, w32, precond, float(p.inner_eps), int(inner_cap), K);
@@ -345,58 +471,52 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
             outer[k] += 1;
             progress |= inner_its[k] > 0;
             // an inner solve that used its whole cap without reaching
-            // inner_eps: this system continues in FP64
-            if (fell_back && uint32_t(inner_its[k]) >= inner_cap) (*fell_back)[k] = 1;
+            // inner_eps while budget is left: this system continues in FP64
+            if (fell_back && inner_its[k] >= cap[k] && uint32_t(its[k]) < p.max_iter)
+                (*fell_back)[k] = 1;
         }
         if (!progress) break;
     }
+    w32.args.cap = nullptr;
 }
 
 // Mixed precision with a fallback (the policy of LAPACK's DSGESV: when the
 // FP32 inner solver does not converge, finish in FP64): systems whose FP32
-// inner solve reaches inner_max_iter without reaching inner_eps stop
-// refining and are solved by the FP64 batched CG from their current iterate;
-// the others refine to eps as in batch_solve_mixed.
+// inner solve reaches its cap without reaching inner_eps stop refining and
+// are solved by the FP64 batched CG from their current iterate, within what
+// is left of their max_iter; the others refine to eps as in
+// batch_solve_mixed.
 void batch_solve_mixed_fallback(Batch& B, const nlhip_solve_params& p, std::vector<int32_t>& its,
                                 std::vector<double>& cgrel, std::vector<uint8_t>& conv,
-                                std::vector<uint32_t>& outer, uint32_t& nfallback) {
+                                std::vector<uint32_t>& outer, std::vector<uint32_t>& extra,
+                                uint32_t& nfallback) {
     if (p.inner_max_iter == 0)
         throw Error(NLHIP_ERR_INVALID, "the fallback needs inner_max_iter > 0");
     std::vector<uint8_t> fb(B.nsys, 0);
     // systems that fall back are dropped from the refinement at once: the
     // FP32 loop skips them from the next outer step on
-    nlhip_solve_params pm = p;
-    batch_solve_mixed(B, pm, its, cgrel, conv, outer, &fb);
+    batch_solve_mixed(B, p, its, cgrel, conv, outer, &fb);
+    extra.assign(B.nsys, 0);
     nfallback = 0;
-    std::vector<int32_t> skip(B.nsys);
+    std::vector<int32_t> skip(B.nsys), cap(B.nsys);
     for (uint32_t k = 0; k < B.nsys; ++k) {
         const bool go = fb[k] && !conv[k];
         skip[k] = go ? 0 : 1;
+        cap[k] = int32_t(p.max_iter - std::min<uint32_t>(p.max_iter, its[k]));
         nfallback += go ? 1u : 0u;
     }
     if (nfallback == 0) return;
-    BatchWork<double>& w = B.w64;
-    hipStream_t s = B.stream.get();
-    NLHIP_CHECK(hipMemcpyAsync(w.skip.get(), skip.data(), B.nsys * 4, hipMemcpyHostToDevice, s));
-    const bool precond = p.precond == NL_PRECOND_JACOBI;
-    w.args.val = B.val.get();
-    w.args.dinv = precond ? B.dinv.get() : nullptr;
-    w.args.skip = w.skip.get();
-    const uint32_t K = p.check_every ? p.check_every : batch_check_every(B);
-    run_batch<double>(B, w, precond, p.eps, int(p.max_iter), K);
-    w.args.skip = nullptr;
-    std::vector<int32_t> its64(B.nsys);
-    std::vector<double> err(B.nsys), thr(B.nsys), bb(B.nsys);
-    NLHIP_CHECK(hipMemcpyAsync(its64.data(), w.iters.get(), B.nsys * 4, hipMemcpyDeviceToHost, s));
-    NLHIP_CHECK(hipMemcpyAsync(err.data(), w.args.err, B.nsys * 8, hipMemcpyDeviceToHost, s));
-    NLHIP_CHECK(hipMemcpyAsync(thr.data(), w.args.thresh, B.nsys * 8, hipMemcpyDeviceToHost, s));
-    NLHIP_CHECK(hipMemcpyAsync(bb.data(), w.args.bb, B.nsys * 8, hipMemcpyDeviceToHost, s));
-    B.stream.sync();
+    std::vector<int32_t> its64;
+    std::vector<double> cgrel64;
+    std::vector<uint8_t> conv64;
+    std::vector<uint32_t> extra64;
+    batch_fp64(B, p, &skip, &cap, its64, cgrel64, conv64, extra64);
     for (uint32_t k = 0; k < B.nsys; ++k) {
         if (skip[k]) continue;
         its[k] += its64[k];
-        cgrel[k] = bb[k] > 0 ? std::sqrt(err[k] / bb[k]) : std::sqrt(err[k]);
-        conv[k] = !(err[k] > thr[k]) ? 1 : 0;
+        cgrel[k] = cgrel64[k];
+        conv[k] = conv64[k];
+        extra[k] = extra64[k];
     }
 }
 
@@ -776,6 +896,7 @@ NLAPI int NLAPIENTRY nlhipBatchAutotune(nlhip_batch b, const double* rhs, uint32
         std::vector<int32_t> its;
         std::vector<double> cgrel;
         std::vector<uint8_t> conv;
+        std::vector<uint32_t> extra;
         // the current shape first, then the others
         std::vector<std::pair<uint32_t, uint32_t>> shapes{{grid0, passes0}};
         for (uint32_t g : grids)
@@ -794,7 +915,7 @@ NLAPI int NLAPIENTRY nlhipBatchAutotune(nlhip_batch b, const double* rhs, uint32
                 NLHIP_CHECK(hipMemsetAsync(B->w64.x.get(), 0, size_t(B->n) * 8, s));
                 Event e0, e1;
                 e0.record(s);
-                batch_solve_fp64(*B, p, its, cgrel, conv);
+                batch_solve_fp64(*B, p, its, cgrel, conv, extra);
                 e1.record(s);
                 e1.sync();
                 t = 1e-3 * elapsed_ms(e0, e1);
@@ -933,14 +1054,14 @@ NLAPI int NLAPIENTRY nlhipBatchSolve(nlhip_batch b, const double* rhs, double* x
         std::vector<int32_t> its;
         std::vector<double> cgrel, rel;
         std::vector<uint8_t> conv;
-        std::vector<uint32_t> outer;
+        std::vector<uint32_t> outer, extra;
         if (p.precision == NLHIP_PREC_FP64) {
-            batch_solve_fp64(*B, p, its, cgrel, conv);
+            batch_solve_fp64(*B, p, its, cgrel, conv, extra);
         } else if (p.precision == NLHIP_PREC_MIXED) {
             batch_solve_mixed(*B, p, its, cgrel, conv, outer);
         } else if (p.precision == NLHIP_PREC_MIXED_FALLBACK) {
             uint32_t nfb = 0;
-            batch_solve_mixed_fallback(*B, p, its, cgrel, conv, outer, nfb);
+            batch_solve_mixed_fallback(*B, p, its, cgrel, conv, outer, extra, nfb);
             B->last_fallbacks = nfb;
         } else {
             throw Error(NLHIP_ERR_INVALID, "unknown precision");
@@ -953,7 +1074,7 @@ NLAPI int NLAPIENTRY nlhipBatchSolve(nlhip_batch b, const double* rhs, double* x
         B->stream.sync();
         const double wall_s =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        fill_stats(*B, its, rel, cgrel, conv, outer.empty() ? nullptr : &outer, per_system,
+        fill_stats(*B, its, rel, cgrel, conv, outer.empty() ? nullptr : &outer, extra, per_system,
                    total, dev_s, wall_s);
     });
 }

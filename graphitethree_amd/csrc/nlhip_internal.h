@@ -334,7 +334,8 @@ template <typename T>
 struct MultiWork {
     uint32_t n = 0, k = 0, P = 0;
     DevBuf<T> x, b, r, h, d, q, part, col;  // part: 4 x P x k; col: 6 x k
-    DevBuf<int32_t> iters, done, skip;
+    DevBuf<int32_t> iters, done, skip, again;
+    DevBuf<T> chk;                          // 2 x k: |b - A x|^2, |b|^2 (confirm)
     DevBuf<uint32_t> n_active;
     PinnedBuf<uint32_t> host;               // two polling slots
     GraphLadder graph;
@@ -342,6 +343,26 @@ struct MultiWork {
     MultiArgs<T> graph_args{};
     int graph_max_iter = -1;
     bool graph_precond = false;
+};
+
+// An FP64 PCG solve on a system's ws64 that can stop and continue (slices of
+// nlhipSolveAdvance, the confirmation of the true residual).  `cur` is the
+// host copy of the device status after the last stop; the device continues
+// from it bitwise.
+struct PcgState {
+    bool active = false;    // a solve is open on the system's ws64
+    bool ran = false;       // the loop has been entered at least once
+    bool finished = false;  // stopped for good (test met and confirmed, cap)
+    bool precond = false;
+    nlhip_solve_params p{};
+    uint32_t K = 0;         // iterations per captured graph
+    CGStatus<double> cur{};
+    double thresh = 0;      // recursive threshold in force
+    uint32_t first_stop = ~0u;  // iterations when the recursive test first held
+    double rr = -1, bb = 0;     // true |b - A x|^2 and |b|^2 at the last check
+    double last_rr = -1;
+    int rounds = 0;
+    double device_seconds = 0;
 };
 
 // A CRS matrix resident on the device, with its Jacobi inverse diagonal and
@@ -370,9 +391,7 @@ struct System {
     MultiWork<double> m64;   // multi right-hand-side workspaces
     MultiWork<float> m32;
     DevBuf<double> mscale;
-    // Jacobi inverse diagonal supplied by a separate preconditioner object
-    // (nlSolveSystemIterative's P); used instead of dinv while set.
-    const double* dinv_override = nullptr;
+    PcgState pcg;            // the FP64 solve open on ws64 (nlhipSolveBegin)
     PinnedBuf<double> host_scalars;
 
     void ensure_fp32();
@@ -480,16 +499,14 @@ System* ls_create(uint32_t nvars, uint32_t nrows, const uint32_t* row_ptr, const
                   uint32_t* nfree_out);
 void default_launch(uint32_t n, uint64_t nnz, int& rpw, uint32_t& grid);
 
-// Solve drivers (System::mu must be held).  b, x: device pointers of length n.
-void solve_fp64(System& S, const double* b_dev, double* x_dev, const nlhip_solve_params& p,
-                nlhip_solve_stats& st);
-void solve_mixed(System& S, const double* b_dev, double* x_dev, const nlhip_solve_params& p,
-                 nlhip_solve_stats& st);
 // |b - A x| / |b| in FP64 on the device.
 double true_rel_residual(System& S, const double* b_dev, const double* x_dev);
-// Full solve with staging of host/device b and x.
+// Full solve with staging of host/device b and x (takes S.mu).  dinv: the
+// Jacobi inverse diagonal of a separate preconditioner object
+// (nlSolveSystemIterative's P) instead of the system's own; null: the
+// system's.
 void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
-           nlhip_solve_stats& st);
+           nlhip_solve_stats& st, const double* dinv = nullptr);
 
 // k right-hand sides at once (B, X column-major n x k, host or device).
 // per (k entries) and total may be null.

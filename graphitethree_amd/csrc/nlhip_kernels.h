@@ -213,6 +213,8 @@ struct BatchArgs {
     int32_t* done;
     int32_t* stop;              // stopping test held (turned into done next launch)
     const int32_t* skip;        // per system, may be null: start as done
+    const int32_t* cap;         // per system, may be null: iteration cap (the
+                                //   launch's max_iter applies when smaller)
     uint32_t* act_tiles;
     uint32_t* n_act_tiles;
     uint32_t* act_sys;
@@ -279,6 +281,9 @@ hipError_t multi_iteration(const MultiArgs<T>& a, bool precond, int max_iter, hi
 template <typename T>
 hipError_t multi_layout(uint32_t n, uint32_t k, const T* src, T* dst, bool to_interleaved,
                         hipStream_t s);
+// Re-opens the FP64 columns flagged in again[k] (done -> 0 with the beta /
+// rho update their stopping launch skipped), then recounts the active ones.
+hipError_t multi_reopen(const MultiArgs<double>& a, const int32_t* again, hipStream_t s);
 hipError_t multi_residual64(const MultiArgs<double>& a, hipStream_t s);
 hipError_t multi_scale_cast(const MultiArgs<double>& a64, const MultiArgs<float>& a32,
                             const double* scale, hipStream_t s);

@@ -51,6 +51,8 @@ void multi_alloc(System& S, MultiWork<T>& w, uint32_t k) {
     w.iters.alloc(k);
     w.done.alloc(k);
     w.skip.alloc(k);
+    w.again.alloc(k);
+    w.chk.alloc(2 * size_t(k));
     w.n_active.alloc(1);
     w.host.ensure(2);
     MultiArgs<T>& a = w.args;
@@ -88,10 +90,12 @@ void multi_alloc(System& S, MultiWork<T>& w, uint32_t k) {
     a.n_active = w.n_active.get();
 }
 
+// Runs the started (or re-opened) recurrences until every column has
+// stopped; `budget` bounds the iterations a column can still run.
 template <typename T>
-void run_multi(System& S, MultiWork<T>& w, bool precond, T eps, int max_iter, uint32_t K) {
+void multi_loop(System& S, MultiWork<T>& w, bool precond, int max_iter, uint32_t K,
+                uint64_t budget) {
     hipStream_t s = S.stream.get();
-    NLHIP_CHECK(multi_start<T>(w.args, precond, eps, max_iter, s));
     if (w.graph_precond != precond || w.graph_max_iter != max_iter ||
         std::memcmp(&w.graph_args, &w.args, sizeof(MultiArgs<T>)) != 0)
         w.graph.reset();
@@ -104,7 +108,7 @@ void run_multi(System& S, MultiWork<T>& w, bool precond, T eps, int max_iter, ui
     w.graph_args = w.args;
     Event ev[2];
     uint32_t* host = w.host.get();
-    const uint64_t limit = uint64_t(std::max(max_iter, 0)) + 3ull * w.graph.cap() + 8;
+    const uint64_t limit = budget + 3ull * w.graph.cap() + 8;
     uint64_t launched = 0, covered = 0;
     for (;;) {
         w.graph.launch(launched, s);
@@ -128,6 +132,12 @@ void run_multi(System& S, MultiWork<T>& w, bool precond, T eps, int max_iter, ui
     S.stream.sync();
 }
 
+template <typename T>
+void run_multi(System& S, MultiWork<T>& w, bool precond, T eps, int max_iter, uint32_t K) {
+    NLHIP_CHECK(multi_start<T>(w.args, precond, eps, max_iter, S.stream.get()));
+    multi_loop<T>(S, w, precond, max_iter, K, uint64_t(std::max(max_iter, 0)));
+}
+
 uint32_t multi_check_every(const System& S, uint32_t k) {
     const double bytes = 12.0 * double(S.nnz) + 108.0 * double(S.n) * k;
     const double iter_s = std::max(bytes / 4.0e12, 2.0e-5);
@@ -145,6 +155,27 @@ void multi_norms(System& S, uint32_t k, std::vector<double>& rr, std::vector<dou
     NLHIP_CHECK(hipMemcpyAsync(bb.data(), a.bb, k * 8, hipMemcpyDeviceToHost, S.stream.get()));
     S.stream.sync();
 }
+
+// |b_j - A x_j|^2 and |b_j|^2 without touching what the stopped FP64
+// recurrences need to continue (R, the r.h / r.r partials, the column
+// scalars): the residual goes to Q and the d.q partials, the norms to chk.
+void confirm_norms(System& S, uint32_t k, std::vector<double>& rr, std::vector<double>& bb) {
+    MultiWork<double>& w = S.m64;
+    MultiArgs<double> a = w.args;
+    a.val = S.val.get();
+    a.r = w.q.get();
+    a.part_rr = a.part_dq;
+    a.err = w.chk.get();
+    a.bb = w.chk.get() + k;
+    NLHIP_CHECK(multi_residual64(a, S.stream.get()));
+    rr.resize(k);
+    bb.resize(k);
+    NLHIP_CHECK(hipMemcpyAsync(rr.data(), a.err, k * 8, hipMemcpyDeviceToHost, S.stream.get()));
+    NLHIP_CHECK(hipMemcpyAsync(bb.data(), a.bb, k * 8, hipMemcpyDeviceToHost, S.stream.get()));
+    S.stream.sync();
+}
+
+constexpr int kMaxConfirmRounds = 8;
 
 }  // namespace
 
@@ -173,22 +204,64 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_
     std::vector<uint32_t> outer(k, 0);
     std::vector<double> cgrel(k, 0.0);
     std::vector<uint8_t> conv(k, 0);
+    std::vector<uint32_t> first(k, ~0u), extra(k, 0);
     if (p.precision == NLHIP_PREC_FP64) {
         w.args.val = S.val.get();
         w.args.dinv = precond ? S.dinv.get() : nullptr;
         w.args.skip = nullptr;
         run_multi<double>(S, w, precond, p.eps, int(p.max_iter), K);
         std::vector<double> err(k), thr(k), bb(k);
-        NLHIP_CHECK(hipMemcpyAsync(its.data(), w.iters.get(), k * 4, hipMemcpyDeviceToHost, s));
-        NLHIP_CHECK(hipMemcpyAsync(err.data(), w.args.err, k * 8, hipMemcpyDeviceToHost, s));
-        NLHIP_CHECK(hipMemcpyAsync(thr.data(), w.args.thresh, k * 8, hipMemcpyDeviceToHost, s));
-        NLHIP_CHECK(hipMemcpyAsync(bb.data(), w.args.bb, k * 8, hipMemcpyDeviceToHost, s));
-        S.stream.sync();
+        auto read = [&] {
+            NLHIP_CHECK(hipMemcpyAsync(its.data(), w.iters.get(), k * 4, hipMemcpyDeviceToHost, s));
+            NLHIP_CHECK(hipMemcpyAsync(err.data(), w.args.err, k * 8, hipMemcpyDeviceToHost, s));
+            NLHIP_CHECK(hipMemcpyAsync(thr.data(), w.args.thresh, k * 8, hipMemcpyDeviceToHost, s));
+            NLHIP_CHECK(hipMemcpyAsync(bb.data(), w.args.bb, k * 8, hipMemcpyDeviceToHost, s));
+            S.stream.sync();
+        };
+        read();
         for (uint32_t j = 0; j < k; ++j) {
             if (!std::isfinite(err[j]) || !std::isfinite(bb[j]))
                 throw Error(NLHIP_ERR_NONFINITE, "right-hand side or iterate is NaN or Inf");
+            if (!(err[j] > thr[j])) first[j] = uint32_t(its[j]);
+        }
+        // confirmation of the true residual, column by column (see
+        // nlhip_solve_params::confirm): a column whose recursive residual met
+        // eps but whose |b - A x| did not continues its recurrence with a
+        // lower threshold; the others stay stopped
+        std::vector<double> rr, tb;
+        if (p.confirm) {
+            std::vector<double> last(k, -1.0);
+            std::vector<int32_t> again(k);
+            for (int round = 0;; ++round) {
+                confirm_norms(S, k, rr, tb);
+                bool any = false;
+                for (uint32_t j = 0; j < k; ++j) {
+                    again[j] = 0;
+                    if (err[j] > thr[j]) continue;
+                    if (first[j] == ~0u) first[j] = uint32_t(its[j]);
+                    const double target = (p.eps * p.eps) * tb[j];
+                    if (rr[j] > target && uint32_t(its[j]) < p.max_iter && err[j] > 0.0 &&
+                        (last[j] < 0.0 || rr[j] <= 0.99 * last[j]) && round < kMaxConfirmRounds) {
+                        again[j] = 1;
+                        thr[j] = err[j] * (target / rr[j]) * 0.5;
+                        last[j] = rr[j];
+                        any = true;
+                    }
+                }
+                if (!any) break;
+                NLHIP_CHECK(hipMemcpyAsync(w.args.thresh, thr.data(), k * 8, hipMemcpyHostToDevice, s));
+                NLHIP_CHECK(hipMemcpyAsync(w.again.get(), again.data(), k * 4,
+                                           hipMemcpyHostToDevice, s));
+                NLHIP_CHECK(multi_reopen(w.args, w.again.get(), s));
+                multi_loop<double>(S, w, precond, int(p.max_iter), K, p.max_iter);
+                read();
+            }
+        }
+        for (uint32_t j = 0; j < k; ++j) {
             cgrel[j] = bb[j] > 0 ? std::sqrt(err[j] / bb[j]) : std::sqrt(err[j]);
-            conv[j] = !(err[j] > thr[j]) ? 1 : 0;
+            conv[j] = p.confirm ? (first[j] != ~0u && !(rr[j] > (p.eps * p.eps) * tb[j]) ? 1 : 0)
+                                : (!(err[j] > thr[j]) ? 1 : 0);
+            extra[j] = first[j] == ~0u ? 0u : uint32_t(its[j]) - first[j];
         }
     } else if (p.precision == NLHIP_PREC_MIXED) {
         S.ensure_fp32();
@@ -265,11 +338,13 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_
             st.cg_rel_residual = cgrel[j];
             st.device_seconds = dev_s;
             st.wall_seconds = wall_s;
+            st.extra_iterations = extra[j];
         }
     }
     if (total) {
         std::memset(total, 0, sizeof(*total));
         total->iterations = it_max;
+        total->extra_iterations = *std::max_element(extra.begin(), extra.end());
         total->converged = nconv == k ? 1 : 0;
         total->rel_residual = worst;
         total->device_seconds = dev_s;

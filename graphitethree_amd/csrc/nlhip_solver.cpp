@@ -265,13 +265,13 @@ static uint32_t auto_check_every(const System& S) {
     return uint32_t(std::clamp(2.0e-3 / iter_s, 8.0, 128.0));
 }
 
-// Runs PCG on ws (args already set, b and x filled) until the device-side
-// stopping test holds.  Returns the final status.
+// Runs the captured iterations of a started CG (arguments set, status open)
+// until the device-side loop has stopped; `budget` bounds the iterations it
+// can still run.  Returns the final status.
 template <typename T>
-static CGStatus<T> run_cg(System& S, CGWorkspace<T>& ws, bool precond, T eps, int max_iter,
-                          uint32_t K) {
+static CGStatus<T> run_ladder(System& S, CGWorkspace<T>& ws, bool precond, uint32_t K,
+                              uint64_t budget) {
     hipStream_t s = S.stream.get();
-    NLHIP_CHECK(cg_start<T>(ws.args, ws.code, precond, eps, max_iter, s));
     if (ws.graph_rpw != ws.code || ws.graph_precond != precond ||
         std::memcmp(&ws.graph_args, &ws.args, sizeof(CGArgs<T>)) != 0)
         ws.graph.reset();
@@ -284,8 +284,8 @@ static CGStatus<T> run_cg(System& S, CGWorkspace<T>& ws, bool precond, T eps, in
     Event ev[2];
     CGStatus<T>* host = ws.st_host.get();
     host[0].done = host[1].done = 0;
-    // the device stops by itself at max_iter; past that the status must say so
-    const uint64_t limit = uint64_t(std::max(max_iter, 0)) + 3ull * ws.graph.cap() + 8;
+    // the device stops by itself at its cap; past that the status must say so
+    const uint64_t limit = budget + 3ull * ws.graph.cap() + 8;
     uint64_t launched = 0, covered = 0;
     for (;;) {
         ws.graph.launch(launched, s);
@@ -310,6 +310,33 @@ static CGStatus<T> run_cg(System& S, CGWorkspace<T>& ws, bool precond, T eps, in
     NLHIP_CHECK(hipMemcpyAsync(&fin, ws.st.get(), sizeof(fin), hipMemcpyDeviceToHost, s));
     S.stream.sync();
     return fin;
+}
+
+// Re-opens a stopped CG loop: stop and done cleared, a new threshold and
+// cap, and alpha = 0, because the direction launch that stopped has already
+// applied the pending x -= alpha d.  The next direction launch recomputes
+// rho and err from the same partials and, the test failing now, goes on
+// exactly as it would have with this threshold and cap from the start
+// (x -= 0 d leaves x as it is, up to the sign of a zero).
+template <typename T>
+static void cg_reopen(System& S, CGWorkspace<T>& ws, CGStatus<T> st, T thresh, int max_iter) {
+    st.stop = 0;
+    st.done = 0;
+    st.alpha = T(0);
+    st.thresh = thresh;
+    st.max_iter = max_iter;
+    NLHIP_CHECK(hipMemcpyAsync(ws.st.get(), &st, sizeof(st), hipMemcpyHostToDevice,
+                               S.stream.get()));
+    S.stream.sync();  // st lives on this stack
+}
+
+template <typename T>
+static CGStatus<T> read_status(System& S, CGWorkspace<T>& ws) {
+    CGStatus<T> st;
+    NLHIP_CHECK(hipMemcpyAsync(&st, ws.st.get(), sizeof(st), hipMemcpyDeviceToHost,
+                               S.stream.get()));
+    S.stream.sync();
+    return st;
 }
 
 // Reads |b - A x|^2 and |b|^2 (FP64) into rr, bb.
@@ -338,27 +365,100 @@ double true_rel_residual(System& S, const double* b, const double* x) {
     return bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
 }
 
-void solve_fp64(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st) {
+// Opens an FP64 PCG solve on ws64 (b and x already there): initial residual
+// and status on the device, the status read back.
+static void pcg_begin(System& S, const nlhip_solve_params& p, const double* dinv) {
+    PcgState& P = S.pcg;
+    P = PcgState{};
     CGWorkspace<double>& ws = S.ws64;
-    const bool precond = p.precond == NL_PRECOND_JACOBI;
-    const double* dinv = S.dinv_override ? S.dinv_override : S.dinv.get();
-    setup_args<double>(S, ws, precond ? dinv : nullptr);
-    const uint32_t K = p.check_every ? p.check_every : auto_check_every(S);
-    CGStatus<double> fin = run_cg<double>(S, ws, precond, p.eps, int(p.max_iter), K);
-    if (!std::isfinite(fin.bb) || !std::isfinite(fin.err))
-        throw Error(NLHIP_ERR_NONFINITE, "right-hand side or iterate is NaN or Inf");
-    st.iterations = uint32_t(fin.iters);
+    P.p = p;
+    P.precond = p.precond == NL_PRECOND_JACOBI;
+    setup_args<double>(S, ws, P.precond ? (dinv ? dinv : S.dinv.get()) : nullptr);
+    P.K = p.check_every ? p.check_every : auto_check_every(S);
+    NLHIP_CHECK(cg_start<double>(ws.args, ws.code, P.precond, p.eps, 0, S.stream.get()));
+    P.cur = read_status(S, ws);
+    P.thresh = P.cur.thresh;
+    P.active = true;
+}
+
+// Most times the confirmation lowers the threshold before it gives up.
+constexpr int kMaxConfirmRounds = 8;
+
+// Runs the open solve until it stops for good or has done `until`
+// iterations in all (the end of a slice).  Each stop of the device loop is
+// the recursive test (err <= thresh), the slice's cap or max_iter; after the
+// test, with p.confirm, the true residual decides: met -> finished; above
+// eps |b| -> the loop re-opens with the threshold scaled down by the ratio
+// measured, until it is met, max_iter is reached, or it stops decreasing.
+static void pcg_advance(System& S, uint64_t until) {
+    PcgState& P = S.pcg;
+    CGWorkspace<double>& ws = S.ws64;
+    const nlhip_solve_params& p = P.p;
+    Event e0, e1;
+    e0.record(S.stream.get());
+    while (!P.finished) {
+        const uint32_t its = uint32_t(P.cur.iters);
+        if (P.ran && its >= until) break;  // end of the slice
+        const uint32_t cap = uint32_t(std::min<uint64_t>(until, p.max_iter));
+        cg_reopen<double>(S, ws, P.cur, P.thresh, int(cap));
+        P.cur = run_ladder<double>(S, ws, P.precond, P.K, uint64_t(cap - std::min(cap, its)) + 1);
+        P.ran = true;
+        if (!std::isfinite(P.cur.bb) || !std::isfinite(P.cur.err))
+            throw Error(NLHIP_ERR_NONFINITE, "right-hand side or iterate is NaN or Inf");
+        if (P.cur.err > P.thresh) {
+            if (uint32_t(P.cur.iters) >= p.max_iter) P.finished = true;
+            continue;
+        }
+        if (P.first_stop == ~0u) P.first_stop = uint32_t(P.cur.iters);
+        if (!p.confirm) {
+            P.finished = true;
+            break;
+        }
+        residual_norms(S, ws.b.get(), ws.x.get(), nullptr, P.rr, P.bb);
+        const double target = (p.eps * p.eps) * P.bb;
+        if (!(P.rr > target) || uint32_t(P.cur.iters) >= p.max_iter || !(P.cur.err > 0.0) ||
+            (P.last_rr >= 0.0 && P.rr > 0.99 * P.last_rr) || P.rounds >= kMaxConfirmRounds) {
+            P.finished = true;
+            break;
+        }
+        P.last_rr = P.rr;
+        ++P.rounds;
+        P.thresh = P.cur.err * (target / P.rr) * 0.5;
+    }
+    e1.record(S.stream.get());
+    e1.sync();
+    P.device_seconds += 1e-3 * elapsed_ms(e0, e1);
+}
+
+// Statistics of the open solve (device_seconds: the loop's own time).
+static void pcg_stats(System& S, nlhip_solve_stats& st) {
+    const PcgState& P = S.pcg;
+    st.iterations = uint32_t(P.cur.iters);
     st.outer_iterations = 0;
-    st.cg_rel_residual = fin.bb > 0 ? std::sqrt(fin.err / fin.bb) : std::sqrt(fin.err);
-    st.converged = !(fin.err > fin.thresh) ? 1 : 0;
+    const double bb = P.cur.bb;
+    st.cg_rel_residual = bb > 0 ? std::sqrt(P.cur.err / bb) : std::sqrt(P.cur.err);
+    if (P.p.confirm && P.rr >= 0.0 && P.first_stop != ~0u)
+        st.converged = !(P.rr > (P.p.eps * P.p.eps) * P.bb) ? 1 : 0;
+    else
+        st.converged = !(P.cur.err > P.cur.thresh) && P.first_stop != ~0u ? 1 : 0;
+    st.extra_iterations = P.first_stop == ~0u ? 0u : uint32_t(P.cur.iters) - P.first_stop;
+    st.device_seconds = P.device_seconds;
+}
+
+static void solve_fp64(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st,
+                       const double* dinv) {
+    pcg_begin(S, p, dinv);
+    pcg_advance(S, p.max_iter);
+    pcg_stats(S, st);
+    S.pcg.active = false;
 }
 
 // FP32 inner PCG inside FP64 refinement.  fallback (NLHIP_PREC_MIXED_
 // FALLBACK): an inner solve that uses its whole cap (inner_max_iter) ends
 // the refinement and the solve finishes with FP64 CG from the current
 // iterate (LAPACK DSGESV's policy).
-void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st,
-                 bool fallback = false) {
+static void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st,
+                        bool fallback) {
     S.ensure_fp32();
     CGWorkspace<double>& w64 = S.ws64;
     CGWorkspace<float>& w32 = S.ws32;
@@ -380,8 +480,8 @@ void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st,
         NLHIP_CHECK(ir_scale_cast(S.n, w64.r.get(), 1.0 / rn, w32.b.get(), s));
         NLHIP_CHECK(fill<float>(S.n, 0.0f, w32.x.get(), s));
         const uint32_t cap = std::min(inner_cap, p.max_iter - total);
-        CGStatus<float> fin =
-            run_cg<float>(S, w32, precond, float(p.inner_eps), int(cap), K);
+        NLHIP_CHECK(cg_start<float>(w32.args, w32.code, precond, float(p.inner_eps), int(cap), s));
+        const CGStatus<float> fin = run_ladder<float>(S, w32, precond, K, cap);
         total += uint32_t(fin.iters);
         NLHIP_CHECK(ir_update(S.n, w64.x.get(), w32.x.get(), rn, s));
         ++outer;
@@ -390,11 +490,12 @@ void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st,
             nlhip_solve_params p64 = p;
             p64.max_iter = p.max_iter - total;
             nlhip_solve_stats s64{};
-            solve_fp64(S, p64, s64);
+            solve_fp64(S, p64, s64, nullptr);
             st.iterations = total + s64.iterations;
             st.outer_iterations = outer;
             st.cg_rel_residual = s64.cg_rel_residual;
             st.converged = s64.converged;
+            st.extra_iterations = s64.extra_iterations;
             return;
         }
     }
@@ -404,12 +505,7 @@ void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st,
     st.converged = !(rr > (p.eps * p.eps) * bb) ? 1 : 0;
 }
 
-void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
-           nlhip_solve_stats& st) {
-    std::lock_guard<std::mutex> lock(S.mu);
-    const auto t0 = std::chrono::steady_clock::now();
-    std::memset(&st, 0, sizeof(st));
-    if (take_injected_failure()) throw Error(NLHIP_ERR_INJECTED, "injected failure");
+static void check_params(const nlhip_solve_params& p) {
     if (p.precision != NLHIP_PREC_FP64 && p.precision != NLHIP_PREC_MIXED &&
         p.precision != NLHIP_PREC_MIXED_FALLBACK)
         throw Error(NLHIP_ERR_INVALID, "unknown precision");
@@ -417,13 +513,23 @@ void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
         throw Error(NLHIP_ERR_INVALID, "the fallback needs inner_max_iter > 0");
     if (p.precond != NL_PRECOND_JACOBI && p.precond != NL_PRECOND_NONE)
         throw Error(NLHIP_ERR_UNSUPPORTED, "only Jacobi or no preconditioner on the GPU");
+}
+
+void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
+           nlhip_solve_stats& st, const double* dinv) {
+    std::lock_guard<std::mutex> lock(S.mu);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::memset(&st, 0, sizeof(st));
+    if (take_injected_failure()) throw Error(NLHIP_ERR_INJECTED, "injected failure");
+    check_params(p);
+    S.pcg.active = false;
     S.ws64.ensure(S.n, S.grid);
     hipStream_t s = S.stream.get();
     copy_in(S.ws64.b.get(), b, size_t(S.n) * 8, s);
     copy_in(S.ws64.x.get(), x, size_t(S.n) * 8, s);
     Event e0, e1;
     e0.record(s);
-    if (p.precision == NLHIP_PREC_FP64) solve_fp64(S, p, st);
+    if (p.precision == NLHIP_PREC_FP64) solve_fp64(S, p, st, dinv);
     else solve_mixed(S, p, st, p.precision == NLHIP_PREC_MIXED_FALLBACK);
     e1.record(s);
     e1.sync();
@@ -434,6 +540,48 @@ void solve(System& S, const double* b, double* x, const nlhip_solve_params& p,
     st.wall_seconds =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     st.status = NLHIP_OK;
+}
+
+// Sliced solves (nlhipSolveBegin / Advance / End).
+static void sliced_begin(System& S, const double* b, const double* x0,
+                         const nlhip_solve_params& p) {
+    std::lock_guard<std::mutex> lock(S.mu);
+    check_params(p);
+    if (p.precision != NLHIP_PREC_FP64)
+        throw Error(NLHIP_ERR_UNSUPPORTED, "sliced solves run FP64 CG only");
+    S.pcg.active = false;
+    S.ws64.ensure(S.n, S.grid);
+    hipStream_t s = S.stream.get();
+    copy_in(S.ws64.b.get(), b, size_t(S.n) * 8, s);
+    if (x0) copy_in(S.ws64.x.get(), x0, size_t(S.n) * 8, s);
+    else NLHIP_CHECK(fill<double>(S.n, 0.0, S.ws64.x.get(), s));
+    pcg_begin(S, p, nullptr);
+}
+
+static void sliced_advance(System& S, uint32_t iters, int* finished, nlhip_solve_stats* st) {
+    std::lock_guard<std::mutex> lock(S.mu);
+    if (!S.pcg.active) throw Error(NLHIP_ERR_INVALID, "no sliced solve is open on this system");
+    pcg_advance(S, uint64_t(uint32_t(S.pcg.cur.iters)) + iters);
+    if (finished) *finished = S.pcg.finished ? 1 : 0;
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        pcg_stats(S, *st);
+        st->rel_residual = S.pcg.rr >= 0.0 && S.pcg.bb > 0 ? std::sqrt(S.pcg.rr / S.pcg.bb)
+                                                            : st->cg_rel_residual;
+    }
+}
+
+static void sliced_end(System& S, double* x, nlhip_solve_stats* st) {
+    std::lock_guard<std::mutex> lock(S.mu);
+    if (!S.pcg.active) throw Error(NLHIP_ERR_INVALID, "no sliced solve is open on this system");
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        pcg_stats(S, *st);
+        st->rel_residual = true_rel_residual(S, S.ws64.b.get(), S.ws64.x.get());
+    }
+    if (x) copy_out(x, S.ws64.x.get(), size_t(S.n) * 8, S.stream.get());
+    S.stream.sync();
+    S.pcg.active = false;
 }
 
 // ------------------------------------------------------------- BiCGSTAB
@@ -670,6 +818,7 @@ namespace nlhip {
 // `iters` iterations of the system's launch shape (HIP events on its stream;
 // S.mu held).
 void profile_cg(System& S, bool fp32, uint32_t iters, double t[3]) {
+    S.pcg.active = false;  // runs on ws64 / ws32
     hipStream_t st = S.stream.get();
     if (iters == 0) iters = 1;
     auto profile = [&](auto& ws, auto* dinv, auto zero) {
@@ -772,6 +921,7 @@ NLAPI void NLAPIENTRY nlhipSolveParamsDefault(nlhip_solve_params* p) {
     p->inner_eps = 1e-4;
     p->max_outer = 30;
     p->inner_max_iter = 0;
+    p->confirm = 1;
 }
 
 NLAPI int NLAPIENTRY nlhipSystemCreate(uint32_t n, const uint32_t* rowptr,
@@ -845,10 +995,32 @@ NLAPI int NLAPIENTRY nlhipSolve(nlhip_system s, const double* b, double* x,
     return rc;
 }
 
+NLAPI int NLAPIENTRY nlhipSolveBegin(nlhip_system s, const double* b, const double* x0,
+                                     const nlhip_solve_params* params) {
+    return guarded("nlhipSolveBegin", [&] {
+        System* S = sys(s);
+        if (!b) throw Error(NLHIP_ERR_INVALID, "null b");
+        nlhip_solve_params p;
+        nlhipSolveParamsDefault(&p);
+        if (params) p = *params;
+        sliced_begin(*S, b, x0, p);
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipSolveAdvance(nlhip_system s, uint32_t iters, int* finished,
+                                       nlhip_solve_stats* stats) {
+    return guarded("nlhipSolveAdvance", [&] { sliced_advance(*sys(s), iters, finished, stats); });
+}
+
+NLAPI int NLAPIENTRY nlhipSolveEnd(nlhip_system s, double* x, nlhip_solve_stats* stats) {
+    return guarded("nlhipSolveEnd", [&] { sliced_end(*sys(s), x, stats); });
+}
+
 NLAPI int NLAPIENTRY nlhipSpMV(nlhip_system s, const double* x, double* y) {
     return guarded("nlhipSpMV", [&] {
         System* S = sys(s);
         std::lock_guard<std::mutex> lock(S->mu);
+        S->pcg.active = false;  // stages through ws64
         hipStream_t st = S->stream.get();
         S->ws64.ensure(S->n, S->grid);
         copy_in(S->ws64.x.get(), x, size_t(S->n) * 8, st);
@@ -865,6 +1037,7 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
     return guarded("nlhipBenchSpMV", [&] {
         System* S = sys(s);
         std::lock_guard<std::mutex> lock(S->mu);
+        S->pcg.active = false;  // runs on ws64 / ws32
         hipStream_t st = S->stream.get();
         if (reps == 0) reps = 1;
         Event e0, e1;

@@ -80,6 +80,16 @@ typedef struct {
     double inner_eps;         /* mixed: relative tolerance of each inner CG  */
     uint32_t max_outer;       /* mixed: refinement steps cap                 */
     uint32_t inner_max_iter;  /* mixed: cap per inner CG (0: max_iter)       */
+    uint32_t confirm;         /* FP64 CG: when the recursive residual meets
+                                 eps, recompute |b - A x| in FP64 and, while
+                                 it is still above eps |b|, continue the same
+                                 recurrence (bitwise the iterates of a solve
+                                 with a smaller threshold) with the recursive
+                                 threshold lowered by the measured gap; stops
+                                 when the true residual meets eps, at max_iter,
+                                 or when it no longer decreases (default 1;
+                                 0: stop on the recursive residual alone, as
+                                 OpenNL's CG does)                          */
 } nlhip_solve_params;
 
 typedef struct {
@@ -91,6 +101,10 @@ typedef struct {
     double cg_rel_residual;   /* the solver's own recursive residual         */
     double device_seconds;    /* HIP-event time of the solve on the device   */
     double wall_seconds;      /* host time including staging                 */
+    uint32_t extra_iterations;/* iterations run after the recursive residual
+                                 first met eps (confirm); iterations minus
+                                 this is where OpenNL's stopping rule stops  */
+    uint32_t reserved;
 } nlhip_solve_stats;
 
 NLAPI void NLAPIENTRY nlhipSolveParamsDefault(nlhip_solve_params* p);
@@ -193,6 +207,24 @@ NLAPI int NLAPIENTRY nlhipSystemAutotune(nlhip_system s, int fp32, uint32_t iter
 NLAPI int NLAPIENTRY nlhipSolve(nlhip_system s, const double* b, double* x,
                                 const nlhip_solve_params* params,
                                 nlhip_solve_stats* stats);
+/* A solve advanced in slices (FP64 Jacobi-PCG, or plain CG with
+ * NL_PRECOND_NONE): nlhipSolveBegin stages b and the initial guess x0 (host
+ * or device) and forms the initial residual; each nlhipSolveAdvance runs at
+ * most `iters` more iterations of the same recurrence and sets *finished
+ * once the solve has stopped (stopping test, with params->confirm as for
+ * nlhipSolve, or max_iter); nlhipSolveEnd copies the iterate into x (host or
+ * device, may be NULL) and ends the solve.  The iterates, iteration count and
+ * result are bitwise those of one nlhipSolve call with the same parameters,
+ * however the iterations are sliced.  One sliced solve per system at a time;
+ * nlhipSolve on the system in between ends it.  stats (may be NULL) describe
+ * the solve so far.  No counterpart in the reference: a caller uses it to
+ * bound the time of one call (time-sliced serving) or to read the iterate
+ * of a long solve between slices. */
+NLAPI int NLAPIENTRY nlhipSolveBegin(nlhip_system s, const double* b, const double* x0,
+                                     const nlhip_solve_params* params);
+NLAPI int NLAPIENTRY nlhipSolveAdvance(nlhip_system s, uint32_t iters, int* finished,
+                                       nlhip_solve_stats* stats);
+NLAPI int NLAPIENTRY nlhipSolveEnd(nlhip_system s, double* x, nlhip_solve_stats* stats);
 /* k right-hand sides at once (1 <= k <= 32), sharing every pass over the
  * matrix: B and X are column-major n x k (system j at B + j n, the layout of
  * OpenNL's NL_NB_SYSTEMS); X holds the initial guesses.  Each column has its

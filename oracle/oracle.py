@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# NL_ORACLE_LIB points at another build of nl_oracle.c (the sanitizer build
+# of oracle/Makefile's `sanitized` target).
+_LIB_PATH = os.environ.get("NL_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 _u32p = C.POINTER(C.c_uint32)
 _f64p = C.POINTER(C.c_double)
