@@ -191,6 +191,8 @@ SIGNATURES = {
     "nlhipBatchAutotune": (C.c_int, [_vp, _vp, C.c_uint32, _u32p, _u32p, _f64p]),
     "nlhipSolve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
                              C.POINTER(SolveStats)]),
+    "nlhipServerSubmitSystem": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp,
+                                          C.POINTER(SolveParams), C.POINTER(C.c_uint64)]),
     "nlhipSolveBegin": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams)]),
     "nlhipSolveAdvance": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_int),
                                     C.POINTER(SolveStats)]),
@@ -623,12 +625,29 @@ class Server:
         check(lib().nlhipServerSubmit(self._h, n, ptr(rp), ptr(ci), ptr(va), k, ptr(B),
                                       None if x0 is None else ptr(x0), C.byref(p),
                                       C.byref(rid)), "nlhipServerSubmit")
-        self._n[int(rid.value)] = (k, n)
+        self._n[int(rid.value)] = (k, n, None)
+        return int(rid.value)
+
+    def submit_system(self, system: "System", B, X0=None, **params) -> int:
+        """A request on a resident System (nlhipServerSubmitSystem): only the
+        right-hand sides B ((k, n) or (n,)) travel.  The System is kept alive
+        until the request has been waited for."""
+        B = np.ascontiguousarray(np.atleast_2d(B), dtype=np.float64)
+        k, n = B.shape
+        if n != system.n:
+            raise ValueError("right-hand sides do not match the system")
+        x0 = None if X0 is None else np.ascontiguousarray(np.atleast_2d(X0), dtype=np.float64)
+        p = default_params(**params)
+        rid = C.c_uint64()
+        check(lib().nlhipServerSubmitSystem(self._h, system.handle, k, ptr(B),
+                                            None if x0 is None else ptr(x0), C.byref(p),
+                                            C.byref(rid)), "nlhipServerSubmitSystem")
+        self._n[int(rid.value)] = (k, n, system)
         return int(rid.value)
 
     def wait(self, rid: int, timeout: float = 0.0):
         """Returns (X (k, n), RequestResult); raises NLHIPError on failure."""
-        k, n = self._n.pop(rid)
+        k, n = self._n.pop(rid)[:2]
         X = np.empty((k, n))
         res = RequestResult()
         rc = lib().nlhipServerWait(self._h, rid, ptr(X), C.byref(res), timeout)

@@ -356,13 +356,41 @@ struct PcgState {
     bool precond = false;
     nlhip_solve_params p{};
     uint32_t K = 0;         // iterations per captured graph
-    CGStatus<double> cur{};
+    CGStatus<double> cur{};  // device status of the current recurrence
+    uint32_t base = 0;      // iterations of the recurrences before it (restarts)
     double thresh = 0;      // recursive threshold in force
     uint32_t first_stop = ~0u;  // iterations when the recursive test first held
     double rr = -1, bb = 0;     // true |b - A x|^2 and |b|^2 at the last check
     double last_rr = -1;
     int rounds = 0;
+    int last_kind = 0;      // last confirmation round: 1 continued, 2 restarted
+    uint32_t restarts = 0;
     double device_seconds = 0;
+    uint32_t total() const { return base + uint32_t(cur.iters); }
+};
+
+// The device buffers of one FP64/FP32 PCG: iterate, right-hand side, the
+// recurrence's vectors, per-workgroup partials ((kSpmvGridMul + 3) x grid)
+// and the status block.
+template <typename T>
+struct CGBuffers {
+    T *x, *b, *r, *h, *d, *q, *part;
+    CGStatus<T>* st;
+};
+
+// One way of running the device loop of an FP64 PCG (captured graphs, or
+// direct launches without any allocation); pcg_drive holds the stopping and
+// confirmation logic common to them.
+struct PcgOps {
+    virtual ~PcgOps() = default;
+    // initial residual from the current x, status read back
+    virtual CGStatus<double> start(double eps) = 0;
+    // stop/done cleared, threshold and cap set (continues bitwise)
+    virtual void reopen(const CGStatus<double>& cur, double thresh, int cap) = 0;
+    // iterations until the device loop stops, K per poll, at most ~budget
+    virtual CGStatus<double> run(uint32_t K, uint64_t budget) = 0;
+    // |b - A x|^2 and |b|^2 in FP64
+    virtual void norms(double& rr, double& bb) = 0;
 };
 
 // A CRS matrix resident on the device, with its Jacobi inverse diagonal and
@@ -501,6 +529,17 @@ void default_launch(uint32_t n, uint64_t nnz, int& rpw, uint32_t& grid);
 
 // |b - A x| / |b| in FP64 on the device.
 double true_rel_residual(System& S, const double* b_dev, const double* x_dev);
+// CG launch arguments of system S over the buffers w (code: SpMV engine).
+template <typename T>
+void fill_cg_args(const System& S, const CGBuffers<T>& w, const T* dinv, CGArgs<T>& a,
+                  int& code);
+// Iterations per status poll of a CG on S (about 2 ms of device work).
+uint32_t auto_check_every(const System& S);
+// The FP64 PCG driver: open (initial residual), run to the end or to
+// `until` iterations in all, statistics.
+void pcg_open(PcgState& P, PcgOps& ops, const nlhip_solve_params& p, bool precond, uint32_t K);
+void pcg_drive(PcgState& P, PcgOps& ops, uint64_t until);
+void pcg_stats(const PcgState& P, nlhip_solve_stats& st);
 // Full solve with staging of host/device b and x (takes S.mu).  dinv: the
 // Jacobi inverse diagonal of a separate preconditioner object
 // (nlSolveSystemIterative's P) instead of the system's own; null: the

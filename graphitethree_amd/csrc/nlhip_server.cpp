@@ -19,6 +19,15 @@
 // Larger and multi-right-hand-side requests run on their own resident
 // system.  Validation and the fault-injection hook act on each request
 // before batching, so a failing request fails alone.
+//
+// Resident systems: a request can name a system the caller keeps resident
+// (nlhipServerSubmitSystem): only its right-hand sides travel, nothing of the
+// matrix is uploaded, and it is charged only its solve workspace.  Requests
+// on one system run one at a time in arrival order (the system has one
+// workspace and one stream); while its system is busy, later requests may
+// be admitted before it.  A request on a resident system is never batched,
+// so its result is bitwise that of nlhipSolve / nlhipSolveMulti on the
+// system, alone or in any mix.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -44,6 +53,7 @@ struct Request {
     uint64_t id = 0;
     uint32_t n = 0, k = 1;
     bool batchable = false;
+    System* resident = nullptr;   // a caller's resident system (no upload)
     std::vector<uint32_t> rowptr, colind;
     std::vector<double> val, B, X;
     nlhip_solve_params params{};
@@ -57,6 +67,14 @@ struct Request {
 // Device bytes a request holds while it runs: matrix, Jacobi diagonal, the
 // FP64 CG workspace (or the n x k multi right-hand-side workspace), FP32
 // copies for mixed precision, and partial-sum buffers.
+// Device bytes of the solve workspace alone (a request on a resident system).
+uint64_t workspace_footprint(uint32_t n, uint32_t k, bool mixed) {
+    const uint64_t parts = 4ull * 8 * 1024;
+    uint64_t b = k <= 1 ? 8ull * 8 * n + parts : 7ull * 8 * n * k + parts * k;
+    if (mixed) b += k <= 1 ? 7ull * 4 * n : 6ull * 4 * n * k;
+    return b + (64ull << 10);
+}
+
 uint64_t footprint(uint32_t n, uint64_t nnz, uint32_t k, bool mixed) {
     const uint64_t parts = 4ull * 8 * 1024;                       // per-workgroup partials
     uint64_t b = 4ull * (n + 1) + 12ull * nnz + 8ull * n;        // rowptr, colind, val, dinv
@@ -78,6 +96,7 @@ struct Server {
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Request>> pending;
     std::map<uint64_t, std::shared_ptr<Request>> all;
+    std::map<System*, uint32_t> busy;  // resident systems with a request running
     uint64_t next_id = 1;
     uint64_t in_use = 0, peak_in_use = 0;
     uint32_t running = 0, peak_running = 0;  // requests on the device
@@ -94,15 +113,26 @@ struct Server {
             uint64_t bytes = 0;
             {
                 std::unique_lock<std::mutex> lock(mu);
-                auto admissible = [&] {
-                    return !pending.empty() && solves < slots &&
-                           in_use + pending.front()->bytes <= budget;
+                // the oldest waiting request whose bytes fit; a request on a
+                // busy resident system is passed over until that system is free
+                auto pick = [&]() -> std::deque<std::shared_ptr<Request>>::iterator {
+                    if (solves >= slots) return pending.end();
+                    for (auto it = pending.begin(); it != pending.end(); ++it) {
+                        if ((*it)->resident && busy.count((*it)->resident)) continue;
+                        return in_use + (*it)->bytes <= budget ? it : pending.end();
+                    }
+                    return pending.end();
                 };
-                cv_work.wait(lock, [&] { return (stop && pending.empty()) || admissible(); });
-                if (!admissible()) return;  // stopping and nothing left to run
-                group.push_back(pending.front());
-                pending.pop_front();
+                auto chosen = pending.end();
+                cv_work.wait(lock, [&] {
+                    chosen = pick();
+                    return (stop && pending.empty()) || chosen != pending.end();
+                });
+                if (chosen == pending.end()) return;  // stopping and nothing left to run
+                group.push_back(*chosen);
+                pending.erase(chosen);
                 bytes = group[0]->bytes;
+                if (group[0]->resident) busy[group[0]->resident] += 1;
                 if (group[0]->batchable) {
                     // compatible waiting requests, in arrival order
                     uint64_t rows = group[0]->n;
@@ -136,6 +166,9 @@ struct Server {
                 in_use -= bytes;
                 --solves;
                 running -= uint32_t(group.size());
+                if (System* Rs = group[0]->resident) {
+                    if (--busy[Rs] == 0) busy.erase(Rs);
+                }
                 const double t = now();
                 for (auto& R : group) {
                     R->t_done = t;
@@ -155,8 +188,12 @@ struct Server {
 
     void run(Request& R) {
         R.status = guarded("nlhip server request", [&] {
-            std::unique_ptr<System> S(system_create(R.n, R.rowptr.data(), R.colind.data(),
-                                                    R.val.data(), 0));
+            std::unique_ptr<System> own;
+            System* S = R.resident;
+            if (!S) {
+                own.reset(system_create(R.n, R.rowptr.data(), R.colind.data(), R.val.data(), 0));
+                S = own.get();
+            }
             if (R.k <= 1) {
                 solve(*S, R.B.data(), R.X.data(), R.params, R.stats);
             } else {
@@ -218,6 +255,30 @@ struct Server {
             }
             R.stats.status = rc;
         }
+    }
+
+    // Registers a new request: queued, or failed at once when it can never
+    // fit the budget.
+    void enqueue(const std::shared_ptr<Request>& R, uint64_t* id) {
+        std::lock_guard<std::mutex> lock(mu);
+        if (stop) throw Error(NLHIP_ERR_INVALID, "server is shutting down");
+        R->id = next_id++;
+        R->t_submit = now();
+        all[R->id] = R;
+        *id = R->id;
+        if (R->bytes > budget) {
+            R->status = NLHIP_ERR_BUDGET;
+            R->stats.status = R->status;
+            R->finished = true;
+            R->t_admit = R->t_done = R->t_submit;
+            failed += 1;
+            set_last_error("request needs " + std::to_string(R->bytes) +
+                           " device bytes, more than the whole budget");
+        } else {
+            pending.push_back(R);
+        }
+        cv_work.notify_all();
+        cv_done.notify_all();
     }
 
     // The checks system creation makes, per request: CRS structure, finite
@@ -296,25 +357,36 @@ NLAPI int NLAPIENTRY nlhipServerSubmit(nlhip_server h, uint32_t n, const uint32_
         if (params) R->params = *params;
         R->bytes = footprint(n, nnz, k, R->params.precision != NLHIP_PREC_FP64);
         R->batchable = (k == 1 && n <= kBatchRows);
-        std::lock_guard<std::mutex> lock(S->mu);
-        if (S->stop) throw Error(NLHIP_ERR_INVALID, "server is shutting down");
-        R->id = S->next_id++;
-        R->t_submit = S->now();
-        S->all[R->id] = R;
-        *id = R->id;
-        if (R->bytes > S->budget) {
-            R->status = NLHIP_ERR_BUDGET;
-            R->stats.status = R->status;
-            R->finished = true;
-            R->t_admit = R->t_done = R->t_submit;
-            S->failed += 1;
-            set_last_error("request needs " + std::to_string(R->bytes) +
-                           " device bytes, more than the whole budget");
-        } else {
-            S->pending.push_back(R);
-        }
-        S->cv_work.notify_all();
-        S->cv_done.notify_all();
+        S->enqueue(R, id);
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipServerSubmitSystem(nlhip_server h, nlhip_system sys, uint32_t k,
+                                             const double* B, const double* X0,
+                                             const nlhip_solve_params* params, uint64_t* id) {
+    return guarded("nlhipServerSubmitSystem", [&] {
+        Server* S = srv(h);
+        if (!id) throw Error(NLHIP_ERR_INVALID, "null id");
+        *id = 0;
+        if (!sys) throw Error(NLHIP_ERR_INVALID, "null system handle");
+        if (!B) throw Error(NLHIP_ERR_INVALID, "null right-hand sides");
+        if (k == 0 || k > 32) throw Error(NLHIP_ERR_UNSUPPORTED, "1 to 32 right-hand sides");
+        System* Sys = reinterpret_cast<System*>(sys);
+        const uint32_t n = Sys->n;
+        auto R = std::make_shared<Request>();
+        R->n = n;
+        R->k = k;
+        R->resident = Sys;
+        R->B.assign(B, B + size_t(n) * k);
+        for (double v : R->B)
+            if (!std::isfinite(v))
+                throw Error(NLHIP_ERR_NONFINITE, "right-hand side is NaN or Inf");
+        if (X0) R->X.assign(X0, X0 + size_t(n) * k);
+        else R->X.assign(size_t(n) * k, 0.0);
+        nlhipSolveParamsDefault(&R->params);
+        if (params) R->params = *params;
+        R->bytes = workspace_footprint(n, k, R->params.precision != NLHIP_PREC_FP64);
+        S->enqueue(R, id);
     });
 }
 

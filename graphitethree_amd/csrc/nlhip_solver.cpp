@@ -225,10 +225,10 @@ template struct CGWorkspace<double>;
 template struct CGWorkspace<float>;
 
 template <typename T>
-static void setup_args(System& S, CGWorkspace<T>& ws, const T* dinv) {
-    CGArgs<T>& a = ws.args;
+void fill_cg_args(const System& S, const CGBuffers<T>& w, const T* dinv, CGArgs<T>& a,
+                  int& code) {
     const SpmvOps<T> o = S.ops<T>();
-    ws.code = o.code;
+    code = o.code;
     a.m = S.n;
     a.P = S.grid;
     a.chunk = uint32_t((uint64_t(S.n) + S.grid - 1) / S.grid);
@@ -245,20 +245,29 @@ static void setup_args(System& S, CGWorkspace<T>& ws, const T* dinv) {
     a.cols = o.cols;
     a.val = o.val;
     a.dinv = dinv;
-    a.x = ws.x.get();
-    a.b = ws.b.get();
-    a.r = ws.r.get();
-    a.h = ws.h.get();
-    a.d = ws.d.get();
-    a.q = ws.q.get();
-    a.part_dq = ws.part.get();  // Ps <= kSpmvGridMul P entries
-    a.part_rh = ws.part.get() + size_t(kSpmvGridMul) * S.grid;
-    a.part_rr = ws.part.get() + size_t(kSpmvGridMul + 1) * S.grid;
-    a.part_bb = ws.part.get() + size_t(kSpmvGridMul + 2) * S.grid;
-    a.st = ws.st.get();
+    a.x = w.x;
+    a.b = w.b;
+    a.r = w.r;
+    a.h = w.h;
+    a.d = w.d;
+    a.q = w.q;
+    a.part_dq = w.part;  // Ps <= kSpmvGridMul P entries
+    a.part_rh = w.part + size_t(kSpmvGridMul) * S.grid;
+    a.part_rr = w.part + size_t(kSpmvGridMul + 1) * S.grid;
+    a.part_bb = w.part + size_t(kSpmvGridMul + 2) * S.grid;
+    a.st = w.st;
+}
+template void fill_cg_args<double>(const System&, const CGBuffers<double>&, const double*,
+                                   CGArgs<double>&, int&);
+
+template <typename T>
+static void setup_args(System& S, CGWorkspace<T>& ws, const T* dinv) {
+    const CGBuffers<T> w{ws.x.get(), ws.b.get(), ws.r.get(), ws.h.get(),
+                         ws.d.get(), ws.q.get(), ws.part.get(), ws.st.get()};
+    fill_cg_args<T>(S, w, dinv, ws.args, ws.code);
 }
 
-static uint32_t auto_check_every(const System& S) {
+uint32_t auto_check_every(const System& S) {
     // about 2 ms of device work per captured graph, within [8, 128]
     const double bytes_per_iter = 12.0 * double(S.nnz) + 100.0 * double(S.n);
     const double iter_s = std::max(bytes_per_iter / 4.0e12, 6.0e-6);
@@ -365,91 +374,142 @@ double true_rel_residual(System& S, const double* b, const double* x) {
     return bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
 }
 
-// Opens an FP64 PCG solve on ws64 (b and x already there): initial residual
-// and status on the device, the status read back.
-static void pcg_begin(System& S, const nlhip_solve_params& p, const double* dinv) {
-    PcgState& P = S.pcg;
+// The ways of running an FP64 PCG on the system's ws64: captured graphs
+// replayed with status polls into pinned memory.
+struct GraphPcgOps final : PcgOps {
+    System& S;
+    explicit GraphPcgOps(System& s) : S(s) {}
+    CGStatus<double> start(double eps) override {
+        CGWorkspace<double>& ws = S.ws64;
+        NLHIP_CHECK(cg_start<double>(ws.args, ws.code, S.pcg.precond, eps, 0, S.stream.get()));
+        return read_status(S, ws);
+    }
+    void reopen(const CGStatus<double>& cur, double thresh, int cap) override {
+        cg_reopen<double>(S, S.ws64, cur, thresh, cap);
+    }
+    CGStatus<double> run(uint32_t K, uint64_t budget) override {
+        return run_ladder<double>(S, S.ws64, S.pcg.precond, K, budget);
+    }
+    void norms(double& rr, double& bb) override {
+        residual_norms(S, S.ws64.b.get(), S.ws64.x.get(), nullptr, rr, bb);
+    }
+};
+
+// Most confirmation rounds of one solve.
+constexpr int kMaxConfirmRounds = 8;
+
+void pcg_open(PcgState& P, PcgOps& ops, const nlhip_solve_params& p, bool precond, uint32_t K) {
     P = PcgState{};
-    CGWorkspace<double>& ws = S.ws64;
     P.p = p;
-    P.precond = p.precond == NL_PRECOND_JACOBI;
-    setup_args<double>(S, ws, P.precond ? (dinv ? dinv : S.dinv.get()) : nullptr);
-    P.K = p.check_every ? p.check_every : auto_check_every(S);
-    NLHIP_CHECK(cg_start<double>(ws.args, ws.code, P.precond, p.eps, 0, S.stream.get()));
-    P.cur = read_status(S, ws);
+    P.precond = precond;
+    P.K = K;
+    P.cur = ops.start(p.eps);
     P.thresh = P.cur.thresh;
     P.active = true;
 }
 
-// Most times the confirmation lowers the threshold before it gives up.
-constexpr int kMaxConfirmRounds = 8;
+// Starts a new recurrence from the current iterate (residual replacement):
+// r = A x - b recomputed in FP64, d = h = P r, the iteration count carried
+// over in `base`.
+static void pcg_restart(PcgState& P, PcgOps& ops) {
+    P.base = P.total();
+    P.cur = ops.start(P.p.eps);
+    P.thresh = P.cur.thresh;
+    ++P.restarts;
+}
 
 // Runs the open solve until it stops for good or has done `until`
 // iterations in all (the end of a slice).  Each stop of the device loop is
-// the recursive test (err <= thresh), the slice's cap or max_iter; after the
-// test, with p.confirm, the true residual decides: met -> finished; above
-// eps |b| -> the loop re-opens with the threshold scaled down by the ratio
-// measured, until it is met, max_iter is reached, or it stops decreasing.
-static void pcg_advance(System& S, uint64_t until) {
-    PcgState& P = S.pcg;
-    CGWorkspace<double>& ws = S.ws64;
+// the recursive test (err <= thresh), the slice's cap or max_iter.  After the
+// test, with p.confirm, the true residual |b - A x| decides: met -> finished;
+// still above eps |b| -> one more round, either
+//  * continued: the same recurrence goes on with the threshold scaled down
+//    by the ratio measured (the iterates stay those of a solve with a smaller
+//    threshold), when the true residual is within 2x of the recursive one, or
+//  * restarted: a new recurrence from the current iterate (residual
+//    replacement), when rounding has made the recursive residual drift below
+//    the true one by more than that, or a continued round did not lower the
+//    true residual by 1 %;
+// and the solve gives up (not converged) at max_iter, after kMaxConfirmRounds
+// rounds, or when a restarted round did not lower the true residual by 1 %.
+void pcg_drive(PcgState& P, PcgOps& ops, uint64_t until) {
     const nlhip_solve_params& p = P.p;
-    Event e0, e1;
-    e0.record(S.stream.get());
     while (!P.finished) {
-        const uint32_t its = uint32_t(P.cur.iters);
+        const uint32_t its = P.total();
         if (P.ran && its >= until) break;  // end of the slice
         const uint32_t cap = uint32_t(std::min<uint64_t>(until, p.max_iter));
-        cg_reopen<double>(S, ws, P.cur, P.thresh, int(cap));
-        P.cur = run_ladder<double>(S, ws, P.precond, P.K, uint64_t(cap - std::min(cap, its)) + 1);
+        ops.reopen(P.cur, P.thresh, int(cap - P.base));
+        P.cur = ops.run(P.K, uint64_t(cap - std::min(cap, its)) + 1);
         P.ran = true;
         if (!std::isfinite(P.cur.bb) || !std::isfinite(P.cur.err))
             throw Error(NLHIP_ERR_NONFINITE, "right-hand side or iterate is NaN or Inf");
         if (P.cur.err > P.thresh) {
-            if (uint32_t(P.cur.iters) >= p.max_iter) P.finished = true;
+            if (P.total() >= p.max_iter) P.finished = true;
             continue;
         }
-        if (P.first_stop == ~0u) P.first_stop = uint32_t(P.cur.iters);
+        if (P.first_stop == ~0u) P.first_stop = P.total();
         if (!p.confirm) {
             P.finished = true;
             break;
         }
-        residual_norms(S, ws.b.get(), ws.x.get(), nullptr, P.rr, P.bb);
+        ops.norms(P.rr, P.bb);
         const double target = (p.eps * p.eps) * P.bb;
-        if (!(P.rr > target) || uint32_t(P.cur.iters) >= p.max_iter || !(P.cur.err > 0.0) ||
-            (P.last_rr >= 0.0 && P.rr > 0.99 * P.last_rr) || P.rounds >= kMaxConfirmRounds) {
+        const bool improved = P.last_rr < 0.0 || P.rr <= 0.99 * P.last_rr;
+        if (!(P.rr > target) || P.total() >= p.max_iter || P.rounds >= kMaxConfirmRounds ||
+            (!improved && P.last_kind == 2)) {
             P.finished = true;
             break;
         }
         P.last_rr = P.rr;
         ++P.rounds;
-        P.thresh = P.cur.err * (target / P.rr) * 0.5;
+        if (!(P.cur.err > 0.0) || P.rr > 4.0 * P.cur.err || !improved) {
+            P.last_kind = 2;
+            pcg_restart(P, ops);
+        } else {
+            P.last_kind = 1;
+            P.thresh = P.cur.err * (target / P.rr) * 0.5;
+        }
     }
-    e1.record(S.stream.get());
-    e1.sync();
-    P.device_seconds += 1e-3 * elapsed_ms(e0, e1);
 }
 
-// Statistics of the open solve (device_seconds: the loop's own time).
-static void pcg_stats(System& S, nlhip_solve_stats& st) {
-    const PcgState& P = S.pcg;
-    st.iterations = uint32_t(P.cur.iters);
-    st.outer_iterations = 0;
+void pcg_stats(const PcgState& P, nlhip_solve_stats& st) {
+    st.iterations = P.total();
+    st.outer_iterations = P.restarts;
     const double bb = P.cur.bb;
     st.cg_rel_residual = bb > 0 ? std::sqrt(P.cur.err / bb) : std::sqrt(P.cur.err);
     if (P.p.confirm && P.rr >= 0.0 && P.first_stop != ~0u)
         st.converged = !(P.rr > (P.p.eps * P.p.eps) * P.bb) ? 1 : 0;
     else
         st.converged = !(P.cur.err > P.cur.thresh) && P.first_stop != ~0u ? 1 : 0;
-    st.extra_iterations = P.first_stop == ~0u ? 0u : uint32_t(P.cur.iters) - P.first_stop;
+    st.extra_iterations = P.first_stop == ~0u ? 0u : P.total() - P.first_stop;
     st.device_seconds = P.device_seconds;
+}
+
+// Opens an FP64 PCG solve on ws64 (b and x already there): initial residual
+// and status on the device, the status read back.
+static void pcg_begin(System& S, const nlhip_solve_params& p, const double* dinv) {
+    const bool precond = p.precond == NL_PRECOND_JACOBI;
+    setup_args<double>(S, S.ws64, precond ? (dinv ? dinv : S.dinv.get()) : nullptr);
+    GraphPcgOps ops(S);
+    pcg_open(S.pcg, ops, p, precond, p.check_every ? p.check_every : auto_check_every(S));
+}
+
+// pcg_drive on the graph path, timed with HIP events.
+static void pcg_advance(System& S, uint64_t until) {
+    GraphPcgOps ops(S);
+    Event e0, e1;
+    e0.record(S.stream.get());
+    pcg_drive(S.pcg, ops, until);
+    e1.record(S.stream.get());
+    e1.sync();
+    S.pcg.device_seconds += 1e-3 * elapsed_ms(e0, e1);
 }
 
 static void solve_fp64(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st,
                        const double* dinv) {
     pcg_begin(S, p, dinv);
     pcg_advance(S, p.max_iter);
-    pcg_stats(S, st);
+    pcg_stats(S.pcg, st);
     S.pcg.active = false;
 }
 
@@ -565,7 +625,7 @@ static void sliced_advance(System& S, uint32_t iters, int* finished, nlhip_solve
     if (finished) *finished = S.pcg.finished ? 1 : 0;
     if (st) {
         std::memset(st, 0, sizeof(*st));
-        pcg_stats(S, *st);
+        pcg_stats(S.pcg, *st);
         st->rel_residual = S.pcg.rr >= 0.0 && S.pcg.bb > 0 ? std::sqrt(S.pcg.rr / S.pcg.bb)
                                                             : st->cg_rel_residual;
     }
@@ -576,7 +636,7 @@ static void sliced_end(System& S, double* x, nlhip_solve_stats* st) {
     if (!S.pcg.active) throw Error(NLHIP_ERR_INVALID, "no sliced solve is open on this system");
     if (st) {
         std::memset(st, 0, sizeof(*st));
-        pcg_stats(S, *st);
+        pcg_stats(S.pcg, *st);
         st->rel_residual = true_rel_residual(S, S.ws64.b.get(), S.ws64.x.get());
     }
     if (x) copy_out(x, S.ws64.x.get(), size_t(S.n) * 8, S.stream.get());

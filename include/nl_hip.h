@@ -81,20 +81,28 @@ typedef struct {
     uint32_t max_outer;       /* mixed: refinement steps cap                 */
     uint32_t inner_max_iter;  /* mixed: cap per inner CG (0: max_iter)       */
     uint32_t confirm;         /* FP64 CG: when the recursive residual meets
-                                 eps, recompute |b - A x| in FP64 and, while
-                                 it is still above eps |b|, continue the same
-                                 recurrence (bitwise the iterates of a solve
-                                 with a smaller threshold) with the recursive
-                                 threshold lowered by the measured gap; stops
-                                 when the true residual meets eps, at max_iter,
-                                 or when it no longer decreases (default 1;
+                                 eps, recompute |b - A x| in FP64; while it
+                                 is still above eps |b|, run another round:
+                                 the same recurrence continued with the
+                                 recursive threshold lowered by the measured
+                                 gap (bitwise the iterates of a solve with a
+                                 smaller threshold) when the true residual
+                                 is within 2x of the recursive one, else a
+                                 restart from the current iterate with the
+                                 residual recomputed (residual replacement,
+                                 for drift from rounding on long solves);
+                                 stops when the true residual meets eps, at
+                                 max_iter, after 8 rounds, or when a
+                                 restart no longer lowers it (default 1;
                                  0: stop on the recursive residual alone, as
-                                 OpenNL's CG does)                          */
+                                 OpenNL's CG does).  Batched and multi-RHS
+                                 solves continue only (no restart).          */
 } nlhip_solve_params;
 
 typedef struct {
     uint32_t iterations;      /* CG iterations (mixed: sum of inner)         */
-    uint32_t outer_iterations;/* mixed: refinement steps                     */
+    uint32_t outer_iterations;/* mixed: refinement steps; FP64 CG: restarts
+                                 of the confirmation                         */
     int32_t converged;        /* 1 when |Ax-b| <= eps |b| was reached        */
     int32_t status;           /* NLHIP_OK or error code                      */
     double rel_residual;      /* |b-Ax|/|b| recomputed in FP64 at the end    */
@@ -357,6 +365,21 @@ NLAPI int NLAPIENTRY nlhipServerSubmit(nlhip_server s, uint32_t n, const uint32_
                                        const uint32_t* colind, const double* val,
                                        uint32_t k, const double* B, const double* X0,
                                        const nlhip_solve_params* params, uint64_t* id);
+/* A request against a system the caller keeps resident (nlhipSystemCreate,
+ * nlhipLSCMCreate, nlhipLeastSquaresCreate, ...): only the k right-hand sides
+ * B (column-major n x k; X0 may be NULL) travel, the matrix is not uploaded
+ * again, and the request is charged only its solve workspace against the
+ * budget.  Requests on one system run one at a time in arrival order; while
+ * the system is busy later requests may be admitted first.  The result is
+ * bitwise that of nlhipSolve (k = 1) or nlhipSolveMulti on the system.  The
+ * system must stay alive, and no other call may use it concurrently, until
+ * nlhipServerWait has returned for every request on it.  No counterpart in
+ * the reference: its callers re-solve one matrix many times (the xyz
+ * smoothing of mesh_grob_surface_commands.cpp:687-705, the Newton loop of
+ * Transport_2d_04.lua:18-32). */
+NLAPI int NLAPIENTRY nlhipServerSubmitSystem(nlhip_server s, nlhip_system system, uint32_t k,
+                                             const double* B, const double* X0,
+                                             const nlhip_solve_params* params, uint64_t* id);
 /* Blocks until request `id` is finished (timeout_seconds <= 0: no limit),
  * copies its solution into X (n x k, may be null) and releases it.  Returns
  * the request's own status. */

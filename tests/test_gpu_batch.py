@@ -37,8 +37,8 @@ def test_batch_matches_oracle_per_chart(hip, small_atlas):
         xr, it, _ = O.pcg(O.CRS(S.rowptr, S.colind, S.val), S.b, eps=1e-10, max_iter=5000)
         print(f"chart {c}: n={S.n} it gpu/cpu {per[c].iterations}/{it} "
               f"rel_res {per[c].rel_residual:.2e} diff {rel(xs[c], xr):.2e}")
-        assert same_iters(per[c].iterations, it)
-        assert per[c].converged == 1 and per[c].rel_residual <= 1.01e-10
+        assert same_iters(per[c].iterations - per[c].extra_iterations, it)
+        assert per[c].converged == 1 and per[c].rel_residual <= 1e-10
         assert rel(xs[c], xr) <= REL_DIFF
     assert tot.converged == 1
     assert tot.iterations == max(p.iterations for p in per)
@@ -85,7 +85,7 @@ def test_batch_mixed_forms_at_any_offset(hip, small_atlas):
     Bm = capi.Batch(mix, fp32=True)
     xm, pm, _ = Bm.solve(np.concatenate([S.b for S in mix]), precision="mixed", eps=1e-10,
                          max_iter=200000, inner_eps=1e-4)
-    assert all(p.rel_residual <= 1.01e-10 for p in pm)
+    assert all(p.rel_residual <= 1e-10 for p in pm)
 
 
 def test_batch_launch_shape_does_not_change_bits(hip, small_atlas):
@@ -138,7 +138,9 @@ def test_batch_mixed_precision(hip, small_atlas):
         r = O.spmv(O.CRS(S.rowptr, S.colind, S.val), a) - S.b
         assert np.linalg.norm(r) / np.linalg.norm(S.b) <= 1e-10, c
         assert pm[c].converged == 1 and pm[c].outer_iterations >= 1
-        assert rel(a, b) < 1e-6
+        xo, _, _ = O.pcg(O.CRS(S.rowptr, S.colind, S.val), S.b, eps=1e-10, max_iter=50000)
+        assert rel(a, xo) <= REL_DIFF, c
+        assert rel(b, xo) <= REL_DIFF, c
     assert tm.converged == 1
 
 
@@ -157,9 +159,10 @@ def test_batch_mixed_with_fp64_fallback(hip, small_atlas):
     for c, (a, b) in enumerate(zip(B.split(xf), B.split(x64))):
         S = A.chart(c)
         r = O.spmv(O.CRS(S.rowptr, S.colind, S.val), a) - S.b
-        assert np.linalg.norm(r) / np.linalg.norm(S.b) <= 1.01e-10, c
+        assert np.linalg.norm(r) / np.linalg.norm(S.b) <= 1e-10, c
         assert pf[c].converged == 1
-        assert rel(a, b) < 1e-6
+        xo, _, _ = O.pcg(O.CRS(S.rowptr, S.colind, S.val), S.b, eps=1e-10, max_iter=50000)
+        assert rel(a, xo) <= REL_DIFF, c
     assert tf.converged == 1
     xm, _, _ = B.solve(A.b, precision="mixed", eps=1e-10, max_iter=200000, inner_eps=1e-4,
                        inner_max_iter=100000)

@@ -32,8 +32,8 @@ def test_multi_rhs_matches_oracle_columns(hip, k):
     A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
     for j in range(k):
         xr, it, _ = O.pcg(A, B[j], eps=1e-10, max_iter=5000)
-        assert same_iters(per[j].iterations, it), (j, per[j].iterations, it)
-        assert per[j].converged == 1 and per[j].rel_residual <= 1.01e-10
+        assert same_iters(per[j].iterations - per[j].extra_iterations, it), (j, per[j].iterations, it)
+        assert per[j].converged == 1 and per[j].rel_residual <= 1e-10
         assert rel(X[j], xr) <= REL_DIFF, j
     assert tot.iterations == max(p.iterations for p in per)
 
@@ -46,7 +46,7 @@ def test_multi_rhs_smoothing_xyz(hip):
     A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
     for j in range(3):
         xr, it, _ = O.pcg(A, Sw.rhs[j], eps=1e-10, max_iter=1000)
-        assert same_iters(per[j].iterations, it)
+        assert same_iters(per[j].iterations - per[j].extra_iterations, it)
         assert rel(X[j], xr) <= REL_DIFF
         xs, st = S.solve(Sw.rhs[j], eps=1e-10, max_iter=1000)
         assert rel(X[j], xs) <= REL_DIFF
