@@ -40,7 +40,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"
               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
 NLHIP_HIP = ["k_spmv.hip", "k_blas.hip", "k_cg.hip", "k_batch.hip", "k_spmm.hip",
-             "k_assemble.hip", "k_eigen.hip", "k_ssor.hip"]
+             "k_assemble.hip", "k_eigen.hip", "k_ssor.hip", "nlhip_noalloc.hip"]
 NLHIP_CPP = ["nl_api.cpp", "nlhip_device.cpp", "nlhip_solver.cpp",
              "nlhip_batch.cpp", "nlhip_multi.cpp", "nlhip_server.cpp", "nlhip_assemble.cpp",
              "nlhip_eigen.cpp", "nlhip_ssor.cpp"]

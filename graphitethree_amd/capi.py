@@ -193,6 +193,8 @@ SIGNATURES = {
                              C.POINTER(SolveStats)]),
     "nlhipServerSubmitSystem": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp,
                                           C.POINTER(SolveParams), C.POINTER(C.c_uint64)]),
+    "nlhipSolveNoAlloc": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
+                                    C.POINTER(SolveStats)]),
     "nlhipSolveBegin": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams)]),
     "nlhipSolveAdvance": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_int),
                                     C.POINTER(SolveStats)]),

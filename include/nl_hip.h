@@ -233,6 +233,21 @@ NLAPI int NLAPIENTRY nlhipSolveBegin(nlhip_system s, const double* b, const doub
 NLAPI int NLAPIENTRY nlhipSolveAdvance(nlhip_system s, uint32_t iters, int* finished,
                                        nlhip_solve_stats* stats);
 NLAPI int NLAPIENTRY nlhipSolveEnd(nlhip_system s, double* x, nlhip_solve_stats* stats);
+/* nlhipSolve without any allocation, for callers that cannot allocate
+ * device or pinned memory (serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  FP64 CG only (Jacobi or no
+ * preconditioner, params->confirm as for nlhipSolve); b and x host or device.
+ * Every work buffer lives in one fixed device array of the library, sized for
+ * systems of at most NLHIP_NOALLOC_MAX_ROWS rows (larger ones are rejected
+ * with NLHIP_ERR_INVALID) and held by each GPU the library runs on until the
+ * process ends; the kernels are launched directly (no graph instantiation).
+ * The result, iteration count and residuals are bitwise nlhipSolve's;
+ * device_seconds is host wall time.  Calls that overlap on one GPU (any
+ * systems) share the array and race: serialise them. */
+#define NLHIP_NOALLOC_MAX_ROWS 2097152u
+NLAPI int NLAPIENTRY nlhipSolveNoAlloc(nlhip_system s, const double* b, double* x,
+                                       const nlhip_solve_params* params,
+                                       nlhip_solve_stats* stats);
 /* k right-hand sides at once (1 <= k <= 32), sharing every pass over the
  * matrix: B and X are column-major n x k (system j at B + j n, the layout of
  * OpenNL's NL_NB_SYSTEMS); X holds the initial guesses.  Each column has its
