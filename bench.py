@@ -43,10 +43,14 @@ COPY_PROBE_BYTES = 1 << 30
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_cg_spmv_lscm2236_traffic.md")
 
 
-def measured_traffic(workload: str):
+def measured_traffic(workload: str, engine: int, workgroups: int):
     """HBM bytes per k_cg_spmv launch from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a plain copy), when
-    it covers this workload; None otherwise."""
+    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a plain copy), for
+    the launch shape this run uses (engine code, workgroups), when the summary
+    covers this workload and that shape; None otherwise (a summary of another
+    shape says nothing of this one).  Engine code 64 is the library's default
+    stream engine, which runs as code 73 on a matrix with the 2x2 block form
+    (every LSCM grid)."""
     try:
         with open(TRAFFIC_FILE) as f:
             kv = dict(line.split(":", 1) for line in f if line.startswith("- ") and ":" in line)
@@ -55,8 +59,9 @@ def measured_traffic(workload: str):
     kv = {k[2:].strip(): v.strip() for k, v in kv.items()}
     if kv.get("workload") != workload:
         return None
+    code = 73 if engine == 64 else engine
     try:
-        return float(kv["traffic_bytes_per_launch"])
+        return float(kv[f"traffic[{code},{workgroups}]"])
     except (KeyError, ValueError):
         return None
 
@@ -334,7 +339,7 @@ def main():
     tuned = None
     if args.rows_per_wave or args.workgroups:
         S.set_launch(args.rows_per_wave or S.launch()[0], args.workgroups or S.launch()[1])
-    elif not args.no_autotune and not args.traffic_probe:
+    elif not args.no_autotune:
         tuned = S.autotune(iters=20)
         log(f"autotuned: engine {tuned[0]}, {tuned[1]} workgroups, "
             f"{tuned[2] * 1e3:.3f} ms per CG iteration")
@@ -345,13 +350,17 @@ def main():
         t_copy = capi.bench_copy(COPY_PROBE_BYTES, reps=3)
         t_dir, t_a, t_upd = capi.profile_cg(S, iters=10)
         print(json.dumps({"probe": "traffic", "copy_bytes": COPY_PROBE_BYTES, "copy_s": t_copy,
-                          "k_cg_spmv_s": t_a, "spmv_bytes": spmv_bytes(n, nnz)}), flush=True)
+                          "k_cg_spmv_s": t_a, "spmv_bytes": spmv_bytes(n, nnz),
+                          "engine": lpr, "workgroups": wg}), flush=True)
         return
 
     # ---- plain copy: the bandwidth this device sustains for a streaming kernel
     t_copy = capi.bench_copy(2 << 30, reps=20)
     copy_gbs = 2 * (2 << 30) / t_copy / 1e9
     log(f"copy 2 GiB: {t_copy * 1e3:.3f} ms  {copy_gbs:.0f} GB/s (read + write)")
+    t_read = capi.bench_read(4 << 30, reps=20)
+    read_gbs = (4 << 30) / t_read / 1e9
+    log(f"read 4 GiB: {t_read * 1e3:.3f} ms  {read_gbs:.0f} GB/s")
 
     # ---- standalone SpMV bandwidth (FP64, and FP32 values)
     t_spmv = S.bench_spmv(fp32=False, reps=50)
@@ -510,9 +519,10 @@ def main():
         "iteration_gbs": iter_gbs,
         "roofline": {"bound": "hbm", "kernel": "k_cg_spmv", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"lscm_grid_{args.k}"),
+                     "traffic": measured_traffic(f"lscm_grid_{args.k}", lpr, wg),
                      "bytes_per_launch": bytes_a, "launch_ms": t_a * 1e3,
-                     "copy_gbs": copy_gbs},
+                     "copy_gbs": copy_gbs, "read_gbs": read_gbs,
+                     "frac_of_read": achieved / read_gbs},
         "kernels_ms": {"k_cg_dir": t_dir * 1e3, "k_cg_spmv": t_a * 1e3,
                        "k_cg_update": t_upd * 1e3},
         "cpu_baseline": cpu,

@@ -99,15 +99,24 @@ def run_atlas(args) -> dict:
     t_plain = 0.0
     worst_diff = 0.0
     its_plain = []
+    per_chart = []   # (n, fp64 s, fp64 it, mixed s, mixed inner it) per sampled chart
+    mp = dict(precision="mixed", inner_eps=args.inner_eps, max_iter=args.max_iter_mixed,
+              eps=args.eps)
     for k, c in enumerate(charts):
         S = A.chart(c)
-        sysc = capi.System.from_crs(S)
+        sysc = capi.System.from_crs(S, fp32=args.per_chart_mixed)
         sysc.solve(S.b, **params)                      # warm-up
         x, st = sysc.solve(S.b, **params)
         t_plain += st.device_seconds
         its_plain.append(int(st.iterations))
         worst_diff = max(worst_diff, float(np.linalg.norm(x - xs64[c]) /
                                            max(np.linalg.norm(xs64[c]), 1e-300)))
+        row = [S.n, st.device_seconds, int(st.iterations), None, None]
+        if args.per_chart_mixed:
+            sysc.solve(S.b, **mp)
+            _, sm = sysc.solve(S.b, **mp)
+            row[3], row[4] = sm.device_seconds, int(sm.iterations)
+        per_chart.append(row)
         sysc.close()
         if k % 200 == 0:
             log(f"  per-chart baseline {k}/{len(charts)} ({t_plain:.2f}s so far)")
@@ -156,6 +165,29 @@ def run_atlas(args) -> dict:
                      "batch_create_from_packed_seconds": t_packed_create,
                      "device_assembly": assembly}
     res["speedup_mixed_batched_vs_per_chart_fp64"] = t_plain_all / tm.device_seconds
+    # per chart-size band: iterations of FP32-inner refinement against FP64
+    # PCG (from the batched solves: each chart's counts are its own) and the
+    # per-chart solve times of both
+    edges = [0, 1000, 10000, 10 ** 9]
+    bands = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        sel = [i for i, c in enumerate(charts) if lo <= per_chart[i][0] < hi]
+        allc = [c for c in range(A.count) if lo <= int(A.row_off[c + 1] - A.row_off[c]) < hi]
+        if not allc:
+            continue
+        it64 = np.array([p64[c].iterations for c in allc], dtype=float)
+        itm = np.array([pm[c].iterations for c in allc], dtype=float)
+        band = {"unknowns": [lo, hi if hi < 10 ** 9 else None], "charts": len(allc),
+                "fp64_iterations_mean": float(it64.mean()),
+                "mixed_inner_iterations_mean": float(itm.mean()),
+                "inner_iteration_inflation": float((itm / np.maximum(it64, 1)).mean())}
+        if sel:
+            band["per_chart_fp64_s"] = float(sum(per_chart[i][1] for i in sel) * scale)
+            if args.per_chart_mixed:
+                band["per_chart_mixed_s"] = float(sum(per_chart[i][3] for i in sel) * scale)
+                band["mixed_over_fp64_time"] = band["per_chart_mixed_s"] / band["per_chart_fp64_s"]
+        bands.append(band)
+    res["bands"] = bands
     res["speedup_fp64_batched_vs_per_chart_fp64"] = t_plain_all / t64.device_seconds
     return res
 
@@ -500,6 +532,8 @@ def main(argv=None):
                    help="inner_max_iter of the mixed solve with FP64 fallback")
     a.add_argument("--direct-launches", action="store_true",
                    help="no graph replay (kernel tracing); same results")
+    a.add_argument("--per-chart-mixed", action="store_true",
+                   help="also time one FP32-inner refined solve per sampled chart")
     at = sub.add_parser("atlas-trace",
                         help="the batched FP64 atlas solve alone, launched directly")
     at.add_argument("--count", type=int, default=2000)

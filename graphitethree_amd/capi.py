@@ -204,6 +204,7 @@ SIGNATURES = {
     "nlhipSpMV": (C.c_int, [_vp, _vp, _vp]),
     "nlhipBenchSpMV": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
     "nlhipBenchCopy": (C.c_int, [C.c_uint64, C.c_uint32, _f64p]),
+    "nlhipBenchRead": (C.c_int, [C.c_uint64, C.c_uint32, _f64p]),
     "nlhipProfileCG": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
     "nlhipBatchCreate": (C.c_int, [C.c_uint32, _vp, _vp, _vp, _vp, C.c_uint32,
                                    C.POINTER(_vp)]),
@@ -683,6 +684,13 @@ def bench_copy(nbytes: int, reps: int = 20) -> float:
     """Mean seconds of a plain device copy of `nbytes` (HIP events)."""
     t = C.c_double()
     check(lib().nlhipBenchCopy(nbytes, reps, C.byref(t)), "nlhipBenchCopy")
+    return float(t.value)
+
+
+def bench_read(nbytes: int, reps: int = 20) -> float:
+    """Mean seconds of a read-only pass over `nbytes` of device memory."""
+    t = C.c_double()
+    check(lib().nlhipBenchRead(nbytes, reps, C.byref(t)), "nlhipBenchRead")
     return float(t.value)
 
 

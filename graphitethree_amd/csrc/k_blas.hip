@@ -132,6 +132,59 @@ hipError_t copy16(uint64_t n16, const void* src, void* dst, uint32_t grid, hipSt
     return hipGetLastError();
 }
 
+// Read-only stream (the bandwidth reference for read-dominated kernels such
+// as the SpMV): every thread loads eight 16-byte words per trip, all loads
+// before their use, and folds them into one word written at the end (so the
+// loads are not removed); NT: non-temporal loads.
+template <bool NT>
+__global__ __launch_bounds__(kBlock) void k_read16(uint64_t n16, const uint4* __restrict__ src,
+                                                  uint4* __restrict__ out) {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+    for (; i + 7 * stride < n16; i += 8 * stride) {
+        uint4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (NT) {
+                using V4 = unsigned int __attribute__((ext_vector_type(4)));
+                const V4 w = __builtin_nontemporal_load(reinterpret_cast<const V4*>(src + i + u * stride));
+                v[u] = make_uint4(w.x, w.y, w.z, w.w);
+            } else {
+                v[u] = src[i + u * stride];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            acc.x ^= v[u].x;
+            acc.y ^= v[u].y;
+            acc.z ^= v[u].z;
+            acc.w ^= v[u].w;
+        }
+    }
+    for (; i < n16; i += stride) {
+        const uint4 v = src[i];
+        acc.x ^= v.x;
+        acc.y ^= v.y;
+        acc.z ^= v.z;
+        acc.w ^= v.w;
+    }
+    out[uint64_t(blockIdx.x) * kBlock + threadIdx.x] = acc;
+}
+
+hipError_t read16(uint64_t n16, const void* src, void* out, uint32_t grid, bool nt,
+                  hipStream_t s) {
+    if (n16 == 0) return hipSuccess;
+    if (grid == 0) return hipErrorInvalidValue;
+    if (nt)
+        k_read16<true><<<grid, kBlock, 0, s>>>(n16, static_cast<const uint4*>(src),
+                                               static_cast<uint4*>(out));
+    else
+        k_read16<false><<<grid, kBlock, 0, s>>>(n16, static_cast<const uint4*>(src),
+                                                static_cast<uint4*>(out));
+    return hipGetLastError();
+}
+
 hipError_t reduce_final(const double* part, uint32_t P, double* out, hipStream_t s) {
     k_reduce_final<<<1, kBlock, 0, s>>>(part, P, out);
     return hipGetLastError();

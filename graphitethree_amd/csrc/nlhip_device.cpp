@@ -1,6 +1,7 @@
 // Device handle, streams, events, staging copies and error reporting.
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 #include "nlhip_internal.h"
 
@@ -107,6 +108,14 @@ void copy_in(void* dst, const void* src, size_t bytes, hipStream_t s) {
 void copy_out(void* dst, const void* src, size_t bytes, hipStream_t s) {
     if (bytes == 0) return;
     NLHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s));
+}
+
+bool direct_launches() {
+    static const bool on = [] {
+        const char* v = std::getenv("NLHIP_DIRECT_LAUNCH");
+        return v != nullptr && v[0] != '\0' && std::strcmp(v, "0") != 0;
+    }();
+    return on;
 }
 
 std::shared_mutex& capture_mutex() {

@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <shared_mutex>
 #include <type_traits>
@@ -259,12 +260,22 @@ class Graph {
 // ..., K, K, ...).  The host learns that a solve stopped one graph late, so
 // a solve that stops after a few iterations replays few launches past its
 // stopping test, and a long one still runs K iterations per graph.
+//
+// With NLHIP_DIRECT_LAUNCH=1 in the environment (read once), nothing is
+// captured: launch() enqueues the same kernels one by one through the
+// recording function of the latest ensure() call, which must come from the
+// same scope as the launches (every caller does both in one function).
+// Same kernels, same arguments, same results; for tools that trace kernel
+// launches but not graph replays.
+bool direct_launches();
+
 class GraphLadder {
   public:
     void reset() {
         for (auto& g : g_) g.reset();
         n_ = 0;
         cap_ = 0;
+        direct_ = nullptr;
     }
     // Captures the graphs for a cap of K on stream s (record(k) enqueues k
     // iterations); keeps them while K is unchanged (reset() when anything
@@ -272,6 +283,19 @@ class GraphLadder {
     template <typename F>
     void ensure(uint32_t K, hipStream_t s, F&& record) {
         if (K == 0) K = 1;
+        if (direct_launches()) {
+            direct_ = std::function<void(uint32_t)>(record);
+            if (n_ && K == cap_) return;
+            n_ = 0;
+            uint32_t k = K < kFirst ? K : kFirst;
+            for (;;) {
+                iters_[n_++] = k;
+                if (k >= K || n_ == kLevels) break;
+                k = 2 * k < K ? 2 * k : K;
+            }
+            cap_ = K;
+            return;
+        }
         if (n_ && K == cap_) return;
         reset();
         uint32_t k = K < kFirst ? K : kFirst;
@@ -288,7 +312,13 @@ class GraphLadder {
     uint32_t cap() const { return cap_; }
     // Iterations in the i-th launch (0-based) of a solve, and its launch.
     uint32_t iters(uint64_t i) const { return iters_[level(i)]; }
-    void launch(uint64_t i, hipStream_t s) const { g_[level(i)].launch(s); }
+    void launch(uint64_t i, hipStream_t s) const {
+        if (direct_) {
+            direct_(iters(i));
+            return;
+        }
+        g_[level(i)].launch(s);
+    }
 
   private:
     static constexpr int kLevels = 12;
@@ -296,6 +326,7 @@ class GraphLadder {
     int level(uint64_t i) const {
         return i == 0 ? 0 : int(i - 1 < uint64_t(n_ - 1) ? i - 1 : uint64_t(n_ - 1));
     }
+    std::function<void(uint32_t)> direct_;
     Graph g_[kLevels];
     uint32_t iters_[kLevels] = {};
     int n_ = 0;

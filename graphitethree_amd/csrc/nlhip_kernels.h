@@ -105,6 +105,9 @@ hipError_t blas_daxpy(uint64_t n, double a, const double* x, double* y, hipStrea
 hipError_t blas_dmul(uint64_t n, const double* a, const double* x, double* y, hipStream_t s);
 // dst[0..16 n16) = src[0..16 n16) with `grid` workgroups.
 hipError_t copy16(uint64_t n16, const void* src, void* dst, uint32_t grid, hipStream_t s);
+// Reads n16 16-byte words; out receives grid x 256 words.
+hipError_t read16(uint64_t n16, const void* src, void* out, uint32_t grid, bool nt,
+                  hipStream_t s);
 template <typename T>
 hipError_t fill(uint64_t n, T v, T* x, hipStream_t s);
 // out[0] = x.y, out[1] = sqrt(x.y); part holds `grid` doubles.

@@ -269,6 +269,11 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
  * seconds per copy of the fastest (HIP events).  The bandwidth reference the
  * solver kernels are compared with. */
 NLAPI int NLAPIENTRY nlhipBenchCopy(uint64_t bytes, uint32_t reps, double* seconds_per_copy);
+/* Times `reps` read-only passes over `bytes` (a multiple of 16) of a buffer it
+ * allocates (16-byte loads, plain and non-temporal, a few grid sizes);
+ * returns the mean seconds per pass of the fastest.  The bandwidth reference
+ * for read-dominated kernels such as the SpMV. */
+NLAPI int NLAPIENTRY nlhipBenchRead(uint64_t bytes, uint32_t reps, double* seconds_per_pass);
 
 /* Runs `iters` PCG iterations (FP64, or FP32 when fp32 = 1, b = 1) launching
  * each kernel directly; seconds[0..2] receive the mean HIP-event duration of

@@ -68,8 +68,8 @@ def test_cg_iterates_and_residual_at_full_size(hip, c2):
 def test_atlas_at_full_size(hip):
     """The 2000-chart atlas of the batched benchmark (N ~ logU[50, 50000]):
     the batch built on the device from the chart meshes is bitwise the host
-    packing, every chart reaches 1e-10 by the oracle's SpMV, and sampled
-    charts solved alone agree with their share of the batched solution."""
+    packing, every chart reaches 1e-10 by the oracle's SpMV, and the largest
+    charts match the oracle's solve."""
     A = W.atlas(2000, seed=1, nmin=50, nmax=50000)
     M = W.atlas_mesh(2000, seed=1, nmin=50, nmax=50000)
     Bd, rhs = capi.Batch.lscm_atlas(M.vertex_off, M.points, M.triangles, M.pins, M.pin_uv)
@@ -86,13 +86,30 @@ def test_atlas_at_full_size(hip):
     for c in range(A.count):
         r0, r1 = int(A.row_off[c]), int(A.row_off[c + 1])
         assert np.linalg.norm(r[r0:r1]) <= 1e-10 * np.linalg.norm(A.b[r0:r1]), c
-    for c in range(0, A.count, 250):
-        S = A.chart(c)
-        Sc = capi.System.from_crs(S)
-        xc, st = Sc.solve(S.b, eps=1e-10, max_iter=50000)
-        assert np.linalg.norm(xc - xs[c]) / np.linalg.norm(xc) <= 1e-8, c
-        Sc.close()
     Bd.close()
+    # the eight largest charts (~1e5 unknowns each) against the oracle's
+    # solve and a sparse direct solve: the batched answer is within 1e-8 of
+    # the oracle's wherever the oracle's own answer is pinned to a quarter of
+    # that (forward error <= 2.5e-9); on a chart whose conditioning leaves the
+    # oracle's 1e-10-residual answer further from the exact solution, the
+    # batched answer must be at least as close to it as the oracle's (within
+    # 2x), and the gap is printed
+    import scipy.sparse.linalg as sla
+    sizes = np.diff(A.row_off.astype(np.int64))
+    for c in np.argsort(-sizes, kind="stable")[:8]:
+        S = A.chart(int(c))
+        M = O.CRS(S.rowptr, S.colind, S.val)
+        xo, it, _ = O.pcg(M, S.b, eps=1e-10, max_iter=100000, variant=1)
+        xd = sla.spsolve(M.to_scipy().tocsc(), S.b)
+        nd = np.linalg.norm(xd)
+        d = np.linalg.norm(xs[c] - xo) / np.linalg.norm(xo)
+        fe_gpu, fe_cpu = np.linalg.norm(xs[c] - xd) / nd, np.linalg.norm(xo - xd) / nd
+        print(f"chart {c}: n={S.n}, GPU {per[c].iterations} it, oracle {it}: rel diff {d:.2e}, "
+              f"forward error GPU {fe_gpu:.2e} oracle {fe_cpu:.2e}")
+        if fe_cpu <= 2.5e-9:
+            assert d <= 1e-8, c
+        else:
+            assert fe_gpu <= 2 * fe_cpu, c
 
 
 def test_multi_rhs_at_full_size(hip):
