@@ -79,8 +79,6 @@ Util& util() {
     return *u;
 }
 
-void nl_err(const std::string& m) { log_error(m); }
-
 // -------------------------------------------------------------- dynamic
 
 void dyn_destroy(NLMatrix M);
@@ -200,17 +198,29 @@ void crs_destroy(NLMatrix Mg) {
 }
 
 // The device system behind any solvable matrix (uploads on first use).
+// Serialises the creation and rebuild of host matrices' device mirrors: two
+// threads multiplying or solving with one matrix (read-only use, which
+// OpenNL allows) must not both build its mirror, nor use one that another
+// thread is replacing.  A matrix that is modified while another thread uses
+// it is outside what OpenNL allows, as upstream.
+static std::mutex& mirror_mutex() {
+    static std::mutex* m = new std::mutex();  // process lifetime
+    return *m;
+}
+
 System* system_of(NLMatrix M) {
     if (!M) throw Error(NLHIP_ERR_INVALID, "null matrix");
     if (M->type == NL_MATRIX_CRS) {
         NLCRSMatrix* C = reinterpret_cast<NLCRSMatrix*>(M);
         if (C->m != C->n) throw Error(NLHIP_ERR_UNSUPPORTED, "non-square CRS matrix");
+        std::lock_guard<std::mutex> lock(mirror_mutex());
         if (!C->hip_mirror) C->hip_mirror = system_create(C->m, C->rowptr, C->colind, C->val, 0);
         return static_cast<System*>(C->hip_mirror);
     }
     if (M->type == NL_MATRIX_SPARSE_DYNAMIC) {
         NLSparseMatrix* D = reinterpret_cast<NLSparseMatrix*>(M);
         if (D->m != D->n) throw Error(NLHIP_ERR_UNSUPPORTED, "non-square dynamic matrix");
+        std::lock_guard<std::mutex> lock(mirror_mutex());
         Mirror* mi = static_cast<Mirror*>(D->hip_mirror);
         if (!mi) {
             mi = new Mirror();
