@@ -77,6 +77,17 @@ def run_atlas(args) -> dict:
     x64, p64, t64 = B.solve(A.b, **params)
     log(f"batched fp64: {t64.device_seconds:.3f}s, max it {t64.iterations}, "
         f"converged {sum(p.converged for p in p64)}/{A.count}")
+    # the same FP64 iterations as one persistent cooperative launch per run
+    B.set_persistent(True)
+    B.solve(A.b, **params)
+    xp, pp, tp = B.solve(A.b, **params)
+    B.set_persistent(False)
+    persistent = {"seconds": tp.device_seconds, "solves_per_s": A.count / tp.device_seconds,
+                  "bitwise_equal_to_graphs": bool(np.array_equal(xp.view(np.uint64),
+                                                                 x64.view(np.uint64)))}
+    log(f"batched fp64, persistent launches: {tp.device_seconds:.3f}s, same bits "
+        f"{persistent['bitwise_equal_to_graphs']}")
+    del xp
     xm, pm, tm = B.solve(A.b, precision="mixed", inner_eps=args.inner_eps,
                          max_iter=args.max_iter_mixed, eps=args.eps)
     xm, pm, tm = B.solve(A.b, precision="mixed", inner_eps=args.inner_eps,
@@ -164,6 +175,7 @@ def run_atlas(args) -> dict:
                                    "the host (OpenMP), then nlhipBatchCreate",
                      "batch_create_from_packed_seconds": t_packed_create,
                      "device_assembly": assembly}
+    res["batched_fp64_persistent"] = persistent
     res["speedup_mixed_batched_vs_per_chart_fp64"] = t_plain_all / tm.device_seconds
     # per chart-size band: iterations of FP32-inner refinement against FP64
     # PCG (from the batched solves: each chart's counts are its own) and the

@@ -215,6 +215,7 @@ SIGNATURES = {
     "nlhipBatchInfo": (C.c_int, [_vp, _u32p, _u32p, C.POINTER(C.c_uint64), _u32p]),
     "nlhipBatchSetLaunch": (C.c_int, [_vp, C.c_uint32]),
     "nlhipBatchSetGraphs": (C.c_int, [_vp, C.c_int]),
+    "nlhipBatchSetPersistent": (C.c_int, [_vp, C.c_int]),
     "nlhipBatchFallbacks": (C.c_int, [_vp, _u32p]),
     "nlhipBatchSetSpmvPasses": (C.c_int, [_vp, C.c_uint32]),
     "nlhipBatchSolve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveParams),
@@ -568,6 +569,9 @@ class Batch:
         n = C.c_uint32()
         check(lib().nlhipBatchFallbacks(self._h, C.byref(n)), "nlhipBatchFallbacks")
         return int(n.value)
+
+    def set_persistent(self, on: bool):
+        check(lib().nlhipBatchSetPersistent(self._h, 1 if on else 0), "nlhipBatchSetPersistent")
 
     def set_graphs(self, on: bool):
         """False: enqueue every launch directly (nlhipBatchSetGraphs)."""

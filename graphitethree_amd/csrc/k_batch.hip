@@ -14,6 +14,8 @@
 // are reduced over that system's tile partials in a fixed order, so a
 // system's result depends only on the system itself: it is bitwise the same
 // solved alone or in any batch.
+#include <hip/hip_cooperative_groups.h>
+
 #include "dev_common.h"
 #include "nlhip_kernels.h"
 
@@ -267,7 +269,7 @@ __device__ __forceinline__ void tile_share(uint32_t n, uint32_t& k0, uint32_t& k
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_dir(BatchArgs<T> a, int max_iter) {
+__device__ __forceinline__ void b_dir_body(const BatchArgs<T>& a, int max_iter) {
     uint32_t k0, k1;
     tile_share(*a.n_act_tiles, k0, k1);
     const int lane = threadIdx.x & (kWave - 1);
@@ -336,6 +338,11 @@ __global__ __launch_bounds__(kBlock) void k_b_dir(BatchArgs<T> a, int max_iter) 
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_dir(BatchArgs<T> a, int max_iter) {
+    b_dir_body<T>(a, max_iter);
+}
+
 // SpMV of the active tiles.  A wavefront streams R consecutive passes of one
 // tile (R = 1, 2 or 4, chosen per launch from the number of active tiles):
 // long runs keep a wavefront's loads in flight when tiles are plentiful,
@@ -345,9 +352,8 @@ __global__ __launch_bounds__(kBlock) void k_b_dir(BatchArgs<T> a, int max_iter) 
 // lane summing its rows in order), then the 4 pass sums in pass order, as
 // block_sum_all does when wavefront w takes pass w.
 template <typename T, int F>
-__global__ __launch_bounds__(kBlock) void k_b_spmv(BatchArgs<T> a) {
-    __shared__ T stage[kBatchStage];
-    __shared__ T pass_sum[kWavesPerBlock][4];  // [tile slot][pass]
+__device__ __forceinline__ void b_spmv_body(const BatchArgs<T>& a, T* stage,
+                                            T (*pass_sum)[4] /* [tile slot][pass] */) {
     const uint32_t n = *a.n_act_tiles;
     const uint32_t R = n >= a.spmv_r4_tiles ? 4u : (n >= a.spmv_r2_tiles ? 2u : 1u);
     const int lane = threadIdx.x & (kWave - 1);
@@ -412,9 +418,15 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(BatchArgs<T> a) {
     }
 }
 
+template <typename T, int F>
+__global__ __launch_bounds__(kBlock) void k_b_spmv(BatchArgs<T> a) {
+    __shared__ T stage[kBatchStage];
+    __shared__ T pass_sum[kWavesPerBlock][4];
+    b_spmv_body<T, F>(a, stage, pass_sum);
+}
+
 template <typename T, bool PREC>
-__global__ __launch_bounds__(kBlock) void k_b_update(BatchArgs<T> a) {
-    __shared__ T lds[kWavesPerBlock];
+__device__ __forceinline__ void b_update_body(const BatchArgs<T>& a, T* lds) {
     uint32_t k0, k1;
     tile_share(*a.n_act_tiles, k0, k1);
     const int lane = threadIdx.x & (kWave - 1);
@@ -469,6 +481,12 @@ __global__ __launch_bounds__(kBlock) void k_b_update(BatchArgs<T> a) {
     }
 }
 
+template <typename T, bool PREC>
+__global__ __launch_bounds__(kBlock) void k_b_update(BatchArgs<T> a) {
+    __shared__ T lds[kWavesPerBlock];
+    b_update_body<T, PREC>(a, lds);
+}
+
 // Active-list rebuild: count reset, then append (order is irrelevant to the
 // results: every tile and system is computed the same wherever it sits).
 __global__ void k_b_reset_counts(uint32_t* n_act_tiles, uint32_t* n_act_sys) {
@@ -479,7 +497,7 @@ __global__ void k_b_reset_counts(uint32_t* n_act_tiles, uint32_t* n_act_sys) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_compact(BatchArgs<T> a) {
+__device__ __forceinline__ void b_compact_body(const BatchArgs<T>& a) {
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < a.ntiles; t += stride) {
         if (!a.done[a.tile_sys[t]]) {
@@ -493,6 +511,42 @@ __global__ __launch_bounds__(kBlock) void k_b_compact(BatchArgs<T> a) {
             a.act_sys[k] = s;
         }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_compact(BatchArgs<T> a) {
+    b_compact_body<T>(a);
+}
+
+// The same iterations in one persistent, cooperative launch: the direction,
+// SpMV and update phases of n_iter iterations and then the active-list
+// rebuild, each phase ended by a grid-wide barrier instead of a kernel
+// boundary.  Every workgroup runs the phase bodies of the three-launch
+// kernels above over its share of the active tiles, so the results are
+// theirs, bit for bit.  All workgroups take the same path: they leave
+// together after n_iter iterations, or after a rebuild that leaves no tile
+// active (read by every workgroup after the same barrier).
+template <typename T, bool PREC, int F>
+__global__ __launch_bounds__(kBlock) void k_b_persist(BatchArgs<T> a, int max_iter,
+                                                      uint32_t n_iter) {
+    __shared__ T stage[kBatchStage];
+    __shared__ T pass_sum[kWavesPerBlock][4];
+    __shared__ T lds[kWavesPerBlock];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    for (uint32_t it = 0; it < n_iter; ++it) {
+        b_dir_body<T>(a, max_iter);
+        grid.sync();
+        b_spmv_body<T, F>(a, stage, pass_sum);
+        grid.sync();
+        b_update_body<T, PREC>(a, lds);
+        grid.sync();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *a.n_act_tiles = 0;
+        *a.n_act_sys = 0;
+    }
+    grid.sync();
+    b_compact_body<T>(a);
 }
 
 // All tiles / systems active (before the initial residual).
@@ -605,6 +659,45 @@ hipError_t batch_iteration(const BatchArgs<T>& a, bool precond, int max_iter, ui
     return hipGetLastError();
 }
 
+// The persistent kernel for a batch: its function and the largest grid whose
+// workgroups are all resident at once (what a cooperative launch needs).
+template <typename T>
+static const void* persist_fn(const BatchArgs<T>& a, bool precond) {
+    if (a.all_blocks && a.bci16)
+        return precond ? reinterpret_cast<const void*>(&k_b_persist<T, true, kFormBlock16>)
+                       : reinterpret_cast<const void*>(&k_b_persist<T, false, kFormBlock16>);
+    if (a.all_blocks)
+        return precond ? reinterpret_cast<const void*>(&k_b_persist<T, true, kFormBlock32>)
+                       : reinterpret_cast<const void*>(&k_b_persist<T, false, kFormBlock32>);
+    return precond ? reinterpret_cast<const void*>(&k_b_persist<T, true, kFormAny>)
+                   : reinterpret_cast<const void*>(&k_b_persist<T, false, kFormAny>);
+}
+
+template <typename T>
+hipError_t batch_persistent_grid(const BatchArgs<T>& a, bool precond, uint32_t* grid) {
+    int per_cu = 0, dev = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persist_fn(a, precond),
+                                                                kBlock, 0);
+    if (e != hipSuccess) return e;
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    if (per_cu <= 0 || cus <= 0) return hipErrorInvalidConfiguration;
+    *grid = uint32_t(per_cu) * uint32_t(cus);
+    return hipSuccess;
+}
+
+template <typename T>
+hipError_t batch_persistent(const BatchArgs<T>& a, bool precond, int max_iter, uint32_t n_iter,
+                            uint32_t grid, hipStream_t s) {
+    BatchArgs<T> args = a;
+    void* params[] = {&args, &max_iter, &n_iter};
+    return hipLaunchCooperativeKernel(persist_fn(a, precond), dim3(grid), dim3(kBlock), params,
+                                      0, s);
+}
+
 template <typename T>
 hipError_t batch_compact(const BatchArgs<T>& a, uint32_t grid, hipStream_t s) {
     k_b_reset_counts<<<1, 64, 0, s>>>(a.n_act_tiles, a.n_act_sys);
@@ -640,6 +733,12 @@ template hipError_t batch_iteration<double>(const BatchArgs<double>&, bool, int,
 template hipError_t batch_iteration<float>(const BatchArgs<float>&, bool, int, uint32_t,
                                            hipStream_t);
 template hipError_t batch_compact<double>(const BatchArgs<double>&, uint32_t, hipStream_t);
+template hipError_t batch_persistent_grid<double>(const BatchArgs<double>&, bool, uint32_t*);
+template hipError_t batch_persistent_grid<float>(const BatchArgs<float>&, bool, uint32_t*);
+template hipError_t batch_persistent<double>(const BatchArgs<double>&, bool, int, uint32_t,
+                                             uint32_t, hipStream_t);
+template hipError_t batch_persistent<float>(const BatchArgs<float>&, bool, int, uint32_t,
+                                            uint32_t, hipStream_t);
 template hipError_t batch_compact<float>(const BatchArgs<float>&, uint32_t, hipStream_t);
 
 }  // namespace nlhip

@@ -48,6 +48,7 @@ struct Batch {
     uint32_t grid = 1;
     uint32_t spmv_passes = 0;  // 0: by active tiles, else 1, 2 or 4
     bool graphs = true;        // false: the same launches enqueued directly
+    bool persistent = false;   // true: each run of iterations is one cooperative launch
     uint32_t last_fallbacks = 0;  // systems the last fallback solve finished in FP64
     Stream stream;
     std::mutex mu;
@@ -150,30 +151,45 @@ template <typename T>
 void batch_loop(Batch& B, BatchWork<T>& w, bool precond, int max_iter, uint32_t K,
                 uint64_t budget) {
     hipStream_t s = B.stream.get();
-    if (w.graph_precond != precond || w.graph_max_iter != max_iter ||
-        std::memcmp(&w.graph_args, &w.args, sizeof(BatchArgs<T>)) != 0)
-        w.graph.reset();
-    w.graph.ensure(K, s, [&](uint32_t k) {
-        for (uint32_t i = 0; i < k; ++i)
-            NLHIP_CHECK(batch_iteration<T>(w.args, precond, max_iter, B.grid, s));
-        NLHIP_CHECK(batch_compact<T>(w.args, B.grid, s));
-    });
-    w.graph_precond = precond;
-    w.graph_max_iter = max_iter;
-    w.graph_args = w.args;
+    // iterations per launch: 4, 4, 8, 16, ... up to K, as the graph ladder
+    auto run_len = [&](uint64_t i) {
+        uint32_t k = K < 4u ? K : 4u;
+        for (uint64_t l = 1; l < i && k < K; ++l) k = 2 * k < K ? 2 * k : K;
+        return k;
+    };
+    uint32_t pgrid = 0;
+    if (B.persistent) {
+        NLHIP_CHECK(batch_persistent_grid<T>(w.args, precond, &pgrid));
+        pgrid = std::min(pgrid, std::max(B.grid, 1u));
+    } else {
+        if (w.graph_precond != precond || w.graph_max_iter != max_iter ||
+            std::memcmp(&w.graph_args, &w.args, sizeof(BatchArgs<T>)) != 0)
+            w.graph.reset();
+        w.graph.ensure(K, s, [&](uint32_t k) {
+            for (uint32_t i = 0; i < k; ++i)
+                NLHIP_CHECK(batch_iteration<T>(w.args, precond, max_iter, B.grid, s));
+            NLHIP_CHECK(batch_compact<T>(w.args, B.grid, s));
+        });
+        w.graph_precond = precond;
+        w.graph_max_iter = max_iter;
+        w.graph_args = w.args;
+    }
     Event ev[2];
     uint32_t* host = w.host_counts.get();
-    const uint64_t limit = budget + 3ull * w.graph.cap() + 8;
+    const uint64_t limit = budget + 3ull * K + 8;
     uint64_t launched = 0, covered = 0;
     for (;;) {
-        if (B.graphs) {
+        const uint32_t its = run_len(launched);
+        if (B.persistent) {
+            NLHIP_CHECK(batch_persistent<T>(w.args, precond, max_iter, its, pgrid, s));
+        } else if (B.graphs) {
             w.graph.launch(launched, s);
         } else {
-            for (uint32_t i = 0; i < w.graph.iters(launched); ++i)
+            for (uint32_t i = 0; i < its; ++i)
                 NLHIP_CHECK(batch_iteration<T>(w.args, precond, max_iter, B.grid, s));
             NLHIP_CHECK(batch_compact<T>(w.args, B.grid, s));
         }
-        covered += w.graph.iters(launched);
+        covered += its;
         const int slot = int(launched & 1);
         NLHIP_CHECK(hipMemcpyAsync(&host[slot], w.counts.get() + 1, sizeof(uint32_t),
                                    hipMemcpyDeviceToHost, s));
@@ -845,6 +861,14 @@ NLAPI int NLAPIENTRY nlhipBatchSetLaunch(nlhip_batch b, uint32_t workgroups) {
         B->grid = workgroups;
         B->w64.graph.reset();
         B->w32.graph.reset();
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipBatchSetPersistent(nlhip_batch b, int on) {
+    return guarded("nlhipBatchSetPersistent", [&] {
+        Batch* B = bat(b);
+        std::lock_guard<std::mutex> lock(B->mu);
+        B->persistent = on != 0;
     });
 }
 

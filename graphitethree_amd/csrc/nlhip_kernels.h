@@ -232,6 +232,13 @@ hipError_t batch_iteration(const BatchArgs<T>& a, bool precond, int max_iter, ui
                            hipStream_t s);
 template <typename T>
 hipError_t batch_compact(const BatchArgs<T>& a, uint32_t grid, hipStream_t s);
+// n_iter iterations and then the list rebuild in one cooperative launch of
+// `grid` workgroups (at most batch_persistent_grid, all resident at once).
+template <typename T>
+hipError_t batch_persistent_grid(const BatchArgs<T>& a, bool precond, uint32_t* grid);
+template <typename T>
+hipError_t batch_persistent(const BatchArgs<T>& a, bool precond, int max_iter, uint32_t n_iter,
+                            uint32_t grid, hipStream_t s);
 // r = b - A x (FP64) and per-system |r|^2 -> err[], |b|^2 -> bb[].
 hipError_t batch_residual64(const BatchArgs<double>& a, uint32_t grid, hipStream_t s);
 // b32 = float(r64 * scale[s]), x32 = 0.

@@ -326,6 +326,14 @@ NLAPI int NLAPIENTRY nlhipBatchFallbacks(nlhip_batch b, uint32_t* n);
  * the same launches in the same order, so the same bits.  No counterpart in
  * the reference. */
 NLAPI int NLAPIENTRY nlhipBatchSetGraphs(nlhip_batch b, int on);
+/* on = 1: each run of iterations of the batched PCG (and of the FP32 inner
+ * solves of its refinement) is one persistent cooperative launch, the
+ * direction, SpMV and update phases separated by grid-wide barriers instead
+ * of kernel boundaries, on as many workgroups as the device holds at once
+ * (at most the batch's workgroup count).  Same kernels' arithmetic, same
+ * bits as the three-launch path; on = 0 (default): captured graphs of three
+ * launches per iteration. */
+NLAPI int NLAPIENTRY nlhipBatchSetPersistent(nlhip_batch b, int on);
 /* Workgroups of the grid that walks the active tiles. */
 NLAPI int NLAPIENTRY nlhipBatchSetLaunch(nlhip_batch b, uint32_t workgroups);
 /* Passes of a tile (64 rows or block rows each, 4 per tile) that one

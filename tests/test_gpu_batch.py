@@ -88,6 +88,27 @@ def test_batch_mixed_forms_at_any_offset(hip, small_atlas):
     assert all(p.rel_residual <= 1e-10 for p in pm)
 
 
+def test_batch_persistent_launch_gives_the_same_bits(hip, small_atlas):
+    """One persistent cooperative launch per run of iterations (grid-wide
+    barriers between the phases) gives the bits of the captured three-launch
+    iterations, in FP64 and in the FP32 inner solves of the refinement, on
+    any workgroup count."""
+    A = small_atlas
+    B = capi.Batch([A.chart(c) for c in range(A.count)], fp32=True)
+    x0, p0, _ = B.solve(A.b, eps=1e-10, max_iter=5000)
+    xm0, pm0, _ = B.solve(A.b, precision="mixed", eps=1e-10, max_iter=200000, inner_eps=1e-4)
+    B.set_persistent(True)
+    for wg in (1, 13, 4096):
+        B.set_launch(wg)
+        x1, p1, _ = B.solve(A.b, eps=1e-10, max_iter=5000)
+        assert np.array_equal(x0.view(np.uint64), x1.view(np.uint64)), wg
+        assert [p.iterations for p in p0] == [p.iterations for p in p1]
+    xm1, pm1, _ = B.solve(A.b, precision="mixed", eps=1e-10, max_iter=200000, inner_eps=1e-4)
+    assert np.array_equal(xm0.view(np.uint64), xm1.view(np.uint64))
+    assert [p.iterations for p in pm0] == [p.iterations for p in pm1]
+    B.close()
+
+
 def test_batch_launch_shape_does_not_change_bits(hip, small_atlas):
     A = small_atlas
     B = capi.Batch([A.chart(c) for c in range(A.count)])
