@@ -19,7 +19,9 @@ data from a fixed seed.
 Per result it also reports the true residual recomputed by the oracle's SpMV
 on the host, and for the 317 x 317 system of the same construction: the GPU
 solve, the oracle's converged solve on the host cores (timed), their relative
-difference and the forward error of both against a sparse direct solve.
+difference and the forward error of both against a sparse direct solve; and
+the batched configuration: 2000 LSCM charts solved to 1e-10 as one batch (FP64,
+and FP32-inner refinement) against one GPU solve per chart.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -180,6 +182,51 @@ def assembly_measurement(args, Sw, capi, W):
     return res
 
 
+def atlas_measurement(args, capi, W):
+    """The batched configuration beside the headline: 2000 disk-like LSCM
+    charts (N ~ logU[50, 50000] vertices, seed 1) solved to 1e-10 as one
+    batch in FP64 and with FP32 inner PCG inside FP64 refinement, against
+    the plain FP64 GPU PCG one chart at a time (every 10th chart timed,
+    scaled by 10)."""
+    t0 = time.perf_counter()
+    A = W.atlas(2000, seed=1, nmin=50, nmax=50000)
+    t_build = time.perf_counter() - t0
+    B = capi.Batch(packed=(A.row_off, A.rowptr, A.colind, A.val), fp32=True)
+    wg, passes, _ = B.autotune(A.b, iters=50)
+    B.solve(A.b, eps=1e-10, max_iter=50000)
+    x64, p64, t64 = B.solve(A.b, eps=1e-10, max_iter=50000)
+    xm, pm, tm = B.solve(A.b, precision="mixed", eps=1e-10, max_iter=200000, inner_eps=1e-4)
+    xs = B.split(x64)
+    t_plain, worst, stride = 0.0, 0.0, 10
+    for c in range(0, A.count, stride):
+        S = A.chart(c)
+        Sc = capi.System.from_crs(S)
+        Sc.solve(S.b, eps=1e-10, max_iter=50000)
+        xc, st = Sc.solve(S.b, eps=1e-10, max_iter=50000)
+        t_plain += st.device_seconds
+        worst = max(worst, float(np.linalg.norm(xc - xs[c]) / np.linalg.norm(xc)))
+        Sc.close()
+    t_plain *= stride
+    B.close()
+    res = {"charts": A.count, "n": A.n, "nnz": A.nnz, "host_build_s": t_build,
+           "workgroups": wg, "spmv_passes": passes,
+           "batched_fp64": {"seconds": t64.device_seconds, "solves_per_s": A.count / t64.device_seconds,
+                            "converged": int(sum(p.converged for p in p64)),
+                            "max_iterations": int(t64.iterations)},
+           "batched_mixed": {"seconds": tm.device_seconds, "solves_per_s": A.count / tm.device_seconds,
+                             "converged": int(sum(p.converged for p in pm)),
+                             "max_inner_iterations": int(tm.iterations)},
+           "per_chart_fp64": {"seconds": t_plain, "solves_per_s": A.count / t_plain,
+                              "sample": f"every {stride}th chart timed, scaled by {stride}",
+                              "max_rel_diff_vs_batched": worst},
+           "speedup_batched_fp64": t_plain / t64.device_seconds,
+           "speedup_batched_mixed": t_plain / tm.device_seconds}
+    log(f"atlas: batched fp64 {t64.device_seconds:.2f}s, batched mixed {tm.device_seconds:.2f}s, "
+        f"per chart fp64 {t_plain:.1f}s -> {res['speedup_batched_fp64']:.2f}x / "
+        f"{res['speedup_batched_mixed']:.2f}x")
+    return res
+
+
 def cpu_leg_main(args):
     """The host-core baseline, run in a child process of its own so that the
     OpenMP runtime starts with the thread binding under test (it reads
@@ -299,6 +346,8 @@ def main():
                     help="SpMV workgroups (with --rows-per-wave: a fixed launch shape)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape instead of the autotuner's")
+    ap.add_argument("--no-atlas", action="store_true",
+                    help="skip the batched 2000-chart atlas measurement")
     ap.add_argument("--no-assembly", action="store_true",
                     help="skip the least-squares assembly measurement")
     ap.add_argument("--assembly-sample-rows", type=int, default=2_000_000,
@@ -476,6 +525,11 @@ def main():
     # sample of the rows, and the multithreaded host gather that built Sw
     if not args.no_assembly:
         result_extra["assembly"] = assembly_measurement(args, Sw, capi, W)
+    del Sw
+
+    # ---- the batched atlas (2000 LSCM charts) beside it
+    if not args.no_atlas:
+        result_extra["atlas"] = atlas_measurement(args, capi, W)
 
     out = {
         "metric": "CG solves/s + SpMV HBM GB/s at 1e-10 rel. residual; p50/p99 solve latency",
