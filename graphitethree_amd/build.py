@@ -2,8 +2,6 @@
 
 * libnlworkloads.so  host C++ (g++, OpenMP): synthetic LSCM / cotan systems.
 * libnlhip.so        the product: OpenNL C API + HIP kernels for gfx950.
-* lib/tsan/libnlhip.so: the same library with ThreadSanitizer host objects
-  (for the GPU test of the server's threads), built with the others.
 * --sanitize: a copy of libnlhip.so whose host C++ objects (the C ABI, the
   dynamic matrix, the context, the solve drivers, the server) are built with
   AddressSanitizer and UndefinedBehaviorSanitizer, linked with the normal
@@ -126,47 +124,9 @@ def build_sanitized(jobs: int = 8):
     return out
 
 
-TSAN_DIR = os.path.join(LIB, "tsan")
-
-
-def build_tsan(jobs: int = 8):
-    """libnlhip.so with ThreadSanitizer host objects (g++ -fsanitize=thread;
-    the device objects of the normal build linked in unchanged) in
-    graphitethree_amd/lib/tsan/, beside a copy of libnlworkloads.so: the GPU
-    test of the server's worker threads, the capture lock and concurrent
-    solves loads it with the TSan runtime preloaded."""
-    for s in NLHIP_HIP:
-        if not os.path.exists(os.path.join(OBJ, s + ".o")):
-            build_nlhip(jobs)
-            break
-    obj = os.path.join(ROOT, "_build", "tsan_obj")
-    os.makedirs(obj, exist_ok=True)
-    os.makedirs(TSAN_DIR, exist_ok=True)
-    flags = [f for f in HOST_FLAGS if f != "-O3"] + ["-O1", "-g", "-fsanitize=thread"]
-
-    def compile_one(s):
-        o = os.path.join(obj, s + ".o")
-        _run(["g++", *flags, "-fopenmp", "-c", os.path.join(CSRC, s), "-o", o])
-        return o
-
-    with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(compile_one, NLHIP_CPP))
-    out = os.path.join(TSAN_DIR, "libnlhip.so")
-    _run(["g++", "-shared", "-o", out + ".tmp", *objs,
-          *[os.path.join(OBJ, s + ".o") for s in NLHIP_HIP], "-fsanitize=thread", "-fopenmp",
-          "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
-    os.replace(out + ".tmp", out)
-    wl = os.path.join(TSAN_DIR, "libnlworkloads.so")
-    with open(os.path.join(LIB, "libnlworkloads.so"), "rb") as f, open(wl + ".tmp", "wb") as g:
-        g.write(f.read())
-    os.replace(wl + ".tmp", wl)
-    return out
-
-
 def build_all():
     build_workloads()
     build_nlhip()
-    build_tsan()
 
 
 if __name__ == "__main__":
