@@ -196,6 +196,32 @@ def test_batch_mixed_with_fp64_fallback(hip, small_atlas):
     B.close()
 
 
+def test_batch_refinement_honours_max_iter(hip, small_atlas):
+    """max_iter bounds every system's iterations in all (FP32 inner solves
+    and, with the fallback, the FP64 phase from the budget left), as in the
+    single-system refinement; a system stopped by it is reported as not
+    converged."""
+    A = small_atlas
+    B = capi.Batch([A.chart(c) for c in range(A.count)], fp32=True)
+    for prec, kw in (("mixed", {}), ("mixed-fallback", {"inner_max_iter": 20})):
+        for cap in (60, 200):
+            x, per, tot = B.solve(A.b, precision=prec, eps=1e-10, max_iter=cap, inner_eps=1e-4,
+                                  **kw)
+            assert all(p.iterations <= cap for p in per), (prec, cap)
+            for c, p in enumerate(per):
+                S = A.chart(c)
+                r = O.spmv(O.CRS(S.rowptr, S.colind, S.val), B.split(x)[c]) - S.b
+                met = np.linalg.norm(r) <= 1e-10 * np.linalg.norm(S.b)
+                assert bool(p.converged) == bool(met), (prec, cap, c)
+    Sw = W.lscm_grid(30, seed=2)
+    S = capi.System.from_crs(Sw, fp32=True)
+    for prec, kw in (("mixed", {}), ("mixed-fallback", {"inner_max_iter": 20})):
+        _, st = S.solve(Sw.b, precision=prec, eps=1e-10, max_iter=60, inner_eps=1e-4, **kw)
+        assert st.iterations <= 60 and st.converged == 0
+    B.close()
+    S.close()
+
+
 def test_batch_rejects_bad_input(hip):
     S1 = W.lscm_grid(6, seed=1)
     S2 = W.lscm_grid(7, seed=2)

@@ -426,3 +426,49 @@ def test_autotuned_system_matches_oracle(hip):
     xr, itr, _ = O.pcg(O.CRS(Sw.rowptr, Sw.colind, Sw.val), Sw.b, eps=1e-10, max_iter=20000)
     assert same_iters(st.iterations - st.extra_iterations, itr) and rel(x1, xr) <= REL_DIFF
     S.close()
+
+
+def test_concurrent_solves_from_two_threads(hip):
+    """Two threads solving on two systems at once (their graph captures take
+    the process-wide capture lock) and two threads multiplying with one
+    OpenNL CRS matrix (its device mirror is built once, under a lock): every
+    result is bitwise the one computed from a single thread."""
+    import threading
+    L = hip
+    A = W.lscm_grid(40, seed=1)
+    Bw = W.lscm_grid(50, seed=2)
+    SA, SB = capi.System.from_crs(A), capi.System.from_crs(Bw)
+    xa0, _ = SA.solve(A.b)
+    xb0, _ = SB.solve(Bw.b)
+    SA.close()
+    SB.close()
+    SA, SB = capi.System.from_crs(A), capi.System.from_crs(Bw)   # fresh graphs
+    res = {}
+
+    def solve(name, S, b):
+        for k in range(3):
+            res[(name, k)] = S.solve(b)[0]
+    ts = [threading.Thread(target=solve, args=("a", SA, A.b)),
+          threading.Thread(target=solve, args=("b", SB, Bw.b))]
+    [t.start() for t in ts]
+    [t.join() for t in ts]
+    for k in range(3):
+        assert np.array_equal(res[("a", k)], xa0) and np.array_equal(res[("b", k)], xb0)
+    SA.close()
+    SB.close()
+    M = L.nlCRSMatrixNewFromArrays(A.n, A.n, capi.ptr(A.rowptr), capi.ptr(A.colind),
+                                   capi.ptr(A.val))
+    x = np.random.default_rng(3).standard_normal(A.n)
+    ref = O.spmv(O.CRS(A.rowptr, A.colind, A.val), x)
+    ys = [np.empty(A.n), np.empty(A.n)]
+    ok = []
+
+    def mult(k):
+        for _ in range(5):
+            L.nlMultMatrixVector(M, capi.ptr(x), capi.ptr(ys[k]))
+            ok.append(np.array_equal(ys[k], ref))
+    ts = [threading.Thread(target=mult, args=(k,)) for k in range(2)]
+    [t.start() for t in ts]
+    [t.join() for t in ts]
+    L.nlDeleteMatrix(M)
+    assert len(ok) == 10 and all(ok)
