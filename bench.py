@@ -336,6 +336,10 @@ def main():
     ap.add_argument("--time-budget", type=float, default=450.0,
                     help="seconds after which the timed solve is cut short and reported "
                          "as not converged")
+    ap.add_argument("--wall-budget", type=float, default=540.0,
+                    help="seconds of the whole run within which the optional legs (CPU "
+                         "baseline, 317 x 317, assembly, atlas) must fit; one that would "
+                         "not is skipped")
     ap.add_argument("--cpu-iters", type=int, default=200,
                     help="oracle PCG iterations timed for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-leg", choices=["c2", "c1"], default=None,
@@ -488,7 +492,20 @@ def main():
     # child processes with the threads bound close and unbound; the faster
     # binding is the baseline
     cpu, bind = None, None
-    if args.cpu_iters > 0:
+    skipped = []
+
+    def fits(name, seconds):
+        """Whether an optional leg of about `seconds` still fits the run's
+        wall-clock allowance (--wall-budget); a leg that does not is skipped
+        and named in the JSON."""
+        if time.time() - t_begin + seconds <= args.wall_budget:
+            return True
+        skipped.append(name)
+        log(f"skipping {name}: {time.time() - t_begin:.0f}s spent, ~{seconds}s more would "
+            f"pass the {args.wall_budget:.0f}s allowance")
+        return False
+
+    if args.cpu_iters > 0 and fits("cpu_baseline", 40):
         legs = {}
         for b_ in ("close", None):
             legs[b_] = run_cpu_leg(args, "c2", b_)
@@ -516,19 +533,19 @@ def main():
     del A
 
     # ---- C1 (317 x 317): whole solves on the GPU and on the host cores
-    if not args.no_c1:
+    if not args.no_c1 and fits("lscm317", 45):
         result_extra["lscm317"] = c1_measurement(args, capi, W, O, bind)
 
     # ---- least-squares assembly of the same system from its LSCM rows:
     # device (nlhipLeastSquaresCreate, host rows in, resident CRS out) against
     # the oracle's row-by-row context (OpenNL's algorithm, one core) on a
     # sample of the rows, and the multithreaded host gather that built Sw
-    if not args.no_assembly:
+    if not args.no_assembly and fits("assembly", 15):
         result_extra["assembly"] = assembly_measurement(args, Sw, capi, W)
     del Sw
 
     # ---- the batched atlas (2000 LSCM charts) beside it
-    if not args.no_atlas:
+    if not args.no_atlas and fits("atlas", 70):
         result_extra["atlas"] = atlas_measurement(args, capi, W)
 
     out = {
@@ -582,6 +599,7 @@ def main():
         "cpu_baseline": cpu,
         "device": dev_name,
         "bench_seconds": time.time() - t_begin,
+        "skipped_for_time": skipped,
         **result_extra,
     }
     print(json.dumps(out), flush=True)
