@@ -16,8 +16,9 @@ constexpr uint32_t spmm_pass_rows() {
     return uint32_t(kBlock / G) > kSpmmPass ? uint32_t(kBlock / G) : kSpmmPass;
 }
 
-// Rows of logical block L, in passes of max(64, kBlock / G) consecutive rows:
-// the block copies the pass's (colind, val) range into LDS with coalesced
+// Rows of logical block L (the passes of max(64, kBlock / G) consecutive rows
+// of its XCD's range that fall to it in turn, see below): for each pass the
+// block copies the pass's (colind, val) range into LDS with coalesced
 // loads, then each G-lane group sums its rows of the pass from LDS, lane j
 // gathering X[col * k + j].  Row sums are left to right with one rounding
 // per product and per add.  For each row, lane j calls pre(row, j) (loads
@@ -44,9 +45,17 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
     const uint32_t g = threadIdx.x / G, j = threadIdx.x % G;
     // columns that have stopped gather nothing (their sums are not used)
     const bool active_col = j < k && !(done && done[j]);
-    const uint64_t rb0 = uint64_t(L) * rows_per_block;
-    const uint32_t r0 = rb0 < m ? uint32_t(rb0) : m;
-    const uint32_t r1 = (uint64_t(r0) + rows_per_block < m) ? r0 + rows_per_block : m;
+    // the blocks that run on one XCD take the passes of their joint row range
+    // in turn, so at any time they work on neighbouring rows and the X rows
+    // those gather (a stencil's rows i - w .. i + w) are shared in that
+    // XCD's L2 instead of fetched once per block
+    uint32_t lo, hi;
+    xcd_block_range(L, gridDim.x, lo, hi);
+    const uint64_t xb0 = uint64_t(lo) * rows_per_block, xb1 = uint64_t(hi) * rows_per_block;
+    const uint32_t r0 = xb0 < m ? uint32_t(xb0) : m;
+    const uint32_t r1 = xb1 < m ? uint32_t(xb1) : m;
+    const uint32_t first = uint32_t(uint64_t(L - lo) * pass);
+    const uint32_t step = (hi - lo) * pass;
     constexpr uint32_t R = pass / groups;  // rows per group in one pass
     // U gathers of entries [p, hi) (LDS offsets), added in order to acc
     auto gather = [&](uint32_t p, uint32_t hi, T& acc) {
@@ -61,7 +70,8 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
         for (int u = 0; u < kSpmmU; ++u)
             if (p + u < hi) acc = add_rn(acc, mul_rn(av[u], xv[u]));
     };
-    for (uint32_t a = r0; a < r1; a += pass) {
+    for (uint64_t a64 = uint64_t(r0) + first; a64 < r1; a64 += step) {
+        const uint32_t a = uint32_t(a64);
         const uint32_t pe = (a + pass < r1) ? a + pass : r1;
         const uint32_t J0 = rp[a], J1 = rp[pe];
         if constexpr (ONE) {
