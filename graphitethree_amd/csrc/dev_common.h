@@ -390,7 +390,7 @@ struct Vec2<float> {
     typedef float type __attribute__((ext_vector_type(2)));
 };
 
-template <int EB, bool D16 = true>
+template <int EB, bool D16 = true, int RUN = 0>
 struct BlockStreamEngine {
     static_assert(64 * EB <= int(kSegEntries), "a chunk must span at most two segments");
 
@@ -552,7 +552,11 @@ struct BlockStreamEngine {
     }
 
     // Block L owns block-row tiles [L*sp, L*sp + sp) (tiles of 64 block
-    // rows); each wavefront takes a contiguous quarter of them.
+    // rows).  RUN = 0: each of its wavefronts takes a contiguous quarter of
+    // them.  RUN > 0: the tiles of all blocks on L's XCD are dealt out in
+    // runs of RUN tiles, round-robin over that XCD's wavefronts, so that at
+    // any time the wavefronts of an XCD read and write one compact stretch
+    // of the matrix and of y instead of one stretch each.
     template <typename T, typename Epi>
     __device__ __forceinline__ static void run(uint32_t m, uint32_t sp, uint32_t L,
                                                const uint32_t* __restrict__ rp,
@@ -561,6 +565,21 @@ struct BlockStreamEngine {
         const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
         const uint32_t mb = m / 2;
         const uint64_t ntiles = (uint64_t(mb) + 63) / 64;
+        if constexpr (RUN > 0) {
+            uint32_t lo, hi;
+            xcd_block_range(L, gridDim.x, lo, hi);
+            const uint64_t tb = uint64_t(lo) * sp < ntiles ? uint64_t(lo) * sp : ntiles;
+            const uint64_t te = uint64_t(hi) * sp < ntiles ? uint64_t(hi) * sp : ntiles;
+            const uint64_t nw = uint64_t(hi - lo) * kWavesPerBlock;
+            const uint64_t v = uint64_t(L - lo) * kWavesPerBlock + uint64_t(w);
+            for (uint64_t t0 = tb + v * RUN; t0 < te; t0 += nw * RUN) {
+                const uint64_t t1 = t0 + RUN < te ? t0 + RUN : te;
+                const uint32_t R1 = uint32_t(t1 * 64 < mb ? t1 * 64 : mb);
+                run_rows(uint32_t(t0 * 64), R1, rp, cols, val, x, stage + w * (64 * EB * 4),
+                         lane, epi);
+            }
+            return;
+        }
         uint64_t tb = uint64_t(L) * sp;
         if (tb > ntiles) tb = ntiles;
         const uint64_t te = tb + sp < ntiles ? tb + sp : ntiles;
@@ -637,6 +656,10 @@ template <>
 struct EngineOf<76> {  // 2x2 blocks, 8 blocks per lane per chunk
     using type = BlockStreamEngine<8>;
 };
+template <>
+struct EngineOf<77> {  // 2x2 blocks, XCD-interleaved runs of 4 tiles
+    using type = BlockStreamEngine<kBlockEB, true, 4>;
+};
 
 template <int RPW>
 struct StageSize {
@@ -645,7 +668,7 @@ struct StageSize {
                                  : RPW == 65 ? kWavesPerBlock * 64 * 4
                                  : RPW == 66 ? kWavesPerBlock * 64 * 8
                                  : (RPW == 67 || RPW == 72) ? kWavesPerBlock * 64 * 16
-                                 : RPW == 73 ? kWavesPerBlock * 64 * kBlockEB * 4
+                                 : (RPW == 73 || RPW == 77) ? kWavesPerBlock * 64 * kBlockEB * 4
                                  : RPW == 74 ? kWavesPerBlock * 64 * 2 * 4
                                  : RPW == 75 ? kWavesPerBlock * 64 * 1 * 4
                                  : RPW == 76 ? kWavesPerBlock * 64 * 8 * 4

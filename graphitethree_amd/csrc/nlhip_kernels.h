@@ -10,7 +10,7 @@ namespace nlhip {
 
 // Rows-per-wavefront dispatch over the compiled SpMV engines (codes of
 // dev_common.h's EngineOf): 2, 4, 8, 16 -> 32, 16, 8, 4 lanes per row;
-// 65..72 -> stream engines (one row per lane); 73..76 -> 2x2 block stream
+// 65..72 -> stream engines (one row per lane); 73..77 -> 2x2 block stream
 // engines (their callers pass the block arrays).  Launchers pass the code
 // through engine_rpw first, which resolves 64.
 #define NLHIP_RPW_SWITCH(rpw, CALL)                       \
@@ -31,15 +31,16 @@ namespace nlhip {
         case 74: { constexpr int RPW = 74; CALL; } break; \
         case 75: { constexpr int RPW = 75; CALL; } break; \
         case 76: { constexpr int RPW = 76; CALL; } break; \
+        case 77: { constexpr int RPW = 77; CALL; } break; \
         default: return hipErrorInvalidValue;             \
     }
 
 // Launch shapes a caller may set: 2, 4, 8, 16, and the stream engines 64
-// (the default stream engine) and 65..76 (variants kept for sweeps; they
-// compute the same sums as 64; 73..76 run as 64 on matrices without 2x2
+// (the default stream engine) and 65..77 (variants kept for sweeps; they
+// compute the same sums as 64; 73..77 run as 64 on matrices without 2x2
 // block form).
 inline bool valid_rpw(int rpw) {
-    return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 76);
+    return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 77);
 }
 
 // Smallest system (rows) on which code 64 runs the 2x2 block engine: below

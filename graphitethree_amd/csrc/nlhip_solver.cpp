@@ -914,7 +914,8 @@ void profile_cg(System& S, bool fp32, uint32_t iters, double t[3]) {
 // Autotuner: times `iters` CG iterations (the three launches) for each
 // candidate launch shape and keeps the fastest.  Candidates: the streamed
 // SpMV engines, which give the same row sums (the 2x2 block engine with 1,
-// 2, 4 or 8 blocks per lane per chunk when the matrix has the block form;
+// 2, 4 or 8 blocks per lane per chunk, and with 4 in XCD-interleaved runs,
+// when the matrix has the block form;
 // the scalar engine with 16-bit or 32-bit columns, contiguous or
 // XCD-interleaved), times workgroup counts of 1, 2, 4 and 8 per CU (the
 // dot products are summed per workgroup, so the count fixes their order:
@@ -926,7 +927,7 @@ void profile_cg(System& S, bool fp32, uint32_t iters, double t[3]) {
 // choice (and with it the order of the sums) from run to run.
 TuneResult autotune(System& S, bool fp32, uint32_t iters) {
     std::vector<int> codes;
-    if (S.blocks()) codes = {73, 74, 75, 76};
+    if (S.blocks()) codes = {73, 74, 75, 76, 77};
     if (S.ci16.get()) {
         codes.push_back(68);
         codes.push_back(70);
@@ -1032,7 +1033,7 @@ NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,
     return guarded("nlhipSystemSetLaunch", [&] {
         System* S = sys(s);
         if (!valid_rpw(rows_per_wave))
-            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64..76");
+            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64..77");
         if (workgroups == 0 || workgroups > 65536)
             throw Error(NLHIP_ERR_INVALID, "workgroups must be in [1, 65536]");
         std::lock_guard<std::mutex> lock(S->mu);

@@ -193,7 +193,7 @@ NLAPI int NLAPIENTRY nlhipLSCMCreate(uint32_t nb_vertices, const double* points,
                                      double* rhs, uint32_t* nb_free);
 /* Launch shape of the SpMV engine: rows per wavefront (2, 4, 8, 16: that many
  * rows share a 64-lane wavefront; 64: one row per lane, nonzeros streamed
- * through LDS, the default) and workgroups per launch.  65..76 select
+ * through LDS, the default) and workgroups per launch.  65..77 select
  * variants of the streamed engine for launch-shape sweeps (chunk width,
  * load kind, column format, tile order, 2x2 blocks); they give the same sums
  * as 64. */

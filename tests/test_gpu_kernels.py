@@ -23,7 +23,7 @@ def test_spmv_exact_all_launch_shapes(hip):
     for name in z["names"]:
         rp, ci, va = z[f"{name}_rowptr"], z[f"{name}_colind"], z[f"{name}_val"]
         S = capi.System(rp, ci, va)
-        for rpw in (2, 4, 8, 16, 64, 65, 66, 67, 68, 69, 70, 71, 72, 73, 74, 75, 76):
+        for rpw in (2, 4, 8, 16, 64, 65, 66, 67, 68, 69, 70, 71, 72, 73, 74, 75, 76, 77):
             for wg in (1, 3, 64, 2048):
                 S.set_launch(rpw, wg)
                 y = S.spmv(z[f"{name}_x"])
@@ -50,7 +50,7 @@ def test_stream_engine_is_bitwise_the_oracle_spmv(hip):
         x = np.random.default_rng(4).standard_normal(Sw.n)
         S = capi.System.from_crs(Sw)
         ref = O.spmv(A, x)
-        for rpw in (64, 65, 66, 67, 68, 69, 70, 71, 72, 73, 74, 75, 76):
+        for rpw in (64, 65, 66, 67, 68, 69, 70, 71, 72, 73, 74, 75, 76, 77):
             for wg in (1, 5, 9, 300):
                 S.set_launch(rpw, wg)
                 assert np.array_equal(S.spmv(x), ref), (rpw, wg)
@@ -110,7 +110,7 @@ def test_block_form_matches_crs(hip):
     B = capi.System.from_crs(Sw)
     C = capi.System.from_crs(Sw, blocks=False)
     assert B.info()[2] > C.info()[2]          # block arrays were built
-    for rpw in (64, 73, 74, 75, 76, 68, 71):  # the stream engines sum rows left to right
+    for rpw in (64, 73, 74, 75, 76, 77, 68, 71):  # the stream engines sum rows left to right
         for wg in (1, 3, 77, 2048):
             B.set_launch(rpw, wg)
             C.set_launch(rpw, wg)
@@ -157,7 +157,7 @@ def test_block_form_solves_like_scalar_form(hip):
 
 def test_set_launch_rejects_bad_shapes(hip):
     S = capi.System([0, 1], [0], [1.0])
-    for rpw, wg in [(3, 10), (32, 10), (64, 0)]:
+    for rpw, wg in [(3, 10), (32, 10), (78, 10), (64, 0)]:
         with pytest.raises(capi.NLHIPError):
             S.set_launch(rpw, wg)
 
