@@ -414,6 +414,16 @@ def main():
     t_read = capi.bench_read(4 << 30, reps=20)
     read_gbs = (4 << 30) / t_read / 1e9
     log(f"read 4 GiB: {t_read * 1e3:.3f} ms  {read_gbs:.0f} GB/s")
+    # the 2x2-block SpMV's traffic without its arithmetic: this system's
+    # block count and blocks per row pair, a 16-byte read and write per pair
+    mix_blocks = nnz // 4
+    mix_bpr = max(1, round(mix_blocks / max(1, n // 2)))
+    t_mix = capi.bench_stream_mix(mix_blocks, mix_bpr, writes=True, reps=20)
+    t_mix_ro = capi.bench_stream_mix(mix_blocks, mix_bpr, writes=False, reps=20)
+    mix_bytes = 34.0 * mix_blocks + 32.0 * (mix_blocks // mix_bpr)
+    log(f"SpMV-shaped stream ({mix_blocks} blocks, {mix_bpr} per row pair): "
+        f"{t_mix * 1e3:.3f} ms  {mix_bytes / t_mix / 1e9:.0f} GB/s; without its writes "
+        f"{t_mix_ro * 1e3:.3f} ms")
 
     # ---- standalone SpMV bandwidth (FP64, and FP32 values)
     t_spmv = S.bench_spmv(fp32=False, reps=50)
@@ -593,7 +603,16 @@ def main():
                      "traffic": measured_traffic(f"lscm_grid_{args.k}", lpr, wg),
                      "bytes_per_launch": bytes_a, "launch_ms": t_a * 1e3,
                      "copy_gbs": copy_gbs, "read_gbs": read_gbs,
-                     "frac_of_read": achieved / read_gbs},
+                     "frac_of_read": achieved / read_gbs,
+                     "stream_mix": {"what": "the block SpMV's traffic without its arithmetic "
+                                            "(nlhipBenchStreamMix): 34 bytes per 2x2 block "
+                                            "streamed, an L2 gather per block, a 16-byte read "
+                                            "and write per row pair",
+                                    "blocks": mix_blocks, "blocks_per_row_pair": mix_bpr,
+                                    "bytes": mix_bytes, "ms": t_mix * 1e3,
+                                    "gbs": mix_bytes / t_mix / 1e9,
+                                    "ms_without_writes": t_mix_ro * 1e3,
+                                    "spmv_time_frac": t_mix / t_a}},
         "kernels_ms": {"k_cg_dir": t_dir * 1e3, "k_cg_spmv": t_a * 1e3,
                        "k_cg_update": t_upd * 1e3},
         "cpu_baseline": cpu,

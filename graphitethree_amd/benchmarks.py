@@ -22,6 +22,13 @@ eigen: the k smallest eigenpairs of a grid mesh's cotangent Laplacian with
   lumped mass (manifold harmonics), nlhipEigenSolve against scipy's
   shift-invert ARPACK with a SuperLU factorization on the host.
 
+spmv-patterns: the CG SpMV step on the 2236 x 2236 LSCM system against two
+  synthetic 2x2-block systems of the same size and row length whose x
+  gathers are (a) a band of 7 block columns around the diagonal (every
+  gather a cache hit) and (b) the grid stencil of the LSCM system (rows
+  i +- 1, i +- w) with random values: how much of the SpMV step's time the
+  gather pattern costs.
+
 serving: a stream of requests with Poisson arrivals (rate --rate per second):
   60% LSCM charts with N ~ logU[1e3, 1e6] vertices, 40% smoothing solves
   (I + 1e-3 L) x = b with 3 right-hand sides (x, y, z) on grids of similar
@@ -216,11 +223,71 @@ def run_atlas_trace(args) -> dict:
     B.solve(A.b, eps=args.eps, max_iter=args.max_iter)
     x, per, tot = B.solve(A.b, eps=args.eps, max_iter=args.max_iter)
     log(f"batched fp64, direct launches: {tot.device_seconds:.3f}s, max it {tot.iterations}")
-    return {"workload": f"atlas_{A.count}_charts_N{int(args.nmin)}-{int(args.nmax)}",
-            "device": dev[0], "n": A.n, "nnz": A.nnz, "launches": "direct",
-            "seconds": tot.device_seconds, "max_iterations": int(tot.iterations),
-            "converged": int(sum(p.converged for p in per)),
-            "spmv_bytes_all_active": 8.5 * A.nnz + 16.0 * A.n}
+    res = {"workload": f"atlas_{A.count}_charts_N{int(args.nmin)}-{int(args.nmax)}",
+           "device": dev[0], "n": A.n, "nnz": A.nnz, "launches": "direct",
+           "seconds": tot.device_seconds, "max_iterations": int(tot.iterations),
+           "converged": int(sum(p.converged for p in per)),
+           "spmv_bytes_all_active": 8.5 * A.nnz + 16.0 * A.n}
+    del B, x
+    if not args.no_one_system:
+        # the whole block-diagonal matrix as ONE system: the single-system CG
+        # kernels on the same nonzeros (every row active), for comparison
+        # with the batched all-active launches
+        S = capi.System(A.rowptr, A.colind, A.val)
+        tuned = S.autotune(iters=20)
+        td, ts, tu = capi.profile_cg(S, iters=50)
+        log(f"one system: engine {tuned[0]}, {tuned[1]} workgroups; CG SpMV step "
+            f"{ts * 1e6:.1f} us, direction {td * 1e6:.1f} us, update {tu * 1e6:.1f} us")
+        res["one_system"] = {"engine": tuned[0], "workgroups": tuned[1],
+                             "spmv_us": ts * 1e6, "dir_us": td * 1e6, "update_us": tu * 1e6}
+        S.close()
+    return res
+
+
+def block_band_system(nb: int, offsets, seed: int = 1):
+    """CRS of a 2 nb x 2 nb matrix in 2x2 block form: block row I holds the
+    blocks (I, I + o) for o in `offsets` that fall inside the matrix, values
+    uniform in [-0.1, 0), diagonal entries 10."""
+    offs = np.asarray(sorted(offsets))
+    J = np.arange(nb)[:, None] + offs[None, :]
+    ok = (J >= 0) & (J < nb)
+    rowcnt = np.repeat(2 * ok.sum(1), 2)
+    rp = np.zeros(2 * nb + 1, dtype=np.int64)
+    np.cumsum(rowcnt, out=rp[1:])
+    cols = np.stack([2 * J, 2 * J + 1], axis=2).reshape(nb, -1)
+    valid = np.repeat(ok, 2, axis=1)
+    ci = np.repeat(cols, 2, axis=0)[np.repeat(valid, 2, axis=0)]
+    val = -np.random.default_rng(seed).random(ci.size) * 0.1
+    val[ci == np.repeat(np.arange(2 * nb), rowcnt)] = 10.0
+    return rp.astype(np.uint32), ci.astype(np.uint32), val
+
+
+def run_spmv_patterns(args) -> dict:
+    dev = init_device()
+    res = {"device": dev[0], "patterns": []}
+
+    def measure(name, rp, ci, val):
+        n, nnz = len(rp) - 1, len(ci)
+        S = capi.System(rp, ci, val)
+        tuned = S.autotune(iters=20)
+        td, ts, tu = capi.profile_cg(S, iters=50)
+        byts = 8.5 * nnz + 26.0 * n
+        log(f"{name}: n={n} nnz={nnz}, engine {tuned[0]} / {tuned[1]} workgroups, CG SpMV "
+            f"step {ts * 1e6:.1f} us ({byts / ts / 1e9:.0f} GB/s on 8.5 nnz + 26 n bytes)")
+        res["patterns"].append({"pattern": name, "n": n, "nnz": nnz, "engine": tuned[0],
+                                "workgroups": tuned[1], "spmv_us": ts * 1e6,
+                                "block_form_gbs": byts / ts / 1e9})
+        S.close()
+
+    G = W.lscm_grid(args.k, seed=args.seed)
+    w = args.k  # block columns between vertically neighbouring vertices
+    nb = G.n // 2
+    measure(f"lscm_grid_{args.k}", G.rowptr, G.colind, G.val)
+    del G
+    measure("band of 7 block columns", *block_band_system(nb, range(-3, 4), args.seed))
+    measure(f"grid stencil (I, I +- 1, I +- {w}, I +- {w + 1}), random values",
+            *block_band_system(nb, (-w - 1, -w, -1, 0, 1, w, w + 1), args.seed))
+    return res
 
 
 def atlas_device_assembly(args, A) -> dict:
@@ -554,6 +621,8 @@ def main(argv=None):
     at.add_argument("--nmax", type=float, default=50000)
     at.add_argument("--eps", type=float, default=1e-10)
     at.add_argument("--max-iter", type=int, default=50000)
+    at.add_argument("--no-one-system", action="store_true",
+                    help="skip the single-system CG kernels on the same matrix")
     a.add_argument("--tune-iters", type=int, default=50,
                    help="iterations per candidate of the batch autotuner (0: no tuning)")
     mr = sub.add_parser("multirhs")
@@ -578,6 +647,9 @@ def main(argv=None):
     so.add_argument("--eps", type=float, default=1e-10)
     so.add_argument("--max-iter", type=int, default=5000)
     so.add_argument("--no-cpu", action="store_true")
+    pa = sub.add_parser("spmv-patterns")
+    pa.add_argument("--k", type=int, default=2236)
+    pa.add_argument("--seed", type=int, default=1)
     sv = sub.add_parser("serving")
     sv.add_argument("--count", type=int, default=120)
     sv.add_argument("--rate", type=float, default=10.0)
@@ -605,6 +677,8 @@ def main(argv=None):
         out = run_eigen(args)
     elif args.cmd == "ssor":
         out = run_ssor(args)
+    elif args.cmd == "spmv-patterns":
+        out = run_spmv_patterns(args)
     print(json.dumps(out), flush=True)
 
 

@@ -205,6 +205,7 @@ SIGNATURES = {
     "nlhipBenchSpMV": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
     "nlhipBenchCopy": (C.c_int, [C.c_uint64, C.c_uint32, _f64p]),
     "nlhipBenchRead": (C.c_int, [C.c_uint64, C.c_uint32, _f64p]),
+    "nlhipBenchStreamMix": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int, C.c_uint32, _f64p]),
     "nlhipProfileCG": (C.c_int, [_vp, C.c_int, C.c_uint32, _f64p]),
     "nlhipBatchCreate": (C.c_int, [C.c_uint32, _vp, _vp, _vp, _vp, C.c_uint32,
                                    C.POINTER(_vp)]),
@@ -695,6 +696,16 @@ def bench_read(nbytes: int, reps: int = 20) -> float:
     """Mean seconds of a read-only pass over `nbytes` of device memory."""
     t = C.c_double()
     check(lib().nlhipBenchRead(nbytes, reps, C.byref(t)), "nlhipBenchRead")
+    return float(t.value)
+
+
+def bench_stream_mix(blocks: int, blocks_per_row: int, writes: bool = True,
+                     reps: int = 20) -> float:
+    """Mean seconds of one pass of the block SpMV's traffic without its
+    arithmetic (nlhipBenchStreamMix)."""
+    t = C.c_double()
+    check(lib().nlhipBenchStreamMix(blocks, blocks_per_row, 1 if writes else 0, reps,
+                                    C.byref(t)), "nlhipBenchStreamMix")
     return float(t.value)
 
 

@@ -185,6 +185,111 @@ hipError_t read16(uint64_t n16, const void* src, void* out, uint32_t grid, bool 
     return hipGetLastError();
 }
 
+// The same read-only stream with each wavefront reading one contiguous run
+// (a multiple of 256 words; wavefront v of the launch takes the v-th run),
+// four words per lane per trip, non-temporal loads.
+__global__ __launch_bounds__(kBlock) void k_read16_runs(uint64_t n16,
+                                                        const uint4* __restrict__ src,
+                                                        uint4* __restrict__ out) {
+    using V4 = unsigned int __attribute__((ext_vector_type(4)));
+    const V4* __restrict__ p = reinterpret_cast<const V4*>(src);
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t W = uint64_t(gridDim.x) * kWavesPerBlock;
+    const uint64_t v = uint64_t(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+    const uint64_t per = ((n16 + W - 1) / W + 255) / 256 * 256;
+    const uint64_t b = v * per < n16 ? v * per : n16;
+    const uint64_t e = b + per < n16 ? b + per : n16;
+    V4 acc = {0u, 0u, 0u, 0u};
+    uint64_t i = b + lane;
+    for (; i + 192 < e; i += 256) {
+        V4 w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[u] = __builtin_nontemporal_load(p + i + 64 * u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc ^= w[u];
+    }
+    for (; i < e; i += 64) acc ^= __builtin_nontemporal_load(p + i);
+    out[uint64_t(blockIdx.x) * kBlock + threadIdx.x] = make_uint4(acc.x, acc.y, acc.z, acc.w);
+}
+
+hipError_t read16_runs(uint64_t n16, const void* src, void* out, uint32_t grid, hipStream_t s) {
+    if (n16 == 0) return hipSuccess;
+    if (grid == 0) return hipErrorInvalidValue;
+    k_read16_runs<<<grid, kBlock, 0, s>>>(n16, static_cast<const uint4*>(src),
+                                          static_cast<uint4*>(out));
+    return hipGetLastError();
+}
+
+// The SpMV's memory traffic without its arithmetic: per 2x2 block, 32 bytes
+// of values and a 2-byte column offset streamed (non-temporal) and one
+// 16-byte word gathered from a vector that stays in L2 (near the block's
+// row); per row pair (every `bpr` blocks) one 16-byte word read and, with
+// WRITE, one written.  Each wavefront streams one contiguous run of blocks,
+// a multiple of 256 (wavefront v of the launch takes the v-th), 4 blocks
+// per lane per trip; the row words of a trip are those of the row pairs
+// whose last block it reads.
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void k_stream_mix(uint64_t nblk, uint32_t bpr,
+                                                       const double2* __restrict__ vals,
+                                                       const uint16_t* __restrict__ off,
+                                                       const double2* __restrict__ xg,
+                                                       const double2* __restrict__ rin,
+                                                       double2* __restrict__ rout,
+                                                       double2* __restrict__ out) {
+    using D2 = double __attribute__((ext_vector_type(2)));
+    const D2* __restrict__ pv = reinterpret_cast<const D2*>(vals);
+    const D2* __restrict__ px = reinterpret_cast<const D2*>(xg);
+    const D2* __restrict__ pr = reinterpret_cast<const D2*>(rin);
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t nrows = nblk / bpr;
+    const uint64_t W = uint64_t(gridDim.x) * kWavesPerBlock;
+    const uint64_t v = uint64_t(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+    const uint64_t per = ((nblk + W - 1) / W + 255) / 256 * 256;
+    const uint64_t b = v * per < nblk ? v * per : nblk;
+    const uint64_t e = b + per < nblk ? b + per : nblk;
+    D2 acc = {0.0, 0.0};
+    for (uint64_t i0 = b; i0 < e; i0 += 256) {
+        D2 lo[4], hi[4];
+        uint32_t c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            uint64_t k = i0 + 64 * u + lane;
+            k = k < e ? k : e - 1;
+            c[u] = __builtin_nontemporal_load(off + k);
+            lo[u] = __builtin_nontemporal_load(pv + 2 * k);
+            hi[u] = __builtin_nontemporal_load(pv + 2 * k + 1);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t k = i0 + 64 * u + lane;
+            const D2 xv = px[(k / bpr + (c[u] & 3u)) % nrows];
+            acc += (lo[u] + hi[u]) * xv;
+        }
+        const uint64_t i1 = i0 + 256 < e ? i0 + 256 : e;
+        for (uint64_t r = i0 / bpr + lane; r < i1 / bpr && r < nrows; r += kWave) {
+            const D2 rv = pr[r];
+            if (WRITE) rout[r] = make_double2(acc.x * rv.x, acc.y * rv.y);
+            else acc += rv;
+        }
+    }
+    if (acc.x == 1.2345e300) out[0] = make_double2(acc.x, acc.y);
+}
+
+hipError_t stream_mix(uint64_t nblk, uint32_t bpr, const double* vals, const uint16_t* off,
+                      const double* xg, const double* rin, double* rout, double* out,
+                      bool write, uint32_t grid, hipStream_t s) {
+    if (nblk == 0) return hipSuccess;
+    if (grid == 0 || bpr == 0 || nblk < bpr) return hipErrorInvalidValue;
+    auto* v = reinterpret_cast<const double2*>(vals);
+    auto* x = reinterpret_cast<const double2*>(xg);
+    auto* ri = reinterpret_cast<const double2*>(rin);
+    auto* ro = reinterpret_cast<double2*>(rout);
+    auto* o = reinterpret_cast<double2*>(out);
+    if (write) k_stream_mix<true><<<grid, kBlock, 0, s>>>(nblk, bpr, v, off, x, ri, ro, o);
+    else k_stream_mix<false><<<grid, kBlock, 0, s>>>(nblk, bpr, v, off, x, ri, ro, o);
+    return hipGetLastError();
+}
+
 hipError_t reduce_final(const double* part, uint32_t P, double* out, hipStream_t s) {
     k_reduce_final<<<1, kBlock, 0, s>>>(part, P, out);
     return hipGetLastError();

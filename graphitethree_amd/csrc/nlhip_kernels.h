@@ -109,6 +109,16 @@ hipError_t copy16(uint64_t n16, const void* src, void* dst, uint32_t grid, hipSt
 // Reads n16 16-byte words; out receives grid x 256 words.
 hipError_t read16(uint64_t n16, const void* src, void* out, uint32_t grid, bool nt,
                   hipStream_t s);
+// Each wavefront reads one contiguous run of the buffer (non-temporal).
+hipError_t read16_runs(uint64_t n16, const void* src, void* out, uint32_t grid, hipStream_t s);
+// The SpMV's traffic without its arithmetic (k_blas.hip): nblk 32-byte
+// blocks and 2-byte offsets streamed, a 16-byte gather per block from an
+// L2-resident vector of nblk / bpr words (xg), and per row pair (bpr
+// blocks) a 16-byte read from rin and, when write, a 16-byte store to rout
+// (both nblk / bpr words); out: 16 bytes.
+hipError_t stream_mix(uint64_t nblk, uint32_t bpr, const double* vals, const uint16_t* off,
+                      const double* xg, const double* rin, double* rout, double* out,
+                      bool write, uint32_t grid, hipStream_t s);
 template <typename T>
 hipError_t fill(uint64_t n, T v, T* x, hipStream_t s);
 // out[0] = x.y, out[1] = sqrt(x.y); part holds `grid` doubles.

@@ -1192,20 +1192,63 @@ NLAPI int NLAPIENTRY nlhipBenchRead(uint64_t bytes, uint32_t reps, double* secon
         hipStream_t st = stream.get();
         NLHIP_CHECK(fill<int32_t>(bytes / 4, 1, reinterpret_cast<int32_t*>(src.get()), st));
         double best = 0;
-        for (bool nt : {false, true}) {
-            for (uint64_t g : {uint64_t(d.cu_count) * 4, uint64_t(d.cu_count) * 16,
-                               uint64_t(d.cu_count) * 64, one_trip}) {
-                const uint32_t grid = uint32_t(std::min<uint64_t>(g, max_grid));
-                NLHIP_CHECK(read16(n16, src.get(), out.get(), grid, nt, st));
-                Event e0, e1;
-                e0.record(st);
-                for (uint32_t k = 0; k < reps; ++k)
-                    NLHIP_CHECK(read16(n16, src.get(), out.get(), grid, nt, st));
-                e1.record(st);
-                e1.sync();
-                const double t = 1e-3 * elapsed_ms(e0, e1) / reps;
-                if (best == 0 || t < best) best = t;
-            }
+        auto time_it = [&](auto launch) {
+            NLHIP_CHECK(launch());
+            Event e0, e1;
+            e0.record(st);
+            for (uint32_t k = 0; k < reps; ++k) NLHIP_CHECK(launch());
+            e1.record(st);
+            e1.sync();
+            const double t = 1e-3 * elapsed_ms(e0, e1) / reps;
+            if (best == 0 || t < best) best = t;
+        };
+        // grid-stride words (plain and non-temporal loads) and one
+        // contiguous run per wavefront, each over a few grid sizes
+        for (uint64_t g : {uint64_t(d.cu_count) * 4, uint64_t(d.cu_count) * 16,
+                           uint64_t(d.cu_count) * 64, one_trip}) {
+            const uint32_t grid = uint32_t(std::min<uint64_t>(g, max_grid));
+            for (bool nt : {false, true})
+                time_it([&] { return read16(n16, src.get(), out.get(), grid, nt, st); });
+        }
+        for (uint64_t m : {2u, 4u, 8u, 16u}) {
+            const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(d.cu_count) * m, max_grid));
+            time_it([&] { return read16_runs(n16, src.get(), out.get(), grid, st); });
+        }
+        if (seconds_per_pass) *seconds_per_pass = best;
+    });
+}
+
+NLAPI int NLAPIENTRY nlhipBenchStreamMix(uint64_t blocks, uint32_t blocks_per_row, int writes,
+                                         uint32_t reps, double* seconds_per_pass) {
+    return guarded("nlhipBenchStreamMix", [&] {
+        if (blocks_per_row == 0 || blocks < blocks_per_row)
+            throw Error(NLHIP_ERR_INVALID, "need blocks >= blocks_per_row > 0");
+        const Device& d = device();
+        if (reps == 0) reps = 1;
+        const uint64_t rows = blocks / blocks_per_row;
+        DevBuf<double> vals(4 * blocks), xg(2 * rows), rin(2 * rows), rout(2 * rows), out(2);
+        DevBuf<uint16_t> off(blocks);
+        Stream stream;
+        hipStream_t st = stream.get();
+        NLHIP_CHECK(fill<double>(4 * blocks, 1.0, vals.get(), st));
+        NLHIP_CHECK(fill<double>(2 * rows, 1.0, xg.get(), st));
+        NLHIP_CHECK(fill<double>(2 * rows, 1.0, rin.get(), st));
+        NLHIP_CHECK(hipMemsetAsync(off.get(), 0, blocks * 2, st));
+        double best = 0;
+        for (uint64_t m : {2u, 4u, 8u, 16u}) {
+            const uint32_t grid = uint32_t(uint64_t(d.cu_count) * m);
+            auto launch = [&] {
+                return stream_mix(blocks, blocks_per_row, vals.get(), off.get(), xg.get(),
+                                  rin.get(), rout.get(), out.get(), writes != 0, grid, st);
+            };
+            NLHIP_CHECK(launch());
+            Event e0, e1;
+            e0.record(st);
+            for (uint32_t k = 0; k < reps; ++k) NLHIP_CHECK(launch());
+            e1.record(st);
+            e1.sync();
+            const double t = 1e-3 * elapsed_ms(e0, e1) / reps;
+            if (best == 0 || t < best) best = t;
         }
         if (seconds_per_pass) *seconds_per_pass = best;
     });

@@ -270,10 +270,21 @@ NLAPI int NLAPIENTRY nlhipBenchSpMV(nlhip_system s, int fp32, uint32_t reps,
  * solver kernels are compared with. */
 NLAPI int NLAPIENTRY nlhipBenchCopy(uint64_t bytes, uint32_t reps, double* seconds_per_copy);
 /* Times `reps` read-only passes over `bytes` (a multiple of 16) of a buffer it
- * allocates (16-byte loads, plain and non-temporal, a few grid sizes);
- * returns the mean seconds per pass of the fastest.  The bandwidth reference
- * for read-dominated kernels such as the SpMV. */
+ * allocates (16-byte loads: grid-stride, plain and non-temporal, or one
+ * contiguous run per wavefront; a few grid sizes each); returns the mean
+ * seconds per pass of the fastest.  The bandwidth reference for
+ * read-dominated kernels such as the SpMV. */
 NLAPI int NLAPIENTRY nlhipBenchRead(uint64_t bytes, uint32_t reps, double* seconds_per_pass);
+/* Times `reps` passes of the 2x2-block SpMV's memory traffic without its
+ * arithmetic: `blocks` 32-byte blocks and 2-byte column offsets streamed, a
+ * 16-byte gather per block from an L2-resident vector, and per row pair
+ * (every blocks_per_row blocks) a 16-byte read and, when writes != 0, a
+ * 16-byte write; each wavefront streams one contiguous run; returns the
+ * mean seconds per pass of the fastest of a few grid sizes.  The attainable
+ * rate for the SpMV's access mix, which the read-only and copy references
+ * bracket.  No counterpart in the reference. */
+NLAPI int NLAPIENTRY nlhipBenchStreamMix(uint64_t blocks, uint32_t blocks_per_row, int writes,
+                                         uint32_t reps, double* seconds_per_pass);
 
 /* Runs `iters` PCG iterations (FP64, or FP32 when fp32 = 1, b = 1) launching
  * each kernel directly; seconds[0..2] receive the mean HIP-event duration of

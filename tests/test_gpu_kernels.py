@@ -261,3 +261,20 @@ def test_jacobi_preconditioner_apply(hip):
     assert np.array_equal(y, O.jacobi(A) * x)
     L.nlDeleteMatrix(P)
     L.nlDeleteMatrix(M)
+
+
+def test_bandwidth_probes(hip):
+    """The copy, read and SpMV-shaped stream probes time real passes (a pass
+    of 64 MiB takes microseconds, not nothing) and reject bad sizes; the
+    stream with its row writes takes at least as long as without them."""
+    t_copy = capi.bench_copy(64 << 20, reps=3)
+    t_read = capi.bench_read(64 << 20, reps=3)
+    assert 1e-6 < t_copy < 1.0 and 1e-6 < t_read < 1.0
+    t_mix = capi.bench_stream_mix(2 << 20, 7, writes=True, reps=3)
+    t_ro = capi.bench_stream_mix(2 << 20, 7, writes=False, reps=3)
+    assert 1e-6 < t_ro < 1.0 and 1e-6 < t_mix < 1.0
+    assert t_mix >= 0.9 * t_ro
+    with pytest.raises(capi.NLHIPError):
+        capi.bench_read(100, reps=1)
+    with pytest.raises(capi.NLHIPError):
+        capi.bench_stream_mix(5, 7)
