@@ -369,15 +369,18 @@ struct StreamEngine {
 //
 // For matrices in 2x2 block form: rows 2I and 2I+1 have the same columns,
 // which come in pairs (2J, 2J+1) (LSCM normal equations: the u and v
-// unknowns of a vertex).  Block row I stores its blocks as (a(2I,2J),
-// a(2I,2J+1), a(2I+1,2J), a(2I+1,2J+1)) with the block column J as a 16-bit
-// segment offset (Cols::ci16 / Cols::segb over the block stream; 8.5 bytes
-// per FP64 nonzero) or, with D16 = false, as a 32-bit index in Cols::ci.  One block row per lane, 64 block rows per wavefront
-// tile, chunks of 64*EB blocks; the four products of every block are staged
-// in LDS and each lane sums its two rows left to right (a(2I,2J) x_2J, then
-// a(2I,2J+1) x_2J+1, for J ascending) -- the order and roundings of the CRS
-// row loop.  `rp` is the block row pointer (m/2 + 1 entries), `val` the
-// block values (4 per block), `m` the number of (scalar) rows.
+// unknowns of a vertex).  Block row I stores its blocks in order, each as
+// an upper pair (a(2I,2J), a(2I,2J+1)) and a lower pair (a(2I+1,2J),
+// a(2I+1,2J+1)) in the grouped layout of block_upper / block_lower, with
+// the block column J as a 16-bit segment offset (Cols::ci16 / Cols::segb
+// over the block stream; 8.5 bytes per FP64 nonzero) or, with D16 = false,
+// as a 32-bit index in Cols::ci.  One block row per lane, 64 block rows per
+// wavefront tile, chunks of 64*EB blocks; the four products of every block
+// are staged in LDS and each lane sums its two rows left to right
+// (a(2I,2J) x_2J, then a(2I,2J+1) x_2J+1, for J ascending) -- the order and
+// roundings of the CRS row loop.  `rp` is the block row pointer (m/2 + 1
+// entries), `val` the block values (block_vals_size), `m` the number of
+// (scalar) rows.
 
 template <typename T>
 struct Vec2;
@@ -394,14 +397,14 @@ template <int EB, bool D16 = true, int RUN = 0>
 struct BlockStreamEngine {
     static_assert(64 * EB <= int(kSegEntries), "a chunk must span at most two segments");
 
-    // block values of block jj: 4 T, 16-byte aligned for double (32-byte
-    // blocks), 16-byte for float
+    // block values of block jj (upper and lower pairs, grouped by 64
+    // blocks: block_upper / block_lower), each pair one aligned 2-vector
     template <typename T>
     __device__ __forceinline__ static void load_vals(const T* __restrict__ val, uint32_t jj,
                                                      T (&a)[4]) {
         using V2 = typename Vec2<T>::type;
-        const V2* p = reinterpret_cast<const V2*>(val + 4ull * jj);
-        const V2 lo = __builtin_nontemporal_load(p), hi = __builtin_nontemporal_load(p + 1);
+        const V2 lo = __builtin_nontemporal_load(reinterpret_cast<const V2*>(val + block_upper(jj)));
+        const V2 hi = __builtin_nontemporal_load(reinterpret_cast<const V2*>(val + block_lower(jj)));
         a[0] = lo.x;
         a[1] = lo.y;
         a[2] = hi.x;

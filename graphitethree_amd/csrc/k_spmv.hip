@@ -199,10 +199,10 @@ __global__ __launch_bounds__(kBlock) void k_block_pack(uint32_t nb,
     for (uint32_t q = 0; q < len; q += 2) {
         const uint64_t k = b0 / 4 + q / 2;
         if (bcol) bcol[k] = ci[b0 + q] / 2;
-        bval[4 * k + 0] = T(val[b0 + q]);
-        bval[4 * k + 1] = T(val[b0 + q + 1]);
-        bval[4 * k + 2] = T(val[b1 + q]);
-        bval[4 * k + 3] = T(val[b1 + q + 1]);
+        bval[block_upper(k)] = T(val[b0 + q]);
+        bval[block_upper(k) + 1] = T(val[b0 + q + 1]);
+        bval[block_lower(k)] = T(val[b1 + q]);
+        bval[block_lower(k) + 1] = T(val[b1 + q + 1]);
     }
 }
 

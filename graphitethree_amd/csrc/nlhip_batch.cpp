@@ -580,7 +580,7 @@ void build_batch_blocks(Batch& B, const uint32_t* row_off, const uint32_t* rowpt
     for (uint32_t I = 0; I < np; ++I) h_brp[I + 1] += h_brp[I];
     const uint64_t nblk = h_brp[np];
     std::vector<uint32_t> h_bcol(nblk);
-    std::vector<double> h_bval(4 * nblk);
+    std::vector<double> h_bval(block_vals_size(nblk), 0.0);
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t s = 0; s < int64_t(nsys); ++s) {
         if (!blk[size_t(s)]) continue;
@@ -590,10 +590,10 @@ void build_batch_blocks(Batch& B, const uint32_t* row_off, const uint32_t* rowpt
             uint64_t k = h_brp[Bk];
             for (uint32_t q = 0; q < len; q += 2, ++k) {
                 h_bcol[k] = p0 + (colind[b0 + q] - lo) / 2;
-                h_bval[4 * k + 0] = val[b0 + q];
-                h_bval[4 * k + 1] = val[b0 + q + 1];
-                h_bval[4 * k + 2] = val[b1 + q];
-                h_bval[4 * k + 3] = val[b1 + q + 1];
+                h_bval[block_upper(k)] = val[b0 + q];
+                h_bval[block_upper(k) + 1] = val[b0 + q + 1];
+                h_bval[block_lower(k)] = val[b1 + q];
+                h_bval[block_lower(k) + 1] = val[b1 + q + 1];
             }
         }
     }
@@ -601,11 +601,11 @@ void build_batch_blocks(Batch& B, const uint32_t* row_off, const uint32_t* rowpt
     B.sys_pair0.alloc(nsys);
     B.brp.alloc(size_t(np) + 1);
     B.bcol.alloc(nblk);
-    B.bval.alloc(4 * nblk);
+    B.bval.alloc(h_bval.size());
     copy_in(B.sys_pair0.get(), pair0.data(), size_t(nsys) * 4, st);
     copy_in(B.brp.get(), h_brp.data(), h_brp.size() * 4, st);
     copy_in(B.bcol.get(), h_bcol.data(), nblk * 4, st);
-    copy_in(B.bval.get(), h_bval.data(), nblk * 32, st);
+    copy_in(B.bval.get(), h_bval.data(), h_bval.size() * 8, st);
     if (build_col_offsets(nblk, B.bcol.get(), st, B.bci16, B.bsegb)) B.bcol.release();
     NLHIP_CHECK(hipStreamSynchronize(st));  // host staging arrays go out of scope
 }
@@ -694,7 +694,7 @@ bool build_batch_blocks_device(Batch& B, std::vector<int32_t>& blk) {
     B.sys_pair0.alloc(B.nsys);
     B.brp.alloc(size_t(nb) + 1);
     B.bcol.alloc(nblk);
-    B.bval.alloc(4 * nblk);
+    B.bval.alloc(block_vals_size(nblk));
     copy_in(B.sys_pair0.get(), pair0.data(), size_t(B.nsys) * 4, st);
     NLHIP_CHECK(block_pack<double>(nb, B.rowptr.get(), B.colind.get(), B.val.get(), B.brp.get(),
                                    B.bcol.get(), B.bval.get(), st));

@@ -87,7 +87,7 @@ bool build_block_form(uint32_t n, uint64_t nnz, const uint32_t* rowptr, const ui
     NLHIP_CHECK(hipStreamSynchronize(s));
     if (h_bad) return false;
     DevBuf<uint32_t> brp(size_t(nb) + 1), bcol(nblk);
-    DevBuf<double> bval(nnz);
+    DevBuf<double> bval(block_vals_size(nblk));
     NLHIP_CHECK(block_pack<double>(nb, rowptr, colind, val, brp.get(), bcol.get(), bval.get(), s));
     DevBuf<uint16_t> bci16;
     DevBuf<uint32_t> bsegb;
@@ -192,7 +192,7 @@ void System::ensure_fp32() {
     NLHIP_CHECK(cast_f64_f32(nnz, val.get(), val32.get(), s));
     NLHIP_CHECK(jacobi_dinv<float>(n, rowptr.get(), colind.get(), val.get(), dinv32.get(), s));
     if (blocks()) {
-        bval32.alloc(nnz);
+        bval32.alloc(block_vals_size(nnz / 4));
         NLHIP_CHECK(block_pack<float>(n / 2, rowptr.get(), colind.get(), val.get(), brp.get(),
                                       nullptr, bval32.get(), s));
     }
