@@ -26,8 +26,10 @@ spmv-patterns: the CG SpMV step on the 2236 x 2236 LSCM system against two
   synthetic 2x2-block systems of the same size and row length whose x
   gathers are (a) a band of 7 block columns around the diagonal (every
   gather a cache hit) and (b) the grid stencil of the LSCM system (rows
-  i +- 1, i +- w) with random values: how much of the SpMV step's time the
-  gather pattern costs.
+  i +- 1, i +- w) with random values, and (c) the LSCM system with its
+  vertices in random order (32-bit block columns, gathers anywhere), with
+  and without the block form: how much of the SpMV step's time the gather
+  pattern costs.
 
 serving: a stream of requests with Poisson arrivals (rate --rate per second):
   60% LSCM charts with N ~ logU[1e3, 1e6] vertices, 40% smoothing solves
@@ -266,9 +268,9 @@ def run_spmv_patterns(args) -> dict:
     dev = init_device()
     res = {"device": dev[0], "patterns": []}
 
-    def measure(name, rp, ci, val):
+    def measure(name, rp, ci, val, blocks=True):
         n, nnz = len(rp) - 1, len(ci)
-        S = capi.System(rp, ci, val)
+        S = capi.System(rp, ci, val, blocks=blocks)
         tuned = S.autotune(iters=20)
         td, ts, tu = capi.profile_cg(S, iters=50)
         byts = 8.5 * nnz + 26.0 * n
@@ -287,6 +289,23 @@ def run_spmv_patterns(args) -> dict:
     measure("band of 7 block columns", *block_band_system(nb, range(-3, 4), args.seed))
     measure(f"grid stencil (I, I +- 1, I +- {w}, I +- {w + 1}), random values",
             *block_band_system(nb, (-w - 1, -w, -1, 0, 1, w, w + 1), args.seed))
+    if not args.no_shuffled:
+        # the LSCM system with its vertices in random order: block columns
+        # too far apart for 16-bit offsets (32-bit block columns), gathers
+        # anywhere in x; and the same matrix without the block form
+        import scipy.sparse as sps
+        G = W.lscm_grid(args.k, seed=args.seed)
+        p = np.random.default_rng(args.seed).permutation(G.n // 2)
+        q = np.empty(G.n, np.int64)
+        q[0::2], q[1::2] = 2 * p, 2 * p + 1
+        M = sps.csr_matrix((G.val, G.colind.astype(np.int64), G.rowptr.astype(np.int64)),
+                           shape=(G.n, G.n))[q][:, q].tocsr()
+        del G
+        M.sort_indices()
+        rp, ci = M.indptr.astype(np.uint32), M.indices.astype(np.uint32)
+        measure(f"lscm_grid_{args.k}, vertices in random order", rp, ci, M.data)
+        measure(f"lscm_grid_{args.k}, vertices in random order, no block form", rp, ci, M.data,
+                blocks=False)
     return res
 
 
@@ -650,6 +669,8 @@ def main(argv=None):
     pa = sub.add_parser("spmv-patterns")
     pa.add_argument("--k", type=int, default=2236)
     pa.add_argument("--seed", type=int, default=1)
+    pa.add_argument("--no-shuffled", action="store_true",
+                    help="skip the system with its vertices in random order")
     sv = sub.add_parser("serving")
     sv.add_argument("--count", type=int, default=120)
     sv.add_argument("--rate", type=float, default=10.0)

@@ -663,6 +663,14 @@ template <>
 struct EngineOf<77> {  // 2x2 blocks, XCD-interleaved runs of 4 tiles
     using type = BlockStreamEngine<kBlockEB, true, 4>;
 };
+template <>
+struct EngineOf<78> {  // 2x2 blocks, 32-bit block columns, 4 per lane per chunk
+    using type = BlockStreamEngine<kBlockEB, false>;
+};
+template <>
+struct EngineOf<79> {  // 2x2 blocks, 32-bit block columns, 8 per lane per chunk
+    using type = BlockStreamEngine<8, false>;
+};
 
 template <int RPW>
 struct StageSize {
@@ -671,10 +679,11 @@ struct StageSize {
                                  : RPW == 65 ? kWavesPerBlock * 64 * 4
                                  : RPW == 66 ? kWavesPerBlock * 64 * 8
                                  : (RPW == 67 || RPW == 72) ? kWavesPerBlock * 64 * 16
-                                 : (RPW == 73 || RPW == 77) ? kWavesPerBlock * 64 * kBlockEB * 4
+                                 : (RPW == 73 || RPW == 77 || RPW == 78)
+                                     ? kWavesPerBlock * 64 * kBlockEB * 4
                                  : RPW == 74 ? kWavesPerBlock * 64 * 2 * 4
                                  : RPW == 75 ? kWavesPerBlock * 64 * 1 * 4
-                                 : RPW == 76 ? kWavesPerBlock * 64 * 8 * 4
+                                 : (RPW == 76 || RPW == 79) ? kWavesPerBlock * 64 * 8 * 4
                                              : 1;
 };
 

@@ -221,13 +221,14 @@ hipError_t read16_runs(uint64_t n16, const void* src, void* out, uint32_t grid, 
 }
 
 // The SpMV's memory traffic without its arithmetic: per 2x2 block, 32 bytes
-// of values and a 2-byte column offset streamed (non-temporal) and one
-// 16-byte word gathered from a vector that stays in L2 (near the block's
-// row); per row pair (every `bpr` blocks) one 16-byte word read and, with
-// WRITE, one written.  Each wavefront streams one contiguous run of blocks,
-// a multiple of 256 (wavefront v of the launch takes the v-th), 4 blocks
-// per lane per trip; the row words of a trip are those of the row pairs
-// whose last block it reads.
+// of values (its upper and lower pairs, in the grouped layout of
+// block_upper / block_lower) and a 2-byte column offset streamed
+// (non-temporal) and one 16-byte word gathered from a vector that stays in
+// L2 (near the block's row); per row pair (every `bpr` blocks) one 16-byte
+// word read and, with WRITE, one written.  Each wavefront streams one
+// contiguous run of blocks, a multiple of 256 (wavefront v of the launch
+// takes the v-th), 4 blocks per lane per trip; the row words of a trip are
+// those of the row pairs whose last block it reads.
 template <bool WRITE>
 __global__ __launch_bounds__(kBlock) void k_stream_mix(uint64_t nblk, uint32_t bpr,
                                                        const double2* __restrict__ vals,
@@ -256,8 +257,8 @@ __global__ __launch_bounds__(kBlock) void k_stream_mix(uint64_t nblk, uint32_t b
             uint64_t k = i0 + 64 * u + lane;
             k = k < e ? k : e - 1;
             c[u] = __builtin_nontemporal_load(off + k);
-            lo[u] = __builtin_nontemporal_load(pv + 2 * k);
-            hi[u] = __builtin_nontemporal_load(pv + 2 * k + 1);
+            lo[u] = __builtin_nontemporal_load(pv + block_upper(k) / 2);
+            hi[u] = __builtin_nontemporal_load(pv + block_lower(k) / 2);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {

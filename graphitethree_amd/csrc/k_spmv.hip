@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_cast_f32(uint64_t n, const double* _
 
 uint32_t spmv_work_per_block(uint32_t m, uint32_t grid, int rpw) {
     if (rpw >= 64) {
-        const uint64_t rows = rpw >= 73 ? m / 2 : m;  // 73..77: tiles of 64 block rows
+        const uint64_t rows = rpw >= 73 ? m / 2 : m;  // 73..79: tiles of 64 block rows
         const uint64_t tiles = (rows + 63) / 64;
         uint64_t t = (tiles + grid - 1) / grid;
         return uint32_t(t ? t : 1);

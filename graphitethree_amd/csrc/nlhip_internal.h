@@ -433,8 +433,10 @@ struct System {
     DevBuf<uint16_t> ci16;   // 16-bit column offsets (empty when the span is too wide)
     DevBuf<uint32_t> segb;   // their per-segment bases
     // 2x2 block form (BlockStreamEngine; empty when the matrix has none):
-    // block row pointer, block columns as 16-bit segment offsets, values
-    DevBuf<uint32_t> brp, bsegb;
+    // block row pointer, block columns as 16-bit segment offsets (bci16,
+    // bsegb) or, when a segment spans more than 16 bits, as 32-bit indices
+    // (bcol), values
+    DevBuf<uint32_t> brp, bsegb, bcol;
     DevBuf<uint16_t> bci16;
     DevBuf<double> bval;
     DevBuf<float> bval32;
@@ -470,20 +472,26 @@ struct System {
             v = val32.get();
             bv = bval32.get();
         }
-        if (blocks() && (rpw >= 73 || (rpw == 64 && n >= kBlockMinRows)))
-            return SpmvOps<T>{brp.get(), Cols{nullptr, bci16.get(), bsegb.get()}, bv,
-                              rpw >= 73 ? rpw : 73};
+        if (blocks() && (rpw >= 73 || (rpw == 64 && n >= kBlockMinRows))) {
+            if (bci16.get())
+                return SpmvOps<T>{brp.get(), Cols{nullptr, bci16.get(), bsegb.get()}, bv,
+                                  block_engine(rpw >= 73 ? rpw : 73, true)};
+            return SpmvOps<T>{brp.get(), Cols{bcol.get(), nullptr, nullptr}, bv,
+                              block_engine(rpw >= 73 ? rpw : 73, false)};
+        }
         const int code = rpw >= 73 ? 64 : rpw;
         return SpmvOps<T>{rowptr.get(), cols(), v, engine_rpw(code, cols())};
     }
 };
 
-// Builds the 2x2 block form (brp, bci16, bsegb, bval; see
-// BlockStreamEngine) of a device CRS matrix when it has that form and its
-// block columns fit 16-bit offsets; false otherwise.
+// Builds the 2x2 block form (brp, bval, and the block columns as 16-bit
+// offsets bci16 / bsegb or, when a segment's span does not fit, as 32-bit
+// indices bcol; see BlockStreamEngine) of a device CRS matrix when it has
+// that form; false otherwise.
 bool build_block_form(uint32_t n, uint64_t nnz, const uint32_t* rowptr, const uint32_t* colind,
                       const double* val, hipStream_t s, DevBuf<uint32_t>& brp,
-                      DevBuf<uint16_t>& bci16, DevBuf<uint32_t>& bsegb, DevBuf<double>& bval);
+                      DevBuf<uint16_t>& bci16, DevBuf<uint32_t>& bsegb, DevBuf<uint32_t>& bcol,
+                      DevBuf<double>& bval);
 bool build_blocks(System& S);
 
 // Builds the 16-bit column offsets of a device colind (see Cols); false when

@@ -10,7 +10,7 @@ namespace nlhip {
 
 // Rows-per-wavefront dispatch over the compiled SpMV engines (codes of
 // dev_common.h's EngineOf): 2, 4, 8, 16 -> 32, 16, 8, 4 lanes per row;
-// 65..72 -> stream engines (one row per lane); 73..77 -> 2x2 block stream
+// 65..72 -> stream engines (one row per lane); 73..79 -> 2x2 block stream
 // engines (their callers pass the block arrays).  Launchers pass the code
 // through engine_rpw first, which resolves 64.
 #define NLHIP_RPW_SWITCH(rpw, CALL)                       \
@@ -32,15 +32,17 @@ namespace nlhip {
         case 75: { constexpr int RPW = 75; CALL; } break; \
         case 76: { constexpr int RPW = 76; CALL; } break; \
         case 77: { constexpr int RPW = 77; CALL; } break; \
+        case 78: { constexpr int RPW = 78; CALL; } break; \
+        case 79: { constexpr int RPW = 79; CALL; } break; \
         default: return hipErrorInvalidValue;             \
     }
 
 // Launch shapes a caller may set: 2, 4, 8, 16, and the stream engines 64
-// (the default stream engine) and 65..77 (variants kept for sweeps; they
-// compute the same sums as 64; 73..77 run as 64 on matrices without 2x2
-// block form).
+// (the default stream engine) and 65..79 (variants kept for sweeps; they
+// compute the same sums as 64; 73..79 run as 64 on matrices without 2x2
+// block form, and as their twin for the matrix's block-column format).
 inline bool valid_rpw(int rpw) {
-    return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 77);
+    return rpw == 2 || rpw == 4 || rpw == 8 || rpw == 16 || (rpw >= 64 && rpw <= 79);
 }
 
 // Smallest system (rows) on which code 64 runs the 2x2 block engine: below
@@ -52,6 +54,14 @@ constexpr uint32_t kBlockMinRows = 131072;
 // gather less but are no faster on large systems and slower on small ones).
 constexpr int kStreamAuto = 68;
 constexpr int kStreamAutoPlain = 71;
+
+// Block engine for a requested block code and the matrix's block-column
+// format: 73..77 read 16-bit offsets, 78 / 79 (4 / 8 blocks per lane) the
+// 32-bit block columns; each maps to its twin for the other format.
+inline int block_engine(int code, bool d16) {
+    if (d16) return code == 78 ? 73 : (code == 79 ? 76 : code);
+    return code == 76 ? 79 : (code == 78 || code == 79 ? code : 78);
+}
 
 // Engine a launcher runs for a requested code: 64 becomes the default stream
 // engine for the matrix's column format; a variant that reads 16-bit offsets

@@ -75,7 +75,7 @@ bool build_col_offsets(uint64_t nnz, const uint32_t* colind, hipStream_t s,
 bool build_block_form(uint32_t n, uint64_t nnz, const uint32_t* rowptr, const uint32_t* colind,
                       const double* val, hipStream_t s, DevBuf<uint32_t>& brp_out,
                       DevBuf<uint16_t>& bci16_out, DevBuf<uint32_t>& bsegb_out,
-                      DevBuf<double>& bval_out) {
+                      DevBuf<uint32_t>& bcol_out, DevBuf<double>& bval_out) {
     if (n < 2 || (n & 1u) || nnz == 0 || (nnz & 3u)) return false;
     const uint32_t nb = n / 2;
     const uint64_t nblk = nnz / 4;
@@ -91,17 +91,21 @@ bool build_block_form(uint32_t n, uint64_t nnz, const uint32_t* rowptr, const ui
     NLHIP_CHECK(block_pack<double>(nb, rowptr, colind, val, brp.get(), bcol.get(), bval.get(), s));
     DevBuf<uint16_t> bci16;
     DevBuf<uint32_t> bsegb;
-    if (!build_col_offsets(nblk, bcol.get(), s, bci16, bsegb)) return false;
+    // block columns as 16-bit offsets where every segment's span allows it,
+    // else the 32-bit block columns (a matrix whose vertex numbering is not
+    // banded, e.g. a mesh in file order)
+    if (build_col_offsets(nblk, bcol.get(), s, bci16, bsegb)) bcol.release();
     brp_out = std::move(brp);
     bval_out = std::move(bval);
     bci16_out = std::move(bci16);
     bsegb_out = std::move(bsegb);
+    bcol_out = std::move(bcol);
     return true;
 }
 
 bool build_blocks(System& S) {
     return build_block_form(S.n, S.nnz, S.rowptr.get(), S.colind.get(), S.val.get(),
-                            S.stream.get(), S.brp, S.bci16, S.bsegb, S.bval);
+                            S.stream.get(), S.brp, S.bci16, S.bsegb, S.bcol, S.bval);
 }
 
 namespace {
@@ -200,7 +204,8 @@ void System::ensure_fp32() {
 
 uint64_t System::device_bytes() const {
     return rowptr.bytes() + colind.bytes() + ci16.bytes() + segb.bytes() + val.bytes() +
-           brp.bytes() + bsegb.bytes() + bci16.bytes() + bval.bytes() + bval32.bytes() +
+           brp.bytes() + bsegb.bytes() + bci16.bytes() + bcol.bytes() + bval.bytes() +
+           bval32.bytes() +
            dinv.bytes() + val32.bytes() +
            dinv32.bytes() + ws64.x.bytes() * 7 + ws32.x.bytes() * 7 + scratch.bytes();
 }
@@ -927,7 +932,8 @@ void profile_cg(System& S, bool fp32, uint32_t iters, double t[3]) {
 // choice (and with it the order of the sums) from run to run.
 TuneResult autotune(System& S, bool fp32, uint32_t iters) {
     std::vector<int> codes;
-    if (S.blocks()) codes = {73, 74, 75, 76, 77};
+    if (S.blocks() && S.bci16.get()) codes = {73, 74, 75, 76, 77};
+    if (S.blocks() && !S.bci16.get()) codes = {78, 79};
     if (S.ci16.get()) {
         codes.push_back(68);
         codes.push_back(70);
@@ -1033,7 +1039,7 @@ NLAPI int NLAPIENTRY nlhipSystemSetLaunch(nlhip_system s, int rows_per_wave,
     return guarded("nlhipSystemSetLaunch", [&] {
         System* S = sys(s);
         if (!valid_rpw(rows_per_wave))
-            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64..77");
+            throw Error(NLHIP_ERR_INVALID, "rows_per_wave must be 2, 4, 8, 16 or 64..79");
         if (workgroups == 0 || workgroups > 65536)
             throw Error(NLHIP_ERR_INVALID, "workgroups must be in [1, 65536]");
         std::lock_guard<std::mutex> lock(S->mu);
@@ -1226,11 +1232,12 @@ NLAPI int NLAPIENTRY nlhipBenchStreamMix(uint64_t blocks, uint32_t blocks_per_ro
         const Device& d = device();
         if (reps == 0) reps = 1;
         const uint64_t rows = blocks / blocks_per_row;
-        DevBuf<double> vals(4 * blocks), xg(2 * rows), rin(2 * rows), rout(2 * rows), out(2);
+        DevBuf<double> vals(block_vals_size(blocks)), xg(2 * rows), rin(2 * rows),
+            rout(2 * rows), out(2);
         DevBuf<uint16_t> off(blocks);
         Stream stream;
         hipStream_t st = stream.get();
-        NLHIP_CHECK(fill<double>(4 * blocks, 1.0, vals.get(), st));
+        NLHIP_CHECK(fill<double>(block_vals_size(blocks), 1.0, vals.get(), st));
         NLHIP_CHECK(fill<double>(2 * rows, 1.0, xg.get(), st));
         NLHIP_CHECK(fill<double>(2 * rows, 1.0, rin.get(), st));
         NLHIP_CHECK(hipMemsetAsync(off.get(), 0, blocks * 2, st));

@@ -193,7 +193,7 @@ NLAPI int NLAPIENTRY nlhipLSCMCreate(uint32_t nb_vertices, const double* points,
                                      double* rhs, uint32_t* nb_free);
 /* Launch shape of the SpMV engine: rows per wavefront (2, 4, 8, 16: that many
  * rows share a 64-lane wavefront; 64: one row per lane, nonzeros streamed
- * through LDS, the default) and workgroups per launch.  65..77 select
+ * through LDS, the default) and workgroups per launch.  65..79 select
  * variants of the streamed engine for launch-shape sweeps (chunk width,
  * load kind, column format, tile order, 2x2 blocks); they give the same sums
  * as 64. */
@@ -276,7 +276,8 @@ NLAPI int NLAPIENTRY nlhipBenchCopy(uint64_t bytes, uint32_t reps, double* secon
  * read-dominated kernels such as the SpMV. */
 NLAPI int NLAPIENTRY nlhipBenchRead(uint64_t bytes, uint32_t reps, double* seconds_per_pass);
 /* Times `reps` passes of the 2x2-block SpMV's memory traffic without its
- * arithmetic: `blocks` 32-byte blocks and 2-byte column offsets streamed, a
+ * arithmetic: `blocks` 32-byte blocks (in the block engine's grouped
+ * layout) and 2-byte column offsets streamed, a
  * 16-byte gather per block from an L2-resident vector, and per row pair
  * (every blocks_per_row blocks) a 16-byte read and, when writes != 0, a
  * 16-byte write; each wavefront streams one contiguous run; returns the

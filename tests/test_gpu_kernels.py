@@ -23,7 +23,7 @@ def test_spmv_exact_all_launch_shapes(hip):
     for name in z["names"]:
         rp, ci, va = z[f"{name}_rowptr"], z[f"{name}_colind"], z[f"{name}_val"]
         S = capi.System(rp, ci, va)
-        for rpw in (2, 4, 8, 16, 64, 65, 66, 67, 68, 69, 70, 71, 72, 73, 74, 75, 76, 77):
+        for rpw in (2, 4, 8, 16, 64, 65, 66, 67, 68, 69, 70, 71, 72, 73, 74, 75, 76, 77, 78, 79):
             for wg in (1, 3, 64, 2048):
                 S.set_launch(rpw, wg)
                 y = S.spmv(z[f"{name}_x"])
@@ -50,7 +50,7 @@ def test_stream_engine_is_bitwise_the_oracle_spmv(hip):
         x = np.random.default_rng(4).standard_normal(Sw.n)
         S = capi.System.from_crs(Sw)
         ref = O.spmv(A, x)
-        for rpw in (64, 65, 66, 67, 68, 69, 70, 71, 72, 73, 74, 75, 76, 77):
+        for rpw in (64, 65, 66, 67, 68, 69, 70, 71, 72, 73, 74, 75, 76, 77, 78, 79):
             for wg in (1, 5, 9, 300):
                 S.set_launch(rpw, wg)
                 assert np.array_equal(S.spmv(x), ref), (rpw, wg)
@@ -110,7 +110,7 @@ def test_block_form_matches_crs(hip):
     B = capi.System.from_crs(Sw)
     C = capi.System.from_crs(Sw, blocks=False)
     assert B.info()[2] > C.info()[2]          # block arrays were built
-    for rpw in (64, 73, 74, 75, 76, 77, 68, 71):  # the stream engines sum rows left to right
+    for rpw in (64, 73, 74, 75, 76, 77, 78, 79, 68, 71):  # the stream engines sum rows left to right
         for wg in (1, 3, 77, 2048):
             B.set_launch(rpw, wg)
             C.set_launch(rpw, wg)
@@ -135,6 +135,43 @@ def test_block_form_matches_crs(hip):
         S.close()
 
 
+def test_block_form_with_32bit_block_columns(hip):
+    """A mesh whose vertex numbering is not banded (here the 300 x 300 grid's
+    vertices in random order) has block columns too far apart for 16-bit
+    offsets: the block form keeps 32-bit block columns (engines 78 / 79) and
+    gives the oracle's row sums bit for bit on every block shape; PCG through
+    it reaches the residual bar and matches the scalar engine's solve to
+    1e-8."""
+    import scipy.sparse as sps
+    Sw = W.lscm_grid(300, seed=2)
+    rng = np.random.default_rng(11)
+    p = rng.permutation(Sw.n // 2)
+    q = np.empty(Sw.n, np.int64)  # new row k holds old row q[k]
+    q[0::2], q[1::2] = 2 * p, 2 * p + 1
+    M = sps.csr_matrix((Sw.val, Sw.colind.astype(np.int64), Sw.rowptr.astype(np.int64)),
+                       shape=(Sw.n, Sw.n))[q][:, q].tocsr()
+    M.sort_indices()
+    rp, ci, va = M.indptr.astype(np.uint32), M.indices.astype(np.uint32), M.data.copy()
+    b = Sw.b[q]
+    x = rng.standard_normal(Sw.n)
+    ref = O.spmv(O.CRS(rp, ci, va), x)
+    B = capi.System(rp, ci, va)
+    C = capi.System(rp, ci, va, blocks=False)
+    assert B.info()[2] > C.info()[2]          # block arrays were built
+    for rpw in (64, 73, 76, 77, 78, 79):
+        for wg in (1, 77, 2048):
+            B.set_launch(rpw, wg)
+            assert np.array_equal(B.spmv(x), ref), (rpw, wg)
+    B.set_launch(78, 1024)
+    C.set_launch(71, 1024)
+    xb, sb = B.solve(b, eps=1e-10, max_iter=100000)
+    xc, sc = C.solve(b, eps=1e-10, max_iter=100000)
+    assert sb.converged and sb.rel_residual <= 1e-10 and sc.rel_residual <= 1e-10
+    assert np.linalg.norm(xb - xc) / np.linalg.norm(xc) <= 1e-8
+    B.close()
+    C.close()
+
+
 def test_block_form_solves_like_scalar_form(hip):
     """PCG with the block engine and with the scalar stream engine: same
     residual target, iterations within the summation-order tolerance, both
@@ -157,7 +194,7 @@ def test_block_form_solves_like_scalar_form(hip):
 
 def test_set_launch_rejects_bad_shapes(hip):
     S = capi.System([0, 1], [0], [1.0])
-    for rpw, wg in [(3, 10), (32, 10), (78, 10), (64, 0)]:
+    for rpw, wg in [(3, 10), (32, 10), (80, 10), (64, 0)]:
         with pytest.raises(capi.NLHIPError):
             S.set_launch(rpw, wg)
 

@@ -418,7 +418,7 @@ def test_autotuned_system_matches_oracle(hip):
     Sw = W.lscm_grid(60, seed=7)
     S = capi.System.from_crs(Sw)
     rpw, wg, t = S.autotune(iters=10)
-    assert rpw in (68, 69, 70, 71, 73, 74, 75, 76, 77) and wg >= 1 and t > 0
+    assert rpw in (68, 69, 70, 71, 73, 74, 75, 76, 77, 78, 79) and wg >= 1 and t > 0
     assert S.launch() == (rpw, wg)
     x1, st = S.solve(Sw.b, eps=1e-10, max_iter=20000)
     x2, _ = S.solve(Sw.b, eps=1e-10, max_iter=20000)
