@@ -397,14 +397,26 @@ template <int EB, bool D16 = true, int RUN = 0>
 struct BlockStreamEngine {
     static_assert(64 * EB <= int(kSegEntries), "a chunk must span at most two segments");
 
-    // block values of block jj (upper and lower pairs, grouped by 64
-    // blocks: block_upper / block_lower), each pair one aligned 2-vector
+    // block values of block jj (block_upper / block_lower): FP64 as two
+    // aligned 2-vectors from the upper and lower pair groups, FP32 as one
+    // aligned 4-vector
     template <typename T>
     __device__ __forceinline__ static void load_vals(const T* __restrict__ val, uint32_t jj,
                                                      T (&a)[4]) {
+        if constexpr (sizeof(T) == 4) {
+            using V4 = T __attribute__((ext_vector_type(4)));
+            const V4 v = __builtin_nontemporal_load(reinterpret_cast<const V4*>(val + 4ull * jj));
+            a[0] = v.x;
+            a[1] = v.y;
+            a[2] = v.z;
+            a[3] = v.w;
+            return;
+        }
         using V2 = typename Vec2<T>::type;
-        const V2 lo = __builtin_nontemporal_load(reinterpret_cast<const V2*>(val + block_upper(jj)));
-        const V2 hi = __builtin_nontemporal_load(reinterpret_cast<const V2*>(val + block_lower(jj)));
+        const V2 lo =
+            __builtin_nontemporal_load(reinterpret_cast<const V2*>(val + block_upper<T>(jj)));
+        const V2 hi =
+            __builtin_nontemporal_load(reinterpret_cast<const V2*>(val + block_lower<T>(jj)));
         a[0] = lo.x;
         a[1] = lo.y;
         a[2] = hi.x;

@@ -257,8 +257,8 @@ __global__ __launch_bounds__(kBlock) void k_stream_mix(uint64_t nblk, uint32_t b
             uint64_t k = i0 + 64 * u + lane;
             k = k < e ? k : e - 1;
             c[u] = __builtin_nontemporal_load(off + k);
-            lo[u] = __builtin_nontemporal_load(pv + block_upper(k) / 2);
-            hi[u] = __builtin_nontemporal_load(pv + block_lower(k) / 2);
+            lo[u] = __builtin_nontemporal_load(pv + block_upper<double>(k) / 2);
+            hi[u] = __builtin_nontemporal_load(pv + block_lower<double>(k) / 2);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {

@@ -199,11 +199,32 @@ __global__ __launch_bounds__(kBlock) void k_block_pack(uint32_t nb,
     for (uint32_t q = 0; q < len; q += 2) {
         const uint64_t k = b0 / 4 + q / 2;
         if (bcol) bcol[k] = ci[b0 + q] / 2;
-        bval[block_upper(k)] = T(val[b0 + q]);
-        bval[block_upper(k) + 1] = T(val[b0 + q + 1]);
-        bval[block_lower(k)] = T(val[b1 + q]);
-        bval[block_lower(k) + 1] = T(val[b1 + q + 1]);
+        bval[block_upper<T>(k)] = T(val[b0 + q]);
+        bval[block_upper<T>(k) + 1] = T(val[b0 + q + 1]);
+        bval[block_lower<T>(k)] = T(val[b1 + q]);
+        bval[block_lower<T>(k) + 1] = T(val[b1 + q + 1]);
     }
+}
+
+// FP32 block values (block by block) from FP64 ones (grouped), nblk blocks.
+__global__ __launch_bounds__(kBlock) void k_block_vals_f32(uint64_t nblk,
+                                                           const double* __restrict__ src,
+                                                           float* __restrict__ dst) {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t k = uint64_t(blockIdx.x) * kBlock + threadIdx.x; k < nblk; k += stride) {
+        const uint64_t u = block_upper<double>(k), l = block_lower<double>(k);
+        dst[block_upper<float>(k)] = float(src[u]);
+        dst[block_upper<float>(k) + 1] = float(src[u + 1]);
+        dst[block_lower<float>(k)] = float(src[l]);
+        dst[block_lower<float>(k) + 1] = float(src[l + 1]);
+    }
+}
+
+hipError_t block_vals_f32(uint64_t nblk, const double* src, float* dst, hipStream_t s) {
+    if (nblk == 0) return hipSuccess;
+    const uint64_t g = (nblk + kBlock - 1) / kBlock;
+    k_block_vals_f32<<<uint32_t(g < 65536 ? g : 65536), kBlock, 0, s>>>(nblk, src, dst);
+    return hipGetLastError();
 }
 
 hipError_t block_check(uint32_t nb, const uint32_t* rp, const uint32_t* ci, uint32_t* bad,

@@ -230,7 +230,8 @@ void ensure_fp32(Batch& B) {
                                    B.dinv32.get(), B.stream.get()));
     if (B.bval.get()) {
         B.bval32.alloc(B.bval.size());
-        NLHIP_CHECK(cast_f64_f32(B.bval.size(), B.bval.get(), B.bval32.get(), B.stream.get()));
+        NLHIP_CHECK(block_vals_f32(B.bval.size() / 4, B.bval.get(), B.bval32.get(),
+                                   B.stream.get()));
     }
 }
 
@@ -590,10 +591,10 @@ void build_batch_blocks(Batch& B, const uint32_t* row_off, const uint32_t* rowpt
             uint64_t k = h_brp[Bk];
             for (uint32_t q = 0; q < len; q += 2, ++k) {
                 h_bcol[k] = p0 + (colind[b0 + q] - lo) / 2;
-                h_bval[block_upper(k)] = val[b0 + q];
-                h_bval[block_upper(k) + 1] = val[b0 + q + 1];
-                h_bval[block_lower(k)] = val[b1 + q];
-                h_bval[block_lower(k) + 1] = val[b1 + q + 1];
+                h_bval[block_upper<double>(k)] = val[b0 + q];
+                h_bval[block_upper<double>(k) + 1] = val[b0 + q + 1];
+                h_bval[block_lower<double>(k)] = val[b1 + q];
+                h_bval[block_lower<double>(k) + 1] = val[b1 + q + 1];
             }
         }
     }

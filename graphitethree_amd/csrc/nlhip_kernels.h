@@ -107,6 +107,9 @@ hipError_t block_check(uint32_t nb, const uint32_t* rp, const uint32_t* ci, uint
 template <typename T>
 hipError_t block_pack(uint32_t nb, const uint32_t* rp, const uint32_t* ci, const double* val,
                       uint32_t* brp, uint32_t* bcol, T* bval, hipStream_t s);
+// FP32 block values from the FP64 ones of the same nblk blocks (each in its
+// type's layout, block_upper / block_lower).
+hipError_t block_vals_f32(uint64_t nblk, const double* src, float* dst, hipStream_t s);
 hipError_t seg_encode(uint64_t nnz, const uint32_t* ci, const uint32_t* segb, uint16_t* ci16,
                       hipStream_t s);
 

@@ -39,15 +39,23 @@ struct CGStatus {
 };
 
 // ---- 2x2 block values ------------------------------------------------------
-// The four values of block k (a00 a01 / a10 a11: rows 2I and 2I + 1 of its
-// block row, columns 2J and 2J + 1) are stored in groups of 64 blocks: the
-// 64 upper pairs (a00, a01) of the group, then its 64 lower pairs (a10,
-// a11).  A wavefront that loads the upper (lower) pairs of 64 consecutive
-// blocks then reads one or two contiguous runs of 16-byte words (FP64)
-// rather than every other word of 2 KiB.  Offsets in values, and the array
-// length for nblk blocks (whole groups).
-constexpr uint64_t block_upper(uint64_t k) { return 256 * (k >> 6) + 2 * (k & 63); }
-constexpr uint64_t block_lower(uint64_t k) { return block_upper(k) + 128; }
+// The four values of block k: its upper pair (a00, a01: row 2I of its block
+// row, columns 2J and 2J + 1) and its lower pair (a10, a11: row 2I + 1).
+// FP64: in groups of 64 blocks, the 64 upper pairs of the group, then its
+// 64 lower pairs, so that a wavefront loading the upper (lower) pairs of 64
+// consecutive blocks reads one or two contiguous runs of 16-byte words
+// rather than every other word of 2 KiB.  FP32: block by block (a00 a01 a10
+// a11), so that one 16-byte load per lane reads a whole block and a
+// wavefront 1 KiB of consecutive words.  Offsets in values of T, and the
+// array length for nblk blocks (whole groups of 64).
+template <typename T>
+constexpr uint64_t block_upper(uint64_t k) {
+    return sizeof(T) == 8 ? 256 * (k >> 6) + 2 * (k & 63) : 4 * k;
+}
+template <typename T>
+constexpr uint64_t block_lower(uint64_t k) {
+    return block_upper<T>(k) + (sizeof(T) == 8 ? 128 : 2);
+}
 constexpr uint64_t block_vals_size(uint64_t nblk) { return 4 * ((nblk + 63) & ~uint64_t(63)); }
 
 }  // namespace nlhip
