@@ -22,6 +22,10 @@ eigen: the k smallest eigenpairs of a grid mesh's cotangent Laplacian with
   lumped mass (manifold harmonics), nlhipEigenSolve against scipy's
   shift-invert ARPACK with a SuperLU factorization on the host.
 
+mixed: the FP32-inner / FP64-refined PCG on the 2236 x 2236 LSCM system,
+  warm-started calls of a fixed number of inner iterations, until 1e-10 or
+  the FP64 solve's device time: the residual it reaches at equal time.
+
 spmv-patterns: the CG SpMV step on the 2236 x 2236 LSCM system against two
   synthetic 2x2-block systems of the same size and row length whose x
   gathers are (a) a band of 7 block columns around the diagonal (every
@@ -307,6 +311,43 @@ def run_spmv_patterns(args) -> dict:
         measure(f"lscm_grid_{args.k}, vertices in random order, no block form", rp, ci, M.data,
                 blocks=False)
     return res
+
+
+def run_mixed(args) -> dict:
+    """FP32-inner / FP64-refined PCG on the k x k LSCM grid system against a
+    time budget (the FP64 solve's time): the refinement runs in calls of
+    `chunk_iters` inner iterations, each warm-started from the previous
+    call's iterate (an outer-iteration boundary of the refinement), until it
+    reaches eps or the budget.  Reports the true residual after each call."""
+    dev = init_device()
+    G = W.lscm_grid(args.k, seed=args.seed)
+    S = capi.System.from_crs(G, fp32=True)
+    tuned = S.autotune(iters=20)
+    log(f"lscm_grid_{args.k}: n={G.n} nnz={G.nnz}; engine {tuned[0]} / {tuned[1]} workgroups")
+    x = np.zeros(G.n)
+    seconds, inner, outer, trace = 0.0, 0, 0, []
+    converged, stalled, last = False, False, float("inf")
+    while seconds < args.seconds and not converged and not stalled:
+        x, st = S.solve(G.b, x0=x, precision="mixed", eps=args.eps, max_iter=args.chunk_iters,
+                        inner_eps=args.inner_eps)
+        seconds += st.device_seconds
+        inner += int(st.iterations)
+        outer += int(st.outer_iterations)
+        converged = bool(st.converged)
+        # a call that ends where it started: its refinement undid its only
+        # correction (it did not lower the residual), and the next call would
+        # repeat it
+        stalled = not converged and not (st.rel_residual < last)
+        last = min(last, st.rel_residual)
+        trace.append({"seconds": seconds, "inner_iterations": inner,
+                      "true_rel_residual": st.rel_residual})
+        log(f"{seconds:.1f} s: {inner} inner iterations, {outer} outer, true rel res "
+            f"{st.rel_residual:.3e}, converged {converged}, stalled {stalled}")
+    return {"workload": f"lscm_grid_{args.k}", "device": dev[0], "n": G.n, "nnz": G.nnz,
+            "eps": args.eps, "inner_eps": args.inner_eps, "chunk_iterations": args.chunk_iters,
+            "budget_s": args.seconds, "seconds": seconds, "inner_iterations": inner,
+            "outer_iterations": outer, "converged": converged, "stalled": stalled,
+            "true_rel_residual": last, "trace": trace}
 
 
 def atlas_device_assembly(args, A) -> dict:
@@ -671,6 +712,14 @@ def main(argv=None):
     pa.add_argument("--seed", type=int, default=1)
     pa.add_argument("--no-shuffled", action="store_true",
                     help="skip the system with its vertices in random order")
+    mx = sub.add_parser("mixed")
+    mx.add_argument("--k", type=int, default=2236)
+    mx.add_argument("--seed", type=int, default=1)
+    mx.add_argument("--eps", type=float, default=1e-10)
+    mx.add_argument("--inner-eps", type=float, default=1e-4)
+    mx.add_argument("--chunk-iters", type=int, default=150000)
+    mx.add_argument("--seconds", type=float, default=306.0,
+                    help="device-time budget (the FP64 solve of the same system)")
     sv = sub.add_parser("serving")
     sv.add_argument("--count", type=int, default=120)
     sv.add_argument("--rate", type=float, default=10.0)
@@ -700,6 +749,8 @@ def main(argv=None):
         out = run_ssor(args)
     elif args.cmd == "spmv-patterns":
         out = run_spmv_patterns(args)
+    elif args.cmd == "mixed":
+        out = run_mixed(args)
     print(json.dumps(out), flush=True)
 
 

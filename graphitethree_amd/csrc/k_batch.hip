@@ -632,6 +632,18 @@ __global__ __launch_bounds__(kBlock) void k_b_ir_update(BatchArgs<double> a64,
     }
 }
 
+// x = x_prev on the tiles of the systems with flag[s].
+__global__ __launch_bounds__(kBlock) void k_b_restore(BatchArgs<double> a,
+                                                      const double* __restrict__ x_prev,
+                                                      const int32_t* __restrict__ flag) {
+    for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+        if (!flag[a.tile_sys[t]]) continue;
+        uint32_t r0, r1;
+        tile_rows(a, t, r0, r1);
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) a.x[i] = x_prev[i];
+    }
+}
+
 // ------------------------------------------------------------- launchers
 
 template <typename T>
@@ -715,6 +727,12 @@ hipError_t batch_residual64(const BatchArgs<double>& a, uint32_t grid, hipStream
 hipError_t batch_scale_cast(const BatchArgs<double>& a64, const BatchArgs<float>& a32,
                             const double* scale, uint32_t grid, hipStream_t s) {
     k_b_scale_cast<<<grid, kBlock, 0, s>>>(a64, a32, scale);
+    return hipGetLastError();
+}
+
+hipError_t batch_restore(const BatchArgs<double>& a, const double* x_prev, const int32_t* flag,
+                         uint32_t grid, hipStream_t s) {
+    k_b_restore<<<grid, kBlock, 0, s>>>(a, x_prev, flag);
     return hipGetLastError();
 }
 

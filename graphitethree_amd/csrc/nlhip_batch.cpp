@@ -43,6 +43,7 @@ struct Batch {
     DevBuf<double> bval;
     DevBuf<float> bval32;
     DevBuf<double> val, dinv, scale;  // scale: 2 x nsys (scale, unscale)
+    DevBuf<int32_t> flags;            // per system, host-set (refinement undo)
     DevBuf<float> val32, dinv32;
     std::vector<uint32_t> h_row_off;
     uint32_t grid = 1;
@@ -449,10 +450,33 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
     conv.assign(B.nsys, 0);
     cgrel.assign(B.nsys, 0.0);
     std::vector<int32_t> skip(B.nsys), inner_its(B.nsys), cap(B.nsys);
-    std::vector<double> sc(2 * size_t(B.nsys));
+    std::vector<int32_t> refined(B.nsys, 0), undo(B.nsys, 0), stalled(B.nsys, 0);
+    std::vector<double> sc(2 * size_t(B.nsys)), rr_prev(B.nsys, 0.0);
+    // the iterates before the last corrections (the FP64 CG's d is free here)
+    double* const x_prev = w64.d.get();
+    B.flags.ensure(B.nsys);
     for (uint32_t step = 0;; ++step) {
         std::vector<double> rel, rr, bb;
         true_residuals(B, rel, &rr, &bb);  // leaves r64 = b - A x in w64.r
+        // a system whose last correction did not lower its FP64 residual (an
+        // FP32 inner solve that lost its accuracy) gets its previous iterate
+        // back and stops refining (with the fallback: continues in FP64)
+        bool any_undo = false;
+        for (uint32_t k = 0; k < B.nsys; ++k) {
+            undo[k] = refined[k] && !(rr[k] < rr_prev[k]) ? 1 : 0;
+            if (undo[k]) {
+                rr[k] = rr_prev[k];
+                rel[k] = bb[k] > 0 ? std::sqrt(rr[k] / bb[k]) : std::sqrt(rr[k]);
+                stalled[k] = 1;
+                if (fell_back && uint32_t(its[k]) < p.max_iter) (*fell_back)[k] = 1;
+                any_undo = true;
+            }
+        }
+        if (any_undo) {
+            NLHIP_CHECK(hipMemcpyAsync(B.flags.get(), undo.data(), B.nsys * 4,
+                                       hipMemcpyHostToDevice, s));
+            NLHIP_CHECK(batch_restore(w64.args, x_prev, B.flags.get(), B.grid, s));
+        }
         uint32_t active = 0;
         for (uint32_t k = 0; k < B.nsys; ++k) {
             if (!std::isfinite(rr[k]) || !std::isfinite(bb[k]))
@@ -462,7 +486,9 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
             cgrel[k] = rel[k];
             const bool budget = uint32_t(its[k]) >= p.max_iter || outer[k] >= p.max_outer;
             const bool gone = fell_back && (*fell_back)[k];
-            skip[k] = (conv[k] || budget || gone) ? 1 : 0;
+            skip[k] = (conv[k] || budget || gone || stalled[k]) ? 1 : 0;
+            refined[k] = skip[k] ? 0 : 1;
+            rr_prev[k] = rr[k];
             // each inner solve is capped by what is left of the system's budget
             cap[k] = int32_t(std::min<uint32_t>(inner_cap, p.max_iter - std::min<uint32_t>(
                                                                       p.max_iter, its[k])));
@@ -479,6 +505,8 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
         run_batch<float>(B, w32, precond, float(p.inner_eps), int(inner_cap), K);
         NLHIP_CHECK(hipMemcpyAsync(inner_its.data(), w32.iters.get(), B.nsys * 4,
                                    hipMemcpyDeviceToHost, s));
+        NLHIP_CHECK(hipMemcpyAsync(x_prev, w64.x.get(), size_t(B.n) * 8,
+                                   hipMemcpyDeviceToDevice, s));
         NLHIP_CHECK(batch_ir_update(w64.args, w32.args, B.scale.get() + B.nsys, B.grid, s));
         B.stream.sync();
         bool progress = false;

@@ -272,11 +272,30 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_
         w32.args.skip = w32.skip.get();
         S.mscale.ensure(2 * size_t(k));
         const uint32_t inner_cap = p.inner_max_iter ? p.inner_max_iter : p.max_iter;
-        std::vector<int32_t> skip(k), inner(k);
-        std::vector<double> sc(2 * size_t(k));
+        std::vector<int32_t> skip(k), inner(k), refined(k, 0), undo(k, 0), stalled(k, 0);
+        std::vector<double> sc(2 * size_t(k)), rr_prev(k, 0.0);
+        // the iterates before the last corrections (w.d is free while refining)
+        double* const x_prev = w.d.get();
         for (;;) {
             std::vector<double> rr, bb;
             multi_norms(S, k, rr, bb);  // r64 = B - A X in w.r
+            // a column whose last correction did not lower its FP64 residual
+            // (an FP32 inner solve that lost its accuracy) gets its previous
+            // iterate back and stops refining
+            bool any_undo = false;
+            for (uint32_t j = 0; j < k; ++j) {
+                undo[j] = refined[j] && !(rr[j] < rr_prev[j]) ? 1 : 0;
+                if (undo[j]) {
+                    rr[j] = rr_prev[j];
+                    stalled[j] = 1;
+                    any_undo = true;
+                }
+            }
+            if (any_undo) {
+                NLHIP_CHECK(hipMemcpyAsync(w.again.get(), undo.data(), k * 4,
+                                           hipMemcpyHostToDevice, s));
+                NLHIP_CHECK(multi_restore(S.n, k, w.x.get(), x_prev, w.again.get(), s));
+            }
             uint32_t active = 0;
             for (uint32_t j = 0; j < k; ++j) {
                 if (!std::isfinite(rr[j]) || !std::isfinite(bb[j]))
@@ -284,7 +303,9 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_
                 conv[j] = !(rr[j] > (p.eps * p.eps) * bb[j]) ? 1 : 0;
                 cgrel[j] = bb[j] > 0 ? std::sqrt(rr[j] / bb[j]) : std::sqrt(rr[j]);
                 const bool budget = uint32_t(its[j]) >= p.max_iter || outer[j] >= p.max_outer;
-                skip[j] = (conv[j] || budget) ? 1 : 0;
+                skip[j] = (conv[j] || budget || stalled[j]) ? 1 : 0;
+                refined[j] = skip[j] ? 0 : 1;
+                rr_prev[j] = rr[j];
                 const double rn = std::sqrt(rr[j]);
                 sc[j] = skip[j] ? 0.0 : 1.0 / rn;
                 sc[k + j] = skip[j] ? 0.0 : rn;
@@ -297,6 +318,8 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_
             NLHIP_CHECK(multi_scale_cast(w.args, w32.args, S.mscale.get(), s));
             run_multi<float>(S, w32, precond, float(p.inner_eps), int(inner_cap), K);
             NLHIP_CHECK(hipMemcpyAsync(inner.data(), w32.iters.get(), k * 4, hipMemcpyDeviceToHost, s));
+            NLHIP_CHECK(hipMemcpyAsync(x_prev, w.x.get(), size_t(S.n) * k * 8,
+                                       hipMemcpyDeviceToDevice, s));
             NLHIP_CHECK(multi_ir_update(w.args, w32.args, S.mscale.get() + k, s));
             S.stream.sync();
             bool progress = false;

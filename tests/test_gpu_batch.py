@@ -165,6 +165,29 @@ def test_batch_mixed_precision(hip, small_atlas):
     assert tm.converged == 1
 
 
+def test_batch_mixed_never_returns_a_worse_iterate(hip, small_atlas):
+    """Each system's refinement undoes a correction that does not lower its
+    FP64 residual and stops: from the converged FP64 batch solution, asked
+    for an unreachable 1e-18, no system ends with a larger true residual.
+    With the fallback those systems continue in FP64 from there and stay at
+    the 1e-10 bar."""
+    A = small_atlas
+    B = capi.Batch([A.chart(c) for c in range(A.count)], fp32=True)
+    x64, p64, _ = B.solve(A.b, eps=1e-10, max_iter=5000)
+    xm, pm, _ = B.solve(A.b, x0=x64, precision="mixed", eps=1e-18, max_iter=5000,
+                        inner_eps=1e-4)
+    assert np.all(np.isfinite(xm))
+    for c in range(A.count):
+        assert pm[c].converged == 0
+        assert pm[c].rel_residual <= p64[c].rel_residual, (c, pm[c].rel_residual,
+                                                           p64[c].rel_residual)
+    xf, pf, _ = B.solve(A.b, x0=x64, precision="mixed-fallback", eps=1e-18, max_iter=5000,
+                        inner_eps=1e-4, inner_max_iter=50)
+    assert np.all(np.isfinite(xf))
+    assert all(p.converged == 0 and p.rel_residual <= 1e-10 for p in pf)
+    B.close()
+
+
 def test_batch_mixed_with_fp64_fallback(hip, small_atlas):
     """NLHIP_PREC_MIXED_FALLBACK: systems whose FP32 inner solve uses its
     whole cap finish with FP64 CG; every system reaches 1e-10 in FP64

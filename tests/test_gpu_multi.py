@@ -66,6 +66,24 @@ def test_multi_rhs_mixed_precision(hip):
     assert tot.converged == 1
 
 
+def test_multi_rhs_mixed_never_returns_a_worse_iterate(hip):
+    """Each column's refinement undoes a correction that does not lower its
+    FP64 residual and stops: from converged FP64 columns, asked for an
+    unreachable 1e-18, no column ends with a larger true residual."""
+    mesh = W.grid_mesh(50, seed=5)
+    Sw = W.cotan_system(mesh, t=1e-3, nrhs=4, rhs="gaussian", seed=2)
+    S = capi.System.from_crs(Sw, fp32=True)
+    X64, p64, _ = S.solve_multi(Sw.rhs, eps=1e-10, max_iter=5000)
+    Xm, pm, tot = S.solve_multi(Sw.rhs, X0=X64, precision="mixed", eps=1e-18, max_iter=5000,
+                                inner_eps=1e-4)
+    assert np.all(np.isfinite(Xm))
+    for j in range(4):
+        assert pm[j].converged == 0
+        assert pm[j].rel_residual <= p64[j].rel_residual, (j, pm[j].rel_residual,
+                                                           p64[j].rel_residual)
+    S.close()
+
+
 def test_multi_rhs_bad_k(hip):
     S = capi.System([0, 1], [0], [2.0])
     for k in (0, 33):

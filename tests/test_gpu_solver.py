@@ -180,6 +180,27 @@ def test_mixed_precision_parity_larger(hip, k):
     S.close()
 
 
+def test_mixed_refinement_never_returns_a_worse_iterate(hip):
+    """A refinement step whose FP32 correction does not lower the FP64
+    residual is undone and ends the refinement: started from a converged FP64
+    solution and asked for an unreachable 1e-18, the mixed solve stops
+    without converging and its true residual is no larger than the one it
+    started from.  With the FP64 fallback the solve continues in FP64 from
+    there and stays at the 1e-10 bar."""
+    Sw = W.lscm_grid(60, seed=3)
+    S = capi.System.from_crs(Sw, fp32=True)
+    x64, s64 = S.solve(Sw.b, eps=1e-10, max_iter=50000)
+    assert s64.converged == 1
+    xm, sm = S.solve(Sw.b, x0=x64, precision="mixed", eps=1e-18, max_iter=20000,
+                     inner_eps=1e-4)
+    assert sm.converged == 0 and np.all(np.isfinite(xm))
+    assert sm.rel_residual <= s64.rel_residual, (sm.rel_residual, s64.rel_residual)
+    xf, sf = S.solve(Sw.b, x0=x64, precision="mixed-fallback", eps=1e-18, max_iter=20000,
+                     inner_eps=1e-4, inner_max_iter=50)
+    assert sf.converged == 0 and np.all(np.isfinite(xf)) and sf.rel_residual <= 1e-10
+    S.close()
+
+
 def test_mixed_with_fp64_fallback(hip):
     """NLHIP_PREC_MIXED_FALLBACK on one system: an FP32 inner solve that uses
     its whole cap hands over to FP64 CG from the current iterate; a cap no
