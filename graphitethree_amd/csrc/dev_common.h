@@ -371,7 +371,8 @@ struct StreamEngine {
 // which come in pairs (2J, 2J+1) (LSCM normal equations: the u and v
 // unknowns of a vertex).  Block row I stores its blocks in order, each as
 // an upper pair (a(2I,2J), a(2I,2J+1)) and a lower pair (a(2I+1,2J),
-// a(2I+1,2J+1)) in the grouped layout of block_upper / block_lower, with
+// a(2I+1,2J+1)) in the layout of block_upper / block_lower (grouped by 64
+// blocks in FP64, block by block in FP32), with
 // the block column J as a 16-bit segment offset (Cols::ci16 / Cols::segb
 // over the block stream; 8.5 bytes per FP64 nonzero) or, with D16 = false,
 // as a 32-bit index in Cols::ci.  One block row per lane, 64 block rows per
@@ -405,7 +406,8 @@ struct BlockStreamEngine {
                                                      T (&a)[4]) {
         if constexpr (sizeof(T) == 4) {
             using V4 = T __attribute__((ext_vector_type(4)));
-            const V4 v = __builtin_nontemporal_load(reinterpret_cast<const V4*>(val + 4ull * jj));
+            const V4 v =
+                __builtin_nontemporal_load(reinterpret_cast<const V4*>(val + block_upper<T>(jj)));
             a[0] = v.x;
             a[1] = v.y;
             a[2] = v.z;

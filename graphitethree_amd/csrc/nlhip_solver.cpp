@@ -949,7 +949,8 @@ void profile_cg(System& S, bool fp32, uint32_t iters, double t[3]) {
 // candidate launch shape and keeps the fastest.  Candidates: the streamed
 // SpMV engines, which give the same row sums (the 2x2 block engine with 1,
 // 2, 4 or 8 blocks per lane per chunk, and with 4 in XCD-interleaved runs,
-// when the matrix has the block form;
+// when the matrix has the block form with 16-bit block columns, with 4 or 8
+// when its block columns are 32-bit;
 // the scalar engine with 16-bit or 32-bit columns, contiguous or
 // XCD-interleaved), times workgroup counts of 1, 2, 4 and 8 per CU (the
 // dot products are summed per workgroup, so the count fixes their order:
