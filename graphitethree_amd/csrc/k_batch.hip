@@ -632,15 +632,16 @@ __global__ __launch_bounds__(kBlock) void k_b_ir_update(BatchArgs<double> a64,
     }
 }
 
-// x = x_prev on the tiles of the systems with flag[s].
-__global__ __launch_bounds__(kBlock) void k_b_restore(BatchArgs<double> a,
-                                                      const double* __restrict__ x_prev,
-                                                      const int32_t* __restrict__ flag) {
+// dst = src on the tiles of the systems with flag[s].
+__global__ __launch_bounds__(kBlock) void k_b_copy_systems(BatchArgs<double> a,
+                                                           double* __restrict__ dst,
+                                                           const double* __restrict__ src,
+                                                           const int32_t* __restrict__ flag) {
     for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
         if (!flag[a.tile_sys[t]]) continue;
         uint32_t r0, r1;
         tile_rows(a, t, r0, r1);
-        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) a.x[i] = x_prev[i];
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) dst[i] = src[i];
     }
 }
 
@@ -730,9 +731,9 @@ hipError_t batch_scale_cast(const BatchArgs<double>& a64, const BatchArgs<float>
     return hipGetLastError();
 }
 
-hipError_t batch_restore(const BatchArgs<double>& a, const double* x_prev, const int32_t* flag,
-                         uint32_t grid, hipStream_t s) {
-    k_b_restore<<<grid, kBlock, 0, s>>>(a, x_prev, flag);
+hipError_t batch_copy_systems(const BatchArgs<double>& a, double* dst, const double* src,
+                              const int32_t* flag, uint32_t grid, hipStream_t s) {
+    k_b_copy_systems<<<grid, kBlock, 0, s>>>(a, dst, src, flag);
     return hipGetLastError();
 }
 

@@ -452,22 +452,22 @@ hipError_t multi_ir_update(const MultiArgs<double>& a64, const MultiArgs<float>&
     return hipGetLastError();
 }
 
-// X[:, j] = X_prev[:, j] for the columns j with flag[j] (interleaved n x k).
-__global__ __launch_bounds__(kBlock) void k_m_restore(uint64_t nk, uint32_t k,
-                                                      double* __restrict__ x,
-                                                      const double* __restrict__ x_prev,
-                                                      const int32_t* __restrict__ flag) {
+// dst[:, j] = src[:, j] for the columns j with flag[j] (interleaved n x k).
+__global__ __launch_bounds__(kBlock) void k_m_copy_cols(uint64_t nk, uint32_t k,
+                                                        double* __restrict__ dst,
+                                                        const double* __restrict__ src,
+                                                        const int32_t* __restrict__ flag) {
     const uint64_t stride = uint64_t(gridDim.x) * kBlock;
     for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += stride)
-        if (flag[i % k]) x[i] = x_prev[i];
+        if (flag[i % k]) dst[i] = src[i];
 }
 
-hipError_t multi_restore(uint32_t n, uint32_t k, double* x, const double* x_prev,
-                         const int32_t* flag, hipStream_t s) {
+hipError_t multi_copy_cols(uint32_t n, uint32_t k, double* dst, const double* src,
+                           const int32_t* flag, hipStream_t s) {
     const uint64_t nk = uint64_t(n) * k;
     if (nk == 0) return hipSuccess;
     const uint64_t g = (nk + kBlock - 1) / kBlock;
-    k_m_restore<<<uint32_t(g < 16384 ? g : 16384), kBlock, 0, s>>>(nk, k, x, x_prev, flag);
+    k_m_copy_cols<<<uint32_t(g < 16384 ? g : 16384), kBlock, 0, s>>>(nk, k, dst, src, flag);
     return hipGetLastError();
 }
 

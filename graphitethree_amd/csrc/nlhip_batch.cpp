@@ -450,45 +450,54 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
     conv.assign(B.nsys, 0);
     cgrel.assign(B.nsys, 0.0);
     std::vector<int32_t> skip(B.nsys), inner_its(B.nsys), cap(B.nsys);
-    std::vector<int32_t> refined(B.nsys, 0), undo(B.nsys, 0), stalled(B.nsys, 0);
-    std::vector<double> sc(2 * size_t(B.nsys)), rr_prev(B.nsys, 0.0);
-    // the iterates before the last corrections (the FP64 CG's d is free here)
-    double* const x_prev = w64.d.get();
+    std::vector<int32_t> refined(B.nsys, 0), save(B.nsys), back(B.nsys), stalled(B.nsys, 0);
+    std::vector<uint32_t> no_gain(B.nsys, 0);
+    std::vector<double> sc(2 * size_t(B.nsys)), rr_best(B.nsys, 0.0);
+    // each system's iterate with the lowest FP64 residual so far (the FP64
+    // CG's d is free here); a system's refinement ends back at it
+    // (kRefineNoGain / kRefineDiverged, as for one system), and with the
+    // fallback it continues in FP64 from there
+    double* const x_best = w64.d.get();
     B.flags.ensure(B.nsys);
-    for (uint32_t step = 0;; ++step) {
+    bool first = true, finish = false;
+    for (;;) {
         std::vector<double> rel, rr, bb;
         true_residuals(B, rel, &rr, &bb);  // leaves r64 = b - A x in w64.r
-        // a system whose last correction did not lower its FP64 residual (an
-        // FP32 inner solve that lost its accuracy) gets its previous iterate
-        // back and stops refining (with the fallback: continues in FP64)
-        bool any_undo = false;
+        if (first)
+            for (uint32_t k = 0; k < B.nsys; ++k)
+                if (!std::isfinite(rr[k]) || !std::isfinite(bb[k]))
+                    throw Error(NLHIP_ERR_NONFINITE,
+                                "system " + std::to_string(k) +
+                                    ": right-hand side or iterate is NaN or Inf");
+        uint32_t active = 0;
+        bool any_save = false, any_back = false;
         for (uint32_t k = 0; k < B.nsys; ++k) {
-            undo[k] = refined[k] && !(rr[k] < rr_prev[k]) ? 1 : 0;
-            if (undo[k]) {
-                rr[k] = rr_prev[k];
-                rel[k] = bb[k] > 0 ? std::sqrt(rr[k] / bb[k]) : std::sqrt(rr[k]);
+            save[k] = back[k] = 0;
+            if (first || (refined[k] && rr[k] < rr_best[k])) {
+                save[k] = 1;
+                rr_best[k] = rr[k];
+                no_gain[k] = 0;
+            } else if (refined[k] && (!std::isfinite(rr[k]) ||
+                                      rr[k] > kRefineDiverged * rr_best[k] ||
+                                      ++no_gain[k] >= kRefineNoGain)) {
                 stalled[k] = 1;
                 if (fell_back && uint32_t(its[k]) < p.max_iter) (*fell_back)[k] = 1;
-                any_undo = true;
             }
-        }
-        if (any_undo) {
-            NLHIP_CHECK(hipMemcpyAsync(B.flags.get(), undo.data(), B.nsys * 4,
-                                       hipMemcpyHostToDevice, s));
-            NLHIP_CHECK(batch_restore(w64.args, x_prev, B.flags.get(), B.grid, s));
-        }
-        uint32_t active = 0;
-        for (uint32_t k = 0; k < B.nsys; ++k) {
-            if (!std::isfinite(rr[k]) || !std::isfinite(bb[k]))
-                throw Error(NLHIP_ERR_NONFINITE, "system " + std::to_string(k) +
-                                                     ": right-hand side or iterate is NaN or Inf");
-            conv[k] = !(rr[k] > (p.eps * p.eps) * bb[k]) ? 1 : 0;
-            cgrel[k] = rel[k];
             const bool budget = uint32_t(its[k]) >= p.max_iter || outer[k] >= p.max_outer;
             const bool gone = fell_back && (*fell_back)[k];
-            skip[k] = (conv[k] || budget || gone || stalled[k]) ? 1 : 0;
-            refined[k] = skip[k] ? 0 : 1;
-            rr_prev[k] = rr[k];
+            const bool done = !(rr[k] > (p.eps * p.eps) * bb[k]) || budget || gone ||
+                              stalled[k] || finish;
+            if (done && !(rr[k] <= rr_best[k])) {  // back to the best iterate
+                back[k] = 1;
+                rr[k] = rr_best[k];
+                rel[k] = bb[k] > 0 ? std::sqrt(rr[k] / bb[k]) : std::sqrt(rr[k]);
+            }
+            any_save |= save[k] != 0;
+            any_back |= back[k] != 0;
+            conv[k] = !(rr[k] > (p.eps * p.eps) * bb[k]) ? 1 : 0;
+            cgrel[k] = rel[k];
+            skip[k] = done ? 1 : 0;
+            refined[k] = done ? 0 : 1;
             // each inner solve is capped by what is left of the system's budget
             cap[k] = int32_t(std::min<uint32_t>(inner_cap, p.max_iter - std::min<uint32_t>(
                                                                       p.max_iter, its[k])));
@@ -496,6 +505,17 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
             sc[k] = skip[k] ? 0.0 : 1.0 / rn;
             sc[B.nsys + k] = skip[k] ? 0.0 : rn;
             active += skip[k] ? 0 : 1;
+        }
+        first = false;
+        if (any_save) {
+            NLHIP_CHECK(hipMemcpyAsync(B.flags.get(), save.data(), B.nsys * 4,
+                                       hipMemcpyHostToDevice, s));
+            NLHIP_CHECK(batch_copy_systems(w64.args, x_best, w64.x.get(), B.flags.get(), B.grid, s));
+        }
+        if (any_back) {
+            NLHIP_CHECK(hipMemcpyAsync(B.flags.get(), back.data(), B.nsys * 4,
+                                       hipMemcpyHostToDevice, s));
+            NLHIP_CHECK(batch_copy_systems(w64.args, w64.x.get(), x_best, B.flags.get(), B.grid, s));
         }
         if (active == 0) break;
         NLHIP_CHECK(hipMemcpyAsync(w32.skip.get(), skip.data(), B.nsys * 4, hipMemcpyHostToDevice, s));
@@ -505,8 +525,6 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
         run_batch<float>(B, w32, precond, float(p.inner_eps), int(inner_cap), K);
         NLHIP_CHECK(hipMemcpyAsync(inner_its.data(), w32.iters.get(), B.nsys * 4,
                                    hipMemcpyDeviceToHost, s));
-        NLHIP_CHECK(hipMemcpyAsync(x_prev, w64.x.get(), size_t(B.n) * 8,
-                                   hipMemcpyDeviceToDevice, s));
         NLHIP_CHECK(batch_ir_update(w64.args, w32.args, B.scale.get() + B.nsys, B.grid, s));
         B.stream.sync();
         bool progress = false;
@@ -520,7 +538,7 @@ void batch_solve_mixed(Batch& B, const nlhip_solve_params& p, std::vector<int32_
             if (fell_back && inner_its[k] >= cap[k] && uint32_t(its[k]) < p.max_iter)
                 (*fell_back)[k] = 1;
         }
-        if (!progress) break;
+        finish = !progress;  // the next pass settles every system and ends
     }
     w32.args.cap = nullptr;
 }

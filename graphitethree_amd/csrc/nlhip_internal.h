@@ -376,6 +376,13 @@ struct MultiWork {
     bool graph_precond = false;
 };
 
+// Mixed-precision refinement (single, multi-RHS, batched): it ends, back at
+// the iterate with the lowest FP64 residual, after kRefineNoGain steps in a
+// row that do not lower that residual, or at once when the squared residual
+// exceeds kRefineDiverged times the lowest (or is NaN / Inf).
+constexpr uint32_t kRefineNoGain = 3;
+constexpr double kRefineDiverged = 1e4;
+
 // An FP64 PCG solve on a system's ws64 that can stop and continue (slices of
 // nlhipSolveAdvance, the confirmation of the true residual).  `cur` is the
 // host copy of the device status after the last stop; the device continues

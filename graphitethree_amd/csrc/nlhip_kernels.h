@@ -271,9 +271,9 @@ hipError_t batch_scale_cast(const BatchArgs<double>& a64, const BatchArgs<float>
 // x64 += unscale[s] * x32 where a32.skip[s] == 0.
 hipError_t batch_ir_update(const BatchArgs<double>& a64, const BatchArgs<float>& a32,
                            const double* unscale, uint32_t grid, hipStream_t s);
-// x = x_prev on the rows of the systems with flag[s] (FP64 batch).
-hipError_t batch_restore(const BatchArgs<double>& a, const double* x_prev, const int32_t* flag,
-                         uint32_t grid, hipStream_t s);
+// dst = src on the rows of the systems with flag[s] (FP64 batch vectors).
+hipError_t batch_copy_systems(const BatchArgs<double>& a, double* dst, const double* src,
+                              const int32_t* flag, uint32_t grid, hipStream_t s);
 
 // ---- multiple right-hand sides (k_spmm.hip) ---------------------------------
 
@@ -326,9 +326,9 @@ hipError_t multi_scale_cast(const MultiArgs<double>& a64, const MultiArgs<float>
                             const double* scale, hipStream_t s);
 hipError_t multi_ir_update(const MultiArgs<double>& a64, const MultiArgs<float>& a32,
                            const double* unscale, hipStream_t s);
-// X[:, j] = X_prev[:, j] (n x k interleaved) for the columns with flag[j].
-hipError_t multi_restore(uint32_t n, uint32_t k, double* x, const double* x_prev,
-                         const int32_t* flag, hipStream_t s);
+// dst[:, j] = src[:, j] (n x k interleaved) for the columns with flag[j].
+hipError_t multi_copy_cols(uint32_t n, uint32_t k, double* dst, const double* src,
+                           const int32_t* flag, hipStream_t s);
 
 // Mixed-precision refinement helpers.
 hipError_t ir_scale_cast(uint64_t n, const double* r, double s, float* out, hipStream_t st);

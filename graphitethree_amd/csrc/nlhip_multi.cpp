@@ -272,44 +272,63 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_
         w32.args.skip = w32.skip.get();
         S.mscale.ensure(2 * size_t(k));
         const uint32_t inner_cap = p.inner_max_iter ? p.inner_max_iter : p.max_iter;
-        std::vector<int32_t> skip(k), inner(k), refined(k, 0), undo(k, 0), stalled(k, 0);
-        std::vector<double> sc(2 * size_t(k)), rr_prev(k, 0.0);
-        // the iterates before the last corrections (w.d is free while refining)
-        double* const x_prev = w.d.get();
+        std::vector<int32_t> skip(k), inner(k), refined(k, 0), save(k), back(k), stalled(k, 0);
+        std::vector<uint32_t> no_gain(k, 0);
+        std::vector<double> sc(2 * size_t(k)), rr_best(k, 0.0);
+        // each column's iterate with the lowest FP64 residual so far (w.d is
+        // free while refining); the refinement of a column ends back at it
+        // (kRefineNoGain / kRefineDiverged, as for one system)
+        double* const x_best = w.d.get();
+        bool first = true, finish = false;
         for (;;) {
             std::vector<double> rr, bb;
             multi_norms(S, k, rr, bb);  // r64 = B - A X in w.r
-            // a column whose last correction did not lower its FP64 residual
-            // (an FP32 inner solve that lost its accuracy) gets its previous
-            // iterate back and stops refining
-            bool any_undo = false;
-            for (uint32_t j = 0; j < k; ++j) {
-                undo[j] = refined[j] && !(rr[j] < rr_prev[j]) ? 1 : 0;
-                if (undo[j]) {
-                    rr[j] = rr_prev[j];
-                    stalled[j] = 1;
-                    any_undo = true;
-                }
-            }
-            if (any_undo) {
-                NLHIP_CHECK(hipMemcpyAsync(w.again.get(), undo.data(), k * 4,
-                                           hipMemcpyHostToDevice, s));
-                NLHIP_CHECK(multi_restore(S.n, k, w.x.get(), x_prev, w.again.get(), s));
-            }
+            if (first)
+                for (uint32_t j = 0; j < k; ++j)
+                    if (!std::isfinite(rr[j]) || !std::isfinite(bb[j]))
+                        throw Error(NLHIP_ERR_NONFINITE,
+                                    "right-hand side or iterate is NaN or Inf");
             uint32_t active = 0;
+            bool any_save = false, any_back = false;
             for (uint32_t j = 0; j < k; ++j) {
-                if (!std::isfinite(rr[j]) || !std::isfinite(bb[j]))
-                    throw Error(NLHIP_ERR_NONFINITE, "right-hand side or iterate is NaN or Inf");
+                save[j] = back[j] = 0;
+                if (first || (refined[j] && rr[j] < rr_best[j])) {
+                    save[j] = 1;
+                    rr_best[j] = rr[j];
+                    no_gain[j] = 0;
+                } else if (refined[j] && (!std::isfinite(rr[j]) ||
+                                          rr[j] > kRefineDiverged * rr_best[j] ||
+                                          ++no_gain[j] >= kRefineNoGain)) {
+                    stalled[j] = 1;
+                }
+                const bool budget = uint32_t(its[j]) >= p.max_iter || outer[j] >= p.max_outer;
+                const bool done = !(rr[j] > (p.eps * p.eps) * bb[j]) || budget || stalled[j] ||
+                                  finish;
+                if (done && !(rr[j] <= rr_best[j])) {  // back to the best iterate
+                    back[j] = 1;
+                    rr[j] = rr_best[j];
+                }
+                any_save |= save[j] != 0;
+                any_back |= back[j] != 0;
                 conv[j] = !(rr[j] > (p.eps * p.eps) * bb[j]) ? 1 : 0;
                 cgrel[j] = bb[j] > 0 ? std::sqrt(rr[j] / bb[j]) : std::sqrt(rr[j]);
-                const bool budget = uint32_t(its[j]) >= p.max_iter || outer[j] >= p.max_outer;
-                skip[j] = (conv[j] || budget || stalled[j]) ? 1 : 0;
-                refined[j] = skip[j] ? 0 : 1;
-                rr_prev[j] = rr[j];
+                skip[j] = done ? 1 : 0;
+                refined[j] = done ? 0 : 1;
                 const double rn = std::sqrt(rr[j]);
                 sc[j] = skip[j] ? 0.0 : 1.0 / rn;
                 sc[k + j] = skip[j] ? 0.0 : rn;
                 active += skip[j] ? 0 : 1;
+            }
+            first = false;
+            if (any_save) {
+                NLHIP_CHECK(hipMemcpyAsync(w.again.get(), save.data(), k * 4,
+                                           hipMemcpyHostToDevice, s));
+                NLHIP_CHECK(multi_copy_cols(S.n, k, x_best, w.x.get(), w.again.get(), s));
+            }
+            if (any_back) {
+                NLHIP_CHECK(hipMemcpyAsync(w.again.get(), back.data(), k * 4,
+                                           hipMemcpyHostToDevice, s));
+                NLHIP_CHECK(multi_copy_cols(S.n, k, w.x.get(), x_best, w.again.get(), s));
             }
             if (active == 0) break;
             NLHIP_CHECK(hipMemcpyAsync(w32.skip.get(), skip.data(), k * 4, hipMemcpyHostToDevice, s));
@@ -318,8 +337,6 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_
             NLHIP_CHECK(multi_scale_cast(w.args, w32.args, S.mscale.get(), s));
             run_multi<float>(S, w32, precond, float(p.inner_eps), int(inner_cap), K);
             NLHIP_CHECK(hipMemcpyAsync(inner.data(), w32.iters.get(), k * 4, hipMemcpyDeviceToHost, s));
-            NLHIP_CHECK(hipMemcpyAsync(x_prev, w.x.get(), size_t(S.n) * k * 8,
-                                       hipMemcpyDeviceToDevice, s));
             NLHIP_CHECK(multi_ir_update(w.args, w32.args, S.mscale.get() + k, s));
             S.stream.sync();
             bool progress = false;
@@ -329,7 +346,7 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_
                 outer[j] += 1;
                 progress |= inner[j] > 0;
             }
-            if (!progress) break;
+            finish = !progress;  // the next pass settles every column and ends
         }
     } else {
         throw Error(NLHIP_ERR_INVALID, "unknown precision");

@@ -521,11 +521,15 @@ static void solve_fp64(System& S, const nlhip_solve_params& p, nlhip_solve_stats
 // FP32 inner PCG inside FP64 refinement.  fallback (NLHIP_PREC_MIXED_
 // FALLBACK): an inner solve that uses its whole cap (inner_max_iter) ends
 // the refinement and the solve finishes with FP64 CG from the current
-// iterate (LAPACK DSGESV's policy).  A correction that does not lower the
-// FP64 residual (an FP32 inner solve that lost its accuracy on an
-// ill-conditioned system, whose recursive residual no longer follows the
-// true one) is undone: the refinement ends with the iterate before it, and
-// with the fallback continues in FP64 from there.
+// iterate (LAPACK DSGESV's policy).  The refinement keeps the iterate with
+// the lowest FP64 residual so far: when the corrections stop improving on it
+// (kRefineNoGain steps in a row, or one residual kRefineDiverged times its
+// square, as when an FP32 inner solve on an ill-conditioned system has lost
+// the agreement between its recursive and true residuals), it goes back to
+// that iterate and ends (with the fallback: continues in FP64 from there);
+// every other exit also returns the best iterate.  Steps that raise the
+// residual for a while are kept: CG lowers the A-norm of the error, not the
+// 2-norm of the residual.
 static void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stats& st,
                         bool fallback) {
     S.ensure_fp32();
@@ -537,61 +541,62 @@ static void solve_mixed(System& S, const nlhip_solve_params& p, nlhip_solve_stat
     hipStream_t s = S.stream.get();
     const uint32_t K = p.check_every ? p.check_every : auto_check_every(S);
     const uint32_t inner_cap = p.inner_max_iter ? p.inner_max_iter : p.max_iter;
-    uint32_t total = 0, outer = 0;
-    double rr = 0, bb = 0, rr_prev = 0;
-    double* const x_prev = w64.d.get();  // the iterate before the last correction
-    bool stalled = false;
+    const size_t xbytes = size_t(S.n) * 8;
+    double* const x_best = w64.d.get();  // free while refining
+    uint32_t total = 0, outer = 0, no_gain = 0;
+    double rr = 0, bb = 0, rr_best = 0;
+    bool stalled = false, to_fp64 = false;
     for (;;) {
         residual_norms(S, w64.b.get(), w64.x.get(), w64.r.get(), rr, bb);
-        if (outer > 0 && !(rr < rr_prev)) {
-            NLHIP_CHECK(hipMemcpyAsync(w64.x.get(), x_prev, size_t(S.n) * 8,
-                                       hipMemcpyDeviceToDevice, s));
-            rr = rr_prev;
+        if (outer == 0 || rr < rr_best) {
+            NLHIP_CHECK(hipMemcpyAsync(x_best, w64.x.get(), xbytes, hipMemcpyDeviceToDevice, s));
+            rr_best = rr;
+            no_gain = 0;
+        } else if (!std::isfinite(rr) || rr > kRefineDiverged * rr_best ||
+                   ++no_gain >= kRefineNoGain) {
             stalled = true;
         }
-        if (!std::isfinite(rr) || !std::isfinite(bb))
+        if (outer == 0 && (!std::isfinite(rr) || !std::isfinite(bb)))
             throw Error(NLHIP_ERR_NONFINITE, "right-hand side or iterate is NaN or Inf");
         if (!(rr > (p.eps * p.eps) * bb)) break;
-        if (stalled && !(fallback && total < p.max_iter)) break;
+        if (stalled) {
+            to_fp64 = fallback && total < p.max_iter;
+            break;
+        }
         if (outer >= p.max_outer || total >= p.max_iter) break;
-        if (!stalled) {
-            const double rn = std::sqrt(rr);
-            NLHIP_CHECK(ir_scale_cast(S.n, w64.r.get(), 1.0 / rn, w32.b.get(), s));
-            NLHIP_CHECK(fill<float>(S.n, 0.0f, w32.x.get(), s));
-            const uint32_t cap = std::min(inner_cap, p.max_iter - total);
-            NLHIP_CHECK(
-                cg_start<float>(w32.args, w32.code, precond, float(p.inner_eps), int(cap), s));
-            const CGStatus<float> fin = run_ladder<float>(S, w32, precond, K, cap);
-            total += uint32_t(fin.iters);
-            NLHIP_CHECK(hipMemcpyAsync(x_prev, w64.x.get(), size_t(S.n) * 8,
-                                       hipMemcpyDeviceToDevice, s));
-            NLHIP_CHECK(ir_update(S.n, w64.x.get(), w32.x.get(), rn, s));
-            rr_prev = rr;
-            ++outer;
-            if (fin.iters == 0) break;
-            if (!(fallback && uint32_t(fin.iters) >= cap && total < p.max_iter)) continue;
-            // the fallback: the residual of the new iterate first (the
-            // correction may have to be undone), then FP64
+        const double rn = std::sqrt(rr);
+        NLHIP_CHECK(ir_scale_cast(S.n, w64.r.get(), 1.0 / rn, w32.b.get(), s));
+        NLHIP_CHECK(fill<float>(S.n, 0.0f, w32.x.get(), s));
+        const uint32_t cap = std::min(inner_cap, p.max_iter - total);
+        NLHIP_CHECK(cg_start<float>(w32.args, w32.code, precond, float(p.inner_eps), int(cap), s));
+        const CGStatus<float> fin = run_ladder<float>(S, w32, precond, K, cap);
+        total += uint32_t(fin.iters);
+        NLHIP_CHECK(ir_update(S.n, w64.x.get(), w32.x.get(), rn, s));
+        ++outer;
+        if (fin.iters == 0) break;  // a zero correction: x is unchanged
+        if (fallback && uint32_t(fin.iters) >= cap && total < p.max_iter) {
             residual_norms(S, w64.b.get(), w64.x.get(), w64.r.get(), rr, bb);
-            if (!(rr < rr_prev)) {
-                NLHIP_CHECK(hipMemcpyAsync(w64.x.get(), x_prev, size_t(S.n) * 8,
-                                           hipMemcpyDeviceToDevice, s));
-                rr = rr_prev;
-            }
             if (!(rr > (p.eps * p.eps) * bb)) break;
+            to_fp64 = true;
+            break;
         }
-        {
-            nlhip_solve_params p64 = p;
-            p64.max_iter = p.max_iter - total;
-            nlhip_solve_stats s64{};
-            solve_fp64(S, p64, s64, nullptr);
-            st.iterations = total + s64.iterations;
-            st.outer_iterations = outer;
-            st.cg_rel_residual = s64.cg_rel_residual;
-            st.converged = s64.converged;
-            st.extra_iterations = s64.extra_iterations;
-            return;
-        }
+    }
+    // the best iterate
+    if (!(rr <= rr_best)) {
+        NLHIP_CHECK(hipMemcpyAsync(w64.x.get(), x_best, xbytes, hipMemcpyDeviceToDevice, s));
+        rr = rr_best;
+    }
+    if (to_fp64) {
+        nlhip_solve_params p64 = p;
+        p64.max_iter = p.max_iter - total;
+        nlhip_solve_stats s64{};
+        solve_fp64(S, p64, s64, nullptr);
+        st.iterations = total + s64.iterations;
+        st.outer_iterations = outer;
+        st.cg_rel_residual = s64.cg_rel_residual;
+        st.converged = s64.converged;
+        st.extra_iterations = s64.extra_iterations;
+        return;
     }
     st.iterations = total;
     st.outer_iterations = outer;
