@@ -523,10 +523,11 @@ static void solve_fp64(System& S, const nlhip_solve_params& p, nlhip_solve_stats
 // the refinement and the solve finishes with FP64 CG from the current
 // iterate (LAPACK DSGESV's policy).  The refinement keeps the iterate with
 // the lowest FP64 residual so far: when the corrections stop improving on it
-// (kRefineNoGain steps in a row, or one residual kRefineDiverged times its
-// square, as when an FP32 inner solve on an ill-conditioned system has lost
-// the agreement between its recursive and true residuals), it goes back to
-// that iterate and ends (with the fallback: continues in FP64 from there);
+// (kRefineNoGain steps in a row, or one squared residual above
+// kRefineDiverged times the lowest, as when an FP32 inner solve on an
+// ill-conditioned system has lost the agreement between its recursive and
+// true residuals), it goes back to that iterate and ends (with the
+// fallback: continues in FP64 from there);
 // every other exit also returns the best iterate.  Steps that raise the
 // residual for a while are kept: CG lowers the A-norm of the error, not the
 // 2-norm of the residual.
