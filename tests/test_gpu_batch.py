@@ -88,6 +88,44 @@ def test_batch_mixed_forms_at_any_offset(hip, small_atlas):
     assert all(p.rel_residual <= 1e-10 for p in pm)
 
 
+def test_batch_with_32bit_block_columns(hip, small_atlas):
+    """A system whose vertex numbering is not banded (the 300 x 300 grid's
+    LSCM system in random vertex order) makes the batch keep 32-bit block
+    columns; a chart keeps its bits in that batch (= alone, with 16-bit
+    block columns), and the scrambled system matches the single-system
+    solve within 1e-8 at a true 1e-10, in FP64 and in FP32-inner refinement
+    with the FP64 fallback."""
+    import scipy.sparse as sps
+    from types import SimpleNamespace
+    G = W.lscm_grid(300, seed=2)
+    p = np.random.default_rng(11).permutation(G.n // 2)
+    q = np.empty(G.n, np.int64)
+    q[0::2], q[1::2] = 2 * p, 2 * p + 1
+    M = sps.csr_matrix((G.val, G.colind.astype(np.int64), G.rowptr.astype(np.int64)),
+                       shape=(G.n, G.n))[q][:, q].tocsr()
+    M.sort_indices()
+    P = SimpleNamespace(n=G.n, rowptr=M.indptr.astype(np.uint32),
+                        colind=M.indices.astype(np.uint32), val=M.data.copy(), b=G.b[q])
+    c = small_atlas.chart(3)
+    B = capi.Batch([c, P], fp32=True)
+    b = np.concatenate([c.b, P.b])
+    x, per, _ = B.solve(b, eps=1e-10, max_iter=50000)
+    xs = B.split(x)
+    alone = capi.Batch([c])
+    xa, _, _ = alone.solve(c.b, eps=1e-10, max_iter=50000)
+    assert np.array_equal(xa.view(np.uint64), xs[0].view(np.uint64))
+    S = capi.System(P.rowptr, P.colind, P.val)
+    xr, sr = S.solve(P.b, eps=1e-10, max_iter=50000)
+    assert per[1].converged == 1 and per[1].rel_residual <= 1e-10 and sr.converged == 1
+    assert rel(xs[1], xr) <= REL_DIFF
+    xm, pm, _ = B.solve(b, precision="mixed-fallback", eps=1e-10, max_iter=400000,
+                        inner_eps=1e-4, inner_max_iter=2000)
+    assert all(q_.rel_residual <= 1e-10 for q_ in pm)
+    assert rel(B.split(xm)[1], xr) <= REL_DIFF
+    for h in (B, alone, S):
+        h.close()
+
+
 def test_batch_persistent_launch_gives_the_same_bits(hip, small_atlas):
     """One persistent cooperative launch per run of iterations (grid-wide
     barriers between the phases) gives the bits of the captured three-launch
