@@ -53,12 +53,18 @@ NLAPI NLMatrix NLAPIENTRY nlHIPJacobiPreconditionerNewFromCRSMatrix(NLMatrix M);
 #define NLHIP_ERR_INJECTED    -8   /* failure forced by the test hook       */
 
 #define NLHIP_PREC_FP64  0   /* FP64 Jacobi-PCG                              */
-#define NLHIP_PREC_MIXED 1   /* FP32 inner PCG inside FP64 refinement        */
+#define NLHIP_PREC_MIXED 1   /* FP32 inner PCG inside FP64 refinement; the
+                                refinement keeps the iterate with the lowest
+                                FP64 residual and ends back at it after 3
+                                steps without a lower one, or when the
+                                residual norm exceeds 100x the lowest       */
 #define NLHIP_PREC_MIXED_FALLBACK 2 /* as MIXED, but a system whose FP32
                                        inner solve reaches inner_max_iter
                                        (> 0) without reaching inner_eps
                                        finishes with FP64 CG from its current
-                                       iterate (LAPACK DSGESV's policy);
+                                       iterate (LAPACK DSGESV's policy), as
+                                       does one whose refinement ends as
+                                       above (from its best iterate);
                                        single systems and batches; no
                                        counterpart in the reference */
 
