@@ -4,6 +4,8 @@
     python -m graphitethree_amd.benchmarks multirhs [--grid 1500] [--ks 3 4 8 16 32]
     python -m graphitethree_amd.benchmarks serving [--count 120] [--rate 10]
     python -m graphitethree_amd.benchmarks eigen [--grid 700] [--k 10]
+    python -m graphitethree_amd.benchmarks spmv-patterns [--k 2236]
+    python -m graphitethree_amd.benchmarks mixed [--k 2236] [--seconds 306]
 
 atlas: `count` disk-like LSCM charts (N ~ logU[50, 50000] vertices) solved
   (1) as one batch in FP64, (2) as one batch with FP32 inner PCG inside FP64
