@@ -118,6 +118,7 @@ struct Epilogue {
         if constexpr (std::is_same<Pre, NoPre>::value) fin(row, sum);
         else fin(row, sum, pv);
     }
+    static constexpr bool kPairs = false;  // see DqEpilogue
 };
 
 // fin(row, sum)
@@ -130,6 +131,39 @@ template <typename Pre, typename Fin>
 __device__ __forceinline__ Epilogue<Pre, Fin> prefetch_epi(Pre pre, Fin fin) {
     return Epilogue<Pre, Fin>{pre, fin};
 }
+
+// The CG SpMV step's epilogue: q[row] = sum, and sum * d[row] added to the
+// lane's partial of d.q (left to right, row by row).  With the 2x2 block
+// engine a lane's two rows form an aligned pair (2I, 2I + 1) on a whole
+// matrix: d is then read and q written as one 2-vector each (one contiguous
+// 1 KiB word run per wavefront instead of every other 8-byte word).
+template <typename T>
+struct DqEpilogue {
+    const T* __restrict__ d;
+    T* __restrict__ q;
+    T& pdq;
+    using PV = T;
+    using V2 = T __attribute__((ext_vector_type(2)));
+    static constexpr bool kPairs = true;
+    __device__ __forceinline__ T pre(uint32_t row) const { return d[row]; }
+    __device__ __forceinline__ void done(uint32_t row, T sum, T dr) {
+        q[row] = sum;
+        pdq += dr * sum;
+    }
+    __device__ __forceinline__ void pre2(uint32_t row, T& d0, T& d1) const {
+        const V2 v = *reinterpret_cast<const V2*>(d + row);
+        d0 = v.x;
+        d1 = v.y;
+    }
+    __device__ __forceinline__ void done2(uint32_t row, T s0, T s1, T d0, T d1) {
+        V2 v;
+        v.x = s0;
+        v.y = s1;
+        *reinterpret_cast<V2*>(q + row) = v;
+        pdq += d0 * s0;
+        pdq += d1 * s1;
+    }
+};
 
 template <typename T>
 struct Pair {
@@ -479,11 +513,30 @@ struct BlockStreamEngine {
         uint32_t base = __builtin_amdgcn_readfirstlane(__shfl(rb, 0, kWave));
         auto row_of = [&](uint32_t B) { return uint32_t(int64_t(2u * B) + delta); };
         typename Epi::PV pv0{}, pv1{};
-        if (uint32_t(lane) < nrows) {
-            pv0 = epi.pre(row_of(r0 + uint32_t(lane)));
-            pv1 = epi.pre(row_of(r0 + uint32_t(lane)) + 1u);
-        }
         T acc0 = T(0), acc1 = T(0);
+        // the epilogue operands of a lane's two rows (one 2-vector when the
+        // epilogue takes pairs and the rows are an aligned pair)
+        auto pre_pair = [&](uint32_t row) {
+            if constexpr (Epi::kPairs) {
+                if (aligned) {
+                    epi.pre2(row, pv0, pv1);
+                    return;
+                }
+            }
+            pv0 = epi.pre(row);
+            pv1 = epi.pre(row + 1u);
+        };
+        auto done_pair = [&](uint32_t row) {
+            if constexpr (Epi::kPairs) {
+                if (aligned) {
+                    epi.done2(row, acc0, acc1, pv0, pv1);
+                    return;
+                }
+            }
+            epi.done(row, acc0, pv0);
+            epi.done(row + 1u, acc1, pv1);
+        };
+        if (uint32_t(lane) < nrows) pre_pair(row_of(r0 + uint32_t(lane)));
         uint32_t c[EB];
         T a[EB][4];
         if (base < J1) load_chunk(cols, val, base, J1, lane, c, a);
@@ -527,8 +580,7 @@ struct BlockStreamEngine {
                 }
                 if (tile_end > cend) break;
                 if (uint32_t(lane) < nrows) {
-                    epi.done(row_of(r0 + uint32_t(lane)), acc0, pv0);
-                    epi.done(row_of(r0 + uint32_t(lane)) + 1u, acc1, pv1);
+                    done_pair(row_of(r0 + uint32_t(lane)));
                 }
                 acc0 = acc1 = T(0);
                 r0 += 64u;
@@ -539,8 +591,7 @@ struct BlockStreamEngine {
                 re = __shfl_down(rb, 1, kWave);
                 if (uint32_t(lane) + 1 >= nrows) re = tile_end;
                 if (uint32_t(lane) < nrows) {
-                    pv0 = epi.pre(row_of(r0 + uint32_t(lane)));
-                    pv1 = epi.pre(row_of(r0 + uint32_t(lane)) + 1u);
+                    pre_pair(row_of(r0 + uint32_t(lane)));
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -554,16 +605,14 @@ struct BlockStreamEngine {
         }
         for (;;) {
             if (uint32_t(lane) < nrows) {
-                epi.done(row_of(r0 + uint32_t(lane)), acc0, pv0);
-                epi.done(row_of(r0 + uint32_t(lane)) + 1u, acc1, pv1);
+                done_pair(row_of(r0 + uint32_t(lane)));
             }
             acc0 = acc1 = T(0);
             r0 += 64u;
             if (r0 >= R1) return;
             nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
             if (uint32_t(lane) < nrows) {
-                pv0 = epi.pre(row_of(r0 + uint32_t(lane)));
-                pv1 = epi.pre(row_of(r0 + uint32_t(lane)) + 1u);
+                pre_pair(row_of(r0 + uint32_t(lane)));
             }
         }
     }

@@ -173,11 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(CGArgs<T> a) {
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const T* __restrict__ d = a.d;
     T pdq = T(0);
-    auto epi = prefetch_epi([&](uint32_t row) { return d[row]; },
-                            [&](uint32_t row, T s, T dr) {
-                                a.q[row] = s;
-                                pdq += dr * s;
-                            });
+    DqEpilogue<T> epi{d, a.q, pdq};
     EngineOf<RPW>::type::run(a.m, a.sps, L, a.rp, a.cols, a.val, d, stage, epi);
     pdq = block_sum_all(pdq, lds);
     if (threadIdx.x == 0) a.part_dq[L] = pdq;
