@@ -127,4 +127,133 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
     }
 }
 
+// ---- stream SpMM for a few right-hand sides --------------------------
+//
+// For k <= 4 right-hand sides the row-group engine above leaves lanes idle
+// (4 lanes for k = 3) and waits for each pass's staging before it gathers.
+// This engine streams the matrix as the scalar stream engine does (one row
+// per lane, 64-row tiles, a wavefront's nonzeros as one contiguous range in
+// chunks of 64E entries, the next chunk's (colind, val) loads issued before
+// the current chunk is summed) and gathers the K values X[col*K + j] of every
+// entry.  The K products of each entry are staged in LDS and lane l sums
+// row l's products column by column, left to right with one rounding per
+// product and per add: the sums of spmm_rows, bitwise.  Columns with
+// act[j] false gather nothing (their sums are zeros, which the callers do
+// not use).  Block L owns tiles [L*sp, L*sp + sp), one contiguous quarter
+// per wavefront; `stage` holds kStage values per wavefront.
+//
+// Epilogue: pv = epi.pre(row) when the row's tile starts, then
+// epi.done(row, sums[K], pv), by the row's lane, rows in order.
+template <int K, int E, bool D16>
+struct MultiStreamEngine {
+    using Loads = StreamEngine<E, true, D16>;
+    static constexpr uint32_t kStage = 64u * uint32_t(E) * uint32_t(K);
+
+    template <typename T, typename Epi>
+    __device__ __forceinline__ static void run_rows(uint32_t R0, uint32_t R1,
+                                                    const uint32_t* __restrict__ rp,
+                                                    const Cols& cols, const T* __restrict__ val,
+                                                    const T* __restrict__ X, const bool (&act)[K],
+                                                    T* __restrict__ st, int lane, Epi& epi) {
+        constexpr uint32_t C = 64u * E;  // entries per chunk
+        const uint32_t ul = uint32_t(lane);
+        const uint32_t J1 = __builtin_amdgcn_readfirstlane(rp[R1]);
+        uint32_t r0 = R0;
+        uint32_t nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
+        uint32_t tile_end = rp[r0 + nrows];
+        uint32_t rb = ul < nrows ? rp[r0 + ul] : tile_end;
+        typename Epi::PV pv{};
+        if (ul < nrows) pv = epi.pre(r0 + ul);
+        uint32_t re = __shfl_down(rb, 1, kWave);
+        if (ul + 1 >= nrows) re = tile_end;
+        uint32_t base = __builtin_amdgcn_readfirstlane(__shfl(rb, 0, kWave));
+        T acc[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] = T(0);
+        uint32_t c[E];
+        T a[E];
+        if (base < J1) Loads::load_chunk(cols, val, base, J1, lane, c, a);
+        while (base < J1) {
+            const uint32_t craw = base + C;
+            const uint32_t cend = craw < J1 ? craw : J1;
+            T p[E][K];
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+#pragma unroll
+                for (int j = 0; j < K; ++j)
+                    p[e][j] = act[j] ? X[uint64_t(c[e]) * K + j] : T(0);
+            // every gather ahead of the prefetch (loads return in order)
+            __builtin_amdgcn_sched_barrier(0);
+            uint32_t cn[E];
+            T an[E];
+            Loads::load_chunk(cols, val, craw, J1, lane, cn, an);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const bool in = base + uint32_t(e) * 64u + ul < J1;
+#pragma unroll
+                for (int j = 0; j < K; ++j)
+                    st[uint32_t(j) * C + uint32_t(e) * 64u + ul] =
+                        in ? mul_rn(a[e], p[e][j]) : T(0);
+            }
+            __builtin_amdgcn_wave_barrier();
+            // consume this chunk; it may complete several tiles
+            for (;;) {
+                const uint32_t lo = rb > base ? rb : base;
+                const uint32_t hi = re < cend ? re : cend;
+                for (uint32_t q = lo; q < hi; ++q) {
+#pragma unroll
+                    for (int j = 0; j < K; ++j) acc[j] = add_rn(acc[j], st[uint32_t(j) * C + q - base]);
+                }
+                if (tile_end > cend) break;
+                if (ul < nrows) epi.done(r0 + ul, acc, pv);
+#pragma unroll
+                for (int j = 0; j < K; ++j) acc[j] = T(0);
+                r0 += 64u;
+                if (r0 >= R1) return;
+                nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
+                tile_end = rp[r0 + nrows];
+                rb = ul < nrows ? rp[r0 + ul] : tile_end;
+                re = __shfl_down(rb, 1, kWave);
+                if (ul + 1 >= nrows) re = tile_end;
+                if (ul < nrows) pv = epi.pre(r0 + ul);
+            }
+            __builtin_amdgcn_wave_barrier();
+            base = craw;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                c[e] = cn[e];
+                a[e] = an[e];
+            }
+        }
+        // tiles left without nonzeros (all their rows are empty)
+        for (;;) {
+            if (ul < nrows) epi.done(r0 + ul, acc, pv);
+            r0 += 64u;
+            if (r0 >= R1) return;
+            nrows = (R1 - r0) < 64u ? (R1 - r0) : 64u;
+            if (ul < nrows) pv = epi.pre(r0 + ul);
+        }
+    }
+
+    template <typename T, typename Epi>
+    __device__ __forceinline__ static void run(uint32_t m, uint32_t sp, uint32_t L,
+                                               const uint32_t* __restrict__ rp,
+                                               const Cols& cols, const T* __restrict__ val,
+                                               const T* __restrict__ X, const bool (&act)[K],
+                                               T* stage, Epi& epi) {
+        const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+        const uint64_t ntiles = (uint64_t(m) + 63) / 64;
+        uint64_t tb = uint64_t(L) * sp;
+        if (tb > ntiles) tb = ntiles;
+        const uint64_t te = tb + sp < ntiles ? tb + sp : ntiles;
+        const uint64_t q = (te - tb + kWavesPerBlock - 1) / kWavesPerBlock;
+        uint64_t t0 = tb + uint64_t(w) * q;
+        if (t0 > te) t0 = te;
+        const uint64_t t1 = t0 + q < te ? t0 + q : te;
+        if (t0 >= t1) return;
+        const uint32_t R1 = uint32_t(t1 * 64 < m ? t1 * 64 : m);
+        run_rows(uint32_t(t0 * 64), R1, rp, cols, val, X, act, stage + w * kStage, lane, epi);
+    }
+};
+
 }  // namespace nlhip

@@ -97,9 +97,10 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_m_sys_init(MultiArgs<T> a, T eps, int max_iter) {
     __shared__ T lds[kWavesPerBlock];
     const uint32_t j = blockIdx.x;
-    const T rho = col_sum(a.part_rh, a.P, a.k, j, lds);
-    const T err = col_sum(a.part_rr, a.P, a.k, j, lds);
-    const T bb = col_sum(a.part_bb, a.P, a.k, j, lds);
+    // the start's partials: one per workgroup of the SpMM-shaped launch
+    const T rho = col_sum(a.part_rh, a.Pdq, a.k, j, lds);
+    const T err = col_sum(a.part_rr, a.Pdq, a.k, j, lds);
+    const T bb = col_sum(a.part_bb, a.Pdq, a.k, j, lds);
     if (threadIdx.x == 0) {
         const T thresh = (eps * eps) * bb;
         a.bb[j] = bb;
@@ -153,12 +154,138 @@ __global__ __launch_bounds__(kBlock) void k_m_spmm(MultiArgs<T> a) {
     column_partials(pdq, k, G, lds, a.part_dq + size_t(L) * k);
 }
 
+// Wavefront entries per chunk / 64 of the stream SpMM.
+constexpr int kMultiStreamE = 4;
+
+// The stream SpMM's epilogue: q[row, :] = sums, and d[row, j] * sums[j]
+// added to column j's partial of d.q (row by row).
+template <int K, typename T>
+struct MultiDqEpilogue {
+    const T* __restrict__ d;
+    T* __restrict__ q;
+    T (&pdq)[K];
+    struct PV {
+        T v[K];
+    };
+    __device__ __forceinline__ PV pre(uint32_t row) const {
+        PV p;
+#pragma unroll
+        for (int j = 0; j < K; ++j) p.v[j] = d[uint64_t(row) * K + j];
+        return p;
+    }
+    __device__ __forceinline__ void done(uint32_t row, const T (&s)[K], const PV& pv) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            q[uint64_t(row) * K + j] = s[j];
+            pdq[j] += pv.v[j] * s[j];
+        }
+    }
+};
+
+// Per-workgroup partials of K columns: each wavefront's sum (xor butterfly),
+// then the wavefronts in index order; out[j] for j < K.  Every thread of
+// the block calls it (it synchronises the block).
+template <int K, typename T>
+__device__ __forceinline__ void stream_partials(const T (&v)[K], T* red, T* out) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();  // red of an earlier call has been read
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const T s = group_sum<kWave>(v[j]);
+        if (lane == 0) red[w * K + j] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < uint32_t(K)) {
+        T s = red[threadIdx.x];
+        for (int x = 1; x < kWavesPerBlock; ++x) s += red[x * K + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+// SpMM step for k = K <= kMultiStreamMaxK with the stream engine: Q = A D
+// for the columns still iterating, Pdq workgroups of sp tiles.
+template <int K, bool D16, typename T>
+__global__ __launch_bounds__(kBlock) void k_m_spmm_stream(MultiArgs<T> a) {
+    using Eng = MultiStreamEngine<K, kMultiStreamE, D16>;
+    __shared__ T stage[kWavesPerBlock * Eng::kStage];
+    __shared__ T red[kWavesPerBlock * K];
+    if (*a.n_active == 0) return;  // every column has stopped (block-uniform)
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    bool act[K];
+    T pdq[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        act[j] = !a.done[j];
+        pdq[j] = T(0);
+    }
+    MultiDqEpilogue<K, T> epi{a.d, a.q, pdq};
+    Eng::run(a.m, a.sp, L, a.rp, a.cols, a.val, a.d, act, stage, epi);
+    stream_partials<K>(pdq, red, a.part_dq + size_t(L) * K);
+}
+
+// The start's epilogue (k_m_init's, row by row): r = A x - b, h = D^-1 r,
+// partials of r.h, r.r and b.b per column.
+template <int K, typename T, bool PREC>
+struct MultiInitEpilogue {
+    const MultiArgs<T>& a;
+    T (&prh)[K];
+    T (&prr)[K];
+    T (&pbb)[K];
+    struct PV {
+        T b[K];
+        T dinv;
+    };
+    __device__ __forceinline__ PV pre(uint32_t row) const {
+        PV p;
+#pragma unroll
+        for (int j = 0; j < K; ++j) p.b[j] = a.b[uint64_t(row) * K + j];
+        p.dinv = PREC ? a.dinv[row] : T(1);
+        return p;
+    }
+    __device__ __forceinline__ void done(uint32_t row, const T (&s)[K], const PV& pv) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint64_t i = uint64_t(row) * K + j;
+            const T bi = pv.b[j];
+            const T ri = add_rn(s[j], mul_rn(T(-1), bi));
+            const T hi = PREC ? mul_rn(pv.dinv, ri) : ri;
+            a.r[i] = ri;
+            a.h[i] = hi;
+            prh[j] += ri * hi;
+            prr[j] += ri * ri;
+            pbb[j] += bi * bi;
+        }
+    }
+};
+
+// k_m_init with the stream engine (k = K <= kMultiStreamMaxK): Pdq
+// workgroups, Pdq partials of each kind.
+template <int K, bool D16, typename T, bool PREC>
+__global__ __launch_bounds__(kBlock) void k_m_init_stream(MultiArgs<T> a) {
+    using Eng = MultiStreamEngine<K, kMultiStreamE, D16>;
+    __shared__ T stage[kWavesPerBlock * Eng::kStage];
+    __shared__ T red[kWavesPerBlock * K];
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    bool act[K];
+    T prh[K], prr[K], pbb[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        act[j] = true;
+        prh[j] = prr[j] = pbb[j] = T(0);
+    }
+    MultiInitEpilogue<K, T, PREC> epi{a, prh, prr, pbb};
+    Eng::run(a.m, a.sp, L, a.rp, a.cols, a.val, a.x, act, stage, epi);
+    stream_partials<K>(prh, red, a.part_rh + size_t(L) * K);
+    stream_partials<K>(prr, red, a.part_rr + size_t(L) * K);
+    stream_partials<K>(pbb, red, a.part_bb + size_t(L) * K);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_m_reduce_alpha(MultiArgs<T> a) {
     __shared__ T lds[kWavesPerBlock];
     const uint32_t j = blockIdx.x;
     if (a.done[j]) return;
-    const T dq = col_sum(a.part_dq, a.P, a.k, j, lds);
+    const T dq = col_sum(a.part_dq, a.Pdq, a.k, j, lds);
     if (threadIdx.x == 0) a.alpha[j] = a.rho[j] / dq;
 }
 
@@ -284,11 +411,58 @@ __global__ __launch_bounds__(kBlock) void k_m_residual64(MultiArgs<double> a) {
     column_partials(pbb, k, G, lds, a.part_bb + size_t(L) * k);
 }
 
+// k_m_residual64's epilogue for the stream engine: r = b - A x, partials of
+// r.r and b.b per column.
+template <int K>
+struct MultiResidualEpilogue {
+    const MultiArgs<double>& a;
+    double (&prr)[K];
+    double (&pbb)[K];
+    struct PV {
+        double b[K];
+    };
+    __device__ __forceinline__ PV pre(uint32_t row) const {
+        PV p;
+#pragma unroll
+        for (int j = 0; j < K; ++j) p.b[j] = a.b[uint64_t(row) * K + j];
+        return p;
+    }
+    __device__ __forceinline__ void done(uint32_t row, const double (&s)[K], const PV& pv) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const double bi = pv.b[j];
+            const double ri = bi - s[j];
+            a.r[uint64_t(row) * K + j] = ri;
+            prr[j] += ri * ri;
+            pbb[j] += bi * bi;
+        }
+    }
+};
+
+template <int K, bool D16>
+__global__ __launch_bounds__(kBlock) void k_m_residual64_stream(MultiArgs<double> a) {
+    using Eng = MultiStreamEngine<K, kMultiStreamE, D16>;
+    __shared__ double stage[kWavesPerBlock * Eng::kStage];
+    __shared__ double red[kWavesPerBlock * K];
+    const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
+    bool act[K];
+    double prr[K], pbb[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        act[j] = true;
+        prr[j] = pbb[j] = 0.0;
+    }
+    MultiResidualEpilogue<K> epi{a, prr, pbb};
+    Eng::run(a.m, a.sp, L, a.rp, a.cols, a.val, a.x, act, stage, epi);
+    stream_partials<K>(prr, red, a.part_rr + size_t(L) * K);
+    stream_partials<K>(pbb, red, a.part_bb + size_t(L) * K);
+}
+
 __global__ __launch_bounds__(kBlock) void k_m_norms64(MultiArgs<double> a) {
     __shared__ double lds[kWavesPerBlock];
     const uint32_t j = blockIdx.x;
-    const double rr = col_sum(a.part_rr, a.P, a.k, j, lds);
-    const double bb = col_sum(a.part_bb, a.P, a.k, j, lds);
+    const double rr = col_sum(a.part_rr, a.Pdq, a.k, j, lds);
+    const double bb = col_sum(a.part_bb, a.Pdq, a.k, j, lds);
     if (threadIdx.x == 0) {
         a.err[j] = rr;
         a.bb[j] = bb;
@@ -342,6 +516,21 @@ __global__ __launch_bounds__(kBlock) void k_m_ir_update(MultiArgs<double> a64,
         default: return hipErrorInvalidValue;            \
     }
 
+// K = k (1 .. kMultiStreamMaxK) and D16 as constants for the stream kernels.
+#define NLHIP_KD_SWITCH(K_, D16_, CALL)                                                   \
+    switch (K_) {                                                                          \
+        case 1: { constexpr int K = 1; if (D16_) { constexpr bool D16 = true; CALL; }      \
+                  else { constexpr bool D16 = false; CALL; } } break;                      \
+        case 2: { constexpr int K = 2; if (D16_) { constexpr bool D16 = true; CALL; }      \
+                  else { constexpr bool D16 = false; CALL; } } break;                      \
+        case 3: { constexpr int K = 3; if (D16_) { constexpr bool D16 = true; CALL; }      \
+                  else { constexpr bool D16 = false; CALL; } } break;                      \
+        case 4: { constexpr int K = 4; if (D16_) { constexpr bool D16 = true; CALL; }      \
+                  else { constexpr bool D16 = false; CALL; } } break;                      \
+        default: return hipErrorInvalidValue;                                              \
+    }
+static_assert(kMultiStreamMaxK == 4, "one case per k up to kMultiStreamMaxK");
+
 static uint32_t ew_grid_m(uint64_t n) {
     uint64_t g = (n + kBlock - 1) / kBlock;
     return uint32_t(g > 4096 ? 4096 : (g == 0 ? 1 : g));
@@ -379,9 +568,45 @@ bool spmm_one_chunk(uint32_t m, const uint32_t* rp, uint32_t G, hipStream_t s) {
     return h <= kSpmmChunk;
 }
 
+namespace {
+// The stream kernels that share the Pdq grid: SpMM step, start, and (FP64)
+// the residual.
+template <typename T>
+hipError_t stream_kernels(uint32_t k, bool d16, const void* (&f)[3]) {
+    NLHIP_KD_SWITCH(k, d16, (f[0] = reinterpret_cast<const void*>(&k_m_spmm_stream<K, D16, T>),
+                             f[1] = reinterpret_cast<const void*>(&k_m_init_stream<K, D16, T, true>),
+                             f[2] = std::is_same<T, double>::value
+                                        ? reinterpret_cast<const void*>(&k_m_residual64_stream<K, D16>)
+                                        : nullptr));
+    return hipSuccess;
+}
+}  // namespace
+
+template <typename T>
+uint32_t multi_stream_per_cu(uint32_t k, bool d16) {
+    const void* f[3] = {nullptr, nullptr, nullptr};
+    if (stream_kernels<T>(k, d16, f) != hipSuccess) return 1;
+    int best = 1 << 30;
+    for (const void* g : f) {
+        if (!g) continue;
+        int per = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, g, kBlock, 0) != hipSuccess || per < 1)
+            return 1;
+        best = std::min(best, per);
+    }
+    return uint32_t(best);
+}
+template uint32_t multi_stream_per_cu<double>(uint32_t, bool);
+template uint32_t multi_stream_per_cu<float>(uint32_t, bool);
+
 template <typename T>
 hipError_t multi_start(const MultiArgs<T>& a, bool precond, T eps, int max_iter, hipStream_t s) {
-    if (precond && a.one_chunk) {
+    const bool d16 = a.cols.ci16 != nullptr;
+    if (a.sp && precond) {
+        NLHIP_KD_SWITCH(a.k, d16, (k_m_init_stream<K, D16, T, true><<<a.Pdq, kBlock, 0, s>>>(a)));
+    } else if (a.sp) {
+        NLHIP_KD_SWITCH(a.k, d16, (k_m_init_stream<K, D16, T, false><<<a.Pdq, kBlock, 0, s>>>(a)));
+    } else if (precond && a.one_chunk) {
         NLHIP_G_SWITCH(a.G, (k_m_init<G, true, T, true><<<a.P, kBlock, 0, s>>>(a)));
     } else if (precond) {
         NLHIP_G_SWITCH(a.G, (k_m_init<G, false, T, true><<<a.P, kBlock, 0, s>>>(a)));
@@ -399,7 +624,10 @@ template <typename T>
 hipError_t multi_iteration(const MultiArgs<T>& a, bool precond, int max_iter, hipStream_t s) {
     const uint32_t cg = a.k;  // one workgroup per column for the reductions
     k_m_dir<T><<<a.P, kBlock, 0, s>>>(a);
-    if (a.one_chunk) {
+    if (a.sp) {
+        NLHIP_KD_SWITCH(a.k, a.cols.ci16 != nullptr,
+                        (k_m_spmm_stream<K, D16, T><<<a.Pdq, kBlock, 0, s>>>(a)));
+    } else if (a.one_chunk) {
         NLHIP_G_SWITCH(a.G, (k_m_spmm<G, true, T><<<a.P, kBlock, 0, s>>>(a)));
     } else {
         NLHIP_G_SWITCH(a.G, (k_m_spmm<G, false, T><<<a.P, kBlock, 0, s>>>(a)));
@@ -431,7 +659,10 @@ hipError_t multi_reopen(const MultiArgs<double>& a, const int32_t* again, hipStr
 }
 
 hipError_t multi_residual64(const MultiArgs<double>& a, hipStream_t s) {
-    if (a.one_chunk) {
+    if (a.sp) {
+        NLHIP_KD_SWITCH(a.k, a.cols.ci16 != nullptr,
+                        (k_m_residual64_stream<K, D16><<<a.Pdq, kBlock, 0, s>>>(a)));
+    } else if (a.one_chunk) {
         NLHIP_G_SWITCH(a.G, (k_m_residual64<G, true><<<a.P, kBlock, 0, s>>>(a)));
     } else {
         NLHIP_G_SWITCH(a.G, (k_m_residual64<G, false><<<a.P, kBlock, 0, s>>>(a)));

@@ -309,7 +309,22 @@ struct MultiArgs {
     int32_t* done;
     const int32_t* skip;      // may be null
     uint32_t* n_active;       // columns still iterating
+    // SpMM step: with sp > 0 (k <= kMultiStreamMaxK) the stream engine, Pdq
+    // workgroups of sp 64-row tiles each, reading the column indices as
+    // `cols` (16-bit offsets when cols.ci16 is set); with sp = 0 the
+    // row-group engine on P workgroups.  Pdq partials of d.q per column.
+    Cols cols;
+    uint32_t sp;
+    uint32_t Pdq;
 };
+
+// Right-hand sides up to which the multi-RHS SpMM step streams the matrix.
+constexpr uint32_t kMultiStreamMaxK = 4;
+// Workgroups of the stream kernels (SpMM step, start, residual; k right-hand
+// sides, 16-bit column offsets or not) that one CU holds at once, the
+// smallest over those kernels.
+template <typename T>
+uint32_t multi_stream_per_cu(uint32_t k, bool d16);
 
 template <typename T>
 hipError_t multi_start(const MultiArgs<T>& a, bool precond, T eps, int max_iter, hipStream_t s);

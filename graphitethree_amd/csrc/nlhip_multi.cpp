@@ -68,6 +68,21 @@ void multi_alloc(System& S, MultiWork<T>& w, uint32_t k) {
     a.one_chunk = spmm_one_chunk(S.n, S.rowptr.get(), a.G, S.stream.get()) ? 1u : 0u;
     a.rp = S.rowptr.get();
     a.ci = S.colind.get();
+    a.cols = S.cols();
+    a.Pdq = P;
+    if (k <= kMultiStreamMaxK) {
+        // stream SpMM: the 64-row tiles dealt to as many workgroups as the
+        // device holds at once (no second, partly filled round), a multiple
+        // of 4 tiles (one quarter per wavefront) to each, at most P workgroups
+        const uint64_t ntiles = (uint64_t(S.n) + 63) / 64;
+        const uint64_t resident = uint64_t(std::max(1, device().cu_count)) *
+                                  multi_stream_per_cu<T>(k, a.cols.ci16 != nullptr);
+        const uint64_t want = std::min<uint64_t>({P, resident, (ntiles + 3) / 4});
+        uint64_t sp = (ntiles + want - 1) / want;
+        sp = (sp + 3) / 4 * 4;
+        a.sp = uint32_t(sp);
+        a.Pdq = uint32_t((ntiles + sp - 1) / sp);
+    }
     a.x = w.x.get();
     a.b = w.b.get();
     a.r = w.r.get();

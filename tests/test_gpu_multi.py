@@ -22,7 +22,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("k", [1, 3, 5, 8, 17, 32])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 8, 17, 32])
 def test_multi_rhs_matches_oracle_columns(hip, k):
     Sw = W.lscm_grid(18, seed=3)
     rng = np.random.default_rng(k)
@@ -36,6 +36,45 @@ def test_multi_rhs_matches_oracle_columns(hip, k):
         assert per[j].converged == 1 and per[j].rel_residual <= 1e-10
         assert rel(X[j], xr) <= REL_DIFF, j
     assert tot.iterations == max(p.iterations for p in per)
+
+
+@pytest.mark.parametrize("offsets", [True, False])
+def test_multi_rhs_ragged_rows_both_column_formats(hip, offsets):
+    """k <= 4 streams the matrix (one row per lane, 64-row tiles): ragged
+    rows of 1 to about 80 entries, a run of diagonal-only rows longer than
+    two tiles, and the column indices as 16-bit offsets or as 32-bit
+    indices."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    lens = rng.integers(0, 41, n)
+    lens[700:850] = 0          # rows 700..849: the diagonal only
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        c = np.unique(rng.integers(0, n, lens[i]))
+        c = c[(c != i) & ((c < 700) | (c >= 850))]
+        w = rng.uniform(0.1, 1.0, len(c))
+        rows += [i] * len(c)
+        cols += list(c)
+        vals += list(-w)
+    import scipy.sparse as sp
+    M = sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+    M = M + M.T
+    d = np.asarray(abs(M).sum(axis=1)).ravel() + 1.0
+    M = (M + sp.diags(d)).tocsr()
+    M.sort_indices()
+    rp = M.indptr.astype(np.uint32)
+    ci = M.indices.astype(np.uint32)
+    va = M.data.astype(np.float64)
+    S = capi.System(rp, ci, va, offsets=offsets)
+    A = O.CRS(rp, ci, va)
+    for k in (3, 4):
+        B = rng.standard_normal((k, n))
+        X, per, tot = S.solve_multi(B, eps=1e-10, max_iter=2000)
+        for j in range(k):
+            xr, it, _ = O.pcg(A, B[j], eps=1e-10, max_iter=2000)
+            assert per[j].converged == 1 and per[j].rel_residual <= 1e-10, (k, j)
+            assert same_iters(per[j].iterations - per[j].extra_iterations, it), (k, j)
+            assert rel(X[j], xr) <= REL_DIFF, (k, j)
 
 
 def test_multi_rhs_smoothing_xyz(hip):
