@@ -138,8 +138,7 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
 // entry.  The K products of each entry are staged in LDS and lane l sums
 // row l's products column by column, left to right with one rounding per
 // product and per add: the sums of spmm_rows, bitwise.  Columns with
-// act[j] false gather nothing (their sums are zeros, which the callers do
-// not use).  Block L owns tiles [L*sp, L*sp + sp), one contiguous quarter
+// act[j] false sum zeros (their sums are not used).  Block L owns tiles [L*sp, L*sp + sp), one contiguous quarter
 // per wavefront; `stage` holds kStage values per wavefront.
 //
 // Epilogue: pv = epi.pre(row) when the row's tile starts, then
@@ -176,12 +175,13 @@ struct MultiStreamEngine {
         while (base < J1) {
             const uint32_t craw = base + C;
             const uint32_t cend = craw < J1 ? craw : J1;
+            // each entry's K values as one copy: the widest loads the
+            // alignment allows (k = 3 FP64: 16 + 8 bytes; FP32: 12 bytes),
+            // so the gather instructions per entry are 1-2 instead of K
             T p[E][K];
 #pragma unroll
             for (int e = 0; e < E; ++e)
-#pragma unroll
-                for (int j = 0; j < K; ++j)
-                    p[e][j] = act[j] ? X[uint64_t(c[e]) * K + j] : T(0);
+                __builtin_memcpy(p[e], X + uint64_t(c[e]) * K, sizeof(T) * K);
             // every gather ahead of the prefetch (loads return in order)
             __builtin_amdgcn_sched_barrier(0);
             uint32_t cn[E];
@@ -193,7 +193,7 @@ struct MultiStreamEngine {
 #pragma unroll
                 for (int j = 0; j < K; ++j)
                     st[uint32_t(j) * C + uint32_t(e) * 64u + ul] =
-                        in ? mul_rn(a[e], p[e][j]) : T(0);
+                        (in && act[j]) ? mul_rn(a[e], p[e][j]) : T(0);
             }
             __builtin_amdgcn_wave_barrier();
             // consume this chunk; it may complete several tiles
