@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kBlock) void k_m_spmm(MultiArgs<T> a) {
     column_partials(pdq, k, G, lds, a.part_dq + size_t(L) * k);
 }
 
-// Wavefront entries per chunk / 64 of the stream SpMM.
+// Wavefront entries per chunk / 64 of the stream kernels.
 constexpr int kMultiStreamE = 4;
 
 // The stream SpMM's epilogue: q[row, :] = sums, and d[row, j] * sums[j]
@@ -569,32 +569,24 @@ bool spmm_one_chunk(uint32_t m, const uint32_t* rp, uint32_t G, hipStream_t s) {
 }
 
 namespace {
-// The stream kernels that share the Pdq grid: SpMM step, start, and (FP64)
-// the residual.
 template <typename T>
-hipError_t stream_kernels(uint32_t k, bool d16, const void* (&f)[3]) {
-    NLHIP_KD_SWITCH(k, d16, (f[0] = reinterpret_cast<const void*>(&k_m_spmm_stream<K, D16, T>),
-                             f[1] = reinterpret_cast<const void*>(&k_m_init_stream<K, D16, T, true>),
-                             f[2] = std::is_same<T, double>::value
-                                        ? reinterpret_cast<const void*>(&k_m_residual64_stream<K, D16>)
-                                        : nullptr));
+hipError_t stream_spmm_kernel(uint32_t k, bool d16, const void*& f) {
+    NLHIP_KD_SWITCH(k, d16, (f = reinterpret_cast<const void*>(&k_m_spmm_stream<K, D16, T>)));
     return hipSuccess;
 }
 }  // namespace
 
+// Sized for the SpMM step, the launch of every iteration; the start and the
+// residual (once per solve, more registers for their epilogues at large k)
+// may hold fewer workgroups per CU and then run a second, partial round.
 template <typename T>
 uint32_t multi_stream_per_cu(uint32_t k, bool d16) {
-    const void* f[3] = {nullptr, nullptr, nullptr};
-    if (stream_kernels<T>(k, d16, f) != hipSuccess) return 1;
-    int best = 1 << 30;
-    for (const void* g : f) {
-        if (!g) continue;
-        int per = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, g, kBlock, 0) != hipSuccess || per < 1)
-            return 1;
-        best = std::min(best, per);
-    }
-    return uint32_t(best);
+    const void* f = nullptr;
+    int per = 0;
+    if (stream_spmm_kernel<T>(k, d16, f) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, kBlock, 0) != hipSuccess || per < 1)
+        return 1;
+    return uint32_t(per);
 }
 template uint32_t multi_stream_per_cu<double>(uint32_t, bool);
 template uint32_t multi_stream_per_cu<float>(uint32_t, bool);

@@ -320,9 +320,8 @@ struct MultiArgs {
 
 // Right-hand sides up to which the multi-RHS SpMM step streams the matrix.
 constexpr uint32_t kMultiStreamMaxK = 4;
-// Workgroups of the stream kernels (SpMM step, start, residual; k right-hand
-// sides, 16-bit column offsets or not) that one CU holds at once, the
-// smallest over those kernels.
+// Workgroups of the stream SpMM step (k right-hand sides, 16-bit column
+// offsets or not) that one CU holds at once.
 template <typename T>
 uint32_t multi_stream_per_cu(uint32_t k, bool d16);
 

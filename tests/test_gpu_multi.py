@@ -22,7 +22,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 8, 17, 32])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7, 8, 9, 17, 32])
 def test_multi_rhs_matches_oracle_columns(hip, k):
     Sw = W.lscm_grid(18, seed=3)
     rng = np.random.default_rng(k)
@@ -40,10 +40,10 @@ def test_multi_rhs_matches_oracle_columns(hip, k):
 
 @pytest.mark.parametrize("offsets", [True, False])
 def test_multi_rhs_ragged_rows_both_column_formats(hip, offsets):
-    """k <= 4 streams the matrix (one row per lane, 64-row tiles): ragged
-    rows of 1 to about 80 entries, a run of diagonal-only rows longer than
-    two tiles, and the column indices as 16-bit offsets or as 32-bit
-    indices."""
+    """k <= 4 streams the matrix (one row per lane, 64-row tiles; k = 7
+    takes the row-group engine): ragged rows of 1 to about 80 entries, a run
+    of diagonal-only rows longer than two tiles, and the column indices as
+    16-bit offsets or as 32-bit indices."""
     rng = np.random.default_rng(11)
     n = 3000
     lens = rng.integers(0, 41, n)
@@ -67,7 +67,7 @@ def test_multi_rhs_ragged_rows_both_column_formats(hip, offsets):
     va = M.data.astype(np.float64)
     S = capi.System(rp, ci, va, offsets=offsets)
     A = O.CRS(rp, ci, va)
-    for k in (3, 4):
+    for k in (3, 4, 7):
         B = rng.standard_normal((k, n))
         X, per, tot = S.solve_multi(B, eps=1e-10, max_iter=2000)
         for j in range(k):
