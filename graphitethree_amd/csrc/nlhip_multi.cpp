@@ -153,10 +153,16 @@ void run_multi(System& S, MultiWork<T>& w, bool precond, T eps, int max_iter, ui
     multi_loop<T>(S, w, precond, max_iter, K, uint64_t(std::max(max_iter, 0)));
 }
 
+// Iterations per captured graph: about 0.25 ms of device work, 3 to 16.
+// The loop checks a graph's stop flag after queuing the next graph, so a
+// solve that stops inside graph i still runs graph i + 1 as launches that
+// exit at once (about 10 us per iteration), while each graph costs about
+// 9 us more than its launches; multi-RHS solves often stop after a few
+// iterations (mesh smoothing: 5 to 60), where short graphs waste least.
 uint32_t multi_check_every(const System& S, uint32_t k) {
     const double bytes = 12.0 * double(S.nnz) + 108.0 * double(S.n) * k;
     const double iter_s = std::max(bytes / 4.0e12, 2.0e-5);
-    return uint32_t(std::clamp(2.0e-3 / iter_s, 4.0, 64.0));
+    return uint32_t(std::clamp(2.5e-4 / iter_s, 3.0, 16.0));
 }
 
 // FP64 |b_j - A x_j|^2 and |b_j|^2 for every column into rr, bb.

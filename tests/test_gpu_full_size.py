@@ -125,7 +125,7 @@ def test_multi_rhs_at_full_size(hip):
     """The multi-RHS stress system ((I + 1e-3 L) on a 1500 x 1500 grid, 32
     N(0,1) right-hand sides): the block solve, FP64 and FP32-inner, reaches
     1e-10 on every column by the oracle's SpMV and agrees with one FP64
-    solve per column."""
+    solve per column; so does the block solve of the first 3 columns."""
     mesh = W.grid_mesh(1500, seed=1)
     Sw = W.cotan_system(mesh, t=1e-3, nrhs=32, rhs="gaussian", seed=1)
     del mesh
@@ -142,4 +142,12 @@ def test_multi_rhs_at_full_size(hip):
         if j % 8 == 0:
             xj, _ = S.solve(b, eps=1e-10, max_iter=5000)
             assert np.linalg.norm(X[j] - xj) / np.linalg.norm(xj) <= 1e-8, j
+    # k = 3 (mesh smoothing's x, y, z) takes the stream engine
+    X3, per3, _ = S.solve_multi(Sw.rhs[:3], eps=1e-10, max_iter=5000)
+    for j in range(3):
+        b = Sw.rhs[j]
+        r = b - O.spmv(A, X3[j], parallel=True)
+        assert per3[j].converged == 1 and np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b), j
+        xj, _ = S.solve(b, eps=1e-10, max_iter=5000)
+        assert np.linalg.norm(X3[j] - xj) / np.linalg.norm(xj) <= 1e-8, j
     S.close()

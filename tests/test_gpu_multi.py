@@ -102,6 +102,8 @@ def test_multi_rhs_mixed_precision(hip):
         r = O.spmv(A, X[j]) - Sw.rhs[j]
         assert np.linalg.norm(r) / np.linalg.norm(Sw.rhs[j]) <= 1e-10
         assert per[j].converged == 1 and per[j].outer_iterations >= 1
+        xr, _, _ = O.pcg(A, Sw.rhs[j], eps=1e-10, max_iter=1000)
+        assert rel(X[j], xr) <= REL_DIFF, j
     assert tot.converged == 1
 
 
