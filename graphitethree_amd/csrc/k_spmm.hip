@@ -34,6 +34,15 @@ __device__ __forceinline__ void block_rows(const uint32_t m, const uint32_t rows
     r1 = (uint64_t(r0) + rows_per_block < m) ? r0 + rows_per_block : m;
 }
 
+// Whether any column still iterates (its done flag, written by an earlier
+// launch, is clear); the same answer in every thread.
+template <typename T>
+__device__ __forceinline__ bool any_active(const MultiArgs<T>& a) {
+    for (uint32_t j = 0; j < a.k; ++j)
+        if (!a.done[j]) return true;
+    return false;
+}
+
 // Per-workgroup partial sums of k columns: lds[kBlock] scratch; the group
 // lanes holding column j combine through LDS in a fixed order.
 template <typename T>
@@ -117,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_m_sys_init(MultiArgs<T> a, T eps, in
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_m_dir(MultiArgs<T> a) {
-    if (*a.n_active == 0) return;  // every column has stopped
+    if (!any_active(a)) return;  // every column has stopped
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t k = a.k, G = a.G;
     const uint32_t g = threadIdx.x / G, j = threadIdx.x % G;
@@ -141,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_m_dir(MultiArgs<T> a) {
 template <int G, bool ONE, typename T>
 __global__ __launch_bounds__(kBlock) void k_m_spmm(MultiArgs<T> a) {
     __shared__ T lds[kBlock];
-    if (*a.n_active == 0) return;  // every column has stopped (block-uniform)
+    if (!any_active(a)) return;  // every column has stopped (block-uniform)
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t k = a.k;
     T pdq = T(0);
@@ -209,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_m_spmm_stream(MultiArgs<T> a) {
     using Eng = MultiStreamEngine<K, kMultiStreamE, D16>;
     __shared__ T stage[kWavesPerBlock * Eng::kStage];
     __shared__ T red[kWavesPerBlock * K];
-    if (*a.n_active == 0) return;  // every column has stopped (block-uniform)
+    if (!any_active(a)) return;  // every column has stopped (block-uniform)
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     bool act[K];
     T pdq[K];
@@ -292,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_m_reduce_alpha(MultiArgs<T> a) {
 template <typename T, bool PREC>
 __global__ __launch_bounds__(kBlock) void k_m_update(MultiArgs<T> a) {
     __shared__ T lds[kBlock];
-    if (*a.n_active == 0) return;  // every column has stopped (block-uniform)
+    if (!any_active(a)) return;  // every column has stopped (block-uniform)
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t k = a.k;
     // element-wise over the same rows as the SpMM block, G lanes per row
@@ -344,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_m_reduce_beta(MultiArgs<T> a, int ma
 // stopping launch skipped (rho_new from the r.h partials, beta = rho_new /
 // rho, rho = rho_new) and done = 0.  The next launches continue those
 // columns exactly as if their threshold had been the lower one from the
-// start.  k_m_count follows.
+// start.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_m_reopen(MultiArgs<T> a, const int32_t* again) {
     __shared__ T lds[kWavesPerBlock];
@@ -358,15 +367,6 @@ __global__ __launch_bounds__(kBlock) void k_m_reopen(MultiArgs<T> a, const int32
     }
 }
 
-// active column count -> *a.n_active (single thread, after reduce_beta)
-template <typename T>
-__global__ void k_m_count(MultiArgs<T> a) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        uint32_t act = 0;
-        for (uint32_t c = 0; c < a.k; ++c) act += a.done[c] ? 0u : 1u;
-        *a.n_active = act;
-    }
-}
 
 // ---------------------------------------------------- layout / refinement
 
@@ -608,7 +608,6 @@ hipError_t multi_start(const MultiArgs<T>& a, bool precond, T eps, int max_iter,
         NLHIP_G_SWITCH(a.G, (k_m_init<G, false, T, false><<<a.P, kBlock, 0, s>>>(a)));
     }
     k_m_sys_init<T><<<a.k, kBlock, 0, s>>>(a, eps, max_iter);
-    k_m_count<T><<<1, 64, 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -628,7 +627,6 @@ hipError_t multi_iteration(const MultiArgs<T>& a, bool precond, int max_iter, hi
     if (precond) k_m_update<T, true><<<a.P, kBlock, 0, s>>>(a);
     else k_m_update<T, false><<<a.P, kBlock, 0, s>>>(a);
     k_m_reduce_beta<T><<<cg, kBlock, 0, s>>>(a, max_iter);
-    k_m_count<T><<<1, 64, 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -646,7 +644,6 @@ hipError_t multi_layout(uint32_t n, uint32_t k, const T* src, T* dst, bool to_in
 
 hipError_t multi_reopen(const MultiArgs<double>& a, const int32_t* again, hipStream_t s) {
     k_m_reopen<double><<<a.k, kBlock, 0, s>>>(a, again);
-    k_m_count<double><<<1, 64, 0, s>>>(a);
     return hipGetLastError();
 }
 

@@ -367,8 +367,7 @@ struct MultiWork {
     DevBuf<T> x, b, r, h, d, q, part, col;  // part: 4 x P x k; col: 6 x k
     DevBuf<int32_t> iters, done, skip, again;
     DevBuf<T> chk;                          // 2 x k: |b - A x|^2, |b|^2 (confirm)
-    DevBuf<uint32_t> n_active;
-    PinnedBuf<uint32_t> host;               // two polling slots
+    PinnedBuf<int32_t> host;                // two polling slots of 32 done flags
     GraphLadder graph;
     MultiArgs<T> args{};
     MultiArgs<T> graph_args{};

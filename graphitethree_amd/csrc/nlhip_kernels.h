@@ -308,7 +308,6 @@ struct MultiArgs {
     int32_t* iters;
     int32_t* done;
     const int32_t* skip;      // may be null
-    uint32_t* n_active;       // columns still iterating
     // SpMM step: with sp > 0 (k <= kMultiStreamMaxK) the stream engine, Pdq
     // workgroups of sp 64-row tiles each, reading the column indices as
     // `cols` (16-bit offsets when cols.ci16 is set); with sp = 0 the

@@ -53,8 +53,7 @@ void multi_alloc(System& S, MultiWork<T>& w, uint32_t k) {
     w.skip.alloc(k);
     w.again.alloc(k);
     w.chk.alloc(2 * size_t(k));
-    w.n_active.alloc(1);
-    w.host.ensure(2);
+    w.host.ensure(2 * 32);
     MultiArgs<T>& a = w.args;
     a = MultiArgs<T>{};
     a.m = S.n;
@@ -102,7 +101,6 @@ void multi_alloc(System& S, MultiWork<T>& w, uint32_t k) {
     a.iters = w.iters.get();
     a.done = w.done.get();
     a.skip = nullptr;
-    a.n_active = w.n_active.get();
 }
 
 // Runs the started (or re-opened) recurrences until every column has
@@ -122,25 +120,32 @@ void multi_loop(System& S, MultiWork<T>& w, bool precond, int max_iter, uint32_t
     w.graph_max_iter = max_iter;
     w.graph_args = w.args;
     Event ev[2];
-    uint32_t* host = w.host.get();
+    int32_t* host = w.host.get();
+    const uint32_t k = w.k;
+    // every column stopped, by the done flags copied to polling slot i
+    auto stopped = [&](int i) {
+        for (uint32_t j = 0; j < k; ++j)
+            if (!host[32 * i + j]) return false;
+        return true;
+    };
     const uint64_t limit = budget + 3ull * w.graph.cap() + 8;
     uint64_t launched = 0, covered = 0;
     for (;;) {
         w.graph.launch(launched, s);
         covered += w.graph.iters(launched);
         const int slot = int(launched & 1);
-        NLHIP_CHECK(hipMemcpyAsync(&host[slot], w.n_active.get(), sizeof(uint32_t),
+        NLHIP_CHECK(hipMemcpyAsync(&host[32 * slot], w.done.get(), k * sizeof(int32_t),
                                    hipMemcpyDeviceToHost, s));
         ev[slot].record(s);
         ++launched;
         if (launched >= 2) {
             const int prev = int((launched - 2) & 1);
             ev[prev].sync();
-            if (host[prev] == 0) break;
+            if (stopped(prev)) break;
         }
         if (covered > limit) {
             S.stream.sync();
-            if (host[slot] == 0) break;
+            if (stopped(slot)) break;
             throw Error(NLHIP_ERR_HIP, "multi right-hand-side CG did not reach its stopping test");
         }
     }
