@@ -5,16 +5,20 @@
 //
 // Layout: vectors are n x k row-major ("interleaved"), element (i, j) at
 // i*k + j, so the gather of x[col] for all right-hand sides is one
-// contiguous k*8-byte segment.  A row is handled by a group of G lanes
-// (G = k rounded up to a power of two), lane j of the group owning right-hand
-// side j; the group walks the row's entries left to right, U at a time, so
-// (val, colind) are loaded once per group and each lane sums its row in
-// column order with one rounding per product and per add.
+// contiguous k*8-byte segment.  The SpMM-shaped launches (SpMM step, start,
+// residual) sum each row in column order with one rounding per product and
+// per add, with one of two engines (dev_spmm.h), same sums:
+// * k <= kMultiStreamMaxK: the stream engine, one row per lane, the matrix
+//   streamed in 64-row tiles and each entry's k values gathered as one copy;
+//   Pdq workgroups of sp tiles, as many as the device holds at once;
+// * larger k: G-lane row groups (G = k rounded up to a power of two, lane j
+//   owning right-hand side j), (val, colind) staged in LDS once per pass.
 //
-// Per iteration: direction D = beta_j D + H; SpMM Q = A D with per-workgroup
-// partials of d_j.q_j; a reduction launch (alpha_j); update X, R, H with
-// partials of r_j.h_j and r_j.r_j; a reduction launch (stopping test, beta_j).
-// Columns whose stopping test held are left untouched.
+// Per iteration, 5 launches: direction D = beta_j D + H; SpMM Q = A D with
+// per-workgroup partials of d_j.q_j; a reduction launch (alpha_j); update
+// X, R, H with partials of r_j.h_j and r_j.r_j; a reduction launch
+// (stopping test, beta_j).  Columns whose stopping test held are left
+// untouched; once every column has stopped, the launches exit at once.
 #include <algorithm>
 
 #include "dev_common.h"
