@@ -125,6 +125,40 @@ def test_multi_rhs_mixed_never_returns_a_worse_iterate(hip):
     S.close()
 
 
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_multi_rhs_tiny_systems(hip, k):
+    """Systems smaller than one 64-row tile: one workgroup, three of its
+    wavefronts without rows."""
+    rng = np.random.default_rng(k)
+    # 1 x 1: x = b / a
+    S = capi.System(np.array([0, 1], np.uint32), np.array([0], np.uint32), np.array([2.0]))
+    B = rng.standard_normal((k, 1))
+    X, per, tot = S.solve_multi(B, eps=1e-10, max_iter=10)
+    assert rel(X[:, 0], B[:, 0] / 2.0) <= 1e-15, X
+    assert all(p.converged == 1 for p in per)
+    S.close()
+    # 5 x 5 tridiagonal (2, -1): every column against the oracle
+    n = 5
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j, v in ((i - 1, -1.0), (i, 2.0), (i + 1, -1.0)):
+            if 0 <= j < n:
+                rows.append(i); cols.append(j); vals.append(v)
+    rp = np.searchsorted(rows, np.arange(n + 1)).astype(np.uint32)
+    ci = np.array(cols, np.uint32)
+    va = np.array(vals)
+    S = capi.System(rp, ci, va)
+    A = O.CRS(rp, ci, va)
+    B = rng.standard_normal((k, n))
+    X, per, tot = S.solve_multi(B, eps=1e-10, max_iter=100)
+    for j in range(k):
+        xr, it, _ = O.pcg(A, B[j], eps=1e-10, max_iter=100)
+        assert per[j].converged == 1, j
+        assert same_iters(per[j].iterations - per[j].extra_iterations, it), j
+        assert rel(X[j], xr) <= REL_DIFF, j
+    S.close()
+
+
 def test_multi_rhs_bad_k(hip):
     S = capi.System([0, 1], [0], [2.0])
     for k in (0, 33):
