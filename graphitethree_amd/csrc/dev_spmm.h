@@ -25,7 +25,8 @@ constexpr uint32_t spmm_pass_rows() {
 // the epilogue needs that do not depend on the sum) and then
 // fin(row, j, sum, pre value), row by row in the same order whatever the
 // path.  Every thread of the block must call this (it synchronises the
-// block).
+// block).  zero_x: X is all zeros; the rows' sums are +0 (what the sums
+// would give) and the matrix is not read.
 //
 // ONE: every pass's entries fit in one LDS chunk (the caller checked it,
 // spmm_one_chunk): a group finishes each row before the next, holding
@@ -37,7 +38,8 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
                                           uint32_t L, const uint32_t* __restrict__ rp,
                                           const uint32_t* __restrict__ ci,
                                           const T* __restrict__ val, const T* __restrict__ X,
-                                          Pre& pre, Fin& fin, const int32_t* done = nullptr) {
+                                          Pre& pre, Fin& fin, const int32_t* done = nullptr,
+                                          bool zero_x = false) {
     __shared__ uint32_t s_ci[kSpmmChunk];
     __shared__ T s_val[kSpmmChunk];
     constexpr uint32_t groups = kBlock / G;
@@ -74,6 +76,14 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
         const uint32_t a = uint32_t(a64);
         const uint32_t pe = (a + pass < r1) ? a + pass : r1;
         const uint32_t J0 = rp[a], J1 = rp[pe];
+        if (zero_x) {  // X = 0: every sum is +0 (block-uniform)
+            for (uint32_t t = 0; t < R; ++t) {
+                const uint32_t row = a + t * groups + g;
+                if (row >= pe) break;
+                if (j < k) fin(row, j, T(0), pre(row, j));
+            }
+            continue;
+        }
         if constexpr (ONE) {
             __syncthreads();  // the previous pass has been consumed
             for (uint32_t q = J0 + threadIdx.x; q < J1; q += kBlock) {
@@ -142,7 +152,9 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
 // per wavefront; `stage` holds kStage values per wavefront.
 //
 // Epilogue: pv = epi.pre(row) when the row's tile starts, then
-// epi.done(row, sums[K], pv), by the row's lane, rows in order.
+// epi.done(row, sums[K], pv), by the row's lane, rows in order.  zero_x: X
+// is all zeros; the sums are +0 (what the sums would give) and the matrix
+// is not read.
 template <int K, int E, bool D16>
 struct MultiStreamEngine {
     using Loads = StreamEngine<E, true, D16>;
@@ -240,7 +252,7 @@ struct MultiStreamEngine {
                                                const uint32_t* __restrict__ rp,
                                                const Cols& cols, const T* __restrict__ val,
                                                const T* __restrict__ X, const bool (&act)[K],
-                                               T* stage, Epi& epi) {
+                                               T* stage, Epi& epi, bool zero_x = false) {
         const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
         const uint64_t ntiles = (uint64_t(m) + 63) / 64;
         uint64_t tb = uint64_t(L) * sp;
@@ -252,6 +264,19 @@ struct MultiStreamEngine {
         const uint64_t t1 = t0 + q < te ? t0 + q : te;
         if (t0 >= t1) return;
         const uint32_t R1 = uint32_t(t1 * 64 < m ? t1 * 64 : m);
+        if (zero_x) {  // X = 0: the same epilogue calls, in order, with sums of +0
+            T z[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) z[j] = T(0);
+            for (uint32_t r0 = uint32_t(t0 * 64); r0 < R1; r0 += 64u) {
+                const uint32_t row = r0 + uint32_t(lane);
+                if (row < R1) {
+                    const typename Epi::PV pv = epi.pre(row);
+                    epi.done(row, z, pv);
+                }
+            }
+            return;
+        }
         run_rows(uint32_t(t0 * 64), R1, rp, cols, val, X, act, stage + w * kStage, lane, epi);
     }
 };

@@ -82,7 +82,9 @@ __global__ __launch_bounds__(kBlock) void k_m_init(MultiArgs<T> a) {
         prr += ri * ri;
         pbb += bi * bi;
     };
-    spmm_rows<G, ONE>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.x, pre, fin);
+    const bool zero_x = a.x_nonzero && *a.x_nonzero == 0;  // X = 0: A X = 0 (block-uniform)
+    spmm_rows<G, ONE>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.x, pre, fin, nullptr,
+                      zero_x);
     column_partials(prh, k, G, lds, a.part_rh + size_t(L) * k);
     column_partials(prr, k, G, lds, a.part_rr + size_t(L) * k);
     column_partials(pbb, k, G, lds, a.part_bb + size_t(L) * k);
@@ -287,7 +289,8 @@ __global__ __launch_bounds__(kBlock) void k_m_init_stream(MultiArgs<T> a) {
         prh[j] = prr[j] = pbb[j] = T(0);
     }
     MultiInitEpilogue<K, T, PREC> epi{a, prh, prr, pbb};
-    Eng::run(a.m, a.sp, L, a.rp, a.cols, a.val, a.x, act, stage, epi);
+    const bool zero_x = a.x_nonzero && *a.x_nonzero == 0;  // X = 0: A X = 0 (block-uniform)
+    Eng::run(a.m, a.sp, L, a.rp, a.cols, a.val, a.x, act, stage, epi, zero_x);
     stream_partials<K>(prh, red, a.part_rh + size_t(L) * K);
     stream_partials<K>(prr, red, a.part_rr + size_t(L) * K);
     stream_partials<K>(pbb, red, a.part_bb + size_t(L) * K);
@@ -379,11 +382,17 @@ __global__ __launch_bounds__(kBlock) void k_m_reopen(MultiArgs<T> a, const int32
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_to_interleaved(uint32_t n, uint32_t k, uint32_t G,
                                                            const T* __restrict__ cm,
-                                                           T* __restrict__ il) {
+                                                           T* __restrict__ il,
+                                                           int32_t* nonzero) {
     const uint32_t g = threadIdx.x / G, j = threadIdx.x % G, per = kBlock / G;
     if (j >= k) return;
-    for (uint64_t i = uint64_t(blockIdx.x) * per + g; i < n; i += uint64_t(gridDim.x) * per)
-        il[i * k + j] = cm[uint64_t(j) * n + i];
+    bool nz = false;
+    for (uint64_t i = uint64_t(blockIdx.x) * per + g; i < n; i += uint64_t(gridDim.x) * per) {
+        const T v = cm[uint64_t(j) * n + i];
+        il[i * k + j] = v;
+        nz |= !(v == T(0));
+    }
+    if (nonzero && nz) *nonzero = 1;  // every writer stores 1
 }
 
 template <typename T>
@@ -636,12 +645,12 @@ hipError_t multi_iteration(const MultiArgs<T>& a, bool precond, int max_iter, hi
 
 template <typename T>
 hipError_t multi_layout(uint32_t n, uint32_t k, const T* src, T* dst, bool to_interleaved,
-                        hipStream_t s) {
+                        hipStream_t s, int32_t* nonzero) {
     if (k == 0 || k > 32) return hipErrorInvalidValue;
     uint32_t G = 1;
     while (G < k) G <<= 1;
     const uint32_t g = ew_grid_m(uint64_t(n) * G);
-    if (to_interleaved) k_to_interleaved<T><<<g, kBlock, 0, s>>>(n, k, G, src, dst);
+    if (to_interleaved) k_to_interleaved<T><<<g, kBlock, 0, s>>>(n, k, G, src, dst, nonzero);
     else k_to_colmajor<T><<<g, kBlock, 0, s>>>(n, k, G, src, dst);
     return hipGetLastError();
 }
@@ -700,6 +709,6 @@ template hipError_t multi_start<float>(const MultiArgs<float>&, bool, float, int
 template hipError_t multi_iteration<double>(const MultiArgs<double>&, bool, int, hipStream_t);
 template hipError_t multi_iteration<float>(const MultiArgs<float>&, bool, int, hipStream_t);
 template hipError_t multi_layout<double>(uint32_t, uint32_t, const double*, double*, bool,
-                                         hipStream_t);
+                                         hipStream_t, int32_t*);
 
 }  // namespace nlhip

@@ -366,6 +366,7 @@ struct MultiWork {
     uint32_t n = 0, k = 0, P = 0;
     DevBuf<T> x, b, r, h, d, q, part, col;  // part: 4 x P x k; col: 6 x k
     DevBuf<int32_t> iters, done, skip, again;
+    DevBuf<int32_t> xnz;                    // X has a nonzero (args.x_nonzero)
     DevBuf<T> chk;                          // 2 x k: |b - A x|^2, |b|^2 (confirm)
     PinnedBuf<int32_t> host;                // two polling slots of 32 done flags
     GraphLadder graph;

@@ -308,6 +308,10 @@ struct MultiArgs {
     int32_t* iters;
     int32_t* done;
     const int32_t* skip;      // may be null
+    // null, or a flag that reads 0 when X is all zeros: the start then takes
+    // every row sum of A X as +0 without reading the matrix (the sum of
+    // products with zeros, from +0, is +0 whatever their signs)
+    const int32_t* x_nonzero;
     // SpMM step: with sp > 0 (k <= kMultiStreamMaxK) the stream engine, Pdq
     // workgroups of sp 64-row tiles each, reading the column indices as
     // `cols` (16-bit offsets when cols.ci16 is set); with sp = 0 the
@@ -328,9 +332,11 @@ template <typename T>
 hipError_t multi_start(const MultiArgs<T>& a, bool precond, T eps, int max_iter, hipStream_t s);
 template <typename T>
 hipError_t multi_iteration(const MultiArgs<T>& a, bool precond, int max_iter, hipStream_t s);
+// nonzero (to_interleaved only, may be null): set to 1 when a value moved
+// is not zero.
 template <typename T>
 hipError_t multi_layout(uint32_t n, uint32_t k, const T* src, T* dst, bool to_interleaved,
-                        hipStream_t s);
+                        hipStream_t s, int32_t* nonzero = nullptr);
 // Re-opens the FP64 columns flagged in again[k] (done -> 0 with the beta /
 // rho update their stopping launch skipped), then recounts the active ones.
 hipError_t multi_reopen(const MultiArgs<double>& a, const int32_t* again, hipStream_t s);

@@ -54,6 +54,10 @@ void multi_alloc(System& S, MultiWork<T>& w, uint32_t k) {
     w.again.alloc(k);
     w.chk.alloc(2 * size_t(k));
     w.host.ensure(2 * 32);
+    // 0 until a transposed X has a nonzero; the FP32 inner solves, whose X is
+    // cleared before every start, never set it
+    w.xnz.alloc(1);
+    NLHIP_CHECK(hipMemsetAsync(w.xnz.get(), 0, sizeof(int32_t), S.stream.get()));
     MultiArgs<T>& a = w.args;
     a = MultiArgs<T>{};
     a.m = S.n;
@@ -101,6 +105,7 @@ void multi_alloc(System& S, MultiWork<T>& w, uint32_t k) {
     a.iters = w.iters.get();
     a.done = w.done.get();
     a.skip = nullptr;
+    a.x_nonzero = w.xnz.get();
 }
 
 // Runs the started (or re-opened) recurrences until every column has
@@ -221,7 +226,8 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_
     copy_in(w.q.get(), B, nk * 8, s);
     NLHIP_CHECK(multi_layout<double>(S.n, k, w.q.get(), w.b.get(), true, s));
     copy_in(w.q.get(), X, nk * 8, s);
-    NLHIP_CHECK(multi_layout<double>(S.n, k, w.q.get(), w.x.get(), true, s));
+    NLHIP_CHECK(hipMemsetAsync(w.xnz.get(), 0, sizeof(int32_t), s));
+    NLHIP_CHECK(multi_layout<double>(S.n, k, w.q.get(), w.x.get(), true, s, w.xnz.get()));
     const bool precond = p.precond == NL_PRECOND_JACOBI;
     const uint32_t K = p.check_every ? p.check_every : multi_check_every(S, k);
     Event e0, e1;

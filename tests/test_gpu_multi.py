@@ -77,6 +77,26 @@ def test_multi_rhs_ragged_rows_both_column_formats(hip, offsets):
             assert rel(X[j], xr) <= REL_DIFF, (k, j)
 
 
+@pytest.mark.parametrize("k", [3, 8])
+def test_multi_rhs_warm_start(hip, k):
+    """From a nonzero X0 the start forms B - A X0 (from X0 = 0 it skips the
+    product, whose sums are all +0): every column follows the oracle's PCG
+    from the same X0."""
+    Sw = W.lscm_grid(18, seed=3)
+    rng = np.random.default_rng(100 + k)
+    B = rng.standard_normal((k, Sw.n))
+    X0 = rng.standard_normal((k, Sw.n))
+    S = capi.System.from_crs(Sw)
+    X, per, tot = S.solve_multi(B, X0=X0, eps=1e-10, max_iter=5000)
+    A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
+    for j in range(k):
+        xr, it, _ = O.pcg(A, B[j], x0=X0[j], eps=1e-10, max_iter=5000)
+        assert per[j].converged == 1, j
+        assert same_iters(per[j].iterations - per[j].extra_iterations, it), (j, per[j].iterations, it)
+        assert rel(X[j], xr) <= REL_DIFF, j
+    S.close()
+
+
 def test_multi_rhs_smoothing_xyz(hip):
     mesh = W.grid_mesh(60, seed=4)
     Sw = W.cotan_system(mesh, t=1e-3, nrhs=3, rhs="xyz")
