@@ -369,6 +369,7 @@ struct MultiWork {
     DevBuf<int32_t> xnz;                    // X has a nonzero (args.x_nonzero)
     DevBuf<T> chk;                          // 2 x k: |b - A x|^2, |b|^2 (confirm)
     PinnedBuf<int32_t> host;                // two polling slots of 32 done flags
+    PinnedBuf<int32_t> iters_host;          // k iteration counts (refinement)
     GraphLadder graph;
     MultiArgs<T> args{};
     MultiArgs<T> graph_args{};

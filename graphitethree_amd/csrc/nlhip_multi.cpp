@@ -54,6 +54,7 @@ void multi_alloc(System& S, MultiWork<T>& w, uint32_t k) {
     w.again.alloc(k);
     w.chk.alloc(2 * size_t(k));
     w.host.ensure(2 * 32);
+    w.iters_host.ensure(32);
     // 0 until a transposed X has a nonzero; the FP32 inner solves, whose X is
     // cleared before every start, never set it
     w.xnz.alloc(1);
@@ -304,17 +305,30 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_
         w32.args.skip = w32.skip.get();
         S.mscale.ensure(2 * size_t(k));
         const uint32_t inner_cap = p.inner_max_iter ? p.inner_max_iter : p.max_iter;
-        std::vector<int32_t> skip(k), inner(k), refined(k, 0), save(k), back(k), stalled(k, 0);
+        std::vector<int32_t> skip(k), refined(k, 0), save(k), back(k), stalled(k, 0);
         std::vector<uint32_t> no_gain(k, 0);
         std::vector<double> sc(2 * size_t(k)), rr_best(k, 0.0);
         // each column's iterate with the lowest FP64 residual so far (w.d is
         // free while refining); the refinement of a column ends back at it
         // (kRefineNoGain / kRefineDiverged, as for one system)
         double* const x_best = w.d.get();
-        bool first = true, finish = false;
+        int32_t* const inner_host = w32.iters_host.get();
+        bool first = true, finish = false, pending = false;
         for (;;) {
             std::vector<double> rr, bb;
-            multi_norms(S, k, rr, bb);  // r64 = B - A X in w.r
+            multi_norms(S, k, rr, bb);  // r64 = B - A X in w.r (waits for the stream)
+            if (pending) {
+                // the previous step's inner iterations, copied behind it
+                bool progress = false;
+                for (uint32_t j = 0; j < k; ++j) {
+                    if (skip[j]) continue;
+                    its[j] += inner_host[j];
+                    outer[j] += 1;
+                    progress |= inner_host[j] > 0;
+                }
+                finish = !progress;  // this pass settles every column and ends
+                pending = false;
+            }
             if (first)
                 for (uint32_t j = 0; j < k; ++j)
                     if (!std::isfinite(rr[j]) || !std::isfinite(bb[j]))
@@ -368,17 +382,9 @@ void solve_multi(System& S, uint32_t k, const double* B, double* X, const nlhip_
                                        hipMemcpyHostToDevice, s));
             NLHIP_CHECK(multi_scale_cast(w.args, w32.args, S.mscale.get(), s));
             run_multi<float>(S, w32, precond, float(p.inner_eps), int(inner_cap), K);
-            NLHIP_CHECK(hipMemcpyAsync(inner.data(), w32.iters.get(), k * 4, hipMemcpyDeviceToHost, s));
+            NLHIP_CHECK(hipMemcpyAsync(inner_host, w32.iters.get(), k * 4, hipMemcpyDeviceToHost, s));
             NLHIP_CHECK(multi_ir_update(w.args, w32.args, S.mscale.get() + k, s));
-            S.stream.sync();
-            bool progress = false;
-            for (uint32_t j = 0; j < k; ++j) {
-                if (skip[j]) continue;
-                its[j] += inner[j];
-                outer[j] += 1;
-                progress |= inner[j] > 0;
-            }
-            finish = !progress;  // the next pass settles every column and ends
+            pending = true;  // counted after the next residual, behind which it lands
         }
     } else {
         throw Error(NLHIP_ERR_INVALID, "unknown precision");
