@@ -33,20 +33,30 @@ constexpr uint32_t spmm_pass_rows() {
 // one accumulator and issuing the row's epilogue loads with its gathers.
 // Otherwise passes are staged in chunks and the group keeps one accumulator
 // per row of the pass until the pass ends.
-template <int G, bool ONE, typename T, typename Pre, typename Fin>
-__device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_per_block,
-                                          uint32_t L, const uint32_t* __restrict__ rp,
-                                          const uint32_t* __restrict__ ci,
-                                          const T* __restrict__ val, const T* __restrict__ X,
-                                          Pre& pre, Fin& fin, const int32_t* done = nullptr,
-                                          bool zero_x = false) {
+// V columns per lane (V = 1 or 2): lane j of a group owns columns
+// j*V .. j*V + V - 1, gathers them as one copy per entry (16 bytes for two
+// FP64 columns; k * V-aligned when V = 2 and k is even) and sums each of
+// them as above; pre / fin are called per column, columns in order.  G
+// lanes per row cover G * V >= k columns.
+template <int G, int V, bool ONE, typename T, typename Pre, typename Fin>
+__device__ __forceinline__ void spmm_rows_v(uint32_t m, uint32_t k, uint32_t rows_per_block,
+                                            uint32_t L, const uint32_t* __restrict__ rp,
+                                            const uint32_t* __restrict__ ci,
+                                            const T* __restrict__ val, const T* __restrict__ X,
+                                            Pre& pre, Fin& fin, const int32_t* done = nullptr,
+                                            bool zero_x = false) {
+    static_assert(V == 1 || V == 2, "one or two columns per lane");
     __shared__ uint32_t s_ci[kSpmmChunk];
     __shared__ T s_val[kSpmmChunk];
     constexpr uint32_t groups = kBlock / G;
     constexpr uint32_t pass = spmm_pass_rows<G>();
-    const uint32_t g = threadIdx.x / G, j = threadIdx.x % G;
+    const uint32_t g = threadIdx.x / G, jb = (threadIdx.x % G) * V;
+    const bool mine = jb < k;  // the lane owns at least one column
     // columns that have stopped gather nothing (their sums are not used)
-    const bool active_col = j < k && !(done && done[j]);
+    bool active_col = false;
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+        active_col |= jb + v < k && !(done && done[jb + v]);
     // the blocks that run on one XCD take the passes of their joint row range
     // in turn, so at any time they work on neighbouring rows and the X rows
     // those gather (a stencil's rows i - w .. i + w) are shared in that
@@ -60,17 +70,36 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
     const uint32_t step = (hi - lo) * pass;
     constexpr uint32_t R = pass / groups;  // rows per group in one pass
     // U gathers of entries [p, hi) (LDS offsets), added in order to acc
-    auto gather = [&](uint32_t p, uint32_t hi, T& acc) {
-        T av[kSpmmU], xv[kSpmmU];
+    auto gather = [&](uint32_t p, uint32_t hi, T (&acc)[V]) {
+        T av[kSpmmU], xv[kSpmmU][V];
 #pragma unroll
         for (int u = 0; u < kSpmmU; ++u) {
             const uint32_t q = p + u < hi ? p + u : hi - 1u;
             av[u] = s_val[q];
-            xv[u] = active_col ? X[uint64_t(s_ci[q]) * k + j] : T(0);
+            if (active_col) {
+                __builtin_memcpy(xv[u], X + uint64_t(s_ci[q]) * k + jb, sizeof(T) * V);
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) xv[u][v] = T(0);
+            }
         }
 #pragma unroll
         for (int u = 0; u < kSpmmU; ++u)
-            if (p + u < hi) acc = add_rn(acc, mul_rn(av[u], xv[u]));
+            if (p + u < hi) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = add_rn(acc[v], mul_rn(av[u], xv[u][v]));
+            }
+    };
+    using PV = decltype(pre(0u, 0u));
+    auto finish = [&](uint32_t row, const T (&acc)[V], const PV (&pv)[V]) {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            if (jb + v < k) fin(row, jb + v, acc[v], pv[v]);
+    };
+    auto fetch = [&](uint32_t row, PV (&pv)[V]) {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            if (jb + v < k) pv[v] = pre(row, jb + v);
     };
     for (uint64_t a64 = uint64_t(r0) + first; a64 < r1; a64 += step) {
         const uint32_t a = uint32_t(a64);
@@ -80,7 +109,13 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
             for (uint32_t t = 0; t < R; ++t) {
                 const uint32_t row = a + t * groups + g;
                 if (row >= pe) break;
-                if (j < k) fin(row, j, T(0), pre(row, j));
+                if (!mine) continue;
+                PV pv[V]{};
+                fetch(row, pv);
+                T z[V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) z[v] = T(0);
+                finish(row, z, pv);
             }
             continue;
         }
@@ -95,15 +130,17 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
                 const uint32_t row = a + t * groups + g;
                 if (row >= pe) break;
                 const uint32_t lo = rp[row] - J0, hi = rp[row + 1] - J0;
-                decltype(pre(row, j)) pv{};
-                if (j < k) pv = pre(row, j);
-                T acc = T(0);
+                PV pv[V]{};
+                if (mine) fetch(row, pv);
+                T acc[V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = T(0);
                 for (uint32_t p = lo; p < hi; p += kSpmmU) gather(p, hi, acc);
-                if (j < k) fin(row, j, acc, pv);
+                if (mine) finish(row, acc, pv);
             }
         } else {
             uint32_t rb[R], re[R];
-            T acc[R];
+            T acc[R][V];
 #pragma unroll
             for (uint32_t t = 0; t < R; ++t) {
                 const uint32_t row = a + t * groups + g;
@@ -112,7 +149,8 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
                     rb[t] = rp[row];
                     re[t] = rp[row + 1];
                 }
-                acc[t] = T(0);
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[t][v] = T(0);
             }
             for (uint32_t c0 = J0; c0 < J1; c0 += kSpmmChunk) {
                 const uint32_t c1 = (c0 + kSpmmChunk < J1) ? c0 + kSpmmChunk : J1;
@@ -131,10 +169,26 @@ __device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_
 #pragma unroll
             for (uint32_t t = 0; t < R; ++t) {
                 const uint32_t row = a + t * groups + g;
-                if (j < k && row < pe) fin(row, j, acc[t], pre(row, j));
+                if (mine && row < pe) {
+                    PV pv[V]{};
+                    fetch(row, pv);
+                    finish(row, acc[t], pv);
+                }
             }
         }
     }
+}
+
+// One column per lane (spmm_rows_v with V = 1): lane j of a group owns
+// column j.
+template <int G, bool ONE, typename T, typename Pre, typename Fin>
+__device__ __forceinline__ void spmm_rows(uint32_t m, uint32_t k, uint32_t rows_per_block,
+                                          uint32_t L, const uint32_t* __restrict__ rp,
+                                          const uint32_t* __restrict__ ci,
+                                          const T* __restrict__ val, const T* __restrict__ X,
+                                          Pre& pre, Fin& fin, const int32_t* done = nullptr,
+                                          bool zero_x = false) {
+    spmm_rows_v<G, 1, ONE>(m, k, rows_per_block, L, rp, ci, val, X, pre, fin, done, zero_x);
 }
 
 // ---- stream SpMM for a few right-hand sides --------------------------

@@ -25,7 +25,8 @@ __global__ __launch_bounds__(kBlock) void k_eig_spmm(uint32_t n, uint32_t rpb, c
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     auto pre = [](uint32_t, uint32_t) { return 0; };
     auto fin = [&](uint32_t row, uint32_t j, double s, int) { AY[size_t(row) * P + j] = s; };
-    spmm_rows<P, ONE>(n, P, rpb, L, rp, ci, val, Y, pre, fin);
+    // two columns per lane: P / 2 lanes per row, one 16-byte gather per entry
+    spmm_rows_v<P / 2, 2, ONE>(n, P, rpb, L, rp, ci, val, Y, pre, fin);
 }
 
 // One step of the Chebyshev recurrence on T = M^-1 A:
@@ -52,7 +53,8 @@ __global__ __launch_bounds__(kBlock) void k_eig_cheb(uint32_t n, uint32_t rpb, c
         if (Xold) v = add_rn(v, -mul_rn(s2, pv.xold));
         Ynew[size_t(row) * P + j] = v;
     };
-    spmm_rows<P, ONE>(n, P, rpb, L, rp, ci, val, Y, pre, fin);
+    // two columns per lane: P / 2 lanes per row, one 16-byte gather per entry
+    spmm_rows_v<P / 2, 2, ONE>(n, P, rpb, L, rp, ci, val, Y, pre, fin);
 }
 
 // Per workgroup b: part[b][i][j] = sum over its rows r (in order) of
