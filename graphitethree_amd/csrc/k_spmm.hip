@@ -62,12 +62,44 @@ __device__ __forceinline__ void column_partials(T v, uint32_t k, uint32_t G, T* 
     }
 }
 
-template <int G, bool ONE, typename T, bool PREC>
+// The same for V columns per lane (lane l of a G-lane row group holds the
+// partials of columns l*V .. l*V + V - 1); lds holds kBlock * V values.
+template <int V, typename T>
+__device__ __forceinline__ void column_partials_v(const T (&v)[V], uint32_t k, uint32_t G,
+                                                  T* lds, T* out /* k values */) {
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < V; ++c) lds[threadIdx.x * V + c] = v[c];
+    __syncthreads();
+    if (threadIdx.x < k) {
+        const uint32_t lane = threadIdx.x / V, c = threadIdx.x % V;
+        T s = T(0);
+        for (uint32_t t = lane; t < kBlock; t += G) s += lds[t * V + c];
+        out[threadIdx.x] = s;
+    }
+}
+
+// Columns per lane of the row-group SpMM kernels for k right-hand sides
+// (G = k rounded up to a power of two lanes per row with one): two, gathered
+// as one 16-byte copy per entry, for FP64 when k is even and at least 6 (4-8 %
+// faster FP64 solves; FP32 pairs, 8 bytes, measured no faster).
+template <typename T>
+bool multi_two_cols(uint32_t k) {
+    return std::is_same<T, double>::value && k % 2 == 0 && k >= 6;
+}
+template <int G>
+constexpr int two_cols_v() {
+    return G >= 8 ? 2 : 1;
+}
+
+template <int G, int V, bool ONE, typename T, bool PREC>
 __global__ __launch_bounds__(kBlock) void k_m_init(MultiArgs<T> a) {
-    __shared__ T lds[kBlock];
+    __shared__ T lds[kBlock * V];
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t k = a.k;
-    T prh = T(0), prr = T(0), pbb = T(0);
+    T prh[V], prr[V], pbb[V];
+#pragma unroll
+    for (int c = 0; c < V; ++c) prh[c] = prr[c] = pbb[c] = T(0);
     auto pre = [&](uint32_t row, uint32_t j) {
         return Pair<T>{a.b[uint64_t(row) * k + j], PREC ? a.dinv[row] : T(1)};
     };
@@ -78,16 +110,17 @@ __global__ __launch_bounds__(kBlock) void k_m_init(MultiArgs<T> a) {
         const T hi = PREC ? mul_rn(pv.b, ri) : ri;
         a.r[i] = ri;
         a.h[i] = hi;
-        prh += ri * hi;
-        prr += ri * ri;
-        pbb += bi * bi;
+        const uint32_t c = j % V;  // the lane's columns are j - c .. j - c + V - 1
+        prh[c] += ri * hi;
+        prr[c] += ri * ri;
+        pbb[c] += bi * bi;
     };
     const bool zero_x = a.x_nonzero && *a.x_nonzero == 0;  // X = 0: A X = 0 (block-uniform)
-    spmm_rows<G, ONE>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.x, pre, fin, nullptr,
-                      zero_x);
-    column_partials(prh, k, G, lds, a.part_rh + size_t(L) * k);
-    column_partials(prr, k, G, lds, a.part_rr + size_t(L) * k);
-    column_partials(pbb, k, G, lds, a.part_bb + size_t(L) * k);
+    spmm_rows_v<G / V, V, ONE>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.x, pre, fin,
+                               nullptr, zero_x);
+    column_partials_v<V>(prh, k, G / V, lds, a.part_rh + size_t(L) * k);
+    column_partials_v<V>(prr, k, G / V, lds, a.part_rr + size_t(L) * k);
+    column_partials_v<V>(pbb, k, G / V, lds, a.part_bb + size_t(L) * k);
 }
 
 // One workgroup per column: sum of the P partials of column j (stride k) in
@@ -153,20 +186,23 @@ __global__ __launch_bounds__(kBlock) void k_m_dir(MultiArgs<T> a) {
     }
 }
 
-template <int G, bool ONE, typename T>
+template <int G, int V, bool ONE, typename T>
 __global__ __launch_bounds__(kBlock) void k_m_spmm(MultiArgs<T> a) {
-    __shared__ T lds[kBlock];
+    __shared__ T lds[kBlock * V];
     if (!any_active(a)) return;  // every column has stopped (block-uniform)
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t k = a.k;
-    T pdq = T(0);
+    T pdq[V];
+#pragma unroll
+    for (int c = 0; c < V; ++c) pdq[c] = T(0);
     auto pre = [&](uint32_t row, uint32_t j) { return a.d[uint64_t(row) * k + j]; };
     auto fin = [&](uint32_t row, uint32_t j, T s, T di) {
         a.q[uint64_t(row) * k + j] = s;
-        pdq += di * s;
+        pdq[j % V] += di * s;
     };
-    spmm_rows<G, ONE>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.d, pre, fin, a.done);
-    column_partials(pdq, k, G, lds, a.part_dq + size_t(L) * k);
+    spmm_rows_v<G / V, V, ONE>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.d, pre, fin,
+                               a.done);
+    column_partials_v<V>(pdq, k, G / V, lds, a.part_dq + size_t(L) * k);
 }
 
 // Wavefront entries per chunk / 64 of the stream kernels.
@@ -406,22 +442,24 @@ __global__ __launch_bounds__(kBlock) void k_to_colmajor(uint32_t n, uint32_t k, 
 }
 
 // R = B - A X (FP64, interleaved), per-column partial |r|^2 and |b|^2.
-template <int G, bool ONE>
+template <int G, int V, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_m_residual64(MultiArgs<double> a) {
-    __shared__ double lds[kBlock];
+    __shared__ double lds[kBlock * V];
     const uint32_t L = xcd_logical_block(blockIdx.x, gridDim.x);
     const uint32_t k = a.k;
-    double prr = 0.0, pbb = 0.0;
+    double prr[V], pbb[V];
+#pragma unroll
+    for (int c = 0; c < V; ++c) prr[c] = pbb[c] = 0.0;
     auto pre = [&](uint32_t row, uint32_t j) { return a.b[uint64_t(row) * k + j]; };
     auto fin = [&](uint32_t row, uint32_t j, double s, double bi) {
         const double ri = bi - s;
         a.r[uint64_t(row) * k + j] = ri;
-        prr += ri * ri;
-        pbb += bi * bi;
+        prr[j % V] += ri * ri;
+        pbb[j % V] += bi * bi;
     };
-    spmm_rows<G, ONE>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.x, pre, fin);
-    column_partials(prr, k, G, lds, a.part_rr + size_t(L) * k);
-    column_partials(pbb, k, G, lds, a.part_bb + size_t(L) * k);
+    spmm_rows_v<G / V, V, ONE>(a.m, k, a.rows_per_block, L, a.rp, a.ci, a.val, a.x, pre, fin);
+    column_partials_v<V>(prr, k, G / V, lds, a.part_rr + size_t(L) * k);
+    column_partials_v<V>(pbb, k, G / V, lds, a.part_bb + size_t(L) * k);
 }
 
 // k_m_residual64's epilogue for the stream engine: r = b - A x, partials of
@@ -607,18 +645,23 @@ template uint32_t multi_stream_per_cu<float>(uint32_t, bool);
 template <typename T>
 hipError_t multi_start(const MultiArgs<T>& a, bool precond, T eps, int max_iter, hipStream_t s) {
     const bool d16 = a.cols.ci16 != nullptr;
+    const bool v2 = multi_two_cols<T>(a.k);
     if (a.sp && precond) {
         NLHIP_KD_SWITCH(a.k, d16, (k_m_init_stream<K, D16, T, true><<<a.Pdq, kBlock, 0, s>>>(a)));
     } else if (a.sp) {
         NLHIP_KD_SWITCH(a.k, d16, (k_m_init_stream<K, D16, T, false><<<a.Pdq, kBlock, 0, s>>>(a)));
     } else if (precond && a.one_chunk) {
-        NLHIP_G_SWITCH(a.G, (k_m_init<G, true, T, true><<<a.P, kBlock, 0, s>>>(a)));
+        NLHIP_G_SWITCH(a.G, (v2 ? k_m_init<G, two_cols_v<G>(), true, T, true><<<a.P, kBlock, 0, s>>>(a)
+                               : k_m_init<G, 1, true, T, true><<<a.P, kBlock, 0, s>>>(a)));
     } else if (precond) {
-        NLHIP_G_SWITCH(a.G, (k_m_init<G, false, T, true><<<a.P, kBlock, 0, s>>>(a)));
+        NLHIP_G_SWITCH(a.G, (v2 ? k_m_init<G, two_cols_v<G>(), false, T, true><<<a.P, kBlock, 0, s>>>(a)
+                               : k_m_init<G, 1, false, T, true><<<a.P, kBlock, 0, s>>>(a)));
     } else if (a.one_chunk) {
-        NLHIP_G_SWITCH(a.G, (k_m_init<G, true, T, false><<<a.P, kBlock, 0, s>>>(a)));
+        NLHIP_G_SWITCH(a.G, (v2 ? k_m_init<G, two_cols_v<G>(), true, T, false><<<a.P, kBlock, 0, s>>>(a)
+                               : k_m_init<G, 1, true, T, false><<<a.P, kBlock, 0, s>>>(a)));
     } else {
-        NLHIP_G_SWITCH(a.G, (k_m_init<G, false, T, false><<<a.P, kBlock, 0, s>>>(a)));
+        NLHIP_G_SWITCH(a.G, (v2 ? k_m_init<G, two_cols_v<G>(), false, T, false><<<a.P, kBlock, 0, s>>>(a)
+                               : k_m_init<G, 1, false, T, false><<<a.P, kBlock, 0, s>>>(a)));
     }
     k_m_sys_init<T><<<a.k, kBlock, 0, s>>>(a, eps, max_iter);
     return hipGetLastError();
@@ -627,14 +670,17 @@ hipError_t multi_start(const MultiArgs<T>& a, bool precond, T eps, int max_iter,
 template <typename T>
 hipError_t multi_iteration(const MultiArgs<T>& a, bool precond, int max_iter, hipStream_t s) {
     const uint32_t cg = a.k;  // one workgroup per column for the reductions
+    const bool v2 = multi_two_cols<T>(a.k);
     k_m_dir<T><<<a.P, kBlock, 0, s>>>(a);
     if (a.sp) {
         NLHIP_KD_SWITCH(a.k, a.cols.ci16 != nullptr,
                         (k_m_spmm_stream<K, D16, T><<<a.Pdq, kBlock, 0, s>>>(a)));
     } else if (a.one_chunk) {
-        NLHIP_G_SWITCH(a.G, (k_m_spmm<G, true, T><<<a.P, kBlock, 0, s>>>(a)));
+        NLHIP_G_SWITCH(a.G, (v2 ? k_m_spmm<G, two_cols_v<G>(), true, T><<<a.P, kBlock, 0, s>>>(a)
+                               : k_m_spmm<G, 1, true, T><<<a.P, kBlock, 0, s>>>(a)));
     } else {
-        NLHIP_G_SWITCH(a.G, (k_m_spmm<G, false, T><<<a.P, kBlock, 0, s>>>(a)));
+        NLHIP_G_SWITCH(a.G, (v2 ? k_m_spmm<G, two_cols_v<G>(), false, T><<<a.P, kBlock, 0, s>>>(a)
+                               : k_m_spmm<G, 1, false, T><<<a.P, kBlock, 0, s>>>(a)));
     }
     k_m_reduce_alpha<T><<<cg, kBlock, 0, s>>>(a);
     if (precond) k_m_update<T, true><<<a.P, kBlock, 0, s>>>(a);
@@ -661,13 +707,16 @@ hipError_t multi_reopen(const MultiArgs<double>& a, const int32_t* again, hipStr
 }
 
 hipError_t multi_residual64(const MultiArgs<double>& a, hipStream_t s) {
+    const bool v2 = multi_two_cols<double>(a.k);
     if (a.sp) {
         NLHIP_KD_SWITCH(a.k, a.cols.ci16 != nullptr,
                         (k_m_residual64_stream<K, D16><<<a.Pdq, kBlock, 0, s>>>(a)));
     } else if (a.one_chunk) {
-        NLHIP_G_SWITCH(a.G, (k_m_residual64<G, true><<<a.P, kBlock, 0, s>>>(a)));
+        NLHIP_G_SWITCH(a.G, (v2 ? k_m_residual64<G, two_cols_v<G>(), true><<<a.P, kBlock, 0, s>>>(a)
+                               : k_m_residual64<G, 1, true><<<a.P, kBlock, 0, s>>>(a)));
     } else {
-        NLHIP_G_SWITCH(a.G, (k_m_residual64<G, false><<<a.P, kBlock, 0, s>>>(a)));
+        NLHIP_G_SWITCH(a.G, (v2 ? k_m_residual64<G, two_cols_v<G>(), false><<<a.P, kBlock, 0, s>>>(a)
+                               : k_m_residual64<G, 1, false><<<a.P, kBlock, 0, s>>>(a)));
     }
     k_m_norms64<<<a.k, kBlock, 0, s>>>(a);
     return hipGetLastError();
