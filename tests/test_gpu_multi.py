@@ -111,14 +111,17 @@ def test_multi_rhs_smoothing_xyz(hip):
         assert rel(X[j], xs) <= REL_DIFF
 
 
-def test_multi_rhs_mixed_precision(hip):
+@pytest.mark.parametrize("k", [4, 16])
+def test_multi_rhs_mixed_precision(hip, k):
+    """k = 4: the stream engine; k = 16: row groups (one FP32 column per lane
+    in the inner solves, two FP64 columns per lane outside)."""
     mesh = W.grid_mesh(50, seed=5)
-    Sw = W.cotan_system(mesh, t=1e-3, nrhs=4, rhs="gaussian", seed=2)
+    Sw = W.cotan_system(mesh, t=1e-3, nrhs=k, rhs="gaussian", seed=2)
     S = capi.System.from_crs(Sw, fp32=True)
     X, per, tot = S.solve_multi(Sw.rhs, precision="mixed", eps=1e-10, max_iter=1000,
                                 inner_eps=1e-4)
     A = O.CRS(Sw.rowptr, Sw.colind, Sw.val)
-    for j in range(4):
+    for j in range(k):
         r = O.spmv(A, X[j]) - Sw.rhs[j]
         assert np.linalg.norm(r) / np.linalg.norm(Sw.rhs[j]) <= 1e-10
         assert per[j].converged == 1 and per[j].outer_iterations >= 1
